@@ -1,0 +1,114 @@
+/*
+ * neuralpom_amd -- C ABI of the MI355X explicit POM step (forward + adjoint).
+ *
+ * The boundary the reference's Python core would bind to (through ctypes) in
+ * place of its eager PyTorch operators.  Plain pointers, sizes and a HIP
+ * stream; no torch types.  All fields use the reference's layout:
+ *   3-D fields (im, jm, 16) row-major, k fastest  (src/neuralpom/cores/explicit/base.py:162-174)
+ *   2-D fields (im, jm) row-major                  (base.py:120-160)
+ * An ensemble packs members back to back: member m's 3-D field starts at
+ * element off3[m], its 2-D field at off2[m] (one allocation per field).
+ *
+ * Every entry point returns 0 on success or a positive error code:
+ *   POM_E_ARG (invalid argument), POM_E_HIP (HIP runtime error),
+ *   POM_E_UNSUPPORTED (configuration not supported by this build).
+ */
+#ifndef NEURALPOM_AMD_H
+#define NEURALPOM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define POM_OK 0
+#define POM_E_ARG 1
+#define POM_E_HIP 2
+#define POM_E_UNSUPPORTED 3
+#define POM_E_VMAX 4
+
+#define POM_F32 0
+#define POM_F64 1
+
+/* Dynamic state fields of one internal step.  Names are the reference's
+ * POM2K_Core attribute names (explicit/base.py:120-189); the step reads the
+ * *_in table and writes the *_out table (src/neuralpom/cores/explicit/
+ * time_loop.py:20-709 updates them in place or by rebinding). */
+typedef enum {
+  /* 3-D */
+  POM_U = 0, POM_UB, POM_V, POM_VB, POM_T, POM_TB, POM_S, POM_SB,
+  POM_Q2, POM_Q2B, POM_Q2L, POM_Q2LB, POM_KM, POM_KH, POM_KQ, POM_AAM, POM_RHO, POM_W, POM_L,
+  POM_ADVX, POM_ADVY, POM_DRHOX, POM_DRHOY,
+  POM_N3D,
+  /* 2-D */
+  POM_EL = POM_N3D, POM_ELB, POM_ELF, POM_UA, POM_UAB, POM_UAF, POM_VA, POM_VAB, POM_VAF,
+  POM_D, POM_DT, POM_ET, POM_ETB, POM_ETF, POM_EGF, POM_EGB, POM_UTF, POM_UTB, POM_VTF, POM_VTB,
+  POM_WUBOT, POM_WVBOT, POM_ADVUA, POM_ADVVA, POM_ADX2D, POM_ADY2D, POM_DRX2D, POM_DRY2D, POM_AAM2D,
+  POM_FLUXUA, POM_FLUXVA, POM_TPS,
+  POM_E_ATMOS, POM_VFLUXF, POM_VFLUXB, POM_SWRAD, POM_WTSURF, POM_WSSURF,
+  POM_NSTATE
+} pom_field;
+
+/* Static geometry and forcing (not differentiated). */
+typedef enum {
+  POM_G_H = 0, POM_G_DX, POM_G_DY, POM_G_ART, POM_G_ARU, POM_G_ARV, POM_G_FSM, POM_G_DUM, POM_G_DVM,
+  POM_G_COR, POM_G_CBC, POM_G_WUSURF, POM_G_WVSURF, POM_G_TSURF, POM_G_SSURF, /* 2-D */
+  POM_G_RMEAN, POM_G_TCLIM, POM_G_SCLIM,                                      /* 3-D */
+  POM_NGEOM
+} pom_geom;
+
+typedef struct {
+  int32_t im, jm;
+  int64_t off2, off3;
+} pom_member_t;
+
+/* Scalars of the step (explicit/base.py:46-118, v1/core_spec.py:8-47). */
+typedef struct {
+  double dti, dte, dti2, dte2, ispi, isp2i;
+  double alpha, smoth, horcon, grav, rhoref, kappa, umol, tprni, small;
+  double tbias, sbias, hmax, rfe, rfw, rfn, rfs;
+  int32_t isplit, ispadv, ntp, nbct, nbcs, iext;
+  double vmaxl;
+} pom_params_t;
+
+typedef struct pom_batch pom_batch;
+
+/* Member table of an ensemble.  Copies the table to the device (one small
+ * allocation owned by the handle).  dtype: POM_F32 or POM_F64. */
+int pom_batch_create(const pom_member_t* members, int32_t n_members, int32_t dtype, pom_batch** out);
+int pom_batch_destroy(pom_batch* b);
+
+/* Workspace bytes needed by pom_explicit_step (keep=0) and by
+ * pom_explicit_step_adj. */
+int pom_explicit_workspace(const pom_batch* b, const pom_params_t* prm, int32_t adjoint, size_t* bytes);
+
+/* One internal step (time_loop.py:75-696) for every member.
+ *   ramp:    per-member ramp factor (device, n_members)  (time_loop.py:79-88)
+ *   vgrid:   per-member (z, zz, dz, dzz), device [n_members][4][16]
+ *   internal: 1 to run the 3-D internal mode, 0 for the first step from
+ *            time0 = 0 (time_loop.py:438)
+ *   vamax_out: device scalar receiving max |vaf| over all substeps
+ * Replaces POM2K_Core.run_time_loop's loop body; entries of state_out that the
+ * step leaves unchanged are copied from state_in. */
+int pom_explicit_step(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vgrid,
+                      const void* const* state_in, void* const* state_out, const void* const* geom,
+                      int32_t internal, void* vamax_out, void* workspace, size_t ws_bytes, void* stream);
+
+/* Vector-Jacobian product of one step: recomputes the forward from state_in
+ * into the workspace, then accumulates
+ *   grad_in[f] += (d state_out / d state_in[f])^T cot_out
+ * for every field f with a non-null grad_in[f] (zero-initialised by the caller
+ * or holding a running sum).  Null cot_out[f] entries are zero cotangents.
+ * Differentiable inputs: u ub v vb t tb s sb q2 q2b q2l q2lb km kh kq aam rho
+ * el elb ua uab va vab d dt et etb egb utf utb vtf vtb wubot wvbot. */
+int pom_explicit_step_adj(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vgrid,
+                          const void* const* state_in, const void* const* geom, int32_t internal,
+                          const void* const* cot_out, void* const* grad_in, void* workspace, size_t ws_bytes,
+                          void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

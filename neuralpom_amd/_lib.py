@@ -1,0 +1,98 @@
+"""ctypes binding of libneuralpom_amd.so (declarations: include/neuralpom_amd.h).
+
+The library is the only compute path of this package: if it is missing or a
+HIP device is absent, every entry point raises instead of falling back.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+_LIB_PATH = Path(__file__).resolve().parent / "libneuralpom_amd.so"
+
+POM_OK, POM_E_ARG, POM_E_HIP, POM_E_UNSUPPORTED, POM_E_VMAX = 0, 1, 2, 3, 4
+POM_F32, POM_F64 = 0, 1
+
+# pom_field order (include/neuralpom_amd.h)
+FIELDS_3D = ("u", "ub", "v", "vb", "t", "tb", "s", "sb", "q2", "q2b", "q2l", "q2lb",
+             "km", "kh", "kq", "aam", "rho", "w", "l", "advx", "advy", "drhox", "drhoy")
+FIELDS_2D = ("el", "elb", "elf", "ua", "uab", "uaf", "va", "vab", "vaf", "d", "dt", "et", "etb",
+             "etf", "egf", "egb", "utf", "utb", "vtf", "vtb", "wubot", "wvbot", "advua", "advva",
+             "adx2d", "ady2d", "drx2d", "dry2d", "aam2d", "fluxua", "fluxva", "tps",
+             "e_atmos", "vfluxf", "vfluxb", "swrad", "wtsurf", "wssurf")
+FIELDS = FIELDS_3D + FIELDS_2D
+FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
+GEOM_2D = ("h", "dx", "dy", "art", "aru", "arv", "fsm", "dum", "dvm", "cor", "cbc",
+           "wusurf", "wvsurf", "tsurf", "ssurf")
+GEOM_3D = ("rmean", "tclim", "sclim")
+GEOM = GEOM_2D + GEOM_3D
+
+# inputs with a cotangent (pom_explicit_step_adj)
+DIFF_INPUTS = ("u", "ub", "v", "vb", "t", "tb", "s", "sb", "q2", "q2b", "q2l", "q2lb", "km",
+               "kh", "kq", "aam", "rho", "el", "elb", "ua", "uab", "va", "vab", "d", "dt", "et",
+               "etb", "egb", "utf", "utb", "vtf", "vtb", "wubot", "wvbot")
+
+
+class Member(C.Structure):
+    _fields_ = [("im", C.c_int32), ("jm", C.c_int32), ("off2", C.c_int64), ("off3", C.c_int64)]
+
+
+class Params(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "dti", "dte", "dti2", "dte2", "ispi", "isp2i", "alpha", "smoth", "horcon", "grav",
+        "rhoref", "kappa", "umol", "tprni", "small", "tbias", "sbias", "hmax", "rfe", "rfw",
+        "rfn", "rfs")] + [(n, C.c_int32) for n in ("isplit", "ispadv", "ntp", "nbct", "nbcs", "iext")] + [
+        ("vmaxl", C.c_double)]
+
+
+_lib = None
+
+
+class HipExtensionError(RuntimeError):
+    pass
+
+
+def load():
+    """Load the library; raise HipExtensionError if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not _LIB_PATH.exists():
+        raise HipExtensionError(
+            f"{_LIB_PATH} is not built; run `python -m neuralpom_amd.build` (hipcc, gfx950)")
+    lib = C.CDLL(str(_LIB_PATH))
+    vp, vpp = C.c_void_p, C.POINTER(C.c_void_p)
+    lib.pom_batch_create.argtypes = [C.POINTER(Member), C.c_int32, C.c_int32, C.POINTER(vp)]
+    lib.pom_batch_destroy.argtypes = [vp]
+    lib.pom_explicit_workspace.argtypes = [vp, C.POINTER(Params), C.c_int32, C.POINTER(C.c_size_t)]
+    lib.pom_explicit_step.argtypes = [vp, C.POINTER(Params), vp, vp, vpp, vpp, vpp, C.c_int32, vp, vp,
+                                      C.c_size_t, vp]
+    lib.pom_explicit_step_adj.argtypes = [vp, C.POINTER(Params), vp, vp, vpp, vpp, C.c_int32, vpp, vpp,
+                                          vp, C.c_size_t, vp]
+    for fn in ("pom_batch_create", "pom_batch_destroy", "pom_explicit_workspace", "pom_explicit_step",
+               "pom_explicit_step_adj"):
+        getattr(lib, fn).restype = C.c_int
+    _lib = lib
+    return lib
+
+
+EXPORTED = ("pom_batch_create", "pom_batch_destroy", "pom_explicit_workspace", "pom_explicit_step",
+            "pom_explicit_step_adj")
+
+
+def check(rc: int, what: str):
+    if rc == POM_OK:
+        return
+    msg = {POM_E_ARG: "invalid argument", POM_E_HIP: "HIP runtime error",
+           POM_E_UNSUPPORTED: "unsupported configuration"}.get(rc, f"error {rc}")
+    if rc == POM_E_ARG:
+        raise ValueError(f"{what}: {msg}")
+    raise RuntimeError(f"{what}: {msg}")
+
+
+def ptr_array(ptrs):
+    arr = (C.c_void_p * len(ptrs))()
+    for i, p in enumerate(ptrs):
+        arr[i] = p
+    return arr

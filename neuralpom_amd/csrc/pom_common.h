@@ -1,0 +1,312 @@
+// Common device infrastructure for the MI355X POM step kernels.
+//
+// Layout: every 3-D field of one member is (im, jm, KB) with k fastest, exactly
+// the reference's tensor layout (src/neuralpom/cores/explicit/base.py:162-174),
+// so torch tensors are used in place.  An ensemble packs members back to back:
+// member m's 3-D field starts at off3[m] elements and its 2-D field at off2[m].
+//
+// Value types: P<T> (passive) and D<T> (active: value + tangent).  Every
+// stencil operator is written once as a point function over an accessor.  The
+// forward kernel instantiates it with an accessor that returns P<T>; the
+// adjoint gather instantiates it once per (input field, stencil offset) with an
+// accessor that returns D<T> only for that compile-time (field, offset) pair.
+// Activity is a property of the type, so inactive arithmetic disappears at
+// compile time and each gather term is the exact partial derivative
+// d out(q - o) / d in(q), evaluated at the saved primal state.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <type_traits>
+#include <utility>
+
+namespace pom {
+
+constexpr int KB = 16;        // sigma levels (reference base.py:27)
+constexpr int KBM1 = KB - 1;
+constexpr int KBM2 = KB - 2;
+constexpr int TPB = 256;      // threads per block for point kernels
+
+// ------------------------------------------------------------------ values
+template <class T> struct P { T v; };
+template <class T> struct D { T v, d; };
+
+template <class X> struct is_active : std::false_type {};
+template <class T> struct is_active<D<T>> : std::true_type {};
+template <class X> inline constexpr bool active_v = is_active<X>::value;
+
+template <class T> __device__ __forceinline__ T val(P<T> a) { return a.v; }
+template <class T> __device__ __forceinline__ T val(D<T> a) { return a.v; }
+template <class T> __device__ __forceinline__ T tan_(P<T>) { return T(0); }
+template <class T> __device__ __forceinline__ T tan_(D<T> a) { return a.d; }
+
+template <class S> inline constexpr bool arith_v = std::is_arithmetic<S>::value;
+
+// P op P
+template <class T> __device__ __forceinline__ P<T> operator+(P<T> a, P<T> b) { return {a.v + b.v}; }
+template <class T> __device__ __forceinline__ P<T> operator-(P<T> a, P<T> b) { return {a.v - b.v}; }
+template <class T> __device__ __forceinline__ P<T> operator*(P<T> a, P<T> b) { return {a.v * b.v}; }
+template <class T> __device__ __forceinline__ P<T> operator/(P<T> a, P<T> b) { return {a.v / b.v}; }
+template <class T> __device__ __forceinline__ P<T> operator-(P<T> a) { return {-a.v}; }
+// D op D
+template <class T> __device__ __forceinline__ D<T> operator+(D<T> a, D<T> b) { return {a.v + b.v, a.d + b.d}; }
+template <class T> __device__ __forceinline__ D<T> operator-(D<T> a, D<T> b) { return {a.v - b.v, a.d - b.d}; }
+template <class T> __device__ __forceinline__ D<T> operator*(D<T> a, D<T> b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+template <class T> __device__ __forceinline__ D<T> operator/(D<T> a, D<T> b) {
+  T q = a.v / b.v;
+  return {q, (a.d - q * b.d) / b.v};
+}
+template <class T> __device__ __forceinline__ D<T> operator-(D<T> a) { return {-a.v, -a.d}; }
+// D op P / P op D
+template <class T> __device__ __forceinline__ D<T> operator+(D<T> a, P<T> b) { return {a.v + b.v, a.d}; }
+template <class T> __device__ __forceinline__ D<T> operator+(P<T> a, D<T> b) { return {a.v + b.v, b.d}; }
+template <class T> __device__ __forceinline__ D<T> operator-(D<T> a, P<T> b) { return {a.v - b.v, a.d}; }
+template <class T> __device__ __forceinline__ D<T> operator-(P<T> a, D<T> b) { return {a.v - b.v, -b.d}; }
+template <class T> __device__ __forceinline__ D<T> operator*(D<T> a, P<T> b) { return {a.v * b.v, a.d * b.v}; }
+template <class T> __device__ __forceinline__ D<T> operator*(P<T> a, D<T> b) { return {a.v * b.v, a.v * b.d}; }
+template <class T> __device__ __forceinline__ D<T> operator/(D<T> a, P<T> b) { return {a.v / b.v, a.d / b.v}; }
+template <class T> __device__ __forceinline__ D<T> operator/(P<T> a, D<T> b) {
+  T q = a.v / b.v;
+  return {q, -q * b.d / b.v};
+}
+// scalar literals (cast to T, as a python float is cast to the tensor dtype)
+#define POM_SCALAR_OPS(OP)                                                                   \
+  template <class T, class S, std::enable_if_t<arith_v<S>, int> = 0>                         \
+  __device__ __forceinline__ auto operator OP(S s, P<T> b) { return P<T>{T(s)} OP b; }       \
+  template <class T, class S, std::enable_if_t<arith_v<S>, int> = 0>                         \
+  __device__ __forceinline__ auto operator OP(P<T> a, S s) { return a OP P<T>{T(s)}; }       \
+  template <class T, class S, std::enable_if_t<arith_v<S>, int> = 0>                         \
+  __device__ __forceinline__ auto operator OP(S s, D<T> b) { return P<T>{T(s)} OP b; }       \
+  template <class T, class S, std::enable_if_t<arith_v<S>, int> = 0>                         \
+  __device__ __forceinline__ auto operator OP(D<T> a, S s) { return a OP P<T>{T(s)}; }
+POM_SCALAR_OPS(+)
+POM_SCALAR_OPS(-)
+POM_SCALAR_OPS(*)
+POM_SCALAR_OPS(/)
+#undef POM_SCALAR_OPS
+
+template <class T> __device__ __forceinline__ P<T> psqrt(P<T> a) { return {sqrt(a.v)}; }
+template <class T> __device__ __forceinline__ D<T> psqrt(D<T> a) {
+  T s = sqrt(a.v);
+  return {s, a.d / (T(2) * s)};
+}
+// |x| with d|x|/dx = sign(x) (0 at 0), as torch.abs
+template <class T> __device__ __forceinline__ P<T> pabs(P<T> a) { return {fabs(a.v)}; }
+template <class T> __device__ __forceinline__ D<T> pabs(D<T> a) {
+  T s = a.v > T(0) ? T(1) : (a.v < T(0) ? T(-1) : T(0));
+  return {fabs(a.v), s * a.d};
+}
+template <class A> __device__ __forceinline__ auto sq(A a) { return a * a; }
+
+// select by a primal condition; activity promotes
+template <class T> __device__ __forceinline__ P<T> sel(bool c, P<T> a, P<T> b) { return c ? a : b; }
+template <class T> __device__ __forceinline__ D<T> sel(bool c, D<T> a, D<T> b) { return c ? a : b; }
+template <class T> __device__ __forceinline__ D<T> sel(bool c, D<T> a, P<T> b) { return c ? a : D<T>{b.v, T(0)}; }
+template <class T> __device__ __forceinline__ D<T> sel(bool c, P<T> a, D<T> b) { return c ? D<T>{a.v, T(0)} : b; }
+
+// torch.clamp(x, min=lo): gradient passes where x >= lo
+template <class X, class T> __device__ __forceinline__ X clamp_min(X a, T lo) {
+  if (val(a) >= lo) return a;
+  X r{};
+  r.v = lo;
+  return r;
+}
+// torch.clamp(x, max=hi): gradient passes where x <= hi
+template <class X, class T> __device__ __forceinline__ X clamp_max(X a, T hi) {
+  if (val(a) <= hi) return a;
+  X r{};
+  r.v = hi;
+  return r;
+}
+// elementwise maximum selected by the primal (tie -> first argument)
+template <class X, class Y> __device__ __forceinline__ auto pmax(X a, Y b);
+
+// common activity type of two values, and conversion into it
+template <class X> __device__ X mkval();   // declared only: for decltype
+template <class X, class Y> using common_t = decltype(mkval<X>() + mkval<Y>());
+template <class R, class X> __device__ __forceinline__ R cvt(X x) {
+  if constexpr (std::is_same_v<R, X>) return x;
+  else return R{x.v, decltype(x.v)(0)};
+}
+template <class X, class Y> __device__ __forceinline__ auto pmax(X a, Y b) {
+  using R = common_t<X, Y>;
+  return val(a) >= val(b) ? cvt<R>(a) : cvt<R>(b);
+}
+
+template <class X> struct scalar_of;
+template <class T> struct scalar_of<P<T>> { using type = T; };
+template <class T> struct scalar_of<D<T>> { using type = T; };
+
+// ------------------------------------------------------------------ batch
+struct Member {
+  int32_t im, jm;
+  int64_t off2, off3;
+};
+
+struct Batch {
+  const Member* mem;   // [M]
+  const int32_t* blk3; // [M+1] prefix of 3-D point blocks
+  const int32_t* blk2; // [M+1] prefix of 2-D point blocks
+  int32_t M;
+};
+
+// member owning block b (all lanes agree -> scalar)
+__device__ __forceinline__ int member_of(const int32_t* blk, int M, int b) {
+  int lo = 0, hi = M;   // invariant blk[lo] <= b < blk[hi]
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (blk[mid] <= b) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// position of the current thread in a 3-D launch; false if past the member
+struct Pos {
+  int m, im, jm, i, j, k;
+  int64_t o2, o3;   // member base offsets
+};
+
+__device__ __forceinline__ bool pos3(const Batch& B, Pos& p) {
+  int b = blockIdx.x;
+  int m = member_of(B.blk3, B.M, b);
+  Member mm = B.mem[m];
+  int64_t local = (int64_t)(b - B.blk3[m]) * TPB + threadIdx.x;
+  int64_t n = (int64_t)mm.im * mm.jm * KB;
+  if (local >= n) return false;
+  p.m = m; p.im = mm.im; p.jm = mm.jm; p.o2 = mm.off2; p.o3 = mm.off3;
+  p.k = (int)(local & (KB - 1));
+  int64_t col = local >> 4;
+  p.i = (int)(col / mm.jm);
+  p.j = (int)(col - (int64_t)p.i * mm.jm);
+  return true;
+}
+
+__device__ __forceinline__ bool pos2(const Batch& B, Pos& p) {
+  int b = blockIdx.x;
+  int m = member_of(B.blk2, B.M, b);
+  Member mm = B.mem[m];
+  int64_t local = (int64_t)(b - B.blk2[m]) * TPB + threadIdx.x;
+  int64_t n = (int64_t)mm.im * mm.jm;
+  if (local >= n) return false;
+  p.m = m; p.im = mm.im; p.jm = mm.jm; p.o2 = mm.off2; p.o3 = mm.off3;
+  p.k = 0;
+  p.i = (int)(local / mm.jm);
+  p.j = (int)(local - (int64_t)p.i * mm.jm);
+  return true;
+}
+
+__device__ __forceinline__ bool interior_pos(const Pos& p) {
+  return p.i >= 1 && p.i <= p.im - 2 && p.j >= 1 && p.j <= p.jm - 2;
+}
+
+__device__ __forceinline__ int64_t ix3(const Pos& p, int i, int j, int k) {
+  return p.o3 + ((int64_t)i * p.jm + j) * KB + k;
+}
+__device__ __forceinline__ int64_t ix2(const Pos& p, int i, int j) {
+  return p.o2 + (int64_t)i * p.jm + j;
+}
+
+// ------------------------------------------------------------------ params
+// Scalars shared by the batch (double; cast to T at use) plus per-member
+// arrays: ramp[m] and the vertical grid vg[m][4][KB] = (z, zz, dz, dzz).
+struct Params {
+  double dti, dte, dti2, dte2, ispi, isp2i;
+  double alpha, smoth, horcon, grav, rhoref, kappa, umol, tprni, small;
+  double tbias, sbias, hmax, rfe, rfw, rfn, rfs;
+  int32_t isplit, ispadv, ntp, nbct, nbcs, iext;   // iext: current substep
+};
+
+template <class T> struct Vert {
+  const T* vg;   // [M][4][KB]
+  __device__ __forceinline__ T z(int m, int k) const { return vg[(m * 4 + 0) * KB + k]; }
+  __device__ __forceinline__ T zz(int m, int k) const { return vg[(m * 4 + 1) * KB + k]; }
+  __device__ __forceinline__ T dz(int m, int k) const { return vg[(m * 4 + 2) * KB + k]; }
+  __device__ __forceinline__ T dzz(int m, int k) const { return vg[(m * 4 + 3) * KB + k]; }
+};
+
+// ------------------------------------------------------------------ accessors
+// Fields of an operator are addressed by small integer ids.  ids below N3 are
+// 3-D, ids in [N3, N3+N2) are 2-D.  Geometry (passive) has its own table.
+constexpr int MAXF = 24;
+constexpr int MAXG = 16;
+
+template <class T> struct FieldTab {
+  const T* f[MAXF];    // state inputs (3-D then 2-D)
+  const T* g[MAXG];    // geometry / passive 2-D
+};
+
+// SF: seeded field id (-1: none); seed offset (SI, SJ, SK) relative to the
+// evaluation point.
+template <class T, int SF = -1, int SI = 0, int SJ = 0, int SK = 0>
+struct Acc {
+  const FieldTab<T>* tab;
+  Pos p;          // evaluation point (p.i, p.j, p.k)
+  Vert<T> vert;
+  const T* ramp;
+
+  // Indices are clamped into the member so that a branch which is computed
+  // but not selected can never read outside the member's storage.
+  __device__ __forceinline__ int ci(int i) const { return i < 0 ? 0 : (i >= p.im ? p.im - 1 : i); }
+  __device__ __forceinline__ int cj(int j) const { return j < 0 ? 0 : (j >= p.jm ? p.jm - 1 : j); }
+  __device__ __forceinline__ int ck(int k) const { return k < 0 ? 0 : (k >= KB ? KB - 1 : k); }
+
+  template <int F, int DI, int DJ, int DK>
+  __device__ __forceinline__ auto at() const {
+    T x = tab->f[F][ix3(p, ci(p.i + DI), cj(p.j + DJ), ck(p.k + DK))];
+    if constexpr (F == SF && DI == SI && DJ == SJ && DK == SK) return D<T>{x, T(1)};
+    else return P<T>{x};
+  }
+  template <int F, int DI, int DJ>
+  __device__ __forceinline__ auto at2() const {
+    T x = tab->f[F][ix2(p, ci(p.i + DI), cj(p.j + DJ))];
+    if constexpr (F == SF && DI == SI && DJ == SJ) return D<T>{x, T(1)};
+    else return P<T>{x};
+  }
+  template <int G, int DI, int DJ>
+  __device__ __forceinline__ P<T> g() const {
+    return P<T>{tab->g[G][ix2(p, ci(p.i + DI), cj(p.j + DJ))]};
+  }
+  __device__ __forceinline__ T z(int k) const { return vert.z(p.m, k); }
+  __device__ __forceinline__ T zz(int k) const { return vert.zz(p.m, k); }
+  __device__ __forceinline__ T dz(int k) const { return vert.dz(p.m, k); }
+  __device__ __forceinline__ T dzz(int k) const { return vert.dzz(p.m, k); }
+  __device__ __forceinline__ T rampv() const { return ramp[p.m]; }
+};
+
+// compile-time loop
+template <int A, int B, class F> __device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (A <= B) {
+    f(std::integral_constant<int, A>{});
+    static_for<A + 1, B>(f);
+  }
+}
+
+template <int A, int B, class F> inline void static_for_host(F&& f) {
+  if constexpr (A <= B) {
+    f(std::integral_constant<int, A>{});
+    static_for_host<A + 1, B>(f);
+  }
+}
+
+// shorthand inside point functions (A is the accessor object)
+// value of f() if cond, else zero of the same activity type; only the taken
+// side is evaluated
+template <class F> __device__ __forceinline__ auto ifz(bool cond, F f) {
+  using R = decltype(f());
+  R r{};
+  if (cond) r = f();
+  return r;
+}
+template <class FA, class FB> __device__ __forceinline__ auto br(bool cond, FA fa, FB fb) {
+  using R = common_t<decltype(fa()), decltype(fb())>;
+  if (cond) return cvt<R>(fa());
+  return cvt<R>(fb());
+}
+#define IFZ(cond, ...) ::pom::ifz((cond), [&]() { return (__VA_ARGS__); })
+// arguments holding template commas must be parenthesised
+#define BR(cond, ea, eb) ::pom::br((cond), [&]() { return (ea); }, [&]() { return (eb); })
+#define AT(F, di, dj, dk) (A.template at<F, di, dj, dk>())
+#define AT2(F, di, dj) (A.template at2<F, di, dj>())
+#define G2(G, di, dj) (A.template g<G, di, dj>())
+
+}  // namespace pom

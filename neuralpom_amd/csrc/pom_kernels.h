@@ -1,0 +1,130 @@
+// Generic map (forward) and gather (adjoint) kernels over the point
+// functions of pom_ops.h.
+//
+// Forward: one thread per output point (3-D: k fastest, so a wavefront covers
+// 4 whole water columns and every load is a contiguous 256-byte run).
+// Adjoint: for a seeded input field F the thread owning input point q sums
+//   sum_O sum_{o in box} obar_O(q - o) * d out_O(q - o) / d in_F(q)
+// where the partial comes from the D<T>-typed instantiation of eval() whose
+// only active access is (F, o).  Offsets whose instantiation is passive are
+// removed at compile time.  No atomics: every cotangent element is written by
+// exactly one thread, so the result is deterministic.
+#pragma once
+#include "pom_ops.h"
+
+namespace pom {
+
+constexpr int MAXO = 12;
+template <class T> struct OutTab { T* o[MAXO]; };
+template <class T> struct COutTab { const T* o[MAXO]; };
+
+template <class T> struct Ctx {
+  Batch B;
+  Vert<T> vert;
+  const T* ramp;
+  Params prm;
+  int32_t nblk3, nblk2;
+  hipStream_t stream;
+};
+
+template <class T, class Op>
+__global__ void __launch_bounds__(TPB) k_map(Batch B, Vert<T> vert, const T* ramp, Params prm, FieldTab<T> tab,
+                                              OutTab<T> out) {
+  Pos p;
+  if constexpr (Op::OUT3) {
+    if (!pos3(B, p)) return;
+  } else {
+    if (!pos2(B, p)) return;
+  }
+  Acc<T> A{&tab, p, vert, ramp};
+  const int64_t idx = Op::OUT3 ? ix3(p, p.i, p.j, p.k) : ix2(p, p.i, p.j);
+  static_for<0, Op::NOUT - 1>([&](auto O) {
+    constexpr int o = decltype(O)::value;
+    if (out.o[o]) out.o[o][idx] = val(Op::template eval<o>(A, prm));
+  });
+}
+
+// partial contribution of outputs at p = q - (DI, DJ, DK) to input F at q
+template <class T, class Op, int F, int DI, int DJ, int DK>
+__device__ __forceinline__ void gather_term(const FieldTab<T>& tab, const COutTab<T>& obar, const Pos& q, int pk,
+                                            const Vert<T>& vert, const T* ramp, const Params& prm, T& acc) {
+  Pos p = q;
+  p.i = q.i - DI;
+  p.j = q.j - DJ;
+  p.k = pk;
+  if (p.i < 0 || p.i >= p.im || p.j < 0 || p.j >= p.jm || p.k < 0 || p.k >= KB) return;
+  Acc<T, F, DI, DJ, DK> A{&tab, p, vert, ramp};
+  const int64_t idx = Op::OUT3 ? ix3(p, p.i, p.j, p.k) : ix2(p, p.i, p.j);
+  static_for<0, Op::NOUT - 1>([&](auto O) {
+    constexpr int o = decltype(O)::value;
+    using R = decltype(Op::template eval<o>(A, prm));
+    if constexpr (active_v<R>) {
+      if (obar.o[o]) {
+        T ob = obar.o[o][idx];
+        R r = Op::template eval<o>(A, prm);
+        acc += ob * r.d;
+      }
+    }
+  });
+}
+
+// 3-D seeded field, 3-D outputs: one thread per input point
+template <class T, class Op, int F>
+__global__ void __launch_bounds__(TPB) k_gather3(Batch B, Vert<T> vert, const T* ramp, Params prm,
+                                                  FieldTab<T> tab, COutTab<T> obar, T* inbar) {
+  Pos q;
+  if (!pos3(B, q)) return;
+  T acc = T(0);
+  static_for<-Op::RI, Op::RI>([&](auto I) {
+    static_for<-Op::RJ, Op::RJ>([&](auto J) {
+      static_for<-Op::RK, Op::RK>([&](auto K) {
+        constexpr int di = decltype(I)::value, dj = decltype(J)::value, dk = decltype(K)::value;
+        gather_term<T, Op, F, di, dj, dk>(tab, obar, q, q.k - dk, vert, ramp, prm, acc);
+      });
+    });
+  });
+  inbar[ix3(q, q.i, q.j, q.k)] += acc;
+}
+
+// 2-D seeded field: one thread per column; 3-D outputs sum over all levels
+template <class T, class Op, int F>
+__global__ void __launch_bounds__(TPB) k_gather2(Batch B, Vert<T> vert, const T* ramp, Params prm,
+                                                  FieldTab<T> tab, COutTab<T> obar, T* inbar) {
+  Pos q;
+  if (!pos2(B, q)) return;
+  T acc = T(0);
+  if constexpr (Op::OUT3) {
+    for (int k = 0; k < KB; ++k) {
+      static_for<-Op::RI, Op::RI>([&](auto I) {
+        static_for<-Op::RJ, Op::RJ>([&](auto J) {
+          constexpr int di = decltype(I)::value, dj = decltype(J)::value;
+          gather_term<T, Op, F, di, dj, 0>(tab, obar, q, k, vert, ramp, prm, acc);
+        });
+      });
+    }
+  } else {
+    static_for<-Op::RI, Op::RI>([&](auto I) {
+      static_for<-Op::RJ, Op::RJ>([&](auto J) {
+        constexpr int di = decltype(I)::value, dj = decltype(J)::value;
+        gather_term<T, Op, F, di, dj, 0>(tab, obar, q, 0, vert, ramp, prm, acc);
+      });
+    });
+  }
+  inbar[ix2(q, q.i, q.j)] += acc;
+}
+
+// ------------------------------------------------------------ host launchers
+template <class T, class Op>
+inline void launch_map(const Ctx<T>& c, const FieldTab<T>& tab, const OutTab<T>& out) {
+  const int nb = Op::OUT3 ? c.nblk3 : c.nblk2;
+  if (nb == 0) return;
+  hipLaunchKernelGGL((k_map<T, Op>), dim3(nb), dim3(TPB), 0, c.stream, c.B, c.vert, c.ramp, c.prm, tab, out);
+}
+
+// accumulate d(outputs)/d(inputs)^T obar into inbar[F] for every F whose
+// inbar pointer is non-null.  Defined in pom_gather_impl.h and instantiated
+// once per operator in gather_<op>.hip (parallel compilation units).
+template <class T, class Op>
+void launch_gather(const Ctx<T>& c, const FieldTab<T>& tab, const COutTab<T>& obar, T* const* inbar);
+
+}  // namespace pom

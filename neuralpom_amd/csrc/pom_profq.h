@@ -1,0 +1,457 @@
+// Mellor-Yamada 2.5 turbulence closure (profq), forward and reverse sweep.
+// reference: src/neuralpom/cores/explicit/tracers.py:158-431
+//
+// One thread per water column.  Inputs read from neighbouring columns (shear
+// uses u(i+1), v(j+1); the surface/bottom stress averages use i+1, j+1) make
+// the reverse sweep write "neighbour" cotangents to scratch columns that a
+// second pass gathers.
+#pragma once
+#include "pom_column.h"
+
+namespace pom {
+
+namespace myc {
+constexpr double A1 = 0.92, B1 = 16.6, A2 = 0.74, B2 = 10.1, C1 = 0.08;
+constexpr double E1 = 1.8, E2 = 1.33, SEF = 1.0, CBCNST = 100.0, SURFL = 2.0e5;
+constexpr double COEF4 = 18.0 * A1 * A1 + 9.0 * A1 * A2;
+constexpr double COEF5 = 9.0 * A1 * A2;
+constexpr double COEF1 = A2 * (1.0 - 6.0 * A1 / B1);
+constexpr double COEF2 = 3.0 * A2 * B2 + 18.0 * A1 * A2;
+constexpr double COEF3 = A1 * (1.0 - 3.0 * C1 - 6.0 * A1 / B1);
+}  // namespace myc
+
+template <class T> struct ProfqArgs {
+  const T *q2f, *q2lf, *q2, *q2b, *q2lb, *km, *kh, *kq, *u, *v, *t, *s, *rho;
+  const T *wusurf, *wvsurf, *wubot, *wvbot, *etf, *dt, *h;
+  T *q2o, *q2lo, *lo, *kqo, *kmo, *kho;
+};
+
+template <class T> __device__ __forceinline__ T sgn(T x) { return x > T(0) ? T(1) : (x < T(0) ? T(-1) : T(0)); }
+
+// all per-column intermediates of the forward sweep
+template <class T> struct ProfqCol {
+  T a[KB], c[KB], cc[KB], boygr[KB], lraw[KB], l[KB], ghr[KB], gh[KB], kn[KB], dtef[KB];
+  T ee1[KB], gg1[KB], q2n[KB], ee2[KB], gg2[KB], q2ln[KB];
+  T dh, utau2, l0, ubt, vbt, qbm, q2bot;
+  bool has_s, inner;
+};
+
+template <class T>
+__device__ __forceinline__ void profq_fwd_col(const Pos& p, const Vert<T>& vg, const Params& prm,
+                                              const ProfqArgs<T>& s, ProfqCol<T>& C) {
+  using namespace myc;
+  const int i = p.i, j = p.j, m = p.m;
+  const int64_t c = ix3(p, i, j, 0), c2 = ix2(p, i, j);
+  const T dti2 = T(prm.dti2), umol = T(prm.umol), grav = T(prm.grav), kappa = T(prm.kappa), small = T(prm.small);
+  C.dh = s.h[c2] + s.etf[c2];
+  const T dh = C.dh;
+  for (int k = 0; k < KB; ++k) C.a[k] = C.c[k] = T(0);
+  for (int k = 1; k < KBM1; ++k) {
+    C.a[k] = -dti2 * (s.kq[c + k + 1] + s.kq[c + k] + T(2.0 * prm.umol)) * T(0.5) /
+             (vg.dzz(m, k - 1) * vg.dz(m, k) * dh * dh);
+    C.c[k] = -dti2 * (s.kq[c + k - 1] + s.kq[c + k] + T(2.0 * prm.umol)) * T(0.5) /
+             (vg.dzz(m, k - 1) * vg.dz(m, k - 1) * dh * dh);
+  }
+  (void)umol;
+  C.has_s = (i < p.im - 1) && (j < p.jm - 1);
+  C.inner = i >= 1 && i <= p.im - 2 && j >= 1 && j <= p.jm - 2;
+  C.utau2 = T(0);
+  C.l0 = T(0);
+  C.q2bot = T(0);
+  C.ubt = C.vbt = C.qbm = T(0);
+  if (C.has_s) {
+    const T ust = T(0.5) * (s.wusurf[c2] + s.wusurf[ix2(p, i + 1, j)]);
+    const T vst = T(0.5) * (s.wvsurf[c2] + s.wvsurf[ix2(p, i, j + 1)]);
+    C.utau2 = sqrt(ust * ust + vst * vst + T(1e-8));
+    C.l0 = T(SURFL) * C.utau2 / grav;
+    C.ubt = T(0.5) * (s.wubot[c2] + s.wubot[ix2(p, i + 1, j)]);
+    C.vbt = T(0.5) * (s.wvbot[c2] + s.wvbot[ix2(p, i, j + 1)]);
+    C.qbm = sqrt(C.ubt * C.ubt + C.vbt * C.vbt + T(1e-8));
+    C.q2bot = C.qbm * T(pow(16.6, 2.0 / 3.0) * SEF);
+  }
+  // sound speed (tracers.py:245-255)
+  for (int k = 0; k < KB; ++k) C.cc[k] = T(0);
+  for (int k = 0; k < KBM1; ++k) {
+    const T tp = s.t[c + k] + T(prm.tbias), sp = s.s[c + k] + T(prm.sbias);
+    const T pp = T(prm.grav * prm.rhoref) * (-vg.zz(m, k) * dh) * T(1.e-4);
+    T cc0 = T(1449.1) + T(0.00821) * pp + T(4.55) * tp - T(0.045) * (tp * tp) + T(1.34) * (sp - T(35.0));
+    const T ccs = fabs(cc0) < T(1e-6) ? T(1e-6) : cc0;
+    const T rad = (T(1) - T(0.01642) * pp / ccs) * (T(1) - T(0.40) * pp / (ccs * ccs));
+    C.cc[k] = ccs / sqrt(fmax(rad, T(1e-8)));
+  }
+  for (int k = 0; k < KB; ++k) C.boygr[k] = T(0);
+  for (int k = 1; k < KBM1; ++k)
+    C.boygr[k] = grav * (s.rho[c + k - 1] - s.rho[c + k]) / (vg.dzz(m, k - 1) * dh) +
+                 T(prm.grav * prm.grav * 2.0) / (C.cc[k - 1] * C.cc[k - 1] + C.cc[k] * C.cc[k]);
+  // length scale (tracers.py:263-285)
+  for (int k = 0; k < KB; ++k) C.lraw[k] = C.ghr[k] = C.gh[k] = T(0);
+  for (int k = 1; k < KBM1; ++k) C.lraw[k] = fabs(s.q2lb[c + k]) / (fabs(s.q2b[c + k]) + T(1e-20));
+  const T kl0 = kappa * C.l0;
+  for (int k = 0; k < KBM1; ++k) C.l[k] = (vg.z(m, k) > T(-0.5)) ? (C.lraw[k] >= kl0 ? C.lraw[k] : kl0) : C.lraw[k];
+  for (int k = 1; k < KBM1; ++k) {
+    C.ghr[k] = (C.l[k] * C.l[k]) * C.boygr[k] / (fabs(s.q2b[c + k]) + T(1e-20));
+    C.gh[k] = C.ghr[k] <= T(0.028) ? C.ghr[k] : T(0.028);
+  }
+  C.l[0] = kappa * C.l0;
+  C.l[KB - 1] = T(0);
+  // shear production (tracers.py:287-300)
+  for (int k = 0; k < KB; ++k) C.kn[k] = T(0);
+  if (C.inner) {
+    const int64_t cu = ix3(p, i + 1, j, 0), cv = ix3(p, i, j + 1, 0);
+    for (int k = 1; k < KBM1; ++k) {
+      const T du = s.u[c + k] - s.u[c + k - 1] + s.u[cu + k] - s.u[cu + k - 1];
+      const T dv = s.v[c + k] - s.v[c + k - 1] + s.v[cv + k] - s.v[cv + k - 1];
+      const T dz2 = vg.dzz(m, k - 1) * dh;
+      const T val = s.km[c + k] * T(0.25) * T(SEF) * (du * du + dv * dv) / (dz2 * dz2);
+      C.kn[k] = val + s.kh[c + k] * C.boygr[k];
+    }
+  }
+  for (int k = 0; k < KB; ++k) C.dtef[k] = sqrt(fabs(s.q2b[c + k]) + T(1e-8)) * T(1) / (T(B1) * C.l[k] + small);
+  // q2 tridiagonal solve (tracers.py:302-346)
+  C.ee1[0] = T(0);
+  C.gg1[0] = C.has_s ? T(pow(15.8 * CBCNST, 2.0 / 3.0)) * C.utau2 : T(0);
+  for (int k = 1; k < KBM1; ++k) {
+    const T den = C.a[k] + C.c[k] * (T(1) - C.ee1[k - 1]) - (T(2) * dti2 * C.dtef[k] + T(1));
+    C.ee1[k] = C.a[k] / (den + T(1e-20));
+    C.gg1[k] = (T(-2.0 * prm.dti2) * C.kn[k] + C.c[k] * C.gg1[k - 1] - s.q2f[c + k]) / (den + T(1e-20));
+  }
+  C.q2n[KB - 1] = C.q2bot;
+  for (int k = KBM1 - 1; k >= 0; --k) C.q2n[k] = C.ee1[k] * C.q2n[k + 1] + C.gg1[k];
+  // q2l tridiagonal solve with wall proximity (tracers.py:348-384)
+  C.ee2[0] = C.gg2[0] = T(0);
+  for (int k = 1; k < KBM1; ++k) {
+    const T W = T(1) / (fabs(vg.z(m, k) - vg.z(m, 0)) + T(1e-20)) + T(1) / (fabs(vg.z(m, k) - vg.z(m, KB - 1)) + T(1e-20));
+    const T wall = W * C.l[k] / (dh * kappa);
+    const T dtk = C.dtef[k] * (T(1) + T(E2) * (wall * wall));
+    const T den = C.a[k] + C.c[k] * (T(1) - C.ee2[k - 1]) - (dti2 * dtk + T(1));
+    C.ee2[k] = C.a[k] / (den + T(1e-20));
+    C.gg2[k] = (dti2 * (-C.kn[k] * C.l[k] * T(E1)) + C.c[k] * C.gg2[k - 1] - s.q2lf[c + k]) / (den + T(1e-20));
+  }
+  C.q2ln[KB - 1] = T(0);
+  for (int k = KBM1 - 1; k >= 0; --k) C.q2ln[k] = C.ee2[k] * C.q2ln[k + 1] + C.gg2[k];
+}
+
+template <class T>
+__global__ void __launch_bounds__(TPB) k_profq(Batch B, Vert<T> vg, Params prm, ProfqArgs<T> s) {
+  using namespace myc;
+  Pos p;
+  if (!pos2(B, p)) return;
+  const int64_t c = ix3(p, p.i, p.j, 0), c2 = ix2(p, p.i, p.j);
+  ProfqCol<T> C;
+  profq_fwd_col(p, vg, prm, s, C);
+  const T small = T(prm.small);
+  for (int k = 0; k < KB; ++k) {
+    T q2c = C.q2n[k], q2lc = C.q2ln[k];
+    if (k >= 1 && k < KBM1 && (C.q2n[k] <= small || C.q2ln[k] <= small)) {
+      q2c = small;
+      q2lc = T(0.1) * s.dt[c2] * small;
+    }
+    const T sh = T(COEF1) / (T(1) - T(COEF2) * C.gh[k] + T(1e-20));
+    const T sm = (T(COEF3) + sh * T(COEF4) * C.gh[k]) / (T(1) - T(COEF5) * C.gh[k] + T(1e-20));
+    const T knf = C.l[k] * sqrt(fabs(s.q2[c + k]) + T(1e-8));
+    s.q2o[c + k] = q2c;
+    s.q2lo[c + k] = q2lc;
+    s.lo[c + k] = C.l[k];
+    s.kqo[c + k] = (knf * T(0.41) * sh + s.kq[c + k]) * T(0.5);
+    s.kmo[c + k] = (knf * sm + s.km[c + k]) * T(0.5);
+    s.kho[c + k] = (knf * sh + s.kh[c + k]) * T(0.5);
+  }
+}
+
+template <class T> struct ProfqAdjArgs {
+  ProfqArgs<T> fw;
+  const T *b_q2o, *b_q2lo, *b_lo, *b_kqo, *b_kmo, *b_kho;   // output cotangents (any may be null)
+  // accumulated input cotangents (own column)
+  T *b_q2f, *b_q2lf, *b_q2, *b_q2b, *b_q2lb, *b_km, *b_kh, *b_kq, *b_u, *b_v, *b_t, *b_s, *b_rho;
+  T *b_wubot, *b_wvbot, *b_etf, *b_dt;
+  // scratch for cotangents of the (i+1, j) / (i, j+1) neighbours
+  T *s_unb, *s_vnb, *s_wubnb, *s_wvbnb;
+};
+
+template <class T> __device__ __forceinline__ T ldb(const T* b, int64_t i) { return b ? b[i] : T(0); }
+
+template <class T>
+__global__ void __launch_bounds__(TPB) k_profq_adj(Batch B, Vert<T> vg, Params prm, ProfqAdjArgs<T> S) {
+  using namespace myc;
+  Pos p;
+  if (!pos2(B, p)) return;
+  const ProfqArgs<T>& s = S.fw;
+  const int m = p.m;
+  const int64_t c = ix3(p, p.i, p.j, 0), c2 = ix2(p, p.i, p.j);
+  ProfqCol<T> C;
+  profq_fwd_col(p, vg, prm, s, C);
+  const T dti2 = T(prm.dti2), grav = T(prm.grav), kappa = T(prm.kappa), small = T(prm.small);
+  const T dh = C.dh;
+
+  T bl[KB], bgh[KB], bq2n[KB], bq2ln[KB];
+  T bkq[KB], bkm[KB], bkh[KB], bq2[KB], bq2b[KB], bdt2 = T(0);
+  for (int k = 0; k < KB; ++k) {
+    bl[k] = ldb(S.b_lo, c + k);
+    bgh[k] = T(0);
+    bkq[k] = bkm[k] = bkh[k] = bq2[k] = bq2b[k] = T(0);
+    // mixing coefficients (tracers.py:405-423)
+    const T gh = C.gh[k];
+    const T Dh = T(1) - T(COEF2) * gh + T(1e-20);
+    const T sh = T(COEF1) / Dh;
+    const T N = T(COEF3) + sh * T(COEF4) * gh;
+    const T Dm = T(1) - T(COEF5) * gh + T(1e-20);
+    const T sm = N / Dm;
+    const T sq2 = sqrt(fabs(s.q2[c + k]) + T(1e-8));
+    const T knf = C.l[k] * sq2;
+    const T gq = ldb(S.b_kqo, c + k) * T(0.5), gm = ldb(S.b_kmo, c + k) * T(0.5), gh_ = ldb(S.b_kho, c + k) * T(0.5);
+    bkq[k] += gq;
+    bkm[k] += gm;
+    bkh[k] += gh_;
+    const T bknf = gq * T(0.41) * sh + gm * sm + gh_ * sh;
+    T bsh = gq * knf * T(0.41) + gh_ * knf;
+    const T bsm = gm * knf;
+    bl[k] += bknf * sq2;
+    bq2[k] += bknf * C.l[k] * sgn(s.q2[c + k]) / (T(2) * sq2);
+    const T bN = bsm / Dm, bDm = -bsm * sm / Dm;
+    bsh += bN * T(COEF4) * gh;
+    bgh[k] += bN * sh * T(COEF4) - bDm * T(COEF5);
+    const T bDh = -bsh * sh / Dh;
+    bgh[k] += -bDh * T(COEF2);
+    // clipping (tracers.py:386-403)
+    const T g2 = ldb(S.b_q2o, c + k), gl2 = ldb(S.b_q2lo, c + k);
+    if (k >= 1 && k < KBM1 && (C.q2n[k] <= small || C.q2ln[k] <= small)) {
+      bq2n[k] = T(0);
+      bq2ln[k] = T(0);
+      bdt2 += gl2 * T(0.1) * small;
+    } else {
+      bq2n[k] = g2;
+      bq2ln[k] = gl2;
+    }
+  }
+  T ba[KB], bc[KB], bkn[KB], bdtef[KB], bq2f[KB], bq2lf[KB];
+  for (int k = 0; k < KB; ++k) ba[k] = bc[k] = bkn[k] = bdtef[k] = bq2f[k] = bq2lf[k] = T(0);
+  T bdh = T(0);
+  // ---- q2l solve reverse
+  {
+    T bee[KB], bgg[KB];
+    for (int k = 0; k < KB; ++k) bee[k] = bgg[k] = T(0);
+    for (int k = 0; k <= KBM1 - 1; ++k) {
+      bee[k] += bq2ln[k] * C.q2ln[k + 1];
+      bq2ln[k + 1] += bq2ln[k] * C.ee2[k];
+      bgg[k] += bq2ln[k];
+    }
+    for (int k = KBM1 - 1; k >= 1; --k) {
+      const T W = T(1) / (fabs(vg.z(m, k) - vg.z(m, 0)) + T(1e-20)) + T(1) / (fabs(vg.z(m, k) - vg.z(m, KB - 1)) + T(1e-20));
+      const T wall = W * C.l[k] / (dh * kappa);
+      const T dtk = C.dtef[k] * (T(1) + T(E2) * (wall * wall));
+      const T den = C.a[k] + C.c[k] * (T(1) - C.ee2[k - 1]) - (dti2 * dtk + T(1));
+      const T de = den + T(1e-20);
+      const T bR = bgg[k] / de;
+      const T bde = -(bgg[k] * C.gg2[k] + bee[k] * C.ee2[k]) / de;
+      ba[k] += bee[k] / de + bde;
+      bkn[k] += bR * dti2 * (-C.l[k] * T(E1));
+      bl[k] += bR * dti2 * (-C.kn[k] * T(E1));
+      bc[k] += bR * C.gg2[k - 1] + bde * (T(1) - C.ee2[k - 1]);
+      bgg[k - 1] += bR * C.c[k];
+      bq2lf[k] -= bR;
+      bee[k - 1] -= bde * C.c[k];
+      const T bdtk = -bde * dti2;
+      bdtef[k] += bdtk * (T(1) + T(E2) * (wall * wall));
+      const T bwall = bdtk * C.dtef[k] * T(E2) * T(2) * wall;
+      bl[k] += bwall * W / (dh * kappa);
+      bdh += -bwall * wall / dh;
+    }
+  }
+  // ---- q2 solve reverse
+  T bq2bot;
+  {
+    T bee[KB], bgg[KB];
+    for (int k = 0; k < KB; ++k) bee[k] = bgg[k] = T(0);
+    for (int k = 0; k <= KBM1 - 1; ++k) {
+      bee[k] += bq2n[k] * C.q2n[k + 1];
+      bq2n[k + 1] += bq2n[k] * C.ee1[k];
+      bgg[k] += bq2n[k];
+    }
+    bq2bot = bq2n[KB - 1];
+    for (int k = KBM1 - 1; k >= 1; --k) {
+      const T den = C.a[k] + C.c[k] * (T(1) - C.ee1[k - 1]) - (T(2) * dti2 * C.dtef[k] + T(1));
+      const T de = den + T(1e-20);
+      const T bR = bgg[k] / de;
+      const T bde = -(bgg[k] * C.gg1[k] + bee[k] * C.ee1[k]) / de;
+      ba[k] += bee[k] / de + bde;
+      bkn[k] += bR * T(-2.0 * prm.dti2);
+      bc[k] += bR * C.gg1[k - 1] + bde * (T(1) - C.ee1[k - 1]);
+      bgg[k - 1] += bR * C.c[k];
+      bq2f[k] -= bR;
+      bee[k - 1] -= bde * C.c[k];
+      bdtef[k] += -bde * T(2) * dti2;
+    }
+    // gg1(0) = const * utau2: utau2 depends only on passive surface stress
+  }
+  // ---- dtef = sqrt(|q2b| + 1e-8) / (B1 l + small)
+  for (int k = 0; k < KB; ++k) {
+    const T qb = s.q2b[c + k];
+    const T sqb = sqrt(fabs(qb) + T(1e-8));
+    const T dd = T(B1) * C.l[k] + small;
+    bq2b[k] += bdtef[k] / dd * sgn(qb) / (T(2) * sqb);
+    bl[k] += -bdtef[k] * C.dtef[k] * T(B1) / dd;
+  }
+  // ---- shear production
+  T bboy[KB], bu[KB], bun[KB], bv[KB], bvn[KB];
+  for (int k = 0; k < KB; ++k) bboy[k] = bu[k] = bun[k] = bv[k] = bvn[k] = T(0);
+  if (C.inner) {
+    const int64_t cu = ix3(p, p.i + 1, p.j, 0), cv = ix3(p, p.i, p.j + 1, 0);
+    for (int k = 1; k < KBM1; ++k) {
+      const T du = s.u[c + k] - s.u[c + k - 1] + s.u[cu + k] - s.u[cu + k - 1];
+      const T dv = s.v[c + k] - s.v[c + k - 1] + s.v[cv + k] - s.v[cv + k - 1];
+      const T dz2 = vg.dzz(m, k - 1) * dh;
+      const T D2 = dz2 * dz2;
+      const T S2 = du * du + dv * dv;
+      const T val = s.km[c + k] * T(0.25) * T(SEF) * S2 / D2;
+      const T g = bkn[k];
+      bkh[k] += g * C.boygr[k];
+      bboy[k] += g * s.kh[c + k];
+      bkm[k] += g * T(0.25) * T(SEF) * S2 / D2;
+      const T bS = g * s.km[c + k] * T(0.25) * T(SEF) / D2;
+      bdh += -g * val / D2 * T(2) * dz2 * vg.dzz(m, k - 1);
+      const T bdu = bS * T(2) * du, bdv = bS * T(2) * dv;
+      bu[k] += bdu;
+      bu[k - 1] -= bdu;
+      bun[k] += bdu;
+      bun[k - 1] -= bdu;
+      bv[k] += bdv;
+      bv[k - 1] -= bdv;
+      bvn[k] += bdv;
+      bvn[k - 1] -= bdv;
+    }
+  }
+  // ---- stability-function argument gh and length scale
+  T bl0 = T(0);
+  T bq2lb[KB];
+  for (int k = 0; k < KB; ++k) bq2lb[k] = T(0);
+  for (int k = 1; k < KBM1; ++k) {
+    if (C.ghr[k] <= T(0.028)) {
+      const T qb = s.q2b[c + k];
+      const T Dq = fabs(qb) + T(1e-20);
+      const T g = bgh[k];
+      bl[k] += g * T(2) * C.l[k] * C.boygr[k] / Dq;
+      bboy[k] += g * C.l[k] * C.l[k] / Dq;
+      bq2b[k] += -g * C.ghr[k] / Dq * sgn(qb);
+    }
+  }
+  bl0 += bl[0] * kappa;
+  const T kl0 = kappa * C.l0;
+  for (int k = 1; k < KBM1; ++k) {
+    T blr = bl[k];
+    if (vg.z(m, k) > T(-0.5) && !(C.lraw[k] >= kl0)) {
+      bl0 += bl[k] * kappa;
+      blr = T(0);
+    }
+    const T qb = s.q2b[c + k], qlb = s.q2lb[c + k];
+    const T Dq = fabs(qb) + T(1e-20);
+    bq2lb[k] += blr * sgn(qlb) / Dq;
+    bq2b[k] += -blr * C.lraw[k] / Dq * sgn(qb);
+  }
+  (void)bl0;   // l0 depends only on the passive surface stress
+  // ---- buoyancy
+  T bcc[KB], brho[KB];
+  for (int k = 0; k < KB; ++k) bcc[k] = brho[k] = T(0);
+  for (int k = 1; k < KBM1; ++k) {
+    const T g = bboy[k];
+    const T dd = vg.dzz(m, k - 1) * dh;
+    const T b1t = grav * (s.rho[c + k - 1] - s.rho[c + k]) / dd;
+    brho[k - 1] += g * grav / dd;
+    brho[k] -= g * grav / dd;
+    bdh += -g * b1t / dh;
+    const T Sc = C.cc[k - 1] * C.cc[k - 1] + C.cc[k] * C.cc[k];
+    const T b2t = T(prm.grav * prm.grav * 2.0) / Sc;
+    const T bSc = -g * b2t / Sc;
+    bcc[k - 1] += bSc * T(2) * C.cc[k - 1];
+    bcc[k] += bSc * T(2) * C.cc[k];
+  }
+  T bt[KB], bs[KB];
+  for (int k = 0; k < KB; ++k) bt[k] = bs[k] = T(0);
+  for (int k = 0; k < KBM1; ++k) {
+    const T tp = s.t[c + k] + T(prm.tbias), sp = s.s[c + k] + T(prm.sbias);
+    const T cz = T(prm.grav * prm.rhoref) * (-vg.zz(m, k));
+    const T pp = cz * dh * T(1.e-4);
+    const T cc0 = T(1449.1) + T(0.00821) * pp + T(4.55) * tp - T(0.045) * (tp * tp) + T(1.34) * (sp - T(35.0));
+    const bool rep = fabs(cc0) < T(1e-6);
+    const T ccs = rep ? T(1e-6) : cc0;
+    const T X = T(1) - T(0.01642) * pp / ccs;
+    const T Y = T(1) - T(0.40) * pp / (ccs * ccs);
+    const T rad = X * Y;
+    const T sq = sqrt(fmax(rad, T(1e-8)));
+    const T g = bcc[k];
+    T bccs = g / sq;
+    const T bsq = -g * C.cc[k] / sq;
+    const T brad = rad >= T(1e-8) ? bsq / (T(2) * sq) : T(0);
+    const T bX = brad * Y, bY = brad * X;
+    T bp = -bX * T(0.01642) / ccs - bY * T(0.40) / (ccs * ccs);
+    bccs += bX * T(0.01642) * pp / (ccs * ccs) + bY * T(0.40) * pp * T(2) / (ccs * ccs * ccs);
+    if (!rep) {
+      bp += bccs * T(0.00821);
+      bt[k] += bccs * (T(4.55) - T(0.045) * T(2) * tp);
+      bs[k] += bccs * T(1.34);
+    }
+    bdh += bp * cz * T(1.e-4);
+  }
+  // ---- bottom boundary value q2bot = const1 * sqrt(ubt^2 + vbt^2 + 1e-8)
+  T bwub = T(0), bwvb = T(0);
+  if (C.has_s) {
+    const T g = bq2bot * T(pow(16.6, 2.0 / 3.0) * SEF) / C.qbm;
+    bwub = T(0.5) * g * C.ubt;
+    bwvb = T(0.5) * g * C.vbt;
+  }
+  // ---- diffusion coefficients a, c from kq and dh
+  for (int k = 1; k < KBM1; ++k) {
+    const T Da = vg.dzz(m, k - 1) * vg.dz(m, k) * dh * dh;
+    const T Dc = vg.dzz(m, k - 1) * vg.dz(m, k - 1) * dh * dh;
+    const T ga = ba[k] * (-dti2) * T(0.5) / Da;
+    const T gc = bc[k] * (-dti2) * T(0.5) / Dc;
+    bkq[k + 1] += ga;
+    bkq[k] += ga + gc;
+    bkq[k - 1] += gc;
+    bdh += -ba[k] * C.a[k] * T(2) / dh - bc[k] * C.c[k] * T(2) / dh;
+  }
+  // ---- store
+  if (S.b_q2f) add_col(S.b_q2f + c, bq2f);
+  if (S.b_q2lf) add_col(S.b_q2lf + c, bq2lf);
+  if (S.b_q2) add_col(S.b_q2 + c, bq2);
+  if (S.b_q2b) add_col(S.b_q2b + c, bq2b);
+  if (S.b_q2lb) add_col(S.b_q2lb + c, bq2lb);
+  if (S.b_km) add_col(S.b_km + c, bkm);
+  if (S.b_kh) add_col(S.b_kh + c, bkh);
+  if (S.b_kq) add_col(S.b_kq + c, bkq);
+  if (S.b_u) add_col(S.b_u + c, bu);
+  if (S.b_v) add_col(S.b_v + c, bv);
+  if (S.b_t) add_col(S.b_t + c, bt);
+  if (S.b_s) add_col(S.b_s + c, bs);
+  if (S.b_rho) add_col(S.b_rho + c, brho);
+  st_col(S.s_unb + c, bun);
+  st_col(S.s_vnb + c, bvn);
+  S.s_wubnb[c2] = bwub;
+  S.s_wvbnb[c2] = bwvb;
+  if (S.b_wubot) S.b_wubot[c2] += bwub;
+  if (S.b_wvbot) S.b_wvbot[c2] += bwvb;
+  if (S.b_etf) S.b_etf[c2] += bdh;
+  if (S.b_dt) S.b_dt[c2] += bdt2;
+}
+
+// gather the neighbour cotangents: column (i, j) receives what column (i-1, j)
+// wrote for u / wubot and what column (i, j-1) wrote for v / wvbot
+template <class T>
+__global__ void __launch_bounds__(TPB) k_profq_adj2(Batch B, ProfqAdjArgs<T> S) {
+  Pos p;
+  if (!pos2(B, p)) return;
+  const int64_t c = ix3(p, p.i, p.j, 0), c2 = ix2(p, p.i, p.j);
+  if (p.i >= 1) {
+    const int64_t o = ix3(p, p.i - 1, p.j, 0);
+    if (S.b_u)
+      for (int k = 0; k < KB; ++k) S.b_u[c + k] += S.s_unb[o + k];
+    if (S.b_wubot) S.b_wubot[c2] += S.s_wubnb[ix2(p, p.i - 1, p.j)];
+  }
+  if (p.j >= 1) {
+    const int64_t o = ix3(p, p.i, p.j - 1, 0);
+    if (S.b_v)
+      for (int k = 0; k < KB; ++k) S.b_v[c + k] += S.s_vnb[o + k];
+    if (S.b_wvbot) S.b_wvbot[c2] += S.s_wvbnb[ix2(p, p.i, p.j - 1)];
+  }
+}
+
+}  // namespace pom

@@ -1,0 +1,1138 @@
+// Host orchestration of the explicit POM step and its adjoint, and the C ABI.
+//
+// Forward: the stages of src/neuralpom/cores/explicit/time_loop.py:75-696 in
+// order, every intermediate in its own workspace buffer (static single
+// assignment), so the same forward doubles as the recompute pass of the
+// adjoint.  Adjoint: the stages reversed; stencil stages through the generated
+// gathers of pom_kernels.h, column stages through the hand-written reverse
+// sweeps of pom_column.h / pom_profq.h.
+#include <vector>
+#include <cstring>
+#include <algorithm>
+
+#include "../../include/neuralpom_amd.h"
+#include "pom_kernels.h"
+#include "pom_profq.h"
+
+using namespace pom;
+
+struct pom_batch {
+  int32_t dtype;
+  int32_t M;
+  Member* d_mem;
+  int32_t* d_blk3;
+  int32_t* d_blk2;
+  int32_t nblk3, nblk2;
+  int64_t n2, n3;
+};
+
+namespace {
+
+constexpr int MAXSPLIT = 256;
+
+// ---------------------------------------------------------------- workspace
+struct Arena {
+  char* base;
+  size_t off, cap;
+  bool measure;
+  template <class T> T* get(int64_t n) {
+    off = (off + 255) & ~size_t(255);
+    T* p = measure ? nullptr : reinterpret_cast<T*>(base + off);
+    off += size_t(n) * sizeof(T);
+    return p;
+  }
+};
+
+template <class T>
+__global__ void k_fill(T* p, int64_t n, T v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+template <class T>
+__global__ void k_copy(T* __restrict__ d, const T* __restrict__ s, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = s[i];
+}
+template <class T>
+__global__ void k_add(T* __restrict__ d, const T* __restrict__ s, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] += s[i];
+}
+// dt = h + et (time_loop.py:693)
+template <class T>
+__global__ void k_hplus(T* __restrict__ d, const T* __restrict__ h, const T* __restrict__ e, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = h[i] + e[i];
+}
+// running max |vaf| (time_loop.py:393)
+__device__ __forceinline__ void amax(float* f, float v) { atomicMax(reinterpret_cast<unsigned int*>(f), __float_as_uint(v)); }
+__device__ __forceinline__ void amax(double* f, double v) {
+  atomicMax(reinterpret_cast<unsigned long long*>(f), (unsigned long long)__double_as_longlong(v));
+}
+template <class T>
+__global__ void k_vamax(const T* __restrict__ vaf, int64_t n, T* out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  T v = i < n ? fabs(vaf[i]) : T(0);
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  if ((threadIdx.x & 63) == 0 && v > T(0)) amax(out, v);
+}
+
+inline int blocks(int64_t n) { return (int)((n + 255) / 256); }
+
+template <class T> struct X {   // state of one barotropic substep
+  T *ua, *uab, *va, *vab, *el, *elb, *d, *egf, *utf, *vtf, *etf, *advua, *advva, *uaf, *vaf, *elf, *fluxua, *fluxva,
+      *tps;
+};
+
+template <class T> struct Step {
+  const pom_batch* b;
+  Ctx<T> c;
+  const T* in[POM_NSTATE];
+  T* out[POM_NSTATE];
+  const T* g[POM_NGEOM];
+  int isplit;
+  bool internal;
+  T* vamax;
+  // intermediates
+  T *rho1, *adx2d_raw, *ady2d_raw, *advua0, *advva0, *fluxua0, *fluxva0, *tps0, *egf0, *utf0, *vtf0, *elfs;
+  X<T> xs[MAXSPLIT + 1];
+  T *u1, *v1, *q2f0, *q2lf0, *q2f1, *q2lf1, *kmraw, *khraw, *tf0, *sf0, *tf1, *sf1, *uf0, *vf0, *uf1, *vf1, *uf2,
+      *vf2, *tpsu;
+
+  int64_t n3() const { return b->n3; }
+  int64_t n2() const { return b->n2; }
+
+  void plan(Arena& A, bool own_out) {
+    const int64_t N3 = n3(), N2 = n2();
+    if (own_out)
+      for (int f = 0; f < POM_NSTATE; ++f) out[f] = A.get<T>(f < POM_N3D ? N3 : N2);
+    rho1 = A.get<T>(N3);
+    adx2d_raw = A.get<T>(N2);
+    ady2d_raw = A.get<T>(N2);
+    advua0 = A.get<T>(N2);
+    advva0 = A.get<T>(N2);
+    fluxua0 = A.get<T>(N2);
+    fluxva0 = A.get<T>(N2);
+    tps0 = A.get<T>(N2);
+    egf0 = A.get<T>(N2);
+    utf0 = A.get<T>(N2);
+    vtf0 = A.get<T>(N2);
+    for (int n = 1; n <= isplit; ++n) {
+      X<T>& x = xs[n];
+      const bool last = n == isplit;
+      x.ua = last ? out[POM_UA] : A.get<T>(N2);
+      x.uab = last ? out[POM_UAB] : A.get<T>(N2);
+      x.va = last ? out[POM_VA] : A.get<T>(N2);
+      x.vab = last ? out[POM_VAB] : A.get<T>(N2);
+      x.el = last ? out[POM_EL] : A.get<T>(N2);
+      x.elb = last ? out[POM_ELB] : A.get<T>(N2);
+      x.d = last ? out[POM_D] : A.get<T>(N2);
+      x.egf = last ? out[POM_EGF] : A.get<T>(N2);
+      x.utf = last ? out[POM_UTF] : A.get<T>(N2);
+      x.vtf = last ? out[POM_VTF] : A.get<T>(N2);
+      x.etf = last ? out[POM_ETF] : A.get<T>(N2);
+      x.uaf = last ? out[POM_UAF] : A.get<T>(N2);
+      x.vaf = last ? out[POM_VAF] : A.get<T>(N2);
+      x.elf = last ? out[POM_ELF] : A.get<T>(N2);
+      if (n % c.prm.ispadv == 0) {
+        const bool lastadv = n + c.prm.ispadv > isplit;
+        x.advua = lastadv ? out[POM_ADVUA] : A.get<T>(N2);
+        x.advva = lastadv ? out[POM_ADVVA] : A.get<T>(N2);
+        x.fluxua = lastadv ? out[POM_FLUXUA] : A.get<T>(N2);
+        x.fluxva = lastadv ? out[POM_FLUXVA] : A.get<T>(N2);
+        x.tps = A.get<T>(N2);
+      } else {
+        x.advua = x.advva = x.fluxua = x.fluxva = x.tps = nullptr;   // carried
+      }
+    }
+    u1 = A.get<T>(N3);
+    v1 = A.get<T>(N3);
+    q2f0 = A.get<T>(N3);
+    q2lf0 = A.get<T>(N3);
+    q2f1 = A.get<T>(N3);
+    q2lf1 = A.get<T>(N3);
+    kmraw = A.get<T>(N3);
+    khraw = A.get<T>(N3);
+    tf0 = A.get<T>(N3);
+    sf0 = A.get<T>(N3);
+    tf1 = A.get<T>(N3);
+    sf1 = A.get<T>(N3);
+    uf0 = A.get<T>(N3);
+    vf0 = A.get<T>(N3);
+    uf1 = A.get<T>(N3);
+    vf1 = A.get<T>(N3);
+    uf2 = A.get<T>(N3);
+    vf2 = A.get<T>(N3);
+    tpsu = A.get<T>(N2);
+  }
+
+  void copy(T* d, const T* s, int64_t n) {
+    if (d == s) return;
+    hipLaunchKernelGGL(k_copy<T>, dim3(blocks(n)), dim3(256), 0, c.stream, d, s, n);
+  }
+
+  // X_0: the state entering the substeps
+  X<T> x0() const {
+    X<T> x;
+    x.ua = (T*)in[POM_UA];
+    x.uab = (T*)in[POM_UAB];
+    x.va = (T*)in[POM_VA];
+    x.vab = (T*)in[POM_VAB];
+    x.el = (T*)in[POM_EL];
+    x.elb = (T*)in[POM_ELB];
+    x.d = (T*)in[POM_D];
+    x.egf = egf0;
+    x.utf = utf0;
+    x.vtf = vtf0;
+    x.etf = (T*)in[POM_ETF];
+    x.advua = advua0;
+    x.advva = advva0;
+    x.fluxua = fluxua0;
+    x.fluxva = fluxva0;
+    x.tps = tps0;
+    x.uaf = (T*)in[POM_UAF];
+    x.vaf = (T*)in[POM_VAF];
+    x.elf = (T*)in[POM_ELF];
+    return x;
+  }
+  // resolve carried advave fields of substep n
+  void resolve(int n, X<T>& x) const {
+    if (!x.advua) {
+      const X<T>& p = n == 1 ? x0_ : xs[n - 1];
+      x.advua = p.advua;
+      x.advva = p.advva;
+      x.fluxua = p.fluxua;
+      x.fluxva = p.fluxva;
+      x.tps = p.tps;
+    }
+  }
+  X<T> x0_;
+
+  // ------------------------------------------------------------ field tables
+  FieldTab<T> t_advct() const {
+    FieldTab<T> t{};
+    t.f[OpAdvct::U] = in[POM_U];
+    t.f[OpAdvct::V] = in[POM_V];
+    t.f[OpAdvct::UB] = in[POM_UB];
+    t.f[OpAdvct::VB] = in[POM_VB];
+    t.f[OpAdvct::AAM] = in[POM_AAM];
+    t.f[OpAdvct::DT] = in[POM_DT];
+    t.g[OpAdvct::GDX] = g[POM_G_DX];
+    t.g[OpAdvct::GDY] = g[POM_G_DY];
+    t.g[OpAdvct::GARU] = g[POM_G_ARU];
+    t.g[OpAdvct::GARV] = g[POM_G_ARV];
+    return t;
+  }
+  FieldTab<T> t_smag() const {
+    FieldTab<T> t{};
+    t.f[OpSmag::U] = in[POM_U];
+    t.f[OpSmag::V] = in[POM_V];
+    t.f[OpSmag::AAM] = in[POM_AAM];
+    t.g[OpSmag::GDX] = g[POM_G_DX];
+    t.g[OpSmag::GDY] = g[POM_G_DY];
+    return t;
+  }
+  FieldTab<T> t_advave(const X<T>& x) const {
+    FieldTab<T> t{};
+    t.f[OpAdvave::UA] = x.ua;
+    t.f[OpAdvave::VA] = x.va;
+    t.f[OpAdvave::UAB] = x.uab;
+    t.f[OpAdvave::VAB] = x.vab;
+    t.f[OpAdvave::D_] = x.d;
+    t.f[OpAdvave::AAM2D] = out[POM_AAM2D];
+    t.g[OpAdvave::GDX] = g[POM_G_DX];
+    t.g[OpAdvave::GDY] = g[POM_G_DY];
+    t.g[OpAdvave::GFLUXUA] = in[POM_FLUXUA];
+    t.g[OpAdvave::GFLUXVA] = in[POM_FLUXVA];
+    t.g[OpAdvave::GTPS] = in[POM_TPS];
+    return t;
+  }
+  FieldTab<T> t_prep() const {
+    FieldTab<T> t{};
+    t.f[OpExtPrep::ADX2D] = adx2d_raw;
+    t.f[OpExtPrep::ADY2D] = ady2d_raw;
+    t.f[OpExtPrep::ADVUA] = advua0;
+    t.f[OpExtPrep::ADVVA] = advva0;
+    t.f[OpExtPrep::EL] = in[POM_EL];
+    t.f[OpExtPrep::UA] = in[POM_UA];
+    t.f[OpExtPrep::VA] = in[POM_VA];
+    t.f[OpExtPrep::D_] = in[POM_D];
+    t.f[OpExtPrep::UTF] = in[POM_UTF];
+    t.f[OpExtPrep::VTF] = in[POM_VTF];
+    return t;
+  }
+  FieldTab<T> t_elf(const X<T>& x) const {
+    FieldTab<T> t{};
+    t.f[OpExtElf::UA] = x.ua;
+    t.f[OpExtElf::VA] = x.va;
+    t.f[OpExtElf::D_] = x.d;
+    t.f[OpExtElf::ELB] = x.elb;
+    t.g[OpExtElf::GDX] = g[POM_G_DX];
+    t.g[OpExtElf::GDY] = g[POM_G_DY];
+    t.g[OpExtElf::GART] = g[POM_G_ART];
+    t.g[OpExtElf::GFSM] = g[POM_G_FSM];
+    t.g[OpExtElf::GVFLUXF] = out[POM_VFLUXF];
+    return t;
+  }
+  FieldTab<T> t_uv(const X<T>& xp, const X<T>& xn) const {
+    FieldTab<T> t{};
+    t.f[OpExtUV::EL] = xp.el;
+    t.f[OpExtUV::ELB] = xp.elb;
+    t.f[OpExtUV::ELF] = xn.elf;
+    t.f[OpExtUV::D_] = xp.d;
+    t.f[OpExtUV::UA] = xp.ua;
+    t.f[OpExtUV::VA] = xp.va;
+    t.f[OpExtUV::UAB] = xp.uab;
+    t.f[OpExtUV::VAB] = xp.vab;
+    t.f[OpExtUV::ADVUA] = xn.advua;
+    t.f[OpExtUV::ADVVA] = xn.advva;
+    t.f[OpExtUV::ADX2D] = out[POM_ADX2D];
+    t.f[OpExtUV::ADY2D] = out[POM_ADY2D];
+    t.f[OpExtUV::DRX2D] = out[POM_DRX2D];
+    t.f[OpExtUV::DRY2D] = out[POM_DRY2D];
+    t.f[OpExtUV::WUBOT] = in[POM_WUBOT];
+    t.f[OpExtUV::WVBOT] = in[POM_WVBOT];
+    t.f[OpExtUV::UAFIN] = xp.uaf;
+    t.f[OpExtUV::VAFIN] = xp.vaf;
+    t.g[OpExtUV::GCOR] = g[POM_G_COR];
+    t.g[OpExtUV::GDX] = g[POM_G_DX];
+    t.g[OpExtUV::GDY] = g[POM_G_DY];
+    t.g[OpExtUV::GH] = g[POM_G_H];
+    t.g[OpExtUV::GARU] = g[POM_G_ARU];
+    t.g[OpExtUV::GARV] = g[POM_G_ARV];
+    t.g[OpExtUV::GDUM] = g[POM_G_DUM];
+    t.g[OpExtUV::GDVM] = g[POM_G_DVM];
+    t.g[OpExtUV::GEATM] = out[POM_E_ATMOS];
+    t.g[OpExtUV::GWUSURF] = g[POM_G_WUSURF];
+    t.g[OpExtUV::GWVSURF] = g[POM_G_WVSURF];
+    return t;
+  }
+  FieldTab<T> t_filter(const X<T>& xp, const X<T>& xn) const {
+    FieldTab<T> t{};
+    t.f[OpExtFilter::UA] = xp.ua;
+    t.f[OpExtFilter::UAB] = xp.uab;
+    t.f[OpExtFilter::VA] = xp.va;
+    t.f[OpExtFilter::VAB] = xp.vab;
+    t.f[OpExtFilter::EL] = xp.el;
+    t.f[OpExtFilter::ELB] = xp.elb;
+    t.f[OpExtFilter::UAF] = xn.uaf;
+    t.f[OpExtFilter::VAF] = xn.vaf;
+    t.f[OpExtFilter::ELF] = xn.elf;
+    t.f[OpExtFilter::EGF] = xp.egf;
+    t.f[OpExtFilter::UTF] = xp.utf;
+    t.f[OpExtFilter::VTF] = xp.vtf;
+    t.f[OpExtFilter::ETF] = xp.etf;
+    t.g[OpExtFilter::GH] = g[POM_G_H];
+    t.g[OpExtFilter::GFSM] = g[POM_G_FSM];
+    return t;
+  }
+  const X<T>& xf() const { return xs[isplit]; }
+  FieldTab<T> t_advq(bool l2) const {
+    FieldTab<T> t{};
+    t.f[OpAdvq::QB] = in[l2 ? POM_Q2LB : POM_Q2B];
+    t.f[OpAdvq::Q] = in[l2 ? POM_Q2L : POM_Q2];
+    t.f[OpAdvq::U] = u1;
+    t.f[OpAdvq::V] = v1;
+    t.f[OpAdvq::W] = out[POM_W];
+    t.f[OpAdvq::AAM] = out[POM_AAM];
+    t.f[OpAdvq::DT] = in[POM_DT];
+    t.f[OpAdvq::ETB] = in[POM_ETB];
+    t.f[OpAdvq::ETF] = xf().etf;
+    t.g[OpAdvq::GDX] = g[POM_G_DX];
+    t.g[OpAdvq::GDY] = g[POM_G_DY];
+    t.g[OpAdvq::GH] = g[POM_G_H];
+    t.g[OpAdvq::GART] = g[POM_G_ART];
+    t.g[OpAdvq::GDUM] = g[POM_G_DUM];
+    t.g[OpAdvq::GDVM] = g[POM_G_DVM];
+    return t;
+  }
+  FieldTab<T> t_kedge() const {
+    FieldTab<T> t{};
+    t.f[OpKEdge::KM] = kmraw;
+    t.f[OpKEdge::KH] = khraw;
+    t.g[OpKEdge::GFSM] = g[POM_G_FSM];
+    return t;
+  }
+  FieldTab<T> t_bc6() const {
+    FieldTab<T> t{};
+    t.f[OpBc6Filter::Q2F] = q2f1;
+    t.f[OpBc6Filter::Q2LF] = q2lf1;
+    t.f[OpBc6Filter::Q2] = in[POM_Q2];
+    t.f[OpBc6Filter::Q2L] = in[POM_Q2L];
+    t.f[OpBc6Filter::Q2B] = in[POM_Q2B];
+    t.f[OpBc6Filter::Q2LB] = in[POM_Q2LB];
+    t.f[OpBc6Filter::U] = u1;
+    t.f[OpBc6Filter::V] = v1;
+    t.g[OpBc6Filter::GDX] = g[POM_G_DX];
+    t.g[OpBc6Filter::GDY] = g[POM_G_DY];
+    t.g[OpBc6Filter::GFSM] = g[POM_G_FSM];
+    return t;
+  }
+  FieldTab<T> t_advt1(bool sal) const {
+    FieldTab<T> t{};
+    t.f[OpAdvt1::FB] = in[sal ? POM_SB : POM_TB];
+    t.f[OpAdvt1::F] = in[sal ? POM_S : POM_T];
+    t.f[OpAdvt1::FCLIM] = g[sal ? POM_G_SCLIM : POM_G_TCLIM];
+    t.f[OpAdvt1::U] = u1;
+    t.f[OpAdvt1::V] = v1;
+    t.f[OpAdvt1::W] = out[POM_W];
+    t.f[OpAdvt1::AAM] = out[POM_AAM];
+    t.f[OpAdvt1::DT] = in[POM_DT];
+    t.f[OpAdvt1::ETB] = in[POM_ETB];
+    t.f[OpAdvt1::ETF] = xf().etf;
+    t.g[OpAdvt1::GDX] = g[POM_G_DX];
+    t.g[OpAdvt1::GDY] = g[POM_G_DY];
+    t.g[OpAdvt1::GH] = g[POM_G_H];
+    t.g[OpAdvt1::GART] = g[POM_G_ART];
+    t.g[OpAdvt1::GDUM] = g[POM_G_DUM];
+    t.g[OpAdvt1::GDVM] = g[POM_G_DVM];
+    return t;
+  }
+  FieldTab<T> t_bc4() const {
+    FieldTab<T> t{};
+    t.f[OpBc4Filter::TF] = tf1;
+    t.f[OpBc4Filter::SF] = sf1;
+    t.f[OpBc4Filter::TT] = in[POM_T];
+    t.f[OpBc4Filter::SS] = in[POM_S];
+    t.f[OpBc4Filter::TB] = in[POM_TB];
+    t.f[OpBc4Filter::SB] = in[POM_SB];
+    t.f[OpBc4Filter::U] = u1;
+    t.f[OpBc4Filter::V] = v1;
+    t.f[OpBc4Filter::W] = out[POM_W];
+    t.f[OpBc4Filter::DT] = in[POM_DT];
+    t.g[OpBc4Filter::GDX] = g[POM_G_DX];
+    t.g[OpBc4Filter::GDY] = g[POM_G_DY];
+    t.g[OpBc4Filter::GFSM] = g[POM_G_FSM];
+    return t;
+  }
+  FieldTab<T> t_dens() const {
+    FieldTab<T> t{};
+    t.f[OpDens::TT] = out[POM_T];
+    t.f[OpDens::SS] = out[POM_S];
+    t.f[OpDens::RHO] = rho1;
+    t.g[OpDens::GH] = g[POM_G_H];
+    t.g[OpDens::GFSM] = g[POM_G_FSM];
+    return t;
+  }
+  FieldTab<T> t_advu() const {
+    FieldTab<T> t{};
+    t.f[OpAdvu::ADVX] = out[POM_ADVX];
+    t.f[OpAdvu::DRHOX] = out[POM_DRHOX];
+    t.f[OpAdvu::U] = u1;
+    t.f[OpAdvu::V] = v1;
+    t.f[OpAdvu::W] = out[POM_W];
+    t.f[OpAdvu::UB] = in[POM_UB];
+    t.f[OpAdvu::DT] = in[POM_DT];
+    t.f[OpAdvu::EGF] = xf().egf;
+    t.f[OpAdvu::EGB] = in[POM_EGB];
+    t.f[OpAdvu::ETF] = xf().etf;
+    t.f[OpAdvu::ETB] = in[POM_ETB];
+    t.g[OpAdvu::GARU] = g[POM_G_ARU];
+    t.g[OpAdvu::GCOR] = g[POM_G_COR];
+    t.g[OpAdvu::GDY] = g[POM_G_DY];
+    t.g[OpAdvu::GH] = g[POM_G_H];
+    t.g[OpAdvu::GEATM] = out[POM_E_ATMOS];
+    return t;
+  }
+  FieldTab<T> t_advv() const {
+    FieldTab<T> t{};
+    t.f[OpAdvv::ADVY] = out[POM_ADVY];
+    t.f[OpAdvv::DRHOY] = out[POM_DRHOY];
+    t.f[OpAdvv::U] = u1;
+    t.f[OpAdvv::V] = v1;
+    t.f[OpAdvv::W] = out[POM_W];
+    t.f[OpAdvv::VB] = in[POM_VB];
+    t.f[OpAdvv::DT] = in[POM_DT];
+    t.f[OpAdvv::EGF] = xf().egf;
+    t.f[OpAdvv::EGB] = in[POM_EGB];
+    t.f[OpAdvv::ETF] = xf().etf;
+    t.f[OpAdvv::ETB] = in[POM_ETB];
+    t.g[OpAdvv::GARV] = g[POM_G_ARV];
+    t.g[OpAdvv::GCOR] = g[POM_G_COR];
+    t.g[OpAdvv::GDX] = g[POM_G_DX];
+    t.g[OpAdvv::GH] = g[POM_G_H];
+    t.g[OpAdvv::GEATM] = out[POM_E_ATMOS];
+    return t;
+  }
+  FieldTab<T> t_bc3() const {
+    FieldTab<T> t{};
+    t.f[OpBc3::UF] = uf1;
+    t.f[OpBc3::VF] = vf1;
+    t.f[OpBc3::U] = u1;
+    t.f[OpBc3::V] = v1;
+    t.g[OpBc3::GH] = g[POM_G_H];
+    t.g[OpBc3::GDUM] = g[POM_G_DUM];
+    t.g[OpBc3::GDVM] = g[POM_G_DVM];
+    return t;
+  }
+
+  ProfqArgs<T> a_profq() const {
+    ProfqArgs<T> s{};
+    s.q2f = q2f0;
+    s.q2lf = q2lf0;
+    s.q2 = in[POM_Q2];
+    s.q2b = in[POM_Q2B];
+    s.q2lb = in[POM_Q2LB];
+    s.km = in[POM_KM];
+    s.kh = in[POM_KH];
+    s.kq = in[POM_KQ];
+    s.u = u1;
+    s.v = v1;
+    s.t = in[POM_T];
+    s.s = in[POM_S];
+    s.rho = rho1;
+    s.wusurf = g[POM_G_WUSURF];
+    s.wvsurf = g[POM_G_WVSURF];
+    s.wubot = in[POM_WUBOT];
+    s.wvbot = in[POM_WVBOT];
+    s.etf = xf().etf;
+    s.dt = in[POM_DT];
+    s.h = g[POM_G_H];
+    s.q2o = q2f1;
+    s.q2lo = q2lf1;
+    s.lo = out[POM_L];
+    s.kqo = out[POM_KQ];
+    s.kmo = kmraw;
+    s.kho = khraw;
+    return s;
+  }
+  ProftArgs<T> a_proft(bool sal) const {
+    ProftArgs<T> s{};
+    s.f = sal ? sf0 : tf0;
+    s.kh = out[POM_KH];
+    s.etf = xf().etf;
+    s.h = g[POM_G_H];
+    s.wfsurf = out[sal ? POM_WSSURF : POM_WTSURF];
+    s.fsurf = g[sal ? POM_G_SSURF : POM_G_TSURF];
+    s.out = sal ? sf1 : tf1;
+    s.nbc = sal ? c.prm.nbcs : c.prm.nbct;
+    return s;
+  }
+  ProfuvArgs<T> a_profuv(bool vdir) const {
+    ProfuvArgs<T> s{};
+    s.f = vdir ? vf0 : uf0;
+    s.km = out[POM_KM];
+    s.etf = xf().etf;
+    s.h = g[POM_G_H];
+    s.wsurf = g[vdir ? POM_G_WVSURF : POM_G_WUSURF];
+    s.wbot = in[vdir ? POM_WVBOT : POM_WUBOT];
+    s.cbc = g[POM_G_CBC];
+    s.ub = in[POM_UB];
+    s.vb = in[POM_VB];
+    s.mask = g[vdir ? POM_G_DVM : POM_G_DUM];
+    s.out = vdir ? vf1 : uf1;
+    s.wbot_out = out[vdir ? POM_WVBOT : POM_WUBOT];
+    s.tps_out = vdir ? out[POM_TPS] : tpsu;
+    return s;
+  }
+  VertvlArgs<T> a_vertvl() const {
+    VertvlArgs<T> s{};
+    s.u = u1;
+    s.v = v1;
+    s.w = in[POM_W];
+    s.dt = in[POM_DT];
+    s.etf = xf().etf;
+    s.etb = in[POM_ETB];
+    s.vfluxb = in[POM_VFLUXB];
+    s.vfluxf = out[POM_VFLUXF];
+    s.dx = g[POM_G_DX];
+    s.dy = g[POM_G_DY];
+    s.fsm = g[POM_G_FSM];
+    s.wout = out[POM_W];
+    return s;
+  }
+
+  // ------------------------------------------------------------ forward
+  void forward() {
+    const int64_t N2 = n2(), N3 = n3();
+    const dim3 g2(c.nblk2), g3(c.nblk3), tb(TPB);
+    hipLaunchKernelGGL(k_fill<T>, dim3(1), dim3(64), 0, c.stream, vamax, (int64_t)1, T(0));
+    {   // surface forcing reset (time_loop.py:118-164)
+      ForcingArgs<T> a{in[POM_E_ATMOS], in[POM_VFLUXF], in[POM_SWRAD], in[POM_WTSURF], in[POM_WSSURF],
+                       in[POM_T], in[POM_S], out[POM_E_ATMOS], out[POM_VFLUXF], out[POM_SWRAD],
+                       out[POM_WTSURF], out[POM_WSSURF]};
+      hipLaunchKernelGGL(k_forcing<T>, g2, tb, 0, c.stream, c.B, c.prm, a);
+    }
+    // 3-D preparation (time_loop.py:169-270)
+    launch_map<T, OpAdvct>(c, t_advct(), OutTab<T>{{out[POM_ADVX], out[POM_ADVY]}});
+    {
+      BaropgArgs<T> a{in[POM_RHO], g[POM_G_RMEAN], in[POM_DT], g[POM_G_DX], g[POM_G_DY], g[POM_G_DUM],
+                      g[POM_G_DVM], rho1, out[POM_DRHOX], out[POM_DRHOY]};
+      hipLaunchKernelGGL(k_baropg<T>, g2, tb, 0, c.stream, c.B, c.vert, c.ramp, c.prm, a);
+    }
+    launch_map<T, OpSmag>(c, t_smag(), OutTab<T>{{out[POM_AAM]}});
+    {
+      ColsumArgs<T> a{};
+      a.f[0] = out[POM_ADVX]; a.out[0] = adx2d_raw;
+      a.f[1] = out[POM_ADVY]; a.out[1] = ady2d_raw;
+      a.f[2] = out[POM_DRHOX]; a.out[2] = out[POM_DRX2D];
+      a.f[3] = out[POM_DRHOY]; a.out[3] = out[POM_DRY2D];
+      a.f[4] = out[POM_AAM]; a.out[4] = out[POM_AAM2D];
+      a.n = 5;
+      hipLaunchKernelGGL(k_colsum<T>, g2, tb, 0, c.stream, c.B, c.vert, a);
+    }
+    x0_ = x0();
+    x0_.fluxua = (T*)in[POM_FLUXUA];
+    x0_.fluxva = (T*)in[POM_FLUXVA];
+    x0_.tps = (T*)in[POM_TPS];
+    {
+      X<T> xin = x0_;
+      launch_map<T, OpAdvave>(c, t_advave(xin), OutTab<T>{{advua0, advva0, fluxua0, fluxva0, tps0}});
+    }
+    x0_ = x0();
+    launch_map<T, OpExtPrep>(c, t_prep(), OutTab<T>{{out[POM_ADX2D], out[POM_ADY2D], egf0, utf0, vtf0}});
+    // barotropic substeps (time_loop.py:292-431)
+    for (int n = 1; n <= isplit; ++n) {
+      const X<T>& xp = n == 1 ? x0_ : xs[n - 1];
+      X<T>& xn = xs[n];
+      Params prm = c.prm;
+      prm.iext = n;
+      Ctx<T> cn = c;
+      cn.prm = prm;
+      launch_map<T, OpExtElf>(cn, t_elf(xp), OutTab<T>{{xn.elf}});
+      if (n % c.prm.ispadv == 0) {
+        launch_map<T, OpAdvave>(cn, t_advave(xp), OutTab<T>{{xn.advua, xn.advva, xn.fluxua, xn.fluxva, xn.tps}});
+      }
+      resolve(n, xn);
+      launch_map<T, OpExtUV>(cn, t_uv(xp, xn), OutTab<T>{{xn.uaf, xn.vaf}});
+      hipLaunchKernelGGL(k_vamax<T>, dim3(blocks(N2)), dim3(256), 0, c.stream, xn.vaf, N2, vamax);
+      launch_map<T, OpExtFilter>(cn, t_filter(xp, xn),
+                                 OutTab<T>{{xn.ua, xn.uab, xn.va, xn.vab, xn.el, xn.elb, xn.d, xn.egf, xn.utf, xn.vtf,
+                                            xn.etf}});
+    }
+    const X<T>& xl = xs[isplit];
+    copy(out[POM_ADVUA], xl.advua, N2);
+    copy(out[POM_ADVVA], xl.advva, N2);
+    copy(out[POM_FLUXUA], xl.fluxua, N2);
+    copy(out[POM_FLUXVA], xl.fluxva, N2);
+    if (internal) {
+      {
+        AdjustArgs<T> a{in[POM_U], in[POM_V], in[POM_UTB], xl.utf, in[POM_VTB], xl.vtf, in[POM_DT], u1, v1};
+        hipLaunchKernelGGL(k_adjust<T>, g2, tb, 0, c.stream, c.B, c.vert, a);
+      }
+      hipLaunchKernelGGL(k_vertvl<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a_vertvl());
+      launch_map<T, OpAdvq>(c, t_advq(false), OutTab<T>{{q2f0}});
+      launch_map<T, OpAdvq>(c, t_advq(true), OutTab<T>{{q2lf0}});
+      hipLaunchKernelGGL(k_profq<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a_profq());
+      launch_map<T, OpKEdge>(c, t_kedge(), OutTab<T>{{out[POM_KM], out[POM_KH]}});
+      launch_map<T, OpBc6Filter>(c, t_bc6(), OutTab<T>{{out[POM_Q2], out[POM_Q2B], out[POM_Q2L], out[POM_Q2LB]}});
+      launch_map<T, OpAdvt1>(c, t_advt1(false), OutTab<T>{{tf0}});
+      launch_map<T, OpAdvt1>(c, t_advt1(true), OutTab<T>{{sf0}});
+      hipLaunchKernelGGL(k_proft<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a_proft(false));
+      hipLaunchKernelGGL(k_proft<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a_proft(true));
+      launch_map<T, OpBc4Filter>(c, t_bc4(), OutTab<T>{{out[POM_T], out[POM_TB], out[POM_S], out[POM_SB]}});
+      launch_map<T, OpDens>(c, t_dens(), OutTab<T>{{out[POM_RHO]}});
+      launch_map<T, OpAdvu>(c, t_advu(), OutTab<T>{{uf0}});
+      launch_map<T, OpAdvv>(c, t_advv(), OutTab<T>{{vf0}});
+      hipLaunchKernelGGL((k_profuv<T, 0>), g2, tb, 0, c.stream, c.B, c.vert, c.prm, a_profuv(false));
+      hipLaunchKernelGGL((k_profuv<T, 1>), g2, tb, 0, c.stream, c.B, c.vert, c.prm, a_profuv(true));
+      launch_map<T, OpBc3>(c, t_bc3(), OutTab<T>{{uf2, vf2}});
+      {
+        UVFiltArgs<T> a{uf2, in[POM_UB], u1, out[POM_UB], out[POM_U]};
+        hipLaunchKernelGGL(k_uvfilt<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a);
+        UVFiltArgs<T> bv{vf2, in[POM_VB], v1, out[POM_VB], out[POM_V]};
+        hipLaunchKernelGGL(k_uvfilt<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, bv);
+      }
+    } else {
+      for (int f : {POM_U, POM_UB, POM_V, POM_VB, POM_T, POM_TB, POM_S, POM_SB, POM_Q2, POM_Q2B, POM_Q2L, POM_Q2LB,
+                    POM_KM, POM_KH, POM_KQ, POM_L})
+        copy(out[f], in[f], N3);
+      copy(out[POM_RHO], rho1, N3);
+      hipLaunchKernelGGL(k_wsurf<T>, g3, tb, 0, c.stream, c.B, in[POM_W], out[POM_VFLUXF], out[POM_W]);
+      copy(out[POM_WUBOT], in[POM_WUBOT], N2);
+      copy(out[POM_WVBOT], in[POM_WVBOT], N2);
+      copy(out[POM_TPS], xl.tps, N2);
+    }
+    // shifts (time_loop.py:680-696)
+    copy(out[POM_EGB], xl.egf, N2);
+    copy(out[POM_ETB], in[POM_ET], N2);
+    copy(out[POM_ET], xl.etf, N2);
+    hipLaunchKernelGGL(k_hplus<T>, dim3(blocks(N2)), dim3(256), 0, c.stream, out[POM_DT], g[POM_G_H], xl.etf, N2);
+    copy(out[POM_UTB], xl.utf, N2);
+    copy(out[POM_VTB], xl.vtf, N2);
+    copy(out[POM_VFLUXB], out[POM_VFLUXF], N2);
+  }
+};
+
+// ---------------------------------------------------------------- adjoint
+template <class T> struct Adj {
+  Step<T>& S;
+  Ctx<T>& c;
+  T* bo[POM_NSTATE];    // cotangents of step outputs (accumulated)
+  T* bi[POM_NSTATE];    // cotangents of step inputs (accumulated)
+  // intermediates
+  T *b_rho1, *b_adx2d_raw, *b_ady2d_raw, *b_advua0, *b_advva0, *b_egf0, *b_utf0, *b_vtf0;
+  X<T> bx[2];           // cotangents of substep states (current / previous)
+  T *b_u1, *b_v1, *b_q2f0, *b_q2lf0, *b_q2f1, *b_q2lf1, *b_kmraw, *b_khraw, *b_tf0, *b_sf0, *b_tf1, *b_sf1;
+  T *b_uf0, *b_vf0, *b_uf1, *b_vf1, *b_uf2, *b_vf2;
+  // scratch
+  T *s3a, *s3b, *s3c, *s3d, *s2a, *s2b, *s2c, *s2d, *s2e;
+
+  Adj(Step<T>& s, Ctx<T>& cc) : S(s), c(cc) {}
+
+  void plan(Arena& A, T* const* grad_in) {
+    const int64_t N3 = S.n3(), N2 = S.n2();
+    for (int f = 0; f < POM_NSTATE; ++f) {
+      const int64_t n = f < POM_N3D ? N3 : N2;
+      bo[f] = A.get<T>(n);
+      bi[f] = grad_in && grad_in[f] ? grad_in[f] : A.get<T>(n);
+    }
+    b_rho1 = A.get<T>(N3);
+    b_adx2d_raw = A.get<T>(N2);
+    b_ady2d_raw = A.get<T>(N2);
+    b_advua0 = A.get<T>(N2);
+    b_advva0 = A.get<T>(N2);
+    b_egf0 = A.get<T>(N2);
+    b_utf0 = A.get<T>(N2);
+    b_vtf0 = A.get<T>(N2);
+    for (int q = 0; q < 2; ++q) {
+      X<T>& x = bx[q];
+      x.ua = A.get<T>(N2); x.uab = A.get<T>(N2); x.va = A.get<T>(N2); x.vab = A.get<T>(N2);
+      x.el = A.get<T>(N2); x.elb = A.get<T>(N2); x.d = A.get<T>(N2); x.egf = A.get<T>(N2);
+      x.utf = A.get<T>(N2); x.vtf = A.get<T>(N2); x.etf = A.get<T>(N2); x.advua = A.get<T>(N2);
+      x.advva = A.get<T>(N2); x.uaf = A.get<T>(N2); x.vaf = A.get<T>(N2); x.elf = A.get<T>(N2);
+      x.fluxua = x.fluxva = x.tps = nullptr;
+    }
+    b_u1 = A.get<T>(N3); b_v1 = A.get<T>(N3); b_q2f0 = A.get<T>(N3); b_q2lf0 = A.get<T>(N3);
+    b_q2f1 = A.get<T>(N3); b_q2lf1 = A.get<T>(N3); b_kmraw = A.get<T>(N3); b_khraw = A.get<T>(N3);
+    b_tf0 = A.get<T>(N3); b_sf0 = A.get<T>(N3); b_tf1 = A.get<T>(N3); b_sf1 = A.get<T>(N3);
+    b_uf0 = A.get<T>(N3); b_vf0 = A.get<T>(N3); b_uf1 = A.get<T>(N3); b_vf1 = A.get<T>(N3);
+    b_uf2 = A.get<T>(N3); b_vf2 = A.get<T>(N3);
+    s3a = A.get<T>(N3); s3b = A.get<T>(N3); s3c = A.get<T>(N3); s3d = A.get<T>(N3);
+    s2a = A.get<T>(N2); s2b = A.get<T>(N2); s2c = A.get<T>(N2); s2d = A.get<T>(N2); s2e = A.get<T>(N2);
+  }
+
+  void zero(T* p, int64_t n) {
+    hipLaunchKernelGGL(k_fill<T>, dim3(blocks(n)), dim3(256), 0, c.stream, p, n, T(0));
+  }
+  void add(T* d, const T* s, int64_t n) {
+    hipLaunchKernelGGL(k_add<T>, dim3(blocks(n)), dim3(256), 0, c.stream, d, s, n);
+  }
+  void zero_x(X<T>& x, int64_t n) {
+    for (T* p : {x.ua, x.uab, x.va, x.vab, x.el, x.elb, x.d, x.egf, x.utf, x.vtf, x.etf, x.advua, x.advva, x.uaf,
+                 x.vaf, x.elf})
+      zero(p, n);
+  }
+
+  void run(const T* const* cot_out, T* const* grad_in) {
+    const int64_t N3 = S.n3(), N2 = S.n2();
+    const dim3 g2(c.nblk2), tb(TPB);
+    // output cotangents (copied: they are accumulated into below)
+    for (int f = 0; f < POM_NSTATE; ++f) {
+      const int64_t n = f < POM_N3D ? N3 : N2;
+      if (cot_out && cot_out[f]) S.copy(bo[f], cot_out[f], n);
+      else zero(bo[f], n);
+      if (!(grad_in && grad_in[f])) zero(bi[f], n);
+    }
+    for (T* p : {b_rho1, b_u1, b_v1, b_q2f0, b_q2lf0, b_q2f1, b_q2lf1, b_kmraw, b_khraw, b_tf0, b_sf0, b_tf1, b_sf1,
+                 b_uf0, b_vf0, b_uf1, b_vf1, b_uf2, b_vf2})
+      zero(p, N3);
+    for (T* p : {b_adx2d_raw, b_ady2d_raw, b_advua0, b_advva0, b_egf0, b_utf0, b_vtf0}) zero(p, N2);
+    const X<T>& xl = S.xs[S.isplit];
+    // ---- shifts reversed: egb<-egf, etb<-et, et<-etf, dt<-h+et, utb<-utf, vtb<-vtf
+    X<T>& cur = bx[S.isplit & 1];
+    zero_x(cur, N2);
+    add(cur.ua, bo[POM_UA], N2); add(cur.uab, bo[POM_UAB], N2); add(cur.va, bo[POM_VA], N2);
+    add(cur.vab, bo[POM_VAB], N2); add(cur.el, bo[POM_EL], N2); add(cur.elb, bo[POM_ELB], N2);
+    add(cur.d, bo[POM_D], N2); add(cur.egf, bo[POM_EGF], N2); add(cur.egf, bo[POM_EGB], N2);
+    add(cur.utf, bo[POM_UTF], N2); add(cur.utf, bo[POM_UTB], N2); add(cur.vtf, bo[POM_VTF], N2);
+    add(cur.vtf, bo[POM_VTB], N2); add(cur.etf, bo[POM_ETF], N2); add(cur.etf, bo[POM_ET], N2);
+    add(cur.etf, bo[POM_DT], N2); add(cur.uaf, bo[POM_UAF], N2); add(cur.vaf, bo[POM_VAF], N2);
+    add(cur.elf, bo[POM_ELF], N2); add(cur.advua, bo[POM_ADVUA], N2); add(cur.advva, bo[POM_ADVVA], N2);
+    add(bi[POM_ET], bo[POM_ETB], N2);
+    // cotangents of the final substep fields used by the internal mode
+    T* b_etf30 = cur.etf;
+    T* b_egf30 = cur.egf;
+    T* b_utf30 = cur.utf;
+    T* b_vtf30 = cur.vtf;
+    if (S.internal) internal_rev(b_etf30, b_egf30, b_utf30, b_vtf30);
+    else {
+      for (int f : {POM_U, POM_UB, POM_V, POM_VB, POM_T, POM_TB, POM_S, POM_SB, POM_Q2, POM_Q2B, POM_Q2L, POM_Q2LB,
+                    POM_KM, POM_KH, POM_KQ})
+        add(bi[f], bo[f], N3);
+      add(b_rho1, bo[POM_RHO], N3);
+      add(bi[POM_WUBOT], bo[POM_WUBOT], N2);
+      add(bi[POM_WVBOT], bo[POM_WVBOT], N2);
+    }
+    // ---- substeps reversed
+    for (int n = S.isplit; n >= 1; --n) {
+      X<T>& bn = bx[n & 1];
+      X<T>& bp = bx[(n - 1) & 1];
+      zero_x(bp, N2);
+      const X<T>& xp = n == 1 ? S.x0_ : S.xs[n - 1];
+      const X<T>& xn = S.xs[n];
+      Ctx<T> cn = c;
+      cn.prm.iext = n;
+      // filter: outputs (ua uab va vab el elb d egf utf vtf etf)
+      {
+        COutTab<T> ob{{bn.ua, bn.uab, bn.va, bn.vab, bn.el, bn.elb, bn.d, bn.egf, bn.utf, bn.vtf, bn.etf}};
+        T* ib[OpExtFilter::NIN] = {bp.ua, bp.uab, bp.va, bp.vab, bp.el, bp.elb, bn.uaf, bn.vaf, bn.elf,
+                                   bp.egf, bp.utf, bp.vtf, bp.etf};
+        launch_gather<T, OpExtFilter>(cn, S.t_filter(xp, xn), ob, ib);
+      }
+      // depth-mean momentum
+      {
+        COutTab<T> ob{{bn.uaf, bn.vaf}};
+        T* ib[OpExtUV::NIN] = {bp.el, bp.elb, bn.elf, bp.d, bp.ua, bp.va, bp.uab, bp.vab, bn.advua, bn.advva,
+                               bo[POM_ADX2D], bo[POM_ADY2D], bo[POM_DRX2D], bo[POM_DRY2D], bi[POM_WUBOT],
+                               bi[POM_WVBOT], bp.uaf, bp.vaf};
+        launch_gather<T, OpExtUV>(cn, S.t_uv(xp, xn), ob, ib);
+      }
+      if (n % c.prm.ispadv == 0) {
+        COutTab<T> ob{{bn.advua, bn.advva, nullptr, nullptr, nullptr}};
+        T* ib[OpAdvave::NIN] = {bp.ua, bp.va, bp.uab, bp.vab, bp.d, bo[POM_AAM2D]};
+        launch_gather<T, OpAdvave>(cn, S.t_advave(xp), ob, ib);
+      } else {
+        add(bp.advua, bn.advua, N2);
+        add(bp.advva, bn.advva, N2);
+      }
+      {
+        COutTab<T> ob{{bn.elf}};
+        T* ib[OpExtElf::NIN] = {bp.ua, bp.va, bp.d, bp.elb};
+        launch_gather<T, OpExtElf>(cn, S.t_elf(xp), ob, ib);
+      }
+    }
+    X<T>& b0 = bx[0];
+    // X_0 -> inputs and pre-loop intermediates
+    add(bi[POM_UA], b0.ua, N2); add(bi[POM_UAB], b0.uab, N2); add(bi[POM_VA], b0.va, N2);
+    add(bi[POM_VAB], b0.vab, N2); add(bi[POM_EL], b0.el, N2); add(bi[POM_ELB], b0.elb, N2);
+    add(bi[POM_D], b0.d, N2); add(bi[POM_ETF], b0.etf, N2); add(bi[POM_UAF], b0.uaf, N2);
+    add(bi[POM_VAF], b0.vaf, N2); add(bi[POM_ELF], b0.elf, N2);
+    add(b_egf0, b0.egf, N2); add(b_utf0, b0.utf, N2); add(b_vtf0, b0.vtf, N2);
+    add(b_advua0, b0.advua, N2); add(b_advva0, b0.advva, N2);
+    // pre-loop: adx2d = raw - advua0 etc.
+    {
+      COutTab<T> ob{{bo[POM_ADX2D], bo[POM_ADY2D], b_egf0, b_utf0, b_vtf0}};
+      T* ib[OpExtPrep::NIN] = {b_adx2d_raw, b_ady2d_raw, b_advua0, b_advva0, bi[POM_EL], bi[POM_UA], bi[POM_VA],
+                               bi[POM_D], bi[POM_UTF], bi[POM_VTF]};
+      launch_gather<T, OpExtPrep>(c, S.t_prep(), ob, ib);
+    }
+    {
+      X<T> xin = S.x0_;
+      xin.fluxua = (T*)S.in[POM_FLUXUA];
+      xin.fluxva = (T*)S.in[POM_FLUXVA];
+      xin.tps = (T*)S.in[POM_TPS];
+      COutTab<T> ob{{b_advua0, b_advva0, nullptr, nullptr, nullptr}};
+      T* ib[OpAdvave::NIN] = {bi[POM_UA], bi[POM_VA], bi[POM_UAB], bi[POM_VAB], bi[POM_D], bo[POM_AAM2D]};
+      launch_gather<T, OpAdvave>(c, S.t_advave(xin), ob, ib);
+    }
+    {
+      ColsumAdjArgs<T> a{};
+      a.bout[0] = b_adx2d_raw; a.bf[0] = bo[POM_ADVX];
+      a.bout[1] = b_ady2d_raw; a.bf[1] = bo[POM_ADVY];
+      a.bout[2] = bo[POM_DRX2D]; a.bf[2] = bo[POM_DRHOX];
+      a.bout[3] = bo[POM_DRY2D]; a.bf[3] = bo[POM_DRHOY];
+      a.bout[4] = bo[POM_AAM2D]; a.bf[4] = bo[POM_AAM];
+      a.n = 5;
+      hipLaunchKernelGGL(k_colsum_adj<T>, g2, tb, 0, c.stream, c.B, c.vert, a);
+    }
+    {
+      COutTab<T> ob{{bo[POM_AAM]}};
+      T* ib[OpSmag::NIN] = {bi[POM_U], bi[POM_V], bi[POM_AAM]};
+      launch_gather<T, OpSmag>(c, S.t_smag(), ob, ib);
+    }
+    {
+      BaropgAdjArgs<T> a{S.in[POM_RHO], S.g[POM_G_RMEAN], S.in[POM_DT], S.g[POM_G_DX], S.g[POM_G_DY],
+                         S.g[POM_G_DUM], S.g[POM_G_DVM], b_rho1, bo[POM_DRHOX], bo[POM_DRHOY], bi[POM_RHO],
+                         bi[POM_DT]};
+      hipLaunchKernelGGL(k_baropg_adj<T>, g2, tb, 0, c.stream, c.B, c.vert, c.ramp, c.prm, a);
+    }
+    {
+      COutTab<T> ob{{bo[POM_ADVX], bo[POM_ADVY]}};
+      T* ib[OpAdvct::NIN] = {bi[POM_U], bi[POM_V], bi[POM_UB], bi[POM_VB], bi[POM_AAM], bi[POM_DT]};
+      launch_gather<T, OpAdvct>(c, S.t_advct(), ob, ib);
+    }
+  }
+
+  void internal_rev(T* b_etf30, T* b_egf30, T* b_utf30, T* b_vtf30) {
+    const dim3 g2(c.nblk2), tb(TPB);
+    Step<T>& s = S;
+    // u/v Asselin filter and shift
+    {
+      UVFiltAdjArgs<T> a{bo[POM_UB], bo[POM_U], b_uf2, bi[POM_UB], b_u1};
+      hipLaunchKernelGGL(k_uvfilt_adj<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a);
+      UVFiltAdjArgs<T> bv{bo[POM_VB], bo[POM_V], b_vf2, bi[POM_VB], b_v1};
+      hipLaunchKernelGGL(k_uvfilt_adj<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, bv);
+    }
+    {   // bcond 3
+      COutTab<T> ob{{b_uf2, b_vf2}};
+      T* ib[OpBc3::NIN] = {b_uf1, b_vf1, b_u1, b_v1};
+      launch_gather<T, OpBc3>(c, s.t_bc3(), ob, ib);
+    }
+    for (int dir = 0; dir < 2; ++dir) {   // profu / profv
+      ProfuvAdjArgs<T> a{};
+      a.fw = s.a_profuv(dir == 1);
+      a.b_out = dir ? b_vf1 : b_uf1;
+      a.b_wbot = bo[dir ? POM_WVBOT : POM_WUBOT];
+      a.b_f = dir ? b_vf0 : b_uf0;
+      a.s_km0 = s3a; a.s_km1 = s3b; a.s_etf0 = s2a; a.s_etf1 = s2b; a.s_mag = s2c;
+      a.b_wbot_in = bi[dir ? POM_WVBOT : POM_WUBOT];
+      ProfuvAdj2Args<T> b2{s.in[POM_UB], s.in[POM_VB], s.g[POM_G_CBC], s3a, s3b, s2a, s2b, s2c,
+                           bo[POM_KM], b_etf30, bi[POM_UB], bi[POM_VB]};
+      if (dir == 0) {
+        hipLaunchKernelGGL((k_profuv_adj1<T, 0>), g2, tb, 0, c.stream, c.B, c.vert, c.prm, a);
+        hipLaunchKernelGGL((k_profuv_adj2<T, 0>), g2, tb, 0, c.stream, c.B, c.vert, b2);
+      } else {
+        hipLaunchKernelGGL((k_profuv_adj1<T, 1>), g2, tb, 0, c.stream, c.B, c.vert, c.prm, a);
+        hipLaunchKernelGGL((k_profuv_adj2<T, 1>), g2, tb, 0, c.stream, c.B, c.vert, b2);
+      }
+    }
+    {   // advu / advv
+      COutTab<T> ob{{b_uf0}};
+      T* ib[OpAdvu::NIN] = {bo[POM_ADVX], bo[POM_DRHOX], b_u1, b_v1, bo[POM_W], bi[POM_UB], bi[POM_DT],
+                            b_egf30, bi[POM_EGB], b_etf30, bi[POM_ETB]};
+      launch_gather<T, OpAdvu>(c, s.t_advu(), ob, ib);
+      COutTab<T> ov{{b_vf0}};
+      T* iv[OpAdvv::NIN] = {bo[POM_ADVY], bo[POM_DRHOY], b_u1, b_v1, bo[POM_W], bi[POM_VB], bi[POM_DT],
+                            b_egf30, bi[POM_EGB], b_etf30, bi[POM_ETB]};
+      launch_gather<T, OpAdvv>(c, s.t_advv(), ov, iv);
+    }
+    {   // dens
+      COutTab<T> ob{{bo[POM_RHO]}};
+      T* ib[OpDens::NIN] = {bo[POM_T], bo[POM_S], b_rho1};
+      launch_gather<T, OpDens>(c, s.t_dens(), ob, ib);
+    }
+    {   // bcond 4 + T/S filter
+      COutTab<T> ob{{bo[POM_T], bo[POM_TB], bo[POM_S], bo[POM_SB]}};
+      T* ib[OpBc4Filter::NIN] = {b_tf1, b_sf1, bi[POM_T], bi[POM_S], bi[POM_TB], bi[POM_SB], b_u1, b_v1,
+                                 bo[POM_W], bi[POM_DT]};
+      launch_gather<T, OpBc4Filter>(c, s.t_bc4(), ob, ib);
+    }
+    for (int sal = 0; sal < 2; ++sal) {   // proft
+      ProftAdjArgs<T> a{};
+      ProftArgs<T> f = s.a_proft(sal == 1);
+      a.f = f.f; a.kh = f.kh; a.etf = f.etf; a.h = f.h; a.wfsurf = f.wfsurf; a.fsurf = f.fsurf;
+      a.b_out = sal ? b_sf1 : b_tf1;
+      a.b_f = sal ? b_sf0 : b_tf0;
+      a.b_kh = bo[POM_KH];
+      a.b_etf = b_etf30;
+      a.nbc = f.nbc;
+      hipLaunchKernelGGL(k_proft_adj<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a);
+    }
+    for (int sal = 0; sal < 2; ++sal) {   // advt1
+      COutTab<T> ob{{sal ? b_sf0 : b_tf0}};
+      T* ib[OpAdvt1::NIN] = {bi[sal ? POM_SB : POM_TB], bi[sal ? POM_S : POM_T], nullptr, b_u1, b_v1, bo[POM_W],
+                             bo[POM_AAM], bi[POM_DT], bi[POM_ETB], b_etf30};
+      launch_gather<T, OpAdvt1>(c, s.t_advt1(sal == 1), ob, ib);
+    }
+    {   // bcond 6 + q2 filter
+      COutTab<T> ob{{bo[POM_Q2], bo[POM_Q2B], bo[POM_Q2L], bo[POM_Q2LB]}};
+      T* ib[OpBc6Filter::NIN] = {b_q2f1, b_q2lf1, bi[POM_Q2], bi[POM_Q2L], bi[POM_Q2B], bi[POM_Q2LB], b_u1, b_v1};
+      launch_gather<T, OpBc6Filter>(c, s.t_bc6(), ob, ib);
+    }
+    {   // km/kh edge copy
+      COutTab<T> ob{{bo[POM_KM], bo[POM_KH]}};
+      T* ib[OpKEdge::NIN] = {b_kmraw, b_khraw};
+      launch_gather<T, OpKEdge>(c, s.t_kedge(), ob, ib);
+    }
+    {   // profq
+      ProfqAdjArgs<T> a{};
+      a.fw = s.a_profq();
+      a.b_q2o = b_q2f1; a.b_q2lo = b_q2lf1; a.b_lo = bo[POM_L]; a.b_kqo = bo[POM_KQ];
+      a.b_kmo = b_kmraw; a.b_kho = b_khraw;
+      a.b_q2f = b_q2f0; a.b_q2lf = b_q2lf0; a.b_q2 = bi[POM_Q2]; a.b_q2b = bi[POM_Q2B]; a.b_q2lb = bi[POM_Q2LB];
+      a.b_km = bi[POM_KM]; a.b_kh = bi[POM_KH]; a.b_kq = bi[POM_KQ]; a.b_u = b_u1; a.b_v = b_v1;
+      a.b_t = bi[POM_T]; a.b_s = bi[POM_S]; a.b_rho = b_rho1; a.b_wubot = bi[POM_WUBOT];
+      a.b_wvbot = bi[POM_WVBOT]; a.b_etf = b_etf30; a.b_dt = bi[POM_DT];
+      a.s_unb = s3a; a.s_vnb = s3b; a.s_wubnb = s2a; a.s_wvbnb = s2b;
+      hipLaunchKernelGGL(k_profq_adj<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a);
+      hipLaunchKernelGGL(k_profq_adj2<T>, g2, tb, 0, c.stream, c.B, a);
+    }
+    for (int l2 = 0; l2 < 2; ++l2) {   // advq
+      COutTab<T> ob{{l2 ? b_q2lf0 : b_q2f0}};
+      T* ib[OpAdvq::NIN] = {bi[l2 ? POM_Q2LB : POM_Q2B], bi[l2 ? POM_Q2L : POM_Q2], b_u1, b_v1, bo[POM_W],
+                            bo[POM_AAM], bi[POM_DT], bi[POM_ETB], b_etf30};
+      launch_gather<T, OpAdvq>(c, s.t_advq(l2 == 1), ob, ib);
+    }
+    {   // vertvl (w input is not differentiated)
+      VertvlAdjArgs<T> a{s.u1, s.v1, s.in[POM_DT], s.g[POM_G_DX], s.g[POM_G_DY], s.g[POM_G_FSM], bo[POM_W], s3a,
+                         b_u1, b_v1, bi[POM_DT], b_etf30, bi[POM_ETB]};
+      hipLaunchKernelGGL(k_vertvl_adj1<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a);
+      hipLaunchKernelGGL(k_vertvl_adj2<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a);
+    }
+    {   // transport adjustment
+      AdjustAdjArgs<T> a{s.in[POM_UTB], s.xs[s.isplit].utf, s.in[POM_VTB], s.xs[s.isplit].vtf, s.in[POM_DT],
+                         b_u1, b_v1, bi[POM_U], bi[POM_V], bi[POM_UTB], b_utf30, bi[POM_VTB], b_vtf30,
+                         bi[POM_DT]};
+      hipLaunchKernelGGL(k_adjust_adj<T>, g2, tb, 0, c.stream, c.B, c.vert, a);
+    }
+  }
+};
+
+template <class T> Ctx<T> make_ctx(const pom_batch* b, const pom_params_t* p, const void* ramp, const void* vg,
+                                    hipStream_t st) {
+  Ctx<T> c{};
+  c.B = Batch{b->d_mem, b->d_blk3, b->d_blk2, b->M};
+  c.vert = Vert<T>{static_cast<const T*>(vg)};
+  c.ramp = static_cast<const T*>(ramp);
+  Params& q = c.prm;
+  q.dti = p->dti; q.dte = p->dte; q.dti2 = p->dti2; q.dte2 = p->dte2; q.ispi = p->ispi; q.isp2i = p->isp2i;
+  q.alpha = p->alpha; q.smoth = p->smoth; q.horcon = p->horcon; q.grav = p->grav; q.rhoref = p->rhoref;
+  q.kappa = p->kappa; q.umol = p->umol; q.tprni = p->tprni; q.small = p->small; q.tbias = p->tbias;
+  q.sbias = p->sbias; q.hmax = p->hmax; q.rfe = p->rfe; q.rfw = p->rfw; q.rfn = p->rfn; q.rfs = p->rfs;
+  q.isplit = p->isplit; q.ispadv = p->ispadv; q.ntp = p->ntp; q.nbct = p->nbct; q.nbcs = p->nbcs; q.iext = 0;
+  c.nblk3 = b->nblk3;
+  c.nblk2 = b->nblk2;
+  c.stream = st;
+  return c;
+}
+
+int check_params(const pom_params_t* p) {
+  if (!p) return POM_E_ARG;
+  if (p->isplit < 3 || p->isplit > MAXSPLIT || p->ispadv < 1) return POM_E_ARG;
+  if (!((p->nbct == 1 || p->nbct == 3) && (p->nbcs == 1 || p->nbcs == 3))) return POM_E_UNSUPPORTED;
+  return POM_OK;
+}
+
+template <class T>
+size_t ws_size(const pom_batch* b, const pom_params_t* p, bool adjoint) {
+  Step<T> S{};
+  S.b = b;
+  S.isplit = p->isplit;
+  S.c.prm.ispadv = p->ispadv;
+  Arena A{nullptr, 0, 0, true};
+  T* dummy[POM_NSTATE] = {};
+  for (int f = 0; f < POM_NSTATE; ++f) S.out[f] = dummy[f];
+  S.plan(A, adjoint);
+  if (adjoint) {
+    Adj<T> J(S, S.c);
+    J.plan(A, nullptr);
+  }
+  return A.off + 256;
+}
+
+template <class T>
+int run_step(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vg, const void* const* in,
+             void* const* out, const void* const* geom, int internal, void* vamax, void* ws, size_t wsb,
+             hipStream_t st) {
+  Step<T> S{};
+  S.b = b;
+  S.c = make_ctx<T>(b, prm, ramp, vg, st);
+  S.isplit = prm->isplit;
+  S.internal = internal != 0;
+  S.vamax = static_cast<T*>(vamax);
+  for (int f = 0; f < POM_NSTATE; ++f) {
+    S.in[f] = static_cast<const T*>(in[f]);
+    S.out[f] = static_cast<T*>(out[f]);
+    if (!S.in[f] || !S.out[f]) return POM_E_ARG;
+  }
+  for (int f = 0; f < POM_NGEOM; ++f) {
+    S.g[f] = static_cast<const T*>(geom[f]);
+    if (!S.g[f]) return POM_E_ARG;
+  }
+  Arena A{static_cast<char*>(ws), 0, wsb, false};
+  S.plan(A, false);
+  if (A.off > wsb) return POM_E_ARG;
+  S.forward();
+  return hipGetLastError() == hipSuccess ? POM_OK : POM_E_HIP;
+}
+
+template <class T>
+int run_adj(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vg, const void* const* in,
+            const void* const* geom, int internal, const void* const* cot, void* const* grad, void* ws, size_t wsb,
+            hipStream_t st) {
+  Step<T> S{};
+  S.b = b;
+  S.c = make_ctx<T>(b, prm, ramp, vg, st);
+  S.isplit = prm->isplit;
+  S.internal = internal != 0;
+  for (int f = 0; f < POM_NSTATE; ++f) {
+    S.in[f] = static_cast<const T*>(in[f]);
+    if (!S.in[f]) return POM_E_ARG;
+  }
+  for (int f = 0; f < POM_NGEOM; ++f) {
+    S.g[f] = static_cast<const T*>(geom[f]);
+    if (!S.g[f]) return POM_E_ARG;
+  }
+  Arena A{static_cast<char*>(ws), 0, wsb, false};
+  S.plan(A, true);
+  S.vamax = A.get<T>(1);
+  Adj<T> J(S, S.c);
+  J.plan(A, reinterpret_cast<T* const*>(grad));
+  if (A.off > wsb) return POM_E_ARG;
+  S.forward();
+  J.run(reinterpret_cast<const T* const*>(cot), reinterpret_cast<T* const*>(grad));
+  return hipGetLastError() == hipSuccess ? POM_OK : POM_E_HIP;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pom_batch_create(const pom_member_t* members, int32_t n, int32_t dtype, pom_batch** out) {
+  if (!members || n < 1 || !out || (dtype != POM_F32 && dtype != POM_F64)) return POM_E_ARG;
+  std::vector<Member> hm(n);
+  std::vector<int32_t> b3(n + 1), b2(n + 1);
+  b3[0] = b2[0] = 0;
+  int64_t n2 = 0, n3 = 0;
+  for (int m = 0; m < n; ++m) {
+    const pom_member_t& s = members[m];
+    if (s.im < 4 || s.jm < 4) return POM_E_ARG;
+    hm[m] = Member{s.im, s.jm, s.off2, s.off3};
+    const int64_t c2 = (int64_t)s.im * s.jm, c3 = c2 * KB;
+    b3[m + 1] = b3[m] + (int32_t)((c3 + TPB - 1) / TPB);
+    b2[m + 1] = b2[m] + (int32_t)((c2 + TPB - 1) / TPB);
+    n2 = std::max(n2, s.off2 + c2);
+    n3 = std::max(n3, s.off3 + c3);
+  }
+  pom_batch* b = new pom_batch{};
+  b->dtype = dtype;
+  b->M = n;
+  b->nblk3 = b3[n];
+  b->nblk2 = b2[n];
+  b->n2 = n2;
+  b->n3 = n3;
+  if (hipMalloc(&b->d_mem, sizeof(Member) * n) != hipSuccess ||
+      hipMalloc(&b->d_blk3, sizeof(int32_t) * (n + 1)) != hipSuccess ||
+      hipMalloc(&b->d_blk2, sizeof(int32_t) * (n + 1)) != hipSuccess) {
+    delete b;
+    return POM_E_HIP;
+  }
+  if (hipMemcpy(b->d_mem, hm.data(), sizeof(Member) * n, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_blk3, b3.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_blk2, b2.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice) != hipSuccess)
+    return POM_E_HIP;
+  *out = b;
+  return POM_OK;
+}
+
+int pom_batch_destroy(pom_batch* b) {
+  if (!b) return POM_E_ARG;
+  (void)hipFree(b->d_mem);
+  (void)hipFree(b->d_blk3);
+  (void)hipFree(b->d_blk2);
+  delete b;
+  return POM_OK;
+}
+
+int pom_explicit_workspace(const pom_batch* b, const pom_params_t* prm, int32_t adjoint, size_t* bytes) {
+  if (!b || !bytes) return POM_E_ARG;
+  if (int e = check_params(prm)) return e;
+  *bytes = b->dtype == POM_F32 ? ws_size<float>(b, prm, adjoint != 0) : ws_size<double>(b, prm, adjoint != 0);
+  return POM_OK;
+}
+
+int pom_explicit_step(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vgrid,
+                      const void* const* state_in, void* const* state_out, const void* const* geom, int32_t internal,
+                      void* vamax_out, void* workspace, size_t ws_bytes, void* stream) {
+  if (!b || !ramp || !vgrid || !state_in || !state_out || !geom || !vamax_out || !workspace) return POM_E_ARG;
+  if (int e = check_params(prm)) return e;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (b->dtype == POM_F32)
+    return run_step<float>(b, prm, ramp, vgrid, state_in, state_out, geom, internal, vamax_out, workspace, ws_bytes, st);
+  return run_step<double>(b, prm, ramp, vgrid, state_in, state_out, geom, internal, vamax_out, workspace, ws_bytes, st);
+}
+
+int pom_explicit_step_adj(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vgrid,
+                          const void* const* state_in, const void* const* geom, int32_t internal,
+                          const void* const* cot_out, void* const* grad_in, void* workspace, size_t ws_bytes,
+                          void* stream) {
+  if (!b || !ramp || !vgrid || !state_in || !geom || !workspace) return POM_E_ARG;
+  if (int e = check_params(prm)) return e;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (b->dtype == POM_F32)
+    return run_adj<float>(b, prm, ramp, vgrid, state_in, geom, internal, cot_out, grad_in, workspace, ws_bytes, st);
+  return run_adj<double>(b, prm, ramp, vgrid, state_in, geom, internal, cot_out, grad_in, workspace, ws_bytes, st);
+}
+
+}  // extern "C"

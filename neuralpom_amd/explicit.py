@@ -1,0 +1,231 @@
+"""Batched explicit POM step on the HIP extension, with its adjoint.
+
+``Ensemble`` packs members of (possibly different) grid sizes back to back in
+one buffer per field and drives ``pom_explicit_step`` / ``pom_explicit_step_adj``
+(include/neuralpom_amd.h).  ``ExplicitStep`` is the ``torch.autograd.Function``
+around one internal step: forward runs the HIP step; backward recomputes the
+step from the saved inputs (per-step checkpointing) and runs the hand-written
+adjoint kernels.  It keeps no reference to a live core object.
+
+Reference: src/neuralpom/cores/explicit/time_loop.py:20-709 (the step),
+src/hybrid/sparse_physics.py:108-149 (checkpointed differentiable advance).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+import torch
+
+from . import _lib as L
+
+KB = 16
+
+
+@dataclass
+class StepScalars:
+    """Scalars of the explicit core (explicit/base.py:46-118, v1/core_spec.py)."""
+    dte: float = 20.0
+    isplit: int = 30
+    alpha: float = 0.225
+    smoth: float = 0.10
+    horcon: float = 0.05
+    grav: float = 9.806
+    rhoref: float = 1035.0
+    kappa: float = 0.40
+    umol: float = 2.e-5
+    tprni: float = 0.20
+    small: float = 1.e-9
+    tbias: float = 0.0
+    sbias: float = 0.0
+    hmax: float = 2000.0
+    rfe: float = 0.0
+    rfw: float = 0.0
+    rfn: float = 0.0
+    rfs: float = 0.0
+    ispadv: int = 5
+    ntp: int = 2
+    nbct: int = 1
+    nbcs: int = 1
+    vmaxl: float = 100.0
+
+    @property
+    def dti(self):
+        return self.dte * self.isplit
+
+    def struct(self) -> L.Params:
+        p = L.Params()
+        p.dti = self.dti
+        p.dte = self.dte
+        p.dti2 = self.dti * 2.0
+        p.dte2 = self.dte * 2.0
+        p.ispi = 1.0 / self.isplit
+        p.isp2i = 1.0 / (2.0 * self.isplit)
+        for n in ("alpha", "smoth", "horcon", "grav", "rhoref", "kappa", "umol", "tprni", "small",
+                  "tbias", "sbias", "hmax", "rfe", "rfw", "rfn", "rfs", "vmaxl"):
+            setattr(p, n, float(getattr(self, n)))
+        for n in ("isplit", "ispadv", "ntp", "nbct", "nbcs"):
+            setattr(p, n, int(getattr(self, n)))
+        p.iext = 0
+        return p
+
+
+class Ensemble:
+    """Members of (im, jm) grids, all with 16 sigma levels, packed per field."""
+
+    def __init__(self, shapes, dtype=torch.float32, device="cuda"):
+        self.lib = L.load()
+        if not torch.cuda.is_available():
+            raise L.HipExtensionError("no HIP device: the explicit step has no CPU path")
+        self.shapes = [(int(a), int(b)) for a, b in shapes]
+        self.dtype = dtype
+        self.device = torch.device(device)
+        self.off2, self.off3 = [], []
+        o2 = o3 = 0
+        for im, jm in self.shapes:
+            self.off2.append(o2)
+            self.off3.append(o3)
+            o2 += im * jm
+            o3 += im * jm * KB
+        self.n2, self.n3 = o2, o3
+        mem = (L.Member * len(self.shapes))()
+        for m, (im, jm) in enumerate(self.shapes):
+            mem[m].im, mem[m].jm, mem[m].off2, mem[m].off3 = im, jm, self.off2[m], self.off3[m]
+        code = L.POM_F64 if dtype == torch.float64 else L.POM_F32
+        h = C.c_void_p()
+        with torch.cuda.device(self.device):
+            L.check(self.lib.pom_batch_create(mem, len(self.shapes), code, C.byref(h)), "pom_batch_create")
+        self._h = h
+        self._ws = {}
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                self.lib.pom_batch_destroy(h)
+            except Exception:
+                pass
+
+    @property
+    def cells(self):
+        return self.n3
+
+    # ------------------------------------------------------------ buffers
+    def numel(self, name):
+        return self.n3 if (name in L.FIELDS_3D or name in L.GEOM_3D) else self.n2
+
+    def zeros(self, name):
+        return torch.zeros(self.numel(name), dtype=self.dtype, device=self.device)
+
+    def view(self, flat, m, name):
+        im, jm = self.shapes[m]
+        if name in L.FIELDS_3D or name in L.GEOM_3D:
+            return flat[self.off3[m]:self.off3[m] + im * jm * KB].view(im, jm, KB)
+        return flat[self.off2[m]:self.off2[m] + im * jm].view(im, jm)
+
+    def pack(self, per_member, name):
+        """Concatenate member tensors of one field into the packed layout."""
+        return torch.cat([t.reshape(-1).to(self.device, self.dtype) for t in per_member]).contiguous()
+
+    def _workspace(self, prm: L.Params, adjoint: bool):
+        n = C.c_size_t()
+        L.check(self.lib.pom_explicit_workspace(self._h, C.byref(prm), int(adjoint), C.byref(n)),
+                "pom_explicit_workspace")
+        key = (adjoint, n.value)
+        ws = self._ws.get(adjoint)
+        if ws is None or ws.numel() < n.value:
+            ws = torch.empty(n.value, dtype=torch.uint8, device=self.device)
+            self._ws[adjoint] = ws
+        return ws, n.value
+
+    def _check(self, tensors, names, nel):
+        for t, n in zip(tensors, names):
+            if t.dtype != self.dtype or t.device != self.device or not t.is_contiguous() or t.numel() != nel(n):
+                raise ValueError(f"field {n}: expected contiguous {self.dtype} on {self.device} "
+                                 f"with {nel(n)} elements, got {t.dtype} {t.device} {tuple(t.shape)}")
+
+    # ------------------------------------------------------------ step
+    def step(self, state, geom, sc: StepScalars, ramp, vgrid, internal=True):
+        """One internal step; returns (new state dict, vamax tensor)."""
+        prm = sc.struct()
+        ins = [state[n] for n in L.FIELDS]
+        gs = [geom[n] for n in L.GEOM]
+        self._check(ins, L.FIELDS, self.numel)
+        self._check(gs, L.GEOM, self.numel)
+        out = {n: torch.empty_like(state[n]) for n in L.FIELDS}
+        vamax = torch.zeros(1, dtype=self.dtype, device=self.device)
+        ws, nb = self._workspace(prm, False)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        rc = self.lib.pom_explicit_step(
+            self._h, C.byref(prm), ramp.data_ptr(), vgrid.data_ptr(),
+            L.ptr_array([t.data_ptr() for t in ins]), L.ptr_array([out[n].data_ptr() for n in L.FIELDS]),
+            L.ptr_array([t.data_ptr() for t in gs]), int(bool(internal)), vamax.data_ptr(),
+            ws.data_ptr(), nb, stream)
+        L.check(rc, "pom_explicit_step")
+        return out, vamax
+
+    def step_adj(self, state, geom, sc: StepScalars, ramp, vgrid, internal, cot):
+        """Gradients of the step inputs for output cotangents ``cot`` (dict, may omit fields)."""
+        prm = sc.struct()
+        ins = [state[n] for n in L.FIELDS]
+        gs = [geom[n] for n in L.GEOM]
+        self._check(ins, L.FIELDS, self.numel)
+        grads = {n: torch.zeros_like(state[n]) for n in L.DIFF_INPUTS}
+        cot_ptrs = []
+        for n in L.FIELDS:
+            c = cot.get(n)
+            if c is not None:
+                c = c.contiguous().to(self.dtype)
+                cot[n] = c
+                cot_ptrs.append(c.data_ptr())
+            else:
+                cot_ptrs.append(None)
+        grad_ptrs = [grads[n].data_ptr() if n in grads else None for n in L.FIELDS]
+        ws, nb = self._workspace(prm, True)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        rc = self.lib.pom_explicit_step_adj(
+            self._h, C.byref(prm), ramp.data_ptr(), vgrid.data_ptr(),
+            L.ptr_array([t.data_ptr() for t in ins]), L.ptr_array([t.data_ptr() for t in gs]),
+            int(bool(internal)), L.ptr_array(cot_ptrs), L.ptr_array(grad_ptrs), ws.data_ptr(), nb, stream)
+        L.check(rc, "pom_explicit_step_adj")
+        return grads
+
+
+class ExplicitStep(torch.autograd.Function):
+    """One internal step as an autograd node (inputs and outputs in FIELDS order).
+
+    Backward recomputes the forward from the saved inputs (per-step
+    checkpointing) and accumulates the input cotangents with the adjoint
+    kernels; non-differentiable inputs get None.
+    """
+
+    @staticmethod
+    def forward(ctx, ens, geom, sc, ramp, vgrid, internal, *inputs):
+        state = dict(zip(L.FIELDS, inputs))
+        out, vamax = ens.step(state, geom, sc, ramp, vgrid, internal)
+        ctx.ens, ctx.geom, ctx.sc, ctx.internal = ens, geom, sc, internal
+        ctx.save_for_backward(ramp, vgrid, *inputs)
+        ctx.mark_non_differentiable(vamax)
+        return tuple(out[n] for n in L.FIELDS) + (vamax,)
+
+    @staticmethod
+    def backward(ctx, *gouts):
+        ramp, vgrid, *inputs = ctx.saved_tensors
+        state = dict(zip(L.FIELDS, inputs))
+        cot = {n: g for n, g in zip(L.FIELDS, gouts[:len(L.FIELDS)]) if g is not None}
+        grads = ctx.ens.step_adj(state, ctx.geom, ctx.sc, ramp, vgrid, ctx.internal, cot)
+        return (None,) * 6 + tuple(grads.get(n) for n in L.FIELDS)
+
+
+def step_autograd(ens, state, geom, sc, ramp, vgrid, internal=True):
+    """Differentiable step: returns (new state dict, vamax)."""
+    outs = ExplicitStep.apply(ens, geom, sc, ramp, vgrid, internal, *[state[n] for n in L.FIELDS])
+    return dict(zip(L.FIELDS, outs[:-1])), outs[-1]
+
+
+def ramp_value(time_days: float, period_days: float, lramp: bool = True) -> float:
+    """time_loop.py:79-88"""
+    if not lramp:
+        return 1.0
+    return min(time_days / float(period_days), 1.0)

@@ -1,0 +1,119 @@
+"""Parity of the HIP explicit step (forward and adjoint) with the reference.
+
+Golden vectors: tests/golden/explicit_{steps,vjp}_12x10.npz written from the
+reference's float64 explicit core (tests/golden/make_golden.py).  Batched
+ragged members are checked against the oracle (oracle/pomref), itself pinned
+bitwise to the same golden vectors by tests/test_oracle_golden.py.
+"""
+import numpy as np
+import pytest
+import torch
+
+from neuralpom_amd import _lib as L
+from oracle.pomref import explicit as X
+from oracle.pomref.state import load_state
+from tests._gpu_inputs import ensemble_inputs, ramp_of, scalars_of, unpack
+from tests._helpers import golden, rel_err
+
+pytestmark = pytest.mark.gpu
+
+SKIP_CMP = set()   # every output field is compared
+
+# max relative error (max |a-b| / max |b|) per precision
+TOL = {torch.float64: 1e-10, torch.float32: 2e-4}
+
+
+def _state(label="a"):
+    g = golden("explicit_steps_12x10.npz")
+    S = load_state(g, "in")
+    S.time0 = float(g[f"{label}/time0"])
+    S.time = S.time0
+    return S, g
+
+
+def _run(states, dtype, nstep, internal_first):
+    ens, state, geom, vg = ensemble_inputs(states, dtype)
+    sc = scalars_of(states[0])
+    S0 = states[0]
+    for it in range(1, nstep + 1):
+        ramp = torch.tensor([ramp_of(S, it) for S in states], dtype=dtype, device="cuda")
+        internal = internal_first or it > 1
+        state, vamax = ens.step(state, geom, sc, ramp, vg, internal)
+    torch.cuda.synchronize()
+    return ens, state, vamax
+
+
+def _compare(ens, state, refs, tol):
+    bad = []
+    for n in L.FIELDS:
+        if n in SKIP_CMP:
+            continue
+        for m, got in enumerate(unpack(ens, state[n], n)):
+            ref = refs[m](n)
+            e = rel_err(got.double().cpu(), ref)
+            if not (e <= tol):
+                bad.append(f"{n}[m{m}]: {e:.2e}")
+    assert not bad, "fields off: " + ", ".join(bad)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("label", ["a", "b", "c"])
+def test_step_matches_reference(dtype, label):
+    S, g = _state(label)
+    nstep = int(g[f"{label}/nstep"])
+    print(f"[gpu] explicit step 12x10 label={label} nstep={nstep} dtype={dtype}", flush=True)
+    ens, state, vamax = _run([S], dtype, nstep, internal_first=(S.time0 != 0.0))
+    ref = lambda n: torch.from_numpy(g[f"{label}/{n}"])
+    _compare(ens, state, [ref], TOL[dtype] * (10 if label == "b" else 1))
+    assert float(vamax) <= S.vmaxl
+
+
+def _perturbed(seed, im=None, jm=None):
+    S, g = _state("a")
+    gen = torch.Generator().manual_seed(seed)
+    for n in ("u", "v", "t", "el", "ua", "va"):
+        x = getattr(S, n)
+        setattr(S, n, x + 0.01 * torch.randn(x.shape, generator=gen, dtype=x.dtype) * x.abs().max())
+    S.d = S.h + S.el
+    return S
+
+
+def test_batched_members_match_oracle():
+    """Three members in one launch, each checked against its own oracle run."""
+    states = [_perturbed(s) for s in (1, 2, 3)]
+    print("[gpu] batched 3x(12x10) float64", flush=True)
+    ens, state, _ = _run([s.copy() for s in states], torch.float64, 1, True)
+    refs = []
+    for S in states:
+        R = S.copy()
+        R.iend = 1
+        assert X.run_time_loop(R)
+        refs.append(lambda n, R=R: getattr(R, n))
+    _compare(ens, state, refs, 1e-10)
+
+
+VJP_IN = ("u", "ub", "v", "vb", "t", "tb", "s", "sb", "el", "elb", "ua", "uab",
+          "va", "vab", "q2", "q2b", "q2l", "q2lb", "km", "kh", "kq", "aam",
+          "et", "etb", "d", "dt", "rho", "egb", "utb", "vtb", "wubot", "wvbot")
+VJP_OUT = ("u", "ub", "v", "vb", "t", "tb", "s", "sb", "el", "elb", "ua", "uab",
+           "va", "vab", "q2", "q2b", "q2l", "q2lb", "km", "kh", "kq", "aam",
+           "et", "etb", "etf", "rho", "egb", "utb", "vtb", "wubot", "wvbot", "w")
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-8), (torch.float32, 2e-3)])
+def test_adjoint_matches_reference_vjp(dtype, tol):
+    S, g = _state("a")
+    v = golden("explicit_vjp_12x10.npz")
+    print(f"[gpu] explicit adjoint 12x10 dtype={dtype}", flush=True)
+    ens, state, geom, vg = ensemble_inputs([S], dtype)
+    sc = scalars_of(S)
+    ramp = torch.tensor([ramp_of(S, 1)], dtype=dtype, device="cuda")
+    cot = {n: torch.from_numpy(v[f"cot/{n}"]).reshape(-1).to("cuda", dtype) for n in VJP_OUT}
+    grads = ens.step_adj(state, geom, sc, ramp, vg, True, cot)
+    torch.cuda.synchronize()
+    bad = []
+    for n in VJP_IN:
+        e = rel_err(grads[n].double().cpu().reshape(v[f"grad/{n}"].shape), torch.from_numpy(v[f"grad/{n}"]))
+        if not (e <= tol):
+            bad.append(f"{n}: {e:.2e}")
+    assert not bad, "gradients off: " + ", ".join(bad)

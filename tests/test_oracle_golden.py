@@ -190,3 +190,27 @@ def test_vjp():
     for n, gr in zip(VJP_INPUTS, grads):
         gr = torch.zeros_like(leaves[n]) if gr is None else gr
         assert_rel(gr, torch.from_numpy(v[f"grad/{n}"]), 1e-8, f"grad {n}")
+
+
+def test_double_gyre_init():
+    """oracle/pomref/cases.py vs the reference's state after construction."""
+    from oracle.pomref.cases import double_gyre
+    g = golden(f"explicit_init_{TAG}.npz")
+    S = double_gyre(12, 10)
+    bad = []
+    for key in g.files:
+        if not key.startswith("t/"):
+            continue
+        n = key[2:]
+        if n in ("io", "jo", "ko", "curv"):
+            continue
+        ref = torch.from_numpy(np.array(g[key]))
+        got = getattr(S, n, None)
+        if got is None:
+            bad.append(f"{n}: missing")
+            continue
+        e = (got - ref).abs().max().item() / max(ref.abs().max().item(), 1e-300)
+        if e > 1e-12:
+            bad.append(f"{n}: {e:.1e}")
+    assert not bad, ", ".join(bad)
+    assert abs(S.period - float(g["scalar/period"])) < 1e-12
