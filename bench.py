@@ -1,0 +1,222 @@
+#!/usr/bin/env python3
+"""Forward+adjoint throughput of the MI355X explicit POM step.
+
+Workload (one "step" = one internal step dti of every member, forward and
+its vector-Jacobian product with per-step recompute): the batched stratified
+seamount ensemble of 64 members on ragged grids (32 x 90x82, 24 x 120x110,
+8 x 177x161, all with 16 sigma levels), float32, closed basins, started from
+rest plus seeded temperature noise and advanced from time0 = 1 day so that
+every step runs the full internal mode.  Inputs are resident in HBM before
+the timed region starts; state advances from step to step.
+
+Prints one JSON line (metric/value/unit/... + roofline + cpu_baseline +
+autograd_baseline).  Only one GPU is used; --gpus N > 1 exits with a message.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import torch
+
+B_FWD = 164.0    # algorithmic bytes per cell-step, forward (fp32)
+B_FA = 468.0     # algorithmic bytes per cell-step, forward + adjoint with recompute (fp32)
+HBM_PEAK = 8.0e12
+
+DTE = 6.0        # external step of the 8 km seamount grid (barotropic CFL); dti = 180 s
+SHAPES = [(90, 82)] * 32 + [(120, 110)] * 24 + [(177, 161)] * 8
+DIFF_OUT = ("u", "ub", "v", "vb", "t", "tb", "s", "sb", "el", "elb", "ua", "uab", "va", "vab",
+            "q2", "q2b", "q2l", "q2lb", "km", "kh", "kq", "aam", "et", "etb", "etf", "rho", "w")
+
+
+def build_ensemble(shapes, dtype, device, seed0=1000):
+    from neuralpom_amd import _lib as L
+    from neuralpom_amd import cases
+    from neuralpom_amd.explicit import Ensemble, StepScalars
+    ens = Ensemble(shapes, dtype=dtype, device=device)
+    members = [cases.seamount(im, jm, seed=seed0 + m, dtype=dtype, device=device)
+               for m, (im, jm) in enumerate(shapes)]
+    state = {n: ens.pack([mb[0][n] for mb in members], n) for n in L.FIELDS}
+    geom = {n: ens.pack([mb[1][n] for mb in members], n) for n in L.GEOM}
+    vg = torch.stack([mb[2] for mb in members]).to(device, dtype).contiguous()
+    periods = [mb[3] for mb in members]
+    sc = StepScalars(dte=DTE, isplit=30, alpha=0.225, smoth=0.10, horcon=0.05, hmax=4500.0)
+    return ens, state, geom, vg, periods, sc, members
+
+
+def ramps(periods, time_days, dtype, device):
+    return torch.tensor([min(time_days / p, 1.0) for p in periods], dtype=dtype, device=device)
+
+
+def run_gpu(args):
+    dtype = torch.float32
+    dev = torch.device("cuda", 0)
+    ens, state, geom, vg, periods, sc, _ = build_ensemble(SHAPES, dtype, dev)
+    cells = ens.n3
+    g = torch.Generator(device="cpu").manual_seed(7)
+    cot = {n: (torch.randn(ens.numel(n), generator=g) * 1e-3).to(dev, dtype) for n in DIFF_OUT}
+    time0 = 1.0
+    it = [0]
+
+    def one_step(st):
+        it[0] += 1
+        t = time0 + sc.dti * it[0] / 86400.0
+        r = ramps(periods, t, dtype, dev)
+        out, vamax = ens.step(st, geom, sc, r, vg, True)
+        grads = ens.step_adj(st, geom, sc, r, vg, True, dict(cot))
+        return out, grads, vamax
+
+    for _ in range(args.warmup):
+        state, _, vamax = one_step(state)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        state, grads, vamax = one_step(state)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    if not float(vamax) <= sc.vmaxl:
+        raise FloatingPointError(f"barotropic velocity bound exceeded: vamax={float(vamax)}")
+    for n in ("u", "t", "el"):
+        if not torch.isfinite(state[n]).all() or not torch.isfinite(grads[n]).all():
+            raise FloatingPointError(f"non-finite {n}")
+    # forward-only rate on the same workload
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    st = state
+    for _ in range(args.steps):
+        it[0] += 1
+        r = ramps(periods, time0 + sc.dti * it[0] / 86400.0, dtype, dev)
+        st, vamax = ens.step(st, geom, sc, r, vg, True)
+    torch.cuda.synchronize()
+    dtf = time.perf_counter() - t1
+    return dict(cells=cells, seconds=dt, seconds_fwd=dtf, ens=ens)
+
+
+def autograd_baseline(args, n_members=3):
+    """The reference algorithm (oracle restatement, eager torch.autograd) on the same GPU.
+
+    Bounded sample: one member of each grid shape, forward + backward of one
+    step each (no checkpointing: a conservative, cheaper baseline).
+    """
+    from oracle.pomref import explicit as X
+    return _eager_rate(X, torch.device("cuda", 0), torch.float32, args.autograd_steps)
+
+
+def cpu_baseline(args):
+    """The oracle restatement of the reference step on the host cores."""
+    from oracle.pomref import explicit as X
+    cores = os.cpu_count() or 1
+    cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
+    torch.set_num_threads(cores)
+    res = _eager_rate(X, torch.device("cpu"), torch.float32, args.cpu_steps, shapes=[(90, 82)])
+    res["cores"] = cores
+    return res
+
+
+def _oracle_state(X, st, g, vg, period, sc, dtype, device):
+    from oracle.pomref.cases import base_state
+    im, jm = st["u"].shape[:2]
+    S = base_state(im, jm, dtype)
+    for k, v in st.items():
+        setattr(S, k, v.detach().to(device, dtype).clone())
+    for k, v in g.items():
+        setattr(S, k, v.detach().to(device, dtype).clone())
+    for n in ("ele", "elw", "uabe", "uabw", "eln", "els", "vabn", "vabs", "sbe", "sbw", "tbe", "tbw",
+              "sbn", "sbs", "tbn", "tbs"):
+        setattr(S, n, getattr(S, n).to(device))
+    S.z, S.zz, S.dz, S.dzz = (vg[i].to(device, dtype) for i in range(4))
+    S.hmax = sc.hmax
+    S.dte, S.isplit = sc.dte, sc.isplit
+    S.dti, S.dte2 = sc.dti, 2.0 * sc.dte
+    S.dti2 = 2.0 * sc.dti
+    S.ispi, S.isp2i = 1.0 / sc.isplit, 0.5 / sc.isplit
+    S.period = period
+    S.time0 = S.time = 1.0
+    S.iend = 1
+    return S
+
+
+def _eager_rate(X, device, dtype, nstep, shapes=((90, 82), (120, 110), (177, 161))):
+    from neuralpom_amd import cases
+    from neuralpom_amd.explicit import StepScalars
+    sc = StepScalars(hmax=4500.0, dte=DTE)
+    total_cells = 0
+    total_s = 0.0
+    for m, (im, jm) in enumerate(shapes):
+        st, g, vg, period = cases.seamount(im, jm, seed=1000 + m, dtype=dtype, device="cpu")
+        S = _oracle_state(X, st, g, vg, period, sc, dtype, device)
+        leaves = {}
+        for n in ("u", "v", "t", "s", "ua", "va", "el"):
+            leaves[n] = getattr(S, n).clone().requires_grad_(True)
+            setattr(S, n, leaves[n])
+        # warm-up
+        W = S.copy()
+        X.step(W, 1)
+        sync = torch.cuda.synchronize if device.type == "cuda" else (lambda: None)
+        sync()
+        t0 = time.perf_counter()
+        for i in range(nstep):
+            R = S.copy()
+            X.step(R, 1)
+            loss = sum((getattr(R, n) ** 2).sum() for n in ("u", "v", "t", "el"))
+            torch.autograd.grad(loss, list(leaves.values()), allow_unused=True)
+        sync()
+        total_s += time.perf_counter() - t0
+        total_cells += im * jm * 16 * nstep
+    return dict(value=total_cells / total_s, unit="grid-cell-steps/s", seconds=total_s,
+                sample=f"fwd+bwd of {nstep} step(s) for one member each of {list(shapes)}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--autograd-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=16)
+    ap.add_argument("--no-baselines", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported in this build "
+              "(members would shard across GPUs; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    res = run_gpu(args)
+    rate = res["cells"] * args.steps / res["seconds"]
+    rate_fwd = res["cells"] * args.steps / res["seconds_fwd"]
+    achieved = rate * B_FA
+    line = {
+        "metric": "fwd+adjoint grid-cell-steps/s vs autograd baseline, same GPU, <=1e-5 rel err",
+        "value": rate,
+        "unit": "grid-cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * res["seconds"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "stratified seamount ensemble, 64 members (32x90x82, 24x120x110, "
+                               "8x177x161) x 16 levels, explicit core, fwd+adjoint per step",
+                   "cells_per_step": res["cells"], "dti_s": DTE * 30, "isplit": 30},
+        "forward_only": {"value": rate_fwd, "unit": "grid-cell-steps/s"},
+        "step_roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                          "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                          "bytes_per_cell_step": B_FA},
+    }
+    if not args.no_baselines:
+        ab = autograd_baseline(args)
+        line["autograd_baseline"] = ab
+        line["vs_autograd"] = rate / ab["value"]
+        cb = cpu_baseline(args)
+        line["cpu_baseline"] = {"value": cb["value"], "unit": cb["unit"], "cores": cb["cores"],
+                                "kind": "port", "sample": cb["sample"]}
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,208 @@
+"""Case construction for the explicit core (one-time set-up, torch on device).
+
+``double_gyre`` follows the reference's construction of the production basin:
+src/neuralpom/cores/explicit/base.py:8-189 (defaults),
+src/neuralpom/cores/explicit/cases.py:30-164 (double gyre),
+src/neuralpom/cores/explicit/grid.py:113-166 (areas, masks),
+src/neuralpom/cores/explicit/initialization.py:10-99 (initial state).
+
+``seamount`` is the stratified seamount ensemble member of the headline
+benchmark: closed basin, uniform dx = dy, h = H - 0.6 H exp(-r^2/L^2) with
+H = 4500 m and L = 25 km, temperature with constant N^2 = 1e-5 s^-2 (linear
+thermal expansion 2e-4 / K), S = 35, an f-plane at 40 N, and seeded Gaussian
+temperature noise; the mean density rmean is the initial density.  It reuses
+the double-gyre initialisation sequence.
+
+Set-up is not on the per-step path, so it is written with torch tensor ops;
+the equation of state used here mirrors the kernel's (momentum.py:467-541).
+"""
+from __future__ import annotations
+
+import math
+
+import torch
+
+KB = 16
+SCALARS = dict(small=1e-9, rhoref=1035.0, hmax=2000.0, grav=9.806, kappa=0.40, z0b=0.010,
+               cbcmin=0.00250, cbcmax=1.0, aam_init=500.0, tbias=0.0, sbias=0.0)
+
+STATE_3D = ("u", "ub", "v", "vb", "t", "tb", "s", "sb", "q2", "q2b", "q2l", "q2lb", "km", "kh", "kq",
+            "aam", "rho", "w", "l", "advx", "advy", "drhox", "drhoy")
+STATE_2D = ("el", "elb", "elf", "ua", "uab", "uaf", "va", "vab", "vaf", "d", "dt", "et", "etb", "etf",
+            "egf", "egb", "utf", "utb", "vtf", "vtb", "wubot", "wvbot", "advua", "advva", "adx2d", "ady2d",
+            "drx2d", "dry2d", "aam2d", "fluxua", "fluxva", "tps", "e_atmos", "vfluxf", "vfluxb", "swrad",
+            "wtsurf", "wssurf")
+GEOM_2D = ("h", "dx", "dy", "art", "aru", "arv", "fsm", "dum", "dvm", "cor", "cbc", "wusurf", "wvsurf",
+           "tsurf", "ssurf")
+GEOM_3D = ("rmean", "tclim", "sclim")
+
+
+def eos(t, s, h, zz, fsm, grav=9.806, rhoref=1035.0):
+    """(density - 1000)/rhoref on levels k < kb-1 (momentum.py:467-541)."""
+    kbm1 = KB - 1
+    tr = t[:, :, :kbm1]
+    sr = s[:, :, :kbm1]
+    tr2 = tr * tr
+    tr3 = tr2 * tr
+    tr4 = tr3 * tr
+    p = grav * rhoref * (-zz[:kbm1].view(1, 1, -1) * h.unsqueeze(2)) * 1.0e-5
+    r = -0.157406 + 6.793952e-2 * tr - 9.095290e-3 * tr2 + 1.001685e-4 * tr3 - 1.120083e-6 * tr4 + \
+        6.536332e-9 * tr4 * tr
+    r = r + (0.824493 - 4.0899e-3 * tr + 7.6438e-5 * tr2 - 8.2467e-7 * tr3 + 5.3875e-9 * tr4) * sr
+    r = r + (-5.72466e-3 + 1.0227e-4 * tr - 1.6546e-6 * tr2) * torch.pow(torch.clamp(sr.abs(), min=1e-8), 1.5)
+    r = r + 4.8314e-4 * sr * sr
+    cr = 1449.1 + 0.0821 * p + 4.55 * tr - 0.045 * tr2 + 1.34 * (sr - 35.0)
+    cr2 = cr * cr
+    r = r + 1.0e5 * p / cr2 * (1.0 - 2.0 * p / cr2)
+    out = torch.zeros_like(t)
+    out[:, :, :kbm1] = (r / rhoref) * fsm.unsqueeze(2)
+    return out
+
+
+def _uniform_sigma(dtype, device):
+    z = torch.tensor([-float(k) / float(KB - 1) for k in range(KB)], dtype=dtype, device=device)
+    dz = torch.zeros_like(z)
+    zz = torch.zeros_like(z)
+    dz[:KB - 1] = z[:KB - 1] - z[1:]
+    zz[:KB - 1] = 0.5 * (z[:KB - 1] + z[1:])
+    dzz = dz.clone()
+    zz[KB - 1] = 2.0 * zz[KB - 2] - zz[KB - 3]
+    return z, zz, dz, dzz
+
+
+def _areas_masks(g):
+    dx, dy, h = g["dx"], g["dy"], g["h"]
+    g["art"] = dx * dy
+    aru = torch.zeros_like(dx)
+    arv = torch.zeros_like(dx)
+    aru[1:, 1:] = 0.25 * (dx[1:, 1:] + dx[:-1, 1:]) * (dy[1:, 1:] + dy[:-1, 1:])
+    arv[1:, 1:] = 0.25 * (dx[1:, 1:] + dx[1:, :-1]) * (dy[1:, 1:] + dy[1:, :-1])
+    aru[0, :] = aru[1, :]
+    arv[0, :] = arv[1, :]
+    aru[:, 0] = aru[:, 1]
+    arv[:, 0] = arv[:, 1]
+    fsm = (h > 1.0).to(h.dtype)
+    dum = torch.zeros_like(dx)
+    dvm = torch.zeros_like(dx)
+    dum[1:, 1:] = fsm[1:, 1:] * fsm[:-1, 1:]
+    dvm[1:, 1:] = fsm[1:, 1:] * fsm[1:, :-1]
+    dum[0, :] = dum[1, :]
+    dvm[0, :] = dvm[1, :]
+    dum[:, 0] = dum[:, 1]
+    dvm[:, 0] = dvm[:, 1]
+    g.update(aru=aru, arv=arv, fsm=fsm, dum=dum, dvm=dvm)
+
+
+def _initialize(st, g, vg, im, jm, period_cor):
+    """initialization.py:16-99 on dict state ``st`` and geometry ``g``."""
+    z, zz, dz, dzz = vg
+    small = SCALARS["small"]
+    kbm1 = KB - 1
+    st["d"] = g["h"] + st["el"]
+    st["dt"] = g["h"] + st["et"]
+    st["w"][:, :, 0] = st["vfluxf"]
+    l = 0.1 * st["dt"].unsqueeze(2).expand(-1, -1, KB).clone()
+    st["l"] = l
+    st["q2b"] = torch.full_like(l, small)
+    st["q2lb"] = l * st["q2b"]
+    st["kh"] = l * torch.sqrt(st["q2b"] + 1e-8)
+    st["km"] = st["kh"].clone()
+    st["kq"] = st["kh"].clone()
+    st["aam"] = torch.full_like(l, SCALARS["aam_init"])
+    for dst, src in (("q2", "q2b"), ("q2l", "q2lb"), ("t", "tb"), ("s", "sb"), ("u", "ub"), ("v", "vb")):
+        st[dst][:, :, :kbm1] = st[src][:, :, :kbm1]
+    st["rho"] = eos(st["t"], st["s"], g["h"], zz, g["fsm"])
+    # baropg at ramp = 0 leaves drhox = drhoy = 0; rho passes through (rho - rmean) + rmean
+    st["rho"] = (st["rho"] - g["rmean"]) + g["rmean"]
+    term = torch.clamp((1.0 + zz[kbm1 - 1]) * g["h"] / SCALARS["z0b"], min=1e-12)
+    g["cbc"] = torch.clamp((SCALARS["kappa"] / torch.log(term)) ** 2, min=SCALARS["cbcmin"], max=SCALARS["cbcmax"])
+    dxs = torch.clamp(g["dx"], min=1e-6)
+    dys = torch.clamp(g["dy"], min=1e-6)
+    st["tps"] = (0.5 / torch.sqrt(1.0 / dxs ** 2 + 1.0 / dys ** 2)) / torch.sqrt(
+        SCALARS["grav"] * (g["h"] + small)) * g["fsm"]
+    period = (2.0 * math.pi) / (abs(period_cor) + 1e-20) / 86400.0
+    return period
+
+
+def _empty(im, jm, dtype, device):
+    st = {n: torch.zeros((im, jm, KB), dtype=dtype, device=device) for n in STATE_3D}
+    st.update({n: torch.zeros((im, jm), dtype=dtype, device=device) for n in STATE_2D})
+    g = {n: torch.zeros((im, jm), dtype=dtype, device=device) for n in GEOM_2D}
+    g.update({n: torch.zeros((im, jm, KB), dtype=dtype, device=device) for n in GEOM_3D})
+    return st, g
+
+
+def _interp_profile(zref, tref, zdst):
+    zf = torch.flip(zref, dims=[0])
+    vf = torch.flip(tref, dims=[0])
+    idx = torch.clamp(torch.bucketize(zdst, zf), min=1, max=zref.shape[0] - 1)
+    z0, z1 = zf[idx - 1], zf[idx]
+    v0, v1 = vf[idx - 1], vf[idx]
+    den = z1 - z0
+    den = torch.where(den.abs() < 1e-6, torch.ones_like(den), den)
+    w = (zdst - z0) / den
+    return v0 * (1.0 - w) + v1 * w
+
+
+def double_gyre(im, jm, dtype=torch.float32, device="cpu"):
+    """Reference double-gyre basin; returns (state, geom, vgrid, period_days)."""
+    st, g = _empty(im, jm, dtype, device)
+    deg2rad = math.pi / 180.0
+    re, omega = 6371.0e3, 7.2921e-5
+    dlat, dlon = 20.0 / float(jm - 2), 22.0 / float(im - 2)
+    j_idx = torch.arange(jm, dtype=dtype, device=device).view(1, -1).repeat(im, 1)
+    north = 30.0 + (j_idx - 1.0) * dlat
+    g["dy"] = torch.full((im, jm), re * dlat * deg2rad, dtype=dtype, device=device)
+    g["dx"] = re * torch.cos(north * deg2rad) * dlon * deg2rad
+    g["cor"] = 2.0 * omega * torch.sin(north * deg2rad)
+    h = torch.full((im, jm), 2500.0, dtype=dtype, device=device)
+    h[0, :] = h[-1, :] = 1.0
+    h[:, 0] = h[:, -1] = 1.0
+    g["h"] = h
+    vg = _uniform_sigma(dtype, device)
+    _areas_masks(g)
+    st["sb"].fill_(35.0)
+    zref = torch.linspace(0, -(float(h.max()) + 100.0), steps=1000, dtype=dtype, device=device)
+    st["tb"] = _interp_profile(zref, 2.0 + 28.0 * torch.exp(zref / 800.0), vg[0].view(1, 1, -1) * h.unsqueeze(2))
+    g["tclim"] = st["tb"].clone()
+    g["sclim"] = st["sb"].clone()
+    g["rmean"] = eos(st["t"], st["s"], h, vg[1], g["fsm"])   # t = s = 0 at this point (cases.py:132)
+    g["wusurf"] = -(0.1 / SCALARS["rhoref"]) * (1.0 - torch.cos(2.0 * math.pi * (north - 30.0) / 20.0)) * g["dum"]
+    period = _initialize(st, g, vg, im, jm, float(g["cor"][im // 2, jm // 2]))
+    return st, g, torch.stack(vg), period
+
+
+def seamount(im, jm, seed=0, dx=8000.0, H=4500.0, L=25.0e3, n2=1.0e-5, noise=0.1,
+             dtype=torch.float32, device="cpu"):
+    """Stratified seamount member; returns (state, geom, vgrid, period_days)."""
+    st, g = _empty(im, jm, dtype, device)
+    omega = 7.2921e-5
+    g["dx"] = torch.full((im, jm), dx, dtype=dtype, device=device)
+    g["dy"] = torch.full((im, jm), dx, dtype=dtype, device=device)
+    g["cor"] = torch.full((im, jm), 2.0 * omega * math.sin(math.radians(40.0)), dtype=dtype, device=device)
+    x = (torch.arange(im, dtype=dtype, device=device) - 0.5 * (im - 1)) * dx
+    y = (torch.arange(jm, dtype=dtype, device=device) - 0.5 * (jm - 1)) * dx
+    r2 = x.view(-1, 1) ** 2 + y.view(1, -1) ** 2
+    h = H - 0.6 * H * torch.exp(-r2 / L ** 2)
+    h[0, :] = h[-1, :] = 1.0
+    h[:, 0] = h[:, -1] = 1.0
+    g["h"] = h
+    vg = _uniform_sigma(dtype, device)
+    _areas_masks(g)
+    st["sb"].fill_(35.0)
+    # constant N^2 with a linear thermal expansion alpha_T = 2e-4 / K: dT/dz = N^2 / (g alpha_T)
+    gradt = n2 / (SCALARS["grav"] * 2.0e-4)
+    zdep = vg[0].view(1, 1, -1) * h.unsqueeze(2)
+    tb = 25.0 + gradt * zdep
+    gen = torch.Generator(device="cpu").manual_seed(int(seed))
+    tb = tb + noise * torch.randn(tb.shape, generator=gen, dtype=torch.float64).to(device, dtype)
+    st["tb"] = tb
+    g["tclim"] = st["tb"].clone()
+    g["sclim"] = st["sb"].clone()
+    # mean density = initial density (standard POM practice; keeps the sigma-
+    # coordinate pressure-gradient error over the seamount small)
+    g["rmean"] = eos(st["tb"], st["sb"], h, vg[1], g["fsm"])
+    g["hmax"] = H
+    period = _initialize(st, g, vg, im, jm, float(g["cor"][im // 2, jm // 2]))
+    g.pop("hmax")
+    return st, g, torch.stack(vg), period

@@ -56,6 +56,7 @@ __device__ __forceinline__ void baropg_face(const Pos& p, const Vert<T>& vg, con
   const T scale = T(0.25) * tdt * mask[ix2(p, i, j)] * (dd2[ix2(p, i, j)] + dd2[ix2(p, ni, nj)]);
   outc[0] = surf * scale * ramp;
   T acc = T(0);
+  #pragma unroll
   for (int k = 1; k < KBM1; ++k) {
     const T ra0 = r0[k] - m0[k], ra1 = r1[k] - m1[k];
     const T zk = vg.zz(p.m, k), zkm = vg.zz(p.m, k - 1);
@@ -75,8 +76,10 @@ __global__ void __launch_bounds__(TPB) k_baropg(Batch B, Vert<T> vg, const T* ra
   Pos p;
   if (!pos2(B, p)) return;
   const int64_t c = ix3(p, p.i, p.j, 0);
+  #pragma unroll
   for (int k = 0; k < KB; ++k) a.rho_out[c + k] = (a.rho[c + k] - a.rmean[c + k]) + a.rmean[c + k];
   T ox[KB], oy[KB];
+  #pragma unroll
   for (int k = 0; k < KB; ++k) ox[k] = oy[k] = T(0);
   if (interior_pos(p)) {
     const T rm = ramp[p.m];
@@ -113,6 +116,7 @@ __device__ __forceinline__ void baropg_face_rev(const Pos& p, const Vert<T>& vg,
   const T mk = mask[ix2(p, i, j)], dd = dd2[ix2(p, i, j)] + dd2[ix2(p, ni, nj)];
   const T scale = T(0.25) * tdt * mk * dd;
   T ra0[KB], ra1[KB];
+  #pragma unroll
   for (int k = 0; k < KBM1; ++k) {
     ra0[k] = r0[k] - m0[k];
     ra1[k] = r1[k] - m1[k];
@@ -122,6 +126,7 @@ __device__ __forceinline__ void baropg_face_rev(const Pos& p, const Vert<T>& vg,
   T col[KB];
   col[0] = surf;
   T acc = T(0);
+  #pragma unroll
   for (int k = 1; k < KBM1; ++k) {
     const T zk = vg.zz(p.m, k), zkm = vg.zz(p.m, k - 1);
     const T t1 = grav * T(0.25) * (zkm - zk) * tdt * ((ra0[k] - ra1[k]) + (ra0[k - 1] - ra1[k - 1]));
@@ -132,6 +137,7 @@ __device__ __forceinline__ void baropg_face_rev(const Pos& p, const Vert<T>& vg,
   // out(k) = col(k) * scale * ramp
   T b_scale = T(0), b_tdt = T(0), b_ddt = T(0);
   T bcol[KB];
+  #pragma unroll
   for (int k = 0; k < KBM1; ++k) {
     bcol[k] = ob[k] * scale * ramp;
     b_scale += ob[k] * col[k] * ramp;
@@ -140,7 +146,9 @@ __device__ __forceinline__ void baropg_face_rev(const Pos& p, const Vert<T>& vg,
   // cumulative sum reverse: inc(k) feeds col(k..kbm1-1); surf feeds all
   T b_surf = T(0);
   T suffix = T(0);
+  #pragma unroll
   for (int k = 0; k < KB; ++k) g0[k] = g1[k] = T(0);
+  #pragma unroll
   for (int k = KBM1 - 1; k >= 1; --k) {
     suffix += bcol[k];
     const T zk = vg.zz(p.m, k), zkm = vg.zz(p.m, k - 1);
@@ -159,6 +167,7 @@ __device__ __forceinline__ void baropg_face_rev(const Pos& p, const Vert<T>& vg,
     g0[k - 1] += bdiff - bsm;
     g1[k - 1] += -bdiff - bsm;
   }
+  #pragma unroll
   for (int k = 0; k < KBM1; ++k) b_surf += bcol[k];
   const T cz = c0 * (-vg.zz(p.m, 0));
   g0[0] += b_surf * cz * tdt;
@@ -175,6 +184,7 @@ __global__ void __launch_bounds__(TPB) k_baropg_adj(Batch B, Vert<T> vg, const T
   if (!pos2(B, q)) return;
   const int64_t c = ix3(q, q.i, q.j, 0);
   T br[KB];
+  #pragma unroll
   for (int k = 0; k < KB; ++k) br[k] = a.b_rho_out ? a.b_rho_out[c + k] : T(0);
   T bdt = T(0);
   T g0[KB], g1[KB], d0, d1;
@@ -183,11 +193,13 @@ __global__ void __launch_bounds__(TPB) k_baropg_adj(Batch B, Vert<T> vg, const T
   if (interior_pos(q)) {
     if (a.b_drhox) {
       baropg_face_rev<T, 0>(q, vg, prm, rm, a.rho, a.rmean, a.dt, a.dy, a.dum, a.b_drhox, g0, g1, d0, d1);
+      #pragma unroll
       for (int k = 0; k < KB; ++k) br[k] += g0[k];
       bdt += d0;
     }
     if (a.b_drhoy) {
       baropg_face_rev<T, 1>(q, vg, prm, rm, a.rho, a.rmean, a.dt, a.dx, a.dvm, a.b_drhoy, g0, g1, d0, d1);
+      #pragma unroll
       for (int k = 0; k < KB; ++k) br[k] += g0[k];
       bdt += d0;
     }
@@ -198,6 +210,7 @@ __global__ void __launch_bounds__(TPB) k_baropg_adj(Batch B, Vert<T> vg, const T
     o.i = q.i + 1;
     if (a.b_drhox && interior_pos(o)) {
       baropg_face_rev<T, 0>(o, vg, prm, rm, a.rho, a.rmean, a.dt, a.dy, a.dum, a.b_drhox, g0, g1, d0, d1);
+      #pragma unroll
       for (int k = 0; k < KB; ++k) br[k] += g1[k];
       bdt += d1;
     }
@@ -208,6 +221,7 @@ __global__ void __launch_bounds__(TPB) k_baropg_adj(Batch B, Vert<T> vg, const T
     o.j = q.j + 1;
     if (a.b_drhoy && interior_pos(o)) {
       baropg_face_rev<T, 1>(o, vg, prm, rm, a.rho, a.rmean, a.dt, a.dx, a.dvm, a.b_drhoy, g0, g1, d0, d1);
+      #pragma unroll
       for (int k = 0; k < KB; ++k) br[k] += g1[k];
       bdt += d1;
     }
@@ -231,6 +245,7 @@ __global__ void __launch_bounds__(TPB) k_colsum(Batch B, Vert<T> vg, ColsumArgs<
   const int64_t c = ix3(p, p.i, p.j, 0);
   for (int n = 0; n < a.n; ++n) {
     T s = T(0);
+    #pragma unroll
     for (int k = 0; k < KBM1; ++k) s += a.f[n][c + k] * vg.dz(p.m, k);
     a.out[n][ix2(p, p.i, p.j)] = s;
   }
@@ -249,6 +264,7 @@ __global__ void __launch_bounds__(TPB) k_colsum_adj(Batch B, Vert<T> vg, ColsumA
   for (int n = 0; n < a.n; ++n) {
     if (!a.bout[n] || !a.bf[n]) continue;
     const T g = a.bout[n][ix2(p, p.i, p.j)];
+    #pragma unroll
     for (int k = 0; k < KBM1; ++k) a.bf[n][c + k] += g * vg.dz(p.m, k);
   }
 }
@@ -271,16 +287,20 @@ __global__ void __launch_bounds__(TPB) k_adjust(Batch B, Vert<T> vg, AdjustArgs<
   ld_col(a.u + c, x);
   if (p.i >= 1) {
     T s = T(0);
+    #pragma unroll
     for (int k = 0; k < KBM1; ++k) s += x[k] * vg.dz(p.m, k);
     const T corr = (a.utb[c2] + a.utf[c2]) / (a.dt[c2] + a.dt[ix2(p, p.i - 1, p.j)] + T(1e-20));
+    #pragma unroll
     for (int k = 0; k < KBM1; ++k) x[k] = (x[k] - s) + corr;
   }
   st_col(a.u1 + c, x);
   ld_col(a.v + c, x);
   if (p.j >= 1) {
     T s = T(0);
+    #pragma unroll
     for (int k = 0; k < KBM1; ++k) s += x[k] * vg.dz(p.m, k);
     const T corr = (a.vtb[c2] + a.vtf[c2]) / (a.dt[c2] + a.dt[ix2(p, p.i, p.j - 1)] + T(1e-20));
+    #pragma unroll
     for (int k = 0; k < KBM1; ++k) x[k] = (x[k] - s) + corr;
   }
   st_col(a.v1 + c, x);
@@ -302,6 +322,7 @@ __global__ void __launch_bounds__(TPB) k_adjust_adj(Batch B, Vert<T> vg, AdjustA
   auto colS = [&](const T* b, int ii, int jj) {
     T s = T(0);
     const int64_t cc = ix3(p, ii, jj, 0);
+    #pragma unroll
     for (int k = 0; k < KBM1; ++k) s += b[cc + k];
     return s;
   };
@@ -311,12 +332,14 @@ __global__ void __launch_bounds__(TPB) k_adjust_adj(Batch B, Vert<T> vg, AdjustA
     ld_col(a.b_u1 + c, g);
     if (p.i >= 1) {
       T S = T(0);
+      #pragma unroll
       for (int k = 0; k < KBM1; ++k) S += g[k];
       const T den = a.dt[c2] + a.dt[ix2(p, p.i - 1, p.j)] + T(1e-20);
       const T corr = (a.utb[c2] + a.utf[c2]) / den;
       if (a.b_utb) a.b_utb[c2] += S / den;
       if (a.b_utf) a.b_utf[c2] += S / den;
       bdt += -S * corr / den;
+      #pragma unroll
       for (int k = 0; k < KBM1; ++k) g[k] = g[k] - S * vg.dz(p.m, k);
     }
     if (a.b_u) add_col(a.b_u + c, g);
@@ -332,12 +355,14 @@ __global__ void __launch_bounds__(TPB) k_adjust_adj(Batch B, Vert<T> vg, AdjustA
     ld_col(a.b_v1 + c, g);
     if (p.j >= 1) {
       T S = T(0);
+      #pragma unroll
       for (int k = 0; k < KBM1; ++k) S += g[k];
       const T den = a.dt[c2] + a.dt[ix2(p, p.i, p.j - 1)] + T(1e-20);
       const T corr = (a.vtb[c2] + a.vtf[c2]) / den;
       if (a.b_vtb) a.b_vtb[c2] += S / den;
       if (a.b_vtf) a.b_vtf[c2] += S / den;
       bdt += -S * corr / den;
+      #pragma unroll
       for (int k = 0; k < KBM1; ++k) g[k] = g[k] - S * vg.dz(p.m, k);
     }
     if (a.b_v) add_col(a.b_v + c, g);
@@ -379,6 +404,7 @@ __global__ void __launch_bounds__(TPB) k_vertvl(Batch B, Vert<T> vg, Params prm,
     const T area = a.dx[c2] * a.dy[c2];
     const T eta = (a.etf[c2] - a.etb[c2]) / T(prm.dti2);
     T acc = T(0);
+    #pragma unroll
     for (int k = 0; k < KBM1; ++k) {
       const T div = vv_xf(p, a, i + 1, j, k) - vv_xf(p, a, i, j, k) + vv_yf(p, a, i, j + 1, k) - vv_yf(p, a, i, j, k);
       const T inc = (div / area + eta) * vg.dz(p.m, k);
@@ -388,6 +414,7 @@ __global__ void __launch_bounds__(TPB) k_vertvl(Batch B, Vert<T> vg, Params prm,
   } else {
     ld_col(a.w + c, w);
   }
+  #pragma unroll
   for (int k = 0; k < KBM1; ++k) w[k] = w[k] * fsm;
   st_col(a.wout + c, w);
 }
@@ -405,12 +432,14 @@ __global__ void __launch_bounds__(TPB) k_vertvl_adj1(Batch B, Vert<T> vg, Params
   if (!pos2(B, p)) return;
   const int64_t c = ix3(p, p.i, p.j, 0), c2 = ix2(p, p.i, p.j);
   T bd[KB];
+  #pragma unroll
   for (int k = 0; k < KB; ++k) bd[k] = T(0);
   if (interior_pos(p)) {
     const T fsm = a.fsm[c2];
     const T area = a.dx[c2] * a.dy[c2];
     T suffix = T(0), beta = T(0);
     // w(k+1) = w0 + sum_{k'<=k} inc(k'); wout(k) = w(k)*fsm for k<kbm1
+    #pragma unroll
     for (int k = KBM1 - 1; k >= 0; --k) {
       const int kw = k + 1;
       suffix += a.b_w[c + kw] * (kw < KBM1 ? fsm : T(1));
@@ -444,7 +473,9 @@ __global__ void __launch_bounds__(TPB) k_vertvl_adj2(Batch B, Vert<T> vg, Params
     const T dty = a.dt[c2] + a.dt[ix2(p, i, j - 1)];
     T gu[KB], gv[KB];
     T bx = T(0), by = T(0);
+    #pragma unroll
     for (int k = 0; k < KB; ++k) gu[k] = gv[k] = T(0);
+    #pragma unroll
     for (int k = 0; k < KBM1; ++k) {
       const T fx = bdv(i - 1, j, k) - bdv(i, j, k);
       gu[k] = cx * dts * fx;
@@ -462,12 +493,14 @@ __global__ void __launch_bounds__(TPB) k_vertvl_adj2(Batch B, Vert<T> vg, Params
     const int64_t n2 = ix2(p, i + 1, j);
     const T cx = T(0.25) * (a.dy[n2] + a.dy[c2]);
     const int64_t nc = ix3(p, i + 1, j, 0);
+    #pragma unroll
     for (int k = 0; k < KBM1; ++k) bdt += cx * a.u[nc + k] * (bdv(i, j, k) - bdv(i + 1, j, k));
   }
   if (j + 1 < p.jm && i >= 1) {
     const int64_t n2 = ix2(p, i, j + 1);
     const T cy = T(0.25) * (a.dx[n2] + a.dx[c2]);
     const int64_t nc = ix3(p, i, j + 1, 0);
+    #pragma unroll
     for (int k = 0; k < KBM1; ++k) bdt += cy * a.v[nc + k] * (bdv(i, j, k) - bdv(i, j + 1, k));
   }
   if (a.b_dt) a.b_dt[c2] += bdt;
@@ -485,7 +518,9 @@ template <class T>
 __device__ __forceinline__ void proft_coef(const Pos& p, const Vert<T>& vg, const Params& prm, const T* kh, T dh,
                                            T* a, T* c) {
   const T dh2 = dh * dh;
+  #pragma unroll
   for (int k = 0; k < KB; ++k) a[k] = c[k] = T(0);
+  #pragma unroll
   for (int k = 1; k < KBM1; ++k) {
     const T num = T(-prm.dti2) * (kh[k] + T(prm.umol));
     a[k - 1] = num / (vg.dz(p.m, k - 1) * vg.dzz(p.m, k - 1) * dh2);
@@ -510,6 +545,7 @@ __global__ void __launch_bounds__(TPB) k_proft(Batch B, Vert<T> vg, Params prm, 
     ee[0] = T(0);
     gg[0] = s.fsurf[c2];
   }
+  #pragma unroll
   for (int k = 1; k < KBM2; ++k) {
     const T den = a[k] + cc[k] * (T(1) - ee[k - 1]) - T(1);
     const T inv = T(1) / (den + T(1e-20));
@@ -520,6 +556,7 @@ __global__ void __launch_bounds__(TPB) k_proft(Batch B, Vert<T> vg, Params prm, 
   T x[KB];
   x[KB - 1] = f[KB - 1];
   x[kb] = (cc[kb] * gg[kb - 1] - f[kb]) / (cc[kb] * (T(1) - ee[kb - 1]) - T(1) + T(1e-20));
+  #pragma unroll
   for (int k = kb - 1; k >= 0; --k) x[k] = ee[k] * x[k + 1] + gg[k];
   st_col(s.out + c, x);
 }
@@ -550,6 +587,7 @@ __global__ void __launch_bounds__(TPB) k_proft_adj(Batch B, Vert<T> vg, Params p
     ee[0] = T(0);
     gg[0] = s.fsurf[c2];   // prescribed surface value: no dependence on the inputs
   }
+  #pragma unroll
   for (int k = 1; k < KBM2; ++k) {
     const T den = a[k] + cc[k] * (T(1) - ee[k - 1]) - T(1);
     inv[k] = T(1) / (den + T(1e-20));
@@ -559,12 +597,15 @@ __global__ void __launch_bounds__(TPB) k_proft_adj(Batch B, Vert<T> vg, Params p
   const int kb = KBM1 - 1;
   const T denb = cc[kb] * (T(1) - ee[kb - 1]) - T(1) + T(1e-20);
   x[kb] = (cc[kb] * gg[kb - 1] - f[kb]) / denb;
+  #pragma unroll
   for (int k = kb - 1; k >= 0; --k) x[k] = ee[k] * x[k + 1] + gg[k];
   // ---- reverse
   T bx[KB], bee[KB], bgg[KB], ba[KB], bc[KB], bf[KB];
   ld_col(s.b_out + c, bx);
+  #pragma unroll
   for (int k = 0; k < KB; ++k) bee[k] = bgg[k] = ba[k] = bc[k] = bf[k] = T(0);
   bf[KB - 1] += bx[KB - 1];
+  #pragma unroll
   for (int k = 0; k <= kb - 1; ++k) {   // x_k = ee_k x_{k+1} + gg_k (forward ran k = kb-1 .. 0)
     bee[k] += bx[k] * x[k + 1];
     bx[k + 1] += bx[k] * ee[k];
@@ -578,6 +619,7 @@ __global__ void __launch_bounds__(TPB) k_proft_adj(Batch B, Vert<T> vg, Params p
     bf[kb] -= bn;
     bee[kb - 1] -= bd * cc[kb];
   }
+  #pragma unroll
   for (int k = KBM2 - 1; k >= 1; --k) {
     const T sk = cc[k] * gg[k - 1] - f[k];
     const T bs = bgg[k] * inv[k];
@@ -603,7 +645,9 @@ __global__ void __launch_bounds__(TPB) k_proft_adj(Batch B, Vert<T> vg, Params p
   const T dh2 = dh * dh;
   T bdh2 = T(0);
   T bkh[KB];
+  #pragma unroll
   for (int k = 0; k < KB; ++k) bkh[k] = T(0);
+  #pragma unroll
   for (int k = 1; k < KBM1; ++k) {
     const T P = vg.dz(p.m, k - 1) * vg.dzz(p.m, k - 1);
     const T Q = vg.dz(p.m, k) * vg.dzz(p.m, k - 1);
@@ -656,9 +700,12 @@ __device__ __forceinline__ void profuv_col(const Pos& p, const Vert<T>& vg, cons
   const T dti2 = T(prm.dti2);
   ld_col(s.f + c, C.f);
   C.dhf = T(0.5) * (s.h[c2] + s.etf[c2] + s.h[n2] + s.etf[n2]);
+  #pragma unroll
   for (int k = 0; k < KB; ++k) C.kmf[k] = T(0.5) * (s.km[c + k] + s.km[cn + k]);
   const T dh2 = C.dhf * C.dhf;
+  #pragma unroll
   for (int k = 0; k < KB; ++k) C.a[k] = C.c[k] = T(0);
+  #pragma unroll
   for (int k = 1; k < KBM1; ++k) {
     const T num = -dti2 * (C.kmf[k] + T(prm.umol));
     C.a[k - 1] = num / (vg.dz(p.m, k - 1) * vg.dzz(p.m, k - 1) * dh2);
@@ -668,6 +715,7 @@ __device__ __forceinline__ void profuv_col(const Pos& p, const Vert<T>& vg, cons
   C.ee[0] = C.a[0] / (C.den0 + T(1e-20));
   C.tflux = -dti2 * s.wsurf[c2] / (-vg.dz(p.m, 0) * C.dhf);
   C.gg[0] = (C.tflux - C.f[0]) / (C.den0 + T(1e-20));
+  #pragma unroll
   for (int k = 1; k < KBM2; ++k) {
     const T den = C.a[k] + C.c[k] * (T(1) - C.ee[k - 1]) - T(1);
     C.inv[k] = T(1) / (den + T(1e-20));
@@ -683,6 +731,7 @@ __device__ __forceinline__ void profuv_col(const Pos& p, const Vert<T>& vg, cons
   C.m = s.mask[c2];
   const T numb = C.c[kb] * C.gg[kb - 1] - C.f[kb];
   C.x[kb] = numb / (C.denb + T(1e-20)) * C.m;
+  #pragma unroll
   for (int k = kb - 1; k >= 0; --k) C.x[k] = (C.ee[k] * C.x[k + 1] + C.gg[k]) * C.m;
   C.x[KB - 1] = C.f[KB - 1];
 }
@@ -724,6 +773,7 @@ __global__ void __launch_bounds__(TPB) k_profuv_adj1(Batch B, Vert<T> vg, Params
   if (!pos2(B, p)) return;
   const int64_t c = ix3(p, p.i, p.j, 0), c2 = ix2(p, p.i, p.j);
   T zero[KB];
+  #pragma unroll
   for (int k = 0; k < KB; ++k) zero[k] = T(0);
   if (!interior_pos(p)) {
     T g[KB];
@@ -741,6 +791,7 @@ __global__ void __launch_bounds__(TPB) k_profuv_adj1(Batch B, Vert<T> vg, Params
   const int kb = KBM1 - 1;
   T bx[KB], bee[KB], bgg[KB], ba[KB], bc[KB], bf[KB];
   ld_col(s.b_out + c, bx);
+  #pragma unroll
   for (int k = 0; k < KB; ++k) bee[k] = bgg[k] = ba[k] = bc[k] = bf[k] = T(0);
   bf[KB - 1] += bx[KB - 1];
   // wbot = -tps * x(kb)
@@ -748,6 +799,7 @@ __global__ void __launch_bounds__(TPB) k_profuv_adj1(Batch B, Vert<T> vg, Params
   bx[kb] += -bw * C.tps;
   T btps = -bw * C.x[kb];
   // x_k = (ee_k x_{k+1} + gg_k) * m
+  #pragma unroll
   for (int k = 0; k <= kb - 1; ++k) {
     const T g = bx[k] * C.m;
     bee[k] += g * C.x[k + 1];
@@ -772,6 +824,7 @@ __global__ void __launch_bounds__(TPB) k_profuv_adj1(Batch B, Vert<T> vg, Params
   T bdhf = -bfric * C.fric / C.dhf;
   // tps = cbca * mag
   s.s_mag[c2] = btps * C.cbca;
+  #pragma unroll
   for (int k = KBM2 - 1; k >= 1; --k) {
     const T sk = C.c[k] * C.gg[k - 1] - C.f[k];
     const T bs = bgg[k] * C.inv[k];
@@ -796,7 +849,9 @@ __global__ void __launch_bounds__(TPB) k_profuv_adj1(Batch B, Vert<T> vg, Params
   const T dh2 = C.dhf * C.dhf;
   T bdh2 = T(0);
   T bkmf[KB];
+  #pragma unroll
   for (int k = 0; k < KB; ++k) bkmf[k] = T(0);
+  #pragma unroll
   for (int k = 1; k < KBM1; ++k) {
     const T P = vg.dz(p.m, k - 1) * vg.dzz(p.m, k - 1);
     const T Q = vg.dz(p.m, k) * vg.dzz(p.m, k - 1);
@@ -805,6 +860,7 @@ __global__ void __launch_bounds__(TPB) k_profuv_adj1(Batch B, Vert<T> vg, Params
     bkmf[k] += bnum * (-dti2);
   }
   bdhf += T(2) * C.dhf * bdh2;
+  #pragma unroll
   for (int k = 0; k < KB; ++k) bkmf[k] *= T(0.5);
   if (s.b_f) add_col(s.b_f + c, bf);
   st_col(s.s_km0 + c, bkmf);
@@ -832,6 +888,7 @@ __global__ void __launch_bounds__(TPB) k_profuv_adj2(Batch B, Vert<T> vg, Profuv
   ld_col(s.s_km0 + c, g);
   if (has_o) {
     const int64_t oc = ix3(p, oi, oj, 0);
+    #pragma unroll
     for (int k = 0; k < KB; ++k) g[k] += s.s_km1[oc + k];
   }
   if (s.b_km) add_col(s.b_km + c, g);
@@ -890,12 +947,14 @@ __global__ void __launch_bounds__(TPB) k_uvfilt(Batch B, Vert<T> vg, Params prm,
   ld_col(a.ub + c, ub);
   ld_col(a.u1 + c, u);
   T s = T(0);
+  #pragma unroll
   for (int k = 0; k < KBM1; ++k) {
     term[k] = uf[k] + ub[k] - T(2) * u[k];
     s += term[k] * vg.dz(p.m, k);
   }
   const T f = T(0.5 * prm.smoth);
   T o[KB];
+  #pragma unroll
   for (int k = 0; k < KBM1; ++k) o[k] = u[k] + f * (term[k] - s);
   o[KB - 1] = u[KB - 1];
   st_col(a.ub_new + c, o);
@@ -912,13 +971,18 @@ __global__ void __launch_bounds__(TPB) k_uvfilt_adj(Batch B, Vert<T> vg, Params 
   const int64_t c = ix3(p, p.i, p.j, 0);
   const T f = T(0.5 * prm.smoth);
   T g[KB], gt[KB], gu[KB], gf[KB];
+  #pragma unroll
   for (int k = 0; k < KB; ++k) g[k] = a.b_ub_new ? a.b_ub_new[c + k] : T(0);
   T S = T(0);
+  #pragma unroll
   for (int k = 0; k < KBM1; ++k) S += g[k];
+  #pragma unroll
   for (int k = 0; k < KBM1; ++k) gt[k] = f * g[k] - f * S * vg.dz(p.m, k);
   gt[KB - 1] = T(0);
+  #pragma unroll
   for (int k = 0; k < KBM1; ++k) gu[k] = g[k] - T(2) * gt[k];
   gu[KB - 1] = g[KB - 1];
+  #pragma unroll
   for (int k = 0; k < KB; ++k) gf[k] = gt[k] + (a.b_u_new ? a.b_u_new[c + k] : T(0));
   if (a.b_uf) add_col(a.b_uf + c, gf);
   if (a.b_ub) add_col(a.b_ub + c, gt);

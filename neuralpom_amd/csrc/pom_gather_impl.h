@@ -15,6 +15,9 @@ void launch_gather(const Ctx<T>& c, const FieldTab<T>& tab, const COutTab<T>& ob
     if constexpr (F < Op::N3) {
       hipLaunchKernelGGL((k_gather3<T, Op, F>), dim3(c.nblk3), dim3(TPB), 0, c.stream, c.B, c.vert, c.ramp, c.prm,
                          tab, obar, inbar[F]);
+    } else if constexpr (Op::OUT3) {
+      hipLaunchKernelGGL((k_gather2z<T, Op, F>), dim3(c.nblk3), dim3(TPB), 0, c.stream, c.B, c.vert, c.ramp,
+                         c.prm, tab, obar, inbar[F]);
     } else {
       hipLaunchKernelGGL((k_gather2<T, Op, F>), dim3(c.nblk2), dim3(TPB), 0, c.stream, c.B, c.vert, c.ramp, c.prm,
                          tab, obar, inbar[F]);

@@ -86,30 +86,44 @@ __global__ void __launch_bounds__(TPB) k_gather3(Batch B, Vert<T> vert, const T*
   inbar[ix3(q, q.i, q.j, q.k)] += acc;
 }
 
-// 2-D seeded field: one thread per column; 3-D outputs sum over all levels
+// 2-D seeded field of a 3-D operator: one thread per (column, level) sums the
+// partials of that level's outputs; the 16 lanes of a water column (k is the
+// fastest index, so a column is 16 consecutive lanes of one wavefront) are
+// reduced with cross-lane shuffles and lane k = 0 accumulates the column.
+template <class T, class Op, int F>
+__global__ void __launch_bounds__(TPB) k_gather2z(Batch B, Vert<T> vert, const T* ramp, Params prm,
+                                                   FieldTab<T> tab, COutTab<T> obar, T* inbar) {
+  Pos q;
+  const bool live = pos3(B, q);
+  T acc = T(0);
+  if (live) {
+    static_for<-Op::RI, Op::RI>([&](auto I) {
+      static_for<-Op::RJ, Op::RJ>([&](auto J) {
+        constexpr int di = decltype(I)::value, dj = decltype(J)::value;
+        gather_term<T, Op, F, di, dj, 0>(tab, obar, q, q.k, vert, ramp, prm, acc);
+      });
+    });
+  }
+  // a member's point count is a multiple of 16, so the 16-lane groups never
+  // straddle columns; inactive lanes contribute zero
+#pragma unroll
+  for (int o = 8; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 16);
+  if (live && q.k == 0) inbar[ix2(q, q.i, q.j)] += acc;
+}
+
+// 2-D seeded field of a 2-D operator: one thread per column
 template <class T, class Op, int F>
 __global__ void __launch_bounds__(TPB) k_gather2(Batch B, Vert<T> vert, const T* ramp, Params prm,
                                                   FieldTab<T> tab, COutTab<T> obar, T* inbar) {
   Pos q;
   if (!pos2(B, q)) return;
   T acc = T(0);
-  if constexpr (Op::OUT3) {
-    for (int k = 0; k < KB; ++k) {
-      static_for<-Op::RI, Op::RI>([&](auto I) {
-        static_for<-Op::RJ, Op::RJ>([&](auto J) {
-          constexpr int di = decltype(I)::value, dj = decltype(J)::value;
-          gather_term<T, Op, F, di, dj, 0>(tab, obar, q, k, vert, ramp, prm, acc);
-        });
-      });
-    }
-  } else {
-    static_for<-Op::RI, Op::RI>([&](auto I) {
-      static_for<-Op::RJ, Op::RJ>([&](auto J) {
-        constexpr int di = decltype(I)::value, dj = decltype(J)::value;
-        gather_term<T, Op, F, di, dj, 0>(tab, obar, q, 0, vert, ramp, prm, acc);
-      });
+  static_for<-Op::RI, Op::RI>([&](auto I) {
+    static_for<-Op::RJ, Op::RJ>([&](auto J) {
+      constexpr int di = decltype(I)::value, dj = decltype(J)::value;
+      gather_term<T, Op, F, di, dj, 0>(tab, obar, q, 0, vert, ramp, prm, acc);
     });
-  }
+  });
   inbar[ix2(q, q.i, q.j)] += acc;
 }
 

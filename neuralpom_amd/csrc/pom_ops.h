@@ -590,38 +590,49 @@ struct OpKEdge {
 // reference: src/neuralpom/cores/explicit/time_loop.py:275-431
 // =====================================================================
 
-// elf after the continuity update and bcond 1 (baroclinic.py:510-524):
-// elf(i,j) = elf_raw(clamp(i), clamp(j)) * fsm(i,j)
-struct OpExtElf {
+// continuity update of the surface elevation on interior cells, zero
+// elsewhere (time_loop.py:299-312)
+struct OpExtElfRaw {
   enum { UA, VA, D_, ELB, N3 = 0, NIN = 4 };
-  enum { GDX, GDY, GART, GFSM, GVFLUXF };
+  enum { GDX, GDY, GART, GVFLUXF };
   static constexpr int NOUT = 1;
   static constexpr bool OUT3 = false;
-  static constexpr int RI = 2, RJ = 2, RK = 0;
+  static constexpr int RI = 1, RJ = 1, RK = 0;
   template <int OI, int OJ, class AC> __device__ static auto fu(const AC& A) {   // :299-301
     return 0.25 * (AT2(D_, OI, OJ) + AT2(D_, OI - 1, OJ)) * (G2(GDY, OI, OJ) + G2(GDY, OI - 1, OJ)) * AT2(UA, OI, OJ);
   }
   template <int OI, int OJ, class AC> __device__ static auto fv(const AC& A) {   // :303-305
     return 0.25 * (AT2(D_, OI, OJ) + AT2(D_, OI, OJ - 1)) * (G2(GDX, OI, OJ) + G2(GDX, OI, OJ - 1)) * AT2(VA, OI, OJ);
   }
-  template <int OI, int OJ, class AC> __device__ static auto raw(const AC& A, const Params& prm) {  // :308-312
-    auto div = fu<OI + 1, OJ>(A) - fu<OI, OJ>(A) + fv<OI, OJ + 1>(A) - fv<OI, OJ>(A);
-    auto term = -(div / G2(GART, OI, OJ)) - G2(GVFLUXF, OI, OJ);
-    return AT2(ELB, OI, OJ) + prm.dte2 * term;
+  template <int O, class AC> __device__ static auto eval(const AC& A, const Params& prm) {   // :308-312
+    return IFZ(interior(A), ([&]() {
+      auto div = fu<1, 0>(A) - fu<0, 0>(A) + fv<0, 1>(A) - fv<0, 0>(A);
+      auto term = -(div / G2(GART, 0, 0)) - G2(GVFLUXF, 0, 0);
+      return AT2(ELB, 0, 0) + prm.dte2 * term;
+    }()));
   }
-  template <int O, class AC> __device__ static auto eval(const AC& A, const Params& prm) {
+};
+
+// bcond 1 (baroclinic.py:510-524): the E, W, N, S edge copies make every
+// edge cell take the value of its nearest interior cell; then * fsm
+struct OpBc1 {
+  enum { RAW, N3 = 0, NIN = 1 };
+  enum { GFSM };
+  static constexpr int NOUT = 1;
+  static constexpr bool OUT3 = false;
+  static constexpr int RI = 1, RJ = 1, RK = 0;
+  template <int O, class AC> __device__ static auto eval(const AC& A, const Params&) {
     const int i = A.p.i, j = A.p.j, im = A.p.im, jm = A.p.jm;
     const int di = (i == 0) ? 1 : (i == im - 1 ? -1 : 0);
     const int dj = (j == 0) ? 1 : (j == jm - 1 ? -1 : 0);
-    using R = decltype(raw<0, 0>(A, prm) + raw<1, 1>(A, prm) + raw<-1, -1>(A, prm) + raw<1, -1>(A, prm) +
-                       raw<-1, 1>(A, prm) + raw<1, 0>(A, prm) + raw<-1, 0>(A, prm) + raw<0, 1>(A, prm) +
-                       raw<0, -1>(A, prm));
+    using R = decltype(AT2(RAW, 0, 0) + AT2(RAW, 1, 1) + AT2(RAW, -1, -1) + AT2(RAW, 1, -1) + AT2(RAW, -1, 1) +
+                       AT2(RAW, 1, 0) + AT2(RAW, -1, 0) + AT2(RAW, 0, 1) + AT2(RAW, 0, -1));
     R r{};
-#define POM_ELF_CASE(a, b) if (di == a && dj == b) r = cvt<R>(raw<a, b>(A, prm));
-    POM_ELF_CASE(0, 0) else POM_ELF_CASE(1, 0) else POM_ELF_CASE(-1, 0) else POM_ELF_CASE(0, 1)
-    else POM_ELF_CASE(0, -1) else POM_ELF_CASE(1, 1) else POM_ELF_CASE(1, -1) else POM_ELF_CASE(-1, 1)
-    else POM_ELF_CASE(-1, -1)
-#undef POM_ELF_CASE
+#define POM_BC1_CASE(a, b) if (di == a && dj == b) r = cvt<R>(AT2(RAW, a, b));
+    POM_BC1_CASE(0, 0) else POM_BC1_CASE(1, 0) else POM_BC1_CASE(-1, 0) else POM_BC1_CASE(0, 1)
+    else POM_BC1_CASE(0, -1) else POM_BC1_CASE(1, 1) else POM_BC1_CASE(1, -1) else POM_BC1_CASE(-1, 1)
+    else POM_BC1_CASE(-1, -1)
+#undef POM_BC1_CASE
     return r * G2(GFSM, 0, 0);
   }
 };

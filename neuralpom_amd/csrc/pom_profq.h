@@ -45,7 +45,9 @@ __device__ __forceinline__ void profq_fwd_col(const Pos& p, const Vert<T>& vg, c
   const T dti2 = T(prm.dti2), umol = T(prm.umol), grav = T(prm.grav), kappa = T(prm.kappa), small = T(prm.small);
   C.dh = s.h[c2] + s.etf[c2];
   const T dh = C.dh;
+  #pragma unroll
   for (int k = 0; k < KB; ++k) C.a[k] = C.c[k] = T(0);
+  #pragma unroll
   for (int k = 1; k < KBM1; ++k) {
     C.a[k] = -dti2 * (s.kq[c + k + 1] + s.kq[c + k] + T(2.0 * prm.umol)) * T(0.5) /
              (vg.dzz(m, k - 1) * vg.dz(m, k) * dh * dh);
@@ -70,7 +72,9 @@ __device__ __forceinline__ void profq_fwd_col(const Pos& p, const Vert<T>& vg, c
     C.q2bot = C.qbm * T(pow(16.6, 2.0 / 3.0) * SEF);
   }
   // sound speed (tracers.py:245-255)
+  #pragma unroll
   for (int k = 0; k < KB; ++k) C.cc[k] = T(0);
+  #pragma unroll
   for (int k = 0; k < KBM1; ++k) {
     const T tp = s.t[c + k] + T(prm.tbias), sp = s.s[c + k] + T(prm.sbias);
     const T pp = T(prm.grav * prm.rhoref) * (-vg.zz(m, k) * dh) * T(1.e-4);
@@ -79,15 +83,21 @@ __device__ __forceinline__ void profq_fwd_col(const Pos& p, const Vert<T>& vg, c
     const T rad = (T(1) - T(0.01642) * pp / ccs) * (T(1) - T(0.40) * pp / (ccs * ccs));
     C.cc[k] = ccs / sqrt(fmax(rad, T(1e-8)));
   }
+  #pragma unroll
   for (int k = 0; k < KB; ++k) C.boygr[k] = T(0);
+  #pragma unroll
   for (int k = 1; k < KBM1; ++k)
     C.boygr[k] = grav * (s.rho[c + k - 1] - s.rho[c + k]) / (vg.dzz(m, k - 1) * dh) +
                  T(prm.grav * prm.grav * 2.0) / (C.cc[k - 1] * C.cc[k - 1] + C.cc[k] * C.cc[k]);
   // length scale (tracers.py:263-285)
+  #pragma unroll
   for (int k = 0; k < KB; ++k) C.lraw[k] = C.ghr[k] = C.gh[k] = T(0);
+  #pragma unroll
   for (int k = 1; k < KBM1; ++k) C.lraw[k] = fabs(s.q2lb[c + k]) / (fabs(s.q2b[c + k]) + T(1e-20));
   const T kl0 = kappa * C.l0;
+  #pragma unroll
   for (int k = 0; k < KBM1; ++k) C.l[k] = (vg.z(m, k) > T(-0.5)) ? (C.lraw[k] >= kl0 ? C.lraw[k] : kl0) : C.lraw[k];
+  #pragma unroll
   for (int k = 1; k < KBM1; ++k) {
     C.ghr[k] = (C.l[k] * C.l[k]) * C.boygr[k] / (fabs(s.q2b[c + k]) + T(1e-20));
     C.gh[k] = C.ghr[k] <= T(0.028) ? C.ghr[k] : T(0.028);
@@ -95,9 +105,11 @@ __device__ __forceinline__ void profq_fwd_col(const Pos& p, const Vert<T>& vg, c
   C.l[0] = kappa * C.l0;
   C.l[KB - 1] = T(0);
   // shear production (tracers.py:287-300)
+  #pragma unroll
   for (int k = 0; k < KB; ++k) C.kn[k] = T(0);
   if (C.inner) {
     const int64_t cu = ix3(p, i + 1, j, 0), cv = ix3(p, i, j + 1, 0);
+    #pragma unroll
     for (int k = 1; k < KBM1; ++k) {
       const T du = s.u[c + k] - s.u[c + k - 1] + s.u[cu + k] - s.u[cu + k - 1];
       const T dv = s.v[c + k] - s.v[c + k - 1] + s.v[cv + k] - s.v[cv + k - 1];
@@ -106,19 +118,23 @@ __device__ __forceinline__ void profq_fwd_col(const Pos& p, const Vert<T>& vg, c
       C.kn[k] = val + s.kh[c + k] * C.boygr[k];
     }
   }
+  #pragma unroll
   for (int k = 0; k < KB; ++k) C.dtef[k] = sqrt(fabs(s.q2b[c + k]) + T(1e-8)) * T(1) / (T(B1) * C.l[k] + small);
   // q2 tridiagonal solve (tracers.py:302-346)
   C.ee1[0] = T(0);
   C.gg1[0] = C.has_s ? T(pow(15.8 * CBCNST, 2.0 / 3.0)) * C.utau2 : T(0);
+  #pragma unroll
   for (int k = 1; k < KBM1; ++k) {
     const T den = C.a[k] + C.c[k] * (T(1) - C.ee1[k - 1]) - (T(2) * dti2 * C.dtef[k] + T(1));
     C.ee1[k] = C.a[k] / (den + T(1e-20));
     C.gg1[k] = (T(-2.0 * prm.dti2) * C.kn[k] + C.c[k] * C.gg1[k - 1] - s.q2f[c + k]) / (den + T(1e-20));
   }
   C.q2n[KB - 1] = C.q2bot;
+  #pragma unroll
   for (int k = KBM1 - 1; k >= 0; --k) C.q2n[k] = C.ee1[k] * C.q2n[k + 1] + C.gg1[k];
   // q2l tridiagonal solve with wall proximity (tracers.py:348-384)
   C.ee2[0] = C.gg2[0] = T(0);
+  #pragma unroll
   for (int k = 1; k < KBM1; ++k) {
     const T W = T(1) / (fabs(vg.z(m, k) - vg.z(m, 0)) + T(1e-20)) + T(1) / (fabs(vg.z(m, k) - vg.z(m, KB - 1)) + T(1e-20));
     const T wall = W * C.l[k] / (dh * kappa);
@@ -128,6 +144,7 @@ __device__ __forceinline__ void profq_fwd_col(const Pos& p, const Vert<T>& vg, c
     C.gg2[k] = (dti2 * (-C.kn[k] * C.l[k] * T(E1)) + C.c[k] * C.gg2[k - 1] - s.q2lf[c + k]) / (den + T(1e-20));
   }
   C.q2ln[KB - 1] = T(0);
+  #pragma unroll
   for (int k = KBM1 - 1; k >= 0; --k) C.q2ln[k] = C.ee2[k] * C.q2ln[k + 1] + C.gg2[k];
 }
 
@@ -140,6 +157,7 @@ __global__ void __launch_bounds__(TPB) k_profq(Batch B, Vert<T> vg, Params prm, 
   ProfqCol<T> C;
   profq_fwd_col(p, vg, prm, s, C);
   const T small = T(prm.small);
+  #pragma unroll
   for (int k = 0; k < KB; ++k) {
     T q2c = C.q2n[k], q2lc = C.q2ln[k];
     if (k >= 1 && k < KBM1 && (C.q2n[k] <= small || C.q2ln[k] <= small)) {
@@ -185,6 +203,7 @@ __global__ void __launch_bounds__(TPB) k_profq_adj(Batch B, Vert<T> vg, Params p
 
   T bl[KB], bgh[KB], bq2n[KB], bq2ln[KB];
   T bkq[KB], bkm[KB], bkh[KB], bq2[KB], bq2b[KB], bdt2 = T(0);
+  #pragma unroll
   for (int k = 0; k < KB; ++k) {
     bl[k] = ldb(S.b_lo, c + k);
     bgh[k] = T(0);
@@ -224,17 +243,21 @@ __global__ void __launch_bounds__(TPB) k_profq_adj(Batch B, Vert<T> vg, Params p
     }
   }
   T ba[KB], bc[KB], bkn[KB], bdtef[KB], bq2f[KB], bq2lf[KB];
+  #pragma unroll
   for (int k = 0; k < KB; ++k) ba[k] = bc[k] = bkn[k] = bdtef[k] = bq2f[k] = bq2lf[k] = T(0);
   T bdh = T(0);
   // ---- q2l solve reverse
   {
     T bee[KB], bgg[KB];
+    #pragma unroll
     for (int k = 0; k < KB; ++k) bee[k] = bgg[k] = T(0);
+    #pragma unroll
     for (int k = 0; k <= KBM1 - 1; ++k) {
       bee[k] += bq2ln[k] * C.q2ln[k + 1];
       bq2ln[k + 1] += bq2ln[k] * C.ee2[k];
       bgg[k] += bq2ln[k];
     }
+    #pragma unroll
     for (int k = KBM1 - 1; k >= 1; --k) {
       const T W = T(1) / (fabs(vg.z(m, k) - vg.z(m, 0)) + T(1e-20)) + T(1) / (fabs(vg.z(m, k) - vg.z(m, KB - 1)) + T(1e-20));
       const T wall = W * C.l[k] / (dh * kappa);
@@ -261,13 +284,16 @@ __global__ void __launch_bounds__(TPB) k_profq_adj(Batch B, Vert<T> vg, Params p
   T bq2bot;
   {
     T bee[KB], bgg[KB];
+    #pragma unroll
     for (int k = 0; k < KB; ++k) bee[k] = bgg[k] = T(0);
+    #pragma unroll
     for (int k = 0; k <= KBM1 - 1; ++k) {
       bee[k] += bq2n[k] * C.q2n[k + 1];
       bq2n[k + 1] += bq2n[k] * C.ee1[k];
       bgg[k] += bq2n[k];
     }
     bq2bot = bq2n[KB - 1];
+    #pragma unroll
     for (int k = KBM1 - 1; k >= 1; --k) {
       const T den = C.a[k] + C.c[k] * (T(1) - C.ee1[k - 1]) - (T(2) * dti2 * C.dtef[k] + T(1));
       const T de = den + T(1e-20);
@@ -284,6 +310,7 @@ __global__ void __launch_bounds__(TPB) k_profq_adj(Batch B, Vert<T> vg, Params p
     // gg1(0) = const * utau2: utau2 depends only on passive surface stress
   }
   // ---- dtef = sqrt(|q2b| + 1e-8) / (B1 l + small)
+  #pragma unroll
   for (int k = 0; k < KB; ++k) {
     const T qb = s.q2b[c + k];
     const T sqb = sqrt(fabs(qb) + T(1e-8));
@@ -293,9 +320,11 @@ __global__ void __launch_bounds__(TPB) k_profq_adj(Batch B, Vert<T> vg, Params p
   }
   // ---- shear production
   T bboy[KB], bu[KB], bun[KB], bv[KB], bvn[KB];
+  #pragma unroll
   for (int k = 0; k < KB; ++k) bboy[k] = bu[k] = bun[k] = bv[k] = bvn[k] = T(0);
   if (C.inner) {
     const int64_t cu = ix3(p, p.i + 1, p.j, 0), cv = ix3(p, p.i, p.j + 1, 0);
+    #pragma unroll
     for (int k = 1; k < KBM1; ++k) {
       const T du = s.u[c + k] - s.u[c + k - 1] + s.u[cu + k] - s.u[cu + k - 1];
       const T dv = s.v[c + k] - s.v[c + k - 1] + s.v[cv + k] - s.v[cv + k - 1];
@@ -323,7 +352,9 @@ __global__ void __launch_bounds__(TPB) k_profq_adj(Batch B, Vert<T> vg, Params p
   // ---- stability-function argument gh and length scale
   T bl0 = T(0);
   T bq2lb[KB];
+  #pragma unroll
   for (int k = 0; k < KB; ++k) bq2lb[k] = T(0);
+  #pragma unroll
   for (int k = 1; k < KBM1; ++k) {
     if (C.ghr[k] <= T(0.028)) {
       const T qb = s.q2b[c + k];
@@ -336,6 +367,7 @@ __global__ void __launch_bounds__(TPB) k_profq_adj(Batch B, Vert<T> vg, Params p
   }
   bl0 += bl[0] * kappa;
   const T kl0 = kappa * C.l0;
+  #pragma unroll
   for (int k = 1; k < KBM1; ++k) {
     T blr = bl[k];
     if (vg.z(m, k) > T(-0.5) && !(C.lraw[k] >= kl0)) {
@@ -350,7 +382,9 @@ __global__ void __launch_bounds__(TPB) k_profq_adj(Batch B, Vert<T> vg, Params p
   (void)bl0;   // l0 depends only on the passive surface stress
   // ---- buoyancy
   T bcc[KB], brho[KB];
+  #pragma unroll
   for (int k = 0; k < KB; ++k) bcc[k] = brho[k] = T(0);
+  #pragma unroll
   for (int k = 1; k < KBM1; ++k) {
     const T g = bboy[k];
     const T dd = vg.dzz(m, k - 1) * dh;
@@ -365,7 +399,9 @@ __global__ void __launch_bounds__(TPB) k_profq_adj(Batch B, Vert<T> vg, Params p
     bcc[k] += bSc * T(2) * C.cc[k];
   }
   T bt[KB], bs[KB];
+  #pragma unroll
   for (int k = 0; k < KB; ++k) bt[k] = bs[k] = T(0);
+  #pragma unroll
   for (int k = 0; k < KBM1; ++k) {
     const T tp = s.t[c + k] + T(prm.tbias), sp = s.s[c + k] + T(prm.sbias);
     const T cz = T(prm.grav * prm.rhoref) * (-vg.zz(m, k));
@@ -399,6 +435,7 @@ __global__ void __launch_bounds__(TPB) k_profq_adj(Batch B, Vert<T> vg, Params p
     bwvb = T(0.5) * g * C.vbt;
   }
   // ---- diffusion coefficients a, c from kq and dh
+  #pragma unroll
   for (int k = 1; k < KBM1; ++k) {
     const T Da = vg.dzz(m, k - 1) * vg.dz(m, k) * dh * dh;
     const T Dc = vg.dzz(m, k - 1) * vg.dz(m, k - 1) * dh * dh;
@@ -443,12 +480,14 @@ __global__ void __launch_bounds__(TPB) k_profq_adj2(Batch B, ProfqAdjArgs<T> S) 
   if (p.i >= 1) {
     const int64_t o = ix3(p, p.i - 1, p.j, 0);
     if (S.b_u)
+      #pragma unroll
       for (int k = 0; k < KB; ++k) S.b_u[c + k] += S.s_unb[o + k];
     if (S.b_wubot) S.b_wubot[c2] += S.s_wubnb[ix2(p, p.i - 1, p.j)];
   }
   if (p.j >= 1) {
     const int64_t o = ix3(p, p.i, p.j - 1, 0);
     if (S.b_v)
+      #pragma unroll
       for (int k = 0; k < KB; ++k) S.b_v[c + k] += S.s_vnb[o + k];
     if (S.b_wvbot) S.b_wvbot[c2] += S.s_wvbnb[ix2(p, p.i, p.j - 1)];
   }

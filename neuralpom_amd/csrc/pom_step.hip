@@ -69,12 +69,20 @@ __device__ __forceinline__ void amax(float* f, float v) { atomicMax(reinterpret_
 __device__ __forceinline__ void amax(double* f, double v) {
   atomicMax(reinterpret_cast<unsigned long long*>(f), (unsigned long long)__double_as_longlong(v));
 }
+// grid-stride max over |vaf|, wavefront shuffles, one atomic per block
 template <class T>
-__global__ void k_vamax(const T* __restrict__ vaf, int64_t n, T* out) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  T v = i < n ? fabs(vaf[i]) : T(0);
+__global__ void __launch_bounds__(256) k_vamax(const T* __restrict__ vaf, int64_t n, T* out) {
+  __shared__ T red[4];
+  T v = T(0);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    v = fmax(v, fabs(vaf[i]));
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-  if ((threadIdx.x & 63) == 0 && v > T(0)) amax(out, v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    v = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    if (v > T(0)) amax(out, v);
+  }
 }
 
 inline int blocks(int64_t n) { return (int)((n + 255) / 256); }
@@ -117,6 +125,7 @@ template <class T> struct Step {
     egf0 = A.get<T>(N2);
     utf0 = A.get<T>(N2);
     vtf0 = A.get<T>(N2);
+    elfs = A.get<T>(N2);
     for (int n = 1; n <= isplit; ++n) {
       X<T>& x = xs[n];
       const bool last = n == isplit;
@@ -263,15 +272,20 @@ template <class T> struct Step {
   }
   FieldTab<T> t_elf(const X<T>& x) const {
     FieldTab<T> t{};
-    t.f[OpExtElf::UA] = x.ua;
-    t.f[OpExtElf::VA] = x.va;
-    t.f[OpExtElf::D_] = x.d;
-    t.f[OpExtElf::ELB] = x.elb;
-    t.g[OpExtElf::GDX] = g[POM_G_DX];
-    t.g[OpExtElf::GDY] = g[POM_G_DY];
-    t.g[OpExtElf::GART] = g[POM_G_ART];
-    t.g[OpExtElf::GFSM] = g[POM_G_FSM];
-    t.g[OpExtElf::GVFLUXF] = out[POM_VFLUXF];
+    t.f[OpExtElfRaw::UA] = x.ua;
+    t.f[OpExtElfRaw::VA] = x.va;
+    t.f[OpExtElfRaw::D_] = x.d;
+    t.f[OpExtElfRaw::ELB] = x.elb;
+    t.g[OpExtElfRaw::GDX] = g[POM_G_DX];
+    t.g[OpExtElfRaw::GDY] = g[POM_G_DY];
+    t.g[OpExtElfRaw::GART] = g[POM_G_ART];
+    t.g[OpExtElfRaw::GVFLUXF] = out[POM_VFLUXF];
+    return t;
+  }
+  FieldTab<T> t_bc1() const {
+    FieldTab<T> t{};
+    t.f[OpBc1::RAW] = elfs;
+    t.g[OpBc1::GFSM] = g[POM_G_FSM];
     return t;
   }
   FieldTab<T> t_uv(const X<T>& xp, const X<T>& xn) const {
@@ -589,13 +603,14 @@ template <class T> struct Step {
       prm.iext = n;
       Ctx<T> cn = c;
       cn.prm = prm;
-      launch_map<T, OpExtElf>(cn, t_elf(xp), OutTab<T>{{xn.elf}});
+      launch_map<T, OpExtElfRaw>(cn, t_elf(xp), OutTab<T>{{elfs}});
+      launch_map<T, OpBc1>(cn, t_bc1(), OutTab<T>{{xn.elf}});
       if (n % c.prm.ispadv == 0) {
         launch_map<T, OpAdvave>(cn, t_advave(xp), OutTab<T>{{xn.advua, xn.advva, xn.fluxua, xn.fluxva, xn.tps}});
       }
       resolve(n, xn);
       launch_map<T, OpExtUV>(cn, t_uv(xp, xn), OutTab<T>{{xn.uaf, xn.vaf}});
-      hipLaunchKernelGGL(k_vamax<T>, dim3(blocks(N2)), dim3(256), 0, c.stream, xn.vaf, N2, vamax);
+      hipLaunchKernelGGL(k_vamax<T>, dim3(std::min(blocks(N2), 1024)), dim3(256), 0, c.stream, xn.vaf, N2, vamax);
       launch_map<T, OpExtFilter>(cn, t_filter(xp, xn),
                                  OutTab<T>{{xn.ua, xn.uab, xn.va, xn.vab, xn.el, xn.elb, xn.d, xn.egf, xn.utf, xn.vtf,
                                             xn.etf}});
@@ -787,9 +802,13 @@ template <class T> struct Adj {
         add(bp.advva, bn.advva, N2);
       }
       {
+        zero(s2d, N2);
         COutTab<T> ob{{bn.elf}};
-        T* ib[OpExtElf::NIN] = {bp.ua, bp.va, bp.d, bp.elb};
-        launch_gather<T, OpExtElf>(cn, S.t_elf(xp), ob, ib);
+        T* ib[OpBc1::NIN] = {s2d};
+        launch_gather<T, OpBc1>(cn, S.t_bc1(), ob, ib);
+        COutTab<T> ob2{{s2d}};
+        T* ib2[OpExtElfRaw::NIN] = {bp.ua, bp.va, bp.d, bp.elb};
+        launch_gather<T, OpExtElfRaw>(cn, S.t_elf(xp), ob2, ib2);
       }
     }
     X<T>& b0 = bx[0];

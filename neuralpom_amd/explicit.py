@@ -80,7 +80,10 @@ class Ensemble:
             raise L.HipExtensionError("no HIP device: the explicit step has no CPU path")
         self.shapes = [(int(a), int(b)) for a, b in shapes]
         self.dtype = dtype
-        self.device = torch.device(device)
+        dev = torch.device(device)
+        if dev.type == "cuda" and dev.index is None:
+            dev = torch.device("cuda", torch.cuda.current_device())
+        self.device = dev
         self.off2, self.off3 = [], []
         o2 = o3 = 0
         for im, jm in self.shapes:

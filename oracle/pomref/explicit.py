@@ -186,8 +186,8 @@ def baropg(S, rho, dt, ramp):
         scale = 0.25 * term_dt * mask * dd
         return col * _b(scale)
 
-    drhox = torch.zeros((im, jm, kb), dtype=rho.dtype)
-    drhoy = torch.zeros((im, jm, kb), dtype=rho.dtype)
+    drhox = torch.zeros((im, jm, kb), dtype=rho.dtype, device=rho.device)
+    drhoy = torch.zeros((im, jm, kb), dtype=rho.dtype, device=rho.device)
     drhox[I, J, :kbm1] = one(rho_a[I, J], rho_a[Im, J], dt[I, J], dt[Im, J],
                              S.dy[I, J] + S.dy[Im, J], S.dum[I, J])
     drhoy[I, J, :kbm1] = one(rho_a[I, J], rho_a[I, Jm], dt[I, J], dt[I, Jm],
@@ -505,9 +505,10 @@ def profq(S, q2f, q2lf, q2, q2b, q2lb, km, kh, kq, u, v, t, s, rho,
     dz, dzz, z, zz = S.dz, S.dzz, S.z, S.zz
     dh = S.h + etf
     dt_ = q2.dtype
+    dev_ = q2.device
 
-    a = torch.zeros((im, jm, kb), dtype=dt_)
-    c = torch.zeros((im, jm, kb), dtype=dt_)
+    a = torch.zeros((im, jm, kb), dtype=dt_, device=dev_)
+    c = torch.zeros((im, jm, kb), dtype=dt_, device=dev_)
     for k in range(1, kbm1):
         a[:, :, k] = -dti2 * (kq[:, :, k + 1] + kq[:, :, k] + 2.0 * umol) * 0.5 \
             / (dzz[k - 1] * dz[k] * dh * dh)
@@ -520,11 +521,11 @@ def profq(S, q2f, q2lf, q2, q2b, q2lb, km, kh, kq, u, v, t, s, rho,
     ust = 0.5 * (wusurf[I0, J0] + wusurf[I1, J0])
     vst = 0.5 * (wvsurf[I0, J0] + wvsurf[I0, J1])
     utau2 = torch.sqrt(ust ** 2 + vst ** 2 + 1e-8)
-    l0 = torch.zeros((im, jm), dtype=dt_)
+    l0 = torch.zeros((im, jm), dtype=dt_, device=dev_)
     l0[I0, J0] = SURFL * utau2 / grav
     ubt = 0.5 * (wubot[I0, J0] + wubot[I1, J0])
     vbt = 0.5 * (wvbot[I0, J0] + wvbot[I0, J1])
-    q2bot = torch.zeros((im, jm), dtype=dt_)
+    q2bot = torch.zeros((im, jm), dtype=dt_, device=dev_)
     q2bot[I0, J0] = torch.sqrt(ubt ** 2 + vbt ** 2 + 1e-8) * const1
 
     # buoyancy with sound-speed compressibility (tracers.py:245-261)
@@ -535,19 +536,19 @@ def profq(S, q2f, q2lf, q2, q2b, q2lb, km, kh, kq, u, v, t, s, rho,
     cc = torch.where(torch.abs(cc) < 1e-6, torch.full_like(cc, 1e-6), cc)
     rad = (1.0 - 0.01642 * p / cc) * (1.0 - 0.40 * p / (cc ** 2))
     cc = cc / torch.sqrt(torch.clamp(rad, min=1e-8))
-    boygr = torch.zeros((im, jm, kb), dtype=dt_)
+    boygr = torch.zeros((im, jm, kb), dtype=dt_, device=dev_)
     for k in range(1, kbm1):
         boygr[:, :, k] = grav * (rho[:, :, k - 1] - rho[:, :, k]) / (dzz[k - 1] * dh) \
             + (grav ** 2) * 2.0 / (cc[:, :, k - 1] ** 2 + cc[:, :, k] ** 2)
 
     # length scale (tracers.py:263-285)
     q2b_abs = torch.abs(q2b)
-    l = torch.zeros((im, jm, kb), dtype=dt_)
+    l = torch.zeros((im, jm, kb), dtype=dt_, device=dev_)
     l[:, :, 1:kbm1] = torch.abs(q2lb[:, :, 1:kbm1]) / (q2b_abs[:, :, 1:kbm1] + 1e-20)
     surf = (z[:kbm1] > -0.5).view(1, 1, -1)
     lk = l[:, :, :kbm1].clone()
     l[:, :, :kbm1] = torch.where(surf, torch.max(lk, kappa * _b(l0)), lk)
-    gh = torch.zeros((im, jm, kb), dtype=dt_)
+    gh = torch.zeros((im, jm, kb), dtype=dt_, device=dev_)
     gh[:, :, 1:kbm1] = (l[:, :, 1:kbm1].clone() ** 2) * boygr[:, :, 1:kbm1] \
         / (q2b_abs[:, :, 1:kbm1] + 1e-20)
     gh = torch.clamp(gh, max=0.028)
@@ -557,7 +558,7 @@ def profq(S, q2f, q2lf, q2, q2b, q2lb, km, kh, kq, u, v, t, s, rho,
     gh[:, :, kb - 1] = 0.0
 
     # shear production (tracers.py:287-300)
-    kn = torch.zeros((im, jm, kb), dtype=dt_)
+    kn = torch.zeros((im, jm, kb), dtype=dt_, device=dev_)
     I, Ip = slice(1, im - 1), slice(2, im)
     J, Jp = slice(1, jm - 1), slice(2, jm)
     for k in range(1, kbm1):
@@ -568,8 +569,8 @@ def profq(S, q2f, q2lf, q2, q2b, q2lb, km, kh, kq, u, v, t, s, rho,
 
     # q2 Thomas solve (tracers.py:302-346)
     dtef = torch.sqrt(q2b_abs + 1e-8) * 1.0 / (B1 * l + small)
-    ee = [torch.zeros((im, jm), dtype=dt_)]
-    g0 = torch.zeros((im, jm), dtype=dt_)
+    ee = [torch.zeros((im, jm), dtype=dt_, device=dev_)]
+    g0 = torch.zeros((im, jm), dtype=dt_, device=dev_)
     g0[I0, J0] = (15.8 * CBCNST) ** (2.0 / 3.0) * utau2
     gg = [g0]
     for k in range(1, kbm1):
@@ -583,8 +584,8 @@ def profq(S, q2f, q2lf, q2, q2b, q2lb, km, kh, kq, u, v, t, s, rho,
     q2n = torch.stack(col, dim=2)
 
     # q2l Thomas solve with wall proximity (tracers.py:348-384)
-    ee = [torch.zeros((im, jm), dtype=dt_)]
-    gg = [torch.zeros((im, jm), dtype=dt_)]
+    ee = [torch.zeros((im, jm), dtype=dt_, device=dev_)]
+    gg = [torch.zeros((im, jm), dtype=dt_, device=dev_)]
     ones = torch.ones_like(dh)
     for k in range(1, kbm1):
         wall = (ones / (torch.abs(z[k] - z[0]) + 1e-20) + ones / (torch.abs(z[k] - z[kb - 1]) + 1e-20)) \
@@ -595,7 +596,7 @@ def profq(S, q2f, q2lf, q2, q2b, q2lb, km, kh, kq, u, v, t, s, rho,
         gg.append((dti2 * (-kn[:, :, k] * l[:, :, k] * E1) + c[:, :, k] * gg[k - 1] - q2lf[:, :, k])
                   / (den + 1e-20))
     col = [None] * kb
-    col[kb - 1] = torch.zeros((im, jm), dtype=dt_)
+    col[kb - 1] = torch.zeros((im, jm), dtype=dt_, device=dev_)
     for k in range(kbm1 - 1, -1, -1):
         col[k] = ee[k] * col[k + 1] + gg[k]
     q2ln = torch.stack(col, dim=2)
@@ -685,14 +686,15 @@ def proft(S, f_in, wfsurf, fsurf, nbc, kh, etf, swrad):
     dti2, dz, dzz, umol = S.dti2, S.dz, S.dzz, S.umol
     dh = S.h + etf
     dt_ = f_in.dtype
-    a = torch.zeros((im, jm, kb), dtype=dt_)
-    c = torch.zeros((im, jm, kb), dtype=dt_)
+    dev_ = f_in.device
+    a = torch.zeros((im, jm, kb), dtype=dt_, device=dev_)
+    c = torch.zeros((im, jm, kb), dtype=dt_, device=dev_)
     for k in range(1, kbm1):
         dh2 = dh ** 2
         num = -dti2 * (kh[:, :, k] + umol)
         a[:, :, k - 1] = num / (dz[k - 1] * dzz[k - 1] * dh2)
         c[:, :, k] = num / (dz[k] * dzz[k - 1] * dh2)
-    rad = torch.zeros((im, jm, kb), dtype=dt_)
+    rad = torch.zeros((im, jm, kb), dtype=dt_, device=dev_)
     i = int(S.ntp) - 1
     if nbc in (2, 4):
         for k in range(kbm1):
@@ -709,7 +711,7 @@ def proft(S, f_in, wfsurf, fsurf, nbc, kh, etf, swrad):
         ee[0] = a[:, :, 0] / den
         gg[0] = (dti2 * (wfsurf + rad[:, :, 0] - rad[:, :, 1]) / (dz[0] * dh) - f_in[:, :, 0]) / den
     else:
-        ee[0] = torch.zeros((im, jm), dtype=dt_)
+        ee[0] = torch.zeros((im, jm), dtype=dt_, device=dev_)
         gg[0] = fsurf
     for k in range(1, kbm2):
         den = a[:, :, k] + c[:, :, k] * (1.0 - ee[k - 1]) - 1.0
@@ -762,7 +764,7 @@ def dens(S, t, s, rho):
 def advu(S, advx, drhox, u, v, w, ub, dt, egf, egb, etf, etb, e_atmos):
     im, jm, kb, kbm1 = S.im, S.jm, S.kb, S.kbm1
     aru, dz, cor, dy, h, grav, dti2 = S.aru, S.dz, S.cor, S.dy, S.h, S.grav, S.dti2
-    vfl = torch.zeros((im, jm, kb), dtype=u.dtype)
+    vfl = torch.zeros((im, jm, kb), dtype=u.dtype, device=u.device)
     A, Am = slice(1, im), slice(0, im - 1)
     Kk, Km = slice(1, kbm1), slice(0, kbm1 - 1)
     vfl[A, :, Kk] = 0.25 * (w[A, :, Kk] + w[Am, :, Kk]) * (u[A, :, Kk] + u[A, :, Km])
@@ -780,7 +782,7 @@ def advu(S, advx, drhox, u, v, w, ub, dt, egf, egb, etf, etb, e_atmos):
     hs = h[I, J] + h[Im, J]
     dnew = _b((hs + etf[I, J] + etf[Im, J]) * aru[I, J])
     dold = _b((hs + etb[I, J] + etb[Im, J]) * aru[I, J])
-    uf = torch.zeros((im, jm, kb), dtype=u.dtype)
+    uf = torch.zeros((im, jm, kb), dtype=u.dtype, device=u.device)
     uf[I, J, K] = (dold * ub[I, J, K] - 2.0 * dti2 * rhs) / (dnew + EPS_DIV)
     return uf
 
@@ -788,7 +790,7 @@ def advu(S, advx, drhox, u, v, w, ub, dt, egf, egb, etf, etb, e_atmos):
 def advv(S, advy, drhoy, u, v, w, vb, dt, egf, egb, etf, etb, e_atmos):
     im, jm, kb, kbm1 = S.im, S.jm, S.kb, S.kbm1
     arv, dz, cor, dx, h, grav, dti2 = S.arv, S.dz, S.cor, S.dx, S.h, S.grav, S.dti2
-    vfl = torch.zeros((im, jm, kb), dtype=u.dtype)
+    vfl = torch.zeros((im, jm, kb), dtype=u.dtype, device=u.device)
     B, Bm = slice(1, jm), slice(0, jm - 1)
     Kk, Km = slice(1, kbm1), slice(0, kbm1 - 1)
     vfl[:, B, Kk] = 0.25 * (w[:, B, Kk] + w[:, Bm, Kk]) * (v[:, B, Kk] + v[:, B, Km])
@@ -806,7 +808,7 @@ def advv(S, advy, drhoy, u, v, w, vb, dt, egf, egb, etf, etb, e_atmos):
     hs = h[I, J] + h[I, Jm]
     dnew = _b((hs + etf[I, J] + etf[I, Jm]) * arv[I, J])
     dold = _b((hs + etb[I, J] + etb[I, Jm]) * arv[I, J])
-    vf = torch.zeros((im, jm, kb), dtype=u.dtype)
+    vf = torch.zeros((im, jm, kb), dtype=u.dtype, device=u.device)
     vf[I, J, K] = (dold * vb[I, J, K] - 2.0 * dti2 * rhs) / (dnew + EPS_DIV)
     return vf
 
@@ -825,8 +827,8 @@ def _profuv(S, f_in, wsurf, wbot, km, etf, cbc, ub, vb, mask, along_i):
     dhf = 0.5 * (h[A, B] + etf[A, B] + h[Am, Bm] + etf[Am, Bm])   # (im-1, jm-1)
     kmf = 0.5 * (km[A, B, :] + km[Am, Bm, :])
     dh2 = dhf ** 2
-    a = torch.zeros((im - 1, jm - 1, kb), dtype=f_in.dtype)
-    c = torch.zeros((im - 1, jm - 1, kb), dtype=f_in.dtype)
+    a = torch.zeros((im - 1, jm - 1, kb), dtype=f_in.dtype, device=f_in.device)
+    c = torch.zeros((im - 1, jm - 1, kb), dtype=f_in.dtype, device=f_in.device)
     for k in range(1, kbm1):
         num = -dti2 * (kmf[:, :, k] + umol)
         a[:, :, k - 1] = num / (dz[k - 1] * dzz[k - 1] * dh2)
@@ -854,7 +856,7 @@ def _profuv(S, f_in, wsurf, wbot, km, etf, cbc, ub, vb, mask, along_i):
         cbca = 0.5 * (cbc[I, J] + cbc[I, 0:jm - 2])
         us = ub[I, J, kb_] + ub[2:im, J, kb_] + ub[I, 0:jm - 2, kb_] + ub[2:im, 0:jm - 2, kb_]
         mag = torch.sqrt((0.25 * us) ** 2 + vb[I, J, kb_] ** 2 + 1e-8)
-    tps = torch.zeros((im, jm), dtype=f_in.dtype)
+    tps = torch.zeros((im, jm), dtype=f_in.dtype, device=f_in.device)
     tps[I, J] = cbca * mag
     sub = (slice(0, n0), slice(0, n1))
     fric = tps[I, J] * dti2 / (-dz[kb_] * dhf[sub])

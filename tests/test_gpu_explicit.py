@@ -43,17 +43,33 @@ def _run(states, dtype, nstep, internal_first):
     return ens, state, vamax
 
 
-def _compare(ens, state, refs, tol):
+def _compare(ens, state, refs, tol, floor=None):
+    """max rel error per field <= tol, or <= 5x the oracle's own error when
+    ``floor`` maps field -> error of the oracle evaluated in the same precision."""
     bad = []
     for n in L.FIELDS:
         if n in SKIP_CMP:
             continue
+        lim = tol if floor is None else max(tol, 5.0 * floor.get(n, 0.0))
         for m, got in enumerate(unpack(ens, state[n], n)):
             ref = refs[m](n)
             e = rel_err(got.double().cpu(), ref)
-            if not (e <= tol):
-                bad.append(f"{n}[m{m}]: {e:.2e}")
+            if not (e <= lim):
+                bad.append(f"{n}[m{m}]: {e:.2e} (limit {lim:.1e})")
     assert not bad, "fields off: " + ", ".join(bad)
+
+
+def _oracle_fp32_error(label, nstep):
+    """Error of the reference algorithm itself evaluated in float32 (oracle on
+    the CPU) against the float64 golden: FP32 cancellation in rho - rmean makes
+    drhox/drhoy inherently ~1e-3 relative, whatever the implementation."""
+    S, g = _state(label)
+    for k, v in list(vars(S).items()):
+        if torch.is_tensor(v):
+            setattr(S, k, v.float())
+    S.iend = nstep
+    X.run_time_loop(S)
+    return {n: rel_err(getattr(S, n).double(), torch.from_numpy(g[f"{label}/{n}"])) for n in L.FIELDS}
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
@@ -64,7 +80,8 @@ def test_step_matches_reference(dtype, label):
     print(f"[gpu] explicit step 12x10 label={label} nstep={nstep} dtype={dtype}", flush=True)
     ens, state, vamax = _run([S], dtype, nstep, internal_first=(S.time0 != 0.0))
     ref = lambda n: torch.from_numpy(g[f"{label}/{n}"])
-    _compare(ens, state, [ref], TOL[dtype] * (10 if label == "b" else 1))
+    floor = _oracle_fp32_error(label, nstep) if dtype == torch.float32 else None
+    _compare(ens, state, [ref], TOL[dtype] * (10 if label == "b" else 1), floor)
     assert float(vamax) <= S.vmaxl
 
 
