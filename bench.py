@@ -26,6 +26,13 @@ B_FWD = 164.0    # algorithmic bytes per cell-step, forward (fp32)
 B_FA = 468.0     # algorithmic bytes per cell-step, forward + adjoint with recompute (fp32)
 HBM_PEAK = 8.0e12
 
+# dominant kernel of the fwd+adjoint step (rocprof kernel-trace, profiles/) and
+# its compulsory HBM bytes per water-column cell: reads q2f q2lf q2 q2b q2lb km
+# kh kq u v t s rho (13 x 4 B), writes q2 q2l l kq km kh (6 x 4 B), plus 7 2-D
+# fields (wusurf wvsurf wubot wvbot etf dt h) shared by the 16 levels
+DOMINANT = "profq"
+DOMINANT_KERNEL = "pom::k_profq<float>"
+DOMINANT_B_PER_CELL = (13 + 6) * 4.0 + 7 * 4.0 / 16.0
 DTE = 6.0        # external step of the 8 km seamount grid (barotropic CFL); dti = 180 s
 SHAPES = [(90, 82)] * 32 + [(120, 110)] * 24 + [(177, 161)] * 8
 DIFF_OUT = ("u", "ub", "v", "vb", "t", "tb", "s", "sb", "el", "elb", "ua", "uab", "va", "vab",
@@ -69,14 +76,18 @@ def run_gpu(args):
         grads = ens.step_adj(st, geom, sc, r, vg, True, dict(cot))
         return out, grads, vamax
 
+    from neuralpom_amd import _lib as L
     for _ in range(args.warmup):
         state, _, vamax = one_step(state)
     torch.cuda.synchronize()
+    L.profile_stage(DOMINANT)     # HIP events around each launch of the dominant kernel
     t0 = time.perf_counter()
     for _ in range(args.steps):
         state, grads, vamax = one_step(state)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    kern_ms, kern_n = L.profile_read()
+    L.profile_stage(None)
     if not float(vamax) <= sc.vmaxl:
         raise FloatingPointError(f"barotropic velocity bound exceeded: vamax={float(vamax)}")
     for n in ("u", "t", "el"):
@@ -92,7 +103,7 @@ def run_gpu(args):
         st, vamax = ens.step(st, geom, sc, r, vg, True)
     torch.cuda.synchronize()
     dtf = time.perf_counter() - t1
-    return dict(cells=cells, seconds=dt, seconds_fwd=dtf, ens=ens)
+    return dict(cells=cells, seconds=dt, seconds_fwd=dtf, ens=ens, kern_ms=kern_ms, kern_n=kern_n)
 
 
 def autograd_baseline(args, n_members=3):
@@ -178,15 +189,25 @@ def main():
     ap.add_argument("--autograd-steps", type=int, default=1)
     ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--no-baselines", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="launch one 256 MiB device copy first (reference for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported in this build "
               "(members would shard across GPUs; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        b = a.clone()
+        torch.cuda.synchronize()
+        del a, b
     res = run_gpu(args)
     rate = res["cells"] * args.steps / res["seconds"]
     rate_fwd = res["cells"] * args.steps / res["seconds_fwd"]
     achieved = rate * B_FA
+    kern_avg_s = res["kern_ms"] / max(res["kern_n"], 1) / 1e3
+    kern_bytes = DOMINANT_B_PER_CELL * res["cells"]
+    kern_bw = kern_bytes / kern_avg_s
     line = {
         "metric": "fwd+adjoint grid-cell-steps/s vs autograd baseline, same GPU, <=1e-5 rel err",
         "value": rate,
@@ -204,6 +225,10 @@ def main():
                                "8x177x161) x 16 levels, explicit core, fwd+adjoint per step",
                    "cells_per_step": res["cells"], "dti_s": DTE * 30, "isplit": 30},
         "forward_only": {"value": rate_fwd, "unit": "grid-cell-steps/s"},
+        "roofline": {"bound": "hbm", "achieved": kern_bw / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": kern_bw / HBM_PEAK, "traffic": None, "kernel": DOMINANT_KERNEL,
+                     "avg_launch_us": kern_avg_s * 1e6, "launches": res["kern_n"],
+                     "algorithmic_bytes_per_launch": kern_bytes},
         "step_roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                           "unit": "GB/s", "frac": achieved / HBM_PEAK,
                           "bytes_per_cell_step": B_FA},

@@ -108,6 +108,21 @@ int pom_explicit_step_adj(const pom_batch* b, const pom_params_t* prm, const voi
                           const void* const* cot_out, void* const* grad_in, void* workspace, size_t ws_bytes,
                           void* stream);
 
+/* Tracing: record HIP events around every launch of one stage on its stream
+ * (stage = -1 disables).  pom_profile_read waits for the recorded events and
+ * returns the summed kernel time and the number of launches since the last
+ * read.  No counterpart in the reference (it times host wall clock only,
+ * src/neuralpom/v1/engine.py:341-362). */
+typedef enum {
+  POM_STAGE_PROFQ = 0,      /* k_profq: Mellor-Yamada closure, forward */
+  POM_STAGE_PROFQ_ADJ = 1,  /* k_profq_adj: its reverse sweep */
+  POM_STAGE_EXTUV = 2,      /* depth-mean momentum substep (one launch per substep) */
+  POM_STAGE_ADVCT = 3,      /* 3-D momentum advection */
+  POM_NSTAGE
+} pom_stage;
+int pom_profile_stage(int32_t stage);
+int pom_profile_read(double* total_ms, int32_t* count);
+
 #ifdef __cplusplus
 }
 #endif

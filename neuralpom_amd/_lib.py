@@ -70,15 +70,31 @@ def load():
                                       C.c_size_t, vp]
     lib.pom_explicit_step_adj.argtypes = [vp, C.POINTER(Params), vp, vp, vpp, vpp, C.c_int32, vpp, vpp,
                                           vp, C.c_size_t, vp]
-    for fn in ("pom_batch_create", "pom_batch_destroy", "pom_explicit_workspace", "pom_explicit_step",
-               "pom_explicit_step_adj"):
+    lib.pom_profile_stage.argtypes = [C.c_int32]
+    lib.pom_profile_read.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_int32)]
+    for fn in EXPORTED:
         getattr(lib, fn).restype = C.c_int
     _lib = lib
     return lib
 
 
 EXPORTED = ("pom_batch_create", "pom_batch_destroy", "pom_explicit_workspace", "pom_explicit_step",
-            "pom_explicit_step_adj")
+            "pom_explicit_step_adj", "pom_profile_stage", "pom_profile_read")
+STAGES = {"profq": 0, "profq_adj": 1, "extuv": 2, "advct": 3}
+
+
+def profile_stage(name):
+    """Start timing every launch of one stage (None stops)."""
+    lib = load()
+    check(lib.pom_profile_stage(-1 if name is None else STAGES[name]), "pom_profile_stage")
+
+
+def profile_read():
+    """(total kernel ms, launch count) of the selected stage since the last read."""
+    lib = load()
+    ms, n = C.c_double(), C.c_int32()
+    check(lib.pom_profile_read(C.byref(ms), C.byref(n)), "pom_profile_read")
+    return ms.value, n.value
 
 
 def check(rc: int, what: str):

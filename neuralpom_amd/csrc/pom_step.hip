@@ -30,6 +30,37 @@ namespace {
 
 constexpr int MAXSPLIT = 256;
 
+// ---------------------------------------------------------------- tracing
+// HIP events around every launch of one selected stage (pom_profile_stage);
+// pom_profile_read sums their durations.  Off by default: no events recorded.
+struct StageProf {
+  int stage = -1;
+  std::vector<hipEvent_t> pool;   // pairs (begin, end)
+  size_t used = 0;
+};
+StageProf g_prof;
+
+struct ProfScope {
+  hipEvent_t b = nullptr, e = nullptr;
+  hipStream_t s;
+  ProfScope(int stage, hipStream_t st) : s(st) {
+    if (g_prof.stage != stage) return;
+    if (g_prof.used + 2 > g_prof.pool.size()) {
+      hipEvent_t x, y;
+      if (hipEventCreate(&x) != hipSuccess || hipEventCreate(&y) != hipSuccess) return;
+      g_prof.pool.push_back(x);
+      g_prof.pool.push_back(y);
+    }
+    b = g_prof.pool[g_prof.used];
+    e = g_prof.pool[g_prof.used + 1];
+    g_prof.used += 2;
+    (void)hipEventRecord(b, s);
+  }
+  ~ProfScope() {
+    if (e) (void)hipEventRecord(e, s);
+  }
+};
+
 // ---------------------------------------------------------------- workspace
 struct Arena {
   char* base;
@@ -568,7 +599,10 @@ template <class T> struct Step {
       hipLaunchKernelGGL(k_forcing<T>, g2, tb, 0, c.stream, c.B, c.prm, a);
     }
     // 3-D preparation (time_loop.py:169-270)
-    launch_map<T, OpAdvct>(c, t_advct(), OutTab<T>{{out[POM_ADVX], out[POM_ADVY]}});
+    {
+      ProfScope ps(POM_STAGE_ADVCT, c.stream);
+      launch_map<T, OpAdvct>(c, t_advct(), OutTab<T>{{out[POM_ADVX], out[POM_ADVY]}});
+    }
     {
       BaropgArgs<T> a{in[POM_RHO], g[POM_G_RMEAN], in[POM_DT], g[POM_G_DX], g[POM_G_DY], g[POM_G_DUM],
                       g[POM_G_DVM], rho1, out[POM_DRHOX], out[POM_DRHOY]};
@@ -609,7 +643,10 @@ template <class T> struct Step {
         launch_map<T, OpAdvave>(cn, t_advave(xp), OutTab<T>{{xn.advua, xn.advva, xn.fluxua, xn.fluxva, xn.tps}});
       }
       resolve(n, xn);
-      launch_map<T, OpExtUV>(cn, t_uv(xp, xn), OutTab<T>{{xn.uaf, xn.vaf}});
+      {
+        ProfScope ps(POM_STAGE_EXTUV, c.stream);
+        launch_map<T, OpExtUV>(cn, t_uv(xp, xn), OutTab<T>{{xn.uaf, xn.vaf}});
+      }
       hipLaunchKernelGGL(k_vamax<T>, dim3(std::min(blocks(N2), 1024)), dim3(256), 0, c.stream, xn.vaf, N2, vamax);
       launch_map<T, OpExtFilter>(cn, t_filter(xp, xn),
                                  OutTab<T>{{xn.ua, xn.uab, xn.va, xn.vab, xn.el, xn.elb, xn.d, xn.egf, xn.utf, xn.vtf,
@@ -628,7 +665,10 @@ template <class T> struct Step {
       hipLaunchKernelGGL(k_vertvl<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a_vertvl());
       launch_map<T, OpAdvq>(c, t_advq(false), OutTab<T>{{q2f0}});
       launch_map<T, OpAdvq>(c, t_advq(true), OutTab<T>{{q2lf0}});
-      hipLaunchKernelGGL(k_profq<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a_profq());
+      {
+        ProfScope ps(POM_STAGE_PROFQ, c.stream);
+        hipLaunchKernelGGL(k_profq<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a_profq());
+      }
       launch_map<T, OpKEdge>(c, t_kedge(), OutTab<T>{{out[POM_KM], out[POM_KH]}});
       launch_map<T, OpBc6Filter>(c, t_bc6(), OutTab<T>{{out[POM_Q2], out[POM_Q2B], out[POM_Q2L], out[POM_Q2LB]}});
       launch_map<T, OpAdvt1>(c, t_advt1(false), OutTab<T>{{tf0}});
@@ -954,7 +994,10 @@ template <class T> struct Adj {
       a.b_t = bi[POM_T]; a.b_s = bi[POM_S]; a.b_rho = b_rho1; a.b_wubot = bi[POM_WUBOT];
       a.b_wvbot = bi[POM_WVBOT]; a.b_etf = b_etf30; a.b_dt = bi[POM_DT];
       a.s_unb = s3a; a.s_vnb = s3b; a.s_wubnb = s2a; a.s_wvbnb = s2b;
-      hipLaunchKernelGGL(k_profq_adj<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a);
+      {
+        ProfScope ps(POM_STAGE_PROFQ_ADJ, c.stream);
+        hipLaunchKernelGGL(k_profq_adj<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a);
+      }
       hipLaunchKernelGGL(k_profq_adj2<T>, g2, tb, 0, c.stream, c.B, a);
     }
     for (int l2 = 0; l2 < 2; ++l2) {   // advq
@@ -1112,6 +1155,28 @@ int pom_batch_create(const pom_member_t* members, int32_t n, int32_t dtype, pom_
       hipMemcpy(b->d_blk2, b2.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice) != hipSuccess)
     return POM_E_HIP;
   *out = b;
+  return POM_OK;
+}
+
+int pom_profile_stage(int32_t stage) {
+  if (stage < -1 || stage >= POM_NSTAGE) return POM_E_ARG;
+  g_prof.stage = stage;
+  g_prof.used = 0;
+  return POM_OK;
+}
+
+int pom_profile_read(double* total_ms, int32_t* count) {
+  if (!total_ms || !count) return POM_E_ARG;
+  double t = 0.0;
+  for (size_t i = 0; i + 1 < g_prof.used; i += 2) {
+    if (hipEventSynchronize(g_prof.pool[i + 1]) != hipSuccess) return POM_E_HIP;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, g_prof.pool[i], g_prof.pool[i + 1]) != hipSuccess) return POM_E_HIP;
+    t += ms;
+  }
+  *total_ms = t;
+  *count = (int32_t)(g_prof.used / 2);
+  g_prof.used = 0;
   return POM_OK;
 }
 

@@ -39,6 +39,8 @@ OUT = Path(__file__).resolve().parent
 
 
 def _install_stubs() -> None:
+    # the reference tree is read-only: importing it must leave no bytecode there
+    sys.dont_write_bytecode = True
     sys.modules.setdefault("netCDF4", types.ModuleType("netCDF4"))
     if str(REF_SRC) not in sys.path:
         sys.path.insert(0, str(REF_SRC))
