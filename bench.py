@@ -33,6 +33,10 @@ HBM_PEAK = 8.0e12
 DOMINANT = "profq"
 DOMINANT_KERNEL = "pom::k_profq<float>"
 DOMINANT_B_PER_CELL = (13 + 6) * 4.0 + 7 * 4.0 / 16.0
+# HBM bytes per k_profq launch on this workload from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (KiB per dispatch), each scaled by what the counter reads for a 256 MiB
+# read + write elementwise kernel in the same run (profiles/k_profq_seamount64.md)
+DOMINANT_TRAFFIC = (436837.6 * 2.000 + 329715.4 * 1.000) * 1024.0
 DTE = 6.0        # external step of the 8 km seamount grid (barotropic CFL); dti = 180 s
 SHAPES = [(90, 82)] * 32 + [(120, 110)] * 24 + [(177, 161)] * 8
 DIFF_OUT = ("u", "ub", "v", "vb", "t", "tb", "s", "sb", "el", "elb", "ua", "uab", "va", "vab",
@@ -197,8 +201,9 @@ def main():
               "(members would shard across GPUs; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
     if args.calibrate_copy:
+        # 256 MiB fill (writes only), then a 256 MiB read + 256 MiB write elementwise kernel
         a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
-        b = a.clone()
+        b = a * 2.0
         torch.cuda.synchronize()
         del a, b
     res = run_gpu(args)
@@ -226,7 +231,7 @@ def main():
                    "cells_per_step": res["cells"], "dti_s": DTE * 30, "isplit": 30},
         "forward_only": {"value": rate_fwd, "unit": "grid-cell-steps/s"},
         "roofline": {"bound": "hbm", "achieved": kern_bw / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": kern_bw / HBM_PEAK, "traffic": None, "kernel": DOMINANT_KERNEL,
+                     "frac": kern_bw / HBM_PEAK, "traffic": DOMINANT_TRAFFIC, "kernel": DOMINANT_KERNEL,
                      "avg_launch_us": kern_avg_s * 1e6, "launches": res["kern_n"],
                      "algorithmic_bytes_per_launch": kern_bytes},
         "step_roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,

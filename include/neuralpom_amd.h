@@ -108,6 +108,31 @@ int pom_explicit_step_adj(const pom_batch* b, const pom_params_t* prm, const voi
                           const void* const* cot_out, void* const* grad_in, void* workspace, size_t ws_bytes,
                           void* stream);
 
+/* Operator-level entry points: the reference's operator methods that callers
+ * invoke outside the time loop (src/neuralpom/hybrid/sparse_coupling.py:10-15
+ * calls dens through its eos(), :69-82 calls baropg).  Same kernels as the step.
+ *
+ * pom_dens: equation of state (src/neuralpom/cores/explicit/momentum.py:467-541)
+ *   rho_out(k) = EOS(t, s, depth) * fsm for k < 15; level 15 copies rho_in.
+ * pom_dens_adj: b_t += dEOS/dt^T b_rho_out, b_s += ..., b_rho_in += level-15
+ *   part (null outputs are skipped). */
+int pom_dens(const pom_batch* b, const pom_params_t* prm, const void* vgrid, const void* t, const void* s,
+             const void* rho_in, const void* h, const void* fsm, void* rho_out, void* stream);
+int pom_dens_adj(const pom_batch* b, const pom_params_t* prm, const void* vgrid, const void* t, const void* s,
+                 const void* rho_in, const void* h, const void* fsm, const void* b_rho_out, void* b_t, void* b_s,
+                 void* b_rho_in, void* stream);
+
+/* pom_baropg: sigma-coordinate baroclinic pressure gradient
+ * (src/neuralpom/cores/explicit/momentum.py:543-700).
+ *   in  = {rho, rmean, dt, dx, dy, dum, dvm}  (rmean 3-D, the rest of dt..dvm 2-D)
+ *   out = {rho_out, drhox, drhoy}
+ * pom_baropg_adj: cot_out = {b_rho_out, b_drhox, b_drhoy} (entries may be
+ *   null), grad_in = {b_rho, b_dt} accumulated (entries may be null). */
+int pom_baropg(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vgrid,
+               const void* const* in, void* const* out, void* stream);
+int pom_baropg_adj(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vgrid,
+                   const void* const* in, const void* const* cot_out, void* const* grad_in, void* stream);
+
 /* Tracing: record HIP events around every launch of one stage on its stream
  * (stage = -1 disables).  pom_profile_read waits for the recorded events and
  * returns the summed kernel time and the number of launches since the last

@@ -70,6 +70,10 @@ def load():
                                       C.c_size_t, vp]
     lib.pom_explicit_step_adj.argtypes = [vp, C.POINTER(Params), vp, vp, vpp, vpp, C.c_int32, vpp, vpp,
                                           vp, C.c_size_t, vp]
+    lib.pom_dens.argtypes = [vp, C.POINTER(Params), vp, vp, vp, vp, vp, vp, vp, vp]
+    lib.pom_dens_adj.argtypes = [vp, C.POINTER(Params), vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    lib.pom_baropg.argtypes = [vp, C.POINTER(Params), vp, vp, vpp, vpp, vp]
+    lib.pom_baropg_adj.argtypes = [vp, C.POINTER(Params), vp, vp, vpp, vpp, vpp, vp]
     lib.pom_profile_stage.argtypes = [C.c_int32]
     lib.pom_profile_read.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_int32)]
     for fn in EXPORTED:
@@ -79,7 +83,8 @@ def load():
 
 
 EXPORTED = ("pom_batch_create", "pom_batch_destroy", "pom_explicit_workspace", "pom_explicit_step",
-            "pom_explicit_step_adj", "pom_profile_stage", "pom_profile_read")
+            "pom_explicit_step_adj", "pom_dens", "pom_dens_adj", "pom_baropg", "pom_baropg_adj",
+            "pom_profile_stage", "pom_profile_read")
 STAGES = {"profq": 0, "profq_adj": 1, "extuv": 2, "advct": 3}
 
 

@@ -1,0 +1,296 @@
+"""Drop-in explicit POM core: the reference's dynamical-core API on the HIP step.
+
+Mirrors the reference's explicit ``POM2K_Core``:
+  src/neuralpom/cores/explicit/core.py:17-61     class, settings overrides, derived parameters
+  src/neuralpom/cores/explicit/base.py:8-189     attribute names and class defaults
+  src/neuralpom/cores/explicit/time_loop.py:20-709  run_time_loop (internal-step loop, ramp,
+                                                    early return when vamax > vmaxl)
+  src/neuralpom/cores/explicit/momentum.py:467-541, 543-700  dens(), baropg()
+and the construction protocol of src/neuralpom/hybrid/core_pool.py:30-48
+(instantiate, set ``device``, ``configure_core_runtime``, ``run_initialization``).
+
+Every state field is a plain torch tensor attribute, (im, jm, 16) or (im, jm),
+on ``self.device``.  ``run_time_loop`` rebinds the attributes to the outputs of
+the HIP step (one ``pom_explicit_step`` launch sequence per internal step), so
+``vars(core)``-based capture / restore and ``torch.utils.checkpoint`` over
+cloned attributes keep working.  When autograd is recording and any state
+tensor requires grad, each internal step is one ``ExplicitStep`` node whose
+backward recomputes the step and runs the hand-written adjoint kernels; the
+node keeps no reference to the core.  There is no CPU path: without the HIP
+library or a GPU every entry point raises.
+"""
+from __future__ import annotations
+
+import math
+import os
+from typing import Any
+
+import torch
+
+from . import _lib as L
+from . import cases
+from .explicit import Ensemble, StepScalars, step_autograd
+
+KB = 16
+_STEP_SCALARS = ("dte", "isplit", "alpha", "smoth", "horcon", "grav", "rhoref", "kappa", "umol", "tprni",
+                 "small", "tbias", "sbias", "hmax", "rfe", "rfw", "rfn", "rfs", "ispadv", "ntp", "nbct", "nbcs",
+                 "vmaxl")
+
+
+class POM2K_Core:
+    """Explicit mode-split POM core (double-gyre problem 4) running on the MI355X kernels."""
+
+    NUMERICS_VERSION = "explicit-dg-v2-split-smag-v2align-closure"
+
+    def __init__(self, im: int, jm: int, problem_id: int = 4, settings_overrides: dict[str, Any] | None = None,
+                 case_name: str | None = None, device=None, dtype=torch.float32):
+        self._requested_problem_id = problem_id
+        self._settings_overrides = dict(settings_overrides or {})
+        self.case_name = case_name
+        if device is None:
+            device = torch.device("cuda", torch.cuda.current_device()) if torch.cuda.is_available() else "cpu"
+        self.device = torch.device(device)
+        self.dtype = dtype
+        self.iproblem = problem_id
+        if self.iproblem != 4:
+            raise ValueError(f"Only the Double-gyre problem_id=4 is supported; got {self.iproblem}.")
+        self.im, self.jm, self.kb = int(im), int(jm), KB
+        self._refresh_derived_indices()
+        # scalars and defaults (base.py:38-117)
+        self.small = 1.e-9
+        self.pi = math.atan(1.0) * 4.0
+        self.mode = 3
+        self.nadv = 1
+        self.nitera = 2
+        self.sw = 0.50
+        self.nread = 0
+        self.lramp = True
+        self.rhoref = 1035.0
+        self.hmax = 2000.0
+        self.dte = 30.0
+        self.isplit = 30
+        self.days = 3650.0
+        self.prtd1 = 1.0
+        self.prtd2 = 1.0
+        self.grav = 9.806
+        self.kappa = 0.40
+        self.z0b = 0.010
+        self.cbcmin = 0.00250
+        self.cbcmax = 1.0
+        self.horcon = 0.20
+        self.tprni = 0.20
+        self.umol = 2.e-5
+        self.vmaxl = 100.0
+        self.slmax = 2.0
+        self.kl1 = 2
+        self.kl2 = 2
+        self.swtch = 99999.0
+        self.iskp, self.jskp = 4, 3
+        self.ntp = 2
+        self.nbct = 1
+        self.nbcs = 1
+        self.ispadv = 5
+        self.smoth = 0.100
+        self.alpha = 0.2250
+        self.aam_init = 500.0
+        self.ramp = 0.0
+        self.tatm = 0.0
+        self.satm = 0.0
+        self.tbias = 0.0
+        self.sbias = 0.0
+        self.rfe = self.rfw = self.rfn = self.rfs = 0.0
+        self.time = 0.0
+        self.time0 = 0.0
+        self.iint = 0
+        self.period = 1.0
+        self._refresh_derived_time_params()
+        # fields, zero until run_initialization (base.py:119-189)
+        st, g = cases._empty(self.im, self.jm, dtype, self.device)
+        for n, v in {**st, **g}.items():
+            setattr(self, n, v)
+        for n in ("z", "zz", "dz", "dzz"):
+            setattr(self, n, torch.zeros(KB, dtype=dtype, device=self.device))
+        self._apply_settings_overrides()
+
+    # ------------------------------------------------------------ construction (core.py:28-61)
+    @classmethod
+    def from_settings(cls, settings) -> "POM2K_Core":
+        return cls(im=settings.im, jm=settings.jm, problem_id=settings.problem_id,
+                   settings_overrides=settings.overrides, case_name=settings.name)
+
+    def _apply_settings_overrides(self) -> None:
+        if not self._settings_overrides:
+            return
+        for key, value in self._settings_overrides.items():
+            setattr(self, key, value)
+        self._refresh_derived_indices()
+        self._refresh_derived_time_params()
+
+    def _refresh_derived_indices(self) -> None:
+        self.imm1, self.imm2 = self.im - 1, self.im - 2
+        self.jmm1, self.jmm2 = self.jm - 1, self.jm - 2
+        self.kbm1, self.kbm2 = self.kb - 1, self.kb - 2
+
+    def _refresh_derived_time_params(self) -> None:
+        self.dti = self.dte * self.isplit
+        self.dte2 = self.dte * 2.0
+        self.dti2 = self.dti * 2.0
+        self.iend = max(int((self.days * 24.0 * 3600.0 / self.dti) + 0.5), 2)
+        self.iprint = int((self.prtd1 * 24.0 * 3600.0 / self.dti) + 0.5)
+        self.iswtch = int((self.swtch * 24.0 * 3600.0 / self.dti) + 0.5)
+        self.ispi = 1.0 / self.isplit
+        self.isp2i = 1.0 / (2.0 * self.isplit)
+
+    def configure_runtime(self, dte: float = 20.0, isplit: int = 30) -> None:
+        """The production clock (src/neuralpom/hybrid/core_pool.py:19-26)."""
+        self.dte = dte
+        self.isplit = isplit
+        self.dti = self.dte * self.isplit
+        self.dte2 = self.dte * 2.0
+        self.dti2 = self.dti * 2.0
+        self.ispi = 1.0 / self.isplit
+        self.isp2i = 1.0 / (2.0 * self.isplit)
+
+    def run_initialization(self) -> None:
+        """Double-gyre basin and initial state (cases.py:30-164, initialization.py:10-99)."""
+        st, g, vg, period = cases.double_gyre(self.im, self.jm, dtype=self.dtype, device=self.device)
+        for n, v in {**st, **g}.items():
+            setattr(self, n, v)
+        self.z, self.zz, self.dz, self.dzz = (vg[i].clone() for i in range(4))
+        self.period = period
+        self.horcon = float(os.environ.get("NEURALPOM_HORCON", "0.05"))   # cases.py:152
+
+    # ------------------------------------------------------------ kernels
+    def _ensemble(self) -> Ensemble:
+        key = (self.im, self.jm, self.dtype, str(self.device))
+        ens = getattr(self, "_ens_cache", None)
+        if ens is None or ens[0] != key:
+            ens = (key, Ensemble([(self.im, self.jm)], dtype=self.dtype, device=self.device))
+            object.__setattr__(self, "_ens_cache", ens)
+        return ens[1]
+
+    def step_scalars(self) -> StepScalars:
+        if self.nadv != 1:
+            raise NotImplementedError("only the centred advection scheme (nadv = 1) is implemented")
+        sc = StepScalars(**{n: getattr(self, n) for n in _STEP_SCALARS})
+        if abs(sc.dti - self.dti) > 1e-9 * abs(self.dti):
+            raise ValueError(f"dti {self.dti} != dte * isplit {sc.dti}")
+        return sc
+
+    def _vgrid(self):
+        return torch.stack([self.z, self.zz, self.dz, self.dzz]).to(self.device, self.dtype).reshape(1, 4, KB) \
+            .detach().contiguous()
+
+    def _flat(self, n):
+        v = getattr(self, n)
+        if v.dtype != self.dtype or v.device != self.device:
+            v = v.to(self.device, self.dtype)
+        return v.contiguous().reshape(-1)
+
+    def run_time_loop(self):
+        """Advance ``iend`` internal steps from ``time0`` (time_loop.py:20-709).
+
+        Sets ``iint``, ``time`` and ``ramp`` as the reference does.  When the
+        depth-mean velocity bound ``vmaxl`` is exceeded it prints
+        "Abnormal end" and returns before ``iend`` (time_loop.py:702-709);
+        ``runtime_vamax_safe_fast_path`` skips that per-step check.
+        """
+        ens = self._ensemble()
+        sc = self.step_scalars()
+        vg = self._vgrid()
+        geom = {n: self._flat(n).detach() for n in L.GEOM}
+        check = not getattr(self, "runtime_vamax_safe_fast_path", False)
+        shapes = {n: tuple(getattr(self, n).shape) for n in L.FIELDS}
+        for iint in range(1, int(self.iend) + 1):
+            self.iint = iint
+            self.time = self.dti * iint * 1.0 / 86400.0 + self.time0
+            if self.lramp:
+                self.ramp = min(self.time / float(self.period), 1.0)
+            else:
+                self.ramp = 1.0
+            ramp = torch.tensor([self.ramp], dtype=self.dtype, device=self.device)
+            internal = not (iint == 1 and self.time0 == 0.0)
+            state = {n: self._flat(n) for n in L.FIELDS}
+            if torch.is_grad_enabled() and any(v.requires_grad for v in state.values()):
+                out, vamax = step_autograd(ens, state, geom, sc, ramp, vg, internal)
+            else:
+                with torch.no_grad():
+                    out, vamax = ens.step({n: v.detach() for n, v in state.items()}, geom, sc, ramp, vg, internal)
+            for n in L.FIELDS:
+                setattr(self, n, out[n].view(shapes[n]))
+            if check and not float(vamax) <= self.vmaxl:
+                print("Abnormal end")
+                return
+
+    # ------------------------------------------------------------ operator overrides
+    def dens(self):
+        """Equation of state into ``self.rho`` (momentum.py:467-541); returns (s, t, rho)."""
+        rho = _Dens.apply(self._ensemble(), self._params(), self._vgrid(), self._flat("t"), self._flat("s"),
+                          self._flat("rho"), self._flat("h").detach(), self._flat("fsm").detach())
+        self.rho = rho.view(self.im, self.jm, KB)
+        return self.s, self.t, self.rho
+
+    def baropg(self):
+        """Baroclinic pressure gradient (momentum.py:543-700); returns (rho, drhox, drhoy)."""
+        ramp = torch.tensor([float(self.ramp)], dtype=self.dtype, device=self.device)
+        rho, drhox, drhoy = _Baropg.apply(self._ensemble(), self._params(), ramp, self._vgrid(),
+                                          self._flat("rho"), self._flat("rmean").detach(), self._flat("dt"),
+                                          *(self._flat(n).detach() for n in ("dx", "dy", "dum", "dvm")))
+        shape = (self.im, self.jm, KB)
+        self.rho = rho.view(shape)
+        return self.rho, drhox.view(shape), drhoy.view(shape)
+
+    def _params(self):
+        return self.step_scalars().struct()
+
+
+def _ptrs(ts):
+    return L.ptr_array([t.data_ptr() if t is not None else None for t in ts])
+
+
+class _Dens(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, ens, prm, vg, t, s, rho, h, fsm):
+        out = torch.empty_like(rho)
+        stream = torch.cuda.current_stream(ens.device).cuda_stream
+        L.check(ens.lib.pom_dens(ens._h, prm, vg.data_ptr(), t.data_ptr(), s.data_ptr(), rho.data_ptr(),
+                                 h.data_ptr(), fsm.data_ptr(), out.data_ptr(), stream), "pom_dens")
+        ctx.ens, ctx.prm = ens, prm
+        ctx.save_for_backward(vg, t, s, rho, h, fsm)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        vg, t, s, rho, h, fsm = ctx.saved_tensors
+        ens = ctx.ens
+        g = g.contiguous()
+        bt, bs, br = torch.zeros_like(t), torch.zeros_like(s), torch.zeros_like(rho)
+        stream = torch.cuda.current_stream(ens.device).cuda_stream
+        L.check(ens.lib.pom_dens_adj(ens._h, ctx.prm, vg.data_ptr(), t.data_ptr(), s.data_ptr(), rho.data_ptr(),
+                                     h.data_ptr(), fsm.data_ptr(), g.data_ptr(), bt.data_ptr(), bs.data_ptr(),
+                                     br.data_ptr(), stream), "pom_dens_adj")
+        return None, None, None, bt, bs, br, None, None
+
+
+class _Baropg(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, ens, prm, ramp, vg, rho, rmean, dt, dx, dy, dum, dvm):
+        outs = [torch.empty_like(rho) for _ in range(3)]
+        ins = (rho, rmean, dt, dx, dy, dum, dvm)
+        stream = torch.cuda.current_stream(ens.device).cuda_stream
+        L.check(ens.lib.pom_baropg(ens._h, prm, ramp.data_ptr(), vg.data_ptr(), _ptrs(ins), _ptrs(outs), stream),
+                "pom_baropg")
+        ctx.ens, ctx.prm = ens, prm
+        ctx.save_for_backward(ramp, vg, *ins)
+        return tuple(outs)
+
+    @staticmethod
+    def backward(ctx, g_rho, g_x, g_y):
+        ramp, vg, *ins = ctx.saved_tensors
+        ens = ctx.ens
+        cot = [None if g is None else g.contiguous() for g in (g_rho, g_x, g_y)]
+        b_rho, b_dt = torch.zeros_like(ins[0]), torch.zeros_like(ins[2])
+        stream = torch.cuda.current_stream(ens.device).cuda_stream
+        L.check(ens.lib.pom_baropg_adj(ens._h, ctx.prm, ramp.data_ptr(), vg.data_ptr(), _ptrs(ins), _ptrs(cot),
+                                       _ptrs((b_rho, b_dt)), stream), "pom_baropg_adj")
+        return None, None, None, None, b_rho, None, b_dt, None, None, None, None

@@ -28,6 +28,117 @@ template <class T> __device__ __forceinline__ void add_col(T* base, const T* x) 
   for (int k = 0; k < KB; ++k) base[k] += x[k];
 }
 
+// ---------------------------------------------------------------------
+// Coalesced column I/O.  In a 2-D launch the TPB threads of a block own TPB
+// consecutive columns (i*jm + j) of one member, and a 3-D field stores those
+// columns back to back, k fastest.  Moving a thread's column through an LDS
+// tile replaces 16 accesses at a 64 B lane stride (each reaching HBM as its
+// own partial-line request) by 16 unit-stride wavefront accesses.
+// Every thread of the block calls these (they contain block barriers):
+// col_pos keeps a thread past the member's last column on that last column
+// with valid = false, instead of returning.
+// ---------------------------------------------------------------------
+constexpr int TILE = TPB * (KB + 1);   // LDS elements of one column tile (padded rows)
+
+template <class T> struct ColIO {
+  T* lds;          // TILE elements of __shared__ memory
+  int64_t base;    // element offset (3-D) of the block's first column
+  int64_t first;   // member-local index of the block's first column
+  int64_t ncol;    // columns of the member (im * jm)
+  int n;           // columns of this block inside the member
+  bool valid;      // this thread owns a column of its own
+};
+
+template <class T>
+__device__ __forceinline__ ColIO<T> col_pos(const Batch& B, Pos& p, T* lds) {
+  const int b = blockIdx.x;
+  const int m = member_of(B.blk2, B.M, b);
+  const Member mm = B.mem[m];
+  const int64_t n2 = (int64_t)mm.im * mm.jm;
+  const int64_t first = (int64_t)(b - B.blk2[m]) * TPB;   // < n2 by construction of blk2
+  int64_t local = first + threadIdx.x;
+  ColIO<T> io;
+  io.lds = lds;
+  io.valid = local < n2;
+  if (!io.valid) local = n2 - 1;
+  io.n = (int)(n2 - first < TPB ? n2 - first : TPB);
+  io.base = mm.off3 + first * KB;
+  io.first = first;
+  io.ncol = n2;
+  p.m = m; p.im = mm.im; p.jm = mm.jm; p.o2 = mm.off2; p.o3 = mm.off3;
+  p.k = 0;
+  p.i = (int)(local / mm.jm);
+  p.j = (int)(local - (int64_t)p.i * mm.jm);
+  return io;
+}
+
+// padded LDS slot of tile element e (column e / KB, level e % KB)
+__device__ __forceinline__ int tile_ix(int e) { return e + e / KB; }
+
+template <class T> __device__ __forceinline__ void col_ld(const ColIO<T>& io, const T* __restrict__ f, T* x) {
+  const T* g = f + io.base;
+  const int tot = io.n * KB;
+  #pragma unroll
+  for (int r = 0; r < KB; ++r) {
+    const int e = r * TPB + (int)threadIdx.x;
+    if (e < tot) io.lds[tile_ix(e)] = g[e];
+  }
+  __syncthreads();
+  #pragma unroll
+  for (int k = 0; k < KB; ++k) x[k] = io.lds[threadIdx.x * (KB + 1) + k];
+  __syncthreads();
+}
+
+// column of the thread's neighbour `shift` columns further on (shift = 1:
+// (i, j+1); shift = jm: (i+1, j)); neighbours past the member's end are not
+// read (their slots hold stale data, which callers must not use)
+template <class T>
+__device__ __forceinline__ void col_ld_shift(const ColIO<T>& io, const T* __restrict__ f, int64_t shift, T* x) {
+  const int64_t avail = io.ncol - io.first - shift;
+  const int tot = (int)(avail < io.n ? (avail > 0 ? avail : 0) : io.n) * KB;
+  const T* g = f + io.base + shift * KB;
+  #pragma unroll
+  for (int r = 0; r < KB; ++r) {
+    const int e = r * TPB + (int)threadIdx.x;
+    if (e < tot) io.lds[tile_ix(e)] = g[e];
+  }
+  __syncthreads();
+  #pragma unroll
+  for (int k = 0; k < KB; ++k) x[k] = io.lds[threadIdx.x * (KB + 1) + k];
+  __syncthreads();
+}
+
+template <class T> __device__ __forceinline__ void col_put(const ColIO<T>& io, const T* x) {
+  #pragma unroll
+  for (int k = 0; k < KB; ++k) io.lds[threadIdx.x * (KB + 1) + k] = x[k];
+  __syncthreads();
+}
+
+template <class T> __device__ __forceinline__ void col_st(const ColIO<T>& io, T* __restrict__ f, const T* x) {
+  col_put(io, x);
+  T* g = f + io.base;
+  const int tot = io.n * KB;
+  #pragma unroll
+  for (int r = 0; r < KB; ++r) {
+    const int e = r * TPB + (int)threadIdx.x;
+    if (e < tot) g[e] = io.lds[tile_ix(e)];
+  }
+  __syncthreads();
+}
+
+// f += x (read-modify-write, both sides unit-stride)
+template <class T> __device__ __forceinline__ void col_add(const ColIO<T>& io, T* __restrict__ f, const T* x) {
+  col_put(io, x);
+  T* g = f + io.base;
+  const int tot = io.n * KB;
+  #pragma unroll
+  for (int r = 0; r < KB; ++r) {
+    const int e = r * TPB + (int)threadIdx.x;
+    if (e < tot) g[e] += io.lds[tile_ix(e)];
+  }
+  __syncthreads();
+}
+
 // =====================================================================
 // baropg
 // =====================================================================

@@ -36,8 +36,194 @@ template <class T> struct ProfqCol {
   bool has_s, inner;
 };
 
+// Forward sweep of one column.  Own-column 3-D inputs are loaded through the
+// block's LDS tile (every thread of the block executes this function).
 template <class T>
-__device__ __forceinline__ void profq_fwd_col(const Pos& p, const Vert<T>& vg, const Params& prm,
+__device__ __forceinline__ void profq_fwd_col(const Pos& p, const ColIO<T>& io, const Vert<T>& vg, const Params& prm,
+                                              const ProfqArgs<T>& s, ProfqCol<T>& C) {
+  using namespace myc;
+  const int i = p.i, j = p.j, m = p.m;
+  const int64_t c2 = ix2(p, i, j);
+  const T dti2 = T(prm.dti2), grav = T(prm.grav), kappa = T(prm.kappa), small = T(prm.small);
+  C.dh = s.h[c2] + s.etf[c2];
+  const T dh = C.dh;
+  T x[KB], y[KB];
+  col_ld(io, s.kq, x);
+  #pragma unroll
+  for (int k = 0; k < KB; ++k) C.a[k] = C.c[k] = T(0);
+  #pragma unroll
+  for (int k = 1; k < KBM1; ++k) {
+    C.a[k] = -dti2 * (x[k + 1] + x[k] + T(2.0 * prm.umol)) * T(0.5) /
+             (vg.dzz(m, k - 1) * vg.dz(m, k) * dh * dh);
+    C.c[k] = -dti2 * (x[k - 1] + x[k] + T(2.0 * prm.umol)) * T(0.5) /
+             (vg.dzz(m, k - 1) * vg.dz(m, k - 1) * dh * dh);
+  }
+  C.has_s = (i < p.im - 1) && (j < p.jm - 1);
+  C.inner = i >= 1 && i <= p.im - 2 && j >= 1 && j <= p.jm - 2;
+  C.utau2 = T(0);
+  C.l0 = T(0);
+  C.q2bot = T(0);
+  C.ubt = C.vbt = C.qbm = T(0);
+  if (C.has_s) {
+    const T ust = T(0.5) * (s.wusurf[c2] + s.wusurf[ix2(p, i + 1, j)]);
+    const T vst = T(0.5) * (s.wvsurf[c2] + s.wvsurf[ix2(p, i, j + 1)]);
+    C.utau2 = sqrt(ust * ust + vst * vst + T(1e-8));
+    C.l0 = T(SURFL) * C.utau2 / grav;
+    C.ubt = T(0.5) * (s.wubot[c2] + s.wubot[ix2(p, i + 1, j)]);
+    C.vbt = T(0.5) * (s.wvbot[c2] + s.wvbot[ix2(p, i, j + 1)]);
+    C.qbm = sqrt(C.ubt * C.ubt + C.vbt * C.vbt + T(1e-8));
+    C.q2bot = C.qbm * T(pow(16.6, 2.0 / 3.0) * SEF);
+  }
+  // sound speed (tracers.py:245-255)
+  col_ld(io, s.t, x);
+  col_ld(io, s.s, y);
+  #pragma unroll
+  for (int k = 0; k < KB; ++k) C.cc[k] = T(0);
+  #pragma unroll
+  for (int k = 0; k < KBM1; ++k) {
+    const T tp = x[k] + T(prm.tbias), sp = y[k] + T(prm.sbias);
+    const T pp = T(prm.grav * prm.rhoref) * (-vg.zz(m, k) * dh) * T(1.e-4);
+    T cc0 = T(1449.1) + T(0.00821) * pp + T(4.55) * tp - T(0.045) * (tp * tp) + T(1.34) * (sp - T(35.0));
+    const T ccs = fabs(cc0) < T(1e-6) ? T(1e-6) : cc0;
+    const T rad = (T(1) - T(0.01642) * pp / ccs) * (T(1) - T(0.40) * pp / (ccs * ccs));
+    C.cc[k] = ccs / sqrt(fmax(rad, T(1e-8)));
+  }
+  col_ld(io, s.rho, x);
+  #pragma unroll
+  for (int k = 0; k < KB; ++k) C.boygr[k] = T(0);
+  #pragma unroll
+  for (int k = 1; k < KBM1; ++k)
+    C.boygr[k] = grav * (x[k - 1] - x[k]) / (vg.dzz(m, k - 1) * dh) +
+                 T(prm.grav * prm.grav * 2.0) / (C.cc[k - 1] * C.cc[k - 1] + C.cc[k] * C.cc[k]);
+  // length scale (tracers.py:263-285); x = q2b, y = q2lb
+  col_ld(io, s.q2b, x);
+  col_ld(io, s.q2lb, y);
+  #pragma unroll
+  for (int k = 0; k < KB; ++k) C.lraw[k] = C.ghr[k] = C.gh[k] = T(0);
+  #pragma unroll
+  for (int k = 1; k < KBM1; ++k) C.lraw[k] = fabs(y[k]) / (fabs(x[k]) + T(1e-20));
+  const T kl0 = kappa * C.l0;
+  #pragma unroll
+  for (int k = 0; k < KBM1; ++k) C.l[k] = (vg.z(m, k) > T(-0.5)) ? (C.lraw[k] >= kl0 ? C.lraw[k] : kl0) : C.lraw[k];
+  #pragma unroll
+  for (int k = 1; k < KBM1; ++k) {
+    C.ghr[k] = (C.l[k] * C.l[k]) * C.boygr[k] / (fabs(x[k]) + T(1e-20));
+    C.gh[k] = C.ghr[k] <= T(0.028) ? C.ghr[k] : T(0.028);
+  }
+  C.l[0] = kappa * C.l0;
+  C.l[KB - 1] = T(0);
+  #pragma unroll
+  for (int k = 0; k < KB; ++k) C.dtef[k] = sqrt(fabs(x[k]) + T(1e-8)) * T(1) / (T(B1) * C.l[k] + small);
+  // shear production (tracers.py:287-300): du, dv over the (i+1, j) / (i, j+1) neighbours
+  T du[KB], dv[KB];
+  col_ld(io, s.u, x);
+  col_ld_shift(io, s.u, (int64_t)p.jm, y);
+  #pragma unroll
+  for (int k = 1; k < KBM1; ++k) du[k] = x[k] - x[k - 1] + y[k] - y[k - 1];
+  col_ld(io, s.v, x);
+  col_ld_shift(io, s.v, (int64_t)1, y);
+  #pragma unroll
+  for (int k = 1; k < KBM1; ++k) dv[k] = x[k] - x[k - 1] + y[k] - y[k - 1];
+  col_ld(io, s.km, x);
+  col_ld(io, s.kh, y);
+  #pragma unroll
+  for (int k = 0; k < KB; ++k) C.kn[k] = T(0);
+  if (C.inner) {
+    #pragma unroll
+    for (int k = 1; k < KBM1; ++k) {
+      const T dz2 = vg.dzz(m, k - 1) * dh;
+      const T val = x[k] * T(0.25) * T(SEF) * (du[k] * du[k] + dv[k] * dv[k]) / (dz2 * dz2);
+      C.kn[k] = val + y[k] * C.boygr[k];
+    }
+  }
+  // q2 tridiagonal solve (tracers.py:302-346)
+  col_ld(io, s.q2f, x);
+  C.ee1[0] = T(0);
+  C.gg1[0] = C.has_s ? T(pow(15.8 * CBCNST, 2.0 / 3.0)) * C.utau2 : T(0);
+  #pragma unroll
+  for (int k = 1; k < KBM1; ++k) {
+    const T den = C.a[k] + C.c[k] * (T(1) - C.ee1[k - 1]) - (T(2) * dti2 * C.dtef[k] + T(1));
+    C.ee1[k] = C.a[k] / (den + T(1e-20));
+    C.gg1[k] = (T(-2.0 * prm.dti2) * C.kn[k] + C.c[k] * C.gg1[k - 1] - x[k]) / (den + T(1e-20));
+  }
+  C.q2n[KB - 1] = C.q2bot;
+  #pragma unroll
+  for (int k = KBM1 - 1; k >= 0; --k) C.q2n[k] = C.ee1[k] * C.q2n[k + 1] + C.gg1[k];
+  // q2l tridiagonal solve with wall proximity (tracers.py:348-384)
+  col_ld(io, s.q2lf, x);
+  C.ee2[0] = C.gg2[0] = T(0);
+  #pragma unroll
+  for (int k = 1; k < KBM1; ++k) {
+    const T W = T(1) / (fabs(vg.z(m, k) - vg.z(m, 0)) + T(1e-20)) + T(1) / (fabs(vg.z(m, k) - vg.z(m, KB - 1)) + T(1e-20));
+    const T wall = W * C.l[k] / (dh * kappa);
+    const T dtk = C.dtef[k] * (T(1) + T(E2) * (wall * wall));
+    const T den = C.a[k] + C.c[k] * (T(1) - C.ee2[k - 1]) - (dti2 * dtk + T(1));
+    C.ee2[k] = C.a[k] / (den + T(1e-20));
+    C.gg2[k] = (dti2 * (-C.kn[k] * C.l[k] * T(E1)) + C.c[k] * C.gg2[k - 1] - x[k]) / (den + T(1e-20));
+  }
+  C.q2ln[KB - 1] = T(0);
+  #pragma unroll
+  for (int k = KBM1 - 1; k >= 0; --k) C.q2ln[k] = C.ee2[k] * C.q2ln[k + 1] + C.gg2[k];
+}
+
+template <class T>
+__global__ void __launch_bounds__(TPB, 2) k_profq(Batch B, Vert<T> vg, Params prm, ProfqArgs<T> s) {
+  using namespace myc;
+  __shared__ T lds[TILE];
+  Pos p;
+  const ColIO<T> io = col_pos(B, p, lds);
+  const int64_t c2 = ix2(p, p.i, p.j);
+  ProfqCol<T> C;
+  profq_fwd_col(p, io, vg, prm, s, C);
+  const T small = T(prm.small);
+  T o1[KB], o2[KB];
+  // clipped q2, q2l and the length scale
+  #pragma unroll
+  for (int k = 0; k < KB; ++k) {
+    T q2c = C.q2n[k], q2lc = C.q2ln[k];
+    if (k >= 1 && k < KBM1 && (C.q2n[k] <= small || C.q2ln[k] <= small)) {
+      q2c = small;
+      q2lc = T(0.1) * s.dt[c2] * small;
+    }
+    o1[k] = q2c;
+    o2[k] = q2lc;
+  }
+  col_st(io, s.q2o, o1);
+  col_st(io, s.q2lo, o2);
+  col_st(io, s.lo, C.l);
+  // stability functions and the time-averaged mixing coefficients (tracers.py:405-423)
+  T knf[KB];
+  col_ld(io, s.q2, knf);
+  #pragma unroll
+  for (int k = 0; k < KB; ++k) knf[k] = C.l[k] * sqrt(fabs(knf[k]) + T(1e-8));
+  col_ld(io, s.kq, o1);
+  #pragma unroll
+  for (int k = 0; k < KB; ++k) {
+    const T sh = T(COEF1) / (T(1) - T(COEF2) * C.gh[k] + T(1e-20));
+    o1[k] = (knf[k] * T(0.41) * sh + o1[k]) * T(0.5);
+  }
+  col_st(io, s.kqo, o1);
+  col_ld(io, s.km, o1);
+  #pragma unroll
+  for (int k = 0; k < KB; ++k) {
+    const T sh = T(COEF1) / (T(1) - T(COEF2) * C.gh[k] + T(1e-20));
+    const T sm = (T(COEF3) + sh * T(COEF4) * C.gh[k]) / (T(1) - T(COEF5) * C.gh[k] + T(1e-20));
+    o1[k] = (knf[k] * sm + o1[k]) * T(0.5);
+  }
+  col_st(io, s.kmo, o1);
+  col_ld(io, s.kh, o1);
+  #pragma unroll
+  for (int k = 0; k < KB; ++k) {
+    const T sh = T(COEF1) / (T(1) - T(COEF2) * C.gh[k] + T(1e-20));
+    o1[k] = (knf[k] * sh + o1[k]) * T(0.5);
+  }
+  col_st(io, s.kho, o1);
+}
+
+// Forward sweep of one column with direct (per-thread, strided) loads: used by
+// the reverse sweep, whose register budget leaves no room for staged columns.
+template <class T>
+__device__ __forceinline__ void profq_fwd_col_direct(const Pos& p, const Vert<T>& vg, const Params& prm,
                                               const ProfqArgs<T>& s, ProfqCol<T>& C) {
   using namespace myc;
   const int i = p.i, j = p.j, m = p.m;
@@ -148,34 +334,6 @@ __device__ __forceinline__ void profq_fwd_col(const Pos& p, const Vert<T>& vg, c
   for (int k = KBM1 - 1; k >= 0; --k) C.q2ln[k] = C.ee2[k] * C.q2ln[k + 1] + C.gg2[k];
 }
 
-template <class T>
-__global__ void __launch_bounds__(TPB) k_profq(Batch B, Vert<T> vg, Params prm, ProfqArgs<T> s) {
-  using namespace myc;
-  Pos p;
-  if (!pos2(B, p)) return;
-  const int64_t c = ix3(p, p.i, p.j, 0), c2 = ix2(p, p.i, p.j);
-  ProfqCol<T> C;
-  profq_fwd_col(p, vg, prm, s, C);
-  const T small = T(prm.small);
-  #pragma unroll
-  for (int k = 0; k < KB; ++k) {
-    T q2c = C.q2n[k], q2lc = C.q2ln[k];
-    if (k >= 1 && k < KBM1 && (C.q2n[k] <= small || C.q2ln[k] <= small)) {
-      q2c = small;
-      q2lc = T(0.1) * s.dt[c2] * small;
-    }
-    const T sh = T(COEF1) / (T(1) - T(COEF2) * C.gh[k] + T(1e-20));
-    const T sm = (T(COEF3) + sh * T(COEF4) * C.gh[k]) / (T(1) - T(COEF5) * C.gh[k] + T(1e-20));
-    const T knf = C.l[k] * sqrt(fabs(s.q2[c + k]) + T(1e-8));
-    s.q2o[c + k] = q2c;
-    s.q2lo[c + k] = q2lc;
-    s.lo[c + k] = C.l[k];
-    s.kqo[c + k] = (knf * T(0.41) * sh + s.kq[c + k]) * T(0.5);
-    s.kmo[c + k] = (knf * sm + s.km[c + k]) * T(0.5);
-    s.kho[c + k] = (knf * sh + s.kh[c + k]) * T(0.5);
-  }
-}
-
 template <class T> struct ProfqAdjArgs {
   ProfqArgs<T> fw;
   const T *b_q2o, *b_q2lo, *b_lo, *b_kqo, *b_kmo, *b_kho;   // output cotangents (any may be null)
@@ -188,6 +346,16 @@ template <class T> struct ProfqAdjArgs {
 
 template <class T> __device__ __forceinline__ T ldb(const T* b, int64_t i) { return b ? b[i] : T(0); }
 
+// column load of an optional array (null: zeros); the null test is uniform
+template <class T> __device__ __forceinline__ void col_ldz(const ColIO<T>& io, const T* f, T* x) {
+  if (f) {
+    col_ld(io, f, x);
+  } else {
+    #pragma unroll
+    for (int k = 0; k < KB; ++k) x[k] = T(0);
+  }
+}
+
 template <class T>
 __global__ void __launch_bounds__(TPB) k_profq_adj(Batch B, Vert<T> vg, Params prm, ProfqAdjArgs<T> S) {
   using namespace myc;
@@ -197,7 +365,7 @@ __global__ void __launch_bounds__(TPB) k_profq_adj(Batch B, Vert<T> vg, Params p
   const int m = p.m;
   const int64_t c = ix3(p, p.i, p.j, 0), c2 = ix2(p, p.i, p.j);
   ProfqCol<T> C;
-  profq_fwd_col(p, vg, prm, s, C);
+  profq_fwd_col_direct(p, vg, prm, s, C);
   const T dti2 = T(prm.dti2), grav = T(prm.grav), kappa = T(prm.kappa), small = T(prm.small);
   const T dh = C.dh;
 
@@ -470,26 +638,21 @@ __global__ void __launch_bounds__(TPB) k_profq_adj(Batch B, Vert<T> vg, Params p
   if (S.b_dt) S.b_dt[c2] += bdt2;
 }
 
-// gather the neighbour cotangents: column (i, j) receives what column (i-1, j)
-// wrote for u / wubot and what column (i, j-1) wrote for v / wvbot
+// gather the neighbour cotangents: point (i, j, k) receives what column
+// (i-1, j) wrote for u / wubot and what column (i, j-1) wrote for v / wvbot
+// (3-D launch, one thread per point: unit-stride on every array)
 template <class T>
 __global__ void __launch_bounds__(TPB) k_profq_adj2(Batch B, ProfqAdjArgs<T> S) {
   Pos p;
-  if (!pos2(B, p)) return;
-  const int64_t c = ix3(p, p.i, p.j, 0), c2 = ix2(p, p.i, p.j);
+  if (!pos3(B, p)) return;
+  const int64_t c = ix3(p, p.i, p.j, p.k);
   if (p.i >= 1) {
-    const int64_t o = ix3(p, p.i - 1, p.j, 0);
-    if (S.b_u)
-      #pragma unroll
-      for (int k = 0; k < KB; ++k) S.b_u[c + k] += S.s_unb[o + k];
-    if (S.b_wubot) S.b_wubot[c2] += S.s_wubnb[ix2(p, p.i - 1, p.j)];
+    if (S.b_u) S.b_u[c] += S.s_unb[c - (int64_t)p.jm * KB];
+    if (S.b_wubot && p.k == 0) S.b_wubot[ix2(p, p.i, p.j)] += S.s_wubnb[ix2(p, p.i - 1, p.j)];
   }
   if (p.j >= 1) {
-    const int64_t o = ix3(p, p.i, p.j - 1, 0);
-    if (S.b_v)
-      #pragma unroll
-      for (int k = 0; k < KB; ++k) S.b_v[c + k] += S.s_vnb[o + k];
-    if (S.b_wvbot) S.b_wvbot[c2] += S.s_wvbnb[ix2(p, p.i, p.j - 1)];
+    if (S.b_v) S.b_v[c] += S.s_vnb[c - KB];
+    if (S.b_wvbot && p.k == 0) S.b_wvbot[ix2(p, p.i, p.j)] += S.s_wvbnb[ix2(p, p.i, p.j - 1)];
   }
 }
 

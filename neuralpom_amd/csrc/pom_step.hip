@@ -74,20 +74,35 @@ struct Arena {
   }
 };
 
+// flat elementwise kernels: grid-stride loops over a capped grid (block
+// dispatch, not bandwidth, limits a one-element-per-thread launch of a 2-D field)
 template <class T>
 __global__ void k_fill(T* p, int64_t n, T v) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+// zero up to 32 arrays in one launch (grid.y selects the array)
+template <class T> struct FillTab {
+  T* p[32];
+  int64_t n[32];
+};
+template <class T>
+__global__ void k_fill_multi(FillTab<T> t) {
+  const int e = blockIdx.y;
+  T* p = t.p[e];
+  const int64_t n = t.n[e];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = T(0);
 }
 template <class T>
 __global__ void k_copy(T* __restrict__ d, const T* __restrict__ s, int64_t n) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) d[i] = s[i];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    d[i] = s[i];
 }
 template <class T>
 __global__ void k_add(T* __restrict__ d, const T* __restrict__ s, int64_t n) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) d[i] += s[i];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    d[i] += s[i];
 }
 // dt = h + et (time_loop.py:693)
 template <class T>
@@ -117,6 +132,11 @@ __global__ void __launch_bounds__(256) k_vamax(const T* __restrict__ vaf, int64_
 }
 
 inline int blocks(int64_t n) { return (int)((n + 255) / 256); }
+// grid of a grid-stride flat kernel: 4 elements per thread, at most 2048 blocks
+inline int flat_blocks(int64_t n) {
+  const int64_t b = (n + 1023) / 1024;
+  return (int)(b < 1 ? 1 : (b > 2048 ? 2048 : b));
+}
 
 template <class T> struct X {   // state of one barotropic substep
   T *ua, *uab, *va, *vab, *el, *elb, *d, *egf, *utf, *vtf, *etf, *advua, *advva, *uaf, *vaf, *elf, *fluxua, *fluxva,
@@ -208,7 +228,7 @@ template <class T> struct Step {
 
   void copy(T* d, const T* s, int64_t n) {
     if (d == s) return;
-    hipLaunchKernelGGL(k_copy<T>, dim3(blocks(n)), dim3(256), 0, c.stream, d, s, n);
+    hipLaunchKernelGGL(k_copy<T>, dim3(flat_blocks(n)), dim3(256), 0, c.stream, d, s, n);
   }
 
   // X_0: the state entering the substeps
@@ -758,15 +778,33 @@ template <class T> struct Adj {
   }
 
   void zero(T* p, int64_t n) {
-    hipLaunchKernelGGL(k_fill<T>, dim3(blocks(n)), dim3(256), 0, c.stream, p, n, T(0));
+    hipLaunchKernelGGL(k_fill<T>, dim3(flat_blocks(n)), dim3(256), 0, c.stream, p, n, T(0));
+  }
+  // batched zeroing: queue arrays, launch once per 32
+  FillTab<T> ft{};
+  int nft = 0;
+  int64_t ftmax = 0;
+  void zq(T* p, int64_t n) {
+    ft.p[nft] = p;
+    ft.n[nft] = n;
+    ftmax = n > ftmax ? n : ftmax;
+    if (++nft == 32) zflush();
+  }
+  void zflush() {
+    if (nft == 0) return;
+    const int64_t per = ftmax / 8;   // the nft arrays share the block budget
+    hipLaunchKernelGGL(k_fill_multi<T>, dim3(flat_blocks(per > 0 ? per : 1), nft), dim3(256), 0, c.stream, ft);
+    nft = 0;
+    ftmax = 0;
   }
   void add(T* d, const T* s, int64_t n) {
-    hipLaunchKernelGGL(k_add<T>, dim3(blocks(n)), dim3(256), 0, c.stream, d, s, n);
+    hipLaunchKernelGGL(k_add<T>, dim3(flat_blocks(n)), dim3(256), 0, c.stream, d, s, n);
   }
   void zero_x(X<T>& x, int64_t n) {
     for (T* p : {x.ua, x.uab, x.va, x.vab, x.el, x.elb, x.d, x.egf, x.utf, x.vtf, x.etf, x.advua, x.advva, x.uaf,
                  x.vaf, x.elf})
-      zero(p, n);
+      zq(p, n);
+    zflush();
   }
 
   void run(const T* const* cot_out, T* const* grad_in) {
@@ -776,13 +814,14 @@ template <class T> struct Adj {
     for (int f = 0; f < POM_NSTATE; ++f) {
       const int64_t n = f < POM_N3D ? N3 : N2;
       if (cot_out && cot_out[f]) S.copy(bo[f], cot_out[f], n);
-      else zero(bo[f], n);
-      if (!(grad_in && grad_in[f])) zero(bi[f], n);
+      else zq(bo[f], n);
+      if (!(grad_in && grad_in[f])) zq(bi[f], n);
     }
     for (T* p : {b_rho1, b_u1, b_v1, b_q2f0, b_q2lf0, b_q2f1, b_q2lf1, b_kmraw, b_khraw, b_tf0, b_sf0, b_tf1, b_sf1,
                  b_uf0, b_vf0, b_uf1, b_vf1, b_uf2, b_vf2})
-      zero(p, N3);
-    for (T* p : {b_adx2d_raw, b_ady2d_raw, b_advua0, b_advva0, b_egf0, b_utf0, b_vtf0}) zero(p, N2);
+      zq(p, N3);
+    for (T* p : {b_adx2d_raw, b_ady2d_raw, b_advua0, b_advva0, b_egf0, b_utf0, b_vtf0}) zq(p, N2);
+    zflush();
     const X<T>& xl = S.xs[S.isplit];
     // ---- shifts reversed: egb<-egf, etb<-et, et<-etf, dt<-h+et, utb<-utf, vtb<-vtf
     X<T>& cur = bx[S.isplit & 1];
@@ -998,7 +1037,7 @@ template <class T> struct Adj {
         ProfScope ps(POM_STAGE_PROFQ_ADJ, c.stream);
         hipLaunchKernelGGL(k_profq_adj<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a);
       }
-      hipLaunchKernelGGL(k_profq_adj2<T>, g2, tb, 0, c.stream, c.B, a);
+      hipLaunchKernelGGL(k_profq_adj2<T>, dim3(c.nblk3), tb, 0, c.stream, c.B, a);
     }
     for (int l2 = 0; l2 < 2; ++l2) {   // advq
       COutTab<T> ob{{l2 ? b_q2lf0 : b_q2f0}};
@@ -1117,6 +1156,51 @@ int run_adj(const pom_batch* b, const pom_params_t* prm, const void* ramp, const
   return hipGetLastError() == hipSuccess ? POM_OK : POM_E_HIP;
 }
 
+// ---------------------------------------------------------------- operator-level entry points
+// The reference's operator methods that callers invoke on their own (dens from
+// the corrector hook, baropg for the pressure-velocity error); each runs the
+// same kernels as the step.
+template <class T>
+int run_dens(const pom_batch* b, const pom_params_t* prm, const void* vg, const void* t, const void* s,
+             const void* rho_in, const void* h, const void* fsm, const void* b_rho_out, void* rho_out, void* b_t,
+             void* b_s, void* b_rho_in, hipStream_t st, bool adjoint) {
+  Ctx<T> c = make_ctx<T>(b, prm, nullptr, vg, st);
+  FieldTab<T> tab{};
+  tab.f[OpDens::TT] = static_cast<const T*>(t);
+  tab.f[OpDens::SS] = static_cast<const T*>(s);
+  tab.f[OpDens::RHO] = static_cast<const T*>(rho_in);
+  tab.g[OpDens::GH] = static_cast<const T*>(h);
+  tab.g[OpDens::GFSM] = static_cast<const T*>(fsm);
+  if (!adjoint) {
+    launch_map<T, OpDens>(c, tab, OutTab<T>{{static_cast<T*>(rho_out)}});
+  } else {
+    COutTab<T> ob{{static_cast<const T*>(b_rho_out)}};
+    T* ib[OpDens::NIN] = {static_cast<T*>(b_t), static_cast<T*>(b_s), static_cast<T*>(b_rho_in)};
+    launch_gather<T, OpDens>(c, tab, ob, ib);
+  }
+  return hipGetLastError() == hipSuccess ? POM_OK : POM_E_HIP;
+}
+
+template <class T>
+int run_baropg(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vg, const void* const* in,
+               void* const* out, const void* const* cot, void* const* grad, hipStream_t st, bool adjoint) {
+  Ctx<T> c = make_ctx<T>(b, prm, ramp, vg, st);
+  const dim3 g2(c.nblk2), tb(TPB);
+  if (c.nblk2 == 0) return POM_OK;
+  auto I = [&](int k) { return static_cast<const T*>(in[k]); };
+  if (!adjoint) {
+    BaropgArgs<T> a{I(0), I(1), I(2), I(3), I(4), I(5), I(6), static_cast<T*>(out[0]), static_cast<T*>(out[1]),
+                    static_cast<T*>(out[2])};
+    hipLaunchKernelGGL(k_baropg<T>, g2, tb, 0, c.stream, c.B, c.vert, c.ramp, c.prm, a);
+  } else {
+    BaropgAdjArgs<T> a{I(0), I(1), I(2), I(3), I(4), I(5), I(6),
+                       static_cast<const T*>(cot[0]), static_cast<const T*>(cot[1]), static_cast<const T*>(cot[2]),
+                       static_cast<T*>(grad[0]), static_cast<T*>(grad[1])};
+    hipLaunchKernelGGL(k_baropg_adj<T>, g2, tb, 0, c.stream, c.B, c.vert, c.ramp, c.prm, a);
+  }
+  return hipGetLastError() == hipSuccess ? POM_OK : POM_E_HIP;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1218,5 +1302,47 @@ int pom_explicit_step_adj(const pom_batch* b, const pom_params_t* prm, const voi
     return run_adj<float>(b, prm, ramp, vgrid, state_in, geom, internal, cot_out, grad_in, workspace, ws_bytes, st);
   return run_adj<double>(b, prm, ramp, vgrid, state_in, geom, internal, cot_out, grad_in, workspace, ws_bytes, st);
 }
+
+int pom_dens(const pom_batch* b, const pom_params_t* prm, const void* vgrid, const void* t, const void* s,
+             const void* rho_in, const void* h, const void* fsm, void* rho_out, void* stream) {
+  if (!b || !prm || !vgrid || !t || !s || !rho_in || !h || !fsm || !rho_out) return POM_E_ARG;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (b->dtype == POM_F32)
+    return run_dens<float>(b, prm, vgrid, t, s, rho_in, h, fsm, nullptr, rho_out, nullptr, nullptr, nullptr, st, false);
+  return run_dens<double>(b, prm, vgrid, t, s, rho_in, h, fsm, nullptr, rho_out, nullptr, nullptr, nullptr, st, false);
+}
+
+int pom_dens_adj(const pom_batch* b, const pom_params_t* prm, const void* vgrid, const void* t, const void* s,
+                 const void* rho_in, const void* h, const void* fsm, const void* b_rho_out, void* b_t, void* b_s,
+                 void* b_rho_in, void* stream) {
+  if (!b || !prm || !vgrid || !t || !s || !rho_in || !h || !fsm || !b_rho_out) return POM_E_ARG;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (b->dtype == POM_F32)
+    return run_dens<float>(b, prm, vgrid, t, s, rho_in, h, fsm, b_rho_out, nullptr, b_t, b_s, b_rho_in, st, true);
+  return run_dens<double>(b, prm, vgrid, t, s, rho_in, h, fsm, b_rho_out, nullptr, b_t, b_s, b_rho_in, st, true);
+}
+
+int pom_baropg(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vgrid,
+               const void* const* in, void* const* out, void* stream) {
+  if (!b || !prm || !ramp || !vgrid || !in || !out) return POM_E_ARG;
+  for (int k = 0; k < 7; ++k)
+    if (!in[k]) return POM_E_ARG;
+  for (int k = 0; k < 3; ++k)
+    if (!out[k]) return POM_E_ARG;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (b->dtype == POM_F32) return run_baropg<float>(b, prm, ramp, vgrid, in, out, nullptr, nullptr, st, false);
+  return run_baropg<double>(b, prm, ramp, vgrid, in, out, nullptr, nullptr, st, false);
+}
+
+int pom_baropg_adj(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vgrid,
+                   const void* const* in, const void* const* cot_out, void* const* grad_in, void* stream) {
+  if (!b || !prm || !ramp || !vgrid || !in || !cot_out || !grad_in) return POM_E_ARG;
+  for (int k = 0; k < 7; ++k)
+    if (!in[k]) return POM_E_ARG;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (b->dtype == POM_F32) return run_baropg<float>(b, prm, ramp, vgrid, in, nullptr, cot_out, grad_in, st, true);
+  return run_baropg<double>(b, prm, ramp, vgrid, in, nullptr, cot_out, grad_in, st, true);
+}
+
 
 }  // extern "C"

@@ -174,7 +174,10 @@ class Ensemble:
         ins = [state[n] for n in L.FIELDS]
         gs = [geom[n] for n in L.GEOM]
         self._check(ins, L.FIELDS, self.numel)
-        grads = {n: torch.zeros_like(state[n]) for n in L.DIFF_INPUTS}
+        # one zero-filled allocation, split into per-field views
+        sizes = [self.numel(n) for n in L.DIFF_INPUTS]
+        flat = torch.zeros(sum(sizes), dtype=self.dtype, device=self.device)
+        grads = dict(zip(L.DIFF_INPUTS, torch.split(flat, sizes)))
         cot_ptrs = []
         for n in L.FIELDS:
             c = cot.get(n)
