@@ -587,8 +587,10 @@ def profq(S, q2f, q2lf, q2, q2b, q2lb, km, kh, kq, u, v, t, s, rho,
     ee = [torch.zeros((im, jm), dtype=dt_, device=dev_)]
     gg = [torch.zeros((im, jm), dtype=dt_, device=dev_)]
     ones = torch.ones_like(dh)
+    # wall-proximity guard: 1e-20 (explicit, tracers.py:362), 1e-8 (implicit core, IM/tracers.py:360)
+    weps = getattr(S, "wall_eps", 1e-20)
     for k in range(1, kbm1):
-        wall = (ones / (torch.abs(z[k] - z[0]) + 1e-20) + ones / (torch.abs(z[k] - z[kb - 1]) + 1e-20)) \
+        wall = (ones / (torch.abs(z[k] - z[0]) + weps) + ones / (torch.abs(z[k] - z[kb - 1]) + weps)) \
             * l[:, :, k] / (dh * kappa)
         dtk = dtef[:, :, k] * (1.0 + E2 * (wall ** 2))
         den = a[:, :, k] + c[:, :, k] * (1.0 - ee[k - 1]) - (dti2 * dtk + 1.0)
