@@ -64,13 +64,24 @@ typedef struct {
   int64_t off2, off3;
 } pom_member_t;
 
-/* Scalars of the step (explicit/base.py:46-118, v1/core_spec.py:8-47). */
+#define POM_CORE_EXPLICIT 0
+#define POM_CORE_IMPLICIT 1
+#define POM_PRECOND_JACOBI 0
+#define POM_PRECOND_SPECTRAL 1
+
+/* Scalars of the step (explicit/base.py:46-118, v1/core_spec.py:8-47).
+ * The semi-implicit core (core = POM_CORE_IMPLICIT) also reads the eta-solver
+ * settings (implicit_p0_fixed/core.py:62-119, numerics/double_gyre.py:186-191):
+ * eta_max_iter, eta_precond, eta_tol (absolute), eta_rel_tol, and the
+ * post-solve Shapiro weight gamma_baro (0 in production, core_spec.py:24-29). */
 typedef struct {
   double dti, dte, dti2, dte2, ispi, isp2i;
   double alpha, smoth, horcon, grav, rhoref, kappa, umol, tprni, small;
   double tbias, sbias, hmax, rfe, rfw, rfn, rfs;
   int32_t isplit, ispadv, ntp, nbct, nbcs, iext;
   double vmaxl;
+  int32_t core, eta_max_iter, eta_precond, reserved;
+  double eta_tol, eta_rel_tol, gamma_baro;
 } pom_params_t;
 
 typedef struct pom_batch pom_batch;
@@ -105,6 +116,39 @@ int pom_explicit_step(const pom_batch* b, const pom_params_t* prm, const void* r
  * el elb ua uab va vab d dt et etb egb utf utb vtf vtb wubot wvbot. */
 int pom_explicit_step_adj(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vgrid,
                           const void* const* state_in, const void* const* geom, int32_t internal,
+                          const void* const* cot_out, void* const* grad_in, void* workspace, size_t ws_bytes,
+                          void* stream);
+
+/* ---- Semi-implicit (P0-fixed) core: src/neuralpom/cores/implicit_p0_fixed/
+ * time_loop.py:38-688 with the closed-basin double-gyre policy
+ * (boundary/closed_basin.py, forcing/surface.py, numerics/double_gyre.py).
+ * prm->core must be POM_CORE_IMPLICIT.  The barotropic mode is predictor ->
+ * eta solve (batched PCG, Jacobi or spectral DST-I preconditioner, core.py:
+ * 460-773) -> corrector; every member stops at its own convergence.
+ *
+ * pom_implicit_prepare: builds the spectral-preconditioner cache (sine
+ *   matrices and inverse eigenvalues at the interior medians of dx, dy, h,
+ *   core.py:730-760) from the member geometry; synchronises the stream.
+ *   Replaces POM2K_Core._get_spectral_eta_preconditioner_cache and the
+ *   acceleration cache of core.py:121-199 (run once after initialisation).
+ * pom_implicit_step: one internal step (replaces the loop body of
+ *   implicit_p0_fixed/time_loop.py:95-688).  eta_iters_out (device, may be
+ *   null) receives each member's PCG iteration count.  The vamax check is a
+ *   device scalar as in pom_explicit_step.
+ * pom_implicit_step_adj: vector-Jacobian product of one step with the
+ *   reference's eta-solve adjoint (second PCG on the area-weighted transpose
+ *   plus the depth term, core.py:24-59, 231-252, 645-675).  Same conventions
+ *   as pom_explicit_step_adj. */
+int pom_implicit_cache_bytes(const pom_batch* b, size_t* bytes);
+int pom_implicit_prepare(const pom_batch* b, const pom_params_t* prm, const void* const* geom, void* cache,
+                         size_t cache_bytes, void* stream);
+int pom_implicit_workspace(const pom_batch* b, const pom_params_t* prm, int32_t adjoint, size_t* bytes);
+int pom_implicit_step(const pom_batch* b, const pom_params_t* prm, const void* cache, const void* ramp,
+                      const void* vgrid, const void* const* state_in, void* const* state_out, const void* const* geom,
+                      int32_t internal, void* vamax_out, int32_t* eta_iters_out, void* workspace, size_t ws_bytes,
+                      void* stream);
+int pom_implicit_step_adj(const pom_batch* b, const pom_params_t* prm, const void* cache, const void* ramp,
+                          const void* vgrid, const void* const* state_in, const void* const* geom, int32_t internal,
                           const void* const* cot_out, void* const* grad_in, void* workspace, size_t ws_bytes,
                           void* stream);
 

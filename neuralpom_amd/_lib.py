@@ -13,6 +13,8 @@ _LIB_PATH = Path(__file__).resolve().parent / "libneuralpom_amd.so"
 
 POM_OK, POM_E_ARG, POM_E_HIP, POM_E_UNSUPPORTED, POM_E_VMAX = 0, 1, 2, 3, 4
 POM_F32, POM_F64 = 0, 1
+POM_CORE_EXPLICIT, POM_CORE_IMPLICIT = 0, 1
+POM_PRECOND_JACOBI, POM_PRECOND_SPECTRAL = 0, 1
 
 # pom_field order (include/neuralpom_amd.h)
 FIELDS_3D = ("u", "ub", "v", "vb", "t", "tb", "s", "sb", "q2", "q2b", "q2l", "q2lb",
@@ -43,7 +45,8 @@ class Params(C.Structure):
         "dti", "dte", "dti2", "dte2", "ispi", "isp2i", "alpha", "smoth", "horcon", "grav",
         "rhoref", "kappa", "umol", "tprni", "small", "tbias", "sbias", "hmax", "rfe", "rfw",
         "rfn", "rfs")] + [(n, C.c_int32) for n in ("isplit", "ispadv", "ntp", "nbct", "nbcs", "iext")] + [
-        ("vmaxl", C.c_double)]
+        ("vmaxl", C.c_double)] + [(n, C.c_int32) for n in ("core", "eta_max_iter", "eta_precond", "reserved")] + [
+        (n, C.c_double) for n in ("eta_tol", "eta_rel_tol", "gamma_baro")]
 
 
 _lib = None
@@ -74,6 +77,13 @@ def load():
     lib.pom_dens_adj.argtypes = [vp, C.POINTER(Params), vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     lib.pom_baropg.argtypes = [vp, C.POINTER(Params), vp, vp, vpp, vpp, vp]
     lib.pom_baropg_adj.argtypes = [vp, C.POINTER(Params), vp, vp, vpp, vpp, vpp, vp]
+    lib.pom_implicit_cache_bytes.argtypes = [vp, C.POINTER(C.c_size_t)]
+    lib.pom_implicit_prepare.argtypes = [vp, C.POINTER(Params), vpp, vp, C.c_size_t, vp]
+    lib.pom_implicit_workspace.argtypes = [vp, C.POINTER(Params), C.c_int32, C.POINTER(C.c_size_t)]
+    lib.pom_implicit_step.argtypes = [vp, C.POINTER(Params), vp, vp, vp, vpp, vpp, vpp, C.c_int32, vp, vp, vp,
+                                      C.c_size_t, vp]
+    lib.pom_implicit_step_adj.argtypes = [vp, C.POINTER(Params), vp, vp, vp, vpp, vpp, C.c_int32, vpp, vpp, vp,
+                                          C.c_size_t, vp]
     lib.pom_profile_stage.argtypes = [C.c_int32]
     lib.pom_profile_read.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_int32)]
     for fn in EXPORTED:
@@ -83,8 +93,9 @@ def load():
 
 
 EXPORTED = ("pom_batch_create", "pom_batch_destroy", "pom_explicit_workspace", "pom_explicit_step",
-            "pom_explicit_step_adj", "pom_dens", "pom_dens_adj", "pom_baropg", "pom_baropg_adj",
-            "pom_profile_stage", "pom_profile_read")
+            "pom_explicit_step_adj", "pom_implicit_cache_bytes", "pom_implicit_prepare", "pom_implicit_workspace",
+            "pom_implicit_step", "pom_implicit_step_adj", "pom_dens", "pom_dens_adj", "pom_baropg",
+            "pom_baropg_adj", "pom_profile_stage", "pom_profile_read")
 STAGES = {"profq": 0, "profq_adj": 1, "extuv": 2, "advct": 3}
 
 

@@ -525,8 +525,11 @@ __global__ void __launch_bounds__(TPB) k_vertvl(Batch B, Vert<T> vg, Params prm,
   } else {
     ld_col(a.w + c, w);
   }
+  // bcond 5: w * fsm on k < kbm1 (explicit, baroclinic.py:849-851); on every
+  // level for the closed-basin policy of the semi-implicit core (closed_basin.py:77-79)
   #pragma unroll
-  for (int k = 0; k < KBM1; ++k) w[k] = w[k] * fsm;
+  for (int k = 0; k < KB; ++k)
+    if (k < KBM1 || prm.core) w[k] = w[k] * fsm;
   st_col(a.wout + c, w);
 }
 
@@ -553,7 +556,7 @@ __global__ void __launch_bounds__(TPB) k_vertvl_adj1(Batch B, Vert<T> vg, Params
     #pragma unroll
     for (int k = KBM1 - 1; k >= 0; --k) {
       const int kw = k + 1;
-      suffix += a.b_w[c + kw] * (kw < KBM1 ? fsm : T(1));
+      suffix += a.b_w[c + kw] * ((kw < KBM1 || prm.core) ? fsm : T(1));
       const T binc = suffix * vg.dz(p.m, k);
       bd[k] = binc / area;
       beta += binc;
@@ -1047,6 +1050,7 @@ __global__ void __launch_bounds__(TPB) k_profuv_adj2(Batch B, Vert<T> vg, Profuv
 template <class T> struct UVFiltArgs {
   const T *uf, *ub, *u1;
   T *ub_new, *u_new;
+  const T* mask;   // dum / dvm applied to the filtered level (semi-implicit core), or null
 };
 template <class T>
 __global__ void __launch_bounds__(TPB) k_uvfilt(Batch B, Vert<T> vg, Params prm, UVFiltArgs<T> a) {
@@ -1068,12 +1072,18 @@ __global__ void __launch_bounds__(TPB) k_uvfilt(Batch B, Vert<T> vg, Params prm,
   #pragma unroll
   for (int k = 0; k < KBM1; ++k) o[k] = u[k] + f * (term[k] - s);
   o[KB - 1] = u[KB - 1];
+  if (a.mask) {   // numerics/double_gyre.py:132-184 with gamma 0
+    const T mk = a.mask[ix2(p, p.i, p.j)];
+    #pragma unroll
+    for (int k = 0; k < KB; ++k) o[k] = o[k] * mk;
+  }
   st_col(a.ub_new + c, o);
   st_col(a.u_new + c, uf);
 }
 template <class T> struct UVFiltAdjArgs {
   const T *b_ub_new, *b_u_new;
   T *b_uf, *b_ub, *b_u1;
+  const T* mask;
 };
 template <class T>
 __global__ void __launch_bounds__(TPB) k_uvfilt_adj(Batch B, Vert<T> vg, Params prm, UVFiltAdjArgs<T> a) {
@@ -1084,6 +1094,11 @@ __global__ void __launch_bounds__(TPB) k_uvfilt_adj(Batch B, Vert<T> vg, Params 
   T g[KB], gt[KB], gu[KB], gf[KB];
   #pragma unroll
   for (int k = 0; k < KB; ++k) g[k] = a.b_ub_new ? a.b_ub_new[c + k] : T(0);
+  if (a.mask) {
+    const T mk = a.mask[ix2(p, p.i, p.j)];
+    #pragma unroll
+    for (int k = 0; k < KB; ++k) g[k] = g[k] * mk;
+  }
   T S = T(0);
   #pragma unroll
   for (int k = 0; k < KBM1; ++k) S += g[k];

@@ -213,7 +213,9 @@ struct Params {
   double dti, dte, dti2, dte2, ispi, isp2i;
   double alpha, smoth, horcon, grav, rhoref, kappa, umol, tprni, small;
   double tbias, sbias, hmax, rfe, rfw, rfn, rfs;
+  double gamma_baro;                               // post-solve Shapiro weight (semi-implicit core)
   int32_t isplit, ispadv, ntp, nbct, nbcs, iext;   // iext: current substep
+  int32_t core;                                    // 0 explicit, 1 semi-implicit
 };
 
 template <class T> struct Vert {

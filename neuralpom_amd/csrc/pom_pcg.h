@@ -1,0 +1,441 @@
+// Batched preconditioned conjugate gradients for the eta equation
+//   A(D) eta = rhs,  A eta = eta - 4 dti^2 g alpha div(D grad eta) / art,
+// eta = 0 on every edge (closed basin), one independent system per member.
+//
+// reference: src/neuralpom/cores/implicit_p0_fixed/core.py:460-589 (eager PCG: update
+// order, stopping rule rs <= tol^2 | rs <= rel_tol^2 rs0 | rs <= 1e-30, p edges zeroed),
+// :591-639 (operator), :677-708 (Jacobi), :710-773 (spectral DST-I preconditioner).
+//
+// Each member keeps its own scalars and an active flag; a member that has
+// converged is frozen (its kernels skip), so running extra iterations for the
+// other members leaves its solution exactly as if it had stopped.  Dot products
+// are reduced per block and then per member in a fixed order (deterministic),
+// accumulated in double.  The spectral preconditioner is two DST-I transforms
+// per axis, written as dense products with the sine matrices
+//   S_n[k][m] = 2 sin(pi (k+1)(m+1) / (n+1))
+// (the reference's FFT-based 2 x DST-I) in an LDS-tiled batched GEMM.
+#pragma once
+#include "pom_common.h"
+
+namespace pom {
+
+// per-member scalar slots
+enum { PC_RHO = 0, PC_THR, PC_TOL2, PC_ALPHA, PC_BETA, PC_RS, PC_RS0, PC_AUX, PC_N };
+
+template <class T> struct EllGeom {
+  const T *dx, *dy, *art, *dum, *dvm;
+  double coeff;   // 4 dti^2 g alpha
+};
+
+// A(D) x at an interior point (same arithmetic as OpEll)
+template <class T>
+__device__ __forceinline__ T ell_point(const Pos& p, const EllGeom<T>& g, const T* __restrict__ d,
+                                       const T* __restrict__ x) {
+  const int i = p.i, j = p.j;
+  const int64_t c = ix2(p, i, j), e = ix2(p, i + 1, j), w = ix2(p, i - 1, j), n = ix2(p, i, j + 1),
+                s = ix2(p, i, j - 1);
+  const T h = T(0.5);
+  const T ex = (h * (g.dy[e] + g.dy[c])) / (h * (g.dx[e] + g.dx[c]) + T(1e-20));
+  const T wx = (h * (g.dy[c] + g.dy[w])) / (h * (g.dx[c] + g.dx[w]) + T(1e-20));
+  const T ny = (h * (g.dx[n] + g.dx[c])) / (h * (g.dy[n] + g.dy[c]) + T(1e-20));
+  const T sy = (h * (g.dx[c] + g.dx[s])) / (h * (g.dy[c] + g.dy[s]) + T(1e-20));
+  const T fe = h * (d[e] + d[c]) * (x[e] - x[c]) * ex * g.dum[e];
+  const T fw = h * (d[c] + d[w]) * (x[c] - x[w]) * wx * g.dum[c];
+  const T fn = h * (d[n] + d[c]) * (x[n] - x[c]) * ny * g.dvm[n];
+  const T fs = h * (d[c] + d[s]) * (x[c] - x[s]) * sy * g.dvm[c];
+  const T div = (fe - fw + fn - fs) * (T(1) / (g.art[c] + T(1e-20)));
+  return x[c] - T(g.coeff) * div;
+}
+
+// Jacobi diagonal at an interior point (core.py:677-699)
+template <class T>
+__device__ __forceinline__ T ell_diag(const Pos& p, const EllGeom<T>& g, const T* __restrict__ d) {
+  const int i = p.i, j = p.j;
+  const int64_t c = ix2(p, i, j), e = ix2(p, i + 1, j), w = ix2(p, i - 1, j), n = ix2(p, i, j + 1),
+                s = ix2(p, i, j - 1);
+  const T h = T(0.5);
+  const T ex = (h * (g.dy[e] + g.dy[c])) / (h * (g.dx[e] + g.dx[c]) + T(1e-20));
+  const T wx = (h * (g.dy[c] + g.dy[w])) / (h * (g.dx[c] + g.dx[w]) + T(1e-20));
+  const T ny = (h * (g.dx[n] + g.dx[c])) / (h * (g.dy[n] + g.dy[c]) + T(1e-20));
+  const T sy = (h * (g.dx[c] + g.dx[s])) / (h * (g.dy[c] + g.dy[s]) + T(1e-20));
+  const T he = h * (d[e] + d[c]), hw = h * (d[c] + d[w]), hn = h * (d[n] + d[c]), hs = h * (d[c] + d[s]);
+  return T(1) + T(g.coeff) * (T(1) / (g.art[c] + T(1e-20))) *
+                    (he * ex * g.dum[e] + hw * wx * g.dum[c] + hn * ny * g.dvm[n] + hs * sy * g.dvm[c]);
+}
+
+__device__ __forceinline__ bool interior_p(const Pos& p) {
+  return p.i >= 1 && p.i <= p.im - 2 && p.j >= 1 && p.j <= p.jm - 2;
+}
+
+// block sum of a double (TPB threads), result valid in thread 0
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = TPB / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+// member and active flag of a 2-D block; pos2-style position of this thread
+__device__ __forceinline__ int block_member(const Batch& B) { return member_of(B.blk2, B.M, blockIdx.x); }
+
+// ---------------------------------------------------------------- set-up
+// x = x0 on the interior, 0 on the edges; r = rhs - A x on the interior, 0 on
+// the edges (rhs edges are zeroed by the reference); z = 0
+template <class T>
+__global__ void __launch_bounds__(TPB) k_pcg_x0(Batch B, const T* __restrict__ x0, T* __restrict__ x,
+                                                T* __restrict__ z) {
+  Pos p;
+  if (!pos2(B, p)) return;
+  const int64_t c = ix2(p, p.i, p.j);
+  x[c] = (x0 && interior_p(p)) ? x0[c] : T(0);
+  z[c] = T(0);
+}
+
+template <class T>
+__global__ void __launch_bounds__(TPB) k_pcg_resid(Batch B, EllGeom<T> g, const T* __restrict__ d,
+                                                   const T* __restrict__ x, const T* __restrict__ rhs,
+                                                   T* __restrict__ r, T* __restrict__ p_out) {
+  Pos p;
+  if (!pos2(B, p)) return;
+  const int64_t c = ix2(p, p.i, p.j);
+  r[c] = interior_p(p) ? rhs[c] - ell_point(p, g, d, x) : T(0);
+  p_out[c] = T(0);
+}
+
+// Jacobi: z = r / (diag + 1e-30) on the interior (active members only), with
+// partial sums of r.z and r.r
+template <class T>
+__global__ void __launch_bounds__(TPB) k_pcg_jacobi(Batch B, EllGeom<T> g, const T* __restrict__ d,
+                                                    const T* __restrict__ r, T* __restrict__ z,
+                                                    const int* __restrict__ act, double* __restrict__ part) {
+  __shared__ double sh[TPB];
+  const int m = block_member(B);
+  if (!act[m]) {
+    if (threadIdx.x == 0) part[2 * blockIdx.x] = part[2 * blockIdx.x + 1] = 0.0;
+    return;
+  }
+  Pos p;
+  double rz = 0.0;
+  if (pos2(B, p) && interior_p(p)) {
+    const int64_t c = ix2(p, p.i, p.j);
+    const T zv = r[c] / (ell_diag(p, g, d) + T(1e-30));
+    z[c] = zv;
+    rz = double(r[c]) * double(zv);
+  }
+  const double s = block_sum(rz, sh);
+  if (threadIdx.x == 0) part[2 * blockIdx.x] = s;
+}
+
+// partial sums of a.b over the interior (active members only) into part[2 b + slot]
+template <class T>
+__global__ void __launch_bounds__(TPB) k_pcg_dot(Batch B, const T* __restrict__ a, const T* __restrict__ b,
+                                                 const int* __restrict__ act, double* __restrict__ part, int slot) {
+  __shared__ double sh[TPB];
+  const int m = block_member(B);
+  if (!act[m]) {
+    if (threadIdx.x == 0) part[2 * blockIdx.x + slot] = 0.0;
+    return;
+  }
+  Pos p;
+  double v = 0.0;
+  if (pos2(B, p) && interior_p(p)) {
+    const int64_t c = ix2(p, p.i, p.j);
+    v = double(a[c]) * double(b[c]);
+  }
+  const double s = block_sum(v, sh);
+  if (threadIdx.x == 0) part[2 * blockIdx.x + slot] = s;
+}
+
+// per-member sum of part[2 b + slot] over the member's blocks (one block per member)
+__device__ __forceinline__ double member_sum(const Batch& B, const double* part, int m, int slot, double* sh) {
+  const int b0 = B.blk2[m], b1 = B.blk2[m + 1];
+  double v = 0.0;
+  for (int b = b0 + (int)threadIdx.x; b < b1; b += TPB) v += part[2 * b + slot];
+  return block_sum(v, sh);
+}
+
+// every member starts active
+__global__ void k_pcg_act1(int* act, int M) {
+  for (int m = threadIdx.x; m < M; m += blockDim.x) act[m] = 1;
+}
+
+// after set-up: rho = r.z, rs0 = r.r, thresholds; all members active
+__global__ void __launch_bounds__(TPB) k_pcg_start(Batch B, const double* __restrict__ part, double* sc,
+                                                   int* act, int* its, const double* __restrict__ tol,
+                                                   double rel_tol) {
+  __shared__ double sh[TPB];
+  const int m = blockIdx.x;
+  const double rz = member_sum(B, part, m, 0, sh);
+  __syncthreads();
+  const double rr = member_sum(B, part, m, 1, sh);
+  if (threadIdx.x == 0) {
+    double* s = sc + PC_N * m;
+    s[PC_RHO] = rz;
+    s[PC_RS0] = rr;
+    s[PC_THR] = rel_tol * rel_tol * fmax(rr, 1e-30);
+    s[PC_TOL2] = tol[m] * tol[m];
+    act[m] = 1;
+    its[m] = 0;
+  }
+}
+
+// ---------------------------------------------------------------- iteration
+// ap = A p (interior), partial sums of p.ap
+template <class T>
+__global__ void __launch_bounds__(TPB) k_pcg_ap(Batch B, EllGeom<T> g, const T* __restrict__ d,
+                                                const T* __restrict__ p_, T* __restrict__ ap,
+                                                const int* __restrict__ act, double* __restrict__ part) {
+  __shared__ double sh[TPB];
+  const int m = block_member(B);
+  if (!act[m]) {
+    if (threadIdx.x == 0) part[2 * blockIdx.x] = 0.0;
+    return;
+  }
+  Pos p;
+  double v = 0.0;
+  if (pos2(B, p)) {
+    const int64_t c = ix2(p, p.i, p.j);
+    if (interior_p(p)) {
+      const T a = ell_point(p, g, d, p_);
+      ap[c] = a;
+      v = double(p_[c]) * double(a);
+    } else {
+      ap[c] = p_[c];
+    }
+  }
+  const double s = block_sum(v, sh);
+  if (threadIdx.x == 0) part[2 * blockIdx.x] = s;
+}
+
+// alpha = rho / (p.ap + 1e-30)
+__global__ void __launch_bounds__(TPB) k_pcg_alpha(Batch B, const double* __restrict__ part, double* sc,
+                                                   const int* __restrict__ act, int* nact) {
+  __shared__ double sh[TPB];
+  const int m = blockIdx.x;
+  if (m == 0 && threadIdx.x == 0) *nact = 0;
+  if (!act[m]) return;
+  const double den = member_sum(B, part, m, 0, sh);
+  if (threadIdx.x == 0) {
+    double* s = sc + PC_N * m;
+    s[PC_ALPHA] = s[PC_RHO] / (den + 1e-30);
+  }
+}
+
+// x += alpha p, r -= alpha ap; partial sums of r.r
+template <class T>
+__global__ void __launch_bounds__(TPB) k_pcg_xr(Batch B, T* __restrict__ x, T* __restrict__ r,
+                                                const T* __restrict__ p_, const T* __restrict__ ap,
+                                                const double* __restrict__ sc, const int* __restrict__ act,
+                                                double* __restrict__ part) {
+  __shared__ double sh[TPB];
+  const int m = block_member(B);
+  if (!act[m]) {
+    if (threadIdx.x == 0) part[2 * blockIdx.x + 1] = 0.0;
+    return;
+  }
+  const T alpha = T(sc[PC_N * m + PC_ALPHA]);
+  Pos p;
+  double v = 0.0;
+  if (pos2(B, p)) {
+    const int64_t c = ix2(p, p.i, p.j);
+    if (interior_p(p)) {
+      x[c] = x[c] + alpha * p_[c];
+      const T rv = r[c] - alpha * ap[c];
+      r[c] = rv;
+      v = double(rv) * double(rv);
+    }
+  }
+  const double s = block_sum(v, sh);
+  if (threadIdx.x == 0) part[2 * blockIdx.x + 1] = s;
+}
+
+// convergence: rs = r.r; a member stops when rs <= tol^2, rs <= rel_tol^2 rs0
+// or rs <= 1e-30 (its x and r keep this iteration's update)
+__global__ void __launch_bounds__(TPB) k_pcg_conv(Batch B, const double* __restrict__ part, double* sc, int* act,
+                                                  int* its, int* nact) {
+  __shared__ double sh[TPB];
+  const int m = blockIdx.x;
+  if (!act[m]) return;
+  const double rs = member_sum(B, part, m, 1, sh);
+  if (threadIdx.x == 0) {
+    double* s = sc + PC_N * m;
+    s[PC_RS] = rs;
+    its[m] += 1;
+    if (rs <= s[PC_TOL2] || rs <= s[PC_THR] || rs <= 1e-30) act[m] = 0;
+    else atomicAdd(nact, 1);
+  }
+}
+
+// beta = (r.z) / (rho + 1e-30); rho = r.z
+__global__ void __launch_bounds__(TPB) k_pcg_beta(Batch B, const double* __restrict__ part, double* sc,
+                                                  const int* __restrict__ act) {
+  __shared__ double sh[TPB];
+  const int m = blockIdx.x;
+  if (!act[m]) return;
+  const double rz = member_sum(B, part, m, 0, sh);
+  if (threadIdx.x == 0) {
+    double* s = sc + PC_N * m;
+    s[PC_BETA] = rz / (s[PC_RHO] + 1e-30);
+    s[PC_RHO] = rz;
+  }
+}
+
+// p = p beta + z on the interior, 0 on the edges
+template <class T>
+__global__ void __launch_bounds__(TPB) k_pcg_p(Batch B, T* __restrict__ p_, const T* __restrict__ z,
+                                               const double* __restrict__ sc, const int* __restrict__ act,
+                                               int first) {
+  Pos p;
+  if (!pos2(B, p)) return;
+  if (!act[p.m]) return;
+  const int64_t c = ix2(p, p.i, p.j);
+  if (!interior_p(p)) {
+    p_[c] = T(0);
+    return;
+  }
+  p_[c] = first ? z[c] : p_[c] * T(sc[PC_N * p.m + PC_BETA]) + z[c];
+}
+
+// ---------------------------------------------------------------- adjoint right-hand side
+// The eta-solve adjoint (core.py:231-252) solves A^T lam = g as
+// lam = area A^-1 (g / area) with area = art / art(1,1) on the interior (1 on
+// the edges), the right-hand side divided by its max-norm (clamped below at 1)
+// and the absolute tolerance scaled alike.
+template <class T>
+__device__ __forceinline__ T adj_area(const Pos& p, const T* __restrict__ art) {
+  return interior_p(p) ? art[ix2(p, p.i, p.j)] / art[ix2(p, 1, 1)] : T(1);
+}
+
+__device__ __forceinline__ double block_max(double v, double* sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = TPB / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+// a = g / area; partial max |a|
+template <class T>
+__global__ void __launch_bounds__(TPB) k_eta_adj_in(Batch B, const T* __restrict__ g, const T* __restrict__ art,
+                                                    T* __restrict__ a, double* __restrict__ part) {
+  __shared__ double sh[TPB];
+  Pos p;
+  double v = 0.0;
+  if (pos2(B, p)) {
+    const int64_t c = ix2(p, p.i, p.j);
+    const T x = g[c] / adj_area(p, art);
+    a[c] = x;
+    v = fabs(double(x));
+  }
+  const double m = block_max(v, sh);
+  if (threadIdx.x == 0) part[2 * blockIdx.x] = m;
+}
+
+// scale = max(max|a|, 1); tol = eta_tol / scale
+__global__ void __launch_bounds__(TPB) k_eta_adj_scale(Batch B, const double* __restrict__ part, double* sc,
+                                                       double* tol, double eta_tol) {
+  __shared__ double sh[TPB];
+  const int m = blockIdx.x;
+  const int b0 = B.blk2[m], b1 = B.blk2[m + 1];
+  double v = 0.0;
+  for (int b = b0 + (int)threadIdx.x; b < b1; b += TPB) v = fmax(v, part[2 * b]);
+  const double mx = block_max(v, sh);
+  if (threadIdx.x == 0) {
+    const double scale = mx < 1.0 ? 1.0 : mx;
+    sc[PC_N * m + PC_AUX] = scale;
+    tol[m] = eta_tol / scale;
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(TPB) k_eta_adj_norm(Batch B, T* __restrict__ a, const double* __restrict__ sc) {
+  Pos p;
+  if (!pos2(B, p)) return;
+  const int64_t c = ix2(p, p.i, p.j);
+  a[c] = a[c] / T(sc[PC_N * p.m + PC_AUX]);
+}
+
+// lam = x area scale (the rhs cotangent), nlam = -lam
+template <class T>
+__global__ void __launch_bounds__(TPB) k_eta_adj_out(Batch B, const T* __restrict__ x, const T* __restrict__ art,
+                                                     const double* __restrict__ sc, T* __restrict__ lam,
+                                                     T* __restrict__ nlam) {
+  Pos p;
+  if (!pos2(B, p)) return;
+  const int64_t c = ix2(p, p.i, p.j);
+  const T l = x[c] * adj_area(p, art) * T(sc[PC_N * p.m + PC_AUX]);
+  lam[c] = l;
+  nlam[c] = -l;
+}
+
+// ---------------------------------------------------------------- spectral preconditioner
+// per-member table of the sine matrices and inverse eigenvalues in the cache
+struct SpecMember {
+  int64_t si, sj, l;   // element offsets of S_ni, S_nj, inverse eigenvalues (ni x nj)
+  int32_t ni, nj;
+  double scale;        // 4 (ni + 1)(nj + 1)
+};
+
+// C = A B (+ epilogue) for every active member: A is M x K (lda), B is K x N (ldb),
+// C is M x N (ldc).  mode 0: plain; 1: C *= L (elementwise, ld = N); 2: C /= scale
+template <class T> struct GemmArg {
+  const T* a;        // base pointers; per-member offsets from the descriptors below
+  const T* b;
+  T* c;
+  int a_kind, b_kind, c_kind;   // 0: S_ni, 1: S_nj, 2: member field interior (ld jm), 3: dense scratch (ld nj)
+  int mode;
+};
+
+template <class T>
+__device__ __forceinline__ const T* gemm_ptr(int kind, const T* base, const SpecMember& s, const Member& mm,
+                                             int64_t& ld) {
+  switch (kind) {
+    case 0: ld = s.ni; return base + s.si;
+    case 1: ld = s.nj; return base + s.sj;
+    case 2: ld = mm.jm; return base + mm.off2 + mm.jm + 1;
+    default: ld = s.nj; return base + mm.off2;
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(TPB) k_spec_gemm(const Member* __restrict__ mem, const SpecMember* __restrict__ sp,
+                                                   const T* __restrict__ cache, const int* __restrict__ act,
+                                                   GemmArg<T> g, int left) {
+  // left: C = S_ni X (M = ni, N = nj, K = ni); else C = X S_nj (K = nj)
+  constexpr int TS = 16;
+  __shared__ T As[TS][TS + 1], Bs[TS][TS + 1];
+  const int m = blockIdx.z;
+  if (!act[m]) return;
+  const SpecMember s = sp[m];
+  const Member mm = mem[m];
+  const int M = s.ni, N = s.nj, K = left ? s.ni : s.nj;
+  const int row0 = blockIdx.y * TS, col0 = blockIdx.x * TS;
+  if (row0 >= M || col0 >= N) return;
+  int64_t lda, ldb, ldc;
+  const T* A = gemm_ptr<T>(g.a_kind, g.a_kind < 2 ? cache : g.a, s, mm, lda);
+  const T* Bm = gemm_ptr<T>(g.b_kind, g.b_kind < 2 ? cache : g.b, s, mm, ldb);
+  T* Cm = const_cast<T*>(gemm_ptr<T>(g.c_kind, g.c, s, mm, ldc));
+  const int tr = threadIdx.x / TS, tc = threadIdx.x % TS;
+  const int row = row0 + tr, col = col0 + tc;
+  T acc = T(0);
+  for (int k0 = 0; k0 < K; k0 += TS) {
+    As[tr][tc] = (row < M && k0 + tc < K) ? A[(int64_t)row * lda + k0 + tc] : T(0);
+    Bs[tr][tc] = (k0 + tr < K && col < N) ? Bm[(int64_t)(k0 + tr) * ldb + col] : T(0);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < TS; ++k) acc += As[tr][k] * Bs[k][tc];
+    __syncthreads();
+  }
+  if (row < M && col < N) {
+    if (g.mode == 1) acc *= cache[s.l + (int64_t)row * N + col];
+    else if (g.mode == 2) acc /= T(s.scale);
+    Cm[(int64_t)row * ldc + col] = acc;
+  }
+}
+
+}  // namespace pom

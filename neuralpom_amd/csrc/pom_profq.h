@@ -45,6 +45,7 @@ __device__ __forceinline__ void profq_fwd_col(const Pos& p, const ColIO<T>& io, 
   const int i = p.i, j = p.j, m = p.m;
   const int64_t c2 = ix2(p, i, j);
   const T dti2 = T(prm.dti2), grav = T(prm.grav), kappa = T(prm.kappa), small = T(prm.small);
+  const T weps = T(prm.core ? 1e-8 : 1e-20);   // wall guard: explicit tracers.py:362, implicit :360
   C.dh = s.h[c2] + s.etf[c2];
   const T dh = C.dh;
   T x[KB], y[KB];
@@ -154,7 +155,7 @@ __device__ __forceinline__ void profq_fwd_col(const Pos& p, const ColIO<T>& io, 
   C.ee2[0] = C.gg2[0] = T(0);
   #pragma unroll
   for (int k = 1; k < KBM1; ++k) {
-    const T W = T(1) / (fabs(vg.z(m, k) - vg.z(m, 0)) + T(1e-20)) + T(1) / (fabs(vg.z(m, k) - vg.z(m, KB - 1)) + T(1e-20));
+    const T W = T(1) / (fabs(vg.z(m, k) - vg.z(m, 0)) + weps) + T(1) / (fabs(vg.z(m, k) - vg.z(m, KB - 1)) + weps);
     const T wall = W * C.l[k] / (dh * kappa);
     const T dtk = C.dtef[k] * (T(1) + T(E2) * (wall * wall));
     const T den = C.a[k] + C.c[k] * (T(1) - C.ee2[k - 1]) - (dti2 * dtk + T(1));
@@ -229,6 +230,7 @@ __device__ __forceinline__ void profq_fwd_col_direct(const Pos& p, const Vert<T>
   const int i = p.i, j = p.j, m = p.m;
   const int64_t c = ix3(p, i, j, 0), c2 = ix2(p, i, j);
   const T dti2 = T(prm.dti2), umol = T(prm.umol), grav = T(prm.grav), kappa = T(prm.kappa), small = T(prm.small);
+  const T weps = T(prm.core ? 1e-8 : 1e-20);
   C.dh = s.h[c2] + s.etf[c2];
   const T dh = C.dh;
   #pragma unroll
@@ -322,7 +324,7 @@ __device__ __forceinline__ void profq_fwd_col_direct(const Pos& p, const Vert<T>
   C.ee2[0] = C.gg2[0] = T(0);
   #pragma unroll
   for (int k = 1; k < KBM1; ++k) {
-    const T W = T(1) / (fabs(vg.z(m, k) - vg.z(m, 0)) + T(1e-20)) + T(1) / (fabs(vg.z(m, k) - vg.z(m, KB - 1)) + T(1e-20));
+    const T W = T(1) / (fabs(vg.z(m, k) - vg.z(m, 0)) + weps) + T(1) / (fabs(vg.z(m, k) - vg.z(m, KB - 1)) + weps);
     const T wall = W * C.l[k] / (dh * kappa);
     const T dtk = C.dtef[k] * (T(1) + T(E2) * (wall * wall));
     const T den = C.a[k] + C.c[k] * (T(1) - C.ee2[k - 1]) - (dti2 * dtk + T(1));
@@ -367,6 +369,7 @@ __global__ void __launch_bounds__(TPB) k_profq_adj(Batch B, Vert<T> vg, Params p
   ProfqCol<T> C;
   profq_fwd_col_direct(p, vg, prm, s, C);
   const T dti2 = T(prm.dti2), grav = T(prm.grav), kappa = T(prm.kappa), small = T(prm.small);
+  const T weps = T(prm.core ? 1e-8 : 1e-20);   // wall guard: explicit tracers.py:362, implicit :360
   const T dh = C.dh;
 
   T bl[KB], bgh[KB], bq2n[KB], bq2ln[KB];
@@ -427,7 +430,7 @@ __global__ void __launch_bounds__(TPB) k_profq_adj(Batch B, Vert<T> vg, Params p
     }
     #pragma unroll
     for (int k = KBM1 - 1; k >= 1; --k) {
-      const T W = T(1) / (fabs(vg.z(m, k) - vg.z(m, 0)) + T(1e-20)) + T(1) / (fabs(vg.z(m, k) - vg.z(m, KB - 1)) + T(1e-20));
+      const T W = T(1) / (fabs(vg.z(m, k) - vg.z(m, 0)) + weps) + T(1) / (fabs(vg.z(m, k) - vg.z(m, KB - 1)) + weps);
       const T wall = W * C.l[k] / (dh * kappa);
       const T dtk = C.dtef[k] * (T(1) + T(E2) * (wall * wall));
       const T den = C.a[k] + C.c[k] * (T(1) - C.ee2[k - 1]) - (dti2 * dtk + T(1));

@@ -1,9 +1,11 @@
-// Host orchestration of the explicit POM step and its adjoint, and the C ABI.
+// Host orchestration of the explicit and semi-implicit POM steps and their
+// adjoints, and the C ABI.
 //
-// Forward: the stages of src/neuralpom/cores/explicit/time_loop.py:75-696 in
-// order, every intermediate in its own workspace buffer (static single
-// assignment), so the same forward doubles as the recompute pass of the
-// adjoint.  Adjoint: the stages reversed; stencil stages through the generated
+// Forward: the stages of src/neuralpom/cores/explicit/time_loop.py:75-696
+// (explicit) or src/neuralpom/cores/implicit_p0_fixed/time_loop.py:95-688
+// (semi-implicit) in order, every intermediate in its own workspace buffer
+// (static single assignment), so the same forward doubles as the recompute
+// pass of the adjoint.  Adjoint: the stages reversed; stencil stages through the generated
 // gathers of pom_kernels.h, column stages through the hand-written reverse
 // sweeps of pom_column.h / pom_profq.h.
 #include <vector>
@@ -12,6 +14,8 @@
 
 #include "../../include/neuralpom_amd.h"
 #include "pom_kernels.h"
+#include "pom_ops_imp.h"
+#include "pom_pcg.h"
 #include "pom_profq.h"
 
 using namespace pom;
@@ -24,6 +28,7 @@ struct pom_batch {
   int32_t* d_blk2;
   int32_t nblk3, nblk2;
   int64_t n2, n3;
+  std::vector<Member> h_mem;   // host copy of the member table
 };
 
 namespace {
@@ -143,6 +148,82 @@ template <class T> struct X {   // state of one barotropic substep
       *tps;
 };
 
+// pinned host word for the eta solver's convergence polls (one per process)
+int* pinned_word() {
+  static int* w = nullptr;
+  if (!w && hipHostMalloc(reinterpret_cast<void**>(&w), sizeof(int)) != hipSuccess) w = nullptr;
+  return w;
+}
+
+// work arrays and driver of the batched eta solve
+template <class T> struct Pcg {
+  T *r, *p, *z, *ap, *t1, *t2;
+  double *part, *sc, *tol;
+  int *act, *its, *nact;
+  const SpecMember* sp = nullptr;   // spectral cache (device)
+  const T* cache = nullptr;
+  int maxni = 0, maxnj = 0;
+  bool spectral = false;
+  static constexpr int POLL = 4;    // iterations between convergence polls
+
+  void plan(Arena& A, int64_t N2, int nblk2, int M) {
+    r = A.get<T>(N2); p = A.get<T>(N2); z = A.get<T>(N2); ap = A.get<T>(N2);
+    t1 = A.get<T>(N2); t2 = A.get<T>(N2);
+    part = A.get<double>(2 * (int64_t)nblk2 + 2);
+    sc = A.get<double>((int64_t)PC_N * M);
+    tol = A.get<double>(M);
+    act = A.get<int>(M); its = A.get<int>(M); nact = A.get<int>(1);
+  }
+
+  void precond(const Ctx<T>& c, const EllGeom<T>& g, const T* d) {
+    const dim3 g2(c.nblk2), tb(TPB);
+    if (spectral) {
+      const dim3 gg((maxnj + 15) / 16, (maxni + 15) / 16, c.B.M);
+      GemmArg<T> a1{nullptr, r, t1, 0, 2, 3, 0};    // T1 = S_ni R
+      hipLaunchKernelGGL(k_spec_gemm<T>, gg, tb, 0, c.stream, c.B.mem, sp, cache, act, a1, 1);
+      GemmArg<T> a2{t1, nullptr, t2, 3, 1, 3, 1};   // T2 = (T1 S_nj) * L
+      hipLaunchKernelGGL(k_spec_gemm<T>, gg, tb, 0, c.stream, c.B.mem, sp, cache, act, a2, 0);
+      GemmArg<T> a3{nullptr, t2, t1, 0, 3, 3, 0};   // T1 = S_ni T2
+      hipLaunchKernelGGL(k_spec_gemm<T>, gg, tb, 0, c.stream, c.B.mem, sp, cache, act, a3, 1);
+      GemmArg<T> a4{t1, nullptr, z, 3, 1, 2, 2};    // Z = T1 S_nj / scale (interior)
+      hipLaunchKernelGGL(k_spec_gemm<T>, gg, tb, 0, c.stream, c.B.mem, sp, cache, act, a4, 0);
+      hipLaunchKernelGGL(k_pcg_dot<T>, g2, tb, 0, c.stream, c.B, r, z, act, part, 0);
+    } else {
+      hipLaunchKernelGGL(k_pcg_jacobi<T>, g2, tb, 0, c.stream, c.B, g, d, r, z, act, part);
+    }
+  }
+
+  // x = A(d)^-1 rhs from x0 (null: zero), per-member absolute tolerance tol[]
+  // (device, set by the caller) and relative tolerance rel_tol
+  void solve(const Ctx<T>& c, const EllGeom<T>& g, const T* rhs, const T* d, const T* x0, T* x, double rel_tol,
+             int max_iter) {
+    const dim3 g2(c.nblk2), tb(TPB), gm(c.B.M);
+    hipLaunchKernelGGL(k_pcg_x0<T>, g2, tb, 0, c.stream, c.B, x0, x, z);
+    hipLaunchKernelGGL(k_pcg_resid<T>, g2, tb, 0, c.stream, c.B, g, d, x, rhs, r, p);
+    hipLaunchKernelGGL(k_pcg_act1, dim3(1), tb, 0, c.stream, act, c.B.M);
+    precond(c, g, d);
+    hipLaunchKernelGGL(k_pcg_dot<T>, g2, tb, 0, c.stream, c.B, r, r, act, part, 1);
+    hipLaunchKernelGGL(k_pcg_start, gm, tb, 0, c.stream, c.B, part, sc, act, its, tol, rel_tol);
+    hipLaunchKernelGGL(k_pcg_p<T>, g2, tb, 0, c.stream, c.B, p, z, sc, act, 1);
+    int* hw = pinned_word();
+    for (int it = 1; it <= max_iter; ++it) {
+      hipLaunchKernelGGL(k_pcg_ap<T>, g2, tb, 0, c.stream, c.B, g, d, p, ap, act, part);
+      hipLaunchKernelGGL(k_pcg_alpha, gm, tb, 0, c.stream, c.B, part, sc, act, nact);
+      hipLaunchKernelGGL(k_pcg_xr<T>, g2, tb, 0, c.stream, c.B, x, r, p, ap, sc, act, part);
+      hipLaunchKernelGGL(k_pcg_conv, gm, tb, 0, c.stream, c.B, part, sc, act, its, nact);
+      if (hw && (it % POLL == 0 || it == max_iter)) {
+        if (hipMemcpyAsync(hw, nact, sizeof(int), hipMemcpyDeviceToHost, c.stream) != hipSuccess) return;
+        if (hipStreamSynchronize(c.stream) != hipSuccess) return;
+        if (*hw == 0) break;
+      }
+      if (it == max_iter) break;
+      precond(c, g, d);
+      hipLaunchKernelGGL(k_pcg_beta, gm, tb, 0, c.stream, c.B, part, sc, act);
+      hipLaunchKernelGGL(k_pcg_p<T>, g2, tb, 0, c.stream, c.B, p, z, sc, act, 0);
+    }
+  }
+};
+
 template <class T> struct Step {
   const pom_batch* b;
   Ctx<T> c;
@@ -155,6 +236,14 @@ template <class T> struct Step {
   // intermediates
   T *rho1, *adx2d_raw, *ady2d_raw, *advua0, *advva0, *fluxua0, *fluxva0, *tps0, *egf0, *utf0, *vtf0, *elfs;
   X<T> xs[MAXSPLIT + 1];
+  // semi-implicit core
+  bool imp = false;
+  const pom_params_t* hp = nullptr;   // host parameters (eta solver settings)
+  const void* icache = nullptr;       // spectral preconditioner cache (pom_implicit_prepare)
+  int32_t* iters_out = nullptr;       // per-member eta-solve iterations (device), may be null
+  X<T> xi;                            // depth-mean state after the semi-implicit barotropic step
+  T *us, *vs, *dus, *dvs, *rhs, *elfraw;
+  Pcg<T> pw;
   T *u1, *v1, *q2f0, *q2lf0, *q2f1, *q2lf1, *kmraw, *khraw, *tf0, *sf0, *tf1, *sf1, *uf0, *vf0, *uf1, *vf1, *uf2,
       *vf2, *tpsu;
 
@@ -177,7 +266,18 @@ template <class T> struct Step {
     utf0 = A.get<T>(N2);
     vtf0 = A.get<T>(N2);
     elfs = A.get<T>(N2);
-    for (int n = 1; n <= isplit; ++n) {
+    if (imp) {
+      us = A.get<T>(N2); vs = A.get<T>(N2); dus = A.get<T>(N2); dvs = A.get<T>(N2);
+      rhs = A.get<T>(N2); elfraw = A.get<T>(N2);
+      pw.plan(A, N2, b->nblk2, b->M);
+      xi = X<T>{};
+      xi.ua = out[POM_UA]; xi.uab = out[POM_UAB]; xi.va = out[POM_VA]; xi.vab = out[POM_VAB];
+      xi.el = out[POM_EL]; xi.elb = out[POM_ELB]; xi.d = out[POM_D]; xi.egf = out[POM_EGF];
+      xi.utf = out[POM_UTF]; xi.vtf = out[POM_VTF]; xi.etf = out[POM_ETF]; xi.uaf = out[POM_UAF];
+      xi.vaf = out[POM_VAF]; xi.elf = out[POM_ELF]; xi.advua = out[POM_ADVUA]; xi.advva = out[POM_ADVVA];
+      xi.fluxua = out[POM_FLUXUA]; xi.fluxva = out[POM_FLUXVA]; xi.tps = tps0;
+    }
+    for (int n = 1; n <= (imp ? 0 : isplit); ++n) {
       X<T>& x = xs[n];
       const bool last = n == isplit;
       x.ua = last ? out[POM_UA] : A.get<T>(N2);
@@ -391,7 +491,92 @@ template <class T> struct Step {
     t.g[OpExtFilter::GFSM] = g[POM_G_FSM];
     return t;
   }
-  const X<T>& xf() const { return xs[isplit]; }
+  const X<T>& xf() const { return imp ? xi : xs[isplit]; }
+  EllGeom<T> ell() const {
+    return EllGeom<T>{g[POM_G_DX], g[POM_G_DY], g[POM_G_ART], g[POM_G_DUM], g[POM_G_DVM],
+                      4.0 * (c.prm.dti * c.prm.dti) * c.prm.grav * c.prm.alpha};
+  }
+  // ---- semi-implicit tables
+  FieldTab<T> t_pred() const {
+    FieldTab<T> t{};
+    t.f[OpImpPred::UA] = in[POM_UA]; t.f[OpImpPred::VA] = in[POM_VA]; t.f[OpImpPred::EL] = in[POM_EL];
+    t.f[OpImpPred::ELB] = in[POM_ELB]; t.f[OpImpPred::D_] = in[POM_D]; t.f[OpImpPred::UAB] = in[POM_UAB];
+    t.f[OpImpPred::VAB] = in[POM_VAB]; t.f[OpImpPred::ADX2D] = out[POM_ADX2D];
+    t.f[OpImpPred::ADY2D] = out[POM_ADY2D]; t.f[OpImpPred::ADVUA] = advua0; t.f[OpImpPred::ADVVA] = advva0;
+    t.f[OpImpPred::DRX2D] = out[POM_DRX2D]; t.f[OpImpPred::DRY2D] = out[POM_DRY2D];
+    t.f[OpImpPred::WUBOT] = in[POM_WUBOT]; t.f[OpImpPred::WVBOT] = in[POM_WVBOT];
+    t.g[OpImpPred::GCOR] = g[POM_G_COR]; t.g[OpImpPred::GDX] = g[POM_G_DX]; t.g[OpImpPred::GDY] = g[POM_G_DY];
+    t.g[OpImpPred::GH] = g[POM_G_H]; t.g[OpImpPred::GARU] = g[POM_G_ARU]; t.g[OpImpPred::GARV] = g[POM_G_ARV];
+    t.g[OpImpPred::GDUM] = g[POM_G_DUM]; t.g[OpImpPred::GDVM] = g[POM_G_DVM];
+    t.g[OpImpPred::GEATM] = out[POM_E_ATMOS]; t.g[OpImpPred::GWUSURF] = g[POM_G_WUSURF];
+    t.g[OpImpPred::GWVSURF] = g[POM_G_WVSURF];
+    return t;
+  }
+  FieldTab<T> t_rhs() const {
+    FieldTab<T> t{};
+    t.f[OpImpRhs::US] = us; t.f[OpImpRhs::VS] = vs; t.f[OpImpRhs::D_] = in[POM_D]; t.f[OpImpRhs::ELB] = in[POM_ELB];
+    t.g[OpImpRhs::GDX] = g[POM_G_DX]; t.g[OpImpRhs::GDY] = g[POM_G_DY]; t.g[OpImpRhs::GART] = g[POM_G_ART];
+    t.g[OpImpRhs::GVFLUXF] = out[POM_VFLUXF];
+    return t;
+  }
+  FieldTab<T> t_surf() const {
+    FieldTab<T> t{};
+    t.f[OpImpSurf::RAW] = elfraw;
+    t.g[OpImpSurf::GFSM] = g[POM_G_FSM];
+    return t;
+  }
+  FieldTab<T> t_corr() const {
+    FieldTab<T> t{};
+    t.f[OpImpCorr::US] = us; t.f[OpImpCorr::VS] = vs; t.f[OpImpCorr::DUS] = dus; t.f[OpImpCorr::DVS] = dvs;
+    t.f[OpImpCorr::ELF] = out[POM_ELF];
+    t.g[OpImpCorr::GDX] = g[POM_G_DX]; t.g[OpImpCorr::GDY] = g[POM_G_DY]; t.g[OpImpCorr::GH] = g[POM_G_H];
+    t.g[OpImpCorr::GARU] = g[POM_G_ARU]; t.g[OpImpCorr::GARV] = g[POM_G_ARV];
+    t.g[OpImpCorr::GDUM] = g[POM_G_DUM]; t.g[OpImpCorr::GDVM] = g[POM_G_DVM];
+    return t;
+  }
+  FieldTab<T> t_ifilter() const {
+    FieldTab<T> t{};
+    t.f[OpImpFilter::UA] = in[POM_UA]; t.f[OpImpFilter::UAB] = in[POM_UAB]; t.f[OpImpFilter::VA] = in[POM_VA];
+    t.f[OpImpFilter::VAB] = in[POM_VAB]; t.f[OpImpFilter::EL] = in[POM_EL]; t.f[OpImpFilter::ELB] = in[POM_ELB];
+    t.f[OpImpFilter::UAF] = out[POM_UAF]; t.f[OpImpFilter::VAF] = out[POM_VAF];
+    t.f[OpImpFilter::ELF] = out[POM_ELF]; t.f[OpImpFilter::UTF0] = utf0; t.f[OpImpFilter::VTF0] = vtf0;
+    t.g[OpImpFilter::GH] = g[POM_G_H]; t.g[OpImpFilter::GFSM] = g[POM_G_FSM];
+    return t;
+  }
+  FieldTab<T> t_ell() const {
+    FieldTab<T> t{};
+    t.f[OpEll::ETA] = elfraw; t.f[OpEll::D_] = in[POM_D];
+    t.g[OpEll::GDX] = g[POM_G_DX]; t.g[OpEll::GDY] = g[POM_G_DY]; t.g[OpEll::GART] = g[POM_G_ART];
+    t.g[OpEll::GDUM] = g[POM_G_DUM]; t.g[OpEll::GDVM] = g[POM_G_DVM];
+    return t;
+  }
+  FieldTab<T> t_ibc6() const {
+    FieldTab<T> t{};
+    t.f[OpImpBc6Filter::Q2F] = q2f1; t.f[OpImpBc6Filter::Q2LF] = q2lf1; t.f[OpImpBc6Filter::Q2] = in[POM_Q2];
+    t.f[OpImpBc6Filter::Q2L] = in[POM_Q2L]; t.f[OpImpBc6Filter::Q2B] = in[POM_Q2B];
+    t.f[OpImpBc6Filter::Q2LB] = in[POM_Q2LB];
+    t.g[OpImpBc6Filter::GFSM] = g[POM_G_FSM];
+    return t;
+  }
+  FieldTab<T> t_ibc4() const {
+    FieldTab<T> t{};
+    t.f[OpImpBc4Filter::TF] = tf1; t.f[OpImpBc4Filter::SF] = sf1; t.f[OpImpBc4Filter::TT] = in[POM_T];
+    t.f[OpImpBc4Filter::SS] = in[POM_S]; t.f[OpImpBc4Filter::TB] = in[POM_TB]; t.f[OpImpBc4Filter::SB] = in[POM_SB];
+    t.g[OpImpBc4Filter::GFSM] = g[POM_G_FSM];
+    return t;
+  }
+  FieldTab<T> t_ibc3() const {
+    FieldTab<T> t{};
+    t.f[OpImpBc3::UF] = uf1; t.f[OpImpBc3::VF] = vf1;
+    t.g[OpImpBc3::GDUM] = g[POM_G_DUM]; t.g[OpImpBc3::GDVM] = g[POM_G_DVM];
+    return t;
+  }
+  FieldTab<T> t_smag_imp() const {
+    FieldTab<T> t{};
+    t.f[OpSmagImp::U] = in[POM_U]; t.f[OpSmagImp::V] = in[POM_V]; t.f[OpSmagImp::AAM] = in[POM_AAM];
+    t.g[OpSmagImp::GDX] = g[POM_G_DX]; t.g[OpSmagImp::GDY] = g[POM_G_DY];
+    return t;
+  }
   FieldTab<T> t_advq(bool l2) const {
     FieldTab<T> t{};
     t.f[OpAdvq::QB] = in[l2 ? POM_Q2LB : POM_Q2B];
@@ -609,8 +794,7 @@ template <class T> struct Step {
 
   // ------------------------------------------------------------ forward
   void forward() {
-    const int64_t N2 = n2(), N3 = n3();
-    const dim3 g2(c.nblk2), g3(c.nblk3), tb(TPB);
+    const dim3 g2(c.nblk2), tb(TPB);
     hipLaunchKernelGGL(k_fill<T>, dim3(1), dim3(64), 0, c.stream, vamax, (int64_t)1, T(0));
     {   // surface forcing reset (time_loop.py:118-164)
       ForcingArgs<T> a{in[POM_E_ATMOS], in[POM_VFLUXF], in[POM_SWRAD], in[POM_WTSURF], in[POM_WSSURF],
@@ -628,7 +812,8 @@ template <class T> struct Step {
                       g[POM_G_DVM], rho1, out[POM_DRHOX], out[POM_DRHOY]};
       hipLaunchKernelGGL(k_baropg<T>, g2, tb, 0, c.stream, c.B, c.vert, c.ramp, c.prm, a);
     }
-    launch_map<T, OpSmag>(c, t_smag(), OutTab<T>{{out[POM_AAM]}});
+    if (imp) launch_map<T, OpSmagImp>(c, t_smag_imp(), OutTab<T>{{out[POM_AAM]}});
+    else launch_map<T, OpSmag>(c, t_smag(), OutTab<T>{{out[POM_AAM]}});
     {
       ColsumArgs<T> a{};
       a.f[0] = out[POM_ADVX]; a.out[0] = adx2d_raw;
@@ -649,7 +834,38 @@ template <class T> struct Step {
     }
     x0_ = x0();
     launch_map<T, OpExtPrep>(c, t_prep(), OutTab<T>{{out[POM_ADX2D], out[POM_ADY2D], egf0, utf0, vtf0}});
-    // barotropic substeps (time_loop.py:292-431)
+    if (imp) {
+      external_implicit();
+    } else {
+      external_explicit();
+    }
+    internal_and_shifts();
+  }
+
+  // semi-implicit barotropic step (implicit_p0_fixed/time_loop.py:247-398)
+  void external_implicit() {
+    const int64_t N2 = n2();
+    launch_map<T, OpImpPred>(c, t_pred(), OutTab<T>{{us, vs, dus, dvs}});
+    launch_map<T, OpImpRhs>(c, t_rhs(), OutTab<T>{{out[POM_FLUXUA], out[POM_FLUXVA], rhs}});
+    hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(256), 0, c.stream, pw.tol, (int64_t)b->M, hp->eta_tol);
+    pw.solve(c, ell(), rhs, in[POM_D], in[POM_ELF], elfraw, hp->eta_rel_tol, hp->eta_max_iter);
+    if (iters_out)
+      hipLaunchKernelGGL(k_copy<int32_t>, dim3(1), dim3(256), 0, c.stream, iters_out, pw.its, (int64_t)b->M);
+    launch_map<T, OpImpSurf>(c, t_surf(), OutTab<T>{{out[POM_ELF]}});
+    launch_map<T, OpImpCorr>(c, t_corr(), OutTab<T>{{out[POM_UAF], out[POM_VAF]}});
+    hipLaunchKernelGGL(k_vamax<T>, dim3(std::min(blocks(N2), 1024)), dim3(256), 0, c.stream, out[POM_VAF], N2,
+                       vamax);
+    launch_map<T, OpImpFilter>(c, t_ifilter(),
+                               OutTab<T>{{out[POM_UA], out[POM_UAB], out[POM_VA], out[POM_VAB], out[POM_EL],
+                                          out[POM_ELB], out[POM_D], out[POM_EGF], out[POM_UTF], out[POM_VTF],
+                                          out[POM_ETF]}});
+    copy(out[POM_ADVUA], advua0, N2);
+    copy(out[POM_ADVVA], advva0, N2);
+  }
+
+  // explicit barotropic substeps (explicit/time_loop.py:292-431)
+  void external_explicit() {
+    const int64_t N2 = n2();
     for (int n = 1; n <= isplit; ++n) {
       const X<T>& xp = n == 1 ? x0_ : xs[n - 1];
       X<T>& xn = xs[n];
@@ -677,6 +893,12 @@ template <class T> struct Step {
     copy(out[POM_ADVVA], xl.advva, N2);
     copy(out[POM_FLUXUA], xl.fluxua, N2);
     copy(out[POM_FLUXVA], xl.fluxva, N2);
+  }
+
+  void internal_and_shifts() {
+    const int64_t N2 = n2(), N3 = n3();
+    const dim3 g2(c.nblk2), g3(c.nblk3), tb(TPB);
+    const X<T>& xl = xf();
     if (internal) {
       {
         AdjustArgs<T> a{in[POM_U], in[POM_V], in[POM_UTB], xl.utf, in[POM_VTB], xl.vtf, in[POM_DT], u1, v1};
@@ -690,22 +912,30 @@ template <class T> struct Step {
         hipLaunchKernelGGL(k_profq<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a_profq());
       }
       launch_map<T, OpKEdge>(c, t_kedge(), OutTab<T>{{out[POM_KM], out[POM_KH]}});
-      launch_map<T, OpBc6Filter>(c, t_bc6(), OutTab<T>{{out[POM_Q2], out[POM_Q2B], out[POM_Q2L], out[POM_Q2LB]}});
+      if (imp)
+        launch_map<T, OpImpBc6Filter>(c, t_ibc6(),
+                                      OutTab<T>{{out[POM_Q2], out[POM_Q2B], out[POM_Q2L], out[POM_Q2LB]}});
+      else
+        launch_map<T, OpBc6Filter>(c, t_bc6(), OutTab<T>{{out[POM_Q2], out[POM_Q2B], out[POM_Q2L], out[POM_Q2LB]}});
       launch_map<T, OpAdvt1>(c, t_advt1(false), OutTab<T>{{tf0}});
       launch_map<T, OpAdvt1>(c, t_advt1(true), OutTab<T>{{sf0}});
       hipLaunchKernelGGL(k_proft<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a_proft(false));
       hipLaunchKernelGGL(k_proft<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a_proft(true));
-      launch_map<T, OpBc4Filter>(c, t_bc4(), OutTab<T>{{out[POM_T], out[POM_TB], out[POM_S], out[POM_SB]}});
+      if (imp)
+        launch_map<T, OpImpBc4Filter>(c, t_ibc4(), OutTab<T>{{out[POM_T], out[POM_TB], out[POM_S], out[POM_SB]}});
+      else
+        launch_map<T, OpBc4Filter>(c, t_bc4(), OutTab<T>{{out[POM_T], out[POM_TB], out[POM_S], out[POM_SB]}});
       launch_map<T, OpDens>(c, t_dens(), OutTab<T>{{out[POM_RHO]}});
       launch_map<T, OpAdvu>(c, t_advu(), OutTab<T>{{uf0}});
       launch_map<T, OpAdvv>(c, t_advv(), OutTab<T>{{vf0}});
       hipLaunchKernelGGL((k_profuv<T, 0>), g2, tb, 0, c.stream, c.B, c.vert, c.prm, a_profuv(false));
       hipLaunchKernelGGL((k_profuv<T, 1>), g2, tb, 0, c.stream, c.B, c.vert, c.prm, a_profuv(true));
-      launch_map<T, OpBc3>(c, t_bc3(), OutTab<T>{{uf2, vf2}});
+      if (imp) launch_map<T, OpImpBc3>(c, t_ibc3(), OutTab<T>{{uf2, vf2}});
+      else launch_map<T, OpBc3>(c, t_bc3(), OutTab<T>{{uf2, vf2}});
       {
-        UVFiltArgs<T> a{uf2, in[POM_UB], u1, out[POM_UB], out[POM_U]};
+        UVFiltArgs<T> a{uf2, in[POM_UB], u1, out[POM_UB], out[POM_U], imp ? g[POM_G_DUM] : nullptr};
         hipLaunchKernelGGL(k_uvfilt<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a);
-        UVFiltArgs<T> bv{vf2, in[POM_VB], v1, out[POM_VB], out[POM_V]};
+        UVFiltArgs<T> bv{vf2, in[POM_VB], v1, out[POM_VB], out[POM_V], imp ? g[POM_G_DVM] : nullptr};
         hipLaunchKernelGGL(k_uvfilt<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, bv);
       }
     } else {
@@ -742,6 +972,8 @@ template <class T> struct Adj {
   T *b_uf0, *b_vf0, *b_uf1, *b_vf1, *b_uf2, *b_vf2;
   // scratch
   T *s3a, *s3b, *s3c, *s3d, *s2a, *s2b, *s2c, *s2d, *s2e;
+  // semi-implicit barotropic step
+  T *b_us = nullptr, *b_vs, *b_dus, *b_dvs, *b_raw, *b_rhs, *nlam, *arhs, *xadj;
 
   Adj(Step<T>& s, Ctx<T>& cc) : S(s), c(cc) {}
 
@@ -775,6 +1007,10 @@ template <class T> struct Adj {
     b_uf2 = A.get<T>(N3); b_vf2 = A.get<T>(N3);
     s3a = A.get<T>(N3); s3b = A.get<T>(N3); s3c = A.get<T>(N3); s3d = A.get<T>(N3);
     s2a = A.get<T>(N2); s2b = A.get<T>(N2); s2c = A.get<T>(N2); s2d = A.get<T>(N2); s2e = A.get<T>(N2);
+    if (S.imp) {
+      b_us = A.get<T>(N2); b_vs = A.get<T>(N2); b_dus = A.get<T>(N2); b_dvs = A.get<T>(N2);
+      b_raw = A.get<T>(N2); b_rhs = A.get<T>(N2); nlam = A.get<T>(N2); arhs = A.get<T>(N2); xadj = A.get<T>(N2);
+    }
   }
 
   void zero(T* p, int64_t n) {
@@ -821,8 +1057,9 @@ template <class T> struct Adj {
                  b_uf0, b_vf0, b_uf1, b_vf1, b_uf2, b_vf2})
       zq(p, N3);
     for (T* p : {b_adx2d_raw, b_ady2d_raw, b_advua0, b_advva0, b_egf0, b_utf0, b_vtf0}) zq(p, N2);
+    if (S.imp)
+      for (T* p : {b_us, b_vs, b_dus, b_dvs, b_raw}) zq(p, N2);
     zflush();
-    const X<T>& xl = S.xs[S.isplit];
     // ---- shifts reversed: egb<-egf, etb<-et, et<-etf, dt<-h+et, utb<-utf, vtb<-vtf
     X<T>& cur = bx[S.isplit & 1];
     zero_x(cur, N2);
@@ -848,56 +1085,60 @@ template <class T> struct Adj {
       add(bi[POM_WUBOT], bo[POM_WUBOT], N2);
       add(bi[POM_WVBOT], bo[POM_WVBOT], N2);
     }
-    // ---- substeps reversed
-    for (int n = S.isplit; n >= 1; --n) {
-      X<T>& bn = bx[n & 1];
-      X<T>& bp = bx[(n - 1) & 1];
-      zero_x(bp, N2);
-      const X<T>& xp = n == 1 ? S.x0_ : S.xs[n - 1];
-      const X<T>& xn = S.xs[n];
-      Ctx<T> cn = c;
-      cn.prm.iext = n;
-      // filter: outputs (ua uab va vab el elb d egf utf vtf etf)
-      {
-        COutTab<T> ob{{bn.ua, bn.uab, bn.va, bn.vab, bn.el, bn.elb, bn.d, bn.egf, bn.utf, bn.vtf, bn.etf}};
-        T* ib[OpExtFilter::NIN] = {bp.ua, bp.uab, bp.va, bp.vab, bp.el, bp.elb, bn.uaf, bn.vaf, bn.elf,
-                                   bp.egf, bp.utf, bp.vtf, bp.etf};
-        launch_gather<T, OpExtFilter>(cn, S.t_filter(xp, xn), ob, ib);
+    if (S.imp) {
+      external_implicit_rev(cur);
+    } else {
+      // ---- substeps reversed
+      for (int n = S.isplit; n >= 1; --n) {
+        X<T>& bn = bx[n & 1];
+        X<T>& bp = bx[(n - 1) & 1];
+        zero_x(bp, N2);
+        const X<T>& xp = n == 1 ? S.x0_ : S.xs[n - 1];
+        const X<T>& xn = S.xs[n];
+        Ctx<T> cn = c;
+        cn.prm.iext = n;
+        // filter: outputs (ua uab va vab el elb d egf utf vtf etf)
+        {
+          COutTab<T> ob{{bn.ua, bn.uab, bn.va, bn.vab, bn.el, bn.elb, bn.d, bn.egf, bn.utf, bn.vtf, bn.etf}};
+          T* ib[OpExtFilter::NIN] = {bp.ua, bp.uab, bp.va, bp.vab, bp.el, bp.elb, bn.uaf, bn.vaf, bn.elf,
+                                     bp.egf, bp.utf, bp.vtf, bp.etf};
+          launch_gather<T, OpExtFilter>(cn, S.t_filter(xp, xn), ob, ib);
+        }
+        // depth-mean momentum
+        {
+          COutTab<T> ob{{bn.uaf, bn.vaf}};
+          T* ib[OpExtUV::NIN] = {bp.el, bp.elb, bn.elf, bp.d, bp.ua, bp.va, bp.uab, bp.vab, bn.advua, bn.advva,
+                                 bo[POM_ADX2D], bo[POM_ADY2D], bo[POM_DRX2D], bo[POM_DRY2D], bi[POM_WUBOT],
+                                 bi[POM_WVBOT], bp.uaf, bp.vaf};
+          launch_gather<T, OpExtUV>(cn, S.t_uv(xp, xn), ob, ib);
+        }
+        if (n % c.prm.ispadv == 0) {
+          COutTab<T> ob{{bn.advua, bn.advva, nullptr, nullptr, nullptr}};
+          T* ib[OpAdvave::NIN] = {bp.ua, bp.va, bp.uab, bp.vab, bp.d, bo[POM_AAM2D]};
+          launch_gather<T, OpAdvave>(cn, S.t_advave(xp), ob, ib);
+        } else {
+          add(bp.advua, bn.advua, N2);
+          add(bp.advva, bn.advva, N2);
+        }
+        {
+          zero(s2d, N2);
+          COutTab<T> ob{{bn.elf}};
+          T* ib[OpBc1::NIN] = {s2d};
+          launch_gather<T, OpBc1>(cn, S.t_bc1(), ob, ib);
+          COutTab<T> ob2{{s2d}};
+          T* ib2[OpExtElfRaw::NIN] = {bp.ua, bp.va, bp.d, bp.elb};
+          launch_gather<T, OpExtElfRaw>(cn, S.t_elf(xp), ob2, ib2);
+        }
       }
-      // depth-mean momentum
-      {
-        COutTab<T> ob{{bn.uaf, bn.vaf}};
-        T* ib[OpExtUV::NIN] = {bp.el, bp.elb, bn.elf, bp.d, bp.ua, bp.va, bp.uab, bp.vab, bn.advua, bn.advva,
-                               bo[POM_ADX2D], bo[POM_ADY2D], bo[POM_DRX2D], bo[POM_DRY2D], bi[POM_WUBOT],
-                               bi[POM_WVBOT], bp.uaf, bp.vaf};
-        launch_gather<T, OpExtUV>(cn, S.t_uv(xp, xn), ob, ib);
-      }
-      if (n % c.prm.ispadv == 0) {
-        COutTab<T> ob{{bn.advua, bn.advva, nullptr, nullptr, nullptr}};
-        T* ib[OpAdvave::NIN] = {bp.ua, bp.va, bp.uab, bp.vab, bp.d, bo[POM_AAM2D]};
-        launch_gather<T, OpAdvave>(cn, S.t_advave(xp), ob, ib);
-      } else {
-        add(bp.advua, bn.advua, N2);
-        add(bp.advva, bn.advva, N2);
-      }
-      {
-        zero(s2d, N2);
-        COutTab<T> ob{{bn.elf}};
-        T* ib[OpBc1::NIN] = {s2d};
-        launch_gather<T, OpBc1>(cn, S.t_bc1(), ob, ib);
-        COutTab<T> ob2{{s2d}};
-        T* ib2[OpExtElfRaw::NIN] = {bp.ua, bp.va, bp.d, bp.elb};
-        launch_gather<T, OpExtElfRaw>(cn, S.t_elf(xp), ob2, ib2);
-      }
+      X<T>& b0 = bx[0];
+      // X_0 -> inputs and pre-loop intermediates
+      add(bi[POM_UA], b0.ua, N2); add(bi[POM_UAB], b0.uab, N2); add(bi[POM_VA], b0.va, N2);
+      add(bi[POM_VAB], b0.vab, N2); add(bi[POM_EL], b0.el, N2); add(bi[POM_ELB], b0.elb, N2);
+      add(bi[POM_D], b0.d, N2); add(bi[POM_ETF], b0.etf, N2); add(bi[POM_UAF], b0.uaf, N2);
+      add(bi[POM_VAF], b0.vaf, N2); add(bi[POM_ELF], b0.elf, N2);
+      add(b_egf0, b0.egf, N2); add(b_utf0, b0.utf, N2); add(b_vtf0, b0.vtf, N2);
+      add(b_advua0, b0.advua, N2); add(b_advva0, b0.advva, N2);
     }
-    X<T>& b0 = bx[0];
-    // X_0 -> inputs and pre-loop intermediates
-    add(bi[POM_UA], b0.ua, N2); add(bi[POM_UAB], b0.uab, N2); add(bi[POM_VA], b0.va, N2);
-    add(bi[POM_VAB], b0.vab, N2); add(bi[POM_EL], b0.el, N2); add(bi[POM_ELB], b0.elb, N2);
-    add(bi[POM_D], b0.d, N2); add(bi[POM_ETF], b0.etf, N2); add(bi[POM_UAF], b0.uaf, N2);
-    add(bi[POM_VAF], b0.vaf, N2); add(bi[POM_ELF], b0.elf, N2);
-    add(b_egf0, b0.egf, N2); add(b_utf0, b0.utf, N2); add(b_vtf0, b0.vtf, N2);
-    add(b_advua0, b0.advua, N2); add(b_advva0, b0.advva, N2);
     // pre-loop: adx2d = raw - advua0 etc.
     {
       COutTab<T> ob{{bo[POM_ADX2D], bo[POM_ADY2D], b_egf0, b_utf0, b_vtf0}};
@@ -927,7 +1168,8 @@ template <class T> struct Adj {
     {
       COutTab<T> ob{{bo[POM_AAM]}};
       T* ib[OpSmag::NIN] = {bi[POM_U], bi[POM_V], bi[POM_AAM]};
-      launch_gather<T, OpSmag>(c, S.t_smag(), ob, ib);
+      if (S.imp) launch_gather<T, OpSmagImp>(c, S.t_smag_imp(), ob, ib);
+      else launch_gather<T, OpSmag>(c, S.t_smag(), ob, ib);
     }
     {
       BaropgAdjArgs<T> a{S.in[POM_RHO], S.g[POM_G_RMEAN], S.in[POM_DT], S.g[POM_G_DX], S.g[POM_G_DY],
@@ -942,17 +1184,70 @@ template <class T> struct Adj {
     }
   }
 
+  // semi-implicit barotropic step reversed (implicit_p0_fixed/time_loop.py:247-398;
+  // eta-solve adjoint as core.py:45-59, 231-252, 645-675)
+  void external_implicit_rev(X<T>& cur) {
+    const int64_t N2 = S.n2();
+    const dim3 g2(c.nblk2), tb(TPB), gm(c.B.M);
+    {   // filter, level shift, endpoint transports
+      COutTab<T> ob{{cur.ua, cur.uab, cur.va, cur.vab, cur.el, cur.elb, cur.d, cur.egf, cur.utf, cur.vtf, cur.etf}};
+      T* ib[OpImpFilter::NIN] = {bi[POM_UA], bi[POM_UAB], bi[POM_VA], bi[POM_VAB], bi[POM_EL], bi[POM_ELB],
+                                 cur.uaf, cur.vaf, cur.elf, b_utf0, b_vtf0};
+      launch_gather<T, OpImpFilter>(c, S.t_ifilter(), ob, ib);
+    }
+    {   // corrector
+      COutTab<T> ob{{cur.uaf, cur.vaf}};
+      T* ib[OpImpCorr::NIN] = {b_us, b_vs, b_dus, b_dvs, cur.elf};
+      launch_gather<T, OpImpCorr>(c, S.t_corr(), ob, ib);
+    }
+    {   // post-solve surface boundary
+      COutTab<T> ob{{cur.elf}};
+      T* ib[OpImpSurf::NIN] = {b_raw};
+      launch_gather<T, OpImpSurf>(c, S.t_surf(), ob, ib);
+    }
+    {   // eta solve: b_rhs = area A^-1 (b_raw / area), b_d -= (dA/dD elfraw)^T b_rhs
+      Pcg<T>& w = S.pw;
+      hipLaunchKernelGGL(k_eta_adj_in<T>, g2, tb, 0, c.stream, c.B, b_raw, S.g[POM_G_ART], arhs, w.part);
+      hipLaunchKernelGGL(k_eta_adj_scale, gm, tb, 0, c.stream, c.B, w.part, w.sc, w.tol, S.hp->eta_tol);
+      hipLaunchKernelGGL(k_eta_adj_norm<T>, g2, tb, 0, c.stream, c.B, arhs, w.sc);
+      w.solve(c, S.ell(), arhs, S.in[POM_D], nullptr, xadj, S.hp->eta_rel_tol, S.hp->eta_max_iter);
+      hipLaunchKernelGGL(k_eta_adj_out<T>, g2, tb, 0, c.stream, c.B, xadj, S.g[POM_G_ART], w.sc, b_rhs, nlam);
+      COutTab<T> ob{{nlam}};
+      T* ib[OpEll::NIN] = {nullptr, bi[POM_D]};
+      launch_gather<T, OpEll>(c, S.t_ell(), ob, ib);
+    }
+    {   // transports and right-hand side
+      COutTab<T> ob{{nullptr, nullptr, b_rhs}};
+      T* ib[OpImpRhs::NIN] = {b_us, b_vs, bi[POM_D], bi[POM_ELB]};
+      launch_gather<T, OpImpRhs>(c, S.t_rhs(), ob, ib);
+    }
+    {   // predictor
+      COutTab<T> ob{{b_us, b_vs, b_dus, b_dvs}};
+      T* ib[OpImpPred::NIN] = {bi[POM_UA], bi[POM_VA], bi[POM_EL], bi[POM_ELB], bi[POM_D], bi[POM_UAB], bi[POM_VAB],
+                               bo[POM_ADX2D], bo[POM_ADY2D], b_advua0, b_advva0, bo[POM_DRX2D], bo[POM_DRY2D],
+                               bi[POM_WUBOT], bi[POM_WVBOT]};
+      launch_gather<T, OpImpPred>(c, S.t_pred(), ob, ib);
+    }
+    // the step's advua / advva are copies of the single advave call's
+    add(b_advua0, cur.advua, N2);
+    add(b_advva0, cur.advva, N2);
+  }
+
   void internal_rev(T* b_etf30, T* b_egf30, T* b_utf30, T* b_vtf30) {
     const dim3 g2(c.nblk2), tb(TPB);
     Step<T>& s = S;
     // u/v Asselin filter and shift
     {
-      UVFiltAdjArgs<T> a{bo[POM_UB], bo[POM_U], b_uf2, bi[POM_UB], b_u1};
+      UVFiltAdjArgs<T> a{bo[POM_UB], bo[POM_U], b_uf2, bi[POM_UB], b_u1, s.imp ? s.g[POM_G_DUM] : nullptr};
       hipLaunchKernelGGL(k_uvfilt_adj<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a);
-      UVFiltAdjArgs<T> bv{bo[POM_VB], bo[POM_V], b_vf2, bi[POM_VB], b_v1};
+      UVFiltAdjArgs<T> bv{bo[POM_VB], bo[POM_V], b_vf2, bi[POM_VB], b_v1, s.imp ? s.g[POM_G_DVM] : nullptr};
       hipLaunchKernelGGL(k_uvfilt_adj<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, bv);
     }
-    {   // bcond 3
+    if (s.imp) {   // closed-basin bcond 3
+      COutTab<T> ob{{b_uf2, b_vf2}};
+      T* ib[OpImpBc3::NIN] = {b_uf1, b_vf1};
+      launch_gather<T, OpImpBc3>(c, s.t_ibc3(), ob, ib);
+    } else {   // bcond 3
       COutTab<T> ob{{b_uf2, b_vf2}};
       T* ib[OpBc3::NIN] = {b_uf1, b_vf1, b_u1, b_v1};
       launch_gather<T, OpBc3>(c, s.t_bc3(), ob, ib);
@@ -990,7 +1285,11 @@ template <class T> struct Adj {
       T* ib[OpDens::NIN] = {bo[POM_T], bo[POM_S], b_rho1};
       launch_gather<T, OpDens>(c, s.t_dens(), ob, ib);
     }
-    {   // bcond 4 + T/S filter
+    if (s.imp) {   // closed-basin bcond 4 + T/S filter + tracer mask
+      COutTab<T> ob{{bo[POM_T], bo[POM_TB], bo[POM_S], bo[POM_SB]}};
+      T* ib[OpImpBc4Filter::NIN] = {b_tf1, b_sf1, bi[POM_T], bi[POM_S], bi[POM_TB], bi[POM_SB]};
+      launch_gather<T, OpImpBc4Filter>(c, s.t_ibc4(), ob, ib);
+    } else {   // bcond 4 + T/S filter
       COutTab<T> ob{{bo[POM_T], bo[POM_TB], bo[POM_S], bo[POM_SB]}};
       T* ib[OpBc4Filter::NIN] = {b_tf1, b_sf1, bi[POM_T], bi[POM_S], bi[POM_TB], bi[POM_SB], b_u1, b_v1,
                                  bo[POM_W], bi[POM_DT]};
@@ -1013,7 +1312,11 @@ template <class T> struct Adj {
                              bo[POM_AAM], bi[POM_DT], bi[POM_ETB], b_etf30};
       launch_gather<T, OpAdvt1>(c, s.t_advt1(sal == 1), ob, ib);
     }
-    {   // bcond 6 + q2 filter
+    if (s.imp) {   // closed-basin bcond 6 + q2 filter
+      COutTab<T> ob{{bo[POM_Q2], bo[POM_Q2B], bo[POM_Q2L], bo[POM_Q2LB]}};
+      T* ib[OpImpBc6Filter::NIN] = {b_q2f1, b_q2lf1, bi[POM_Q2], bi[POM_Q2L], bi[POM_Q2B], bi[POM_Q2LB]};
+      launch_gather<T, OpImpBc6Filter>(c, s.t_ibc6(), ob, ib);
+    } else {   // bcond 6 + q2 filter
       COutTab<T> ob{{bo[POM_Q2], bo[POM_Q2B], bo[POM_Q2L], bo[POM_Q2LB]}};
       T* ib[OpBc6Filter::NIN] = {b_q2f1, b_q2lf1, bi[POM_Q2], bi[POM_Q2L], bi[POM_Q2B], bi[POM_Q2LB], b_u1, b_v1};
       launch_gather<T, OpBc6Filter>(c, s.t_bc6(), ob, ib);
@@ -1052,7 +1355,7 @@ template <class T> struct Adj {
       hipLaunchKernelGGL(k_vertvl_adj2<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a);
     }
     {   // transport adjustment
-      AdjustAdjArgs<T> a{s.in[POM_UTB], s.xs[s.isplit].utf, s.in[POM_VTB], s.xs[s.isplit].vtf, s.in[POM_DT],
+      AdjustAdjArgs<T> a{s.in[POM_UTB], s.xf().utf, s.in[POM_VTB], s.xf().vtf, s.in[POM_DT],
                          b_u1, b_v1, bi[POM_U], bi[POM_V], bi[POM_UTB], b_utf30, bi[POM_VTB], b_vtf30,
                          bi[POM_DT]};
       hipLaunchKernelGGL(k_adjust_adj<T>, g2, tb, 0, c.stream, c.B, c.vert, a);
@@ -1072,17 +1375,130 @@ template <class T> Ctx<T> make_ctx(const pom_batch* b, const pom_params_t* p, co
   q.kappa = p->kappa; q.umol = p->umol; q.tprni = p->tprni; q.small = p->small; q.tbias = p->tbias;
   q.sbias = p->sbias; q.hmax = p->hmax; q.rfe = p->rfe; q.rfw = p->rfw; q.rfn = p->rfn; q.rfs = p->rfs;
   q.isplit = p->isplit; q.ispadv = p->ispadv; q.ntp = p->ntp; q.nbct = p->nbct; q.nbcs = p->nbcs; q.iext = 0;
+  q.core = p->core;
+  q.gamma_baro = p->core == POM_CORE_IMPLICIT ? p->gamma_baro : 0.0;
   c.nblk3 = b->nblk3;
   c.nblk2 = b->nblk2;
   c.stream = st;
   return c;
 }
 
-int check_params(const pom_params_t* p) {
-  if (!p) return POM_E_ARG;
+int check_params(const pom_params_t* p, int core) {
+  if (!p || p->core != core) return POM_E_ARG;
   if (p->isplit < 3 || p->isplit > MAXSPLIT || p->ispadv < 1) return POM_E_ARG;
   if (!((p->nbct == 1 || p->nbct == 3) && (p->nbcs == 1 || p->nbcs == 3))) return POM_E_UNSUPPORTED;
+  if (core == POM_CORE_IMPLICIT) {
+    if (p->eta_max_iter < 1 || p->eta_max_iter > 100000 || !(p->eta_rel_tol >= 0.0) || !(p->eta_tol >= 0.0))
+      return POM_E_ARG;
+    if (p->eta_precond != POM_PRECOND_JACOBI && p->eta_precond != POM_PRECOND_SPECTRAL) return POM_E_ARG;
+  }
   return POM_OK;
+}
+
+// ---------------------------------------------------------------- spectral cache
+// [256 B head][SpecMember x M][data: per member S_ni, S_nj, inverse eigenvalues]
+constexpr size_t CACHE_HEAD = 256;
+inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+inline size_t cache_data_off(int M) { return CACHE_HEAD + align256(sizeof(SpecMember) * (size_t)M); }
+inline size_t cache_elems(const pom_batch* b) {
+  size_t n = 0;
+  for (const Member& m : b->h_mem) {
+    const size_t ni = m.im - 2, nj = m.jm - 2;
+    n += ni * ni + nj * nj + ni * nj;
+  }
+  return n;
+}
+inline size_t cache_bytes(const pom_batch* b) {
+  return cache_data_off(b->M) + cache_elems(b) * (b->dtype == POM_F32 ? 4 : 8) + 256;
+}
+
+// lower median of the interior values of a member's 2-D field (torch.median)
+inline double interior_median(const std::vector<double>& f, const Member& m) {
+  std::vector<double> v;
+  v.reserve((size_t)(m.im - 2) * (m.jm - 2));
+  for (int i = 1; i <= m.im - 2; ++i)
+    for (int j = 1; j <= m.jm - 2; ++j) v.push_back(f[m.off2 + (int64_t)i * m.jm + j]);
+  const size_t k = (v.size() - 1) / 2;
+  std::nth_element(v.begin(), v.begin() + k, v.end());
+  return v[k];
+}
+
+template <class T>
+int prepare_cache(const pom_batch* b, const pom_params_t* prm, const void* const* geom, void* cache,
+                  hipStream_t st) {
+  const int64_t N2 = b->n2;
+  std::vector<T> hx(N2), hy(N2), hh(N2);
+  if (hipMemcpyAsync(hx.data(), geom[POM_G_DX], sizeof(T) * N2, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipMemcpyAsync(hy.data(), geom[POM_G_DY], sizeof(T) * N2, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipMemcpyAsync(hh.data(), geom[POM_G_H], sizeof(T) * N2, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return POM_E_HIP;
+  const std::vector<double> dx(hx.begin(), hx.end()), dy(hy.begin(), hy.end()), h(hh.begin(), hh.end());
+  const double coeff = 4.0 * (prm->dti * prm->dti) * prm->grav * prm->alpha;   // core.py:175
+  const double pi = 3.14159265358979323846;
+  std::vector<SpecMember> tab(b->M);
+  std::vector<T> data(cache_elems(b));
+  int64_t o = 0;
+  for (int m = 0; m < b->M; ++m) {
+    const Member& mm = b->h_mem[m];
+    const int ni = mm.im - 2, nj = mm.jm - 2;
+    SpecMember& sm = tab[m];
+    sm.ni = ni;
+    sm.nj = nj;
+    sm.scale = 4.0 * (ni + 1) * (nj + 1);
+    auto sine = [&](int n, int64_t at) {   // S_n[k][q] = 2 sin(pi (k+1)(q+1) / (n+1))
+      for (int k = 0; k < n; ++k)
+        for (int q = 0; q < n; ++q) {
+          const int64_t r = ((int64_t)(k + 1) * (q + 1)) % (2 * (n + 1));   // exact argument reduction
+          data[at + (int64_t)k * n + q] = T(2.0 * std::sin(pi * double(r) / double(n + 1)));
+        }
+    };
+    sm.si = o;
+    sine(ni, o);
+    o += (int64_t)ni * ni;
+    sm.sj = o;
+    sine(nj, o);
+    o += (int64_t)nj * nj;
+    sm.l = o;
+    // inverse eigenvalues of the constant-coefficient operator at the interior
+    // medians of dx, dy, h (core.py:730-760)
+    const double dx0 = interior_median(dx, mm), dy0 = interior_median(dy, mm), h0 = interior_median(h, mm);
+    for (int a = 0; a < ni; ++a) {
+      const double sx = std::sin(pi * (a + 1) / (2.0 * (ni + 1)));
+      const double lx = 4.0 * (sx * sx) / (dx0 * dx0 + 1e-30);
+      for (int c2 = 0; c2 < nj; ++c2) {
+        const double sy = std::sin(pi * (c2 + 1) / (2.0 * (nj + 1)));
+        const double ly = 4.0 * (sy * sy) / (dy0 * dy0 + 1e-30);
+        data[o + (int64_t)a * nj + c2] = T(1.0 / (1.0 + coeff * h0 * (lx + ly)));
+      }
+    }
+    o += (int64_t)ni * nj;
+  }
+  char* base = static_cast<char*>(cache);
+  if (hipMemcpyAsync(base + CACHE_HEAD, tab.data(), sizeof(SpecMember) * b->M, hipMemcpyHostToDevice, st) !=
+          hipSuccess ||
+      hipMemcpyAsync(base + cache_data_off(b->M), data.data(), sizeof(T) * data.size(), hipMemcpyHostToDevice, st) !=
+          hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return POM_E_HIP;
+  return POM_OK;
+}
+
+// attach the semi-implicit settings and cache to a step
+template <class T> void setup_implicit(Step<T>& S, const pom_params_t* prm, const void* cache) {
+  S.imp = prm->core == POM_CORE_IMPLICIT;
+  S.hp = prm;
+  if (!S.imp) return;
+  S.pw.spectral = prm->eta_precond == POM_PRECOND_SPECTRAL;
+  S.pw.maxni = S.pw.maxnj = 0;
+  for (const Member& m : S.b->h_mem) {
+    S.pw.maxni = std::max(S.pw.maxni, m.im - 2);
+    S.pw.maxnj = std::max(S.pw.maxnj, m.jm - 2);
+  }
+  if (cache) {
+    S.pw.sp = reinterpret_cast<const SpecMember*>(static_cast<const char*>(cache) + CACHE_HEAD);
+    S.pw.cache = reinterpret_cast<const T*>(static_cast<const char*>(cache) + cache_data_off(S.b->M));
+  }
 }
 
 template <class T>
@@ -1091,6 +1507,7 @@ size_t ws_size(const pom_batch* b, const pom_params_t* p, bool adjoint) {
   S.b = b;
   S.isplit = p->isplit;
   S.c.prm.ispadv = p->ispadv;
+  setup_implicit(S, p, nullptr);
   Arena A{nullptr, 0, 0, true};
   T* dummy[POM_NSTATE] = {};
   for (int f = 0; f < POM_NSTATE; ++f) S.out[f] = dummy[f];
@@ -1105,10 +1522,12 @@ size_t ws_size(const pom_batch* b, const pom_params_t* p, bool adjoint) {
 template <class T>
 int run_step(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vg, const void* const* in,
              void* const* out, const void* const* geom, int internal, void* vamax, void* ws, size_t wsb,
-             hipStream_t st) {
+             hipStream_t st, const void* cache = nullptr, int32_t* iters = nullptr) {
   Step<T> S{};
   S.b = b;
   S.c = make_ctx<T>(b, prm, ramp, vg, st);
+  setup_implicit(S, prm, cache);
+  S.iters_out = iters;
   S.isplit = prm->isplit;
   S.internal = internal != 0;
   S.vamax = static_cast<T*>(vamax);
@@ -1131,10 +1550,11 @@ int run_step(const pom_batch* b, const pom_params_t* prm, const void* ramp, cons
 template <class T>
 int run_adj(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vg, const void* const* in,
             const void* const* geom, int internal, const void* const* cot, void* const* grad, void* ws, size_t wsb,
-            hipStream_t st) {
+            hipStream_t st, const void* cache = nullptr) {
   Step<T> S{};
   S.b = b;
   S.c = make_ctx<T>(b, prm, ramp, vg, st);
+  setup_implicit(S, prm, cache);
   S.isplit = prm->isplit;
   S.internal = internal != 0;
   for (int f = 0; f < POM_NSTATE; ++f) {
@@ -1222,6 +1642,7 @@ int pom_batch_create(const pom_member_t* members, int32_t n, int32_t dtype, pom_
     n3 = std::max(n3, s.off3 + c3);
   }
   pom_batch* b = new pom_batch{};
+  b->h_mem = hm;
   b->dtype = dtype;
   b->M = n;
   b->nblk3 = b3[n];
@@ -1275,7 +1696,7 @@ int pom_batch_destroy(pom_batch* b) {
 
 int pom_explicit_workspace(const pom_batch* b, const pom_params_t* prm, int32_t adjoint, size_t* bytes) {
   if (!b || !bytes) return POM_E_ARG;
-  if (int e = check_params(prm)) return e;
+  if (int e = check_params(prm, POM_CORE_EXPLICIT)) return e;
   *bytes = b->dtype == POM_F32 ? ws_size<float>(b, prm, adjoint != 0) : ws_size<double>(b, prm, adjoint != 0);
   return POM_OK;
 }
@@ -1284,7 +1705,7 @@ int pom_explicit_step(const pom_batch* b, const pom_params_t* prm, const void* r
                       const void* const* state_in, void* const* state_out, const void* const* geom, int32_t internal,
                       void* vamax_out, void* workspace, size_t ws_bytes, void* stream) {
   if (!b || !ramp || !vgrid || !state_in || !state_out || !geom || !vamax_out || !workspace) return POM_E_ARG;
-  if (int e = check_params(prm)) return e;
+  if (int e = check_params(prm, POM_CORE_EXPLICIT)) return e;
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (b->dtype == POM_F32)
     return run_step<float>(b, prm, ramp, vgrid, state_in, state_out, geom, internal, vamax_out, workspace, ws_bytes, st);
@@ -1296,7 +1717,7 @@ int pom_explicit_step_adj(const pom_batch* b, const pom_params_t* prm, const voi
                           const void* const* cot_out, void* const* grad_in, void* workspace, size_t ws_bytes,
                           void* stream) {
   if (!b || !ramp || !vgrid || !state_in || !geom || !workspace) return POM_E_ARG;
-  if (int e = check_params(prm)) return e;
+  if (int e = check_params(prm, POM_CORE_EXPLICIT)) return e;
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (b->dtype == POM_F32)
     return run_adj<float>(b, prm, ramp, vgrid, state_in, geom, internal, cot_out, grad_in, workspace, ws_bytes, st);
@@ -1344,5 +1765,59 @@ int pom_baropg_adj(const pom_batch* b, const pom_params_t* prm, const void* ramp
   return run_baropg<double>(b, prm, ramp, vgrid, in, nullptr, cot_out, grad_in, st, true);
 }
 
+
+int pom_implicit_cache_bytes(const pom_batch* b, size_t* bytes) {
+  if (!b || !bytes) return POM_E_ARG;
+  *bytes = cache_bytes(b);
+  return POM_OK;
+}
+
+int pom_implicit_prepare(const pom_batch* b, const pom_params_t* prm, const void* const* geom, void* cache,
+                         size_t cache_bytes_, void* stream) {
+  if (!b || !geom || !cache) return POM_E_ARG;
+  if (int e = check_params(prm, POM_CORE_IMPLICIT)) return e;
+  if (cache_bytes_ < cache_bytes(b)) return POM_E_ARG;
+  for (int f : {POM_G_DX, POM_G_DY, POM_G_H})
+    if (!geom[f]) return POM_E_ARG;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (b->dtype == POM_F32) return prepare_cache<float>(b, prm, geom, cache, st);
+  return prepare_cache<double>(b, prm, geom, cache, st);
+}
+
+int pom_implicit_workspace(const pom_batch* b, const pom_params_t* prm, int32_t adjoint, size_t* bytes) {
+  if (!b || !bytes) return POM_E_ARG;
+  if (int e = check_params(prm, POM_CORE_IMPLICIT)) return e;
+  *bytes = b->dtype == POM_F32 ? ws_size<float>(b, prm, adjoint != 0) : ws_size<double>(b, prm, adjoint != 0);
+  return POM_OK;
+}
+
+int pom_implicit_step(const pom_batch* b, const pom_params_t* prm, const void* cache, const void* ramp,
+                      const void* vgrid, const void* const* state_in, void* const* state_out, const void* const* geom,
+                      int32_t internal, void* vamax_out, int32_t* eta_iters_out, void* workspace, size_t ws_bytes,
+                      void* stream) {
+  if (!b || !cache || !ramp || !vgrid || !state_in || !state_out || !geom || !vamax_out || !workspace)
+    return POM_E_ARG;
+  if (int e = check_params(prm, POM_CORE_IMPLICIT)) return e;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (b->dtype == POM_F32)
+    return run_step<float>(b, prm, ramp, vgrid, state_in, state_out, geom, internal, vamax_out, workspace, ws_bytes,
+                           st, cache, eta_iters_out);
+  return run_step<double>(b, prm, ramp, vgrid, state_in, state_out, geom, internal, vamax_out, workspace, ws_bytes,
+                          st, cache, eta_iters_out);
+}
+
+int pom_implicit_step_adj(const pom_batch* b, const pom_params_t* prm, const void* cache, const void* ramp,
+                          const void* vgrid, const void* const* state_in, const void* const* geom, int32_t internal,
+                          const void* const* cot_out, void* const* grad_in, void* workspace, size_t ws_bytes,
+                          void* stream) {
+  if (!b || !cache || !ramp || !vgrid || !state_in || !geom || !workspace) return POM_E_ARG;
+  if (int e = check_params(prm, POM_CORE_IMPLICIT)) return e;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (b->dtype == POM_F32)
+    return run_adj<float>(b, prm, ramp, vgrid, state_in, geom, internal, cot_out, grad_in, workspace, ws_bytes, st,
+                          cache);
+  return run_adj<double>(b, prm, ramp, vgrid, state_in, geom, internal, cot_out, grad_in, workspace, ws_bytes, st,
+                         cache);
+}
 
 }  // extern "C"

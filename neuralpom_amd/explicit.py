@@ -68,6 +68,7 @@ class StepScalars:
         for n in ("isplit", "ispadv", "ntp", "nbct", "nbcs"):
             setattr(p, n, int(getattr(self, n)))
         p.iext = 0
+        p.core = L.POM_CORE_EXPLICIT
         return p
 
 

@@ -1,0 +1,160 @@
+"""Batched semi-implicit (P0-fixed) POM step on the HIP extension, with its adjoint.
+
+``ImplicitEnsemble`` extends the explicit ``Ensemble`` (same member packing
+and field tables) with the spectral-preconditioner cache and drives
+``pom_implicit_step`` / ``pom_implicit_step_adj`` (include/neuralpom_amd.h).
+``ImplicitStep`` is the ``torch.autograd.Function`` of one internal step:
+backward recomputes the step (per-step checkpointing) and runs the adjoint
+kernels, including the eta-solve adjoint (a second PCG on the area-weighted
+transpose).  No reference to a live core is kept.
+
+Reference: src/neuralpom/cores/implicit_p0_fixed/time_loop.py:38-688 (the
+step), core.py:24-59 (eta-solve autograd), src/neuralpom/v1/core_spec.py:22-33
+(production settings: alpha .333, smoth .15, Shapiro gamma 0, eta relative
+tolerance 1e-5, spectral preconditioner).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import torch
+
+from . import _lib as L
+from .explicit import Ensemble, StepScalars
+
+
+@dataclass
+class ImplicitScalars(StepScalars):
+    """Scalars of the semi-implicit core (explicit defaults + core_spec.py:22-33 + core.py:62-119)."""
+    alpha: float = 0.333
+    smoth: float = 0.15
+    eta_tol: float = 1.0e-12
+    eta_rel_tol: float = 1.0e-5
+    eta_max_iter: int = 200
+    eta_preconditioner: str = "spectral"
+    barotropic_shapiro: float = 0.0
+
+    def struct(self) -> L.Params:
+        p = super().struct()
+        p.core = L.POM_CORE_IMPLICIT
+        p.eta_tol = float(self.eta_tol)
+        p.eta_rel_tol = float(self.eta_rel_tol)
+        p.eta_max_iter = int(self.eta_max_iter)
+        pre = str(self.eta_preconditioner).strip().lower()
+        if pre not in ("spectral", "jacobi"):
+            raise ValueError(f"eta preconditioner must be 'spectral' or 'jacobi', got {self.eta_preconditioner!r}")
+        p.eta_precond = L.POM_PRECOND_SPECTRAL if pre == "spectral" else L.POM_PRECOND_JACOBI
+        p.gamma_baro = float(self.barotropic_shapiro)
+        return p
+
+
+class ImplicitEnsemble(Ensemble):
+    """Members of the semi-implicit core; ``prepare(geom, sc)`` builds the
+    preconditioner cache once after initialisation (core.py:254-259)."""
+
+    def __init__(self, shapes, dtype=torch.float32, device="cuda"):
+        super().__init__(shapes, dtype=dtype, device=device)
+        n = C.c_size_t()
+        L.check(self.lib.pom_implicit_cache_bytes(self._h, C.byref(n)), "pom_implicit_cache_bytes")
+        self.cache = torch.empty(n.value, dtype=torch.uint8, device=self.device)
+        self._prepared = False
+
+    def prepare(self, geom, sc: ImplicitScalars):
+        prm = sc.struct()
+        gs = [geom[n] for n in L.GEOM]
+        self._check(gs, L.GEOM, self.numel)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        L.check(self.lib.pom_implicit_prepare(self._h, C.byref(prm), L.ptr_array([t.data_ptr() for t in gs]),
+                                              self.cache.data_ptr(), self.cache.numel(), stream),
+                "pom_implicit_prepare")
+        self._prepared = True
+
+    def _ws_implicit(self, prm, adjoint):
+        n = C.c_size_t()
+        L.check(self.lib.pom_implicit_workspace(self._h, C.byref(prm), int(adjoint), C.byref(n)),
+                "pom_implicit_workspace")
+        key = ("imp", adjoint)
+        ws = self._ws.get(key)
+        if ws is None or ws.numel() < n.value:
+            ws = torch.empty(n.value, dtype=torch.uint8, device=self.device)
+            self._ws[key] = ws
+        return ws, n.value
+
+    def step(self, state, geom, sc: ImplicitScalars, ramp, vgrid, internal=True, iters=None):
+        """One internal step; returns (new state dict, vamax).  ``iters``: optional
+        int32 device tensor [members] receiving the eta-solve iteration counts."""
+        if not self._prepared:
+            raise RuntimeError("ImplicitEnsemble.prepare(geom, scalars) must run before the first step")
+        prm = sc.struct()
+        ins = [state[n] for n in L.FIELDS]
+        gs = [geom[n] for n in L.GEOM]
+        self._check(ins, L.FIELDS, self.numel)
+        self._check(gs, L.GEOM, self.numel)
+        out = {n: torch.empty_like(state[n]) for n in L.FIELDS}
+        vamax = torch.zeros(1, dtype=self.dtype, device=self.device)
+        ws, nb = self._ws_implicit(prm, False)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        rc = self.lib.pom_implicit_step(
+            self._h, C.byref(prm), self.cache.data_ptr(), ramp.data_ptr(), vgrid.data_ptr(),
+            L.ptr_array([t.data_ptr() for t in ins]), L.ptr_array([out[n].data_ptr() for n in L.FIELDS]),
+            L.ptr_array([t.data_ptr() for t in gs]), int(bool(internal)), vamax.data_ptr(),
+            iters.data_ptr() if iters is not None else None, ws.data_ptr(), nb, stream)
+        L.check(rc, "pom_implicit_step")
+        return out, vamax
+
+    def step_adj(self, state, geom, sc: ImplicitScalars, ramp, vgrid, internal, cot):
+        """Gradients of the step inputs for output cotangents ``cot`` (dict, may omit fields)."""
+        if not self._prepared:
+            raise RuntimeError("ImplicitEnsemble.prepare(geom, scalars) must run before the first step")
+        prm = sc.struct()
+        ins = [state[n] for n in L.FIELDS]
+        gs = [geom[n] for n in L.GEOM]
+        self._check(ins, L.FIELDS, self.numel)
+        sizes = [self.numel(n) for n in L.DIFF_INPUTS]
+        flat = torch.zeros(sum(sizes), dtype=self.dtype, device=self.device)
+        grads = dict(zip(L.DIFF_INPUTS, torch.split(flat, sizes)))
+        cot_ptrs = []
+        for n in L.FIELDS:
+            c = cot.get(n)
+            if c is not None:
+                c = c.contiguous().to(self.dtype)
+                cot[n] = c
+                cot_ptrs.append(c.data_ptr())
+            else:
+                cot_ptrs.append(None)
+        grad_ptrs = [grads[n].data_ptr() if n in grads else None for n in L.FIELDS]
+        ws, nb = self._ws_implicit(prm, True)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        rc = self.lib.pom_implicit_step_adj(
+            self._h, C.byref(prm), self.cache.data_ptr(), ramp.data_ptr(), vgrid.data_ptr(),
+            L.ptr_array([t.data_ptr() for t in ins]), L.ptr_array([t.data_ptr() for t in gs]),
+            int(bool(internal)), L.ptr_array(cot_ptrs), L.ptr_array(grad_ptrs), ws.data_ptr(), nb, stream)
+        L.check(rc, "pom_implicit_step_adj")
+        return grads
+
+
+class ImplicitStep(torch.autograd.Function):
+    """One semi-implicit internal step as an autograd node (inputs/outputs in FIELDS order)."""
+
+    @staticmethod
+    def forward(ctx, ens, geom, sc, ramp, vgrid, internal, *inputs):
+        state = dict(zip(L.FIELDS, inputs))
+        out, vamax = ens.step(state, geom, sc, ramp, vgrid, internal)
+        ctx.ens, ctx.geom, ctx.sc, ctx.internal = ens, geom, sc, internal
+        ctx.save_for_backward(ramp, vgrid, *inputs)
+        ctx.mark_non_differentiable(vamax)
+        return tuple(out[n] for n in L.FIELDS) + (vamax,)
+
+    @staticmethod
+    def backward(ctx, *gouts):
+        ramp, vgrid, *inputs = ctx.saved_tensors
+        state = dict(zip(L.FIELDS, inputs))
+        cot = {n: g for n, g in zip(L.FIELDS, gouts[:len(L.FIELDS)]) if g is not None}
+        grads = ctx.ens.step_adj(state, ctx.geom, ctx.sc, ramp, vgrid, ctx.internal, cot)
+        return (None,) * 6 + tuple(grads.get(n) for n in L.FIELDS)
+
+
+def step_autograd(ens, state, geom, sc, ramp, vgrid, internal=True):
+    outs = ImplicitStep.apply(ens, geom, sc, ramp, vgrid, internal, *[state[n] for n in L.FIELDS])
+    return dict(zip(L.FIELDS, outs[:-1])), outs[-1]

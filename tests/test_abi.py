@@ -33,12 +33,29 @@ def test_python_binding_lists_every_declared_symbol():
     assert set(_lib.EXPORTED) == _declared()
 
 
-def test_param_struct_layout_matches_header():
+def test_param_struct_layout_matches_header(tmp_path):
+    """ctypes layouts equal the C compiler's for the header's structs."""
+    import shutil
+    import subprocess
     from neuralpom_amd import _lib
-    # 22 doubles, 6 int32, 1 double (8-byte aligned) = 208 bytes
-    assert ctypes.sizeof(_lib.Params) == 22 * 8 + 6 * 4 + 8
-    assert ctypes.sizeof(_lib.Member) == 24
     assert len(_lib.FIELDS) == 61 and len(_lib.GEOM) == 18
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    fields = [f for f, _ in _lib.Params._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "neuralpom_amd.h"\nint main(void){\n'
+        '  printf("%zu %zu\\n", sizeof(pom_params_t), sizeof(pom_member_t));\n'
+        + "".join(f'  printf("%zu\\n", offsetof(pom_params_t, {f}));\n' for f in fields)
+        + "  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    assert int(out[0]) == ctypes.sizeof(_lib.Params)
+    assert int(out[1]) == ctypes.sizeof(_lib.Member)
+    for f, off in zip(fields, out[2:]):
+        assert getattr(_lib.Params, f).offset == int(off), f
 
 
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):
