@@ -110,6 +110,47 @@ def run_gpu(args):
     return dict(cells=cells, seconds=dt, seconds_fwd=dtf, ens=ens, kern_ms=kern_ms, kern_n=kern_n)
 
 
+def run_gpu_implicit(args):
+    """The same seamount ensemble on the semi-implicit core (production clock dti = 600 s,
+    spectral-preconditioned PCG, relative tolerance 1e-5), fwd+adjoint per step."""
+    from neuralpom_amd import _lib as L
+    from neuralpom_amd.implicit import ImplicitEnsemble, ImplicitScalars
+    dtype = torch.float32
+    dev = torch.device("cuda", 0)
+    _, state, geom, vg, periods, _, _ = build_ensemble(SHAPES, dtype, dev)
+    ens = ImplicitEnsemble(SHAPES, dtype=dtype, device=dev)
+    sc = ImplicitScalars(dte=20.0, isplit=30, hmax=4500.0, horcon=0.05)
+    ens.prepare(geom, sc)
+    g = torch.Generator(device="cpu").manual_seed(7)
+    cot = {n: (torch.randn(ens.numel(n), generator=g) * 1e-3).to(dev, dtype) for n in DIFF_OUT}
+    iters = torch.zeros(len(SHAPES), dtype=torch.int32, device=dev)
+    it = [0]
+
+    def one(st):
+        it[0] += 1
+        r = ramps(periods, 1.0 + sc.dti * it[0] / 86400.0, dtype, dev)
+        out, vamax = ens.step(st, geom, sc, r, vg, True, iters)
+        ens.step_adj(st, geom, sc, r, vg, True, dict(cot))
+        return out, vamax
+
+    for _ in range(args.warmup):
+        state, vamax = one(state)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    its = 0.0
+    for _ in range(args.implicit_steps):
+        state, vamax = one(state)
+        its += float(iters.float().mean())
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    if not float(vamax) <= sc.vmaxl or not torch.isfinite(state["el"]).all():
+        raise FloatingPointError("semi-implicit seamount ensemble diverged")
+    cells = ens.n3
+    return {"value": cells * args.implicit_steps / dt, "unit": "grid-cell-steps/s",
+            "ms_per_step": 1e3 * dt / args.implicit_steps, "dti_s": sc.dti,
+            "eta_iterations_mean": its / args.implicit_steps, "steps": args.implicit_steps}
+
+
 def autograd_baseline(args, n_members=3):
     """The reference algorithm (oracle restatement, eager torch.autograd) on the same GPU.
 
@@ -193,6 +234,8 @@ def main():
     ap.add_argument("--autograd-steps", type=int, default=1)
     ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--no-baselines", action="store_true")
+    ap.add_argument("--implicit-steps", type=int, default=5,
+                    help="fwd+adjoint steps of the same ensemble on the semi-implicit core (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="launch one 256 MiB device copy first (reference for HBM counter scaling)")
     args = ap.parse_args()
@@ -238,6 +281,8 @@ def main():
                           "unit": "GB/s", "frac": achieved / HBM_PEAK,
                           "bytes_per_cell_step": B_FA},
     }
+    if args.implicit_steps > 0:
+        line["implicit_core"] = run_gpu_implicit(args)
     if not args.no_baselines:
         ab = autograd_baseline(args)
         line["autograd_baseline"] = ab
