@@ -124,10 +124,7 @@ template <class X, class Y> __device__ __forceinline__ auto pmax(X a, Y b);
 // common activity type of two values, and conversion into it
 template <class X> __device__ X mkval();   // declared only: for decltype
 template <class X, class Y> using common_t = decltype(mkval<X>() + mkval<Y>());
-template <class R, class X> __device__ __forceinline__ R cvt(X x) {
-  if constexpr (std::is_same_v<R, X>) return x;
-  else return R{x.v, decltype(x.v)(0)};
-}
+template <class R, class X> __device__ __forceinline__ R cvt(X x);
 template <class X, class Y> __device__ __forceinline__ auto pmax(X a, Y b) {
   using R = common_t<X, Y>;
   return val(a) >= val(b) ? cvt<R>(a) : cvt<R>(b);
@@ -136,6 +133,209 @@ template <class X, class Y> __device__ __forceinline__ auto pmax(X a, Y b) {
 template <class X> struct scalar_of;
 template <class T> struct scalar_of<P<T>> { using type = T; };
 template <class T> struct scalar_of<D<T>> { using type = T; };
+
+// ------------------------------------------------------------------ multi-seed tangents
+// DS<T, M>: a value and its partial derivatives with respect to the input
+// fields whose bits are set in the compile-time mask M (one lane per field,
+// stored densely in bit order).  Used by the fused adjoint gathers: one
+// evaluation of a point function at a stencil offset gives the partials with
+// respect to every input field at that offset.  Arithmetic per lane follows D.
+constexpr int popc(uint32_t x) { return x ? int(x & 1u) + popc(x >> 1) : 0; }
+constexpr int slot(uint32_t m, int b) { return popc(m & ((1u << b) - 1u)); }
+
+template <class T, uint32_t M> struct DS {
+  T v;
+  T d[popc(M) > 0 ? popc(M) : 1];
+};
+template <class T, uint32_t M> struct is_active<DS<T, M>> : std::true_type {};
+template <class T, uint32_t M> struct scalar_of<DS<T, M>> { using type = T; };
+template <class T, uint32_t M> __device__ __forceinline__ T val(const DS<T, M>& a) { return a.v; }
+
+template <class X> struct mask_of { static constexpr uint32_t value = 0; };
+template <class T, uint32_t M> struct mask_of<DS<T, M>> { static constexpr uint32_t value = M; };
+
+// f(integral_constant<int, b>) for every set bit b of M, in increasing order
+template <uint32_t M, int B = 0, class F> __device__ __forceinline__ void for_bits(F&& f) {
+  if constexpr (B < 32) {
+    if constexpr ((M >> B) & 1u) f(std::integral_constant<int, B>{});
+    for_bits<M, B + 1>(f);
+  }
+}
+// lane b of x (0 when x does not carry b)
+template <int B, class T, uint32_t M> __device__ __forceinline__ T lane(const DS<T, M>& x) {
+  if constexpr ((M >> B) & 1u) return x.d[slot(M, B)];
+  else return T(0);
+}
+
+// DS op DS
+template <class T, uint32_t A, uint32_t B>
+__device__ __forceinline__ DS<T, A | B> operator+(const DS<T, A>& a, const DS<T, B>& b) {
+  DS<T, A | B> r;
+  r.v = a.v + b.v;
+  for_bits<A | B>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    constexpr bool ha = (A >> i) & 1u, hb = (B >> i) & 1u;
+    if constexpr (ha && hb) r.d[slot(A | B, i)] = a.d[slot(A, i)] + b.d[slot(B, i)];
+    else if constexpr (ha) r.d[slot(A | B, i)] = a.d[slot(A, i)];
+    else r.d[slot(A | B, i)] = b.d[slot(B, i)];
+  });
+  return r;
+}
+template <class T, uint32_t A, uint32_t B>
+__device__ __forceinline__ DS<T, A | B> operator-(const DS<T, A>& a, const DS<T, B>& b) {
+  DS<T, A | B> r;
+  r.v = a.v - b.v;
+  for_bits<A | B>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    constexpr bool ha = (A >> i) & 1u, hb = (B >> i) & 1u;
+    if constexpr (ha && hb) r.d[slot(A | B, i)] = a.d[slot(A, i)] - b.d[slot(B, i)];
+    else if constexpr (ha) r.d[slot(A | B, i)] = a.d[slot(A, i)];
+    else r.d[slot(A | B, i)] = -b.d[slot(B, i)];
+  });
+  return r;
+}
+template <class T, uint32_t A, uint32_t B>
+__device__ __forceinline__ DS<T, A | B> operator*(const DS<T, A>& a, const DS<T, B>& b) {
+  DS<T, A | B> r;
+  r.v = a.v * b.v;
+  for_bits<A | B>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    constexpr bool ha = (A >> i) & 1u, hb = (B >> i) & 1u;
+    if constexpr (ha && hb) r.d[slot(A | B, i)] = a.d[slot(A, i)] * b.v + a.v * b.d[slot(B, i)];
+    else if constexpr (ha) r.d[slot(A | B, i)] = a.d[slot(A, i)] * b.v;
+    else r.d[slot(A | B, i)] = a.v * b.d[slot(B, i)];
+  });
+  return r;
+}
+template <class T, uint32_t A, uint32_t B>
+__device__ __forceinline__ DS<T, A | B> operator/(const DS<T, A>& a, const DS<T, B>& b) {
+  DS<T, A | B> r;
+  const T q = a.v / b.v;
+  r.v = q;
+  for_bits<A | B>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    constexpr bool ha = (A >> i) & 1u, hb = (B >> i) & 1u;
+    if constexpr (ha && hb) r.d[slot(A | B, i)] = (a.d[slot(A, i)] - q * b.d[slot(B, i)]) / b.v;
+    else if constexpr (ha) r.d[slot(A | B, i)] = a.d[slot(A, i)] / b.v;
+    else r.d[slot(A | B, i)] = -q * b.d[slot(B, i)] / b.v;
+  });
+  return r;
+}
+template <class T, uint32_t A> __device__ __forceinline__ DS<T, A> operator-(const DS<T, A>& a) {
+  DS<T, A> r;
+  r.v = -a.v;
+  for (int i = 0; i < popc(A); ++i) r.d[i] = -a.d[i];
+  return r;
+}
+// DS op P / P op DS
+template <class T, uint32_t A> __device__ __forceinline__ DS<T, A> operator+(const DS<T, A>& a, P<T> b) {
+  DS<T, A> r = a;
+  r.v = a.v + b.v;
+  return r;
+}
+template <class T, uint32_t A> __device__ __forceinline__ DS<T, A> operator+(P<T> a, const DS<T, A>& b) {
+  DS<T, A> r = b;
+  r.v = a.v + b.v;
+  return r;
+}
+template <class T, uint32_t A> __device__ __forceinline__ DS<T, A> operator-(const DS<T, A>& a, P<T> b) {
+  DS<T, A> r = a;
+  r.v = a.v - b.v;
+  return r;
+}
+template <class T, uint32_t A> __device__ __forceinline__ DS<T, A> operator-(P<T> a, const DS<T, A>& b) {
+  DS<T, A> r;
+  r.v = a.v - b.v;
+  for (int i = 0; i < popc(A); ++i) r.d[i] = -b.d[i];
+  return r;
+}
+template <class T, uint32_t A> __device__ __forceinline__ DS<T, A> operator*(const DS<T, A>& a, P<T> b) {
+  DS<T, A> r;
+  r.v = a.v * b.v;
+  for (int i = 0; i < popc(A); ++i) r.d[i] = a.d[i] * b.v;
+  return r;
+}
+template <class T, uint32_t A> __device__ __forceinline__ DS<T, A> operator*(P<T> a, const DS<T, A>& b) {
+  DS<T, A> r;
+  r.v = a.v * b.v;
+  for (int i = 0; i < popc(A); ++i) r.d[i] = a.v * b.d[i];
+  return r;
+}
+template <class T, uint32_t A> __device__ __forceinline__ DS<T, A> operator/(const DS<T, A>& a, P<T> b) {
+  DS<T, A> r;
+  r.v = a.v / b.v;
+  for (int i = 0; i < popc(A); ++i) r.d[i] = a.d[i] / b.v;
+  return r;
+}
+template <class T, uint32_t A> __device__ __forceinline__ DS<T, A> operator/(P<T> a, const DS<T, A>& b) {
+  DS<T, A> r;
+  const T q = a.v / b.v;
+  r.v = q;
+  for (int i = 0; i < popc(A); ++i) r.d[i] = -q * b.d[i] / b.v;
+  return r;
+}
+#define POM_DS_SCALAR_OPS(OP)                                                                        \
+  template <class T, uint32_t A, class S, std::enable_if_t<arith_v<S>, int> = 0>                     \
+  __device__ __forceinline__ auto operator OP(S s, const DS<T, A>& b) { return P<T>{T(s)} OP b; }    \
+  template <class T, uint32_t A, class S, std::enable_if_t<arith_v<S>, int> = 0>                     \
+  __device__ __forceinline__ auto operator OP(const DS<T, A>& a, S s) { return a OP P<T>{T(s)}; }
+POM_DS_SCALAR_OPS(+)
+POM_DS_SCALAR_OPS(-)
+POM_DS_SCALAR_OPS(*)
+POM_DS_SCALAR_OPS(/)
+#undef POM_DS_SCALAR_OPS
+
+template <class T, uint32_t A> __device__ __forceinline__ DS<T, A> psqrt(const DS<T, A>& a) {
+  DS<T, A> r;
+  const T s = sqrt(a.v);
+  r.v = s;
+  for (int i = 0; i < popc(A); ++i) r.d[i] = a.d[i] / (T(2) * s);
+  return r;
+}
+template <class T, uint32_t A> __device__ __forceinline__ DS<T, A> pabs(const DS<T, A>& a) {
+  DS<T, A> r;
+  const T sg = a.v > T(0) ? T(1) : (a.v < T(0) ? T(-1) : T(0));
+  r.v = fabs(a.v);
+  for (int i = 0; i < popc(A); ++i) r.d[i] = sg * a.d[i];
+  return r;
+}
+// every lane of x multiplied by s (value unchanged)
+template <class T, uint32_t A> __device__ __forceinline__ DS<T, A> lanes_scaled(const DS<T, A>& x, T s) {
+  DS<T, A> r;
+  r.v = x.v;
+  for (int i = 0; i < popc(A); ++i) r.d[i] = x.d[i] * s;
+  return r;
+}
+
+// conversion into a common activity type (P -> D / DS, DS<A> -> DS<A|B>)
+template <class R, class X> __device__ __forceinline__ R cvt(X x) {
+  if constexpr (std::is_same_v<R, X>) {
+    return x;
+  } else if constexpr (mask_of<R>::value != 0) {
+    constexpr uint32_t MR = mask_of<R>::value, MX = mask_of<X>::value;
+    static_assert((MX & ~MR) == 0u, "cvt would drop tangent lanes");
+    R r{};
+    r.v = x.v;
+    if constexpr (MX != 0u) for_bits<MX>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      r.d[slot(MR, i)] = x.d[slot(MX, i)];
+    });
+    return r;
+  } else {
+    return R{x.v, decltype(x.v)(0)};
+  }
+}
+template <class T, uint32_t A, uint32_t B>
+__device__ __forceinline__ DS<T, A | B> sel(bool c, const DS<T, A>& a, const DS<T, B>& b) {
+  using R = DS<T, A | B>;
+  return c ? cvt<R>(a) : cvt<R>(b);
+}
+template <class T, uint32_t A> __device__ __forceinline__ DS<T, A> sel(bool c, const DS<T, A>& a, P<T> b) {
+  return c ? a : cvt<DS<T, A>>(b);
+}
+template <class T, uint32_t A> __device__ __forceinline__ DS<T, A> sel(bool c, P<T> a, const DS<T, A>& b) {
+  return c ? cvt<DS<T, A>>(a) : b;
+}
 
 // ------------------------------------------------------------------ batch
 struct Member {
@@ -262,6 +462,41 @@ struct Acc {
   __device__ __forceinline__ auto at2() const {
     T x = tab->f[F][ix2(p, ci(p.i + DI), cj(p.j + DJ))];
     if constexpr (F == SF && DI == SI && DJ == SJ) return D<T>{x, T(1)};
+    else return P<T>{x};
+  }
+  template <int G, int DI, int DJ>
+  __device__ __forceinline__ P<T> g() const {
+    return P<T>{tab->g[G][ix2(p, ci(p.i + DI), cj(p.j + DJ))]};
+  }
+  __device__ __forceinline__ T z(int k) const { return vert.z(p.m, k); }
+  __device__ __forceinline__ T zz(int k) const { return vert.zz(p.m, k); }
+  __device__ __forceinline__ T dz(int k) const { return vert.dz(p.m, k); }
+  __device__ __forceinline__ T dzz(int k) const { return vert.dzz(p.m, k); }
+  __device__ __forceinline__ T rampv() const { return ramp[p.m]; }
+};
+
+// Accessor seeding every field of MASK at the offset (SI, SJ, SK): 3-D fields
+// at (SI, SJ, SK), 2-D fields at (SI, SJ) (the caller leaves 2-D bits out of
+// MASK when SK != 0).
+template <class T, uint32_t MASK, int SI, int SJ, int SK>
+struct AccM {
+  const FieldTab<T>* tab;
+  Pos p;
+  Vert<T> vert;
+  const T* ramp;
+  __device__ __forceinline__ int ci(int i) const { return i < 0 ? 0 : (i >= p.im ? p.im - 1 : i); }
+  __device__ __forceinline__ int cj(int j) const { return j < 0 ? 0 : (j >= p.jm ? p.jm - 1 : j); }
+  __device__ __forceinline__ int ck(int k) const { return k < 0 ? 0 : (k >= KB ? KB - 1 : k); }
+  template <int F, int DI, int DJ, int DK>
+  __device__ __forceinline__ auto at() const {
+    T x = tab->f[F][ix3(p, ci(p.i + DI), cj(p.j + DJ), ck(p.k + DK))];
+    if constexpr (((MASK >> F) & 1u) && DI == SI && DJ == SJ && DK == SK) return DS<T, (1u << F)>{x, {T(1)}};
+    else return P<T>{x};
+  }
+  template <int F, int DI, int DJ>
+  __device__ __forceinline__ auto at2() const {
+    T x = tab->f[F][ix2(p, ci(p.i + DI), cj(p.j + DJ))];
+    if constexpr (((MASK >> F) & 1u) && DI == SI && DJ == SJ) return DS<T, (1u << F)>{x, {T(1)}};
     else return P<T>{x};
   }
   template <int G, int DI, int DJ>

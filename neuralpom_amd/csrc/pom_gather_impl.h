@@ -9,20 +9,25 @@ namespace pom {
 // inbar pointer is non-null
 template <class T, class Op>
 void launch_gather(const Ctx<T>& c, const FieldTab<T>& tab, const COutTab<T>& obar, T* const* inbar) {
-  static_for_host<0, Op::NIN - 1>([&](auto Fc) {
-    constexpr int F = decltype(Fc)::value;
-    if (!inbar[F]) return;
-    if constexpr (F < Op::N3) {
-      hipLaunchKernelGGL((k_gather3<T, Op, F>), dim3(c.nblk3), dim3(TPB), 0, c.stream, c.B, c.vert, c.ramp, c.prm,
-                         tab, obar, inbar[F]);
-    } else if constexpr (Op::OUT3) {
-      hipLaunchKernelGGL((k_gather2z<T, Op, F>), dim3(c.nblk3), dim3(TPB), 0, c.stream, c.B, c.vert, c.ramp,
-                         c.prm, tab, obar, inbar[F]);
-    } else {
-      hipLaunchKernelGGL((k_gather2<T, Op, F>), dim3(c.nblk2), dim3(TPB), 0, c.stream, c.B, c.vert, c.ramp, c.prm,
-                         tab, obar, inbar[F]);
-    }
-  });
+  InTab<T> it{};
+  bool any = false;
+  for (int f = 0; f < Op::NIN; ++f) {
+    it.o[f] = inbar[f];
+    any = any || inbar[f] != nullptr;
+  }
+  if (!any) return;
+  constexpr uint32_t ALL = (Op::NIN >= 32) ? ~0u : ((1u << Op::NIN) - 1u);
+  constexpr uint32_t M3 = (1u << Op::N3) - 1u;
+  constexpr uint32_t M2 = ALL & ~M3;
+  if constexpr (Op::OUT3) {
+    if (c.nblk3 == 0) return;
+    hipLaunchKernelGGL((k_gather_all3<T, Op, M3, M2>), dim3(c.nblk3), dim3(TPB), 0, c.stream, c.B, c.vert, c.ramp,
+                       c.prm, tab, obar, it);
+  } else {
+    if (c.nblk2 == 0) return;
+    hipLaunchKernelGGL((k_gather_all2<T, Op, M2>), dim3(c.nblk2), dim3(TPB), 0, c.stream, c.B, c.vert, c.ramp,
+                       c.prm, tab, obar, it);
+  }
 }
 
 }  // namespace pom

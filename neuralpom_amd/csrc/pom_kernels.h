@@ -127,6 +127,98 @@ __global__ void __launch_bounds__(TPB) k_gather2(Batch B, Vert<T> vert, const T*
   inbar[ix2(q, q.i, q.j)] += acc;
 }
 
+// ------------------------------------------------------------ fused gathers
+// All seeded input fields of an operator in one pass: at each stencil offset
+// the point function is evaluated once with every field seeded there (DS
+// tangents with one lane per field), and the lanes go to per-field
+// accumulators.  Same sums, in the same order, as one gather per field.
+template <class T> struct InTab { T* o[MAXF]; };
+
+template <class T, class Op, int DI, int DJ, int DK, uint32_t MASK>
+__device__ __forceinline__ void gather_all_term(const FieldTab<T>& tab, const COutTab<T>& obar, const Pos& q, int pk,
+                                                const Vert<T>& vert, const T* ramp, const Params& prm, T* acc) {
+  if constexpr (MASK != 0u) {
+    Pos p = q;
+    p.i = q.i - DI;
+    p.j = q.j - DJ;
+    p.k = pk;
+    if (p.i < 0 || p.i >= p.im || p.j < 0 || p.j >= p.jm || p.k < 0 || p.k >= KB) return;
+    AccM<T, MASK, DI, DJ, DK> A{&tab, p, vert, ramp};
+    const int64_t idx = Op::OUT3 ? ix3(p, p.i, p.j, p.k) : ix2(p, p.i, p.j);
+    static_for<0, Op::NOUT - 1>([&](auto O) {
+      constexpr int o = decltype(O)::value;
+      using R = decltype(Op::template eval<o>(A, prm));
+      constexpr uint32_t MR = mask_of<R>::value;
+      if constexpr (MR != 0u) {
+        if (obar.o[o]) {
+          const T ob = obar.o[o][idx];
+          const R r = Op::template eval<o>(A, prm);
+          for_bits<MR>([&](auto I) {
+            constexpr int f = decltype(I)::value;
+            acc[f] += ob * r.d[slot(MR, f)];
+          });
+        }
+      }
+    });
+  }
+}
+
+// 3-D operators: one thread per 3-D point q; 3-D fields accumulate at q, 2-D
+// fields per level (offsets with dk = 0 only) and are reduced over the 16
+// lanes of the water column
+template <class T, class Op, uint32_t M3, uint32_t M2>
+__global__ void __launch_bounds__(TPB) k_gather_all3(Batch B, Vert<T> vert, const T* ramp, Params prm,
+                                                      FieldTab<T> tab, COutTab<T> obar, InTab<T> inbar) {
+  Pos q;
+  const bool live = pos3(B, q);
+  T acc[Op::NIN];
+#pragma unroll
+  for (int f = 0; f < Op::NIN; ++f) acc[f] = T(0);
+  if (live) {
+    static_for<-Op::RI, Op::RI>([&](auto I) {
+      static_for<-Op::RJ, Op::RJ>([&](auto J) {
+        static_for<-Op::RK, Op::RK>([&](auto K) {
+          constexpr int di = decltype(I)::value, dj = decltype(J)::value, dk = decltype(K)::value;
+          gather_all_term<T, Op, di, dj, dk, (M3 | (dk == 0 ? M2 : 0u))>(tab, obar, q, q.k - dk, vert, ramp, prm,
+                                                                         acc);
+        });
+      });
+    });
+  }
+  static_for<0, Op::NIN - 1>([&](auto Fc) {
+    constexpr int f = decltype(Fc)::value;
+    if constexpr ((M3 >> f) & 1u) {
+      if (live && inbar.o[f]) inbar.o[f][ix3(q, q.i, q.j, q.k)] += acc[f];
+    } else if constexpr ((M2 >> f) & 1u) {
+      T v = acc[f];
+#pragma unroll
+      for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o, 16);
+      if (live && q.k == 0 && inbar.o[f]) inbar.o[f][ix2(q, q.i, q.j)] += v;
+    }
+  });
+}
+
+// 2-D operators: one thread per column
+template <class T, class Op, uint32_t M2>
+__global__ void __launch_bounds__(TPB) k_gather_all2(Batch B, Vert<T> vert, const T* ramp, Params prm,
+                                                      FieldTab<T> tab, COutTab<T> obar, InTab<T> inbar) {
+  Pos q;
+  if (!pos2(B, q)) return;
+  T acc[Op::NIN];
+#pragma unroll
+  for (int f = 0; f < Op::NIN; ++f) acc[f] = T(0);
+  static_for<-Op::RI, Op::RI>([&](auto I) {
+    static_for<-Op::RJ, Op::RJ>([&](auto J) {
+      constexpr int di = decltype(I)::value, dj = decltype(J)::value;
+      gather_all_term<T, Op, di, dj, 0, M2>(tab, obar, q, 0, vert, ramp, prm, acc);
+    });
+  });
+  static_for<0, Op::NIN - 1>([&](auto Fc) {
+    constexpr int f = decltype(Fc)::value;
+    if (inbar.o[f]) inbar.o[f][ix2(q, q.i, q.j)] += acc[f];
+  });
+}
+
 // ------------------------------------------------------------ host launchers
 template <class T, class Op>
 inline void launch_map(const Ctx<T>& c, const FieldTab<T>& tab, const OutTab<T>& out) {

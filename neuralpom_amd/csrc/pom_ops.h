@@ -118,8 +118,16 @@ template <class X, class Y, class Z> __device__ __forceinline__ auto norm3(X a, 
   using T = typename scalar_of<X>::type;
   T av = val(a), bv = val(b), cv = val(c);
   T n = sqrt(av * av + bv * bv + cv * cv);
-  T d = n > T(0) ? (av * tan_(a) + bv * tan_(b) + cv * tan_(c)) / n : T(0);
-  return D<T>{n, d};
+  if constexpr ((mask_of<X>::value | mask_of<Y>::value | mask_of<Z>::value) != 0u) {
+    auto num = a * P<T>{av} + b * P<T>{bv} + c * P<T>{cv};
+    auto r = num;
+    r.v = n;
+    for (int i = 0; i < popc(mask_of<decltype(num)>::value); ++i) r.d[i] = n > T(0) ? num.d[i] / n : T(0);
+    return r;
+  } else {
+    T d = n > T(0) ? (av * tan_(a) + bv * tan_(b) + cv * tan_(c)) / n : T(0);
+    return D<T>{n, d};
+  }
 }
 
 struct OpSmag {
@@ -311,8 +319,15 @@ template <class X> __device__ __forceinline__ auto abs_pow15(X x) {
   using T = typename scalar_of<X>::type;
   auto a = clamp_min(pabs(x), T(1e-8));
   T r = pow(val(a), T(1.5));
-  if constexpr (active_v<decltype(a)>) return D<T>{r, T(1.5) * sqrt(val(a)) * a.d};
-  else return P<T>{r};
+  if constexpr (mask_of<decltype(a)>::value != 0u) {
+    auto o = lanes_scaled(a, T(1.5) * sqrt(val(a)));
+    o.v = r;
+    return o;
+  } else if constexpr (active_v<decltype(a)>) {
+    return D<T>{r, T(1.5) * sqrt(val(a)) * a.d};
+  } else {
+    return P<T>{r};
+  }
 }
 
 struct OpDens {

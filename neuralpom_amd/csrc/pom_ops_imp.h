@@ -20,6 +20,12 @@ template <class T> __device__ __forceinline__ D<T> strain(D<T> a) {
   T s = sqrt(a.v);
   return {s, s > T(0) ? a.d * (T(0.5) / s) : T(0)};
 }
+template <class T, uint32_t M> __device__ __forceinline__ DS<T, M> strain(const DS<T, M>& a) {
+  const T s = sqrt(a.v);
+  DS<T, M> r = lanes_scaled(a, s > T(0) ? T(0.5) / s : T(0));
+  r.v = s;
+  return r;
+}
 
 // =====================================================================
 // Smagorinsky: aam = horcon dx dy sqrt(t1 + t2 + t3)  (time_loop.py:152-187)
