@@ -152,6 +152,42 @@ int pom_implicit_step_adj(const pom_batch* b, const pom_params_t* prm, const voi
                           const void* const* cot_out, void* const* grad_in, void* workspace, size_t ws_bytes,
                           void* stream);
 
+/* The semi-implicit core's elliptic operator and eta solve on their own, for
+ * callers that invoke or override them outside the step (case adapters,
+ * src/neuralpom/cores/case_adapters/coastal_boundary.py:62-64; tests).
+ * geom is the POM_NGEOM table (dx, dy, art, dum, dvm are read); the operator
+ * coefficient is 4 dti^2 grav alpha from prm.
+ *
+ * pom_elliptic_apply: out = A(d) eta = eta - coeff div(d grad eta) / art on
+ *   interior cells, out = eta on the edge ring.  Replaces
+ *   POM2K_Core.apply_elliptic_operator (implicit_p0_fixed/core.py:591-643).
+ * pom_elliptic_apply_adj: grad_eta += A(d)^T cot_out and
+ *   grad_d += (d(A(d) eta)/dd)^T cot_out (either may be null).
+ * pom_eta_workspace: workspace bytes of pom_eta_solve / pom_eta_solve_adj.
+ * pom_eta_solve: x = A(d)^-1 rhs by the step's batched PCG (core.py:460-589,
+ *   solve_implicit_eta_cg :775-793): start from the interior of x0 (null:
+ *   zero), rhs edge values ignored, x zero on the edge ring; absolute
+ *   tolerance prm->eta_tol, relative prm->eta_rel_tol, at most
+ *   prm->eta_max_iter iterations, preconditioner prm->eta_precond (the
+ *   spectral one needs the pom_implicit_prepare cache).  iters_out (device
+ *   int32 [n_members], may be null) receives each member's iteration count.
+ * pom_eta_solve_adj: backward of the solve (_ImplicitEtaSolveFunction,
+ *   core.py:24-59, 231-252, 645-675): writes grad_rhs = lambda =
+ *   area A^-1 (cot_x / area) (max-normalised right-hand side, absolute
+ *   tolerance scaled alike) and accumulates grad_d += -(d(A x)/dd)^T lambda
+ *   (grad_d may be null); x is the forward solution. */
+int pom_elliptic_apply(const pom_batch* b, const pom_params_t* prm, const void* const* geom, const void* eta,
+                       const void* d, void* out, void* stream);
+int pom_elliptic_apply_adj(const pom_batch* b, const pom_params_t* prm, const void* const* geom, const void* eta,
+                           const void* d, const void* cot_out, void* grad_eta, void* grad_d, void* stream);
+int pom_eta_workspace(const pom_batch* b, size_t* bytes);
+int pom_eta_solve(const pom_batch* b, const pom_params_t* prm, const void* cache, const void* const* geom,
+                  const void* rhs, const void* d, const void* x0, void* x, int32_t* iters_out, void* workspace,
+                  size_t ws_bytes, void* stream);
+int pom_eta_solve_adj(const pom_batch* b, const pom_params_t* prm, const void* cache, const void* const* geom,
+                      const void* x, const void* d, const void* cot_x, void* grad_rhs, void* grad_d,
+                      int32_t* iters_out, void* workspace, size_t ws_bytes, void* stream);
+
 /* Operator-level entry points: the reference's operator methods that callers
  * invoke outside the time loop (src/neuralpom/hybrid/sparse_coupling.py:10-15
  * calls dens through its eos(), :69-82 calls baropg).  Same kernels as the step.

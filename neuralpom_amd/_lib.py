@@ -84,6 +84,11 @@ def load():
                                       C.c_size_t, vp]
     lib.pom_implicit_step_adj.argtypes = [vp, C.POINTER(Params), vp, vp, vp, vpp, vpp, C.c_int32, vpp, vpp, vp,
                                           C.c_size_t, vp]
+    lib.pom_elliptic_apply.argtypes = [vp, C.POINTER(Params), vpp, vp, vp, vp, vp]
+    lib.pom_elliptic_apply_adj.argtypes = [vp, C.POINTER(Params), vpp, vp, vp, vp, vp, vp, vp]
+    lib.pom_eta_workspace.argtypes = [vp, C.POINTER(C.c_size_t)]
+    lib.pom_eta_solve.argtypes = [vp, C.POINTER(Params), vp, vpp, vp, vp, vp, vp, vp, vp, C.c_size_t, vp]
+    lib.pom_eta_solve_adj.argtypes = [vp, C.POINTER(Params), vp, vpp, vp, vp, vp, vp, vp, vp, vp, C.c_size_t, vp]
     lib.pom_profile_stage.argtypes = [C.c_int32]
     lib.pom_profile_read.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_int32)]
     for fn in EXPORTED:
@@ -95,7 +100,8 @@ def load():
 EXPORTED = ("pom_batch_create", "pom_batch_destroy", "pom_explicit_workspace", "pom_explicit_step",
             "pom_explicit_step_adj", "pom_implicit_cache_bytes", "pom_implicit_prepare", "pom_implicit_workspace",
             "pom_implicit_step", "pom_implicit_step_adj", "pom_dens", "pom_dens_adj", "pom_baropg",
-            "pom_baropg_adj", "pom_profile_stage", "pom_profile_read")
+            "pom_baropg_adj", "pom_elliptic_apply", "pom_elliptic_apply_adj", "pom_eta_workspace", "pom_eta_solve",
+            "pom_eta_solve_adj", "pom_profile_stage", "pom_profile_read")
 STAGES = {"profq": 0, "profq_adj": 1, "extuv": 2, "advct": 3}
 
 

@@ -152,6 +152,9 @@ def double_gyre(im, jm, dtype=torch.float32, device="cpu"):
     dlat, dlon = 20.0 / float(jm - 2), 22.0 / float(im - 2)
     j_idx = torch.arange(jm, dtype=dtype, device=device).view(1, -1).repeat(im, 1)
     north = 30.0 + (j_idx - 1.0) * dlat
+    i_idx = torch.arange(im, dtype=dtype, device=device).view(-1, 1).repeat(1, jm)
+    g["north_e"] = north.clone()                   # cases.py:54-55 (grid-point latitude / longitude)
+    g["east_e"] = (i_idx - 1.0) * dlon
     g["dy"] = torch.full((im, jm), re * dlat * deg2rad, dtype=dtype, device=device)
     g["dx"] = re * torch.cos(north * deg2rad) * dlon * deg2rad
     g["cor"] = 2.0 * omega * torch.sin(north * deg2rad)

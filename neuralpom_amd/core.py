@@ -50,7 +50,6 @@ class POM2K_Core:
         if device is None:
             device = torch.device("cuda", torch.cuda.current_device()) if torch.cuda.is_available() else "cpu"
         self.device = torch.device(device)
-        self.dtype = dtype
         self.iproblem = problem_id
         if self.iproblem != 4:
             raise ValueError(f"Only the Double-gyre problem_id=4 is supported; got {self.iproblem}.")
@@ -110,7 +109,11 @@ class POM2K_Core:
             setattr(self, n, v)
         for n in ("z", "zz", "dz", "dzz"):
             setattr(self, n, torch.zeros(KB, dtype=dtype, device=self.device))
+        self._core_defaults()
         self._apply_settings_overrides()
+
+    def _core_defaults(self) -> None:
+        """Class defaults that differ between the cores (set before settings overrides)."""
 
     # ------------------------------------------------------------ construction (core.py:28-61)
     @classmethod
@@ -161,13 +164,32 @@ class POM2K_Core:
         self.horcon = float(os.environ.get("NEURALPOM_HORCON", "0.05"))   # cases.py:152
 
     # ------------------------------------------------------------ kernels
+    @property
+    def dtype(self) -> torch.dtype:
+        """Arithmetic type of the step: that of the state tensors."""
+        return self.u.dtype
+
+    def _accel(self) -> dict:
+        """The kernel handles live in ``_accel_cache``, a name the reference's
+        capture / restore treats as rebuildable workspace (v1/checkpoint.py:17-18,
+        93-95), so ``vars(core)`` holds physical state only."""
+        acc = getattr(self, "_accel_cache", None)
+        if not isinstance(acc, dict):
+            acc = {}
+            self._accel_cache = acc
+        return acc
+
     def _ensemble(self) -> Ensemble:
+        acc = self._accel()
         key = (self.im, self.jm, self.dtype, str(self.device))
-        ens = getattr(self, "_ens_cache", None)
-        if ens is None or ens[0] != key:
-            ens = (key, Ensemble([(self.im, self.jm)], dtype=self.dtype, device=self.device))
-            object.__setattr__(self, "_ens_cache", ens)
-        return ens[1]
+        if acc.get("key") != key:
+            acc.clear()
+            acc["key"] = key
+            acc["ensemble"] = self._make_ensemble()
+        return acc["ensemble"]
+
+    def _make_ensemble(self):
+        return Ensemble([(self.im, self.jm)], dtype=self.dtype, device=self.device)
 
     def step_scalars(self) -> StepScalars:
         if self.nadv != 1:
@@ -195,11 +217,12 @@ class POM2K_Core:
         "Abnormal end" and returns before ``iend`` (time_loop.py:702-709);
         ``runtime_vamax_safe_fast_path`` skips that per-step check.
         """
+        self._before_time_loop()
         ens = self._ensemble()
         sc = self.step_scalars()
         vg = self._vgrid()
         geom = {n: self._flat(n).detach() for n in L.GEOM}
-        check = not getattr(self, "runtime_vamax_safe_fast_path", False)
+        check = self._vamax_checked()
         shapes = {n: tuple(getattr(self, n).shape) for n in L.FIELDS}
         for iint in range(1, int(self.iend) + 1):
             self.iint = iint
@@ -211,16 +234,34 @@ class POM2K_Core:
             ramp = torch.tensor([self.ramp], dtype=self.dtype, device=self.device)
             internal = not (iint == 1 and self.time0 == 0.0)
             state = {n: self._flat(n) for n in L.FIELDS}
-            if torch.is_grad_enabled() and any(v.requires_grad for v in state.values()):
-                out, vamax = step_autograd(ens, state, geom, sc, ramp, vg, internal)
-            else:
-                with torch.no_grad():
-                    out, vamax = ens.step({n: v.detach() for n, v in state.items()}, geom, sc, ramp, vg, internal)
+            out, vamax = self._advance(ens, state, geom, sc, ramp, vg, internal)
             for n in L.FIELDS:
                 setattr(self, n, out[n].view(shapes[n]))
             if check and not float(vamax) <= self.vmaxl:
-                print("Abnormal end")
+                self._after_time_loop()
+                self._vamax_exceeded(float(vamax))
                 return
+        self._after_time_loop()
+
+    # hooks of the time loop (the semi-implicit core overrides them)
+    def _before_time_loop(self) -> None:
+        pass
+
+    def _after_time_loop(self) -> None:
+        pass
+
+    def _vamax_checked(self) -> bool:
+        return not getattr(self, "runtime_vamax_safe_fast_path", False)
+
+    def _vamax_exceeded(self, vamax: float) -> None:
+        """time_loop.py:702-709: print and return before iend."""
+        print("Abnormal end")
+
+    def _advance(self, ens, state, geom, sc, ramp, vg, internal):
+        if torch.is_grad_enabled() and any(v.requires_grad for v in state.values()):
+            return step_autograd(ens, state, geom, sc, ramp, vg, internal)
+        with torch.no_grad():
+            return ens.step({n: v.detach() for n, v in state.items()}, geom, sc, ramp, vg, internal)
 
     # ------------------------------------------------------------ operator overrides
     def dens(self):

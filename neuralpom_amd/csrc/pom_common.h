@@ -475,6 +475,67 @@ struct Acc {
   __device__ __forceinline__ T rampv() const { return ramp[p.m]; }
 };
 
+// Clamped neighbour indices around a gather thread's point q: every access of
+// a gather (evaluation point q - o, access offset a) lands on q + (a - o),
+// |a - o| <= 4 horizontally and <= 2 vertically, so one table per thread
+// replaces the per-access clamping and index arithmetic.
+struct NbrIdx {
+  int32_t ri[9];   // clamp(q.i + d) * jm, d = -4..4
+  int32_t cj[9];   // clamp(q.j + d)
+  int32_t ck[5];   // clamp(q.k + d), d = -2..2
+};
+__device__ __forceinline__ void make_nbr(const Pos& q, NbrIdx& t) {
+#pragma unroll
+  for (int d = -4; d <= 4; ++d) {
+    const int i = q.i + d, j = q.j + d;
+    t.ri[d + 4] = (i < 0 ? 0 : (i >= q.im ? q.im - 1 : i)) * q.jm;
+    t.cj[d + 4] = j < 0 ? 0 : (j >= q.jm ? q.jm - 1 : j);
+  }
+#pragma unroll
+  for (int d = -2; d <= 2; ++d) {
+    const int k = q.k + d;
+    t.ck[d + 2] = k < 0 ? 0 : (k >= KB ? KB - 1 : k);
+  }
+}
+
+// Gather accessor: evaluation point p = q - (OI, OJ, OK); every field of MASK
+// is seeded at the offset (OI, OJ, OK) from p, i.e. at q (2-D fields when
+// the caller keeps them in MASK, which it does only for OK = 0).
+template <class T, uint32_t MASK, int OI, int OJ, int OK>
+struct AccT {
+  const FieldTab<T>* tab;
+  Pos p;
+  Vert<T> vert;
+  const T* ramp;
+  const NbrIdx* nb;
+  template <int DI, int DJ> __device__ __forceinline__ int32_t loc() const {
+    static_assert(DI - OI >= -4 && DI - OI <= 4 && DJ - OJ >= -4 && DJ - OJ <= 4, "stencil beyond table");
+    return nb->ri[DI - OI + 4] + nb->cj[DJ - OJ + 4];
+  }
+  template <int F, int DI, int DJ, int DK>
+  __device__ __forceinline__ auto at() const {
+    static_assert(DK - OK >= -2 && DK - OK <= 2, "stencil beyond table");
+    T x = tab->f[F][p.o3 + (int64_t)(loc<DI, DJ>() * KB + nb->ck[DK - OK + 2])];
+    if constexpr (((MASK >> F) & 1u) && DI == OI && DJ == OJ && DK == OK) return DS<T, (1u << F)>{x, {T(1)}};
+    else return P<T>{x};
+  }
+  template <int F, int DI, int DJ>
+  __device__ __forceinline__ auto at2() const {
+    T x = tab->f[F][p.o2 + loc<DI, DJ>()];
+    if constexpr (((MASK >> F) & 1u) && DI == OI && DJ == OJ) return DS<T, (1u << F)>{x, {T(1)}};
+    else return P<T>{x};
+  }
+  template <int G, int DI, int DJ>
+  __device__ __forceinline__ P<T> g() const {
+    return P<T>{tab->g[G][p.o2 + loc<DI, DJ>()]};
+  }
+  __device__ __forceinline__ T z(int k) const { return vert.z(p.m, k); }
+  __device__ __forceinline__ T zz(int k) const { return vert.zz(p.m, k); }
+  __device__ __forceinline__ T dz(int k) const { return vert.dz(p.m, k); }
+  __device__ __forceinline__ T dzz(int k) const { return vert.dzz(p.m, k); }
+  __device__ __forceinline__ T rampv() const { return ramp[p.m]; }
+};
+
 // Accessor seeding every field of MASK at the offset (SI, SJ, SK): 3-D fields
 // at (SI, SJ, SK), 2-D fields at (SI, SJ) (the caller leaves 2-D bits out of
 // MASK when SK != 0).

@@ -136,14 +136,15 @@ template <class T> struct InTab { T* o[MAXF]; };
 
 template <class T, class Op, int DI, int DJ, int DK, uint32_t MASK>
 __device__ __forceinline__ void gather_all_term(const FieldTab<T>& tab, const COutTab<T>& obar, const Pos& q, int pk,
-                                                const Vert<T>& vert, const T* ramp, const Params& prm, T* acc) {
+                                                const Vert<T>& vert, const T* ramp, const Params& prm,
+                                                const NbrIdx& nb, T* acc) {
   if constexpr (MASK != 0u) {
     Pos p = q;
     p.i = q.i - DI;
     p.j = q.j - DJ;
     p.k = pk;
     if (p.i < 0 || p.i >= p.im || p.j < 0 || p.j >= p.jm || p.k < 0 || p.k >= KB) return;
-    AccM<T, MASK, DI, DJ, DK> A{&tab, p, vert, ramp};
+    AccT<T, MASK, DI, DJ, DK> A{&tab, p, vert, ramp, &nb};
     const int64_t idx = Op::OUT3 ? ix3(p, p.i, p.j, p.k) : ix2(p, p.i, p.j);
     static_for<0, Op::NOUT - 1>([&](auto O) {
       constexpr int o = decltype(O)::value;
@@ -175,12 +176,14 @@ __global__ void __launch_bounds__(TPB) k_gather_all3(Batch B, Vert<T> vert, cons
 #pragma unroll
   for (int f = 0; f < Op::NIN; ++f) acc[f] = T(0);
   if (live) {
+    NbrIdx nb;
+    make_nbr(q, nb);
     static_for<-Op::RI, Op::RI>([&](auto I) {
       static_for<-Op::RJ, Op::RJ>([&](auto J) {
         static_for<-Op::RK, Op::RK>([&](auto K) {
           constexpr int di = decltype(I)::value, dj = decltype(J)::value, dk = decltype(K)::value;
           gather_all_term<T, Op, di, dj, dk, (M3 | (dk == 0 ? M2 : 0u))>(tab, obar, q, q.k - dk, vert, ramp, prm,
-                                                                         acc);
+                                                                         nb, acc);
         });
       });
     });
@@ -207,10 +210,12 @@ __global__ void __launch_bounds__(TPB) k_gather_all2(Batch B, Vert<T> vert, cons
   T acc[Op::NIN];
 #pragma unroll
   for (int f = 0; f < Op::NIN; ++f) acc[f] = T(0);
+  NbrIdx nb;
+  make_nbr(q, nb);
   static_for<-Op::RI, Op::RI>([&](auto I) {
     static_for<-Op::RJ, Op::RJ>([&](auto J) {
       constexpr int di = decltype(I)::value, dj = decltype(J)::value;
-      gather_all_term<T, Op, di, dj, 0, M2>(tab, obar, q, 0, vert, ramp, prm, acc);
+      gather_all_term<T, Op, di, dj, 0, M2>(tab, obar, q, 0, vert, ramp, prm, nb, acc);
     });
   });
   static_for<0, Op::NIN - 1>([&](auto Fc) {

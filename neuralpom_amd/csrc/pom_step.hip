@@ -1484,21 +1484,25 @@ int prepare_cache(const pom_batch* b, const pom_params_t* prm, const void* const
   return POM_OK;
 }
 
+// preconditioner choice and spectral cache of a PCG driver
+template <class T> void setup_pcg(Pcg<T>& w, const pom_batch* b, const pom_params_t* prm, const void* cache) {
+  w.spectral = prm->eta_precond == POM_PRECOND_SPECTRAL;
+  w.maxni = w.maxnj = 0;
+  for (const Member& m : b->h_mem) {
+    w.maxni = std::max(w.maxni, m.im - 2);
+    w.maxnj = std::max(w.maxnj, m.jm - 2);
+  }
+  if (cache) {
+    w.sp = reinterpret_cast<const SpecMember*>(static_cast<const char*>(cache) + CACHE_HEAD);
+    w.cache = reinterpret_cast<const T*>(static_cast<const char*>(cache) + cache_data_off(b->M));
+  }
+}
+
 // attach the semi-implicit settings and cache to a step
 template <class T> void setup_implicit(Step<T>& S, const pom_params_t* prm, const void* cache) {
   S.imp = prm->core == POM_CORE_IMPLICIT;
   S.hp = prm;
-  if (!S.imp) return;
-  S.pw.spectral = prm->eta_precond == POM_PRECOND_SPECTRAL;
-  S.pw.maxni = S.pw.maxnj = 0;
-  for (const Member& m : S.b->h_mem) {
-    S.pw.maxni = std::max(S.pw.maxni, m.im - 2);
-    S.pw.maxnj = std::max(S.pw.maxnj, m.jm - 2);
-  }
-  if (cache) {
-    S.pw.sp = reinterpret_cast<const SpecMember*>(static_cast<const char*>(cache) + CACHE_HEAD);
-    S.pw.cache = reinterpret_cast<const T*>(static_cast<const char*>(cache) + cache_data_off(S.b->M));
-  }
+  if (S.imp) setup_pcg(S.pw, S.b, prm, cache);
 }
 
 template <class T>
@@ -1619,6 +1623,117 @@ int run_baropg(const pom_batch* b, const pom_params_t* prm, const void* ramp, co
     hipLaunchKernelGGL(k_baropg_adj<T>, g2, tb, 0, c.stream, c.B, c.vert, c.ramp, c.prm, a);
   }
   return hipGetLastError() == hipSuccess ? POM_OK : POM_E_HIP;
+}
+
+// ---------------------------------------------------------------- eta solve entry points
+// The semi-implicit core's elliptic operator and eta solve as stand-alone
+// calls (solve_implicit_eta_cg / apply_elliptic_operator and the adjoint of
+// _ImplicitEtaSolveFunction), on the same kernels as the step.
+template <class T> struct EtaWs {
+  Pcg<T> w;
+  T *a, *x, *nlam;
+  void plan(Arena& A, const pom_batch* b) {
+    w.plan(A, b->n2, b->nblk2, b->M);
+    a = A.get<T>(b->n2);
+    x = A.get<T>(b->n2);
+    nlam = A.get<T>(b->n2);
+  }
+};
+
+template <class T> size_t eta_ws_size(const pom_batch* b) {
+  Arena A{nullptr, 0, 0, true};
+  EtaWs<T> e{};
+  e.plan(A, b);
+  return A.off + 256;
+}
+
+template <class T> EllGeom<T> ell_geom(const Ctx<T>& c, const void* const* geom) {
+  auto G = [&](int f) { return static_cast<const T*>(geom[f]); };
+  return EllGeom<T>{G(POM_G_DX), G(POM_G_DY), G(POM_G_ART), G(POM_G_DUM), G(POM_G_DVM),
+                    4.0 * (c.prm.dti * c.prm.dti) * c.prm.grav * c.prm.alpha};
+}
+
+template <class T> FieldTab<T> ell_tab(const void* const* geom, const void* eta, const void* d) {
+  FieldTab<T> t{};
+  t.f[OpEll::ETA] = static_cast<const T*>(eta);
+  t.f[OpEll::D_] = static_cast<const T*>(d);
+  t.g[OpEll::GDX] = static_cast<const T*>(geom[POM_G_DX]);
+  t.g[OpEll::GDY] = static_cast<const T*>(geom[POM_G_DY]);
+  t.g[OpEll::GART] = static_cast<const T*>(geom[POM_G_ART]);
+  t.g[OpEll::GDUM] = static_cast<const T*>(geom[POM_G_DUM]);
+  t.g[OpEll::GDVM] = static_cast<const T*>(geom[POM_G_DVM]);
+  return t;
+}
+
+template <class T>
+int run_elliptic(const pom_batch* b, const pom_params_t* prm, const void* const* geom, const void* eta, const void* d,
+                 void* out, const void* gout, void* b_eta, void* b_d, hipStream_t st, bool adjoint) {
+  Ctx<T> c = make_ctx<T>(b, prm, nullptr, nullptr, st);
+  if (c.nblk2 == 0) return POM_OK;
+  const FieldTab<T> t = ell_tab<T>(geom, eta, d);
+  if (!adjoint) {
+    launch_map<T, OpEll>(c, t, OutTab<T>{{static_cast<T*>(out)}});
+  } else {
+    COutTab<T> ob{{static_cast<const T*>(gout)}};
+    T* ib[OpEll::NIN] = {static_cast<T*>(b_eta), static_cast<T*>(b_d)};
+    launch_gather<T, OpEll>(c, t, ob, ib);
+  }
+  return hipGetLastError() == hipSuccess ? POM_OK : POM_E_HIP;
+}
+
+template <class T>
+int run_eta_solve(const pom_batch* b, const pom_params_t* prm, const void* cache, const void* const* geom,
+                  const void* rhs, const void* d, const void* x0, void* x, int32_t* iters, void* ws, size_t wsb,
+                  hipStream_t st) {
+  Ctx<T> c = make_ctx<T>(b, prm, nullptr, nullptr, st);
+  if (c.nblk2 == 0) return POM_OK;
+  Arena A{static_cast<char*>(ws), 0, wsb, false};
+  EtaWs<T> e{};
+  e.plan(A, b);
+  if (A.off > wsb) return POM_E_ARG;
+  setup_pcg(e.w, b, prm, cache);
+  hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(256), 0, st, e.w.tol, (int64_t)b->M, prm->eta_tol);
+  e.w.solve(c, ell_geom<T>(c, geom), static_cast<const T*>(rhs), static_cast<const T*>(d),
+            static_cast<const T*>(x0), static_cast<T*>(x), prm->eta_rel_tol, prm->eta_max_iter);
+  if (iters) hipLaunchKernelGGL(k_copy<int32_t>, dim3(1), dim3(256), 0, st, iters, e.w.its, (int64_t)b->M);
+  return hipGetLastError() == hipSuccess ? POM_OK : POM_E_HIP;
+}
+
+// lambda = area A^-1 (g / area) on the max-normalised right-hand side
+// (core.py:231-252); b_rhs = lambda; b_d += -(d(A x)/dd)^T lambda (core.py:645-675)
+template <class T>
+int run_eta_solve_adj(const pom_batch* b, const pom_params_t* prm, const void* cache, const void* const* geom,
+                      const void* x, const void* d, const void* gx, void* b_rhs, void* b_d, int32_t* iters, void* ws,
+                      size_t wsb, hipStream_t st) {
+  Ctx<T> c = make_ctx<T>(b, prm, nullptr, nullptr, st);
+  if (c.nblk2 == 0) return POM_OK;
+  Arena A{static_cast<char*>(ws), 0, wsb, false};
+  EtaWs<T> e{};
+  e.plan(A, b);
+  if (A.off > wsb) return POM_E_ARG;
+  setup_pcg(e.w, b, prm, cache);
+  const dim3 g2(c.nblk2), tb(TPB), gm(c.B.M);
+  const T* art = static_cast<const T*>(geom[POM_G_ART]);
+  hipLaunchKernelGGL(k_eta_adj_in<T>, g2, tb, 0, st, c.B, static_cast<const T*>(gx), art, e.a, e.w.part);
+  hipLaunchKernelGGL(k_eta_adj_scale, gm, tb, 0, st, c.B, e.w.part, e.w.sc, e.w.tol, prm->eta_tol);
+  hipLaunchKernelGGL(k_eta_adj_norm<T>, g2, tb, 0, st, c.B, e.a, e.w.sc);
+  e.w.solve(c, ell_geom<T>(c, geom), e.a, static_cast<const T*>(d), nullptr, e.x, prm->eta_rel_tol,
+            prm->eta_max_iter);
+  hipLaunchKernelGGL(k_eta_adj_out<T>, g2, tb, 0, st, c.B, e.x, art, e.w.sc, static_cast<T*>(b_rhs), e.nlam);
+  if (iters) hipLaunchKernelGGL(k_copy<int32_t>, dim3(1), dim3(256), 0, st, iters, e.w.its, (int64_t)b->M);
+  if (b_d) {
+    COutTab<T> ob{{e.nlam}};
+    T* ib[OpEll::NIN] = {nullptr, static_cast<T*>(b_d)};
+    launch_gather<T, OpEll>(c, ell_tab<T>(geom, x, d), ob, ib);
+  }
+  return hipGetLastError() == hipSuccess ? POM_OK : POM_E_HIP;
+}
+
+bool ell_geom_ok(const void* const* geom) {
+  if (!geom) return false;
+  for (int f : {POM_G_DX, POM_G_DY, POM_G_ART, POM_G_DUM, POM_G_DVM})
+    if (!geom[f]) return false;
+  return true;
 }
 
 }  // namespace
@@ -1765,6 +1880,55 @@ int pom_baropg_adj(const pom_batch* b, const pom_params_t* prm, const void* ramp
   return run_baropg<double>(b, prm, ramp, vgrid, in, nullptr, cot_out, grad_in, st, true);
 }
 
+
+int pom_elliptic_apply(const pom_batch* b, const pom_params_t* prm, const void* const* geom, const void* eta,
+                       const void* d, void* out, void* stream) {
+  if (!b || !prm || !ell_geom_ok(geom) || !eta || !d || !out) return POM_E_ARG;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (b->dtype == POM_F32) return run_elliptic<float>(b, prm, geom, eta, d, out, nullptr, nullptr, nullptr, st, false);
+  return run_elliptic<double>(b, prm, geom, eta, d, out, nullptr, nullptr, nullptr, st, false);
+}
+
+int pom_elliptic_apply_adj(const pom_batch* b, const pom_params_t* prm, const void* const* geom, const void* eta,
+                           const void* d, const void* cot_out, void* grad_eta, void* grad_d, void* stream) {
+  if (!b || !prm || !ell_geom_ok(geom) || !eta || !d || !cot_out) return POM_E_ARG;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (b->dtype == POM_F32)
+    return run_elliptic<float>(b, prm, geom, eta, d, nullptr, cot_out, grad_eta, grad_d, st, true);
+  return run_elliptic<double>(b, prm, geom, eta, d, nullptr, cot_out, grad_eta, grad_d, st, true);
+}
+
+int pom_eta_workspace(const pom_batch* b, size_t* bytes) {
+  if (!b || !bytes) return POM_E_ARG;
+  *bytes = b->dtype == POM_F32 ? eta_ws_size<float>(b) : eta_ws_size<double>(b);
+  return POM_OK;
+}
+
+int pom_eta_solve(const pom_batch* b, const pom_params_t* prm, const void* cache, const void* const* geom,
+                  const void* rhs, const void* d, const void* x0, void* x, int32_t* iters_out, void* workspace,
+                  size_t ws_bytes, void* stream) {
+  if (!b || !ell_geom_ok(geom) || !rhs || !d || !x || !workspace) return POM_E_ARG;
+  if (int e = check_params(prm, POM_CORE_IMPLICIT)) return e;
+  if (prm->eta_precond == POM_PRECOND_SPECTRAL && !cache) return POM_E_ARG;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (b->dtype == POM_F32)
+    return run_eta_solve<float>(b, prm, cache, geom, rhs, d, x0, x, iters_out, workspace, ws_bytes, st);
+  return run_eta_solve<double>(b, prm, cache, geom, rhs, d, x0, x, iters_out, workspace, ws_bytes, st);
+}
+
+int pom_eta_solve_adj(const pom_batch* b, const pom_params_t* prm, const void* cache, const void* const* geom,
+                      const void* x, const void* d, const void* cot_x, void* grad_rhs, void* grad_d,
+                      int32_t* iters_out, void* workspace, size_t ws_bytes, void* stream) {
+  if (!b || !ell_geom_ok(geom) || !x || !d || !cot_x || !grad_rhs || !workspace) return POM_E_ARG;
+  if (int e = check_params(prm, POM_CORE_IMPLICIT)) return e;
+  if (prm->eta_precond == POM_PRECOND_SPECTRAL && !cache) return POM_E_ARG;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (b->dtype == POM_F32)
+    return run_eta_solve_adj<float>(b, prm, cache, geom, x, d, cot_x, grad_rhs, grad_d, iters_out, workspace,
+                                    ws_bytes, st);
+  return run_eta_solve_adj<double>(b, prm, cache, geom, x, d, cot_x, grad_rhs, grad_d, iters_out, workspace,
+                                   ws_bytes, st);
+}
 
 int pom_implicit_cache_bytes(const pom_batch* b, size_t* bytes) {
   if (!b || !bytes) return POM_E_ARG;

@@ -134,13 +134,125 @@ class ImplicitEnsemble(Ensemble):
         return grads
 
 
+    # ---------------------------------------------------------------- eta solve on its own
+    def _ws_eta(self):
+        n = C.c_size_t()
+        L.check(self.lib.pom_eta_workspace(self._h, C.byref(n)), "pom_eta_workspace")
+        ws = self._ws.get("eta")
+        if ws is None or ws.numel() < n.value:
+            ws = torch.empty(n.value, dtype=torch.uint8, device=self.device)
+            self._ws["eta"] = ws
+        return ws, n.value
+
+    def _geom_ptrs(self, geom):
+        gs = [geom[n] for n in L.GEOM]
+        self._check(gs, L.GEOM, self.numel)
+        return L.ptr_array([t.data_ptr() for t in gs])
+
+    def _flat2(self, x, name):
+        x = x.detach().to(self.device, self.dtype).contiguous().reshape(-1)
+        if x.numel() != self.numel("el"):
+            raise ValueError(f"{name}: expected {self.numel('el')} elements, got {x.numel()}")
+        return x
+
+    def elliptic_apply(self, geom, sc: ImplicitScalars, eta, d):
+        """A(d) eta (implicit_p0_fixed/core.py:591-643) for every member."""
+        eta, d = self._flat2(eta, "eta"), self._flat2(d, "d")
+        out = torch.empty_like(eta)
+        prm = sc.struct()
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        L.check(self.lib.pom_elliptic_apply(self._h, C.byref(prm), self._geom_ptrs(geom), eta.data_ptr(),
+                                            d.data_ptr(), out.data_ptr(), stream), "pom_elliptic_apply")
+        return out
+
+    def elliptic_apply_adj(self, geom, sc: ImplicitScalars, eta, d, g):
+        """(A^T g, (d(A eta)/dd)^T g)."""
+        eta, d, g = self._flat2(eta, "eta"), self._flat2(d, "d"), self._flat2(g, "cotangent")
+        b_eta, b_d = torch.zeros_like(eta), torch.zeros_like(d)
+        prm = sc.struct()
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        L.check(self.lib.pom_elliptic_apply_adj(self._h, C.byref(prm), self._geom_ptrs(geom), eta.data_ptr(),
+                                                d.data_ptr(), g.data_ptr(), b_eta.data_ptr(), b_d.data_ptr(), stream),
+                "pom_elliptic_apply_adj")
+        return b_eta, b_d
+
+    def eta_solve(self, geom, sc: ImplicitScalars, rhs, d, x0=None, iters=None):
+        """x = A(d)^-1 rhs by the batched PCG (core.py:460-589); ``x0`` is the initial
+        guess (interior values; None: zero); ``iters`` an optional int32 device
+        tensor [members] receiving the iteration counts."""
+        if not self._prepared:
+            raise RuntimeError("ImplicitEnsemble.prepare(geom, scalars) must run before the eta solve")
+        rhs, d = self._flat2(rhs, "rhs"), self._flat2(d, "d")
+        x0 = self._flat2(x0, "initial_guess") if x0 is not None else None
+        x = torch.empty_like(rhs)
+        prm = sc.struct()
+        ws, nb = self._ws_eta()
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        L.check(self.lib.pom_eta_solve(self._h, C.byref(prm), self.cache.data_ptr(), self._geom_ptrs(geom),
+                                       rhs.data_ptr(), d.data_ptr(), x0.data_ptr() if x0 is not None else None,
+                                       x.data_ptr(), iters.data_ptr() if iters is not None else None,
+                                       ws.data_ptr(), nb, stream), "pom_eta_solve")
+        return x
+
+    def eta_solve_adj(self, geom, sc: ImplicitScalars, x, d, g, iters=None):
+        """Backward of the solve (core.py:24-59): (lambda, -(d(A x)/dd)^T lambda)."""
+        if not self._prepared:
+            raise RuntimeError("ImplicitEnsemble.prepare(geom, scalars) must run before the eta solve")
+        x, d, g = self._flat2(x, "x"), self._flat2(d, "d"), self._flat2(g, "cotangent")
+        lam, b_d = torch.empty_like(x), torch.zeros_like(d)
+        prm = sc.struct()
+        ws, nb = self._ws_eta()
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        L.check(self.lib.pom_eta_solve_adj(self._h, C.byref(prm), self.cache.data_ptr(), self._geom_ptrs(geom),
+                                           x.data_ptr(), d.data_ptr(), g.data_ptr(), lam.data_ptr(), b_d.data_ptr(),
+                                           iters.data_ptr() if iters is not None else None, ws.data_ptr(), nb,
+                                           stream), "pom_eta_solve_adj")
+        return lam, b_d
+
+
+class EtaSolve(torch.autograd.Function):
+    """``_ImplicitEtaSolveFunction`` (core.py:24-59): x = A(d)^-1 rhs; backward
+    returns (lambda, -grad_D) from a second PCG.  ctx keeps the ensemble, the
+    detached geometry and the scalars, never a core."""
+
+    @staticmethod
+    def forward(ctx, ens, geom, sc, rhs, d, x0, iters=None):
+        x = ens.eta_solve(geom, sc, rhs, d, x0, iters).view(rhs.shape)
+        ctx.ens, ctx.geom, ctx.sc = ens, geom, sc
+        ctx.save_for_backward(x, d)
+        return x
+
+    @staticmethod
+    def backward(ctx, g):
+        x, d = ctx.saved_tensors
+        lam, b_d = ctx.ens.eta_solve_adj(ctx.geom, ctx.sc, x, d, g)
+        return None, None, None, lam.view(x.shape), b_d.view(d.shape), None, None
+
+
+class Elliptic(torch.autograd.Function):
+    """A(d) eta as an autograd node (linear in eta; bilinear with d)."""
+
+    @staticmethod
+    def forward(ctx, ens, geom, sc, eta, d):
+        out = ens.elliptic_apply(geom, sc, eta, d).view(eta.shape)
+        ctx.ens, ctx.geom, ctx.sc = ens, geom, sc
+        ctx.save_for_backward(eta, d)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        eta, d = ctx.saved_tensors
+        b_eta, b_d = ctx.ens.elliptic_apply_adj(ctx.geom, ctx.sc, eta, d, g)
+        return None, None, None, b_eta.view(eta.shape), b_d.view(d.shape)
+
+
 class ImplicitStep(torch.autograd.Function):
     """One semi-implicit internal step as an autograd node (inputs/outputs in FIELDS order)."""
 
     @staticmethod
-    def forward(ctx, ens, geom, sc, ramp, vgrid, internal, *inputs):
+    def forward(ctx, ens, geom, sc, ramp, vgrid, internal, iters, *inputs):
         state = dict(zip(L.FIELDS, inputs))
-        out, vamax = ens.step(state, geom, sc, ramp, vgrid, internal)
+        out, vamax = ens.step(state, geom, sc, ramp, vgrid, internal, iters)
         ctx.ens, ctx.geom, ctx.sc, ctx.internal = ens, geom, sc, internal
         ctx.save_for_backward(ramp, vgrid, *inputs)
         ctx.mark_non_differentiable(vamax)
@@ -152,9 +264,9 @@ class ImplicitStep(torch.autograd.Function):
         state = dict(zip(L.FIELDS, inputs))
         cot = {n: g for n, g in zip(L.FIELDS, gouts[:len(L.FIELDS)]) if g is not None}
         grads = ctx.ens.step_adj(state, ctx.geom, ctx.sc, ramp, vgrid, ctx.internal, cot)
-        return (None,) * 6 + tuple(grads.get(n) for n in L.FIELDS)
+        return (None,) * 7 + tuple(grads.get(n) for n in L.FIELDS)
 
 
-def step_autograd(ens, state, geom, sc, ramp, vgrid, internal=True):
-    outs = ImplicitStep.apply(ens, geom, sc, ramp, vgrid, internal, *[state[n] for n in L.FIELDS])
+def step_autograd(ens, state, geom, sc, ramp, vgrid, internal=True, iters=None):
+    outs = ImplicitStep.apply(ens, geom, sc, ramp, vgrid, internal, iters, *[state[n] for n in L.FIELDS])
     return dict(zip(L.FIELDS, outs[:-1])), outs[-1]
