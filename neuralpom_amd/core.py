@@ -204,10 +204,7 @@ class POM2K_Core:
             .detach().contiguous()
 
     def _flat(self, n):
-        v = getattr(self, n)
-        if v.dtype != self.dtype or v.device != self.device:
-            v = v.to(self.device, self.dtype)
-        return v.contiguous().reshape(-1)
+        return self._as_flat(getattr(self, n))
 
     def run_time_loop(self):
         """Advance ``iend`` internal steps from ``time0`` (time_loop.py:20-709).
@@ -266,20 +263,37 @@ class POM2K_Core:
     # ------------------------------------------------------------ operator overrides
     def dens(self):
         """Equation of state into ``self.rho`` (momentum.py:467-541); returns (s, t, rho)."""
-        rho = _Dens.apply(self._ensemble(), self._params(), self._vgrid(), self._flat("t"), self._flat("s"),
-                          self._flat("rho"), self._flat("h").detach(), self._flat("fsm").detach())
-        self.rho = rho.view(self.im, self.jm, KB)
+        self.rho = self.eos(self.t, self.s)
         return self.s, self.t, self.rho
+
+    def eos(self, t, s):
+        """The core's equation of state for given t, s without touching the core:
+        levels 0-14 from (t, s), level 15 kept from ``self.rho`` (the ghost rule
+        of dens, as src/neuralpom/hybrid/sparse_coupling.py:10-15 evaluates it)."""
+        rho = _Dens.apply(self._ensemble(), self._params(), self._vgrid(), self._as_flat(t), self._as_flat(s),
+                          self._flat("rho"), self._flat("h").detach(), self._flat("fsm").detach())
+        return rho.view(self.im, self.jm, KB)
 
     def baropg(self):
         """Baroclinic pressure gradient (momentum.py:543-700); returns (rho, drhox, drhoy)."""
+        rho, drhox, drhoy = self.baropg_of(self.rho)
+        self.rho = rho
+        return self.rho, drhox, drhoy
+
+    def baropg_of(self, rho, rmean=None):
+        """baropg for a given density (and mean density, default ``self.rmean``)
+        without touching the core; returns (rho, drhox, drhoy)."""
         ramp = torch.tensor([float(self.ramp)], dtype=self.dtype, device=self.device)
-        rho, drhox, drhoy = _Baropg.apply(self._ensemble(), self._params(), ramp, self._vgrid(),
-                                          self._flat("rho"), self._flat("rmean").detach(), self._flat("dt"),
-                                          *(self._flat(n).detach() for n in ("dx", "dy", "dum", "dvm")))
+        rm = self._flat("rmean") if rmean is None else self._as_flat(rmean)
+        out = _Baropg.apply(self._ensemble(), self._params(), ramp, self._vgrid(), self._as_flat(rho), rm.detach(),
+                            self._flat("dt"), *(self._flat(n).detach() for n in ("dx", "dy", "dum", "dvm")))
         shape = (self.im, self.jm, KB)
-        self.rho = rho.view(shape)
-        return self.rho, drhox.view(shape), drhoy.view(shape)
+        return tuple(x.view(shape) for x in out)
+
+    def _as_flat(self, v):
+        if v.dtype != self.dtype or v.device != self.device:
+            v = v.to(self.device, self.dtype)
+        return v.contiguous().reshape(-1)
 
     def _params(self):
         return self.step_scalars().struct()

@@ -795,7 +795,7 @@ template <class T> struct Step {
   // ------------------------------------------------------------ forward
   void forward() {
     const dim3 g2(c.nblk2), tb(TPB);
-    hipLaunchKernelGGL(k_fill<T>, dim3(1), dim3(64), 0, c.stream, vamax, (int64_t)1, T(0));
+    if (vamax) hipLaunchKernelGGL(k_fill<T>, dim3(1), dim3(64), 0, c.stream, vamax, (int64_t)1, T(0));
     {   // surface forcing reset (time_loop.py:118-164)
       ForcingArgs<T> a{in[POM_E_ATMOS], in[POM_VFLUXF], in[POM_SWRAD], in[POM_WTSURF], in[POM_WSSURF],
                        in[POM_T], in[POM_S], out[POM_E_ATMOS], out[POM_VFLUXF], out[POM_SWRAD],
@@ -853,8 +853,9 @@ template <class T> struct Step {
       hipLaunchKernelGGL(k_copy<int32_t>, dim3(1), dim3(256), 0, c.stream, iters_out, pw.its, (int64_t)b->M);
     launch_map<T, OpImpSurf>(c, t_surf(), OutTab<T>{{out[POM_ELF]}});
     launch_map<T, OpImpCorr>(c, t_corr(), OutTab<T>{{out[POM_UAF], out[POM_VAF]}});
-    hipLaunchKernelGGL(k_vamax<T>, dim3(std::min(blocks(N2), 1024)), dim3(256), 0, c.stream, out[POM_VAF], N2,
-                       vamax);
+    if (vamax)
+      hipLaunchKernelGGL(k_vamax<T>, dim3(std::min(blocks(N2), 1024)), dim3(256), 0, c.stream, out[POM_VAF], N2,
+                         vamax);
     launch_map<T, OpImpFilter>(c, t_ifilter(),
                                OutTab<T>{{out[POM_UA], out[POM_UAB], out[POM_VA], out[POM_VAB], out[POM_EL],
                                           out[POM_ELB], out[POM_D], out[POM_EGF], out[POM_UTF], out[POM_VTF],
@@ -883,7 +884,9 @@ template <class T> struct Step {
         ProfScope ps(POM_STAGE_EXTUV, c.stream);
         launch_map<T, OpExtUV>(cn, t_uv(xp, xn), OutTab<T>{{xn.uaf, xn.vaf}});
       }
-      hipLaunchKernelGGL(k_vamax<T>, dim3(std::min(blocks(N2), 1024)), dim3(256), 0, c.stream, xn.vaf, N2, vamax);
+      if (vamax)
+        hipLaunchKernelGGL(k_vamax<T>, dim3(std::min(blocks(N2), 1024)), dim3(256), 0, c.stream, xn.vaf, N2,
+                           vamax);
       launch_map<T, OpExtFilter>(cn, t_filter(xp, xn),
                                  OutTab<T>{{xn.ua, xn.uab, xn.va, xn.vab, xn.el, xn.elb, xn.d, xn.egf, xn.utf, xn.vtf,
                                             xn.etf}});
@@ -1571,7 +1574,7 @@ int run_adj(const pom_batch* b, const pom_params_t* prm, const void* ramp, const
   }
   Arena A{static_cast<char*>(ws), 0, wsb, false};
   S.plan(A, true);
-  S.vamax = A.get<T>(1);
+  S.vamax = nullptr;   // the recompute needs no depth-mean velocity check
   Adj<T> J(S, S.c);
   J.plan(A, reinterpret_cast<T* const*>(grad));
   if (A.off > wsb) return POM_E_ARG;
@@ -1767,13 +1770,15 @@ int pom_batch_create(const pom_member_t* members, int32_t n, int32_t dtype, pom_
   if (hipMalloc(&b->d_mem, sizeof(Member) * n) != hipSuccess ||
       hipMalloc(&b->d_blk3, sizeof(int32_t) * (n + 1)) != hipSuccess ||
       hipMalloc(&b->d_blk2, sizeof(int32_t) * (n + 1)) != hipSuccess) {
-    delete b;
+    pom_batch_destroy(b);
     return POM_E_HIP;
   }
   if (hipMemcpy(b->d_mem, hm.data(), sizeof(Member) * n, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(b->d_blk3, b3.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(b->d_blk2, b2.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice) != hipSuccess)
+      hipMemcpy(b->d_blk2, b2.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice) != hipSuccess) {
+    pom_batch_destroy(b);
     return POM_E_HIP;
+  }
   *out = b;
   return POM_OK;
 }
