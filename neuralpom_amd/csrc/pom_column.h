@@ -75,14 +75,23 @@ __device__ __forceinline__ ColIO<T> col_pos(const Batch& B, Pos& p, T* lds) {
 // padded LDS slot of tile element e (column e / KB, level e % KB)
 __device__ __forceinline__ int tile_ix(int e) { return e + e / KB; }
 
-template <class T> __device__ __forceinline__ void col_ld(const ColIO<T>& io, const T* __restrict__ f, T* x) {
-  const T* g = f + io.base;
-  const int tot = io.n * KB;
+// Loads are issued for all KB rows before the first LDS store (indices past
+// the block's last column re-read its last element), so a thread's loads
+// overlap instead of each waiting for the previous store.
+template <class T> __device__ __forceinline__ void tile_fill(const ColIO<T>& io, const T* __restrict__ g, int tot) {
+  if (tot <= 0) return;   // uniform over the block
+  T v[KB];
   #pragma unroll
   for (int r = 0; r < KB; ++r) {
     const int e = r * TPB + (int)threadIdx.x;
-    if (e < tot) io.lds[tile_ix(e)] = g[e];
+    v[r] = g[e < tot ? e : tot - 1];
   }
+  #pragma unroll
+  for (int r = 0; r < KB; ++r) io.lds[tile_ix(r * TPB + (int)threadIdx.x)] = v[r];
+}
+
+template <class T> __device__ __forceinline__ void col_ld(const ColIO<T>& io, const T* __restrict__ f, T* x) {
+  tile_fill(io, f + io.base, io.n * KB);
   __syncthreads();
   #pragma unroll
   for (int k = 0; k < KB; ++k) x[k] = io.lds[threadIdx.x * (KB + 1) + k];
@@ -96,12 +105,7 @@ template <class T>
 __device__ __forceinline__ void col_ld_shift(const ColIO<T>& io, const T* __restrict__ f, int64_t shift, T* x) {
   const int64_t avail = io.ncol - io.first - shift;
   const int tot = (int)(avail < io.n ? (avail > 0 ? avail : 0) : io.n) * KB;
-  const T* g = f + io.base + shift * KB;
-  #pragma unroll
-  for (int r = 0; r < KB; ++r) {
-    const int e = r * TPB + (int)threadIdx.x;
-    if (e < tot) io.lds[tile_ix(e)] = g[e];
-  }
+  tile_fill(io, f + io.base + shift * KB, tot);
   __syncthreads();
   #pragma unroll
   for (int k = 0; k < KB; ++k) x[k] = io.lds[threadIdx.x * (KB + 1) + k];
@@ -187,9 +191,16 @@ __global__ void __launch_bounds__(TPB) k_baropg(Batch B, Vert<T> vg, const T* ra
   Pos p;
   if (!pos2(B, p)) return;
   const int64_t c = ix3(p, p.i, p.j, 0);
-  #pragma unroll
-  for (int k = 0; k < KB; ++k) a.rho_out[c + k] = (a.rho[c + k] - a.rmean[c + k]) + a.rmean[c + k];
   T ox[KB], oy[KB];
+  // both columns loaded before the first store (rho_out may alias rho)
+  #pragma unroll
+  for (int k = 0; k < KB; ++k) {
+    ox[k] = a.rho[c + k];
+    oy[k] = a.rmean[c + k];
+  }
+  #pragma unroll
+  for (int k = 0; k < KB; ++k) ox[k] = (ox[k] - oy[k]) + oy[k];
+  st_col(a.rho_out + c, ox);
   #pragma unroll
   for (int k = 0; k < KB; ++k) ox[k] = oy[k] = T(0);
   if (interior_pos(p)) {
@@ -372,11 +383,18 @@ __global__ void __launch_bounds__(TPB) k_colsum_adj(Batch B, Vert<T> vg, ColsumA
   Pos p;
   if (!pos2(B, p)) return;
   const int64_t c = ix3(p, p.i, p.j, 0);
+  T dz[KBM1];
+  #pragma unroll
+  for (int k = 0; k < KBM1; ++k) dz[k] = vg.dz(p.m, k);
   for (int n = 0; n < a.n; ++n) {
     if (!a.bout[n] || !a.bf[n]) continue;
+    // whole column loaded before the first store (the stores may alias the loads)
     const T g = a.bout[n][ix2(p, p.i, p.j)];
+    T x[KBM1];
     #pragma unroll
-    for (int k = 0; k < KBM1; ++k) a.bf[n][c + k] += g * vg.dz(p.m, k);
+    for (int k = 0; k < KBM1; ++k) x[k] = a.bf[n][c + k];
+    #pragma unroll
+    for (int k = 0; k < KBM1; ++k) a.bf[n][c + k] = x[k] + g * dz[k];
   }
 }
 
