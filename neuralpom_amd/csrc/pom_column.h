@@ -23,9 +23,14 @@ template <class T> __device__ __forceinline__ void st_col(T* base, const T* x) {
 #pragma unroll
   for (int k = 0; k < KB; ++k) base[k] = x[k];
 }
+// the whole column is read before the first store: otherwise every store may
+// alias the next load and the KB read-modify-writes run one memory latency each
 template <class T> __device__ __forceinline__ void add_col(T* base, const T* x) {
+  T y[KB];
 #pragma unroll
-  for (int k = 0; k < KB; ++k) base[k] += x[k];
+  for (int k = 0; k < KB; ++k) y[k] = base[k];
+#pragma unroll
+  for (int k = 0; k < KB; ++k) base[k] = y[k] + x[k];
 }
 
 // ---------------------------------------------------------------------

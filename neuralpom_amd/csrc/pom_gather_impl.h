@@ -8,8 +8,10 @@ namespace pom {
 // accumulate d(outputs)/d(inputs)^T obar into inbar[F] for every F whose
 // inbar pointer is non-null
 template <class T, class Op>
-void launch_gather(const Ctx<T>& c, const FieldTab<T>& tab, const COutTab<T>& obar, T* const* inbar) {
+void launch_gather(const Ctx<T>& c, const FieldTab<T>& tab, const COutTab<T>& obar, T* const* inbar,
+                   uint32_t assign) {
   InTab<T> it{};
+  it.assign = assign;
   bool any = false;
   for (int f = 0; f < Op::NIN; ++f) {
     it.o[f] = inbar[f];
@@ -34,6 +36,6 @@ void launch_gather(const Ctx<T>& c, const FieldTab<T>& tab, const COutTab<T>& ob
 
 #define POM_INSTANTIATE_GATHER(OP)                                                                      \
   template void pom::launch_gather<float, pom::OP>(const pom::Ctx<float>&, const pom::FieldTab<float>&, \
-                                                   const pom::COutTab<float>&, float* const*);          \
+                                                   const pom::COutTab<float>&, float* const*, uint32_t); \
   template void pom::launch_gather<double, pom::OP>(const pom::Ctx<double>&, const pom::FieldTab<double>&, \
-                                                    const pom::COutTab<double>&, double* const*);
+                                                    const pom::COutTab<double>&, double* const*, uint32_t);

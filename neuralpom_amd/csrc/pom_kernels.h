@@ -44,6 +44,60 @@ __global__ void __launch_bounds__(TPB) k_map(Batch B, Vert<T> vert, const T* ram
   });
 }
 
+// ---- running max of |x| over a launch (the depth-mean velocity check).  NaN
+// propagates, as torch.max does: fabs(NaN) is a positive NaN, whose bits
+// exceed every finite value's, so the unsigned atomic max keeps it.
+__device__ __forceinline__ void amax(float* f, float v) { atomicMax(reinterpret_cast<unsigned int*>(f), __float_as_uint(v)); }
+__device__ __forceinline__ void amax(double* f, double v) {
+  atomicMax(reinterpret_cast<unsigned long long*>(f), (unsigned long long)__double_as_longlong(v));
+}
+template <class T> __device__ __forceinline__ T nanmax(T a, T b) { return (a != a || a > b) ? a : b; }
+// block reduction of per-thread |x| values (TPB threads), one atomic per block
+template <class T> __device__ __forceinline__ void block_amax(T v, T* out) {
+  __shared__ T red[TPB / 64];
+  for (int o = 32; o > 0; o >>= 1) v = nanmax(v, __shfl_xor(v, o));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    v = red[0];
+#pragma unroll
+    for (int w = 1; w < TPB / 64; ++w) v = nanmax(v, red[w]);
+    if (!(v <= T(0))) amax(out, v);
+  }
+}
+
+// Two 2-D operators in one pass over the points: A's outputs at p are stored,
+// then B's, where B may read A's outputs at p itself (the same thread wrote
+// them) and nothing of A's at other points.  With amax non-null, the running
+// max of |A output ia| is accumulated into *amax.
+template <class T, class OpA, class OpB>
+__global__ void __launch_bounds__(TPB) k_map2(Batch B, Vert<T> vert, const T* ramp, Params prm, FieldTab<T> ta,
+                                               OutTab<T> oa, FieldTab<T> tb, OutTab<T> ob, int ia, T* amax) {
+  static_assert(!OpA::OUT3 && !OpB::OUT3, "2-D operators only");
+  Pos p;
+  const bool live = pos2(B, p);
+  T vmax = T(0);
+  if (live) {
+    const int64_t idx = ix2(p, p.i, p.j);
+    Acc<T> A{&ta, p, vert, ramp};
+    static_for<0, OpA::NOUT - 1>([&](auto O) {
+      constexpr int o = decltype(O)::value;
+      const T x = val(OpA::template eval<o>(A, prm));
+      if (oa.o[o]) oa.o[o][idx] = x;
+      if (o == ia) vmax = fabs(x);
+    });
+    Acc<T> Bc{&tb, p, vert, ramp};
+    {
+      const auto& A = Bc;   // the point functions name their accessor A
+      static_for<0, OpB::NOUT - 1>([&](auto O) {
+        constexpr int o = decltype(O)::value;
+        if (ob.o[o]) ob.o[o][idx] = val(OpB::template eval<o>(A, prm));
+      });
+    }
+  }
+  if (amax) block_amax(vmax, amax);   // uniform over the launch
+}
+
 // partial contribution of outputs at p = q - (DI, DJ, DK) to input F at q
 template <class T, class Op, int F, int DI, int DJ, int DK>
 __device__ __forceinline__ void gather_term(const FieldTab<T>& tab, const COutTab<T>& obar, const Pos& q, int pk,
@@ -132,7 +186,16 @@ __global__ void __launch_bounds__(TPB) k_gather2(Batch B, Vert<T> vert, const T*
 // the point function is evaluated once with every field seeded there (DS
 // tangents with one lane per field), and the lanes go to per-field
 // accumulators.  Same sums, in the same order, as one gather per field.
-template <class T> struct InTab { T* o[MAXF]; };
+// input cotangents: o[f] += partial sums, or o[f] = when bit f of assign is set
+// (the gather is then the first writer of that buffer and it need not be zeroed)
+template <class T> struct InTab {
+  T* o[MAXF];
+  uint32_t assign;
+};
+template <class T> __device__ __forceinline__ void put_in(const InTab<T>& t, int f, int64_t i, T v) {
+  if ((t.assign >> f) & 1u) t.o[f][i] = v;
+  else t.o[f][i] += v;
+}
 
 template <class T, class Op, int DI, int DJ, int DK, uint32_t MASK>
 __device__ __forceinline__ void gather_all_term(const FieldTab<T>& tab, const COutTab<T>& obar, const Pos& q, int pk,
@@ -191,12 +254,12 @@ __global__ void __launch_bounds__(TPB) k_gather_all3(Batch B, Vert<T> vert, cons
   static_for<0, Op::NIN - 1>([&](auto Fc) {
     constexpr int f = decltype(Fc)::value;
     if constexpr ((M3 >> f) & 1u) {
-      if (live && inbar.o[f]) inbar.o[f][ix3(q, q.i, q.j, q.k)] += acc[f];
+      if (live && inbar.o[f]) put_in(inbar, f, ix3(q, q.i, q.j, q.k), acc[f]);
     } else if constexpr ((M2 >> f) & 1u) {
       T v = acc[f];
 #pragma unroll
       for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o, 16);
-      if (live && q.k == 0 && inbar.o[f]) inbar.o[f][ix2(q, q.i, q.j)] += v;
+      if (live && q.k == 0 && inbar.o[f]) put_in(inbar, f, ix2(q, q.i, q.j), v);
     }
   });
 }
@@ -220,7 +283,7 @@ __global__ void __launch_bounds__(TPB) k_gather_all2(Batch B, Vert<T> vert, cons
   });
   static_for<0, Op::NIN - 1>([&](auto Fc) {
     constexpr int f = decltype(Fc)::value;
-    if (inbar.o[f]) inbar.o[f][ix2(q, q.i, q.j)] += acc[f];
+    if (inbar.o[f]) put_in(inbar, f, ix2(q, q.i, q.j), acc[f]);
   });
 }
 
@@ -232,10 +295,19 @@ inline void launch_map(const Ctx<T>& c, const FieldTab<T>& tab, const OutTab<T>&
   hipLaunchKernelGGL((k_map<T, Op>), dim3(nb), dim3(TPB), 0, c.stream, c.B, c.vert, c.ramp, c.prm, tab, out);
 }
 
+template <class T, class OpA, class OpB>
+inline void launch_map2(const Ctx<T>& c, const FieldTab<T>& ta, const OutTab<T>& oa, const FieldTab<T>& tb,
+                        const OutTab<T>& ob, int ia, T* amax) {
+  if (c.nblk2 == 0) return;
+  hipLaunchKernelGGL((k_map2<T, OpA, OpB>), dim3(c.nblk2), dim3(TPB), 0, c.stream, c.B, c.vert, c.ramp, c.prm, ta, oa,
+                     tb, ob, ia, amax);
+}
+
 // accumulate d(outputs)/d(inputs)^T obar into inbar[F] for every F whose
 // inbar pointer is non-null.  Defined in pom_gather_impl.h and instantiated
 // once per operator in gather_<op>.hip (parallel compilation units).
 template <class T, class Op>
-void launch_gather(const Ctx<T>& c, const FieldTab<T>& tab, const COutTab<T>& obar, T* const* inbar);
+void launch_gather(const Ctx<T>& c, const FieldTab<T>& tab, const COutTab<T>& obar, T* const* inbar,
+                   uint32_t assign = 0u);
 
 }  // namespace pom

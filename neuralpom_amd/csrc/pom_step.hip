@@ -116,24 +116,13 @@ __global__ void k_hplus(T* __restrict__ d, const T* __restrict__ h, const T* __r
   if (i < n) d[i] = h[i] + e[i];
 }
 // running max |vaf| (time_loop.py:393)
-__device__ __forceinline__ void amax(float* f, float v) { atomicMax(reinterpret_cast<unsigned int*>(f), __float_as_uint(v)); }
-__device__ __forceinline__ void amax(double* f, double v) {
-  atomicMax(reinterpret_cast<unsigned long long*>(f), (unsigned long long)__double_as_longlong(v));
-}
-// grid-stride max over |vaf|, wavefront shuffles, one atomic per block
+// grid-stride max over |vaf| (NaN propagates), one atomic per block
 template <class T>
-__global__ void __launch_bounds__(256) k_vamax(const T* __restrict__ vaf, int64_t n, T* out) {
-  __shared__ T red[4];
+__global__ void __launch_bounds__(TPB) k_vamax(const T* __restrict__ vaf, int64_t n, T* out) {
   T v = T(0);
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    v = fmax(v, fabs(vaf[i]));
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    v = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-    if (v > T(0)) amax(out, v);
-  }
+    v = nanmax(v, fabs(vaf[i]));
+  block_amax(v, out);
 }
 
 inline int blocks(int64_t n) { return (int)((n + 255) / 256); }
@@ -880,16 +869,12 @@ template <class T> struct Step {
         launch_map<T, OpAdvave>(cn, t_advave(xp), OutTab<T>{{xn.advua, xn.advva, xn.fluxua, xn.fluxva, xn.tps}});
       }
       resolve(n, xn);
-      {
+      {   // depth-mean momentum, then its filter / shift / time means at the same point, and max |vaf|
         ProfScope ps(POM_STAGE_EXTUV, c.stream);
-        launch_map<T, OpExtUV>(cn, t_uv(xp, xn), OutTab<T>{{xn.uaf, xn.vaf}});
+        launch_map2<T, OpExtUV, OpExtFilter>(
+            cn, t_uv(xp, xn), OutTab<T>{{xn.uaf, xn.vaf}}, t_filter(xp, xn),
+            OutTab<T>{{xn.ua, xn.uab, xn.va, xn.vab, xn.el, xn.elb, xn.d, xn.egf, xn.utf, xn.vtf, xn.etf}}, 1, vamax);
       }
-      if (vamax)
-        hipLaunchKernelGGL(k_vamax<T>, dim3(std::min(blocks(N2), 1024)), dim3(256), 0, c.stream, xn.vaf, N2,
-                           vamax);
-      launch_map<T, OpExtFilter>(cn, t_filter(xp, xn),
-                                 OutTab<T>{{xn.ua, xn.uab, xn.va, xn.vab, xn.el, xn.elb, xn.d, xn.egf, xn.utf, xn.vtf,
-                                            xn.etf}});
     }
     const X<T>& xl = xs[isplit];
     copy(out[POM_ADVUA], xl.advua, N2);
@@ -1095,17 +1080,27 @@ template <class T> struct Adj {
       for (int n = S.isplit; n >= 1; --n) {
         X<T>& bn = bx[n & 1];
         X<T>& bp = bx[(n - 1) & 1];
-        zero_x(bp, N2);
+        // No zeroing of bp: the first gather to write each of its buffers in
+        // this substep assigns instead of accumulating (ExtFilter: ua uab va
+        // vab el elb egf utf vtf etf; ExtUV: d uaf vaf), bp.elf is assigned
+        // by the next substep's ExtFilter (first = n - 1 < isplit), and
+        // advua / advva are copied or zeroed below.
         const X<T>& xp = n == 1 ? S.x0_ : S.xs[n - 1];
         const X<T>& xn = S.xs[n];
         Ctx<T> cn = c;
         cn.prm.iext = n;
+        const bool first_elf = n < S.isplit;   // bn.elf not written yet in this pass
         // filter: outputs (ua uab va vab el elb d egf utf vtf etf)
         {
           COutTab<T> ob{{bn.ua, bn.uab, bn.va, bn.vab, bn.el, bn.elb, bn.d, bn.egf, bn.utf, bn.vtf, bn.etf}};
           T* ib[OpExtFilter::NIN] = {bp.ua, bp.uab, bp.va, bp.vab, bp.el, bp.elb, bn.uaf, bn.vaf, bn.elf,
                                      bp.egf, bp.utf, bp.vtf, bp.etf};
-          launch_gather<T, OpExtFilter>(cn, S.t_filter(xp, xn), ob, ib);
+          uint32_t as = 0u;
+          for (int f : {OpExtFilter::UA, OpExtFilter::UAB, OpExtFilter::VA, OpExtFilter::VAB, OpExtFilter::EL,
+                        OpExtFilter::ELB, OpExtFilter::EGF, OpExtFilter::UTF, OpExtFilter::VTF, OpExtFilter::ETF})
+            as |= 1u << f;
+          if (first_elf) as |= 1u << OpExtFilter::ELF;
+          launch_gather<T, OpExtFilter>(cn, S.t_filter(xp, xn), ob, ib, as);
         }
         // depth-mean momentum
         {
@@ -1113,15 +1108,19 @@ template <class T> struct Adj {
           T* ib[OpExtUV::NIN] = {bp.el, bp.elb, bn.elf, bp.d, bp.ua, bp.va, bp.uab, bp.vab, bn.advua, bn.advva,
                                  bo[POM_ADX2D], bo[POM_ADY2D], bo[POM_DRX2D], bo[POM_DRY2D], bi[POM_WUBOT],
                                  bi[POM_WVBOT], bp.uaf, bp.vaf};
-          launch_gather<T, OpExtUV>(cn, S.t_uv(xp, xn), ob, ib);
+          const uint32_t as = (1u << OpExtUV::D_) | (1u << OpExtUV::UAFIN) | (1u << OpExtUV::VAFIN);
+          launch_gather<T, OpExtUV>(cn, S.t_uv(xp, xn), ob, ib, as);
         }
         if (n % c.prm.ispadv == 0) {
+          zq(bp.advua, N2);
+          zq(bp.advva, N2);
+          zflush();
           COutTab<T> ob{{bn.advua, bn.advva, nullptr, nullptr, nullptr}};
           T* ib[OpAdvave::NIN] = {bp.ua, bp.va, bp.uab, bp.vab, bp.d, bo[POM_AAM2D]};
           launch_gather<T, OpAdvave>(cn, S.t_advave(xp), ob, ib);
         } else {
-          add(bp.advua, bn.advua, N2);
-          add(bp.advva, bn.advva, N2);
+          S.copy(bp.advua, bn.advua, N2);
+          S.copy(bp.advva, bn.advva, N2);
         }
         {
           zero(s2d, N2);
@@ -1134,6 +1133,9 @@ template <class T> struct Adj {
         }
       }
       X<T>& b0 = bx[0];
+      // no substep reads the entering elf, and nothing assigned b0.elf
+      zq(b0.elf, N2);
+      zflush();
       // X_0 -> inputs and pre-loop intermediates
       add(bi[POM_UA], b0.ua, N2); add(bi[POM_UAB], b0.uab, N2); add(bi[POM_VA], b0.va, N2);
       add(bi[POM_VAB], b0.vab, N2); add(bi[POM_EL], b0.el, N2); add(bi[POM_ELB], b0.elb, N2);
