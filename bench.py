@@ -26,17 +26,22 @@ B_FWD = 164.0    # algorithmic bytes per cell-step, forward (fp32)
 B_FA = 468.0     # algorithmic bytes per cell-step, forward + adjoint with recompute (fp32)
 HBM_PEAK = 8.0e12
 
-# dominant kernel of the fwd+adjoint step (rocprof kernel-trace, profiles/) and
-# its compulsory HBM bytes per water-column cell: reads q2f q2lf q2 q2b q2lb km
-# kh kq u v t s rho (13 x 4 B), writes q2 q2l l kq km kh (6 x 4 B), plus 7 2-D
-# fields (wusurf wvsurf wubot wvbot etf dt h) shared by the 16 levels
-DOMINANT = "profq"
-DOMINANT_KERNEL = "pom::k_profq<float>"
-DOMINANT_B_PER_CELL = (13 + 6) * 4.0 + 7 * 4.0 / 16.0
-# HBM bytes per k_profq launch on this workload from rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE (KiB per dispatch), each scaled by what the counter reads for a 256 MiB
-# read + write elementwise kernel in the same run (profiles/k_profq_seamount64.md)
-DOMINANT_TRAFFIC = (436837.6 * 2.000 + 329715.4 * 1.000) * 1024.0
+# Dominant kernel of the fwd+adjoint step by total time (rocprof kernel-trace,
+# profiles/): the fused external substep k_map2<OpExtUV, OpExtFilter>, 30
+# launches per forward pass and two forward passes per fwd+adjoint step.  Its
+# compulsory bytes per 2-D point: reads 22 state fields (el elb elf d ua va uab
+# vab advua advva adx2d ady2d drx2d dry2d wubot wvbot uaf vaf of the previous
+# substep, egf utf vtf etf) and 12 geometry fields (cor dx dy h aru arv dum dvm
+# e_atmos wusurf wvsurf fsm), writes 13 (uaf vaf ua uab va vab el elb d egf utf
+# vtf etf), 4 B each.
+DOMINANT = "extuv"
+DOMINANT_KERNEL = "pom::k_map2<float, pom::OpExtUV, pom::OpExtFilter, 6, 7>"
+DOMINANT_B_PER_POINT = (22 + 12 + 13) * 4.0
+# HBM bytes per launch on this workload from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (KiB per dispatch), each scaled by what the counter reads for a
+# 256 MiB read + write elementwise kernel in the same run
+# (profiles/external_substep_seamount64.md)
+DOMINANT_TRAFFIC = (73664.3 * 2.000 + 40332.6 * 1.000) * 1024.0
 DTE = 6.0        # external step of the 8 km seamount grid (barotropic CFL); dti = 180 s
 SHAPES = [(90, 82)] * 32 + [(120, 110)] * 24 + [(177, 161)] * 8
 DIFF_OUT = ("u", "ub", "v", "vb", "t", "tb", "s", "sb", "el", "elb", "ua", "uab", "va", "vab",
@@ -254,7 +259,7 @@ def main():
     rate_fwd = res["cells"] * args.steps / res["seconds_fwd"]
     achieved = rate * B_FA
     kern_avg_s = res["kern_ms"] / max(res["kern_n"], 1) / 1e3
-    kern_bytes = DOMINANT_B_PER_CELL * res["cells"]
+    kern_bytes = DOMINANT_B_PER_POINT * (res["cells"] // 16)
     kern_bw = kern_bytes / kern_avg_s
     line = {
         "metric": "fwd+adjoint grid-cell-steps/s vs autograd baseline, same GPU, <=1e-5 rel err",

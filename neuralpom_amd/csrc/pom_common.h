@@ -395,6 +395,8 @@ __device__ __forceinline__ bool pos2(const Batch& B, Pos& p) {
   return true;
 }
 
+__device__ __forceinline__ int clampi(int x, int n) { return x < 0 ? 0 : (x >= n ? n - 1 : x); }
+
 __device__ __forceinline__ bool interior_pos(const Pos& p) {
   return p.i >= 1 && p.i <= p.im - 2 && p.j >= 1 && p.j <= p.jm - 2;
 }
@@ -418,12 +420,16 @@ struct Params {
   int32_t core;                                    // 0 explicit, 1 semi-implicit
 };
 
+// Levels are clamped into [0, KB): operators evaluate both sides of their
+// level conditions (IFZ / BR select) and the discarded side may ask for k - 1
+// at the surface or k + 1 at the bottom.
 template <class T> struct Vert {
   const T* vg;   // [M][4][KB]
-  __device__ __forceinline__ T z(int m, int k) const { return vg[(m * 4 + 0) * KB + k]; }
-  __device__ __forceinline__ T zz(int m, int k) const { return vg[(m * 4 + 1) * KB + k]; }
-  __device__ __forceinline__ T dz(int m, int k) const { return vg[(m * 4 + 2) * KB + k]; }
-  __device__ __forceinline__ T dzz(int m, int k) const { return vg[(m * 4 + 3) * KB + k]; }
+  __device__ __forceinline__ static int kc(int k) { return k < 0 ? 0 : (k > KB - 1 ? KB - 1 : k); }
+  __device__ __forceinline__ T z(int m, int k) const { return vg[(m * 4 + 0) * KB + kc(k)]; }
+  __device__ __forceinline__ T zz(int m, int k) const { return vg[(m * 4 + 1) * KB + kc(k)]; }
+  __device__ __forceinline__ T dz(int m, int k) const { return vg[(m * 4 + 2) * KB + kc(k)]; }
+  __device__ __forceinline__ T dzz(int m, int k) const { return vg[(m * 4 + 3) * KB + kc(k)]; }
 };
 
 // ------------------------------------------------------------------ accessors
@@ -473,6 +479,23 @@ struct Acc {
   __device__ __forceinline__ T dz(int k) const { return vert.dz(p.m, k); }
   __device__ __forceinline__ T dzz(int k) const { return vert.dzz(p.m, k); }
   __device__ __forceinline__ T rampv() const { return ramp[p.m]; }
+};
+
+// Forward accessor whose fields F0 and F1 at the evaluation point itself are
+// given values (the outputs of an operator evaluated just before at the same
+// point); every other access reads memory as Acc does.
+template <class T, int F0, int F1>
+struct AccSub : Acc<T> {
+  T v0, v1;
+  template <int F, int DI, int DJ>
+  __device__ __forceinline__ P<T> at2() const {
+    if constexpr (F == F0 || F == F1) {
+      static_assert(DI == 0 && DJ == 0, "a substituted field is read at the evaluation point only");
+      return P<T>{F == F0 ? v0 : v1};
+    } else {
+      return Acc<T>::template at2<F, DI, DJ>();
+    }
+  }
 };
 
 // Clamped neighbour indices around a gather thread's point q: every access of
@@ -587,18 +610,33 @@ template <int A, int B, class F> inline void static_for_host(F&& f) {
 }
 
 // shorthand inside point functions (A is the accessor object)
-// value of f() if cond, else zero of the same activity type; only the taken
-// side is evaluated
+// value of f() if cond, else zero of the same activity type.  For passive
+// values (the forward maps) both sides are evaluated and the result
+// selected: every accessor clamps its indices, so the unselected side only
+// reads valid memory (its value, possibly inf or NaN at a boundary, is
+// discarded), and without a branch the compiler issues all loads of an
+// operator together.  Active values (the adjoint gathers) branch, since
+// their tangent lanes make the unselected side expensive.
 template <class F> __device__ __forceinline__ auto ifz(bool cond, F f) {
   using R = decltype(f());
-  R r{};
-  if (cond) r = f();
-  return r;
+  if constexpr (active_v<R>) {
+    R r{};
+    if (cond) r = f();
+    return r;
+  } else {
+    const R v = f();
+    return cond ? v : R{};
+  }
 }
 template <class FA, class FB> __device__ __forceinline__ auto br(bool cond, FA fa, FB fb) {
   using R = common_t<decltype(fa()), decltype(fb())>;
-  if (cond) return cvt<R>(fa());
-  return cvt<R>(fb());
+  if constexpr (active_v<R>) {
+    if (cond) return cvt<R>(fa());
+    return cvt<R>(fb());
+  } else {
+    const R a = cvt<R>(fa()), b = cvt<R>(fb());
+    return cond ? a : b;
+  }
 }
 #define IFZ(cond, ...) ::pom::ifz((cond), [&]() { return (__VA_ARGS__); })
 // arguments holding template commas must be parenthesised

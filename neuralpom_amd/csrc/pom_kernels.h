@@ -38,9 +38,17 @@ __global__ void __launch_bounds__(TPB) k_map(Batch B, Vert<T> vert, const T* ram
   }
   Acc<T> A{&tab, p, vert, ramp};
   const int64_t idx = Op::OUT3 ? ix3(p, p.i, p.j, p.k) : ix2(p, p.i, p.j);
+  // every output evaluated before the first store: a store may alias the
+  // inputs, so loads after it could neither be shared between outputs nor
+  // issued early
+  T res[Op::NOUT];
   static_for<0, Op::NOUT - 1>([&](auto O) {
     constexpr int o = decltype(O)::value;
-    if (out.o[o]) out.o[o][idx] = val(Op::template eval<o>(A, prm));
+    res[o] = val(Op::template eval<o>(A, prm));
+  });
+  static_for<0, Op::NOUT - 1>([&](auto O) {
+    constexpr int o = decltype(O)::value;
+    if (out.o[o]) out.o[o][idx] = res[o];
   });
 }
 
@@ -66,34 +74,40 @@ template <class T> __device__ __forceinline__ void block_amax(T v, T* out) {
   }
 }
 
-// Two 2-D operators in one pass over the points: A's outputs at p are stored,
-// then B's, where B may read A's outputs at p itself (the same thread wrote
-// them) and nothing of A's at other points.  With amax non-null, the running
-// max of |A output ia| is accumulated into *amax.
-template <class T, class OpA, class OpB>
+// Two 2-D operators in one pass over the points: B reads A's outputs 0 and 1
+// at the point itself as its fields FB0 and FB1 (from registers, through
+// AccSub) and nothing of A's at other points.  All outputs are evaluated
+// before the first store.  With amax non-null, the running max of |A output
+// 1| is accumulated into *amax.
+template <class T, class OpA, class OpB, int FB0, int FB1>
 __global__ void __launch_bounds__(TPB) k_map2(Batch B, Vert<T> vert, const T* ramp, Params prm, FieldTab<T> ta,
-                                               OutTab<T> oa, FieldTab<T> tb, OutTab<T> ob, int ia, T* amax) {
-  static_assert(!OpA::OUT3 && !OpB::OUT3, "2-D operators only");
+                                               OutTab<T> oa, FieldTab<T> tb, OutTab<T> ob, T* amax) {
+  static_assert(!OpA::OUT3 && !OpB::OUT3 && OpA::NOUT == 2, "two 2-D outputs feeding a 2-D operator");
   Pos p;
   const bool live = pos2(B, p);
   T vmax = T(0);
   if (live) {
     const int64_t idx = ix2(p, p.i, p.j);
-    Acc<T> A{&ta, p, vert, ramp};
-    static_for<0, OpA::NOUT - 1>([&](auto O) {
-      constexpr int o = decltype(O)::value;
-      const T x = val(OpA::template eval<o>(A, prm));
-      if (oa.o[o]) oa.o[o][idx] = x;
-      if (o == ia) vmax = fabs(x);
-    });
-    Acc<T> Bc{&tb, p, vert, ramp};
+    T ra[2], rb[OpB::NOUT];
     {
-      const auto& A = Bc;   // the point functions name their accessor A
+      Acc<T> A{&ta, p, vert, ramp};
+      ra[0] = val(OpA::template eval<0>(A, prm));
+      ra[1] = val(OpA::template eval<1>(A, prm));
+    }
+    {
+      AccSub<T, FB0, FB1> A{{&tb, p, vert, ramp}, ra[0], ra[1]};
       static_for<0, OpB::NOUT - 1>([&](auto O) {
         constexpr int o = decltype(O)::value;
-        if (ob.o[o]) ob.o[o][idx] = val(OpB::template eval<o>(A, prm));
+        rb[o] = val(OpB::template eval<o>(A, prm));
       });
     }
+    vmax = fabs(ra[1]);
+    if (oa.o[0]) oa.o[0][idx] = ra[0];
+    if (oa.o[1]) oa.o[1][idx] = ra[1];
+    static_for<0, OpB::NOUT - 1>([&](auto O) {
+      constexpr int o = decltype(O)::value;
+      if (ob.o[o]) ob.o[o][idx] = rb[o];
+    });
   }
   if (amax) block_amax(vmax, amax);   // uniform over the launch
 }
@@ -206,9 +220,19 @@ __device__ __forceinline__ void gather_all_term(const FieldTab<T>& tab, const CO
     p.i = q.i - DI;
     p.j = q.j - DJ;
     p.k = pk;
-    if (p.i < 0 || p.i >= p.im || p.j < 0 || p.j >= p.jm || p.k < 0 || p.k >= KB) return;
+    // Operators without a vertical stencil are evaluated by every thread and
+    // the term discarded (select, not multiply: the discarded value may be
+    // inf or NaN) where the evaluation point lies outside the member: all
+    // reads are clamped into the member, and without a branch per offset the
+    // loads of successive offsets overlap.  With a vertical stencil the out-of-
+    // column evaluations are a larger share and are skipped (measured faster).
+    const bool valid = p.i >= 0 && p.i < p.im && p.j >= 0 && p.j < p.jm && p.k >= 0 && p.k < KB;
+    if constexpr (Op::RK > 0) {
+      if (!valid) return;
+    }
     AccT<T, MASK, DI, DJ, DK> A{&tab, p, vert, ramp, &nb};
-    const int64_t idx = Op::OUT3 ? ix3(p, p.i, p.j, p.k) : ix2(p, p.i, p.j);
+    const int ci = clampi(p.i, p.im), cj = clampi(p.j, p.jm);
+    const int64_t idx = Op::OUT3 ? ix3(p, ci, cj, clampi(p.k, KB)) : ix2(p, ci, cj);
     static_for<0, Op::NOUT - 1>([&](auto O) {
       constexpr int o = decltype(O)::value;
       using R = decltype(Op::template eval<o>(A, prm));
@@ -219,7 +243,8 @@ __device__ __forceinline__ void gather_all_term(const FieldTab<T>& tab, const CO
           const R r = Op::template eval<o>(A, prm);
           for_bits<MR>([&](auto I) {
             constexpr int f = decltype(I)::value;
-            acc[f] += ob * r.d[slot(MR, f)];
+            const T t = ob * r.d[slot(MR, f)];
+            acc[f] += valid ? t : T(0);
           });
         }
       }
@@ -295,12 +320,12 @@ inline void launch_map(const Ctx<T>& c, const FieldTab<T>& tab, const OutTab<T>&
   hipLaunchKernelGGL((k_map<T, Op>), dim3(nb), dim3(TPB), 0, c.stream, c.B, c.vert, c.ramp, c.prm, tab, out);
 }
 
-template <class T, class OpA, class OpB>
+template <class T, class OpA, class OpB, int FB0, int FB1>
 inline void launch_map2(const Ctx<T>& c, const FieldTab<T>& ta, const OutTab<T>& oa, const FieldTab<T>& tb,
-                        const OutTab<T>& ob, int ia, T* amax) {
+                        const OutTab<T>& ob, T* amax) {
   if (c.nblk2 == 0) return;
-  hipLaunchKernelGGL((k_map2<T, OpA, OpB>), dim3(c.nblk2), dim3(TPB), 0, c.stream, c.B, c.vert, c.ramp, c.prm, ta, oa,
-                     tb, ob, ia, amax);
+  hipLaunchKernelGGL((k_map2<T, OpA, OpB, FB0, FB1>), dim3(c.nblk2), dim3(TPB), 0, c.stream, c.B, c.vert, c.ramp,
+                     c.prm, ta, oa, tb, ob, amax);
 }
 
 // accumulate d(outputs)/d(inputs)^T obar into inbar[F] for every F whose

@@ -642,11 +642,11 @@ struct OpBc1 {
     const int dj = (j == 0) ? 1 : (j == jm - 1 ? -1 : 0);
     using R = decltype(AT2(RAW, 0, 0) + AT2(RAW, 1, 1) + AT2(RAW, -1, -1) + AT2(RAW, 1, -1) + AT2(RAW, -1, 1) +
                        AT2(RAW, 1, 0) + AT2(RAW, -1, 0) + AT2(RAW, 0, 1) + AT2(RAW, 0, -1));
+    // the nine candidates are plain reads: select instead of branching
     R r{};
-#define POM_BC1_CASE(a, b) if (di == a && dj == b) r = cvt<R>(AT2(RAW, a, b));
-    POM_BC1_CASE(0, 0) else POM_BC1_CASE(1, 0) else POM_BC1_CASE(-1, 0) else POM_BC1_CASE(0, 1)
-    else POM_BC1_CASE(0, -1) else POM_BC1_CASE(1, 1) else POM_BC1_CASE(1, -1) else POM_BC1_CASE(-1, 1)
-    else POM_BC1_CASE(-1, -1)
+#define POM_BC1_CASE(a, b) r = (di == a && dj == b) ? cvt<R>(AT2(RAW, a, b)) : r;
+    POM_BC1_CASE(-1, -1) POM_BC1_CASE(-1, 1) POM_BC1_CASE(1, -1) POM_BC1_CASE(1, 1) POM_BC1_CASE(0, -1)
+    POM_BC1_CASE(0, 1) POM_BC1_CASE(-1, 0) POM_BC1_CASE(1, 0) POM_BC1_CASE(0, 0)
 #undef POM_BC1_CASE
     return r * G2(GFSM, 0, 0);
   }
@@ -779,12 +779,23 @@ struct OpExtUV {
       using R = common_t<common_t<common_t<decltype(uaf_in(A, prm)), decltype(east())>,
                                   common_t<decltype(west1()), decltype(west0())>>,
                          decltype(AT2(UAFIN, 0, 0))>;
-      R r = cvt<R>(AT2(UAFIN, 0, 0));
-      if (jin && i == im - 1) r = cvt<R>(east());
-      else if (jin && i == 1) r = cvt<R>(west1());
-      else if (jin && i == 0) r = cvt<R>(west0());
-      else if (iin && (j == jm - 1 || j == 0)) r = R{};
-      else if (iin && jin) r = cvt<R>(uaf_in(A, prm));
+      R r{};
+      if constexpr (active_v<R>) {
+        r = cvt<R>(AT2(UAFIN, 0, 0));
+        if (jin && i == im - 1) r = cvt<R>(east());
+        else if (jin && i == 1) r = cvt<R>(west1());
+        else if (jin && i == 0) r = cvt<R>(west0());
+        else if (iin && (j == jm - 1 || j == 0)) r = R{};
+        else if (iin && jin) r = cvt<R>(uaf_in(A, prm));
+      } else {   // passive: every candidate evaluated (indices are clamped), then selected
+        const R c_in = cvt<R>(AT2(UAFIN, 0, 0)), c_e = cvt<R>(east()), c_w1 = cvt<R>(west1()),
+                c_w0 = cvt<R>(west0()), c_int = cvt<R>(uaf_in(A, prm));
+        r = (jin && i == im - 1) ? c_e
+          : (jin && i == 1) ? c_w1
+          : (jin && i == 0) ? c_w0
+          : (iin && (j == jm - 1 || j == 0)) ? R{}
+          : (iin && jin) ? c_int : c_in;
+      }
       return r * G2(GDUM, 0, 0);
     } else {
       auto north = [&]() { return (prm.rfn * cwave(G2(GH, 0, -1), prm.grav) * (AT2(EL, 0, -1) - 0.0)) * ramp; };
@@ -793,12 +804,23 @@ struct OpExtUV {
       using R = common_t<common_t<common_t<decltype(vaf_in(A, prm)), decltype(north())>,
                                   common_t<decltype(south1()), decltype(south0())>>,
                          decltype(AT2(VAFIN, 0, 0))>;
-      R r = cvt<R>(AT2(VAFIN, 0, 0));
-      if (jin && (i == im - 1 || i == 0)) r = R{};
-      else if (iin && j == jm - 1) r = cvt<R>(north());
-      else if (iin && j == 1) r = cvt<R>(south1());
-      else if (iin && j == 0) r = cvt<R>(south0());
-      else if (iin && jin) r = cvt<R>(vaf_in(A, prm));
+      R r{};
+      if constexpr (active_v<R>) {
+        r = cvt<R>(AT2(VAFIN, 0, 0));
+        if (jin && (i == im - 1 || i == 0)) r = R{};
+        else if (iin && j == jm - 1) r = cvt<R>(north());
+        else if (iin && j == 1) r = cvt<R>(south1());
+        else if (iin && j == 0) r = cvt<R>(south0());
+        else if (iin && jin) r = cvt<R>(vaf_in(A, prm));
+      } else {
+        const R c_in = cvt<R>(AT2(VAFIN, 0, 0)), c_n = cvt<R>(north()), c_s1 = cvt<R>(south1()),
+                c_s0 = cvt<R>(south0()), c_int = cvt<R>(vaf_in(A, prm));
+        r = (jin && (i == im - 1 || i == 0)) ? R{}
+          : (iin && j == jm - 1) ? c_n
+          : (iin && j == 1) ? c_s1
+          : (iin && j == 0) ? c_s0
+          : (iin && jin) ? c_int : c_in;
+      }
       return r * G2(GDVM, 0, 0);
     }
   }

@@ -871,9 +871,9 @@ template <class T> struct Step {
       resolve(n, xn);
       {   // depth-mean momentum, then its filter / shift / time means at the same point, and max |vaf|
         ProfScope ps(POM_STAGE_EXTUV, c.stream);
-        launch_map2<T, OpExtUV, OpExtFilter>(
+        launch_map2<T, OpExtUV, OpExtFilter, OpExtFilter::UAF, OpExtFilter::VAF>(
             cn, t_uv(xp, xn), OutTab<T>{{xn.uaf, xn.vaf}}, t_filter(xp, xn),
-            OutTab<T>{{xn.ua, xn.uab, xn.va, xn.vab, xn.el, xn.elb, xn.d, xn.egf, xn.utf, xn.vtf, xn.etf}}, 1, vamax);
+            OutTab<T>{{xn.ua, xn.uab, xn.va, xn.vab, xn.el, xn.elb, xn.d, xn.egf, xn.utf, xn.vtf, xn.etf}}, vamax);
       }
     }
     const X<T>& xl = xs[isplit];
