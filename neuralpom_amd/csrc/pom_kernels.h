@@ -252,6 +252,24 @@ __device__ __forceinline__ void gather_all_term(const FieldTab<T>& tab, const CO
   }
 }
 
+// Boundary operators read horizontal neighbours only at evaluation points
+// within EDGE_BAND of the member's edge; elsewhere they are pointwise in
+// (i, j).  An input point q farther than EDGE_BAND + R from every edge then
+// receives contributions from its own column only, and the gather skips the
+// other offsets there.
+template <class Op, class = void> struct edge_band { static constexpr int value = -1; };
+template <class Op> struct edge_band<Op, std::void_t<decltype(Op::EDGE_BAND)>> {
+  static constexpr int value = Op::EDGE_BAND;
+};
+template <class Op> __device__ __forceinline__ bool deep_point(const Pos& q) {
+  constexpr int EB = edge_band<Op>::value;
+  if constexpr (EB < 0) {
+    return false;
+  } else {
+    return q.i >= EB + Op::RI && q.i <= q.im - 1 - EB - Op::RI && q.j >= EB + Op::RJ && q.j <= q.jm - 1 - EB - Op::RJ;
+  }
+}
+
 // 3-D operators: one thread per 3-D point q; 3-D fields accumulate at q, 2-D
 // fields per level (offsets with dk = 0 only) and are reduced over the 16
 // lanes of the water column
@@ -266,12 +284,14 @@ __global__ void __launch_bounds__(TPB) k_gather_all3(Batch B, Vert<T> vert, cons
   if (live) {
     NbrIdx nb;
     make_nbr(q, nb);
+    const bool deep = deep_point<Op>(q);
     static_for<-Op::RI, Op::RI>([&](auto I) {
       static_for<-Op::RJ, Op::RJ>([&](auto J) {
         static_for<-Op::RK, Op::RK>([&](auto K) {
           constexpr int di = decltype(I)::value, dj = decltype(J)::value, dk = decltype(K)::value;
-          gather_all_term<T, Op, di, dj, dk, (M3 | (dk == 0 ? M2 : 0u))>(tab, obar, q, q.k - dk, vert, ramp, prm,
-                                                                         nb, acc);
+          if (!(edge_band<Op>::value >= 0 && (di != 0 || dj != 0) && deep))
+            gather_all_term<T, Op, di, dj, dk, (M3 | (dk == 0 ? M2 : 0u))>(tab, obar, q, q.k - dk, vert, ramp,
+                                                                           prm, nb, acc);
         });
       });
     });
@@ -300,10 +320,12 @@ __global__ void __launch_bounds__(TPB) k_gather_all2(Batch B, Vert<T> vert, cons
   for (int f = 0; f < Op::NIN; ++f) acc[f] = T(0);
   NbrIdx nb;
   make_nbr(q, nb);
+  const bool deep = deep_point<Op>(q);
   static_for<-Op::RI, Op::RI>([&](auto I) {
     static_for<-Op::RJ, Op::RJ>([&](auto J) {
       constexpr int di = decltype(I)::value, dj = decltype(J)::value;
-      gather_all_term<T, Op, di, dj, 0, M2>(tab, obar, q, 0, vert, ramp, prm, nb, acc);
+      if (!(edge_band<Op>::value >= 0 && (di != 0 || dj != 0) && deep))
+        gather_all_term<T, Op, di, dj, 0, M2>(tab, obar, q, 0, vert, ramp, prm, nb, acc);
     });
   });
   static_for<0, Op::NIN - 1>([&](auto Fc) {

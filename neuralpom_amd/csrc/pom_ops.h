@@ -370,6 +370,7 @@ struct OpBc3 {
   static constexpr int NOUT = 2;
   static constexpr bool OUT3 = true;
   static constexpr int RI = 2, RJ = 2, RK = 0;
+  static constexpr int EDGE_BAND = 2;   // only points within 2 of the edge read neighbours
   template <int F, int OI, class AC> __device__ static auto smj(const AC& A) {
     return 0.25 * AT(F, OI, -1, 0) + 0.50 * AT(F, OI, 0, 0) + 0.25 * AT(F, OI, 1, 0);
   }
@@ -446,6 +447,7 @@ struct OpBc4Filter {
   static constexpr int NOUT = 4;
   static constexpr bool OUT3 = true;
   static constexpr int RI = 1, RJ = 1, RK = 1;
+  static constexpr int EDGE_BAND = 1;   // only points within 1 of the edge read neighbours
 
   // edge value of tracer FC (TT or SS) for k < kbm1; side: 0 E, 1 W, 2 N, 3 S
   template <int FC, int SIDE, class AC> __device__ static auto edge(const AC& A, const Params& prm) {
@@ -523,6 +525,7 @@ struct OpBc6Filter {
   static constexpr int NOUT = 4;
   static constexpr bool OUT3 = true;
   static constexpr int RI = 1, RJ = 1, RK = 0;
+  static constexpr int EDGE_BAND = 1;   // only points within 1 of the edge read neighbours
   template <int FC, int SIDE, class AC> __device__ static auto edge(const AC& A, const Params& prm) {
     constexpr int II = (SIDE == 0) ? -1 : (SIDE == 1 ? 1 : 0);
     constexpr int JJ = (SIDE == 2) ? -1 : (SIDE == 3 ? 1 : 0);
@@ -580,6 +583,7 @@ struct OpKEdge {
   static constexpr int NOUT = 2;
   static constexpr bool OUT3 = true;
   static constexpr int RI = 2, RJ = 2, RK = 0;
+  static constexpr int EDGE_BAND = 1;   // only points within 1 of the edge read neighbours
   // value after the E/W copies at column (i, j+OJ)
   template <int F, int OJ, class AC> __device__ static auto ew(const AC& A) {
     const int i = A.p.i, im = A.p.im;
@@ -636,6 +640,7 @@ struct OpBc1 {
   static constexpr int NOUT = 1;
   static constexpr bool OUT3 = false;
   static constexpr int RI = 1, RJ = 1, RK = 0;
+  static constexpr int EDGE_BAND = 1;   // only points within 1 of the edge read neighbours
   template <int O, class AC> __device__ static auto eval(const AC& A, const Params&) {
     const int i = A.p.i, j = A.p.j, im = A.p.im, jm = A.p.jm;
     const int di = (i == 0) ? 1 : (i == im - 1 ? -1 : 0);

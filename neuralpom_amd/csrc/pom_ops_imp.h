@@ -93,6 +93,7 @@ struct OpImpBc4Filter {
   static constexpr int NOUT = 4;
   static constexpr bool OUT3 = true;
   static constexpr int RI = 1, RJ = 1, RK = 1;
+  static constexpr int EDGE_BAND = 1;   // only edge points read neighbours (edge_copy3)
   template <int O, class AC> __device__ static auto eval(const AC& A, const Params& prm) {
     constexpr int FNEW = (O < 2) ? TF : SF;
     constexpr int FC = (O < 2) ? TT : SS;

@@ -134,3 +134,49 @@ def test_adjoint_matches_reference_vjp(dtype, tol):
         if not (e <= tol):
             bad.append(f"{n}: {e:.2e}")
     assert not bad, "gradients off: " + ", ".join(bad)
+
+
+def test_adjoint_matches_oracle_on_a_larger_ragged_batch():
+    """VJP of one step for members large enough to have columns far from every
+    edge (the gathers of boundary operators skip off-column offsets there) and
+    sizes that are not multiples of the launch blocks, against torch autograd
+    through the oracle (itself pinned to the reference's VJP on 12 x 10)."""
+    from oracle.pomref.cases import double_gyre
+    shapes = ((26, 22), (19, 31))
+    states = []
+    for n, (im, jm) in enumerate(shapes):
+        S = double_gyre(im, jm)
+        gen = torch.Generator().manual_seed(40 + n)
+        for f, a in (("t", 0.5), ("el", 0.05), ("u", 0.02), ("v", 0.02), ("ua", 0.01), ("va", 0.01)):
+            x = getattr(S, f)
+            setattr(S, f, x + a * torch.randn(x.shape, generator=gen, dtype=x.dtype))
+        S.d = S.h + S.el
+        S.time0 = S.time = 0.25
+        states.append(S)
+    print(f"[gpu] explicit adjoint, members {shapes}, float64", flush=True)
+    ens, state, geom, vg = ensemble_inputs(states, torch.float64)
+    sc = scalars_of(states[0])
+    ramp = torch.tensor([ramp_of(S, 1) for S in states], dtype=torch.float64, device="cuda")
+    gen = torch.Generator().manual_seed(77)
+    cots = [{n: torch.randn(getattr(S, n).shape, generator=gen, dtype=torch.float64) for n in VJP_OUT}
+            for S in states]
+    cot = {n: ens.pack([c[n] for c in cots], n) for n in VJP_OUT}
+    grads = ens.step_adj(state, geom, sc, ramp, vg, True, cot)
+    torch.cuda.synchronize()
+    bad = []
+    for m, S in enumerate(states):
+        R = S.copy()
+        leaves = {n: getattr(R, n).clone().requires_grad_(True) for n in VJP_IN}
+        for n, v in leaves.items():
+            setattr(R, n, v)
+        R.iend = 1
+        assert X.run_time_loop(R)
+        loss = sum((getattr(R, n) * cots[m][n]).sum() for n in VJP_OUT)
+        ref = torch.autograd.grad(loss, list(leaves.values()), allow_unused=True)
+        for n, r in zip(VJP_IN, ref):
+            if r is None:
+                r = torch.zeros_like(leaves[n])
+            e = rel_err(unpack(ens, grads[n], n)[m].cpu(), r)
+            if not e <= 1e-8:
+                bad.append(f"{n}[m{m}]: {e:.2e}")
+    assert not bad, "gradients off: " + ", ".join(bad)
