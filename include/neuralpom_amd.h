@@ -213,6 +213,38 @@ int pom_baropg(const pom_batch* b, const pom_params_t* prm, const void* ramp, co
 int pom_baropg_adj(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vgrid,
                    const void* const* in, const void* const* cot_out, void* const* grad_in, void* stream);
 
+/* ---- State movement around the step (no physics).
+ *
+ * pom_pack_input: the corrector network's input, 101 channels per member in
+ *   NCHW (src/neuralpom/hybrid/state.py:181-226, pack order constants.py:2-7):
+ *   fields = {u, ub, v, vb, t, s (3-D, all 16 levels), el, ua, uab, va, vab};
+ *   member m's planes (101 x im x jm) start at element 101 * off2[m] of out.
+ *   mean / scale (each may be null as a whole; entries null together) are
+ *   per-cell statistics of one member's shape: channel = (x - mean) / scale,
+ *   as StateNormalizer.normalize (state.py:156-179); with statistics the
+ *   members must share (im, jm).
+ * pom_pack_input_adj: grad_fields[f] += d(pack)/d(field f)^T cot_out
+ *   (null entries skipped).
+ * pom_restrict4: volume-weighted conservative 4 x 4 restriction of a fine
+ *   batch onto a coarse one (member m to member m, coarse im = (im - 2)/4 + 2,
+ *   likewise jm): coarse interior cell (I, J) averages fine interior cells
+ *   4(I-1)+1 .. 4I with weights art*mask (volume_weighted = 0) or
+ *   art*mask*(h + el)[*dz(k) for 3-D fields] (volume_weighted = 1); the coarse
+ *   edge ring is 0.  The fine-to-coarse projection of the reference's training
+ *   data (paper/paper_zh.md:625), whose code is not in the reference.
+ * pom_copy_arrays: n device-to-device copies (sizes in bytes, multiples of 4)
+ *   in one launch: device-resident snapshots of a state for restart and the
+ *   per-step recompute, replacing the host round trip of capture / restore
+ *   (src/neuralpom/v1/checkpoint.py:51-95). */
+int pom_pack_input(const pom_batch* b, const void* const* fields, const void* const* mean,
+                   const void* const* scale, void* out, void* stream);
+int pom_pack_input_adj(const pom_batch* b, const void* const* scale, const void* cot_out, void* const* grad_fields,
+                       void* stream);
+int pom_restrict4(const pom_batch* fine, const pom_batch* coarse, int32_t three_d, int32_t volume_weighted,
+                  const void* field, const void* art, const void* mask, const void* h, const void* el,
+                  const void* fine_vgrid, void* out, void* stream);
+int pom_copy_arrays(int32_t n, void* const* dst, const void* const* src, const int64_t* bytes, void* stream);
+
 /* Tracing: record HIP events around every launch of one stage on its stream
  * (stage = -1 disables).  pom_profile_read waits for the recorded events and
  * returns the summed kernel time and the number of launches since the last

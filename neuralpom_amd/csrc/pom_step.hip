@@ -18,18 +18,9 @@
 #include "pom_pcg.h"
 #include "pom_profq.h"
 
-using namespace pom;
+#include "pom_batch.h"
 
-struct pom_batch {
-  int32_t dtype;
-  int32_t M;
-  Member* d_mem;
-  int32_t* d_blk3;
-  int32_t* d_blk2;
-  int32_t nblk3, nblk2;
-  int64_t n2, n3;
-  std::vector<Member> h_mem;   // host copy of the member table
-};
+using namespace pom;
 
 namespace {
 
