@@ -316,13 +316,16 @@ def bcond2(S, uaf, vaf, el, ramp):
     return uaf * S.dum, vaf * S.dvm
 
 
-def bcond3(S, uf, vf, u, v):
+def bcond3(S, uf, vf, u, v, depth=None):
+    """``depth`` replaces h in the edge weight ga = sqrt(depth / hmax): the
+    case-adapter kernel copy uses the total depth d
+    (case_adapters/kernels/coastal_explicit_baroclinic.py:586, 602, 627, 641)."""
     uf = uf.clone()
     vf = vf.clone()
     K = slice(0, S.kbm1)
     J, Jm, Jp = slice(1, S.jm - 1), slice(0, S.jm - 2), slice(2, S.jm)
     I, Im, Ip = slice(1, S.im - 1), slice(0, S.im - 2), slice(2, S.im)
-    h, hmax = S.h, S.hmax
+    h, hmax = (S.h if depth is None else depth), S.hmax
 
     def sm_j(f, i):
         return 0.25 * f[i, Jm, K] + 0.50 * f[i, J, K] + 0.25 * f[i, Jp, K]

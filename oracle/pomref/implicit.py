@@ -17,13 +17,17 @@ src/neuralpom/cores/implicit_p0_fixed/:
   forcing/surface.py:20-84      double-gyre wind and surface heat/salt fluxes
   numerics/double_gyre.py:78-191  tracer / velocity filters (Shapiro gamma 0 in production)
   autograd.py:6-29              strain magnitude with zero subgradient at zero strain
+and the open-boundary tide policy of the case adapter
+src/neuralpom/cores/case_adapters/coastal_boundary.py:9-67 (implicit family),
+installed with ``install_tide``.
 The 3-D operators (advct, baropg, advave, vertvl, advq, profq, advt1, proft,
 dens, advu/advv, profu/profv) are those of the explicit oracle; the implicit
 copies differ only in the profq wall guard (IM/tracers.py:360, 1e-8 instead of
 1e-20), selected here by ``S.wall_eps``.
 
 The restatement is pinned against vectors written from the reference itself
-(tests/golden/make_golden_implicit.py, tests/test_oracle_implicit.py).
+(tests/golden/make_golden_implicit.py, tests/test_oracle_implicit.py; the tide
+policy by tests/golden/make_golden_tide.py, tests/test_oracle_tide.py).
 """
 from __future__ import annotations
 
@@ -50,11 +54,127 @@ def _zero_edges(x):
 
 
 def bc1(S, elf):
+    if getattr(S, "tide", None) is not None:
+        return tide_bc1(S, elf)
     return _zero_edges(elf * S.fsm)
 
 
 def bc2(S, uaf, vaf):
+    if getattr(S, "tide", None) is not None:
+        return tide_bc2(S, uaf, vaf)
     return _zero_edges(uaf * S.dum), _zero_edges(vaf * S.dvm)
+
+
+# --------------------------------------------------------------------------
+# open boundaries with a prescribed tidal elevation at fixed nodes: the
+# implicit family of src/neuralpom/cores/case_adapters/coastal_boundary.py:9-64
+# (bcond 1 and 2 :11-30, the lifted area-weighted CG solve :41-58, node and
+# fixed-set construction :60-67).  bcond 3-6 are the explicit core's open
+# forms (case_adapters/kernels/coastal_explicit_baroclinic.py:496-928, the
+# edge weight of bcond 3 from d instead of h).
+# --------------------------------------------------------------------------
+TIDE_SPEED = 29.03225806451612   # degrees per hour (coastal_boundary.py:9)
+TIDE_PI = 3.1416                 # the adapter's pi (coastal_boundary.py:9)
+
+
+class Tide:
+    """Tide nodes (0-based i, j), amplitudes, phases (degrees) and the fixed
+    set of the eta solve: the outer ring, dry cells and the nodes
+    (coastal_boundary.py:60-67)."""
+
+    def __init__(self, S, i, j, amp, phase, speed=TIDE_SPEED, pi=TIDE_PI):
+        self.i = torch.as_tensor(i, dtype=torch.long)
+        self.j = torch.as_tensor(j, dtype=torch.long)
+        self.amp = torch.as_tensor(amp, dtype=S.el.dtype)
+        self.phase = torch.as_tensor(phase, dtype=S.el.dtype)
+        self.speed, self.pi = float(speed), float(pi)
+        fixed = torch.ones(S.fsm.shape, dtype=torch.bool)
+        fixed[1:-1, 1:-1] = False
+        fixed |= S.fsm == 0
+        fixed[self.i, self.j] = True
+        self.fixed = fixed
+
+
+def install_tide(S, i, j, amp, phase, **kw):
+    S.tide = Tide(S, i, j, amp, phase, **kw)
+    S.rfe = S.rfw = S.rfn = S.rfs = 1.0
+    return S
+
+
+def tide_value(S, seconds):
+    """coastal_boundary.py:9-10"""
+    T = S.tide
+    return T.amp * torch.cos((T.speed * T.pi / 180) * seconds / 3600 - T.phase * T.pi / 180)
+
+
+def tide_seconds(S):
+    """the implicit family's evaluation time (coastal_boundary.py:15, 36, 45)"""
+    return S.time * 86400 - S.dte
+
+
+def tide_bc1(S, elf):
+    """coastal_boundary.py:12-23 (implicit family)"""
+    elf = elf.clone()
+    elf[S.tide.i, S.tide.j] = tide_value(S, tide_seconds(S))
+    return elf * S.fsm
+
+
+def tide_bc2(S, uaf, vaf):
+    """radiation conditions on c.el and c.d (coastal_boundary.py:24-30)"""
+    uaf = uaf.clone()
+    vaf = vaf.clone()
+    J = I = slice(1, -1)
+    g, d, el = S.grav, S.d, S.el
+    uaf[-1, J] = uaf[-2, J] + S.rfe * torch.sqrt(g / d[-2, J]) * (el[-2, J] - el[-1, J])
+    vaf[-1, J] = 0
+    uaf[1, J] = uaf[1, J] - S.rfw * torch.sqrt(g / d[1, J]) * (el[1, J] - el[0, J])
+    uaf[0, J] = uaf[1, J]
+    vaf[0, J] = 0
+    vaf[I, -1] = vaf[I, -2] + S.rfn * torch.sqrt(g / d[I, -2]) * (el[I, -2] - el[I, -1])
+    uaf[I, -1] = 0
+    vaf[I, 1] = vaf[I, 1] - S.rfs * torch.sqrt(g / d[I, 1]) * (el[I, 1] - el[I, 0])
+    vaf[I, 0] = vaf[I, 1]
+    uaf[I, 0] = 0
+    return uaf * S.dum, vaf * S.dvm
+
+
+def tide_eta_solve(S, rhs, d, max_iter=200):
+    """coastal_boundary.py:41-58: subtract the tidal lift, solve the homogeneous
+    problem on the free nodes by CG in the area-weighted inner product from a
+    zero start (stop when r.r <= max(1e-30, 1e-24 r0.r0), at most 500
+    iterations), add the lift back.  Differentiated by autograd through the
+    iterations, as the reference is."""
+    T = S.tide
+    lift = torch.zeros_like(rhs)
+    lift[T.i, T.j] = tide_value(S, tide_seconds(S))
+    fixed = T.fixed
+    weight = S.art * (~fixed)
+
+    def op(v):
+        return elliptic_apply(S, v.masked_fill(fixed, 0), d).masked_fill(fixed, 0)
+
+    b = (rhs - elliptic_apply(S, lift, d)).masked_fill(fixed, 0)
+    x = torch.zeros_like(rhs)
+    r = b.clone()
+    p = r.clone()
+    rr = (weight * r * r).sum()
+    rr0 = rr
+    it = 0
+    for it in range(min(max_iter, 500)):
+        if float(rr.detach()) <= max(1e-30, float(rr0.detach()) * 1e-24):
+            break
+        ap = op(p)
+        den = (weight * p * ap).sum()
+        a = rr / (den + 1e-40)
+        x = x + a * p
+        r = r - a * ap
+        new = (weight * r * r).sum()
+        p = r + (new / (rr + 1e-40)) * p
+        rr = new
+    info = getattr(S, "eta_iters", None)
+    if info is not None:
+        info.append(it + 1)
+    return x + lift
 
 
 def bc3(S, uf, vf):
@@ -351,6 +471,8 @@ def solver_options(S):
 
 def eta_solve(S, rhs, d, x0):
     opts = solver_options(S)
+    if getattr(S, "tide", None) is not None:
+        return tide_eta_solve(S, rhs, d, opts["max_iter"])
     opts["info"] = getattr(S, "eta_iters", None)   # list collecting iteration counts
     if torch.is_grad_enabled() and (rhs.requires_grad or d.requires_grad):
         return _EtaSolve.apply(S, rhs, d, x0, opts)
@@ -480,7 +602,8 @@ def external_mode(S):
 
 
 def internal_mode(S):
-    """IM/time_loop.py:400-650 with the closed-basin boundary policy."""
+    """IM/time_loop.py:400-650 with the closed-basin boundary policy, or the
+    open forms of bcond 3-6 when a tide policy is installed."""
     im, jm, kbm1 = S.im, S.jm, S.kbm1
     A, Am = slice(1, im), slice(0, im - 1)
     B, Bm = slice(1, jm), slice(0, jm - 1)
@@ -496,13 +619,15 @@ def internal_mode(S):
     v[:, B, K] = (S.v[:, B, K] - _b(tps_v[:, B])) + _b(corr)
     S.v = v
 
-    S.w = bc5(S, X.vertvl(S, S.u, S.v, S.w, S.dt, S.etf, S.etb, S.vfluxb, S.vfluxf))
+    tide = getattr(S, "tide", None) is not None
+    w = X.vertvl(S, S.u, S.v, S.w, S.dt, S.etf, S.etb, S.vfluxb, S.vfluxf)
+    S.w = X.bcond5(S, w) if tide else bc5(S, w)
     q2f = X.advq(S, S.q2b, S.q2, S.u, S.v, S.w, S.aam, S.dt, S.etb, S.etf)
     q2lf = X.advq(S, S.q2lb, S.q2l, S.u, S.v, S.w, S.aam, S.dt, S.etb, S.etf)
     q2f, q2lf, S.l, S.kq, S.km, S.kh = X.profq(
         S, q2f, q2lf, S.q2, S.q2b, S.q2lb, S.km, S.kh, S.kq, S.u, S.v, S.t, S.s, S.rho,
         S.wusurf, S.wvsurf, S.wubot, S.wvbot, S.etf, S.dt)
-    q2f, q2lf = bc6(S, q2f, q2lf)
+    q2f, q2lf = X.bcond6(S, q2f, q2lf, S.q2, S.q2l, S.u, S.v) if tide else bc6(S, q2f, q2lf)
     q2 = S.q2 + 0.5 * S.smoth * (q2f + S.q2b - 2.0 * S.q2)
     q2l = S.q2l + 0.5 * S.smoth * (q2lf + S.q2lb - 2.0 * S.q2l)
     S.q2b, S.q2, S.q2lb, S.q2l = q2, q2f, q2l, q2lf
@@ -517,7 +642,7 @@ def internal_mode(S):
     S.tb, S.t, S.sb, S.s = tb, t, sb, s
     tf = X.proft(S, tf, S.wtsurf, S.tsurf, S.nbct, S.kh, S.etf, S.swrad)
     sf = X.proft(S, sf, S.wssurf, S.ssurf, S.nbcs, S.kh, S.etf, S.swrad)
-    tf, sf = bc4(S, tf, sf)
+    tf, sf = X.bcond4(S, tf, sf, S.t, S.s, S.u, S.v, S.w, S.dt) if tide else bc4(S, tf, sf)
     t = S.t + 0.5 * S.smoth * (tf + S.tb - 2.0 * S.t)
     s = S.s + 0.5 * S.smoth * (sf + S.sb - 2.0 * S.s)
     tf, sf = _shapiro3(S, tf, "tracer_shapiro", S.fsm), _shapiro3(S, sf, "tracer_shapiro", S.fsm)
@@ -528,7 +653,7 @@ def internal_mode(S):
     vf = X.advv(S, S.advy, S.drhoy, S.u, S.v, S.w, S.vb, S.dt, S.egf, S.egb, S.etf, S.etb, S.e_atmos)
     uf, S.wubot, _ = X.profu(S, uf, S.wusurf, S.wubot, S.km, S.etf, S.ub, S.vb)
     vf, S.wvbot, S.tps = X.profv(S, vf, S.wvsurf, S.wvbot, S.km, S.etf, S.ub, S.vb)
-    uf, vf = bc3(S, uf, vf)
+    uf, vf = X.bcond3(S, uf, vf, S.u, S.v, depth=S.d) if tide else bc3(S, uf, vf)
     dzw = S.dz[:kbm1].view(1, 1, -1)
     term = uf[:, :, K] + S.ub[:, :, K] - 2.0 * S.u[:, :, K]
     tps = torch.sum(term * dzw, dim=2)
