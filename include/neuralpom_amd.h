@@ -51,11 +51,26 @@ typedef enum {
   POM_NSTATE
 } pom_field;
 
-/* Static geometry and forcing (not differentiated). */
+/* Static geometry and forcing (not differentiated).  Entries from
+ * POM_NGEOM_BASE on belong to the open-boundary tide policy
+ * (src/neuralpom/cores/case_adapters/coastal_boundary.py:9-67, implicit family)
+ * and are read only when prm->boundary == POM_BOUNDARY_TIDE (may be null
+ * otherwise):
+ *   POM_G_TIDE_NODE  1 at tide nodes, 0 elsewhere
+ *   POM_G_TIDE_AMP   tide amplitude at the nodes (m)
+ *   POM_G_TIDE_PHASE tide phase at the nodes, in radians (phase_deg * pi / 180)
+ *   POM_G_ETA_FIXED  1 at interior points the eta solve holds fixed (tide
+ *                    nodes, dry cells); the edges are always held
+ *   POM_G_TBND, POM_G_SBND  exterior T / S of open edges, (im, jm, 16) with
+ *                    each edge cell holding its tbe/tbw/tbn/tbs value
+ *                    (explicit/base.py:181-189; j edges take precedence at corners) */
 typedef enum {
   POM_G_H = 0, POM_G_DX, POM_G_DY, POM_G_ART, POM_G_ARU, POM_G_ARV, POM_G_FSM, POM_G_DUM, POM_G_DVM,
   POM_G_COR, POM_G_CBC, POM_G_WUSURF, POM_G_WVSURF, POM_G_TSURF, POM_G_SSURF, /* 2-D */
   POM_G_RMEAN, POM_G_TCLIM, POM_G_SCLIM,                                      /* 3-D */
+  POM_NGEOM_BASE,
+  POM_G_TIDE_NODE = POM_NGEOM_BASE, POM_G_TIDE_AMP, POM_G_TIDE_PHASE, POM_G_ETA_FIXED, /* 2-D */
+  POM_G_TBND, POM_G_SBND,                                                              /* 3-D */
   POM_NGEOM
 } pom_geom;
 
@@ -68,20 +83,30 @@ typedef struct {
 #define POM_CORE_IMPLICIT 1
 #define POM_PRECOND_JACOBI 0
 #define POM_PRECOND_SPECTRAL 1
+#define POM_BOUNDARY_CLOSED 0   /* closed basin (implicit_p0_fixed/boundary/closed_basin.py) */
+#define POM_BOUNDARY_TIDE 1     /* open edges with tide nodes (case_adapters/coastal_boundary.py:9-67) */
 
 /* Scalars of the step (explicit/base.py:46-118, v1/core_spec.py:8-47).
  * The semi-implicit core (core = POM_CORE_IMPLICIT) also reads the eta-solver
  * settings (implicit_p0_fixed/core.py:62-119, numerics/double_gyre.py:186-191):
  * eta_max_iter, eta_precond, eta_tol (absolute), eta_rel_tol, and the
- * post-solve Shapiro weight gamma_baro (0 in production, core_spec.py:24-29). */
+ * post-solve Shapiro weight gamma_baro (0 in production, core_spec.py:24-29),
+ * and the lateral boundary policy: boundary = POM_BOUNDARY_CLOSED, or
+ * POM_BOUNDARY_TIDE with tide_wt for the open-boundary tide policy of the case
+ * adapter (coastal_boundary.py:9-67): tide elevation at the nodes, radiation
+ * conditions on the depth-mean transports with rfe rfw rfn rfs, the eta solve
+ * holding the edges, dry cells and nodes fixed, and the open forms of bcond
+ * 3-6 (kernels/coastal_explicit_baroclinic.py:496-928, exterior T / S from
+ * POM_G_TBND / POM_G_SBND).  The explicit core takes POM_BOUNDARY_CLOSED only. */
 typedef struct {
   double dti, dte, dti2, dte2, ispi, isp2i;
   double alpha, smoth, horcon, grav, rhoref, kappa, umol, tprni, small;
   double tbias, sbias, hmax, rfe, rfw, rfn, rfs;
   int32_t isplit, ispadv, ntp, nbct, nbcs, iext;
   double vmaxl;
-  int32_t core, eta_max_iter, eta_precond, reserved;
+  int32_t core, eta_max_iter, eta_precond, boundary;
   double eta_tol, eta_rel_tol, gamma_baro;
+  double tide_wt;   /* tide policy: (speed_deg_per_h * pi / 180) * t_s / 3600 of the step */
 } pom_params_t;
 
 typedef struct pom_batch pom_batch;

@@ -15,6 +15,7 @@ POM_OK, POM_E_ARG, POM_E_HIP, POM_E_UNSUPPORTED, POM_E_VMAX = 0, 1, 2, 3, 4
 POM_F32, POM_F64 = 0, 1
 POM_CORE_EXPLICIT, POM_CORE_IMPLICIT = 0, 1
 POM_PRECOND_JACOBI, POM_PRECOND_SPECTRAL = 0, 1
+POM_BOUNDARY_CLOSED, POM_BOUNDARY_TIDE = 0, 1
 
 # pom_field order (include/neuralpom_amd.h)
 FIELDS_3D = ("u", "ub", "v", "vb", "t", "tb", "s", "sb", "q2", "q2b", "q2l", "q2lb",
@@ -29,6 +30,11 @@ GEOM_2D = ("h", "dx", "dy", "art", "aru", "arv", "fsm", "dum", "dvm", "cor", "cb
            "wusurf", "wvsurf", "tsurf", "ssurf")
 GEOM_3D = ("rmean", "tclim", "sclim")
 GEOM = GEOM_2D + GEOM_3D
+# geometry of the open-boundary tide policy (read by the step only in that mode)
+GEOM_OPEN_2D = ("tide_node", "tide_amp", "tide_phase", "eta_fixed")
+GEOM_OPEN_3D = ("tbnd", "sbnd")
+GEOM_ALL = GEOM + GEOM_OPEN_2D + GEOM_OPEN_3D   # pom_geom order
+THREE_D = frozenset(FIELDS_3D + GEOM_3D + GEOM_OPEN_3D)
 
 # inputs with a cotangent (pom_explicit_step_adj)
 DIFF_INPUTS = ("u", "ub", "v", "vb", "t", "tb", "s", "sb", "q2", "q2b", "q2l", "q2lb", "km",
@@ -45,8 +51,8 @@ class Params(C.Structure):
         "dti", "dte", "dti2", "dte2", "ispi", "isp2i", "alpha", "smoth", "horcon", "grav",
         "rhoref", "kappa", "umol", "tprni", "small", "tbias", "sbias", "hmax", "rfe", "rfw",
         "rfn", "rfs")] + [(n, C.c_int32) for n in ("isplit", "ispadv", "ntp", "nbct", "nbcs", "iext")] + [
-        ("vmaxl", C.c_double)] + [(n, C.c_int32) for n in ("core", "eta_max_iter", "eta_precond", "reserved")] + [
-        (n, C.c_double) for n in ("eta_tol", "eta_rel_tol", "gamma_baro")]
+        ("vmaxl", C.c_double)] + [(n, C.c_int32) for n in ("core", "eta_max_iter", "eta_precond", "boundary")] + [
+        (n, C.c_double) for n in ("eta_tol", "eta_rel_tol", "gamma_baro", "tide_wt")]
 
 
 _lib = None
@@ -139,3 +145,10 @@ def ptr_array(ptrs):
     for i, p in enumerate(ptrs):
         arr[i] = p
     return arr
+
+
+def geom_ptrs(geom):
+    """The pom_geom pointer table: every base entry from ``geom`` (required),
+    the tide-policy entries when present (null otherwise)."""
+    return ptr_array([geom[n].data_ptr() for n in GEOM] +
+                     [geom[n].data_ptr() if geom.get(n) is not None else None for n in GEOM_ALL[len(GEOM):]])

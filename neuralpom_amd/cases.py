@@ -35,6 +35,12 @@ STATE_2D = ("el", "elb", "elf", "ua", "uab", "uaf", "va", "vab", "vaf", "d", "dt
 GEOM_2D = ("h", "dx", "dy", "art", "aru", "arv", "fsm", "dum", "dvm", "cor", "cbc", "wusurf", "wvsurf",
            "tsurf", "ssurf")
 GEOM_3D = ("rmean", "tclim", "sclim")
+# open-boundary values along the edges (base.py:181-194): per-j arrays of the
+# east / west edges, per-i arrays of the north / south edges
+EDGE_J = ("ele", "elw", "uabe", "uabw")
+EDGE_I = ("eln", "els", "vabn", "vabs")
+EDGE_J3 = ("sbe", "sbw", "tbe", "tbw", "ube", "ubw")
+EDGE_I3 = ("sbn", "sbs", "tbn", "tbs", "vbn", "vbs")
 
 
 def eos(t, s, h, zz, fsm, grav=9.806, rhoref=1035.0):
@@ -129,6 +135,10 @@ def _empty(im, jm, dtype, device):
     st.update({n: torch.zeros((im, jm), dtype=dtype, device=device) for n in STATE_2D})
     g = {n: torch.zeros((im, jm), dtype=dtype, device=device) for n in GEOM_2D}
     g.update({n: torch.zeros((im, jm, KB), dtype=dtype, device=device) for n in GEOM_3D})
+    g.update({n: torch.zeros(jm, dtype=dtype, device=device) for n in EDGE_J})
+    g.update({n: torch.zeros(im, dtype=dtype, device=device) for n in EDGE_I})
+    g.update({n: torch.zeros((jm, KB), dtype=dtype, device=device) for n in EDGE_J3})
+    g.update({n: torch.zeros((im, KB), dtype=dtype, device=device) for n in EDGE_I3})
     return st, g
 
 

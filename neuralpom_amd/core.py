@@ -218,7 +218,7 @@ class POM2K_Core:
         ens = self._ensemble()
         sc = self.step_scalars()
         vg = self._vgrid()
-        geom = {n: self._flat(n).detach() for n in L.GEOM}
+        geom = self._geom()
         check = self._vamax_checked()
         shapes = {n: tuple(getattr(self, n).shape) for n in L.FIELDS}
         for iint in range(1, int(self.iend) + 1):
@@ -231,7 +231,7 @@ class POM2K_Core:
             ramp = torch.tensor([self.ramp], dtype=self.dtype, device=self.device)
             internal = not (iint == 1 and self.time0 == 0.0)
             state = {n: self._flat(n) for n in L.FIELDS}
-            out, vamax = self._advance(ens, state, geom, sc, ramp, vg, internal)
+            out, vamax = self._advance(ens, state, geom, self._scalars_at_step(sc), ramp, vg, internal)
             for n in L.FIELDS:
                 setattr(self, n, out[n].view(shapes[n]))
             if check and not float(vamax) <= self.vmaxl:
@@ -240,7 +240,13 @@ class POM2K_Core:
                 return
         self._after_time_loop()
 
+    def _geom(self):
+        return {n: self._flat(n).detach() for n in L.GEOM}
+
     # hooks of the time loop (the semi-implicit core overrides them)
+    def _scalars_at_step(self, sc):
+        return sc
+
     def _before_time_loop(self) -> None:
         pass
 

@@ -1,0 +1,5 @@
+// Adjoint gathers of OpImpRadNew (float and double).
+#include "pom_ops_imp.h"
+#include "pom_gather_impl.h"
+
+POM_INSTANTIATE_GATHER(OpImpRadNew)

@@ -516,6 +516,7 @@ __global__ void __launch_bounds__(TPB) k_adjust_adj(Batch B, Vert<T> vg, AdjustA
 template <class T> struct VertvlArgs {
   const T *u, *v, *w, *dt, *etf, *etb, *vfluxb, *vfluxf, *dx, *dy, *fsm;
   T* wout;
+  int all_levels;   // bcond 5 masks every level (closed basin of the semi-implicit core), else k < kbm1
 };
 template <class T> __device__ __forceinline__ T vv_xf(const Pos& p, const VertvlArgs<T>& a, int i, int j, int k) {
   return T(0.25) * (a.dy[ix2(p, i, j)] + a.dy[ix2(p, i - 1, j)]) * (a.dt[ix2(p, i, j)] + a.dt[ix2(p, i - 1, j)]) *
@@ -548,11 +549,12 @@ __global__ void __launch_bounds__(TPB) k_vertvl(Batch B, Vert<T> vg, Params prm,
   } else {
     ld_col(a.w + c, w);
   }
-  // bcond 5: w * fsm on k < kbm1 (explicit, baroclinic.py:849-851); on every
-  // level for the closed-basin policy of the semi-implicit core (closed_basin.py:77-79)
+  // bcond 5: w * fsm on k < kbm1 (explicit, baroclinic.py:849-851, and the open
+  // forms of the tide policy); on every level for the closed-basin policy of
+  // the semi-implicit core (closed_basin.py:77-79)
   #pragma unroll
   for (int k = 0; k < KB; ++k)
-    if (k < KBM1 || prm.core) w[k] = w[k] * fsm;
+    if (k < KBM1 || a.all_levels) w[k] = w[k] * fsm;
   st_col(a.wout + c, w);
 }
 
@@ -562,6 +564,7 @@ template <class T> struct VertvlAdjArgs {
   const T* b_w;
   T* bdiv;   // scratch [3-D]: cotangent of div(k) / area at interior columns, else 0
   T *b_u, *b_v, *b_dt, *b_etf, *b_etb;
+  int all_levels;   // as VertvlArgs::all_levels
 };
 template <class T>
 __global__ void __launch_bounds__(TPB) k_vertvl_adj1(Batch B, Vert<T> vg, Params prm, VertvlAdjArgs<T> a) {
@@ -579,7 +582,7 @@ __global__ void __launch_bounds__(TPB) k_vertvl_adj1(Batch B, Vert<T> vg, Params
     #pragma unroll
     for (int k = KBM1 - 1; k >= 0; --k) {
       const int kw = k + 1;
-      suffix += a.b_w[c + kw] * ((kw < KBM1 || prm.core) ? fsm : T(1));
+      suffix += a.b_w[c + kw] * ((kw < KBM1 || a.all_levels) ? fsm : T(1));
       const T binc = suffix * vg.dz(p.m, k);
       bd[k] = binc / area;
       beta += binc;

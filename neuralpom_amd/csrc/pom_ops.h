@@ -362,10 +362,13 @@ struct OpDens {
 
 // =====================================================================
 // bcond 3: internal velocity lateral boundaries + masks
-// reference: src/neuralpom/cores/explicit/baroclinic.py:576-661
+// reference: src/neuralpom/cores/explicit/baroclinic.py:576-661.
+// DEPTH: the case-adapter kernel copy, whose edge weight ga takes the total
+// depth d (a differentiated 2-D input) instead of h
+// (case_adapters/kernels/coastal_explicit_baroclinic.py:586, 602, 627, 641)
 // =====================================================================
-struct OpBc3 {
-  enum { UF, VF, U, V, N3, NIN = N3 };
+template <bool DEPTH> struct OpBc3G {
+  enum { UF, VF, U, V, N3, D_ = N3, NIN = DEPTH ? N3 + 1 : N3 };
   enum { GH, GDUM, GDVM };
   static constexpr int NOUT = 2;
   static constexpr bool OUT3 = true;
@@ -377,9 +380,17 @@ struct OpBc3 {
   template <int F, int OJ, class AC> __device__ static auto smi(const AC& A) {
     return 0.25 * AT(F, -1, OJ, 0) + 0.50 * AT(F, 0, OJ, 0) + 0.25 * AT(F, 1, OJ, 0);
   }
-  template <class AC, class X> __device__ static auto ga(const AC& A, X h, double hmax) {
-    using T = typename scalar_of<X>::type;
-    return P<T>{sqrt(fmax(val(h) / T(hmax), T(1e-8)))};
+  // ga = safe_sqrt(depth / hmax) at (OI, OJ) (utils/safety.py:25-26)
+  template <int OI, int OJ, class AC> __device__ static auto ga(const AC& A, double hmax) {
+    if constexpr (DEPTH) {
+      auto x = AT2(D_, OI, OJ) / hmax;
+      using T = typename scalar_of<decltype(x)>::type;
+      return BR(val(x) > T(1e-8), psqrt(x), P<T>{T(sqrt(T(1e-8)))});
+    } else {
+      auto h = G2(GH, OI, OJ);
+      using T = typename scalar_of<decltype(h)>::type;
+      return P<T>{sqrt(fmax(val(h) / T(hmax), T(1e-8)))};
+    }
   }
   template <int O, class AC> __device__ static auto eval(const AC& A, const Params& prm) {
     const int i = A.p.i, j = A.p.j, k = A.p.k, im = A.p.im, jm = A.p.jm;
@@ -387,15 +398,15 @@ struct OpBc3 {
     const bool jin = j >= 1 && j <= jm - 2, iin = i >= 1 && i <= im - 2;
     if constexpr (O == 0) {
       auto east = [&]() {   // :585-599
-        auto g = ga(A, G2(GH, 0, 0), prm.hmax);
+        auto g = ga<0, 0>(A, prm.hmax);
         return g * smj<U, -1>(A) + (1.0 - g) * smj<U, 0>(A);
       };
       auto west1 = [&]() {  // :601-617 at i = 1
-        auto g = ga(A, G2(GH, -1, 0), prm.hmax);
+        auto g = ga<-1, 0>(A, prm.hmax);
         return g * smj<U, 1>(A) + (1.0 - g) * smj<U, 0>(A);
       };
       auto west0 = [&]() {  // :618 uf(0) = uf(1)
-        auto g = ga(A, G2(GH, 0, 0), prm.hmax);
+        auto g = ga<0, 0>(A, prm.hmax);
         return g * smj<U, 2>(A) + (1.0 - g) * smj<U, 1>(A);
       };
       using R = common_t<common_t<common_t<decltype(east()), decltype(west1())>, decltype(west0())>,
@@ -410,15 +421,15 @@ struct OpBc3 {
       return r * G2(GDUM, 0, 0);
     } else {
       auto north = [&]() {  // :626-638
-        auto g = ga(A, G2(GH, 0, 0), prm.hmax);
+        auto g = ga<0, 0>(A, prm.hmax);
         return g * smi<V, -1>(A) + (1.0 - g) * smi<V, 0>(A);
       };
       auto south1 = [&]() { // :640-652 at j = 1
-        auto g = ga(A, G2(GH, 0, -1), prm.hmax);
+        auto g = ga<0, -1>(A, prm.hmax);
         return g * smi<V, 1>(A) + (1.0 - g) * smi<V, 0>(A);
       };
       auto south0 = [&]() { // :653 vf(:,0) = vf(:,1)
-        auto g = ga(A, G2(GH, 0, 0), prm.hmax);
+        auto g = ga<0, 0>(A, prm.hmax);
         return g * smi<V, 2>(A) + (1.0 - g) * smi<V, 1>(A);
       };
       using R = common_t<common_t<common_t<decltype(north()), decltype(south1())>, decltype(south0())>,
@@ -434,6 +445,8 @@ struct OpBc3 {
     }
   }
 };
+using OpBc3 = OpBc3G<false>;
+using OpBc3D = OpBc3G<true>;
 
 // =====================================================================
 // bcond 4 (upstream T/S at the edges) fused with the T/S Asselin filter
@@ -441,8 +454,11 @@ struct OpBc3 {
 // copy f(kb) = f(kbm1) (tracers.py:462-465)
 // outputs: 0 t_new, 1 tb_new, 2 s_new, 3 sb_new
 // =====================================================================
-struct OpBc4Filter {
-  enum { TF, SF, TT, SS, TB, SB, U, V, W, N3, DT = N3, NIN };
+// EXT: exterior tracer values tbe/tbw/tbn/tbs (sbe...) of open boundaries
+// (explicit/base.py:181-189), as 3-D fields TX / SX holding each edge cell's
+// exterior value; without EXT they are zero (closed basins).
+template <bool EXT> struct OpBc4FilterG {
+  enum { TF, SF, TT, SS, TB, SB, U, V, W, TX, SX, N3 = EXT ? SX + 1 : TX, DT = N3, NIN };
   enum { GDX, GDY, GFSM };
   static constexpr int NOUT = 4;
   static constexpr bool OUT3 = true;
@@ -468,8 +484,11 @@ struct OpBc4Filter {
     auto u1 = 2.0 * vel * prm.dti / dsum;
     auto fc = AT(FC, 0, 0, 0);
     auto fi = AT(FC, II, JJ, 0);
-    // boundary (exterior) values tbe/tbw/tbn/tbs are zero for closed basins
-    auto f_in = BR(SIDE == 0 || SIDE == 2, fc - u1 * (0.0 - fc), fc - u1 * (fc - 0.0));
+    auto fx = [&]() {
+      if constexpr (EXT) return AT(FC == TT ? TX : SX, 0, 0, 0);
+      else return 0.0;
+    }();
+    auto f_in = BR(SIDE == 0 || SIDE == 2, fc - u1 * (fx - fc), fc - u1 * (fc - fx));
     auto f_out = BR(SIDE == 0 || SIDE == 2, fc - u1 * (fc - fi), fc - u1 * (fi - fc));
     const bool inflow = (SIDE == 0 || SIDE == 2) ? (val(u1) <= 0) : (val(u1) >= 0);
     auto res = BR(inflow, f_in, f_out);
@@ -513,6 +532,8 @@ struct OpBc4Filter {
     }
   }
 };
+using OpBc4Filter = OpBc4FilterG<false>;
+using OpBc4FilterExt = OpBc4FilterG<true>;
 
 // =====================================================================
 // bcond 6 (upstream q2/q2l at the edges, +1e-10) fused with the q2/q2l

@@ -272,6 +272,102 @@ struct OpImpCorr {
 };
 
 // =====================================================================
+// open-boundary tide policy (case_adapters/coastal_boundary.py:9-67).
+//
+// bcond 2 as radiation conditions on the depth-mean transports (:24-30),
+// applied to the predictor's or the corrector's interior values:
+//   east  uaf(E, j) = uaf(E-1, j) + rfe sqrt(g / d(E-1, j)) (el(E-1, j) - el(E, j)), vaf(E, j) = 0
+//   west  uaf(1, j) -= rfw sqrt(g / d(1, j)) (el(1, j) - el(0, j)), uaf(0, j) = uaf(1, j), vaf(0, j) = 0
+//   north vaf(i, N) = vaf(i, N-1) + rfn sqrt(g / d(i, N-1)) (el(i, N-1) - el(i, N)), uaf(i, N) = 0
+//   south vaf(i, 1) -= rfs sqrt(g / d(i, 1)) (el(i, 1) - el(i, 0)), vaf(i, 0) = vaf(i, 1), uaf(i, 0) = 0
+// for interior j (east / west) and interior i (north / south), then * dum / dvm.
+// UI, VI: interior values (their edges are not read); UE, VE: the array the
+// reference starts from, whose four corners no statement writes; EL: the
+// current surface elevation; D_: the depth d (NEWD false: the predictor,
+// core.py:853-856), or the new surface elf with d = h + elf (NEWD true: the
+// corrector, which runs after the time loop sets d = h + elf).
+// outputs 0 uaf, 1 vaf.
+// =====================================================================
+template <bool NEWD> struct OpImpRadG {
+  enum { UI, VI, UE, VE, EL, D_, N3 = 0, NIN = 6 };
+  enum { GDUM, GDVM, GH };
+  static constexpr int NOUT = 2;
+  static constexpr bool OUT3 = false;
+  static constexpr int RI = 1, RJ = 1, RK = 0;
+  static constexpr int EDGE_BAND = 1;   // only points within 1 of the edge read neighbours
+  // rf sqrt(g / d(a)) (el(a) - el(b)) at offsets a = (AI, AJ), b = (BI, BJ)
+  template <int AI, int AJ, class AC> __device__ static auto depth(const AC& A) {
+    if constexpr (NEWD) return G2(GH, AI, AJ) + AT2(D_, AI, AJ);
+    else return AT2(D_, AI, AJ);
+  }
+  template <int AI, int AJ, int BI, int BJ, class AC> __device__ static auto rad(const AC& A, double rf, double grav) {
+    return rf * psqrt(grav / depth<AI, AJ>(A)) * (AT2(EL, AI, AJ) - AT2(EL, BI, BJ));
+  }
+  template <int O, class AC> __device__ static auto eval(const AC& A, const Params& prm) {
+    const int i = A.p.i, j = A.p.j, E = A.p.im - 1, N = A.p.jm - 1;
+    const bool jin = j >= 1 && j <= N - 1, iin = i >= 1 && i <= E - 1;
+    if constexpr (O == 0) {
+      auto east = [&]() { return AT2(UI, -1, 0) + rad<-1, 0, 0, 0>(A, prm.rfe, prm.grav); };
+      auto west1 = [&]() { return AT2(UI, 0, 0) - rad<0, 0, -1, 0>(A, prm.rfw, prm.grav); };
+      auto west0 = [&]() { return AT2(UI, 1, 0) - rad<1, 0, 0, 0>(A, prm.rfw, prm.grav); };
+      using R = common_t<common_t<common_t<decltype(east()), decltype(west1())>, decltype(west0())>,
+                         decltype(AT2(UE, 0, 0))>;
+      R r{};
+      if (jin && i == E) r = cvt<R>(east());
+      else if (jin && i == 1) r = cvt<R>(west1());
+      else if (jin && i == 0) r = cvt<R>(west0());
+      else if (iin && (j == N || j == 0)) r = R{};
+      else if (iin && jin) r = cvt<R>(AT2(UI, 0, 0));
+      else r = cvt<R>(AT2(UE, 0, 0));
+      return r * G2(GDUM, 0, 0);
+    } else {
+      auto north = [&]() { return AT2(VI, 0, -1) + rad<0, -1, 0, 0>(A, prm.rfn, prm.grav); };
+      auto south1 = [&]() { return AT2(VI, 0, 0) - rad<0, 0, 0, -1>(A, prm.rfs, prm.grav); };
+      auto south0 = [&]() { return AT2(VI, 0, 1) - rad<0, 1, 0, 0>(A, prm.rfs, prm.grav); };
+      using R = common_t<common_t<common_t<decltype(north()), decltype(south1())>, decltype(south0())>,
+                         decltype(AT2(VE, 0, 0))>;
+      R r{};
+      if (iin && j == N) r = cvt<R>(north());
+      else if (iin && j == 1) r = cvt<R>(south1());
+      else if (iin && j == 0) r = cvt<R>(south0());
+      else if (jin && (i == E || i == 0)) r = R{};
+      else if (iin && jin) r = cvt<R>(AT2(VI, 0, 0));
+      else r = cvt<R>(AT2(VE, 0, 0));
+      return r * G2(GDVM, 0, 0);
+    }
+  }
+};
+using OpImpRad = OpImpRadG<false>;
+using OpImpRadNew = OpImpRadG<true>;
+
+// =====================================================================
+// post-solve surface of the tide policy: Shapiro (gamma) * fsm on the
+// interior (core.py:893-911), then bcond 1 of the adapter (coastal_boundary.py:
+// 12-23): the tide at the nodes, then * fsm; edges keep the solution.
+// GLIFT: the tide at the nodes at this step (zero elsewhere).
+// =====================================================================
+struct OpImpSurfTide {
+  enum { RAW, N3 = 0, NIN = 1 };
+  enum { GFSM, GNODE, GLIFT };
+  static constexpr int NOUT = 1;
+  static constexpr bool OUT3 = false;
+  static constexpr int RI = 1, RJ = 1, RK = 0;
+  template <int O, class AC> __device__ static auto eval(const AC& A, const Params& prm) {
+    auto sm = [&]() {
+      return (AT2(RAW, 0, 0) + (prm.gamma_baro / 4.0) * (AT2(RAW, 1, 0) + AT2(RAW, -1, 0) + AT2(RAW, 0, 1) +
+                                                         AT2(RAW, 0, -1) - 4.0 * AT2(RAW, 0, 0))) *
+             G2(GFSM, 0, 0);
+    };
+    using R = common_t<decltype(sm()), decltype(G2(GLIFT, 0, 0))>;
+    R r{};
+    if (val(G2(GNODE, 0, 0)) != 0) r = cvt<R>(G2(GLIFT, 0, 0));
+    else if (interior(A)) r = cvt<R>(sm());
+    else r = cvt<R>(AT2(RAW, 0, 0));
+    return r * G2(GFSM, 0, 0);
+  }
+};
+
+// =====================================================================
 // Asselin filter and level shift of the depth-mean state, egf, and the
 // endpoint-mean face transports (time_loop.py:323-398).  outputs
 // 0 ua, 1 uab, 2 va, 3 vab, 4 el, 5 elb, 6 d, 7 egf, 8 utf, 9 vtf, 10 etf

@@ -1,6 +1,7 @@
 // Batched preconditioned conjugate gradients for the eta equation
 //   A(D) eta = rhs,  A eta = eta - 4 dti^2 g alpha div(D grad eta) / art,
-// eta = 0 on every edge (closed basin), one independent system per member.
+// eta = 0 on every edge (closed basin) or, with the tide policy, eta held at
+// the tide at the fixed points, one independent system per member.
 //
 // reference: src/neuralpom/cores/implicit_p0_fixed/core.py:460-589 (eager PCG: update
 // order, stopping rule rs <= tol^2 | rs <= rel_tol^2 rs0 | rs <= 1e-30, p edges zeroed),
@@ -24,7 +25,8 @@ enum { PC_RHO = 0, PC_THR, PC_TOL2, PC_ALPHA, PC_BETA, PC_RS, PC_RS0, PC_AUX, PC
 
 template <class T> struct EllGeom {
   const T *dx, *dy, *art, *dum, *dvm;
-  double coeff;   // 4 dti^2 g alpha
+  double coeff;             // 4 dti^2 g alpha
+  const T* fixed = nullptr; // nonzero at interior points held fixed (tide nodes, dry cells); null: none
 };
 
 // A(D) x at an interior point (same arithmetic as OpEll)
@@ -67,6 +69,12 @@ __device__ __forceinline__ bool interior_p(const Pos& p) {
   return p.i >= 1 && p.i <= p.im - 2 && p.j >= 1 && p.j <= p.jm - 2;
 }
 
+// an unknown of the solve: interior and not held fixed (case_adapters/
+// coastal_boundary.py:44-49 holds the edges, dry cells and tide nodes)
+template <class T> __device__ __forceinline__ bool free_p(const Pos& p, const T* fixed, int64_t c) {
+  return interior_p(p) && !(fixed && fixed[c] != T(0));
+}
+
 // block sum of a double (TPB threads), result valid in thread 0
 __device__ __forceinline__ double block_sum(double v, double* sh) {
   sh[threadIdx.x] = v;
@@ -81,16 +89,30 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 // member and active flag of a 2-D block; pos2-style position of this thread
 __device__ __forceinline__ int block_member(const Batch& B) { return member_of(B.blk2, B.M, blockIdx.x); }
 
-// ---------------------------------------------------------------- set-up
-// x = x0 on the interior, 0 on the edges; r = rhs - A x on the interior, 0 on
-// the edges (rhs edges are zeroed by the reference); z = 0
+// ---------------------------------------------------------------- tide
+// the tide at the nodes, amp cos(wt - phase), zero elsewhere
+// (case_adapters/coastal_boundary.py:9-10; wt = (speed pi / 180) t / 3600 and
+// phase in radians are formed by the caller)
 template <class T>
-__global__ void __launch_bounds__(TPB) k_pcg_x0(Batch B, const T* __restrict__ x0, T* __restrict__ x,
-                                                T* __restrict__ z) {
+__global__ void __launch_bounds__(TPB) k_tide_lift(Batch B, const T* __restrict__ node, const T* __restrict__ amp,
+                                                   const T* __restrict__ phase, double wt, T* __restrict__ lift) {
   Pos p;
   if (!pos2(B, p)) return;
   const int64_t c = ix2(p, p.i, p.j);
-  x[c] = (x0 && interior_p(p)) ? x0[c] : T(0);
+  lift[c] = node[c] != T(0) ? T(double(amp[c]) * cos(wt - double(phase[c]))) : T(0);
+}
+
+// ---------------------------------------------------------------- set-up
+// x = x0 at the unknowns, the lift (zero when null) at the edges and fixed
+// points; r = rhs - A x at the unknowns, 0 elsewhere (rhs edges are zeroed by
+// the reference); z = 0
+template <class T>
+__global__ void __launch_bounds__(TPB) k_pcg_x0(Batch B, const T* __restrict__ x0, const T* __restrict__ fixed,
+                                                const T* __restrict__ lift, T* __restrict__ x, T* __restrict__ z) {
+  Pos p;
+  if (!pos2(B, p)) return;
+  const int64_t c = ix2(p, p.i, p.j);
+  x[c] = free_p(p, fixed, c) ? (x0 ? x0[c] : T(0)) : (lift ? lift[c] : T(0));
   z[c] = T(0);
 }
 
@@ -101,7 +123,7 @@ __global__ void __launch_bounds__(TPB) k_pcg_resid(Batch B, EllGeom<T> g, const 
   Pos p;
   if (!pos2(B, p)) return;
   const int64_t c = ix2(p, p.i, p.j);
-  r[c] = interior_p(p) ? rhs[c] - ell_point(p, g, d, x) : T(0);
+  r[c] = free_p(p, g.fixed, c) ? rhs[c] - ell_point(p, g, d, x) : T(0);
   p_out[c] = T(0);
 }
 
@@ -119,7 +141,7 @@ __global__ void __launch_bounds__(TPB) k_pcg_jacobi(Batch B, EllGeom<T> g, const
   }
   Pos p;
   double rz = 0.0;
-  if (pos2(B, p) && interior_p(p)) {
+  if (pos2(B, p) && free_p(p, g.fixed, ix2(p, p.i, p.j))) {
     const int64_t c = ix2(p, p.i, p.j);
     const T zv = r[c] / (ell_diag(p, g, d) + T(1e-30));
     z[c] = zv;
@@ -198,7 +220,7 @@ __global__ void __launch_bounds__(TPB) k_pcg_ap(Batch B, EllGeom<T> g, const T* 
   double v = 0.0;
   if (pos2(B, p)) {
     const int64_t c = ix2(p, p.i, p.j);
-    if (interior_p(p)) {
+    if (free_p(p, g.fixed, c)) {
       const T a = ell_point(p, g, d, p_);
       ap[c] = a;
       v = double(p_[c]) * double(a);
@@ -283,16 +305,16 @@ __global__ void __launch_bounds__(TPB) k_pcg_beta(Batch B, const double* __restr
   }
 }
 
-// p = p beta + z on the interior, 0 on the edges
+// p = p beta + z at the unknowns, 0 at the edges and fixed points
 template <class T>
 __global__ void __launch_bounds__(TPB) k_pcg_p(Batch B, T* __restrict__ p_, const T* __restrict__ z,
-                                               const double* __restrict__ sc, const int* __restrict__ act,
-                                               int first) {
+                                               const T* __restrict__ fixed, const double* __restrict__ sc,
+                                               const int* __restrict__ act, int first) {
   Pos p;
   if (!pos2(B, p)) return;
   if (!act[p.m]) return;
   const int64_t c = ix2(p, p.i, p.j);
-  if (!interior_p(p)) {
+  if (!free_p(p, fixed, c)) {
     p_[c] = T(0);
     return;
   }

@@ -174,17 +174,18 @@ template <class T> struct Pcg {
   }
 
   // x = A(d)^-1 rhs from x0 (null: zero), per-member absolute tolerance tol[]
-  // (device, set by the caller) and relative tolerance rel_tol
+  // (device, set by the caller) and relative tolerance rel_tol.  The edges and
+  // the points g.fixed marks keep the lift (null: zero).
   void solve(const Ctx<T>& c, const EllGeom<T>& g, const T* rhs, const T* d, const T* x0, T* x, double rel_tol,
-             int max_iter) {
+             int max_iter, const T* lift = nullptr) {
     const dim3 g2(c.nblk2), tb(TPB), gm(c.B.M);
-    hipLaunchKernelGGL(k_pcg_x0<T>, g2, tb, 0, c.stream, c.B, x0, x, z);
+    hipLaunchKernelGGL(k_pcg_x0<T>, g2, tb, 0, c.stream, c.B, x0, g.fixed, lift, x, z);
     hipLaunchKernelGGL(k_pcg_resid<T>, g2, tb, 0, c.stream, c.B, g, d, x, rhs, r, p);
     hipLaunchKernelGGL(k_pcg_act1, dim3(1), tb, 0, c.stream, act, c.B.M);
     precond(c, g, d);
     hipLaunchKernelGGL(k_pcg_dot<T>, g2, tb, 0, c.stream, c.B, r, r, act, part, 1);
     hipLaunchKernelGGL(k_pcg_start, gm, tb, 0, c.stream, c.B, part, sc, act, its, tol, rel_tol);
-    hipLaunchKernelGGL(k_pcg_p<T>, g2, tb, 0, c.stream, c.B, p, z, sc, act, 1);
+    hipLaunchKernelGGL(k_pcg_p<T>, g2, tb, 0, c.stream, c.B, p, z, g.fixed, sc, act, 1);
     int* hw = pinned_word();
     for (int it = 1; it <= max_iter; ++it) {
       hipLaunchKernelGGL(k_pcg_ap<T>, g2, tb, 0, c.stream, c.B, g, d, p, ap, act, part);
@@ -199,7 +200,7 @@ template <class T> struct Pcg {
       if (it == max_iter) break;
       precond(c, g, d);
       hipLaunchKernelGGL(k_pcg_beta, gm, tb, 0, c.stream, c.B, part, sc, act);
-      hipLaunchKernelGGL(k_pcg_p<T>, g2, tb, 0, c.stream, c.B, p, z, sc, act, 0);
+      hipLaunchKernelGGL(k_pcg_p<T>, g2, tb, 0, c.stream, c.B, p, z, g.fixed, sc, act, 0);
     }
   }
 };
@@ -223,6 +224,10 @@ template <class T> struct Step {
   int32_t* iters_out = nullptr;       // per-member eta-solve iterations (device), may be null
   X<T> xi;                            // depth-mean state after the semi-implicit barotropic step
   T *us, *vs, *dus, *dvs, *rhs, *elfraw;
+  // open-boundary tide policy (case_adapters/coastal_boundary.py:9-67)
+  bool open = false;
+  double tide_wt = 0.0;               // (speed pi / 180) t / 3600 of the step's tide
+  T *lift = nullptr, *us2, *vs2, *uac, *vac;   // tide at the nodes; radiated predictor; closed corrector
   Pcg<T> pw;
   T *u1, *v1, *q2f0, *q2lf0, *q2f1, *q2lf1, *kmraw, *khraw, *tf0, *sf0, *tf1, *sf1, *uf0, *vf0, *uf1, *vf1, *uf2,
       *vf2, *tpsu;
@@ -249,6 +254,9 @@ template <class T> struct Step {
     if (imp) {
       us = A.get<T>(N2); vs = A.get<T>(N2); dus = A.get<T>(N2); dvs = A.get<T>(N2);
       rhs = A.get<T>(N2); elfraw = A.get<T>(N2);
+      if (open) {
+        lift = A.get<T>(N2); us2 = A.get<T>(N2); vs2 = A.get<T>(N2); uac = A.get<T>(N2); vac = A.get<T>(N2);
+      }
       pw.plan(A, N2, b->nblk2, b->M);
       xi = X<T>{};
       xi.ua = out[POM_UA]; xi.uab = out[POM_UAB]; xi.va = out[POM_VA]; xi.vab = out[POM_VAB];
@@ -474,8 +482,12 @@ template <class T> struct Step {
   const X<T>& xf() const { return imp ? xi : xs[isplit]; }
   EllGeom<T> ell() const {
     return EllGeom<T>{g[POM_G_DX], g[POM_G_DY], g[POM_G_ART], g[POM_G_DUM], g[POM_G_DVM],
-                      4.0 * (c.prm.dti * c.prm.dti) * c.prm.grav * c.prm.alpha};
+                      4.0 * (c.prm.dti * c.prm.dti) * c.prm.grav * c.prm.alpha,
+                      open ? g[POM_G_ETA_FIXED] : nullptr};
   }
+  // predictor values the transport and the corrector read (after bcond 2)
+  const T* usr() const { return open ? us2 : us; }
+  const T* vsr() const { return open ? vs2 : vs; }
   // ---- semi-implicit tables
   FieldTab<T> t_pred() const {
     FieldTab<T> t{};
@@ -494,7 +506,8 @@ template <class T> struct Step {
   }
   FieldTab<T> t_rhs() const {
     FieldTab<T> t{};
-    t.f[OpImpRhs::US] = us; t.f[OpImpRhs::VS] = vs; t.f[OpImpRhs::D_] = in[POM_D]; t.f[OpImpRhs::ELB] = in[POM_ELB];
+    t.f[OpImpRhs::US] = usr(); t.f[OpImpRhs::VS] = vsr(); t.f[OpImpRhs::D_] = in[POM_D];
+    t.f[OpImpRhs::ELB] = in[POM_ELB];
     t.g[OpImpRhs::GDX] = g[POM_G_DX]; t.g[OpImpRhs::GDY] = g[POM_G_DY]; t.g[OpImpRhs::GART] = g[POM_G_ART];
     t.g[OpImpRhs::GVFLUXF] = out[POM_VFLUXF];
     return t;
@@ -507,11 +520,29 @@ template <class T> struct Step {
   }
   FieldTab<T> t_corr() const {
     FieldTab<T> t{};
-    t.f[OpImpCorr::US] = us; t.f[OpImpCorr::VS] = vs; t.f[OpImpCorr::DUS] = dus; t.f[OpImpCorr::DVS] = dvs;
+    t.f[OpImpCorr::US] = usr(); t.f[OpImpCorr::VS] = vsr(); t.f[OpImpCorr::DUS] = dus;
+    t.f[OpImpCorr::DVS] = dvs;
     t.f[OpImpCorr::ELF] = out[POM_ELF];
     t.g[OpImpCorr::GDX] = g[POM_G_DX]; t.g[OpImpCorr::GDY] = g[POM_G_DY]; t.g[OpImpCorr::GH] = g[POM_G_H];
     t.g[OpImpCorr::GARU] = g[POM_G_ARU]; t.g[OpImpCorr::GARV] = g[POM_G_ARV];
     t.g[OpImpCorr::GDUM] = g[POM_G_DUM]; t.g[OpImpCorr::GDVM] = g[POM_G_DVM];
+    return t;
+  }
+  // radiation bcond 2 on interior values ui, vi; corners from ue, ve
+  // dd: the depth d (predictor) or the new surface elf, with d = h + elf (corrector)
+  FieldTab<T> t_rad(const T* ui, const T* vi, const T* ue, const T* ve, const T* dd) const {
+    FieldTab<T> t{};
+    t.f[OpImpRad::UI] = ui; t.f[OpImpRad::VI] = vi; t.f[OpImpRad::UE] = ue; t.f[OpImpRad::VE] = ve;
+    t.f[OpImpRad::EL] = in[POM_EL]; t.f[OpImpRad::D_] = dd;
+    t.g[OpImpRad::GDUM] = g[POM_G_DUM]; t.g[OpImpRad::GDVM] = g[POM_G_DVM]; t.g[OpImpRad::GH] = g[POM_G_H];
+    return t;
+  }
+  FieldTab<T> t_surf_tide() const {
+    FieldTab<T> t{};
+    t.f[OpImpSurfTide::RAW] = elfraw;
+    t.g[OpImpSurfTide::GFSM] = g[POM_G_FSM];
+    t.g[OpImpSurfTide::GNODE] = g[POM_G_TIDE_NODE];
+    t.g[OpImpSurfTide::GLIFT] = lift;
     return t;
   }
   FieldTab<T> t_ifilter() const {
@@ -695,6 +726,18 @@ template <class T> struct Step {
     t.g[OpBc3::GDVM] = g[POM_G_DVM];
     return t;
   }
+  FieldTab<T> t_bc3d() const {   // edge weight from the depth after the barotropic step
+    FieldTab<T> t = t_bc3();
+    t.f[OpBc3D::D_] = xf().d;
+    return t;
+  }
+  FieldTab<T> t_bc4ext() const {
+    FieldTab<T> t = t_bc4();
+    t.f[OpBc4FilterExt::TX] = g[POM_G_TBND];
+    t.f[OpBc4FilterExt::SX] = g[POM_G_SBND];
+    t.f[OpBc4FilterExt::DT] = in[POM_DT];
+    return t;
+  }
 
   ProfqArgs<T> a_profq() const {
     ProfqArgs<T> s{};
@@ -769,6 +812,7 @@ template <class T> struct Step {
     s.dy = g[POM_G_DY];
     s.fsm = g[POM_G_FSM];
     s.wout = out[POM_W];
+    s.all_levels = imp && !open;
     return s;
   }
 
@@ -826,13 +870,24 @@ template <class T> struct Step {
   void external_implicit() {
     const int64_t N2 = n2();
     launch_map<T, OpImpPred>(c, t_pred(), OutTab<T>{{us, vs, dus, dvs}});
+    if (open) launch_map<T, OpImpRad>(c, t_rad(us, vs, in[POM_UA], in[POM_VA], in[POM_D]), OutTab<T>{{us2, vs2}});
     launch_map<T, OpImpRhs>(c, t_rhs(), OutTab<T>{{out[POM_FLUXUA], out[POM_FLUXVA], rhs}});
+    if (open)
+      hipLaunchKernelGGL(k_tide_lift<T>, dim3(c.nblk2), dim3(TPB), 0, c.stream, c.B, g[POM_G_TIDE_NODE],
+                         g[POM_G_TIDE_AMP], g[POM_G_TIDE_PHASE], tide_wt, lift);
     hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(256), 0, c.stream, pw.tol, (int64_t)b->M, hp->eta_tol);
-    pw.solve(c, ell(), rhs, in[POM_D], in[POM_ELF], elfraw, hp->eta_rel_tol, hp->eta_max_iter);
+    pw.solve(c, ell(), rhs, in[POM_D], in[POM_ELF], elfraw, hp->eta_rel_tol, hp->eta_max_iter, lift);
     if (iters_out)
       hipLaunchKernelGGL(k_copy<int32_t>, dim3(1), dim3(256), 0, c.stream, iters_out, pw.its, (int64_t)b->M);
-    launch_map<T, OpImpSurf>(c, t_surf(), OutTab<T>{{out[POM_ELF]}});
-    launch_map<T, OpImpCorr>(c, t_corr(), OutTab<T>{{out[POM_UAF], out[POM_VAF]}});
+    if (open) {
+      launch_map<T, OpImpSurfTide>(c, t_surf_tide(), OutTab<T>{{out[POM_ELF]}});
+      launch_map<T, OpImpCorr>(c, t_corr(), OutTab<T>{{uac, vac}});
+      launch_map<T, OpImpRadNew>(c, t_rad(uac, vac, in[POM_UAF], in[POM_VAF], out[POM_ELF]),
+                                 OutTab<T>{{out[POM_UAF], out[POM_VAF]}});
+    } else {
+      launch_map<T, OpImpSurf>(c, t_surf(), OutTab<T>{{out[POM_ELF]}});
+      launch_map<T, OpImpCorr>(c, t_corr(), OutTab<T>{{out[POM_UAF], out[POM_VAF]}});
+    }
     if (vamax)
       hipLaunchKernelGGL(k_vamax<T>, dim3(std::min(blocks(N2), 1024)), dim3(256), 0, c.stream, out[POM_VAF], N2,
                          vamax);
@@ -891,7 +946,7 @@ template <class T> struct Step {
         hipLaunchKernelGGL(k_profq<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a_profq());
       }
       launch_map<T, OpKEdge>(c, t_kedge(), OutTab<T>{{out[POM_KM], out[POM_KH]}});
-      if (imp)
+      if (imp && !open)
         launch_map<T, OpImpBc6Filter>(c, t_ibc6(),
                                       OutTab<T>{{out[POM_Q2], out[POM_Q2B], out[POM_Q2L], out[POM_Q2LB]}});
       else
@@ -900,7 +955,9 @@ template <class T> struct Step {
       launch_map<T, OpAdvt1>(c, t_advt1(true), OutTab<T>{{sf0}});
       hipLaunchKernelGGL(k_proft<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a_proft(false));
       hipLaunchKernelGGL(k_proft<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a_proft(true));
-      if (imp)
+      if (open)
+        launch_map<T, OpBc4FilterExt>(c, t_bc4ext(), OutTab<T>{{out[POM_T], out[POM_TB], out[POM_S], out[POM_SB]}});
+      else if (imp)
         launch_map<T, OpImpBc4Filter>(c, t_ibc4(), OutTab<T>{{out[POM_T], out[POM_TB], out[POM_S], out[POM_SB]}});
       else
         launch_map<T, OpBc4Filter>(c, t_bc4(), OutTab<T>{{out[POM_T], out[POM_TB], out[POM_S], out[POM_SB]}});
@@ -909,7 +966,8 @@ template <class T> struct Step {
       launch_map<T, OpAdvv>(c, t_advv(), OutTab<T>{{vf0}});
       hipLaunchKernelGGL((k_profuv<T, 0>), g2, tb, 0, c.stream, c.B, c.vert, c.prm, a_profuv(false));
       hipLaunchKernelGGL((k_profuv<T, 1>), g2, tb, 0, c.stream, c.B, c.vert, c.prm, a_profuv(true));
-      if (imp) launch_map<T, OpImpBc3>(c, t_ibc3(), OutTab<T>{{uf2, vf2}});
+      if (open) launch_map<T, OpBc3D>(c, t_bc3d(), OutTab<T>{{uf2, vf2}});
+      else if (imp) launch_map<T, OpImpBc3>(c, t_ibc3(), OutTab<T>{{uf2, vf2}});
       else launch_map<T, OpBc3>(c, t_bc3(), OutTab<T>{{uf2, vf2}});
       {
         UVFiltArgs<T> a{uf2, in[POM_UB], u1, out[POM_UB], out[POM_U], imp ? g[POM_G_DUM] : nullptr};
@@ -953,6 +1011,7 @@ template <class T> struct Adj {
   T *s3a, *s3b, *s3c, *s3d, *s2a, *s2b, *s2c, *s2d, *s2e;
   // semi-implicit barotropic step
   T *b_us = nullptr, *b_vs, *b_dus, *b_dvs, *b_raw, *b_rhs, *nlam, *arhs, *xadj;
+  T *b_us0 = nullptr, *b_vs0, *b_uc, *b_vc;   // tide policy: closed predictor / corrector outputs
 
   Adj(Step<T>& s, Ctx<T>& cc) : S(s), c(cc) {}
 
@@ -989,6 +1048,7 @@ template <class T> struct Adj {
     if (S.imp) {
       b_us = A.get<T>(N2); b_vs = A.get<T>(N2); b_dus = A.get<T>(N2); b_dvs = A.get<T>(N2);
       b_raw = A.get<T>(N2); b_rhs = A.get<T>(N2); nlam = A.get<T>(N2); arhs = A.get<T>(N2); xadj = A.get<T>(N2);
+      if (S.open) { b_us0 = A.get<T>(N2); b_vs0 = A.get<T>(N2); b_uc = A.get<T>(N2); b_vc = A.get<T>(N2); }
     }
   }
 
@@ -1055,7 +1115,7 @@ template <class T> struct Adj {
     T* b_egf30 = cur.egf;
     T* b_utf30 = cur.utf;
     T* b_vtf30 = cur.vtf;
-    if (S.internal) internal_rev(b_etf30, b_egf30, b_utf30, b_vtf30);
+    if (S.internal) internal_rev(b_etf30, b_egf30, b_utf30, b_vtf30, cur.d);
     else {
       for (int f : {POM_U, POM_UB, POM_V, POM_VB, POM_T, POM_TB, POM_S, POM_SB, POM_Q2, POM_Q2B, POM_Q2L, POM_Q2LB,
                     POM_KM, POM_KH, POM_KQ})
@@ -1191,12 +1251,22 @@ template <class T> struct Adj {
                                  cur.uaf, cur.vaf, cur.elf, b_utf0, b_vtf0};
       launch_gather<T, OpImpFilter>(c, S.t_ifilter(), ob, ib);
     }
-    {   // corrector
+    if (S.open) {   // radiation bcond 2 after the corrector, d = h + elf (assigns b_uc, b_vc)
       COutTab<T> ob{{cur.uaf, cur.vaf}};
+      T* ib[OpImpRadNew::NIN] = {b_uc, b_vc, nullptr, nullptr, bi[POM_EL], cur.elf};
+      launch_gather<T, OpImpRadNew>(c, S.t_rad(S.uac, S.vac, S.in[POM_UAF], S.in[POM_VAF], S.out[POM_ELF]), ob, ib,
+                                    (1u << OpImpRadNew::UI) | (1u << OpImpRadNew::VI));
+    }
+    {   // corrector
+      COutTab<T> ob{{S.open ? b_uc : cur.uaf, S.open ? b_vc : cur.vaf}};
       T* ib[OpImpCorr::NIN] = {b_us, b_vs, b_dus, b_dvs, cur.elf};
       launch_gather<T, OpImpCorr>(c, S.t_corr(), ob, ib);
     }
-    {   // post-solve surface boundary
+    if (S.open) {   // post-solve surface boundary with the tide at the nodes
+      COutTab<T> ob{{cur.elf}};
+      T* ib[OpImpSurfTide::NIN] = {b_raw};
+      launch_gather<T, OpImpSurfTide>(c, S.t_surf_tide(), ob, ib);
+    } else {   // post-solve surface boundary
       COutTab<T> ob{{cur.elf}};
       T* ib[OpImpSurf::NIN] = {b_raw};
       launch_gather<T, OpImpSurf>(c, S.t_surf(), ob, ib);
@@ -1217,8 +1287,14 @@ template <class T> struct Adj {
       T* ib[OpImpRhs::NIN] = {b_us, b_vs, bi[POM_D], bi[POM_ELB]};
       launch_gather<T, OpImpRhs>(c, S.t_rhs(), ob, ib);
     }
+    if (S.open) {   // radiation bcond 2 after the predictor (assigns b_us0, b_vs0)
+      COutTab<T> ob{{b_us, b_vs}};
+      T* ib[OpImpRad::NIN] = {b_us0, b_vs0, bi[POM_UA], bi[POM_VA], bi[POM_EL], bi[POM_D]};
+      launch_gather<T, OpImpRad>(c, S.t_rad(S.us, S.vs, S.in[POM_UA], S.in[POM_VA], S.in[POM_D]), ob, ib,
+                                 (1u << OpImpRad::UI) | (1u << OpImpRad::VI));
+    }
     {   // predictor
-      COutTab<T> ob{{b_us, b_vs, b_dus, b_dvs}};
+      COutTab<T> ob{{S.open ? b_us0 : b_us, S.open ? b_vs0 : b_vs, b_dus, b_dvs}};
       T* ib[OpImpPred::NIN] = {bi[POM_UA], bi[POM_VA], bi[POM_EL], bi[POM_ELB], bi[POM_D], bi[POM_UAB], bi[POM_VAB],
                                bo[POM_ADX2D], bo[POM_ADY2D], b_advua0, b_advva0, bo[POM_DRX2D], bo[POM_DRY2D],
                                bi[POM_WUBOT], bi[POM_WVBOT]};
@@ -1229,7 +1305,7 @@ template <class T> struct Adj {
     add(b_advva0, cur.advva, N2);
   }
 
-  void internal_rev(T* b_etf30, T* b_egf30, T* b_utf30, T* b_vtf30) {
+  void internal_rev(T* b_etf30, T* b_egf30, T* b_utf30, T* b_vtf30, T* b_d30) {
     const dim3 g2(c.nblk2), tb(TPB);
     Step<T>& s = S;
     // u/v Asselin filter and shift
@@ -1239,7 +1315,11 @@ template <class T> struct Adj {
       UVFiltAdjArgs<T> bv{bo[POM_VB], bo[POM_V], b_vf2, bi[POM_VB], b_v1, s.imp ? s.g[POM_G_DVM] : nullptr};
       hipLaunchKernelGGL(k_uvfilt_adj<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, bv);
     }
-    if (s.imp) {   // closed-basin bcond 3
+    if (s.open) {   // open bcond 3, edge weight from d
+      COutTab<T> ob{{b_uf2, b_vf2}};
+      T* ib[OpBc3D::NIN] = {b_uf1, b_vf1, b_u1, b_v1, b_d30};
+      launch_gather<T, OpBc3D>(c, s.t_bc3d(), ob, ib);
+    } else if (s.imp) {   // closed-basin bcond 3
       COutTab<T> ob{{b_uf2, b_vf2}};
       T* ib[OpImpBc3::NIN] = {b_uf1, b_vf1};
       launch_gather<T, OpImpBc3>(c, s.t_ibc3(), ob, ib);
@@ -1281,7 +1361,12 @@ template <class T> struct Adj {
       T* ib[OpDens::NIN] = {bo[POM_T], bo[POM_S], b_rho1};
       launch_gather<T, OpDens>(c, s.t_dens(), ob, ib);
     }
-    if (s.imp) {   // closed-basin bcond 4 + T/S filter + tracer mask
+    if (s.open) {   // open bcond 4 with exterior values + T/S filter
+      COutTab<T> ob{{bo[POM_T], bo[POM_TB], bo[POM_S], bo[POM_SB]}};
+      T* ib[OpBc4FilterExt::NIN] = {b_tf1, b_sf1, bi[POM_T], bi[POM_S], bi[POM_TB], bi[POM_SB], b_u1, b_v1,
+                                    bo[POM_W], nullptr, nullptr, bi[POM_DT]};
+      launch_gather<T, OpBc4FilterExt>(c, s.t_bc4ext(), ob, ib);
+    } else if (s.imp) {   // closed-basin bcond 4 + T/S filter + tracer mask
       COutTab<T> ob{{bo[POM_T], bo[POM_TB], bo[POM_S], bo[POM_SB]}};
       T* ib[OpImpBc4Filter::NIN] = {b_tf1, b_sf1, bi[POM_T], bi[POM_S], bi[POM_TB], bi[POM_SB]};
       launch_gather<T, OpImpBc4Filter>(c, s.t_ibc4(), ob, ib);
@@ -1308,7 +1393,7 @@ template <class T> struct Adj {
                              bo[POM_AAM], bi[POM_DT], bi[POM_ETB], b_etf30};
       launch_gather<T, OpAdvt1>(c, s.t_advt1(sal == 1), ob, ib);
     }
-    if (s.imp) {   // closed-basin bcond 6 + q2 filter
+    if (s.imp && !s.open) {   // closed-basin bcond 6 + q2 filter
       COutTab<T> ob{{bo[POM_Q2], bo[POM_Q2B], bo[POM_Q2L], bo[POM_Q2LB]}};
       T* ib[OpImpBc6Filter::NIN] = {b_q2f1, b_q2lf1, bi[POM_Q2], bi[POM_Q2L], bi[POM_Q2B], bi[POM_Q2LB]};
       launch_gather<T, OpImpBc6Filter>(c, s.t_ibc6(), ob, ib);
@@ -1346,7 +1431,7 @@ template <class T> struct Adj {
     }
     {   // vertvl (w input is not differentiated)
       VertvlAdjArgs<T> a{s.u1, s.v1, s.in[POM_DT], s.g[POM_G_DX], s.g[POM_G_DY], s.g[POM_G_FSM], bo[POM_W], s3a,
-                         b_u1, b_v1, bi[POM_DT], b_etf30, bi[POM_ETB]};
+                         b_u1, b_v1, bi[POM_DT], b_etf30, bi[POM_ETB], s.imp && !s.open};
       hipLaunchKernelGGL(k_vertvl_adj1<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a);
       hipLaunchKernelGGL(k_vertvl_adj2<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a);
     }
@@ -1381,6 +1466,8 @@ template <class T> Ctx<T> make_ctx(const pom_batch* b, const pom_params_t* p, co
 
 int check_params(const pom_params_t* p, int core) {
   if (!p || p->core != core) return POM_E_ARG;
+  if (p->boundary != POM_BOUNDARY_CLOSED && p->boundary != POM_BOUNDARY_TIDE) return POM_E_ARG;
+  if (p->boundary == POM_BOUNDARY_TIDE && core != POM_CORE_IMPLICIT) return POM_E_UNSUPPORTED;
   if (p->isplit < 3 || p->isplit > MAXSPLIT || p->ispadv < 1) return POM_E_ARG;
   if (!((p->nbct == 1 || p->nbct == 3) && (p->nbcs == 1 || p->nbcs == 3))) return POM_E_UNSUPPORTED;
   if (core == POM_CORE_IMPLICIT) {
@@ -1497,6 +1584,8 @@ template <class T> void setup_pcg(Pcg<T>& w, const pom_batch* b, const pom_param
 // attach the semi-implicit settings and cache to a step
 template <class T> void setup_implicit(Step<T>& S, const pom_params_t* prm, const void* cache) {
   S.imp = prm->core == POM_CORE_IMPLICIT;
+  S.open = S.imp && prm->boundary == POM_BOUNDARY_TIDE;
+  S.tide_wt = prm->tide_wt;
   S.hp = prm;
   if (S.imp) setup_pcg(S.pw, S.b, prm, cache);
 }
@@ -1538,7 +1627,7 @@ int run_step(const pom_batch* b, const pom_params_t* prm, const void* ramp, cons
   }
   for (int f = 0; f < POM_NGEOM; ++f) {
     S.g[f] = static_cast<const T*>(geom[f]);
-    if (!S.g[f]) return POM_E_ARG;
+    if (!S.g[f] && (f < POM_NGEOM_BASE || S.open)) return POM_E_ARG;
   }
   Arena A{static_cast<char*>(ws), 0, wsb, false};
   S.plan(A, false);
@@ -1563,7 +1652,7 @@ int run_adj(const pom_batch* b, const pom_params_t* prm, const void* ramp, const
   }
   for (int f = 0; f < POM_NGEOM; ++f) {
     S.g[f] = static_cast<const T*>(geom[f]);
-    if (!S.g[f]) return POM_E_ARG;
+    if (!S.g[f] && (f < POM_NGEOM_BASE || S.open)) return POM_E_ARG;
   }
   Arena A{static_cast<char*>(ws), 0, wsb, false};
   S.plan(A, true);
@@ -1627,12 +1716,13 @@ int run_baropg(const pom_batch* b, const pom_params_t* prm, const void* ramp, co
 // _ImplicitEtaSolveFunction), on the same kernels as the step.
 template <class T> struct EtaWs {
   Pcg<T> w;
-  T *a, *x, *nlam;
+  T *a, *x, *nlam, *lift;
   void plan(Arena& A, const pom_batch* b) {
     w.plan(A, b->n2, b->nblk2, b->M);
     a = A.get<T>(b->n2);
     x = A.get<T>(b->n2);
     nlam = A.get<T>(b->n2);
+    lift = A.get<T>(b->n2);
   }
 };
 
@@ -1643,10 +1733,11 @@ template <class T> size_t eta_ws_size(const pom_batch* b) {
   return A.off + 256;
 }
 
-template <class T> EllGeom<T> ell_geom(const Ctx<T>& c, const void* const* geom) {
+template <class T> EllGeom<T> ell_geom(const Ctx<T>& c, const void* const* geom, const pom_params_t* prm) {
   auto G = [&](int f) { return static_cast<const T*>(geom[f]); };
   return EllGeom<T>{G(POM_G_DX), G(POM_G_DY), G(POM_G_ART), G(POM_G_DUM), G(POM_G_DVM),
-                    4.0 * (c.prm.dti * c.prm.dti) * c.prm.grav * c.prm.alpha};
+                    4.0 * (c.prm.dti * c.prm.dti) * c.prm.grav * c.prm.alpha,
+                    prm->boundary == POM_BOUNDARY_TIDE ? G(POM_G_ETA_FIXED) : nullptr};
 }
 
 template <class T> FieldTab<T> ell_tab(const void* const* geom, const void* eta, const void* d) {
@@ -1688,9 +1779,15 @@ int run_eta_solve(const pom_batch* b, const pom_params_t* prm, const void* cache
   e.plan(A, b);
   if (A.off > wsb) return POM_E_ARG;
   setup_pcg(e.w, b, prm, cache);
+  const bool tide = prm->boundary == POM_BOUNDARY_TIDE;
+  if (tide)
+    hipLaunchKernelGGL(k_tide_lift<T>, dim3(c.nblk2), dim3(TPB), 0, st, c.B,
+                       static_cast<const T*>(geom[POM_G_TIDE_NODE]), static_cast<const T*>(geom[POM_G_TIDE_AMP]),
+                       static_cast<const T*>(geom[POM_G_TIDE_PHASE]), prm->tide_wt, e.lift);
   hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(256), 0, st, e.w.tol, (int64_t)b->M, prm->eta_tol);
-  e.w.solve(c, ell_geom<T>(c, geom), static_cast<const T*>(rhs), static_cast<const T*>(d),
-            static_cast<const T*>(x0), static_cast<T*>(x), prm->eta_rel_tol, prm->eta_max_iter);
+  e.w.solve(c, ell_geom<T>(c, geom, prm), static_cast<const T*>(rhs), static_cast<const T*>(d),
+            static_cast<const T*>(x0), static_cast<T*>(x), prm->eta_rel_tol, prm->eta_max_iter,
+            tide ? e.lift : nullptr);
   if (iters) hipLaunchKernelGGL(k_copy<int32_t>, dim3(1), dim3(256), 0, st, iters, e.w.its, (int64_t)b->M);
   return hipGetLastError() == hipSuccess ? POM_OK : POM_E_HIP;
 }
@@ -1713,7 +1810,7 @@ int run_eta_solve_adj(const pom_batch* b, const pom_params_t* prm, const void* c
   hipLaunchKernelGGL(k_eta_adj_in<T>, g2, tb, 0, st, c.B, static_cast<const T*>(gx), art, e.a, e.w.part);
   hipLaunchKernelGGL(k_eta_adj_scale, gm, tb, 0, st, c.B, e.w.part, e.w.sc, e.w.tol, prm->eta_tol);
   hipLaunchKernelGGL(k_eta_adj_norm<T>, g2, tb, 0, st, c.B, e.a, e.w.sc);
-  e.w.solve(c, ell_geom<T>(c, geom), e.a, static_cast<const T*>(d), nullptr, e.x, prm->eta_rel_tol,
+  e.w.solve(c, ell_geom<T>(c, geom, prm), e.a, static_cast<const T*>(d), nullptr, e.x, prm->eta_rel_tol,
             prm->eta_max_iter);
   hipLaunchKernelGGL(k_eta_adj_out<T>, g2, tb, 0, st, c.B, e.x, art, e.w.sc, static_cast<T*>(b_rhs), e.nlam);
   if (iters) hipLaunchKernelGGL(k_copy<int32_t>, dim3(1), dim3(256), 0, st, iters, e.w.its, (int64_t)b->M);
@@ -1730,6 +1827,13 @@ bool ell_geom_ok(const void* const* geom) {
   for (int f : {POM_G_DX, POM_G_DY, POM_G_ART, POM_G_DUM, POM_G_DVM})
     if (!geom[f]) return false;
   return true;
+}
+
+// the tide policy's fixed set (and, for the forward solve, its nodes)
+bool tide_geom_ok(const pom_params_t* prm, const void* const* geom, bool nodes) {
+  if (prm->boundary != POM_BOUNDARY_TIDE) return true;
+  if (!geom[POM_G_ETA_FIXED]) return false;
+  return !nodes || (geom[POM_G_TIDE_NODE] && geom[POM_G_TIDE_AMP] && geom[POM_G_TIDE_PHASE]);
 }
 
 }  // namespace
@@ -1907,6 +2011,7 @@ int pom_eta_solve(const pom_batch* b, const pom_params_t* prm, const void* cache
                   size_t ws_bytes, void* stream) {
   if (!b || !ell_geom_ok(geom) || !rhs || !d || !x || !workspace) return POM_E_ARG;
   if (int e = check_params(prm, POM_CORE_IMPLICIT)) return e;
+  if (!tide_geom_ok(prm, geom, true)) return POM_E_ARG;
   if (prm->eta_precond == POM_PRECOND_SPECTRAL && !cache) return POM_E_ARG;
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (b->dtype == POM_F32)
@@ -1919,6 +2024,7 @@ int pom_eta_solve_adj(const pom_batch* b, const pom_params_t* prm, const void* c
                       int32_t* iters_out, void* workspace, size_t ws_bytes, void* stream) {
   if (!b || !ell_geom_ok(geom) || !x || !d || !cot_x || !grad_rhs || !workspace) return POM_E_ARG;
   if (int e = check_params(prm, POM_CORE_IMPLICIT)) return e;
+  if (!tide_geom_ok(prm, geom, false)) return POM_E_ARG;
   if (prm->eta_precond == POM_PRECOND_SPECTRAL && !cache) return POM_E_ARG;
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (b->dtype == POM_F32)

@@ -117,14 +117,14 @@ class Ensemble:
 
     # ------------------------------------------------------------ buffers
     def numel(self, name):
-        return self.n3 if (name in L.FIELDS_3D or name in L.GEOM_3D) else self.n2
+        return self.n3 if name in L.THREE_D else self.n2
 
     def zeros(self, name):
         return torch.zeros(self.numel(name), dtype=self.dtype, device=self.device)
 
     def view(self, flat, m, name):
         im, jm = self.shapes[m]
-        if name in L.FIELDS_3D or name in L.GEOM_3D:
+        if name in L.THREE_D:
             return flat[self.off3[m]:self.off3[m] + im * jm * KB].view(im, jm, KB)
         return flat[self.off2[m]:self.off2[m] + im * jm].view(im, jm)
 
@@ -164,7 +164,7 @@ class Ensemble:
         rc = self.lib.pom_explicit_step(
             self._h, C.byref(prm), ramp.data_ptr(), vgrid.data_ptr(),
             L.ptr_array([t.data_ptr() for t in ins]), L.ptr_array([out[n].data_ptr() for n in L.FIELDS]),
-            L.ptr_array([t.data_ptr() for t in gs]), int(bool(internal)), vamax.data_ptr(),
+            L.geom_ptrs(geom), int(bool(internal)), vamax.data_ptr(),
             ws.data_ptr(), nb, stream)
         L.check(rc, "pom_explicit_step")
         return out, vamax
@@ -173,7 +173,6 @@ class Ensemble:
         """Gradients of the step inputs for output cotangents ``cot`` (dict, may omit fields)."""
         prm = sc.struct()
         ins = [state[n] for n in L.FIELDS]
-        gs = [geom[n] for n in L.GEOM]
         self._check(ins, L.FIELDS, self.numel)
         # one zero-filled allocation, split into per-field views
         sizes = [self.numel(n) for n in L.DIFF_INPUTS]
@@ -193,7 +192,7 @@ class Ensemble:
         stream = torch.cuda.current_stream(self.device).cuda_stream
         rc = self.lib.pom_explicit_step_adj(
             self._h, C.byref(prm), ramp.data_ptr(), vgrid.data_ptr(),
-            L.ptr_array([t.data_ptr() for t in ins]), L.ptr_array([t.data_ptr() for t in gs]),
+            L.ptr_array([t.data_ptr() for t in ins]), L.geom_ptrs(geom),
             int(bool(internal)), L.ptr_array(cot_ptrs), L.ptr_array(grad_ptrs), ws.data_ptr(), nb, stream)
         L.check(rc, "pom_explicit_step_adj")
         return grads

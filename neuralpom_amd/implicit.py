@@ -34,6 +34,8 @@ class ImplicitScalars(StepScalars):
     eta_max_iter: int = 200
     eta_preconditioner: str = "spectral"
     barotropic_shapiro: float = 0.0
+    boundary: int = L.POM_BOUNDARY_CLOSED   # or POM_BOUNDARY_TIDE (open edges, tide nodes)
+    tide_wt: float = 0.0                    # (speed pi / 180) t / 3600 of the tide this step
 
     def struct(self) -> L.Params:
         p = super().struct()
@@ -46,6 +48,8 @@ class ImplicitScalars(StepScalars):
             raise ValueError(f"eta preconditioner must be 'spectral' or 'jacobi', got {self.eta_preconditioner!r}")
         p.eta_precond = L.POM_PRECOND_SPECTRAL if pre == "spectral" else L.POM_PRECOND_JACOBI
         p.gamma_baro = float(self.barotropic_shapiro)
+        p.boundary = int(self.boundary)
+        p.tide_wt = float(self.tide_wt)
         return p
 
 
@@ -62,10 +66,8 @@ class ImplicitEnsemble(Ensemble):
 
     def prepare(self, geom, sc: ImplicitScalars):
         prm = sc.struct()
-        gs = [geom[n] for n in L.GEOM]
-        self._check(gs, L.GEOM, self.numel)
         stream = torch.cuda.current_stream(self.device).cuda_stream
-        L.check(self.lib.pom_implicit_prepare(self._h, C.byref(prm), L.ptr_array([t.data_ptr() for t in gs]),
+        L.check(self.lib.pom_implicit_prepare(self._h, C.byref(prm), self._geom_ptrs(geom),
                                               self.cache.data_ptr(), self.cache.numel(), stream),
                 "pom_implicit_prepare")
         self._prepared = True
@@ -88,9 +90,8 @@ class ImplicitEnsemble(Ensemble):
             raise RuntimeError("ImplicitEnsemble.prepare(geom, scalars) must run before the first step")
         prm = sc.struct()
         ins = [state[n] for n in L.FIELDS]
-        gs = [geom[n] for n in L.GEOM]
         self._check(ins, L.FIELDS, self.numel)
-        self._check(gs, L.GEOM, self.numel)
+        gp = self._geom_ptrs(geom)
         out = {n: torch.empty_like(state[n]) for n in L.FIELDS}
         vamax = torch.zeros(1, dtype=self.dtype, device=self.device)
         ws, nb = self._ws_implicit(prm, False)
@@ -98,7 +99,7 @@ class ImplicitEnsemble(Ensemble):
         rc = self.lib.pom_implicit_step(
             self._h, C.byref(prm), self.cache.data_ptr(), ramp.data_ptr(), vgrid.data_ptr(),
             L.ptr_array([t.data_ptr() for t in ins]), L.ptr_array([out[n].data_ptr() for n in L.FIELDS]),
-            L.ptr_array([t.data_ptr() for t in gs]), int(bool(internal)), vamax.data_ptr(),
+            gp, int(bool(internal)), vamax.data_ptr(),
             iters.data_ptr() if iters is not None else None, ws.data_ptr(), nb, stream)
         L.check(rc, "pom_implicit_step")
         return out, vamax
@@ -109,8 +110,8 @@ class ImplicitEnsemble(Ensemble):
             raise RuntimeError("ImplicitEnsemble.prepare(geom, scalars) must run before the first step")
         prm = sc.struct()
         ins = [state[n] for n in L.FIELDS]
-        gs = [geom[n] for n in L.GEOM]
         self._check(ins, L.FIELDS, self.numel)
+        gp = self._geom_ptrs(geom)
         sizes = [self.numel(n) for n in L.DIFF_INPUTS]
         flat = torch.zeros(sum(sizes), dtype=self.dtype, device=self.device)
         grads = dict(zip(L.DIFF_INPUTS, torch.split(flat, sizes)))
@@ -128,7 +129,7 @@ class ImplicitEnsemble(Ensemble):
         stream = torch.cuda.current_stream(self.device).cuda_stream
         rc = self.lib.pom_implicit_step_adj(
             self._h, C.byref(prm), self.cache.data_ptr(), ramp.data_ptr(), vgrid.data_ptr(),
-            L.ptr_array([t.data_ptr() for t in ins]), L.ptr_array([t.data_ptr() for t in gs]),
+            L.ptr_array([t.data_ptr() for t in ins]), gp,
             int(bool(internal)), L.ptr_array(cot_ptrs), L.ptr_array(grad_ptrs), ws.data_ptr(), nb, stream)
         L.check(rc, "pom_implicit_step_adj")
         return grads
@@ -145,9 +146,9 @@ class ImplicitEnsemble(Ensemble):
         return ws, n.value
 
     def _geom_ptrs(self, geom):
-        gs = [geom[n] for n in L.GEOM]
-        self._check(gs, L.GEOM, self.numel)
-        return L.ptr_array([t.data_ptr() for t in gs])
+        names = [n for n in L.GEOM_ALL if n in L.GEOM or geom.get(n) is not None]
+        self._check([geom[n] for n in names], names, self.numel)
+        return L.geom_ptrs(geom)
 
     def _flat2(self, x, name):
         x = x.detach().to(self.device, self.dtype).contiguous().reshape(-1)

@@ -12,18 +12,23 @@ Mirrors ``neuralpom.cores.implicit_p0_fixed.POM2K_Core``:
 and the construction protocol of src/neuralpom/hybrid/core_pool.py:30-48.
 
 The closed-basin boundary, double-gyre forcing and double-gyre numerics
-policies (case_policy.py:14-60) are what the HIP step implements; a core
-whose policy objects, eta solver or elliptic operator were replaced (case
-adapters) cannot run the fused step, and ``run_time_loop`` raises instead of
-silently ignoring the replacement.  There is no CPU path.
+policies (case_policy.py:14-60) are what the HIP step implements, and, in
+place of the closed basin, the open-boundary tide policy of the case adapter
+(case_adapters/coastal_boundary.py:9-67) installed with
+``neuralpom_amd.boundary.install``.  A core whose policy objects, eta solver or
+elliptic operator were replaced otherwise cannot run the fused step, and
+``run_time_loop`` raises instead of silently ignoring the replacement.  There
+is no CPU path.
 """
 from __future__ import annotations
 
 import os
+from dataclasses import replace
 
 import torch
 
 from . import _lib as L
+from .boundary import TideBoundary
 from .core import POM2K_Core as _ExplicitCore, _STEP_SCALARS
 from .implicit import Elliptic, EtaSolve, ImplicitEnsemble, ImplicitScalars, step_autograd
 
@@ -94,20 +99,45 @@ class POM2K_Core(_ExplicitCore):
                   eta_max_iter=int(getattr(self, "eta_solver_max_iter", 200) if max_iter is None else max_iter),
                   eta_preconditioner=self.eta_solver_preconditioner,
                   barotropic_shapiro=float(self.barotropic_shapiro))
+        tide = self._tide_policy()
+        if tide is not None:
+            kw.update(boundary=L.POM_BOUNDARY_TIDE, tide_wt=tide.wt(tide.seconds(self)))
         sc = ImplicitScalars(**kw)
         if abs(sc.dti - self.dti) > 1e-9 * abs(self.dti):
             raise ValueError(f"dti {self.dti} != dte * isplit {sc.dti}")
         return sc
 
+    def _tide_policy(self):
+        pol = getattr(self, "_implicit_boundary_policy", None)
+        return pol if isinstance(pol, TideBoundary) else None
+
     def _geom(self):
-        return {n: self._flat(n).detach() for n in L.GEOM}
+        geom = {n: self._flat(n).detach() for n in L.GEOM}
+        tide = self._tide_policy()
+        if tide is not None:
+            acc = self._accel()
+            src = tuple((t.data_ptr(), t._version) for t in
+                        (self._tide_i, self._tide_j, self._tide_amp, self._tide_phase, self._fixed, self.tbe,
+                         self.tbw, self.tbn, self.tbs, self.sbe, self.sbw, self.sbn, self.sbs))
+            key = (src, tide.speed, tide.pi, self.dtype)
+            if acc.get("tide_key") != key:
+                acc["tide_geom"] = {n: self._as_flat(v).detach() for n, v in tide.geometry(self).items()}
+                acc["tide_key"] = key
+            geom.update(acc["tide_geom"])
+        return geom
+
+    def _scalars_at_step(self, sc):
+        """The tide's time argument at this step's self.time (coastal_boundary.py:36, 45)."""
+        tide = self._tide_policy()
+        return sc if tide is None else replace(sc, tide_wt=tide.wt(tide.seconds(self)))
 
     # ------------------------------------------------------------ time loop
     def _before_time_loop(self) -> None:
         for name in _POLICY_ATTRS:
-            if getattr(self, name, None) is not None:
-                raise NotImplementedError(f"{name} replaced: the HIP step implements the closed-basin double-gyre "
-                                          "policies only")
+            pol = getattr(self, name, None)
+            if pol is not None and not (name == "_implicit_boundary_policy" and isinstance(pol, TideBoundary)):
+                raise NotImplementedError(f"{name} replaced: the HIP step implements the closed-basin (or tide) "
+                                          "boundary and the double-gyre forcing and numerics policies only")
         for name in ("solve_implicit_eta_cg", "apply_elliptic_operator"):
             if name in vars(self) or getattr(type(self), name) is not getattr(POM2K_Core, name):
                 raise NotImplementedError(f"{name} is overridden: the fused step cannot call a replaced operator")

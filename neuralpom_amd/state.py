@@ -126,7 +126,7 @@ def restrict4(fine, coarse, field, name, fine_geom, fine_el=None, fine_vgrid=Non
     """Conservative 4 x 4 restriction of the fine batch's ``field`` (flat, the
     layout of field ``name``) onto the coarse batch.  Weights art * mask
     [* (h + el) * dz]; mask is dum for u / ua / ub / uab, dvm for v, fsm else."""
-    three_d = name in L.FIELDS_3D or name in L.GEOM_3D
+    three_d = name in L.THREE_D
     mask = {"u": "dum", "ub": "dum", "ua": "dum", "uab": "dum",
             "v": "dvm", "vb": "dvm", "va": "dvm", "vab": "dvm"}.get(name, "fsm")
     out = torch.empty(coarse.numel(name), dtype=coarse.dtype, device=coarse.device)
