@@ -9,7 +9,8 @@ The semi-implicit core runs one of two policies inside its HIP step:
   family), installed with :func:`install` exactly as the adapter's ``install``
   is: tide nodes read from an ``i j amplitude phase`` table (1-based indices,
   phase in degrees, at most 99 rows), all four radiation coefficients set to
-  1, and the eta solve holding the edges, dry cells and nodes fixed.
+  1 (or to ``radiation``), and the eta solve holding the edges, dry cells and
+  nodes fixed.
 
 Where the adapter replaces ``bcond`` and ``solve_implicit_eta_cg`` with
 Python closures, here the policy object is a description the core hands to
@@ -91,12 +92,14 @@ def fixed_set(fsm, i, j):
     return fixed
 
 
-def install(core, family: str, path=None, nodes=None, speed: float = TIDE_SPEED, pi: float = TIDE_PI):
+def install(core, family: str, path=None, nodes=None, speed: float = TIDE_SPEED, pi: float = TIDE_PI,
+            radiation: float = 1.0):
     """Install the tide policy on a semi-implicit core (coastal_boundary.py:60-67).
 
     ``path``: text table ``i j amplitude phase`` (1-based, at most 99 rows are
     read, as the adapter's ``np.loadtxt(path, max_rows=99)``); or ``nodes``, an
-    (n, 4) array of the same columns.  Only the implicit family runs on the
+    (n, 4) array of the same columns.  ``radiation`` is the value given to
+    rfe rfw rfn rfs (the adapter's: 1).  Only the implicit family runs on the
     HIP step."""
     if family != "implicit":
         raise NotImplementedError("the tide policy runs on the semi-implicit core only (family 'implicit')")
@@ -115,7 +118,7 @@ def install(core, family: str, path=None, nodes=None, speed: float = TIDE_SPEED,
     core._tide_i, core._tide_j = i, j
     core._tide_amp = torch.as_tensor(data[:, 2], dtype=core.el.dtype, device=core.device)
     core._tide_phase = torch.as_tensor(data[:, 3], dtype=core.el.dtype, device=core.device)
-    core.rfe = core.rfw = core.rfn = core.rfs = 1.0
+    core.rfe = core.rfw = core.rfn = core.rfs = float(radiation)
     core._fixed = fixed_set(core.fsm, i, j)
     core._implicit_boundary_policy = TideBoundary(speed, pi)
     return core

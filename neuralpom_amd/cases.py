@@ -219,3 +219,74 @@ def seamount(im, jm, seed=0, dx=8000.0, H=4500.0, L=25.0e3, n2=1.0e-5, noise=0.1
     period = _initialize(st, g, vg, im, jm, float(g["cor"][im // 2, jm // 2]))
     g.pop("hmax")
     return st, g, torch.stack(vg), period
+
+
+def gaussian_random_field(im, jm, corr, seed, dtype=torch.float32, device="cpu"):
+    """Seeded stationary Gaussian random field with correlation exp(-r^2 / (2 corr^2))
+    (r in cells): white noise filtered by a Gaussian of width corr / sqrt(2), by FFT
+    on a periodic domain padded by 4 corr so the ends do not wrap."""
+    pad = int(math.ceil(4 * corr))
+    ni, nj = im + 2 * pad, jm + 2 * pad
+    gen = torch.Generator(device="cpu").manual_seed(int(seed))
+    w = torch.randn((ni, nj), generator=gen, dtype=torch.float64)
+    sig = corr / math.sqrt(2.0)
+    ki = torch.fft.fftfreq(ni, dtype=torch.float64).view(-1, 1)
+    kj = torch.fft.fftfreq(nj, dtype=torch.float64).view(1, -1)
+    filt = torch.exp(-2.0 * (math.pi * sig) ** 2 * (ki ** 2 + kj ** 2))
+    f = torch.fft.ifft2(torch.fft.fft2(w) * filt).real[pad:pad + im, pad:pad + jm]
+    return f.to(device, dtype)
+
+
+def tidal_channel(im=354, jm=322, seed=0, dx=500.0, hmin=10.0, hmax=60.0, corr=20.0, amp=1.0, phase=90.0,
+                  t0=15.0, s0=32.0, dtdz=0.0, lat=36.0, open_east=False, dtype=torch.float32, device="cpu"):
+    """Open-boundary tidal channel of the semi-implicit core; returns (state, geom,
+    vgrid, period_days, nodes).
+
+    Land on the north, south and east edges; the west end is open.  Depth:
+    a seeded Gaussian random field (correlation length ``corr`` cells) mapped
+    to [hmin, hmax]; dx = dy; an f-plane at ``lat``; T = t0 + dtdz z (z the
+    depth in metres, negative downwards; dtdz = 0: uniform) and uniform S, the
+    exterior values of the open edges from the same profiles, the initial
+    density as the mean density; no wind.  ``nodes`` is the tide table of the open-boundary policy
+    (1-based i j, amplitude, phase in degrees): the wet edge cells of the west
+    end, amplitude ``amp``, phase ``phase`` (90: the tide rises from zero at
+    t = 0, amp sin(omega t), so the start carries no elevation jump); the
+    policy's speed for an M2-like period P hours is 360 / P degrees per hour.
+
+    Install the policy with ``radiation=0``: the elevation is clamped at the
+    west edge and the face velocity next to it is the corrector's, so the
+    barotropic transport stays consistent with the eta solve.  With the
+    adapter's radiation coefficients of 1 the corrected face velocity is
+    overwritten after the solve; the column next to an open edge then gains
+    or loses volume the continuity equation does not see, and its bottom
+    layer drifts in T (a property of the reference adapter, reproduced by the
+    HIP step)."""
+    st, g = _empty(im, jm, dtype, device)
+    omega = 7.2921e-5
+    g["dx"] = torch.full((im, jm), dx, dtype=dtype, device=device)
+    g["dy"] = torch.full((im, jm), dx, dtype=dtype, device=device)
+    g["cor"] = torch.full((im, jm), 2.0 * omega * math.sin(math.radians(lat)), dtype=dtype, device=device)
+    g["north_e"] = torch.full((im, jm), lat, dtype=dtype, device=device)
+    f = gaussian_random_field(im, jm, corr, seed, torch.float64, device)
+    h = (hmin + (f - f.min()) / (f.max() - f.min()) * (hmax - hmin)).to(dtype)
+    h[:, 0] = h[:, -1] = 1.0
+    if not open_east:
+        h[-1, :] = 1.0
+    g["h"] = h
+    vg = _uniform_sigma(dtype, device)
+    _areas_masks(g)
+    st["tb"] = t0 + dtdz * vg[0].view(1, 1, -1) * h.unsqueeze(2)
+    st["sb"].fill_(s0)
+    g["tclim"] = st["tb"].clone()
+    g["sclim"] = st["sb"].clone()
+    g["rmean"] = eos(st["tb"], st["sb"], h, vg[1], g["fsm"])
+    g["tbe"], g["tbw"] = st["tb"][-1].clone(), st["tb"][0].clone()
+    g["tbn"], g["tbs"] = st["tb"][:, -1].clone(), st["tb"][:, 0].clone()
+    for n in ("sbe", "sbw", "sbn", "sbs"):
+        g[n].fill_(s0)
+    period = _initialize(st, g, vg, im, jm, float(g["cor"][im // 2, jm // 2]))
+    js = torch.arange(1, jm - 1, dtype=torch.float64)
+    nodes = torch.stack([torch.full_like(js, 1.0), js + 1.0, torch.full_like(js, float(amp)),
+                         torch.full_like(js, float(phase))], dim=1).numpy()
+    return st, g, torch.stack(vg), period, nodes
+
