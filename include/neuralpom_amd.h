@@ -112,7 +112,9 @@ typedef struct {
 typedef struct pom_batch pom_batch;
 
 /* Member table of an ensemble.  Copies the table to the device (one small
- * allocation owned by the handle).  dtype: POM_F32 or POM_F64. */
+ * allocation owned by the handle).  dtype: POM_F32 or POM_F64.  Each member
+ * is at least 4 x 4 columns and one 3-D field of a member is below 2 GiB
+ * (im * jm * 16 * sizeof(element) < 2^31); otherwise POM_E_ARG. */
 int pom_batch_create(const pom_member_t* members, int32_t n_members, int32_t dtype, pom_batch** out);
 int pom_batch_destroy(pom_batch* b);
 

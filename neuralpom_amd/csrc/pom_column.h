@@ -56,7 +56,7 @@ template <class T> struct ColIO {
 
 template <class T>
 __device__ __forceinline__ ColIO<T> col_pos(const Batch& B, Pos& p, T* lds) {
-  const int b = blockIdx.x;
+  const int b = logical_block();
   const int m = member_of(B.blk2, B.M, b);
   const Member mm = B.mem[m];
   const int64_t n2 = (int64_t)mm.im * mm.jm;

@@ -5,14 +5,15 @@
 // so torch tensors are used in place.  An ensemble packs members back to back:
 // member m's 3-D field starts at off3[m] elements and its 2-D field at off2[m].
 //
-// Value types: P<T> (passive) and D<T> (active: value + tangent).  Every
-// stencil operator is written once as a point function over an accessor.  The
-// forward kernel instantiates it with an accessor that returns P<T>; the
-// adjoint gather instantiates it once per (input field, stencil offset) with an
-// accessor that returns D<T> only for that compile-time (field, offset) pair.
-// Activity is a property of the type, so inactive arithmetic disappears at
-// compile time and each gather term is the exact partial derivative
-// d out(q - o) / d in(q), evaluated at the saved primal state.
+// Value types: P<T> (passive), D<T> (value + one tangent) and DS<T, M> (value
+// + one tangent lane per input field in the mask M).  Every stencil operator
+// is written once as a point function over an accessor.  The forward kernel
+// instantiates it with an accessor that returns P<T>; the adjoint gather
+// instantiates it once per stencil offset o with an accessor that returns DS
+// for every input field read at exactly o.  Activity is a property of the
+// type, so inactive arithmetic disappears at compile time and each gather term
+// is the exact partial derivative d out(q - o) / d in(q), evaluated at the
+// saved primal state.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -207,17 +208,26 @@ __device__ __forceinline__ DS<T, A | B> operator*(const DS<T, A>& a, const DS<T,
   });
   return r;
 }
+// Tangent lanes of a quotient are scaled by one reciprocal of the divisor
+// (the hardware reciprocal in float, 1 ulp; an IEEE division in double)
+// instead of one division per lane.  The primal value stays the exact
+// quotient, so every primal comparison of an adjoint gather sees the values
+// the forward map computed.
+__device__ __forceinline__ float rcp_lane(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcp_lane(double x) { return 1.0 / x; }
+
 template <class T, uint32_t A, uint32_t B>
 __device__ __forceinline__ DS<T, A | B> operator/(const DS<T, A>& a, const DS<T, B>& b) {
   DS<T, A | B> r;
   const T q = a.v / b.v;
+  const T rb = rcp_lane(b.v);
   r.v = q;
   for_bits<A | B>([&](auto I) {
     constexpr int i = decltype(I)::value;
     constexpr bool ha = (A >> i) & 1u, hb = (B >> i) & 1u;
-    if constexpr (ha && hb) r.d[slot(A | B, i)] = (a.d[slot(A, i)] - q * b.d[slot(B, i)]) / b.v;
-    else if constexpr (ha) r.d[slot(A | B, i)] = a.d[slot(A, i)] / b.v;
-    else r.d[slot(A | B, i)] = -q * b.d[slot(B, i)] / b.v;
+    if constexpr (ha && hb) r.d[slot(A | B, i)] = (a.d[slot(A, i)] - q * b.d[slot(B, i)]) * rb;
+    else if constexpr (ha) r.d[slot(A | B, i)] = a.d[slot(A, i)] * rb;
+    else r.d[slot(A | B, i)] = -q * b.d[slot(B, i)] * rb;
   });
   return r;
 }
@@ -264,14 +274,16 @@ template <class T, uint32_t A> __device__ __forceinline__ DS<T, A> operator*(P<T
 template <class T, uint32_t A> __device__ __forceinline__ DS<T, A> operator/(const DS<T, A>& a, P<T> b) {
   DS<T, A> r;
   r.v = a.v / b.v;
-  for (int i = 0; i < popc(A); ++i) r.d[i] = a.d[i] / b.v;
+  const T rb = rcp_lane(b.v);
+  for (int i = 0; i < popc(A); ++i) r.d[i] = a.d[i] * rb;
   return r;
 }
 template <class T, uint32_t A> __device__ __forceinline__ DS<T, A> operator/(P<T> a, const DS<T, A>& b) {
   DS<T, A> r;
   const T q = a.v / b.v;
   r.v = q;
-  for (int i = 0; i < popc(A); ++i) r.d[i] = -q * b.d[i] / b.v;
+  const T qr = -q * rcp_lane(b.v);
+  for (int i = 0; i < popc(A); ++i) r.d[i] = qr * b.d[i];
   return r;
 }
 #define POM_DS_SCALAR_OPS(OP)                                                                        \
@@ -289,7 +301,8 @@ template <class T, uint32_t A> __device__ __forceinline__ DS<T, A> psqrt(const D
   DS<T, A> r;
   const T s = sqrt(a.v);
   r.v = s;
-  for (int i = 0; i < popc(A); ++i) r.d[i] = a.d[i] / (T(2) * s);
+  const T h = rcp_lane(T(2) * s);
+  for (int i = 0; i < popc(A); ++i) r.d[i] = a.d[i] * h;
   return r;
 }
 template <class T, uint32_t A> __device__ __forceinline__ DS<T, A> pabs(const DS<T, A>& a) {
@@ -350,6 +363,25 @@ struct Batch {
   int32_t M;
 };
 
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup x runs on
+// XCD x % 8), and each XCD has its own L2.  The logical block index gives XCD
+// k one contiguous range of blocks, so the neighbouring rows (i +- 1, i +- 2)
+// a stencil reads were mostly fetched into the same L2 by blocks running at
+// about the same time.  A bijection of [0, gridDim.x) for any grid size.
+#ifndef POM_XCD_REMAP
+#define POM_XCD_REMAP 1
+#endif
+constexpr int NXCD = 8;
+__device__ __forceinline__ int logical_block() {
+  const int x = blockIdx.x;
+  if constexpr (POM_XCD_REMAP) {
+    const int n = gridDim.x, q = n / NXCD, r = n % NXCD, k = x % NXCD;
+    return k * q + (k < r ? k : r) + x / NXCD;
+  } else {
+    return x;
+  }
+}
+
 // member owning block b (all lanes agree -> scalar)
 __device__ __forceinline__ int member_of(const int32_t* blk, int M, int b) {
   int lo = 0, hi = M;   // invariant blk[lo] <= b < blk[hi]
@@ -367,7 +399,7 @@ struct Pos {
 };
 
 __device__ __forceinline__ bool pos3(const Batch& B, Pos& p) {
-  int b = blockIdx.x;
+  int b = logical_block();
   int m = member_of(B.blk3, B.M, b);
   Member mm = B.mem[m];
   int64_t local = (int64_t)(b - B.blk3[m]) * TPB + threadIdx.x;
@@ -382,7 +414,7 @@ __device__ __forceinline__ bool pos3(const Batch& B, Pos& p) {
 }
 
 __device__ __forceinline__ bool pos2(const Batch& B, Pos& p) {
-  int b = blockIdx.x;
+  int b = logical_block();
   int m = member_of(B.blk2, B.M, b);
   Member mm = B.mem[m];
   int64_t local = (int64_t)(b - B.blk2[m]) * TPB + threadIdx.x;
@@ -406,6 +438,21 @@ __device__ __forceinline__ int64_t ix3(const Pos& p, int i, int j, int k) {
 }
 __device__ __forceinline__ int64_t ix2(const Pos& p, int i, int j) {
   return p.o2 + (int64_t)i * p.jm + j;
+}
+
+// Element idx of a member's field, given the member's base pointer.  The
+// base is uniform over a block (a block never straddles members) and the
+// byte offset is formed in 32 bits (a member's field is below 2 GiB, checked
+// when the batch is created), so the load takes a scalar 64-bit base and a
+// 32-bit lane offset instead of per-lane 64-bit address arithmetic.
+template <class T> __device__ __forceinline__ T ld_local(const T* base, uint32_t idx) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (idx * (uint32_t)sizeof(T)));
+}
+__device__ __forceinline__ uint32_t lix3(const Pos& p, int i, int j, int k) {
+  return ((uint32_t)i * (uint32_t)p.jm + (uint32_t)j) * (uint32_t)KB + (uint32_t)k;
+}
+__device__ __forceinline__ uint32_t lix2(const Pos& p, int i, int j) {
+  return (uint32_t)i * (uint32_t)p.jm + (uint32_t)j;
 }
 
 // ------------------------------------------------------------------ params
@@ -460,19 +507,19 @@ struct Acc {
 
   template <int F, int DI, int DJ, int DK>
   __device__ __forceinline__ auto at() const {
-    T x = tab->f[F][ix3(p, ci(p.i + DI), cj(p.j + DJ), ck(p.k + DK))];
+    T x = ld_local(tab->f[F] + p.o3, lix3(p, ci(p.i + DI), cj(p.j + DJ), ck(p.k + DK)));
     if constexpr (F == SF && DI == SI && DJ == SJ && DK == SK) return D<T>{x, T(1)};
     else return P<T>{x};
   }
   template <int F, int DI, int DJ>
   __device__ __forceinline__ auto at2() const {
-    T x = tab->f[F][ix2(p, ci(p.i + DI), cj(p.j + DJ))];
+    T x = ld_local(tab->f[F] + p.o2, lix2(p, ci(p.i + DI), cj(p.j + DJ)));
     if constexpr (F == SF && DI == SI && DJ == SJ) return D<T>{x, T(1)};
     else return P<T>{x};
   }
   template <int G, int DI, int DJ>
   __device__ __forceinline__ P<T> g() const {
-    return P<T>{tab->g[G][ix2(p, ci(p.i + DI), cj(p.j + DJ))]};
+    return P<T>{ld_local(tab->g[G] + p.o2, lix2(p, ci(p.i + DI), cj(p.j + DJ)))};
   }
   __device__ __forceinline__ T z(int k) const { return vert.z(p.m, k); }
   __device__ __forceinline__ T zz(int k) const { return vert.zz(p.m, k); }
@@ -538,19 +585,19 @@ struct AccT {
   template <int F, int DI, int DJ, int DK>
   __device__ __forceinline__ auto at() const {
     static_assert(DK - OK >= -2 && DK - OK <= 2, "stencil beyond table");
-    T x = tab->f[F][p.o3 + (int64_t)(loc<DI, DJ>() * KB + nb->ck[DK - OK + 2])];
+    T x = ld_local(tab->f[F] + p.o3, (uint32_t)(loc<DI, DJ>() * KB + nb->ck[DK - OK + 2]));
     if constexpr (((MASK >> F) & 1u) && DI == OI && DJ == OJ && DK == OK) return DS<T, (1u << F)>{x, {T(1)}};
     else return P<T>{x};
   }
   template <int F, int DI, int DJ>
   __device__ __forceinline__ auto at2() const {
-    T x = tab->f[F][p.o2 + loc<DI, DJ>()];
+    T x = ld_local(tab->f[F] + p.o2, (uint32_t)loc<DI, DJ>());
     if constexpr (((MASK >> F) & 1u) && DI == OI && DJ == OJ) return DS<T, (1u << F)>{x, {T(1)}};
     else return P<T>{x};
   }
   template <int G, int DI, int DJ>
   __device__ __forceinline__ P<T> g() const {
-    return P<T>{tab->g[G][p.o2 + loc<DI, DJ>()]};
+    return P<T>{ld_local(tab->g[G] + p.o2, (uint32_t)loc<DI, DJ>())};
   }
   __device__ __forceinline__ T z(int k) const { return vert.z(p.m, k); }
   __device__ __forceinline__ T zz(int k) const { return vert.zz(p.m, k); }
@@ -573,19 +620,19 @@ struct AccM {
   __device__ __forceinline__ int ck(int k) const { return k < 0 ? 0 : (k >= KB ? KB - 1 : k); }
   template <int F, int DI, int DJ, int DK>
   __device__ __forceinline__ auto at() const {
-    T x = tab->f[F][ix3(p, ci(p.i + DI), cj(p.j + DJ), ck(p.k + DK))];
+    T x = ld_local(tab->f[F] + p.o3, lix3(p, ci(p.i + DI), cj(p.j + DJ), ck(p.k + DK)));
     if constexpr (((MASK >> F) & 1u) && DI == SI && DJ == SJ && DK == SK) return DS<T, (1u << F)>{x, {T(1)}};
     else return P<T>{x};
   }
   template <int F, int DI, int DJ>
   __device__ __forceinline__ auto at2() const {
-    T x = tab->f[F][ix2(p, ci(p.i + DI), cj(p.j + DJ))];
+    T x = ld_local(tab->f[F] + p.o2, lix2(p, ci(p.i + DI), cj(p.j + DJ)));
     if constexpr (((MASK >> F) & 1u) && DI == SI && DJ == SJ) return DS<T, (1u << F)>{x, {T(1)}};
     else return P<T>{x};
   }
   template <int G, int DI, int DJ>
   __device__ __forceinline__ P<T> g() const {
-    return P<T>{tab->g[G][ix2(p, ci(p.i + DI), cj(p.j + DJ))]};
+    return P<T>{ld_local(tab->g[G] + p.o2, lix2(p, ci(p.i + DI), cj(p.j + DJ)))};
   }
   __device__ __forceinline__ T z(int k) const { return vert.z(p.m, k); }
   __device__ __forceinline__ T zz(int k) const { return vert.zz(p.m, k); }

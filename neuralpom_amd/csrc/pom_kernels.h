@@ -3,12 +3,13 @@
 //
 // Forward: one thread per output point (3-D: k fastest, so a wavefront covers
 // 4 whole water columns and every load is a contiguous 256-byte run).
-// Adjoint: for a seeded input field F the thread owning input point q sums
+// Adjoint: for each input field F the thread owning input point q sums
 //   sum_O sum_{o in box} obar_O(q - o) * d out_O(q - o) / d in_F(q)
-// where the partial comes from the D<T>-typed instantiation of eval() whose
-// only active access is (F, o).  Offsets whose instantiation is passive are
-// removed at compile time.  No atomics: every cotangent element is written by
-// exactly one thread, so the result is deterministic.
+// where the partials come from the DS-typed instantiation of eval() whose
+// active accesses are the fields seeded at offset o.  Offsets whose
+// instantiation is passive are removed at compile time.  No atomics: every
+// cotangent element is written by exactly one thread, so the result is
+// deterministic.
 #pragma once
 #include "pom_ops.h"
 
@@ -112,89 +113,6 @@ __global__ void __launch_bounds__(TPB) k_map2(Batch B, Vert<T> vert, const T* ra
   if (amax) block_amax(vmax, amax);   // uniform over the launch
 }
 
-// partial contribution of outputs at p = q - (DI, DJ, DK) to input F at q
-template <class T, class Op, int F, int DI, int DJ, int DK>
-__device__ __forceinline__ void gather_term(const FieldTab<T>& tab, const COutTab<T>& obar, const Pos& q, int pk,
-                                            const Vert<T>& vert, const T* ramp, const Params& prm, T& acc) {
-  Pos p = q;
-  p.i = q.i - DI;
-  p.j = q.j - DJ;
-  p.k = pk;
-  if (p.i < 0 || p.i >= p.im || p.j < 0 || p.j >= p.jm || p.k < 0 || p.k >= KB) return;
-  Acc<T, F, DI, DJ, DK> A{&tab, p, vert, ramp};
-  const int64_t idx = Op::OUT3 ? ix3(p, p.i, p.j, p.k) : ix2(p, p.i, p.j);
-  static_for<0, Op::NOUT - 1>([&](auto O) {
-    constexpr int o = decltype(O)::value;
-    using R = decltype(Op::template eval<o>(A, prm));
-    if constexpr (active_v<R>) {
-      if (obar.o[o]) {
-        T ob = obar.o[o][idx];
-        R r = Op::template eval<o>(A, prm);
-        acc += ob * r.d;
-      }
-    }
-  });
-}
-
-// 3-D seeded field, 3-D outputs: one thread per input point
-template <class T, class Op, int F>
-__global__ void __launch_bounds__(TPB) k_gather3(Batch B, Vert<T> vert, const T* ramp, Params prm,
-                                                  FieldTab<T> tab, COutTab<T> obar, T* inbar) {
-  Pos q;
-  if (!pos3(B, q)) return;
-  T acc = T(0);
-  static_for<-Op::RI, Op::RI>([&](auto I) {
-    static_for<-Op::RJ, Op::RJ>([&](auto J) {
-      static_for<-Op::RK, Op::RK>([&](auto K) {
-        constexpr int di = decltype(I)::value, dj = decltype(J)::value, dk = decltype(K)::value;
-        gather_term<T, Op, F, di, dj, dk>(tab, obar, q, q.k - dk, vert, ramp, prm, acc);
-      });
-    });
-  });
-  inbar[ix3(q, q.i, q.j, q.k)] += acc;
-}
-
-// 2-D seeded field of a 3-D operator: one thread per (column, level) sums the
-// partials of that level's outputs; the 16 lanes of a water column (k is the
-// fastest index, so a column is 16 consecutive lanes of one wavefront) are
-// reduced with cross-lane shuffles and lane k = 0 accumulates the column.
-template <class T, class Op, int F>
-__global__ void __launch_bounds__(TPB) k_gather2z(Batch B, Vert<T> vert, const T* ramp, Params prm,
-                                                   FieldTab<T> tab, COutTab<T> obar, T* inbar) {
-  Pos q;
-  const bool live = pos3(B, q);
-  T acc = T(0);
-  if (live) {
-    static_for<-Op::RI, Op::RI>([&](auto I) {
-      static_for<-Op::RJ, Op::RJ>([&](auto J) {
-        constexpr int di = decltype(I)::value, dj = decltype(J)::value;
-        gather_term<T, Op, F, di, dj, 0>(tab, obar, q, q.k, vert, ramp, prm, acc);
-      });
-    });
-  }
-  // a member's point count is a multiple of 16, so the 16-lane groups never
-  // straddle columns; inactive lanes contribute zero
-#pragma unroll
-  for (int o = 8; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 16);
-  if (live && q.k == 0) inbar[ix2(q, q.i, q.j)] += acc;
-}
-
-// 2-D seeded field of a 2-D operator: one thread per column
-template <class T, class Op, int F>
-__global__ void __launch_bounds__(TPB) k_gather2(Batch B, Vert<T> vert, const T* ramp, Params prm,
-                                                  FieldTab<T> tab, COutTab<T> obar, T* inbar) {
-  Pos q;
-  if (!pos2(B, q)) return;
-  T acc = T(0);
-  static_for<-Op::RI, Op::RI>([&](auto I) {
-    static_for<-Op::RJ, Op::RJ>([&](auto J) {
-      constexpr int di = decltype(I)::value, dj = decltype(J)::value;
-      gather_term<T, Op, F, di, dj, 0>(tab, obar, q, 0, vert, ramp, prm, acc);
-    });
-  });
-  inbar[ix2(q, q.i, q.j)] += acc;
-}
-
 // ------------------------------------------------------------ fused gathers
 // All seeded input fields of an operator in one pass: at each stencil offset
 // the point function is evaluated once with every field seeded there (DS
@@ -232,14 +150,14 @@ __device__ __forceinline__ void gather_all_term(const FieldTab<T>& tab, const CO
     }
     AccT<T, MASK, DI, DJ, DK> A{&tab, p, vert, ramp, &nb};
     const int ci = clampi(p.i, p.im), cj = clampi(p.j, p.jm);
-    const int64_t idx = Op::OUT3 ? ix3(p, ci, cj, clampi(p.k, KB)) : ix2(p, ci, cj);
+    const uint32_t idx = Op::OUT3 ? lix3(p, ci, cj, clampi(p.k, KB)) : lix2(p, ci, cj);
     static_for<0, Op::NOUT - 1>([&](auto O) {
       constexpr int o = decltype(O)::value;
       using R = decltype(Op::template eval<o>(A, prm));
       constexpr uint32_t MR = mask_of<R>::value;
       if constexpr (MR != 0u) {
         if (obar.o[o]) {
-          const T ob = obar.o[o][idx];
+          const T ob = ld_local(obar.o[o] + (Op::OUT3 ? p.o3 : p.o2), idx);
           const R r = Op::template eval<o>(A, prm);
           for_bits<MR>([&](auto I) {
             constexpr int f = decltype(I)::value;

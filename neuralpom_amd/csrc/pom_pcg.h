@@ -86,8 +86,9 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return sh[0];
 }
 
-// member and active flag of a 2-D block; pos2-style position of this thread
-__device__ __forceinline__ int block_member(const Batch& B) { return member_of(B.blk2, B.M, blockIdx.x); }
+// member of a 2-D block (the logical block of pos2); block partial sums are
+// stored at the logical block index, so a member's partials are contiguous
+__device__ __forceinline__ int block_member(const Batch& B) { return member_of(B.blk2, B.M, logical_block()); }
 
 // ---------------------------------------------------------------- tide
 // the tide at the nodes, amp cos(wt - phase), zero elsewhere
@@ -136,7 +137,7 @@ __global__ void __launch_bounds__(TPB) k_pcg_jacobi(Batch B, EllGeom<T> g, const
   __shared__ double sh[TPB];
   const int m = block_member(B);
   if (!act[m]) {
-    if (threadIdx.x == 0) part[2 * blockIdx.x] = part[2 * blockIdx.x + 1] = 0.0;
+    if (threadIdx.x == 0) part[2 * logical_block()] = part[2 * logical_block() + 1] = 0.0;
     return;
   }
   Pos p;
@@ -148,7 +149,7 @@ __global__ void __launch_bounds__(TPB) k_pcg_jacobi(Batch B, EllGeom<T> g, const
     rz = double(r[c]) * double(zv);
   }
   const double s = block_sum(rz, sh);
-  if (threadIdx.x == 0) part[2 * blockIdx.x] = s;
+  if (threadIdx.x == 0) part[2 * logical_block()] = s;
 }
 
 // partial sums of a.b over the interior (active members only) into part[2 b + slot]
@@ -158,7 +159,7 @@ __global__ void __launch_bounds__(TPB) k_pcg_dot(Batch B, const T* __restrict__ 
   __shared__ double sh[TPB];
   const int m = block_member(B);
   if (!act[m]) {
-    if (threadIdx.x == 0) part[2 * blockIdx.x + slot] = 0.0;
+    if (threadIdx.x == 0) part[2 * logical_block() + slot] = 0.0;
     return;
   }
   Pos p;
@@ -168,7 +169,7 @@ __global__ void __launch_bounds__(TPB) k_pcg_dot(Batch B, const T* __restrict__ 
     v = double(a[c]) * double(b[c]);
   }
   const double s = block_sum(v, sh);
-  if (threadIdx.x == 0) part[2 * blockIdx.x + slot] = s;
+  if (threadIdx.x == 0) part[2 * logical_block() + slot] = s;
 }
 
 // per-member sum of part[2 b + slot] over the member's blocks (one block per member)
@@ -213,7 +214,7 @@ __global__ void __launch_bounds__(TPB) k_pcg_ap(Batch B, EllGeom<T> g, const T* 
   __shared__ double sh[TPB];
   const int m = block_member(B);
   if (!act[m]) {
-    if (threadIdx.x == 0) part[2 * blockIdx.x] = 0.0;
+    if (threadIdx.x == 0) part[2 * logical_block()] = 0.0;
     return;
   }
   Pos p;
@@ -229,7 +230,7 @@ __global__ void __launch_bounds__(TPB) k_pcg_ap(Batch B, EllGeom<T> g, const T* 
     }
   }
   const double s = block_sum(v, sh);
-  if (threadIdx.x == 0) part[2 * blockIdx.x] = s;
+  if (threadIdx.x == 0) part[2 * logical_block()] = s;
 }
 
 // alpha = rho / (p.ap + 1e-30)
@@ -255,7 +256,7 @@ __global__ void __launch_bounds__(TPB) k_pcg_xr(Batch B, T* __restrict__ x, T* _
   __shared__ double sh[TPB];
   const int m = block_member(B);
   if (!act[m]) {
-    if (threadIdx.x == 0) part[2 * blockIdx.x + 1] = 0.0;
+    if (threadIdx.x == 0) part[2 * logical_block() + 1] = 0.0;
     return;
   }
   const T alpha = T(sc[PC_N * m + PC_ALPHA]);
@@ -271,7 +272,7 @@ __global__ void __launch_bounds__(TPB) k_pcg_xr(Batch B, T* __restrict__ x, T* _
     }
   }
   const double s = block_sum(v, sh);
-  if (threadIdx.x == 0) part[2 * blockIdx.x + 1] = s;
+  if (threadIdx.x == 0) part[2 * logical_block() + 1] = s;
 }
 
 // convergence: rs = r.r; a member stops when rs <= tol^2, rs <= rel_tol^2 rs0
@@ -355,7 +356,7 @@ __global__ void __launch_bounds__(TPB) k_eta_adj_in(Batch B, const T* __restrict
     v = fabs(double(x));
   }
   const double m = block_max(v, sh);
-  if (threadIdx.x == 0) part[2 * blockIdx.x] = m;
+  if (threadIdx.x == 0) part[2 * logical_block()] = m;
 }
 
 // scale = max(max|a|, 1); tol = eta_tol / scale

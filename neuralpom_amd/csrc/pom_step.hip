@@ -1851,6 +1851,8 @@ int pom_batch_create(const pom_member_t* members, int32_t n, int32_t dtype, pom_
     if (s.im < 4 || s.jm < 4) return POM_E_ARG;
     hm[m] = Member{s.im, s.jm, s.off2, s.off3};
     const int64_t c2 = (int64_t)s.im * s.jm, c3 = c2 * KB;
+    // kernels address a member's field with 32-bit byte offsets
+    if (c3 * (dtype == POM_F64 ? 8 : 4) >= (int64_t(1) << 31)) return POM_E_ARG;
     b3[m + 1] = b3[m] + (int32_t)((c3 + TPB - 1) / TPB);
     b2[m + 1] = b2[m] + (int32_t)((c2 + TPB - 1) / TPB);
     n2 = std::max(n2, s.off2 + c2);

@@ -112,10 +112,14 @@ def _perturbed(seed):
     return S
 
 
-def test_batched_members_match_oracle():
-    """Three members, each stopping its eta solve at its own convergence."""
-    states = [_perturbed(s) for s in (1, 2, 3)]
-    print("[gpu] implicit batched 3x(12x10) float64", flush=True)
+@pytest.mark.parametrize("members", [3, 12])
+def test_batched_members_match_oracle(members):
+    """Members each stopping its eta solve at its own convergence.  Twelve
+    members make twelve 2-D blocks, more than the eight XCDs, so the logical
+    block order differs from the dispatch order and each member's dot-product
+    partials must still be summed over that member's blocks alone."""
+    states = [_perturbed(s) for s in range(1, members + 1)]
+    print(f"[gpu] implicit batched {members}x(12x10) float64", flush=True)
     ens, state, _, its = _run([s.copy() for s in states], torch.float64, 2, True)
     bad = []
     for m, S in enumerate(states):
