@@ -89,12 +89,34 @@ def run_gpu(args):
     for _ in range(args.warmup):
         state, _, vamax = one_step(state)
     torch.cuda.synchronize()
-    L.profile_stage(DOMINANT)     # HIP events around each launch of the dominant kernel
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        state, grads, vamax = one_step(state)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+    if args.graph:
+        # the fwd+adjoint step recorded once into a HIP graph, replayed per step
+        from neuralpom_amd.explicit import StepGraph
+        it[0] += 1
+        G = StepGraph(ens, state, geom, sc, ramps(periods, time0 + sc.dti * it[0] / 86400.0, dtype, dev), vg,
+                      True, cot)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            it[0] += 1
+            G.ramp.copy_(ramps(periods, time0 + sc.dti * it[0] / 86400.0, dtype, dev))
+            vamax = G.replay()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        state, grads = {n: t.clone() for n, t in G.state.items()}, G.grads
+        del G
+        # the dominant kernel's launch time is measured on the eager path (events cannot be recorded in a graph)
+        L.profile_stage(DOMINANT)
+        for _ in range(2):
+            state, _, vamax = one_step(state)
+        torch.cuda.synchronize()
+    else:
+        L.profile_stage(DOMINANT)     # HIP events around each launch of the dominant kernel
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            state, grads, vamax = one_step(state)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
     kern_ms, kern_n = L.profile_read()
     L.profile_stage(None)
     if not float(vamax) <= sc.vmaxl:
@@ -113,6 +135,42 @@ def run_gpu(args):
     torch.cuda.synchronize()
     dtf = time.perf_counter() - t1
     return dict(cells=cells, seconds=dt, seconds_fwd=dtf, ens=ens, kern_ms=kern_ms, kern_n=kern_n)
+
+
+def graph_vs_eager(args, shapes=((90, 82),), nstep=20):
+    """fwd+adjoint ms per step of a small ensemble, eager launches against a HIP
+    graph replay (StepGraph).  One 90x82 member is launch-bound: the step
+    issues about a thousand kernels of a few microseconds each."""
+    from neuralpom_amd.explicit import StepGraph
+    dtype = torch.float32
+    dev = torch.device("cuda", 0)
+    ens, state, geom, vg, periods, sc, _ = build_ensemble(list(shapes), dtype, dev)
+    g = torch.Generator(device="cpu").manual_seed(7)
+    cot = {n: (torch.randn(ens.numel(n), generator=g) * 1e-3).to(dev, dtype) for n in DIFF_OUT}
+    r = ramps(periods, 1.0, dtype, dev)
+    st = state
+    for _ in range(3):
+        out, _ = ens.step(st, geom, sc, r, vg, True)
+        ens.step_adj(st, geom, sc, r, vg, True, dict(cot))
+        st = out
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(nstep):
+        out, _ = ens.step(st, geom, sc, r, vg, True)
+        ens.step_adj(st, geom, sc, r, vg, True, dict(cot))
+        st = out
+    torch.cuda.synchronize()
+    eager = (time.perf_counter() - t0) / nstep
+    G = StepGraph(ens, st, geom, sc, r, vg, True, cot)
+    G.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(nstep):
+        G.replay()
+    torch.cuda.synchronize()
+    graph = (time.perf_counter() - t0) / nstep
+    return {"members": [list(s) for s in shapes], "eager_ms_per_step": 1e3 * eager,
+            "graph_ms_per_step": 1e3 * graph}
 
 
 def run_gpu_implicit(args):
@@ -241,6 +299,8 @@ def main():
     ap.add_argument("--no-baselines", action="store_true")
     ap.add_argument("--implicit-steps", type=int, default=5,
                     help="fwd+adjoint steps of the same ensemble on the semi-implicit core (0: skip)")
+    ap.add_argument("--graph", action="store_true",
+                    help="time the fwd+adjoint step replayed from a HIP graph (StepGraph)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="launch one 256 MiB device copy first (reference for HBM counter scaling)")
     args = ap.parse_args()
@@ -276,7 +336,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "stratified seamount ensemble, 64 members (32x90x82, 24x120x110, "
                                "8x177x161) x 16 levels, explicit core, fwd+adjoint per step",
-                   "cells_per_step": res["cells"], "dti_s": DTE * 30, "isplit": 30},
+                   "cells_per_step": res["cells"], "dti_s": DTE * 30, "isplit": 30,
+                   "launch": "hip graph" if args.graph else "eager"},
         "forward_only": {"value": rate_fwd, "unit": "grid-cell-steps/s"},
         "roofline": {"bound": "hbm", "achieved": kern_bw / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": kern_bw / HBM_PEAK, "traffic": DOMINANT_TRAFFIC, "kernel": DOMINANT_KERNEL,
@@ -286,6 +347,7 @@ def main():
                           "unit": "GB/s", "frac": achieved / HBM_PEAK,
                           "bytes_per_cell_step": B_FA},
     }
+    line["launch_bound_case"] = graph_vs_eager(args)
     if args.implicit_steps > 0:
         line["implicit_core"] = run_gpu_implicit(args)
     if not args.no_baselines:

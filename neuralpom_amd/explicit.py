@@ -224,6 +224,53 @@ class ExplicitStep(torch.autograd.Function):
         return (None,) * 6 + tuple(grads.get(n) for n in L.FIELDS)
 
 
+class StepGraph:
+    """One explicit internal step (and optionally its adjoint) recorded once
+    into a HIP graph and replayed.
+
+    The graph reads the static buffers ``state`` (every field of FIELDS),
+    ``ramp`` and, with ``cot``, the output cotangents; a replay advances
+    ``state`` in place by one step (the step's outputs are copied back into the
+    input buffers inside the graph) and, with ``cot``, leaves the input
+    cotangents of the step it just took in ``grads``.  Refill ``ramp`` (and
+    ``cot``) between replays to change them.  The step launches ~100 kernels
+    per barotropic substep cycle and has no host synchronisation, so a replay
+    submits the whole step with one call.  Only the explicit core: the
+    semi-implicit eta solve polls its convergence counter from the host.
+    """
+
+    def __init__(self, ens, state, geom, sc, ramp, vgrid, internal=True, cot=None, warmup=1):
+        self.ens = ens
+        self.state = {n: state[n].clone() for n in L.FIELDS}
+        self.geom, self.sc, self.vgrid, self.internal = geom, sc, vgrid, internal
+        self.ramp = ramp.clone()
+        self.cot = None if cot is None else {n: c.contiguous().to(ens.dtype).clone() for n, c in cot.items()}
+        self.grads = None
+        s = torch.cuda.Stream(device=ens.device)
+        s.wait_stream(torch.cuda.current_stream(ens.device))
+        with torch.cuda.stream(s):   # warm-up outside the graph (on copies): workspaces are allocated here
+            for _ in range(warmup):
+                self._body({n: t.clone() for n, t in self.state.items()})
+        torch.cuda.current_stream(ens.device).wait_stream(s)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.grads, self.vamax = self._body(self.state)
+
+    def _body(self, st):
+        out, vamax = self.ens.step(st, self.geom, self.sc, self.ramp, self.vgrid, self.internal)
+        grads = None
+        if self.cot is not None:
+            grads = self.ens.step_adj(st, self.geom, self.sc, self.ramp, self.vgrid, self.internal, dict(self.cot))
+        for n in L.FIELDS:
+            st[n].copy_(out[n])
+        return grads, vamax
+
+    def replay(self):
+        """Advance ``state`` by one step (and fill ``grads``); returns the step's max |vaf| (device)."""
+        self.graph.replay()
+        return self.vamax
+
+
 def step_autograd(ens, state, geom, sc, ramp, vgrid, internal=True):
     """Differentiable step: returns (new state dict, vamax)."""
     outs = ExplicitStep.apply(ens, geom, sc, ramp, vgrid, internal, *[state[n] for n in L.FIELDS])
