@@ -173,6 +173,110 @@ def graph_vs_eager(args, shapes=((90, 82),), nstep=20):
             "graph_ms_per_step": 1e3 * graph}
 
 
+def coupled_interval(args, intervals=3, steps=4):
+    """The differentiable coupled interval on the 90x82 shelf basin (fp32, dte 30 s,
+    isplit 30, so 4 steps = 1 h): per interval the physics steps, then the
+    width-64 residual U-Net and the corrector hook; loss = volume-weighted MSE of
+    u, v, t, s against a seeded perturbation of the start state.  Times loss +
+    backward through the hand-written adjoint, and the same rollout with the
+    physics as the oracle's eager torch ops under autograd on the same GPU, and
+    reports their gradient agreement on the start temperature."""
+    import copy
+    from neuralpom_amd import cases
+    from neuralpom_amd import coupling as CP
+    from neuralpom_amd.core import POM2K_Core
+    from neuralpom_amd.hybrid import CorrectorNet, coupled_rollout
+    from oracle.pomref import explicit as X
+    from oracle.pomref.cases import double_gyre as oracle_dg
+    dtype = torch.float32
+    dev = torch.device("cuda", 0)
+    im, jm = 90, 82
+    case = cases.shelf_basin(im, jm, dtype=torch.float64)
+    scales = {"u": 0.02, "v": 0.02, "t": 0.1, "s": 0.02}
+    fields = ("u", "v", "t", "s")
+    leaves_n = ("t", "s", "u", "v", "el")
+    net = CorrectorNet(hidden_dim=64, head_std=1e-3, generator=torch.Generator().manual_seed(5)).to(dev)
+    gen = torch.Generator().manual_seed(9)
+    st0 = case[0]
+    target = {f: (st0[f][:, :, :CP.LEVELS] + 0.01 * torch.randn((im, jm, CP.LEVELS), generator=gen,
+                                                                dtype=torch.float64)).to(dev, dtype) for f in fields}
+
+    def loss_of(obj, w):
+        return sum(((getattr(obj, f)[:, :, :CP.LEVELS] - target[f].to(w[f].dtype)) ** 2 * w[f]).sum() / w[f].sum()
+                   for f in fields)
+
+    def hip_run(net=net, dtype=dtype):
+        core = POM2K_Core(im, jm, device=dev, dtype=dtype)
+        core.configure_runtime(dte=30.0, isplit=30)
+        core.run_initialization()
+        core.load_case(case)
+        core.time0 = core.time = 0.25
+        w = {f: CP.volume_weights(core, f) for f in fields}
+        lv = {n: getattr(core, n).clone().requires_grad_(True) for n in leaves_n}
+        for n, v in lv.items():
+            setattr(core, n, v)
+        coupled_rollout(core, net, scales, intervals, steps)
+        loss = loss_of(core, w)
+        return loss, torch.autograd.grad(loss, [lv["t"]] + list(net.parameters()))
+
+    def oracle_run(onet):
+        S = oracle_dg(im, jm, dte=30.0, isplit=30)
+        st, g, vg, period = case
+        for n, v in {**st, **g}.items():
+            setattr(S, n, v.clone())
+        S.z, S.zz, S.dz, S.dzz = (vg[i].clone() for i in range(4))
+        S.period = period
+        for k, v in list(vars(S).items()):
+            if torch.is_tensor(v):
+                setattr(S, k, v.to(dev, dtype))
+        S.time0 = S.time = 0.25
+        w = {f: CP.volume_weights(S, f) for f in fields}
+        lv = {n: getattr(S, n).clone().requires_grad_(True) for n in leaves_n}
+        for n, v in lv.items():
+            setattr(S, n, v)
+        S.eos = lambda t, s: X.dens(S, t, s, S.rho)
+        for _ in range(intervals):
+            S.time0 = S.time
+            S.iend = steps
+            X.run_time_loop(S)
+            planes = [getattr(S, n).permute(2, 0, 1) for n in ("u", "ub", "v", "vb", "t", "s")]
+            planes += [getattr(S, n).unsqueeze(0) for n in ("el", "ua", "uab", "va", "vab")]
+            CP.apply_joint_correction(S, onet(torch.cat(planes).unsqueeze(0))[0], scales)
+        loss = loss_of(S, w)
+        return loss, torch.autograd.grad(loss, [lv["t"]] + list(onet.parameters()))
+
+    hip_run()   # warm-up (workspaces, MIOpen kernels)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss, g_hip = hip_run()
+    torch.cuda.synchronize()
+    t_hip = time.perf_counter() - t0
+    onet = copy.deepcopy(net)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    oloss, g_ora = oracle_run(onet)
+    torch.cuda.synchronize()
+    t_ora = time.perf_counter() - t0
+    # float64 run of the same rollout (HIP adjoint), against which both float32 gradients are measured
+    _, g64 = hip_run(copy.deepcopy(net).double(), torch.float64)
+
+    def rel(a, b):
+        return float((a.double() - b).abs().max() / b.abs().max().clamp_min(1e-30))
+
+    gerr = rel(g_hip[0], g_ora[0])
+    perr = max(rel(a, b) for a, b in zip(g_hip[1:], g_ora[1:]))
+    err64 = {"hip_fp32_grad_t": rel(g_hip[0], g64[0]), "autograd_fp32_grad_t": rel(g_ora[0], g64[0]),
+             "hip_fp32_grad_params": max(rel(a, b) for a, b in zip(g_hip[1:], g64[1:])),
+             "autograd_fp32_grad_params": max(rel(a, b) for a, b in zip(g_ora[1:], g64[1:]))}
+    cells = im * jm * 16 * intervals * steps
+    return {"workload": f"{im}x{jm}x16 shelf basin, {intervals} intervals x {steps} steps of 900 s, "
+                        "U-Net width 64 after each interval, fp32",
+            "seconds_fwd_bwd": t_hip, "autograd_seconds_fwd_bwd": t_ora, "speedup_vs_autograd": t_ora / t_hip,
+            "cell_steps_per_s": cells / t_hip, "loss": float(loss), "autograd_loss": float(oloss),
+            "grad_t_rel_err_vs_autograd": gerr, "grad_params_rel_err_vs_autograd": perr,
+            "rel_err_vs_fp64": err64}
+
+
 def run_gpu_implicit(args):
     """The same seamount ensemble on the semi-implicit core (production clock dti = 600 s,
     spectral-preconditioned PCG, relative tolerance 1e-5), fwd+adjoint per step."""
@@ -348,6 +452,8 @@ def main():
                           "bytes_per_cell_step": B_FA},
     }
     line["launch_bound_case"] = graph_vs_eager(args)
+    if not args.no_baselines:
+        line["coupled_interval"] = coupled_interval(args)
     if args.implicit_steps > 0:
         line["implicit_core"] = run_gpu_implicit(args)
     if not args.no_baselines:

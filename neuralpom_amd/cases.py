@@ -185,6 +185,46 @@ def double_gyre(im, jm, dtype=torch.float32, device="cpu"):
     return st, g, torch.stack(vg), period
 
 
+def shelf_basin(im=90, jm=82, dx=10.0e3, dtype=torch.float32, device="cpu"):
+    """Closed rectangular basin with a shelf: uniform dx = dy, depth
+    200 + 1800 tanh(d / 5) m with d the distance in cells from the wet cells next to the land ring
+    (h = 1 there), zonal wind stress 0.1 cos(2 pi y / Ly) N/m^2 with y from the
+    southern wet row and Ly = (jm - 2) dy, T = 5 + 15 exp(z / 500 m) on the
+    interface depths z h, S = 35, Coriolis 2 Omega sin(lat) over 30-50 N as in
+    the double gyre.  The mean density rmean is the initial density (the double
+    gyre's rmean = rho(T = 0, S = 0) puts a large sigma-coordinate pressure
+    gradient error on the shelf break).  Returns (state, geom, vgrid, period_days)."""
+    st, g = _empty(im, jm, dtype, device)
+    deg2rad = math.pi / 180.0
+    omega = 7.2921e-5
+    dlat = 20.0 / float(jm - 2)
+    j_idx = torch.arange(jm, dtype=dtype, device=device).view(1, -1).repeat(im, 1)
+    i_idx = torch.arange(im, dtype=dtype, device=device).view(-1, 1).repeat(1, jm)
+    north = 30.0 + (j_idx - 1.0) * dlat
+    g["north_e"] = north.clone()
+    g["east_e"] = (i_idx - 1.0) * dx / 1.0e3
+    g["dx"] = torch.full((im, jm), dx, dtype=dtype, device=device)
+    g["dy"] = torch.full((im, jm), dx, dtype=dtype, device=device)
+    g["cor"] = 2.0 * omega * torch.sin(north * deg2rad)
+    dist = torch.minimum(torch.minimum(i_idx, (im - 1) - i_idx), torch.minimum(j_idx, (jm - 1) - j_idx))
+    h = 200.0 + 1800.0 * torch.tanh((dist - 1.0) / 5.0)   # 200 m on the cells next to the land ring
+    h[0, :] = h[-1, :] = 1.0
+    h[:, 0] = h[:, -1] = 1.0
+    g["h"] = h
+    vg = _uniform_sigma(dtype, device)
+    _areas_masks(g)
+    st["sb"].fill_(35.0)
+    st["tb"] = 5.0 + 15.0 * torch.exp(vg[0].view(1, 1, -1) * h.unsqueeze(2) / 500.0)
+    g["tclim"] = st["tb"].clone()
+    g["sclim"] = st["sb"].clone()
+    ly = (jm - 2) * dx
+    y = (j_idx - 1.0) * dx
+    g["wusurf"] = -(0.1 * torch.cos(2.0 * math.pi * y / ly) / SCALARS["rhoref"]) * g["dum"]
+    g["rmean"] = eos(st["tb"], st["sb"], h, vg[1], g["fsm"])
+    period = _initialize(st, g, vg, im, jm, float(g["cor"][im // 2, jm // 2]))
+    return st, g, torch.stack(vg), period
+
+
 def seamount(im, jm, seed=0, dx=8000.0, H=4500.0, L=25.0e3, n2=1.0e-5, noise=0.1,
              dtype=torch.float32, device="cpu"):
     """Stratified seamount member; returns (state, geom, vgrid, period_days)."""

@@ -163,6 +163,19 @@ class POM2K_Core:
         self.period = period
         self.horcon = float(os.environ.get("NEURALPOM_HORCON", "0.05"))   # cases.py:152
 
+    def load_case(self, case) -> None:
+        """Fields, vertical grid and inertial period of a case built by
+        ``neuralpom_amd.cases`` (state, geom, vgrid, period_days), in place of
+        the double gyre of ``run_initialization``."""
+        st, g, vg, period = case
+        if tuple(st["u"].shape[:2]) != (self.im, self.jm):
+            raise ValueError(f"case grid {tuple(st['u'].shape[:2])} != core grid {(self.im, self.jm)}")
+        dtype = self.dtype
+        for n, v in {**st, **g}.items():
+            setattr(self, n, v.to(self.device, dtype))
+        self.z, self.zz, self.dz, self.dzz = (vg[i].to(self.device, dtype).clone() for i in range(4))
+        self.period = float(period)
+
     # ------------------------------------------------------------ kernels
     @property
     def dtype(self) -> torch.dtype:
