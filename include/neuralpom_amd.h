@@ -272,6 +272,35 @@ int pom_restrict4(const pom_batch* fine, const pom_batch* coarse, int32_t three_
                   const void* fine_vgrid, void* out, void* stream);
 int pom_copy_arrays(int32_t n, void* const* dst, const void* const* src, const int64_t* bytes, void* stream);
 
+/* ---- The corrector hook between physics intervals, fused
+ * (src/neuralpom/hybrid/sparse_coupling.py:18-67: volume_weights,
+ * budget_project, paired_limit, apply_joint_correction).
+ *
+ * pom_correction_workspace: device scratch bytes of the two calls below.
+ * pom_joint_correction: for every member, raw = the network's 60 output planes
+ *   (u, v, t, s on levels 0-14) of im x jm starting at element 60 * off2[m];
+ *   scales = {u, v, t, s} (nonzero), cap > 0; geom = {dum, dvm, fsm, art, h, et}
+ *   (2-D); vgrid as for the step (dz is its row 2).  Velocity increments lose
+ *   their dz-weighted column mean and are masked; tracer increments lose their
+ *   layer mean weighted by w = art fsm (h + et) dz (wet cells only); each pair
+ *   is scaled by one factor min(cap / max(peak, cap), 1), peak = max |x / scale|
+ *   over the pair.  state_in / state_out = {u, ub, v, vb, t, tb, s, sb}: out =
+ *   in + increment on levels 0-14, level 15 copied.  diag (device [M][3], may
+ *   be null) = (tracer factor, t budget, s budget) as the reference returns
+ *   them.  The density refresh that follows is pom_dens.
+ * pom_joint_correction_adj: grad_raw (written, the raw layout) and grad_et
+ *   (accumulated, may be null) for cotangents cot_out of the 8 outputs
+ *   (entries may be null) and cot_diag [M][3] (may be null); the state inputs'
+ *   cotangents are cot_out themselves.  At a peak attained by several elements
+ *   the factor's cotangent is shared evenly by all of them. */
+int pom_correction_workspace(const pom_batch* b, size_t* bytes);
+int pom_joint_correction(const pom_batch* b, const void* raw, const void* const* geom, const void* vgrid,
+                         const double* scales, double cap, void* work, const void* const* state_in,
+                         void* const* state_out, void* diag, void* stream);
+int pom_joint_correction_adj(const pom_batch* b, const void* raw, const void* const* geom, const void* vgrid,
+                             const double* scales, double cap, void* work, const void* const* cot_out,
+                             const void* cot_diag, void* grad_raw, void* grad_et, void* stream);
+
 /* Tracing: record HIP events around every launch of one stage on its stream
  * (stage = -1 disables).  pom_profile_read waits for the recorded events and
  * returns the summed kernel time and the number of launches since the last
