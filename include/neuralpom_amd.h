@@ -240,6 +240,26 @@ int pom_baropg(const pom_batch* b, const pom_params_t* prm, const void* ramp, co
 int pom_baropg_adj(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vgrid,
                    const void* const* in, const void* const* cot_out, void* const* grad_in, void* stream);
 
+/* pom_bcond: one lateral boundary condition of the explicit core alone,
+ *   POM2K_Core.bcond(idx, elf, uaf, vaf, uf, vf, w) (cores/explicit/
+ *   baroclinic.py:496-928), as callers outside the time loop apply it (the
+ *   restart path: cores/implicit_p0_fixed/checkpointing.py:55-77).  fields / out per idx:
+ *     1: {elf}                          -> {elf}   (edge copy, * fsm)
+ *     2: {uaf, vaf, el}                 -> {uaf, vaf}   (radiation forms with rfe..rfs and ramp, * dum / dvm)
+ *     3: {uf, vf, u, v}                 -> {uf, vf}   (edge velocities, * dum / dvm)
+ *     4: {uf, vf, t, s, u, v, w, dt}    -> {uf, vf} = new t, s   (upstream edge tracers, * fsm)
+ *     5: {w}                            -> {w}   (levels 0-14 * fsm)
+ *     6: {uf, vf, q2, q2l, u, v}        -> {uf, vf} = new q2, q2l   (upstream edge turbulence, + 1e-10)
+ *   (dt is 2-D, the others 3-D except elf, uaf, vaf, el); geom is the
+ *   pom_geom table.  prm->core must be POM_CORE_EXPLICIT.
+ * pom_bcond_adj: grad_fields[q] += d(out)/d(fields[q])^T cot_out (entries
+ *   may be null). */
+int pom_bcond(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vgrid, int32_t idx,
+              const void* const* fields, const void* const* geom, void* const* out, void* stream);
+int pom_bcond_adj(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vgrid, int32_t idx,
+                  const void* const* fields, const void* const* geom, const void* const* cot_out,
+                  void* const* grad_fields, void* stream);
+
 /* ---- State movement around the step (no physics).
  *
  * pom_pack_input: the corrector network's input, 101 channels per member in

@@ -309,6 +309,43 @@ class POM2K_Core:
         shape = (self.im, self.jm, KB)
         return tuple(x.view(shape) for x in out)
 
+    def bcond(self, idx, elf, uaf, vaf, uf, vf, w):
+        """One lateral boundary condition (baroclinic.py:496-928) on the HIP
+        boundary operators; returns (elf, uaf, vaf, uf, vf, w) with the fields
+        of ``idx`` replaced, reading the core's el and ramp (idx 2), u and v
+        (3, 4, 6), t, s, w and dt (4), q2 and q2l (6) as the reference does.  The
+        exterior values of open edges (ele, uabe, tbe, ...) are zero: the
+        double gyre's closed basin.  Differentiable."""
+        F = self._as_flat
+        if idx == 1:
+            fields = [F(elf)]
+        elif idx == 2:
+            fields = [F(uaf), F(vaf), F(self.el)]
+        elif idx == 3:
+            fields = [F(uf), F(vf), F(self.u), F(self.v)]
+        elif idx == 4:
+            fields = [F(uf), F(vf), F(self.t), F(self.s), F(self.u), F(self.v), F(self.w), F(self.dt)]
+        elif idx == 5:
+            fields = [F(w)]
+        elif idx == 6:
+            fields = [F(uf), F(vf), F(self.q2), F(self.q2l), F(self.u), F(self.v)]
+        else:
+            raise ValueError(f"bcond index {idx} is not one of 1-6")
+        ramp = torch.tensor([float(self.ramp)], dtype=self.dtype, device=self.device)
+        outs = _Bcond.apply(self._ensemble(), self._params(), ramp, self._vgrid(), int(idx), self._geom(), *fields)
+        if torch.is_tensor(outs):
+            outs = (outs,)
+        shape3, shape2 = (self.im, self.jm, self.kb), (self.im, self.jm)
+        if idx == 1:
+            elf = outs[0].view(shape2)
+        elif idx == 2:
+            uaf, vaf = outs[0].view(shape2), outs[1].view(shape2)
+        elif idx == 5:
+            w = outs[0].view(shape3)
+        else:
+            uf, vf = outs[0].view(shape3), outs[1].view(shape3)
+        return elf, uaf, vaf, uf, vf, w
+
     def _as_flat(self, v):
         if v.dtype != self.dtype or v.device != self.device:
             v = v.to(self.device, self.dtype)
@@ -344,6 +381,33 @@ class _Dens(torch.autograd.Function):
                                      h.data_ptr(), fsm.data_ptr(), g.data_ptr(), bt.data_ptr(), bs.data_ptr(),
                                      br.data_ptr(), stream), "pom_dens_adj")
         return None, None, None, bt, bs, br, None, None
+
+
+class _Bcond(torch.autograd.Function):
+    """pom_bcond / pom_bcond_adj: inputs the idx-specific field list, outputs
+    the replaced fields."""
+
+    @staticmethod
+    def forward(ctx, ens, prm, ramp, vg, idx, geom, *fields):
+        nout = 1 if idx in (1, 5) else 2
+        outs = [torch.empty_like(fields[q]) for q in range(nout)]
+        stream = torch.cuda.current_stream(ens.device).cuda_stream
+        L.check(ens.lib.pom_bcond(ens._h, prm, ramp.data_ptr(), vg.data_ptr(), idx, _ptrs(fields),
+                                  L.geom_ptrs(geom), _ptrs(outs), stream), "pom_bcond")
+        ctx.ens, ctx.prm, ctx.idx, ctx.geom = ens, prm, idx, geom
+        ctx.save_for_backward(ramp, vg, *fields)
+        return tuple(outs) if nout > 1 else outs[0]
+
+    @staticmethod
+    def backward(ctx, *g):
+        ramp, vg, *fields = ctx.saved_tensors
+        ens = ctx.ens
+        cots = [x.contiguous() if x is not None else None for x in g]
+        grads = [torch.zeros_like(f) if ctx.needs_input_grad[6 + q] else None for q, f in enumerate(fields)]
+        stream = torch.cuda.current_stream(ens.device).cuda_stream
+        L.check(ens.lib.pom_bcond_adj(ens._h, ctx.prm, ramp.data_ptr(), vg.data_ptr(), ctx.idx, _ptrs(fields),
+                                      L.geom_ptrs(ctx.geom), _ptrs(cots), _ptrs(grads), stream), "pom_bcond_adj")
+        return (None,) * 6 + tuple(grads)
 
 
 class _Baropg(torch.autograd.Function):

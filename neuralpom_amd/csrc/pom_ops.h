@@ -595,6 +595,32 @@ struct OpBc6Filter {
 };
 
 // =====================================================================
+// Outputs O0 and O1 of an operator as a two-output operator (same fields,
+// stencil and edge band): bcond 4 and bcond 6 alone are outputs 0 and 2 of
+// the fused boundary + filter operators.
+// =====================================================================
+template <class Base, int O0, int O1> struct OpPick : Base {
+  static constexpr int NOUT = 2;
+  template <int O, class AC> __device__ static auto eval(const AC& A, const Params& prm) {
+    return Base::template eval<O == 0 ? O0 : O1>(A, prm);
+  }
+};
+using OpBcond4 = OpPick<OpBc4Filter, 0, 2>;   // uf, vf = new t, s with bcond 4 (baroclinic.py:576-846)
+using OpBcond6 = OpPick<OpBc6Filter, 0, 2>;   // uf, vf = new q2, q2l with bcond 6 (baroclinic.py:856-925)
+
+// bcond 5 (baroclinic.py:849-851): w(:, :, 0:kbm1) * fsm
+struct OpBcond5 {
+  enum { W, N3, NIN = N3 };
+  enum { GFSM };
+  static constexpr int NOUT = 1;
+  static constexpr bool OUT3 = true;
+  static constexpr int RI = 0, RJ = 0, RK = 0;
+  template <int O, class AC> __device__ static auto eval(const AC& A, const Params&) {
+    return BR(A.p.k < KBM1, AT(W, 0, 0, 0) * G2(GFSM, 0, 0), AT(W, 0, 0, 0));
+  }
+};
+
+// =====================================================================
 // km/kh lateral edge copy (tracers.py:425-429): arr(edge) = arr(inner)*fsm,
 // applied E, W, N, S in that order.  outputs: 0 km, 1 kh
 // =====================================================================
@@ -750,8 +776,9 @@ struct OpAdvave {
 };
 
 // depth-mean momentum update + bcond 2 (time_loop.py:326-383;
-// baroclinic.py:529-571).  outputs 0 uaf, 1 vaf.
-struct OpExtUV {
+// baroclinic.py:529-571).  outputs 0 uaf, 1 vaf.  BC_ONLY: bcond 2 alone on
+// given uaf / vaf (UAFIN / VAFIN): interior points keep their input.
+template <bool BC_ONLY> struct OpExtUVG {
   enum { EL, ELB, ELF, D_, UA, VA, UAB, VAB, ADVUA, ADVVA, ADX2D, ADY2D, DRX2D, DRY2D, WUBOT, WVBOT,
          UAFIN, VAFIN, N3 = 0, NIN = 18 };
   enum { GCOR, GDX, GDY, GH, GARU, GARV, GDUM, GDVM, GEATM, GWUSURF, GWVSURF };
@@ -812,10 +839,12 @@ struct OpExtUV {
         else if (jin && i == 1) r = cvt<R>(west1());
         else if (jin && i == 0) r = cvt<R>(west0());
         else if (iin && (j == jm - 1 || j == 0)) r = R{};
-        else if (iin && jin) r = cvt<R>(uaf_in(A, prm));
+        else if (iin && jin && !BC_ONLY) r = cvt<R>(uaf_in(A, prm));
       } else {   // passive: every candidate evaluated (indices are clamped), then selected
         const R c_in = cvt<R>(AT2(UAFIN, 0, 0)), c_e = cvt<R>(east()), c_w1 = cvt<R>(west1()),
-                c_w0 = cvt<R>(west0()), c_int = cvt<R>(uaf_in(A, prm));
+                c_w0 = cvt<R>(west0());
+        R c_int = c_in;
+        if constexpr (!BC_ONLY) c_int = cvt<R>(uaf_in(A, prm));
         r = (jin && i == im - 1) ? c_e
           : (jin && i == 1) ? c_w1
           : (jin && i == 0) ? c_w0
@@ -837,10 +866,12 @@ struct OpExtUV {
         else if (iin && j == jm - 1) r = cvt<R>(north());
         else if (iin && j == 1) r = cvt<R>(south1());
         else if (iin && j == 0) r = cvt<R>(south0());
-        else if (iin && jin) r = cvt<R>(vaf_in(A, prm));
+        else if (iin && jin && !BC_ONLY) r = cvt<R>(vaf_in(A, prm));
       } else {
         const R c_in = cvt<R>(AT2(VAFIN, 0, 0)), c_n = cvt<R>(north()), c_s1 = cvt<R>(south1()),
-                c_s0 = cvt<R>(south0()), c_int = cvt<R>(vaf_in(A, prm));
+                c_s0 = cvt<R>(south0());
+        R c_int = c_in;
+        if constexpr (!BC_ONLY) c_int = cvt<R>(vaf_in(A, prm));
         r = (jin && (i == im - 1 || i == 0)) ? R{}
           : (iin && j == jm - 1) ? c_n
           : (iin && j == 1) ? c_s1
@@ -851,6 +882,9 @@ struct OpExtUV {
     }
   }
 };
+
+using OpExtUV = OpExtUVG<false>;
+using OpBcond2 = OpExtUVG<true>;
 
 // Robert-Asselin filter, level shift and time-mean accumulation of one
 // external substep (time_loop.py:386-431).  The substep index selects the

@@ -1690,6 +1690,101 @@ int run_dens(const pom_batch* b, const pom_params_t* prm, const void* vg, const 
   return hipGetLastError() == hipSuccess ? POM_OK : POM_E_HIP;
 }
 
+// bcond(idx) of the explicit core alone (baroclinic.py:496-928), forward or
+// adjoint, on the boundary point operators the fused step uses.  fields and
+// outputs per idx as documented in the header.
+template <class T, class Op>
+void bcond_op(const Ctx<T>& c, const FieldTab<T>& tab, const int* fid, int nf, const int* oid, int no,
+              void* const* out, const void* const* cot, void* const* grad, bool adjoint) {
+  if (!adjoint) {
+    OutTab<T> o{};
+    for (int q = 0; q < no; ++q) o.o[oid[q]] = static_cast<T*>(out[q]);
+    launch_map<T, Op>(c, tab, o);
+  } else {
+    COutTab<T> ob{};
+    for (int q = 0; q < no; ++q) ob.o[oid[q]] = static_cast<const T*>(cot[q]);
+    T* ib[Op::NIN] = {};
+    for (int q = 0; q < nf; ++q) ib[fid[q]] = static_cast<T*>(grad[q]);
+    launch_gather<T, Op>(c, tab, ob, ib);
+  }
+}
+
+int bcond_nfields(int idx) {
+  static const int n[7] = {0, 1, 3, 4, 8, 1, 6};
+  return (idx >= 1 && idx <= 6) ? n[idx] : -1;
+}
+int bcond_nout(int idx) { return (idx == 1 || idx == 5) ? 1 : 2; }
+
+template <class T>
+int run_bcond(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vg, int idx,
+              const void* const* fields, const void* const* geom, void* const* out, const void* const* cot,
+              void* const* grad, hipStream_t st, bool adjoint) {
+  Ctx<T> c = make_ctx<T>(b, prm, ramp, vg, st);
+  auto F = [&](int q) { return static_cast<const T*>(fields[q]); };
+  auto G = [&](int q) { return static_cast<const T*>(geom[q]); };
+  FieldTab<T> t{};
+  switch (idx) {
+    case 1: {
+      t.f[OpBc1::RAW] = F(0);
+      t.g[OpBc1::GFSM] = G(POM_G_FSM);
+      const int fid[1] = {OpBc1::RAW}, oid[1] = {0};
+      bcond_op<T, OpBc1>(c, t, fid, 1, oid, 1, out, cot, grad, adjoint);
+      break;
+    }
+    case 2: {
+      t.f[OpBcond2::UAFIN] = F(0);
+      t.f[OpBcond2::VAFIN] = F(1);
+      t.f[OpBcond2::EL] = F(2);
+      t.g[OpBcond2::GH] = G(POM_G_H);
+      t.g[OpBcond2::GDUM] = G(POM_G_DUM);
+      t.g[OpBcond2::GDVM] = G(POM_G_DVM);
+      const int fid[3] = {OpBcond2::UAFIN, OpBcond2::VAFIN, OpBcond2::EL}, oid[2] = {0, 1};
+      bcond_op<T, OpBcond2>(c, t, fid, 3, oid, 2, out, cot, grad, adjoint);
+      break;
+    }
+    case 3: {
+      t.f[OpBc3::UF] = F(0);
+      t.f[OpBc3::VF] = F(1);
+      t.f[OpBc3::U] = F(2);
+      t.f[OpBc3::V] = F(3);
+      t.g[OpBc3::GH] = G(POM_G_H);
+      t.g[OpBc3::GDUM] = G(POM_G_DUM);
+      t.g[OpBc3::GDVM] = G(POM_G_DVM);
+      const int fid[4] = {OpBc3::UF, OpBc3::VF, OpBc3::U, OpBc3::V}, oid[2] = {0, 1};
+      bcond_op<T, OpBc3>(c, t, fid, 4, oid, 2, out, cot, grad, adjoint);
+      break;
+    }
+    case 4: {
+      using O = OpBcond4;
+      const int fid[8] = {O::TF, O::SF, O::TT, O::SS, O::U, O::V, O::W, O::DT}, oid[2] = {0, 1};
+      for (int q = 0; q < 8; ++q) t.f[fid[q]] = F(q);
+      t.g[O::GDX] = G(POM_G_DX);
+      t.g[O::GDY] = G(POM_G_DY);
+      t.g[O::GFSM] = G(POM_G_FSM);
+      bcond_op<T, O>(c, t, fid, 8, oid, 2, out, cot, grad, adjoint);
+      break;
+    }
+    case 5: {
+      t.f[OpBcond5::W] = F(0);
+      t.g[OpBcond5::GFSM] = G(POM_G_FSM);
+      const int fid[1] = {OpBcond5::W}, oid[1] = {0};
+      bcond_op<T, OpBcond5>(c, t, fid, 1, oid, 1, out, cot, grad, adjoint);
+      break;
+    }
+    default: {
+      using O = OpBcond6;
+      const int fid[6] = {O::Q2F, O::Q2LF, O::Q2, O::Q2L, O::U, O::V}, oid[2] = {0, 1};
+      for (int q = 0; q < 6; ++q) t.f[fid[q]] = F(q);
+      t.g[O::GDX] = G(POM_G_DX);
+      t.g[O::GDY] = G(POM_G_DY);
+      t.g[O::GFSM] = G(POM_G_FSM);
+      bcond_op<T, O>(c, t, fid, 6, oid, 2, out, cot, grad, adjoint);
+      break;
+    }
+  }
+  return hipGetLastError() == hipSuccess ? POM_OK : POM_E_HIP;
+}
+
 template <class T>
 int run_baropg(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vg, const void* const* in,
                void* const* out, const void* const* cot, void* const* grad, hipStream_t st, bool adjoint) {
@@ -1960,6 +2055,38 @@ int pom_dens_adj(const pom_batch* b, const pom_params_t* prm, const void* vgrid,
   if (b->dtype == POM_F32)
     return run_dens<float>(b, prm, vgrid, t, s, rho_in, h, fsm, b_rho_out, nullptr, b_t, b_s, b_rho_in, st, true);
   return run_dens<double>(b, prm, vgrid, t, s, rho_in, h, fsm, b_rho_out, nullptr, b_t, b_s, b_rho_in, st, true);
+}
+
+int pom_bcond(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vgrid, int32_t idx,
+              const void* const* fields, const void* const* geom, void* const* out, void* stream) {
+  const int nf = bcond_nfields(idx);
+  if (!b || !prm || !ramp || !vgrid || !fields || !geom || !out || nf < 0) return POM_E_ARG;
+  if (prm->core != POM_CORE_EXPLICIT) return POM_E_UNSUPPORTED;
+  for (int q = 0; q < nf; ++q)
+    if (!fields[q]) return POM_E_ARG;
+  for (int q = 0; q < bcond_nout(idx); ++q)
+    if (!out[q]) return POM_E_ARG;
+  for (int q = 0; q < POM_NGEOM_BASE; ++q)
+    if (!geom[q]) return POM_E_ARG;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (b->dtype == POM_F32) return run_bcond<float>(b, prm, ramp, vgrid, idx, fields, geom, out, nullptr, nullptr, st, false);
+  return run_bcond<double>(b, prm, ramp, vgrid, idx, fields, geom, out, nullptr, nullptr, st, false);
+}
+
+int pom_bcond_adj(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vgrid, int32_t idx,
+                  const void* const* fields, const void* const* geom, const void* const* cot_out,
+                  void* const* grad_fields, void* stream) {
+  const int nf = bcond_nfields(idx);
+  if (!b || !prm || !ramp || !vgrid || !fields || !geom || !cot_out || !grad_fields || nf < 0) return POM_E_ARG;
+  if (prm->core != POM_CORE_EXPLICIT) return POM_E_UNSUPPORTED;
+  for (int q = 0; q < nf; ++q)
+    if (!fields[q]) return POM_E_ARG;
+  for (int q = 0; q < POM_NGEOM_BASE; ++q)
+    if (!geom[q]) return POM_E_ARG;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (b->dtype == POM_F32)
+    return run_bcond<float>(b, prm, ramp, vgrid, idx, fields, geom, nullptr, cot_out, grad_fields, st, true);
+  return run_bcond<double>(b, prm, ramp, vgrid, idx, fields, geom, nullptr, cot_out, grad_fields, st, true);
 }
 
 int pom_baropg(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vgrid,
