@@ -67,8 +67,7 @@ def ramps(periods, time_days, dtype, device):
     return torch.tensor([min(time_days / p, 1.0) for p in periods], dtype=dtype, device=device)
 
 
-def run_gpu(args):
-    dtype = torch.float32
+def run_gpu(args, dtype=torch.float32):
     dev = torch.device("cuda", 0)
     ens, state, geom, vg, periods, sc, _ = build_ensemble(SHAPES, dtype, dev)
     cells = ens.n3
@@ -89,6 +88,12 @@ def run_gpu(args):
     for _ in range(args.warmup):
         state, _, vamax = one_step(state)
     torch.cuda.synchronize()
+    if dtype != torch.float32:   # a precision leg: the timed fwd+adjoint steps only
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            state, grads, vamax = one_step(state)
+        torch.cuda.synchronize()
+        return dict(cells=cells, seconds=time.perf_counter() - t0)
     if args.graph:
         # the fwd+adjoint step recorded once into a HIP graph, replayed per step
         from neuralpom_amd.explicit import StepGraph
@@ -452,6 +457,12 @@ def main():
                           "bytes_per_cell_step": B_FA},
     }
     if not args.no_baselines:
+        # the same workload in float64: the precision in which the step meets the 1e-5
+        # relative target over 24 h (float32 carries the reference algorithm's own
+        # float32 error, about 4e-3 on the surface elevation, tests/test_gpu_shelf24h.py)
+        r64 = run_gpu(args, torch.float64)
+        line["float64"] = {"value": r64["cells"] * args.steps / r64["seconds"], "unit": "grid-cell-steps/s",
+                           "ms_per_step": 1e3 * r64["seconds"] / args.steps}
         line["launch_bound_case"] = graph_vs_eager(args)
         line["coupled_interval"] = coupled_interval(args)
     if args.implicit_steps > 0:
