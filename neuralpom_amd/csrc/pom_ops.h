@@ -785,6 +785,14 @@ template <bool BC_ONLY> struct OpExtUVG {
   static constexpr int NOUT = 2;
   static constexpr bool OUT3 = false;
   static constexpr int RI = 1, RJ = 1, RK = 0;
+  // a radiation form of one edge: zero, without evaluating it, when its
+  // coefficient is zero (a closed edge; the coefficient is uniform over the
+  // launch, so this is a scalar branch)
+  template <class F> __device__ static auto radiation(double rf, F f) {
+    using R = decltype(f());
+    if (rf == 0.0) return R{};
+    return f();
+  }
   template <class X> __device__ static auto cwave(X h, double grav) {   // safe_sqrt(g / clamp(h))
     using T = typename scalar_of<X>::type;
     T hv = fmax(val(h), T(1e-8));
@@ -826,9 +834,15 @@ template <bool BC_ONLY> struct OpExtUVG {
     const double ramp = A.rampv();
     if constexpr (O == 0) {
       // east (rfe) / west (rfw) open-boundary forms; exterior values are zero
-      auto east = [&]() { return (prm.rfe * cwave(G2(GH, -1, 0), prm.grav) * (AT2(EL, -1, 0) - 0.0)) * ramp; };
-      auto west1 = [&]() { return (0.0 - prm.rfw * cwave(G2(GH, 0, 0), prm.grav) * (AT2(EL, 0, 0) - 0.0)) * ramp; };
-      auto west0 = [&]() { return (0.0 - prm.rfw * cwave(G2(GH, 1, 0), prm.grav) * (AT2(EL, 1, 0) - 0.0)) * ramp; };
+      auto east = [&]() {
+        return radiation(prm.rfe, [&]() { return (prm.rfe * cwave(G2(GH, -1, 0), prm.grav) * (AT2(EL, -1, 0) - 0.0)) * ramp; });
+      };
+      auto west1 = [&]() {
+        return radiation(prm.rfw, [&]() { return (0.0 - prm.rfw * cwave(G2(GH, 0, 0), prm.grav) * (AT2(EL, 0, 0) - 0.0)) * ramp; });
+      };
+      auto west0 = [&]() {
+        return radiation(prm.rfw, [&]() { return (0.0 - prm.rfw * cwave(G2(GH, 1, 0), prm.grav) * (AT2(EL, 1, 0) - 0.0)) * ramp; });
+      };
       using R = common_t<common_t<common_t<decltype(uaf_in(A, prm)), decltype(east())>,
                                   common_t<decltype(west1()), decltype(west0())>>,
                          decltype(AT2(UAFIN, 0, 0))>;
@@ -853,9 +867,15 @@ template <bool BC_ONLY> struct OpExtUVG {
       }
       return r * G2(GDUM, 0, 0);
     } else {
-      auto north = [&]() { return (prm.rfn * cwave(G2(GH, 0, -1), prm.grav) * (AT2(EL, 0, -1) - 0.0)) * ramp; };
-      auto south1 = [&]() { return (0.0 - prm.rfs * cwave(G2(GH, 0, 0), prm.grav) * (AT2(EL, 0, 0) - 0.0)) * ramp; };
-      auto south0 = [&]() { return (0.0 - prm.rfs * cwave(G2(GH, 0, 1), prm.grav) * (AT2(EL, 0, 1) - 0.0)) * ramp; };
+      auto north = [&]() {
+        return radiation(prm.rfn, [&]() { return (prm.rfn * cwave(G2(GH, 0, -1), prm.grav) * (AT2(EL, 0, -1) - 0.0)) * ramp; });
+      };
+      auto south1 = [&]() {
+        return radiation(prm.rfs, [&]() { return (0.0 - prm.rfs * cwave(G2(GH, 0, 0), prm.grav) * (AT2(EL, 0, 0) - 0.0)) * ramp; });
+      };
+      auto south0 = [&]() {
+        return radiation(prm.rfs, [&]() { return (0.0 - prm.rfs * cwave(G2(GH, 0, 1), prm.grav) * (AT2(EL, 0, 1) - 0.0)) * ramp; });
+      };
       using R = common_t<common_t<common_t<decltype(vaf_in(A, prm)), decltype(north())>,
                                   common_t<decltype(south1()), decltype(south0())>>,
                          decltype(AT2(VAFIN, 0, 0))>;
