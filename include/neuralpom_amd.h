@@ -279,6 +279,10 @@ int pom_bcond_adj(const pom_batch* b, const pom_params_t* prm, const void* ramp,
  *   art*mask*(h + el)[*dz(k) for 3-D fields] (volume_weighted = 1); the coarse
  *   edge ring is 0.  The fine-to-coarse projection of the reference's training
  *   data (paper/paper_zh.md:625), whose code is not in the reference.
+ * pom_restrict4_adj: grad_field += the restriction's transpose applied to
+ *   cot_out (a fine interior cell gets w / W of its coarse cell's cotangent);
+ *   with volume weights and el given, grad_el (may be null) += the
+ *   cotangent through the weights.  Arguments as pom_restrict4. 
  * pom_copy_arrays: n device-to-device copies (sizes in bytes, multiples of 4)
  *   in one launch: device-resident snapshots of a state for restart and the
  *   per-step recompute, replacing the host round trip of capture / restore
@@ -290,6 +294,9 @@ int pom_pack_input_adj(const pom_batch* b, const void* const* scale, const void*
 int pom_restrict4(const pom_batch* fine, const pom_batch* coarse, int32_t three_d, int32_t volume_weighted,
                   const void* field, const void* art, const void* mask, const void* h, const void* el,
                   const void* fine_vgrid, void* out, void* stream);
+int pom_restrict4_adj(const pom_batch* fine, const pom_batch* coarse, int32_t three_d, int32_t volume_weighted,
+                      const void* field, const void* art, const void* mask, const void* h, const void* el,
+                      const void* fine_vgrid, const void* cot_out, void* grad_field, void* grad_el, void* stream);
 int pom_copy_arrays(int32_t n, void* const* dst, const void* const* src, const int64_t* bytes, void* stream);
 
 /* ---- The corrector hook between physics intervals, fused

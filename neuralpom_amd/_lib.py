@@ -98,6 +98,7 @@ def load():
     lib.pom_pack_input.argtypes = [vp, vpp, vpp, vpp, vp, vp]
     lib.pom_pack_input_adj.argtypes = [vp, vpp, vp, vpp, vp]
     lib.pom_restrict4.argtypes = [vp, vp, C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp, vp, vp]
+    lib.pom_restrict4_adj.argtypes = [vp, vp, C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     lib.pom_copy_arrays.argtypes = [C.c_int32, vpp, vpp, C.POINTER(C.c_int64), vp]
     lib.pom_bcond.argtypes = [vp, C.POINTER(Params), vp, vp, C.c_int32, vpp, vpp, vpp, vp]
     lib.pom_bcond_adj.argtypes = [vp, C.POINTER(Params), vp, vp, C.c_int32, vpp, vpp, vpp, vpp, vp]
@@ -117,7 +118,7 @@ EXPORTED = ("pom_batch_create", "pom_batch_destroy", "pom_explicit_workspace", "
             "pom_explicit_step_adj", "pom_implicit_cache_bytes", "pom_implicit_prepare", "pom_implicit_workspace",
             "pom_implicit_step", "pom_implicit_step_adj", "pom_dens", "pom_dens_adj", "pom_baropg",
             "pom_baropg_adj", "pom_elliptic_apply", "pom_elliptic_apply_adj", "pom_eta_workspace", "pom_eta_solve",
-            "pom_eta_solve_adj", "pom_pack_input", "pom_pack_input_adj", "pom_restrict4", "pom_copy_arrays",
+            "pom_eta_solve_adj", "pom_pack_input", "pom_pack_input_adj", "pom_restrict4", "pom_restrict4_adj", "pom_copy_arrays",
             "pom_bcond", "pom_bcond_adj",
             "pom_correction_workspace", "pom_joint_correction", "pom_joint_correction_adj",
             "pom_profile_stage", "pom_profile_read")

@@ -136,6 +136,61 @@ __global__ void __launch_bounds__(TPB) k_restrict4(Batch fine, Batch coarse, Res
   a.out[co] = sw > T(0) ? sf / sw : T(0);
 }
 
+// Adjoint of the restriction: one thread per fine water column.  A fine cell
+// of block (I, J) receives w / W * cbar of its coarse cell (per level for 3-D
+// fields); with volume weights and el given, el also receives
+// sum_k cbar (f - c) / W * dw/del, dw/del = art * mask [* dz], through the
+// weights of the layer means.
+template <class T> struct RestrictAdjArgs {
+  RestrictArgs<T> fw;
+  const T* cbar;   // coarse cotangent
+  T* gf;           // fine field cotangent (accumulated)
+  T* gel;          // fine el cotangent (accumulated), may be null
+};
+
+template <class T>
+__global__ void __launch_bounds__(TPB) k_restrict4_adj(Batch fine, Batch coarse, RestrictAdjArgs<T> r) {
+  Pos p;
+  if (!pos2(fine, p)) return;
+  if (p.i < 1 || p.i > p.im - 2 || p.j < 1 || p.j > p.jm - 2) return;   // the fine edge ring is not sampled
+  const RestrictArgs<T>& a = r.fw;
+  const Member cm = coarse.mem[p.m];
+  const int I = (p.i - 1) / 4 + 1, J = (p.j - 1) / 4 + 1;
+  const int i0 = 4 * (I - 1) + 1, j0 = 4 * (J - 1) + 1;
+  const int64_t my2 = ix2(p, p.i, p.j);
+  const T base_w = a.art[my2] * a.mask[my2];
+  const T dmy = a.volume ? a.h[my2] + (a.el ? a.el[my2] : T(0)) : T(1);
+  const int nk = a.three_d ? KB : 1;
+  T gel = T(0);
+  for (int k = 0; k < nk; ++k) {
+    const T dz = a.three_d && a.volume ? a.vg[(p.m * 4 + 2) * KB + k] : T(1);
+    T sw = T(0), sf = T(0);
+#pragma unroll
+    for (int di = 0; di < 4; ++di)
+#pragma unroll
+      for (int dj = 0; dj < 4; ++dj) {
+        const int64_t c2 = p.o2 + (int64_t)(i0 + di) * p.jm + (j0 + dj);
+        T w = a.art[c2] * a.mask[c2];
+        if (a.volume) w = w * ((a.h[c2] + (a.el ? a.el[c2] : T(0))) * dz);
+        const T v = a.three_d ? a.f[p.o3 + ((int64_t)(i0 + di) * p.jm + (j0 + dj)) * KB + k] : a.f[c2];
+        sw += w;
+        sf += w * v;
+      }
+    if (!(sw > T(0))) continue;
+    const int64_t co = a.three_d ? cm.off3 + ((int64_t)I * cm.jm + J) * KB + k : cm.off2 + (int64_t)I * cm.jm + J;
+    const T cb = r.cbar[co];
+    const T w = a.volume ? base_w * dmy * dz : base_w;
+    const int64_t fo = a.three_d ? ix3(p, p.i, p.j, k) : my2;
+    r.gf[fo] += w / sw * cb;
+    if (r.gel && a.volume && a.el) {
+      const T c = sf / sw;
+      const T v = a.f[fo];
+      gel += cb * (v - c) / sw * base_w * dz;
+    }
+  }
+  if (r.gel && a.volume && a.el) r.gel[my2] += gel;
+}
+
 // ---------------------------------------------------------------- batched copy
 constexpr int MAXCOPY = 64;
 struct CopyTab {
@@ -236,6 +291,39 @@ int pom_restrict4(const pom_batch* fine, const pom_batch* coarse, int32_t three_
                       static_cast<const T*>(h), static_cast<const T*>(el), static_cast<const T*>(fine_vgrid),
                       static_cast<T*>(out), three_d != 0, volume_weighted != 0};
     hipLaunchKernelGGL(k_restrict4<T>, dim3(nb), dim3(TPB), 0, st, F, C, a);
+  };
+  if (fine->dtype == POM_F32) go(float{});
+  else go(double{});
+  return hipGetLastError() == hipSuccess ? POM_OK : POM_E_HIP;
+}
+
+int pom_restrict4_adj(const pom_batch* fine, const pom_batch* coarse, int32_t three_d, int32_t volume_weighted,
+                      const void* field, const void* art, const void* mask, const void* h, const void* el,
+                      const void* fine_vgrid, const void* cot_out, void* grad_field, void* grad_el, void* stream) {
+  if (!fine || !coarse || !field || !art || !mask || !cot_out || !grad_field) return POM_E_ARG;
+  if (volume_weighted && !h) return POM_E_ARG;
+  if (three_d && volume_weighted && !fine_vgrid) return POM_E_ARG;
+  if (fine->M != coarse->M || fine->dtype != coarse->dtype) return POM_E_ARG;
+  for (int m = 0; m < fine->M; ++m) {
+    const Member& f = fine->h_mem[m];
+    const Member& c = coarse->h_mem[m];
+    if ((f.im - 2) % 4 || (f.jm - 2) % 4 || c.im != (f.im - 2) / 4 + 2 || c.jm != (f.jm - 2) / 4 + 2)
+      return POM_E_ARG;
+  }
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const Batch F{fine->d_mem, fine->d_blk3, fine->d_blk2, fine->M};
+  const Batch C{coarse->d_mem, coarse->d_blk3, coarse->d_blk2, coarse->M};
+  if (fine->nblk2 == 0) return POM_OK;
+  auto go = [&](auto tag) {
+    using T = decltype(tag);
+    RestrictAdjArgs<T> r{};
+    r.fw = RestrictArgs<T>{static_cast<const T*>(field), static_cast<const T*>(art), static_cast<const T*>(mask),
+                           static_cast<const T*>(h), static_cast<const T*>(el), static_cast<const T*>(fine_vgrid),
+                           nullptr, three_d != 0, volume_weighted != 0};
+    r.cbar = static_cast<const T*>(cot_out);
+    r.gf = static_cast<T*>(grad_field);
+    r.gel = static_cast<T*>(grad_el);
+    hipLaunchKernelGGL(k_restrict4_adj<T>, dim3(fine->nblk2), dim3(TPB), 0, st, F, C, r);
   };
   if (fine->dtype == POM_F32) go(float{});
   else go(double{});

@@ -125,16 +125,46 @@ def inject_state(ens, state, m, values):
 def restrict4(fine, coarse, field, name, fine_geom, fine_el=None, fine_vgrid=None, volume=True):
     """Conservative 4 x 4 restriction of the fine batch's ``field`` (flat, the
     layout of field ``name``) onto the coarse batch.  Weights art * mask
-    [* (h + el) * dz]; mask is dum for u / ua / ub / uab, dvm for v, fsm else."""
+    [* (h + el) * dz]; mask is dum for u / ua / ub / uab, dvm for v, fsm else.
+    Differentiable in ``field`` and, with volume weights, in ``fine_el``."""
     three_d = name in L.THREE_D
     mask = {"u": "dum", "ub": "dum", "ua": "dum", "uab": "dum",
             "v": "dvm", "vb": "dvm", "va": "dvm", "vab": "dvm"}.get(name, "fsm")
-    out = torch.empty(coarse.numel(name), dtype=coarse.dtype, device=coarse.device)
-    args = [field.contiguous(), fine_geom["art"], fine_geom[mask], fine_geom["h"], fine_el, fine_vgrid]
-    ptrs = [a.data_ptr() if a is not None else None for a in args]
-    L.check(fine.lib.pom_restrict4(fine._h, coarse._h, int(three_d), int(bool(volume)), *ptrs, out.data_ptr(),
-                                   _stream(fine.device)), "pom_restrict4")
-    return out
+    geom = (fine_geom["art"].detach(), fine_geom[mask].detach(), fine_geom["h"].detach())
+    vg = fine_vgrid.detach() if fine_vgrid is not None else None
+    return Restrict4.apply(fine, coarse, int(three_d), bool(volume), geom, vg, field.contiguous(), fine_el)
+
+
+class Restrict4(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, fine, coarse, three_d, volume, geom, vg, field, el):
+        out = torch.empty(coarse.n3 if three_d else coarse.n2, dtype=coarse.dtype, device=coarse.device)
+        el_c = el.contiguous() if el is not None else None
+        ptrs = [t.data_ptr() if t is not None else None for t in (field,) + tuple(geom) + (el_c, vg)]
+        L.check(fine.lib.pom_restrict4(fine._h, coarse._h, three_d, int(volume), *ptrs, out.data_ptr(),
+                                       _stream(fine.device)), "pom_restrict4")
+        ctx.fine, ctx.coarse, ctx.three_d, ctx.volume = fine, coarse, three_d, volume
+        ctx.has_el = el is not None
+        ctx.save_for_backward(field, el_c if el is not None else field, vg if vg is not None else field, *geom)
+        ctx.has_vg = vg is not None
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        field, el, vg, art, mask, h = ctx.saved_tensors
+        el = el if ctx.has_el else None
+        vg = vg if ctx.has_vg else None
+        gf = torch.zeros_like(field) if ctx.needs_input_grad[6] else None
+        gel = torch.zeros_like(el) if (el is not None and ctx.needs_input_grad[7]) else None
+        if gf is None and gel is None:
+            return (None,) * 8
+        gf_buf = gf if gf is not None else torch.zeros_like(field)
+        ptrs = [t.data_ptr() if t is not None else None for t in (field, art, mask, h, el, vg)]
+        L.check(ctx.fine.lib.pom_restrict4_adj(ctx.fine._h, ctx.coarse._h, ctx.three_d, int(ctx.volume), *ptrs,
+                                               g.contiguous().data_ptr(), gf_buf.data_ptr(),
+                                               gel.data_ptr() if gel is not None else None,
+                                               _stream(ctx.fine.device)), "pom_restrict4_adj")
+        return (None,) * 6 + (gf, gel)
 
 
 def copy_arrays(dst, src):

@@ -200,6 +200,48 @@ def test_restriction_conserves_and_matches_brute_force():
         assert rel_err(lhs.cpu(), rhs.cpu()) < 1e-12
 
 
+def _torch_restrict(fine, m, f, geom, el, vg, three_d, mask):
+    """The restriction of member m as torch ops (differentiable reference)."""
+    im, jm = fine.shapes[m]
+    w = fine.view(geom["art"], m, "art") * fine.view(geom[mask], m, mask) * (fine.view(geom["h"], m, "h") + el)
+    if three_d:
+        w = w.unsqueeze(2) * vg[m, 2].view(1, 1, 16)
+    wi, fi = w[1:-1, 1:-1], f[1:-1, 1:-1]
+    shp = ((im - 2) // 4, 4, (jm - 2) // 4, 4) + ((16,) if three_d else ())
+    sw = wi.reshape(shp).sum((1, 3))
+    sf = (wi * fi).reshape(shp).sum((1, 3))
+    inner = torch.where(sw > 0, sf / torch.where(sw > 0, sw, torch.ones_like(sw)), torch.zeros_like(sw))
+    if three_d:   # the coarse edge ring is zero
+        return torch.nn.functional.pad(inner.permute(2, 0, 1), (1, 1, 1, 1)).permute(1, 2, 0)
+    return torch.nn.functional.pad(inner, (1, 1, 1, 1))
+
+
+@pytest.mark.gpu
+def test_restriction_gradient_matches_autograd():
+    """The transpose of the restriction and the cotangent of el through the
+    volume weights, against torch autograd of the same restriction."""
+    print("[gpu] restrict4 adjoint 18x14, 26x10", flush=True)
+    fine, coarse, geom, el, vg = _restrict_case()
+    gen = torch.Generator().manual_seed(12)
+    for name, mask in (("t", "fsm"), ("u", "dum"), ("el", "fsm")):
+        three_d = name in ("t", "u")
+        f = torch.randn(fine.numel(name), generator=gen, dtype=torch.float64).cuda().requires_grad_(True)
+        e = el.clone().requires_grad_(True)
+        out = ST.restrict4(fine, coarse, f, name, geom, e, vg)
+        cot = torch.randn(out.shape, generator=gen, dtype=torch.float64).cuda()
+        gf, ge = torch.autograd.grad((out * cot).sum(), [f, e])
+        f2 = f.detach().clone().requires_grad_(True)
+        e2 = el.clone().requires_grad_(True)
+        loss = 0.0
+        for m in range(len(fine.shapes)):
+            ref = _torch_restrict(fine, m, fine.view(f2, m, name), geom, fine.view(e2, m, "el"), vg, three_d, mask)
+            assert rel_err(coarse.view(out, m, name).detach().cpu(), ref.detach().cpu()) < 1e-13, name
+            loss = loss + (ref * coarse.view(cot, m, name)).sum()
+        rf, re = torch.autograd.grad(loss, [f2, e2])
+        assert rel_err(gf.cpu(), rf.cpu()) < 1e-12, f"{name} field gradient"
+        assert rel_err(ge.cpu(), re.cpu()) < 1e-12, f"{name} el gradient"
+
+
 @pytest.mark.gpu
 def test_restriction_rejects_mismatched_grids():
     fine = _ens([(18, 14)])
