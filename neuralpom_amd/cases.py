@@ -330,3 +330,24 @@ def tidal_channel(im=354, jm=322, seed=0, dx=500.0, hmin=10.0, hmax=60.0, corr=2
                          torch.full_like(js, float(phase))], dim=1).numpy()
     return st, g, torch.stack(vg), period, nodes
 
+
+
+def forecast_geometry(fine, coarse, fine_geom, dx):
+    """Geometry of the 4x coarser forecast grid of the fine members (the
+    354 x 322 -> 90 x 82 pair of the coupled rollout): the depth is the
+    area-weighted restriction of the fine depth over wet cells, land (h = 1)
+    where a 4 x 4 block holds no wet cell and on the edge ring; masks and
+    areas follow from it as for every case here (dx = dy = ``dx``).  Returns
+    flat fields in the coarse batch's layout: h, dx, dy, art, aru, arv, fsm,
+    dum, dvm."""
+    from .state import restrict4
+    hc = restrict4(fine, coarse, fine_geom["h"].detach(), "h", fine_geom, volume=False)
+    out = {n: [] for n in ("h", "dx", "dy", "art", "aru", "arv", "fsm", "dum", "dvm")}
+    for m, (im, jm) in enumerate(coarse.shapes):
+        h = coarse.view(hc, m, "h").clone()
+        h[h <= 1.0] = 1.0
+        g = {"h": h, "dx": torch.full_like(h, float(dx)), "dy": torch.full_like(h, float(dx))}
+        _areas_masks(g)
+        for n in out:
+            out[n].append(g[n])
+    return {n: coarse.pack(v, n) for n, v in out.items()}

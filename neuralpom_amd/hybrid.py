@@ -16,6 +16,13 @@ endpoint corrector applied after each interval.
   src/neuralpom/hybrid/state.py:197-226).
 * ``coupled_rollout``: intervals of physics, each followed by the corrector
   hook (``coupling.apply_joint_correction``, sparse_coupling.py:42-67).
+* ``integrate`` / ``truncated_windows`` / ``restrict_state``: batched members
+  over windows of many steps with the window's loss differentiated back to
+  its start and the state detached between windows (the daily truncated
+  update, src/neuralpom/v1/posttrain/truncated.py:23-33 and
+  rollout.py:57-74), the fine members restricted to the forecast grid for the
+  loss.  A window can be long: runs of ``segment`` steps are recomputed in
+  backward, so the graph holds about steps / segment + segment states.
 """
 from __future__ import annotations
 
@@ -23,8 +30,13 @@ import torch
 import torch.nn as nn
 import torch.nn.functional as F
 
+from torch.utils.checkpoint import checkpoint
+
+from . import _lib as L
 from . import coupling
-from .state import INPUT_CHANNELS, INPUT_PACK_ORDER, pack_input
+from .explicit import ExplicitStep
+from .implicit import ImplicitEnsemble, ImplicitStep
+from .state import INPUT_CHANNELS, INPUT_PACK_ORDER, pack_input, restrict4
 
 
 def _group_norm(channels: int) -> nn.GroupNorm:
@@ -147,3 +159,102 @@ def coupled_rollout(core, net, scales, intervals: int, steps: int, cap: float = 
         raw = net(x.to(next(net.parameters()).dtype))[0].to(core.dtype)
         diags.append(coupling.apply_joint_correction(core, raw, scales, cap))
     return diags
+
+
+def _step(ens, state, geom, vgrid, schedule, it):
+    sc, ramp, internal = schedule(it)
+    ins = [state[n] for n in L.FIELDS]
+    if isinstance(ens, ImplicitEnsemble):
+        outs = ImplicitStep.apply(ens, geom, sc, ramp, vgrid, internal, None, *ins)
+    else:
+        outs = ExplicitStep.apply(ens, geom, sc, ramp, vgrid, internal, *ins)
+    return dict(zip(L.FIELDS, outs[:-1])), outs[-1]
+
+
+def _segment(ens, geom, vgrid, schedule, first, count, *inputs):
+    state = dict(zip(L.FIELDS, inputs))
+    vamax = []
+    for it in range(first, first + count):
+        state, va = _step(ens, state, geom, vgrid, schedule, it)
+        vamax.append(va)
+    return tuple(state[n] for n in L.FIELDS) + (torch.stack(vamax).amax(),)
+
+
+def integrate(ens, state, geom, vgrid, schedule, first: int, steps: int, segment: int = 1):
+    """Internal steps ``first`` .. ``first + steps - 1`` (1-based step numbers)
+    of every member of ``ens`` (an explicit ``Ensemble`` or an
+    ``ImplicitEnsemble``); differentiable in ``state``.
+
+    ``schedule(it)`` gives the step's (scalars, ramp tensor [members],
+    internal flag), as the core's time loop forms them.  Each step's autograd
+    node keeps its input state and recomputes its forward in backward.  With
+    ``segment`` > 1, runs of ``segment`` steps are wrapped in
+    torch.utils.checkpoint: the graph keeps only each run's input state and
+    rebuilds the run's states when its backward comes, so a window of n steps
+    holds about n / segment + segment states instead of n, for one extra
+    forward of the window.  The kernels are deterministic, so the gradients
+    are bitwise those of ``segment`` = 1.
+
+    Returns (end state, max barotropic speed of the steps as a device scalar)."""
+    steps, segment = int(steps), max(1, int(segment))
+    if steps < 1:
+        raise ValueError("steps must be positive")
+    vamax = []
+    it, last = int(first), int(first) + steps
+    while it < last:
+        count = min(segment, last - it)
+        ins = [state[n] for n in L.FIELDS]
+        if count > 1 and torch.is_grad_enabled() and any(t.requires_grad for t in ins):
+            outs = checkpoint(_segment, ens, geom, vgrid, schedule, it, count, *ins, use_reentrant=False)
+        else:
+            outs = _segment(ens, geom, vgrid, schedule, it, count, *ins)
+        state = dict(zip(L.FIELDS, outs[:-1]))
+        vamax.append(outs[-1].detach())
+        it += count
+    return state, torch.stack(vamax).amax()
+
+
+def restrict_state(fine, coarse, state, geom, vgrid, names):
+    """The fine members' fields ``names`` on the coarse members: 3-D fields
+    volume-weighted with the fine surface elevation (differentiable in the
+    field and in el), 2-D fields area-weighted (the 4 x 4 conservative
+    restriction, paper/paper_zh.md:625)."""
+    out = {}
+    for n in names:
+        three_d = n in L.THREE_D
+        out[n] = restrict4(fine, coarse, state[n], n, geom, state["el"] if three_d else None,
+                           vgrid, volume=three_d)
+    return out
+
+
+def truncated_windows(ens, state, geom, vgrid, schedule, windows: int, steps: int, loss_of,
+                      first: int = 1, segment: int = 1, vmaxl: float | None = None):
+    """``windows`` windows of ``steps`` internal steps, each trained on its own
+    (truncated.py:23-33 with daily windows; rollout.py:57-74 detaches the state
+    between them).
+
+    Per window: the start state is detached and made a leaf, the window is
+    integrated (``integrate`` with ``segment``), ``loss_of(end_state, window)``
+    gives a scalar loss whose backward runs at once, accumulating into the
+    ``.grad`` of whatever parameters the loss uses (a corrector's), and the
+    end state, detached, starts the next window.  ``vmaxl``: raise
+    FloatingPointError when a window's barotropic speed exceeds it, as the
+    core's time loop does.
+
+    Returns (losses [windows] as floats, per-window start-state gradients
+    {field: tensor} of the differentiable fields, final state)."""
+    losses, grads = [], []
+    it = int(first)
+    diff = set(L.DIFF_INPUTS)
+    for w in range(int(windows)):
+        start = {n: v.detach().requires_grad_(n in diff) for n, v in state.items()}
+        end, vamax = integrate(ens, start, geom, vgrid, schedule, it, steps, segment)
+        if vmaxl is not None and float(vamax) > vmaxl:
+            raise FloatingPointError(f"Barotropic velocity {float(vamax)} exceeds limit {vmaxl} in window {w}")
+        loss = loss_of(end, w)
+        loss.backward()
+        losses.append(float(loss.detach()))
+        grads.append({n: start[n].grad for n in L.DIFF_INPUTS if start[n].grad is not None})
+        state = {n: v.detach() for n, v in end.items()}
+        it += int(steps)
+    return losses, grads, state
