@@ -200,6 +200,7 @@ def coupled_interval(args, intervals=3, steps=4):
     scales = {"u": 0.02, "v": 0.02, "t": 0.1, "s": 0.02}
     fields = ("u", "v", "t", "s")
     leaves_n = ("t", "s", "u", "v", "el")
+    torch.manual_seed(5)   # the U-Net's convolutions take torch's default initialisation
     net = CorrectorNet(hidden_dim=64, head_std=1e-3, generator=torch.Generator().manual_seed(5)).to(dev)
     gen = torch.Generator().manual_seed(9)
     st0 = case[0]
@@ -280,6 +281,103 @@ def coupled_interval(args, intervals=3, steps=4):
             "cell_steps_per_s": cells / t_hip, "loss": float(loss), "autograd_loss": float(oloss),
             "grad_t_rel_err_vs_autograd": gerr, "grad_params_rel_err_vs_autograd": perr,
             "rel_err_vs_fp64": err64}
+
+
+def truncated_window(args, members=16, runs=((12, 1), (12, 4), (96, (24, 6)))):
+    """One training window of the 10-day hybrid rollout's shape: 16 members of
+    the 354x322x16 open-boundary tidal channel (semi-implicit core, fp32,
+    dti 30 s, eta tolerance 1e-5 relative, seeded random depth 5-80 m, M2-like
+    1 m tide at the west end), ``steps`` steps, the end state restricted 4x4 to
+    the 90x82 forecast grid, the width-64 U-Net and the corrector hook there,
+    MSE against a seeded target, backward to the window's start.  ``runs``:
+    (window steps, segment) pairs, each timed with its peak memory; the
+    segment is the length of the runs of steps recomputed in backward, or a
+    nested pair (runs of 24 steps recomputed as runs of 6)."""
+    from neuralpom_amd import _lib as L
+    from neuralpom_amd import boundary as B
+    from neuralpom_amd import cases
+    from neuralpom_amd import coupling as CP
+    from neuralpom_amd.explicit import Ensemble
+    from neuralpom_amd.hybrid import CorrectorNet, integrate, restrict_state, truncated_windows
+    from neuralpom_amd.implicit import ImplicitEnsemble
+    from neuralpom_amd.implicit_core import POM2K_Core
+    from neuralpom_amd.state import INPUT_CHANNELS, pack_input
+    dtype, dev = torch.float32, torch.device("cuda", 0)
+    fine, coarse_shape = (354, 322), (90, 82)
+    names = ("u", "ub", "v", "vb", "t", "tb", "s", "sb", "el", "et", "ua", "uab", "va", "vab")
+    sc4 = [0.05, 0.04, 0.2, 0.05]
+    cores = []
+    for m in range(members):
+        st, g, vg, period, nodes = cases.tidal_channel(*fine, seed=100 + m, hmin=5.0, hmax=80.0, dtdz=0.05,
+                                                       dtype=dtype, device=dev)
+        c = POM2K_Core(*fine, device=dev, dtype=dtype)
+        c.configure_runtime(dte=1.0, isplit=30)
+        for n, v in {**st, **g}.items():
+            setattr(c, n, v)
+        c.z, c.zz, c.dz, c.dzz = vg
+        c.period, c.horcon, c.double_gyre_wind_stress_pa = period, 0.05, 0.0
+        B.install(c, "implicit", nodes=nodes, speed=360.0 / 12.42, radiation=0.0)
+        c.time0 = c.time = 0.0
+        cores.append(c)
+    ens = ImplicitEnsemble([fine] * members, dtype=dtype, device=dev)
+    g0 = [c._geom() for c in cores]
+    geom = {n: ens.pack([g[n] for g in g0], n) for n in g0[0]}
+    state0 = {n: ens.pack([c._flat(n).detach() for c in cores], n) for n in L.FIELDS}
+    vgr = torch.stack([c._vgrid()[0] for c in cores]).contiguous()
+    core0, sc = cores[0], cores[0].step_scalars()
+    del cores, g0
+
+    def schedule(it):
+        core0.time = core0.dti * it / 86400.0
+        ramp = torch.full((members,), min(core0.time / float(core0.period), 1.0), dtype=dtype, device=dev)
+        return core0._scalars_at_step(sc), ramp, it > 1
+
+    ens.prepare(geom, schedule(1)[0])
+    coarse = Ensemble([coarse_shape] * members, dtype=dtype, device=dev)
+    cgeom = cases.forecast_geometry(ens, coarse, geom, 4 * 500.0)
+    torch.manual_seed(5)   # the U-Net's convolutions take torch's default initialisation
+    net = CorrectorNet(hidden_dim=64, head_std=1e-3, generator=torch.Generator().manual_seed(5)).to(dev)
+    with torch.no_grad():
+        c0 = restrict_state(ens, coarse, state0, geom, vgr, names)
+    gen = torch.Generator().manual_seed(6)
+    target = {n: c0[n] + 0.01 * torch.randn(c0[n].shape, generator=gen).to(dev) for n in ("u", "v", "t", "s")}
+
+    def loss_of(end, w):
+        cs = restrict_state(ens, coarse, end, geom, vgr, names)
+        x = pack_input(coarse, cs).view(members, INPUT_CHANNELS, *coarse_shape)
+        raw = net(x).reshape(-1)
+        outs = CP._JointCorrection.apply(coarse, vgr, sc4, 5.0, [cgeom[n] for n in CP.CORR_GEOM], raw, cs["et"],
+                                         *[cs[n] for n in CP.STATE_FIELDS])
+        got = dict(zip(CP.STATE_FIELDS, outs[:8]))
+        return sum(((got[n] - target[n]) ** 2).mean() / s ** 2 for n, s in zip(("u", "v", "t", "s"), sc4))
+
+    state_bytes = sum(t.numel() * t.element_size() for t in state0.values())
+    out = {"workload": f"{members} x {fine[0]}x{fine[1]}x16 tidal channel, semi-implicit, fp32, dti 30 s, "
+                       f"restriction to {coarse_shape[0]}x{coarse_shape[1]}, U-Net width 64 and corrector hook, "
+                       "backward to the window start", "state_gb": state_bytes / 1e9, "windows": []}
+    cell_step = members * fine[0] * fine[1] * 16
+    with torch.no_grad():
+        integrate(ens, state0, geom, vgr, schedule, 1, 4)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        integrate(ens, state0, geom, vgr, schedule, 1, 12)
+        torch.cuda.synchronize()
+        out["forward_ms_per_step"] = 1e3 * (time.perf_counter() - t0) / 12
+    truncated_windows(ens, state0, geom, vgr, schedule, 1, 5, loss_of, segment=4)   # warm-up
+    for steps, seg in runs:
+        net.zero_grad(set_to_none=True)
+        torch.cuda.synchronize()
+        torch.cuda.reset_peak_memory_stats(dev)
+        base = torch.cuda.memory_allocated(dev)
+        t0 = time.perf_counter()
+        losses, _, _ = truncated_windows(ens, state0, geom, vgr, schedule, 1, steps, loss_of, segment=seg)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out["windows"].append({"steps": steps, "segment": seg, "seconds": dt,
+                               "ms_per_step_fwd_adj": 1e3 * dt / steps,
+                               "cell_steps_per_s": cell_step * steps / dt, "loss": losses[0],
+                               "peak_gb_above_start": (torch.cuda.max_memory_allocated(dev) - base) / 1e9})
+    return out
 
 
 def run_gpu_implicit(args):
@@ -465,6 +563,7 @@ def main():
                            "ms_per_step": 1e3 * r64["seconds"] / args.steps}
         line["launch_bound_case"] = graph_vs_eager(args)
         line["coupled_interval"] = coupled_interval(args)
+        line["truncated_window"] = truncated_window(args)
     if args.implicit_steps > 0:
         line["implicit_core"] = run_gpu_implicit(args)
     if not args.no_baselines:

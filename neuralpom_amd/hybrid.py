@@ -171,16 +171,32 @@ def _step(ens, state, geom, vgrid, schedule, it):
     return dict(zip(L.FIELDS, outs[:-1])), outs[-1]
 
 
-def _segment(ens, geom, vgrid, schedule, first, count, *inputs):
+def _run(ens, geom, vgrid, schedule, first, count, segments, *inputs):
+    """Steps first .. first + count - 1; ``segments`` (outermost first) are
+    the run lengths recomputed in backward, each level inside the one before."""
     state = dict(zip(L.FIELDS, inputs))
     vamax = []
-    for it in range(first, first + count):
-        state, va = _step(ens, state, geom, vgrid, schedule, it)
-        vamax.append(va)
-    return tuple(state[n] for n in L.FIELDS) + (torch.stack(vamax).amax(),)
+    if not segments:
+        for it in range(first, first + count):
+            state, va = _step(ens, state, geom, vgrid, schedule, it)
+            vamax.append(va)
+    else:
+        seg, inner = segments[0], segments[1:]
+        it, last = first, first + count
+        while it < last:
+            n = min(seg, last - it)
+            ins = [state[f] for f in L.FIELDS]
+            if n > 1 and torch.is_grad_enabled() and any(t.requires_grad for t in ins):
+                outs = checkpoint(_run, ens, geom, vgrid, schedule, it, n, inner, *ins, use_reentrant=False)
+            else:
+                outs = _run(ens, geom, vgrid, schedule, it, n, inner, *ins)
+            state = dict(zip(L.FIELDS, outs[:-1]))
+            vamax.append(outs[-1])
+            it += n
+    return tuple(state[f] for f in L.FIELDS) + (torch.stack(vamax).amax(),)
 
 
-def integrate(ens, state, geom, vgrid, schedule, first: int, steps: int, segment: int = 1):
+def integrate(ens, state, geom, vgrid, schedule, first: int, steps: int, segment=1):
     """Internal steps ``first`` .. ``first + steps - 1`` (1-based step numbers)
     of every member of ``ens`` (an explicit ``Ensemble`` or an
     ``ImplicitEnsemble``); differentiable in ``state``.
@@ -188,30 +204,25 @@ def integrate(ens, state, geom, vgrid, schedule, first: int, steps: int, segment
     ``schedule(it)`` gives the step's (scalars, ramp tensor [members],
     internal flag), as the core's time loop forms them.  Each step's autograd
     node keeps its input state and recomputes its forward in backward.  With
-    ``segment`` > 1, runs of ``segment`` steps are wrapped in
+    ``segment`` = s > 1, runs of s steps are wrapped in
     torch.utils.checkpoint: the graph keeps only each run's input state and
     rebuilds the run's states when its backward comes, so a window of n steps
-    holds about n / segment + segment states instead of n, for one extra
-    forward of the window.  The kernels are deterministic, so the gradients
-    are bitwise those of ``segment`` = 1.
+    holds about n / s + s states instead of n, for one extra forward of the
+    window.  A sequence (s1, s2, ...) nests the runs: runs of s1 steps, each
+    recomputed as runs of s2 steps, and so on, about n / s1 + s1 / s2 + ... +
+    s_last states for one extra forward per level.  The kernels are
+    deterministic, so the gradients are bitwise those of ``segment`` = 1.
 
     Returns (end state, max barotropic speed of the steps as a device scalar)."""
-    steps, segment = int(steps), max(1, int(segment))
+    steps = int(steps)
     if steps < 1:
         raise ValueError("steps must be positive")
-    vamax = []
-    it, last = int(first), int(first) + steps
-    while it < last:
-        count = min(segment, last - it)
-        ins = [state[n] for n in L.FIELDS]
-        if count > 1 and torch.is_grad_enabled() and any(t.requires_grad for t in ins):
-            outs = checkpoint(_segment, ens, geom, vgrid, schedule, it, count, *ins, use_reentrant=False)
-        else:
-            outs = _segment(ens, geom, vgrid, schedule, it, count, *ins)
-        state = dict(zip(L.FIELDS, outs[:-1]))
-        vamax.append(outs[-1].detach())
-        it += count
-    return state, torch.stack(vamax).amax()
+    segs = (segment,) if isinstance(segment, int) else tuple(segment)
+    if any(int(x) < 1 for x in segs):
+        raise ValueError("segment lengths must be positive")
+    segs = tuple(int(x) for x in segs if int(x) > 1)
+    outs = _run(ens, geom, vgrid, schedule, int(first), steps, segs, *[state[n] for n in L.FIELDS])
+    return dict(zip(L.FIELDS, outs[:-1])), outs[-1].detach()
 
 
 def restrict_state(fine, coarse, state, geom, vgrid, names):

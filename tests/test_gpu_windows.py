@@ -6,8 +6,9 @@ backward through all of it to the window's start, the state detached between
 windows.  Float64, two members of 42 x 34 (forecast grid 12 x 10), windows of
 six 30 s steps.
 
-* Recomputing runs of three steps in backward (``segment``) gives bitwise the
-  losses and gradients of per-step recompute.
+* Recomputing runs of three steps in backward (``segment``), or runs of four
+  recomputed as runs of two, gives bitwise the losses and gradients of
+  per-step recompute.
 * Detaching between windows: the second window's loss and start gradient are
   bitwise those of a fresh run started from the first window's end state.
 * The start-state gradient of the whole chain (physics, restriction, pack,
@@ -111,22 +112,23 @@ def setup():
 
 
 def test_segment_recompute_is_bitwise(setup):
-    print(f"[gpu] windows: 2 members {FINE[0]} -> {COARSE[0]}, 2 windows of {STEPS} steps, segment 1 vs {SEG}",
+    print(f"[gpu] windows: 2 members {FINE[0]} -> {COARSE[0]}, 2 windows of {STEPS} steps, segment 1 vs {SEG} and (4, 2)",
           flush=True)
-    la, ga, ea, pa = setup.run(2, SEG)
     lb, gb, eb, pb = setup.run(2, 1)
-    print(f"  window losses {la}", flush=True)
-    assert la == lb
-    for w in range(2):
-        assert set(ga[w]) == set(gb[w]) and ga[w]
-        for n in ga[w]:
-            assert torch.equal(ga[w][n], gb[w][n]), (w, n)
-        assert float(ga[w]["t"].abs().max()) > 0.0
-    for a, b in zip(pa, pb):
-        assert torch.equal(a, b)
-    assert any(float(a.abs().max()) > 0.0 for a in pa)
-    for n in L.FIELDS:
-        assert torch.equal(ea[n], eb[n]), n
+    print(f"  window losses {lb}", flush=True)
+    for seg in (SEG, (4, 2)):
+        la, ga, ea, pa = setup.run(2, seg)
+        assert la == lb, seg
+        for w in range(2):
+            assert set(ga[w]) == set(gb[w]) and ga[w]
+            for n in ga[w]:
+                assert torch.equal(ga[w][n], gb[w][n]), (seg, w, n)
+            assert float(ga[w]["t"].abs().max()) > 0.0
+        for a, b in zip(pa, pb):
+            assert torch.equal(a, b), seg
+        assert any(float(a.abs().max()) > 0.0 for a in pa)
+        for n in L.FIELDS:
+            assert torch.equal(ea[n], eb[n]), (seg, n)
 
 
 def test_detach_between_windows(setup):

@@ -35,3 +35,39 @@ def test_parameter_names_follow_the_reference_layout():
     assert net.inc.double_conv[1].num_groups == 32
     assert CorrectorNet(hidden_dim=12).inc.double_conv[1].num_groups == 12
     assert CorrectorNet(hidden_dim=40).inc.double_conv[1].num_groups == 20
+
+
+def _toy_windows(monkeypatch, segment, steps=10):
+    """integrate's run structure with a smooth torch map in place of the HIP
+    step: returns (gradient of the end state's loss at the start, step calls)."""
+    from neuralpom_amd import _lib as L
+    from neuralpom_amd import hybrid
+    calls = []
+
+    def toy_step(ens, state, geom, vgrid, schedule, it):
+        calls.append(it)
+        a = 0.1 * it
+        out = {n: x + a * torch.sin(x) * state["t"].mean() for n, x in state.items()}
+        return out, torch.tensor(float(it))
+
+    monkeypatch.setattr(hybrid, "_step", toy_step)
+    gen = torch.Generator().manual_seed(0)
+    start = {n: torch.randn(3, generator=gen, dtype=torch.float64).requires_grad_(True) for n in L.FIELDS}
+    end, vamax = hybrid.integrate(None, start, None, None, None, 1, steps, segment)
+    assert float(vamax) == steps
+    loss = sum((v ** 2).sum() for v in end.values())
+    g = torch.autograd.grad(loss, [start[n] for n in L.FIELDS])
+    return g, len(calls)
+
+
+def test_integrate_segment_recompute_levels(monkeypatch):
+    """Runs recomputed in one or two nested levels give bitwise the gradient
+    of plain autograd, with at most one extra forward per level (a level's
+    recompute stops once the states it keeps are rebuilt, torch's early stop)."""
+    ref, n0 = _toy_windows(monkeypatch, 1)
+    assert n0 == 10
+    for segment, levels in ((4, 1), ((6, 2), 2), ((4, 3), 2), ((5, 1), 1)):
+        g, n = _toy_windows(monkeypatch, segment)
+        assert 10 < n <= 10 * (1 + levels), (segment, n)
+        for a, b in zip(g, ref):
+            assert torch.equal(a, b), segment
