@@ -413,8 +413,8 @@ __device__ __forceinline__ bool pos3(const Batch& B, Pos& p) {
   return true;
 }
 
-__device__ __forceinline__ bool pos2(const Batch& B, Pos& p) {
-  int b = logical_block();
+__device__ __forceinline__ bool pos2_at(const Batch& B, int b, Pos& p) {
+  if (b >= B.blk2[B.M]) return false;
   int m = member_of(B.blk2, B.M, b);
   Member mm = B.mem[m];
   int64_t local = (int64_t)(b - B.blk2[m]) * TPB + threadIdx.x;
@@ -426,6 +426,8 @@ __device__ __forceinline__ bool pos2(const Batch& B, Pos& p) {
   p.j = (int)(local - (int64_t)p.i * mm.jm);
   return true;
 }
+
+__device__ __forceinline__ bool pos2(const Batch& B, Pos& p) { return pos2_at(B, logical_block(), p); }
 
 __device__ __forceinline__ int clampi(int x, int n) { return x < 0 ? 0 : (x >= n ? n - 1 : x); }
 
