@@ -510,7 +510,13 @@ def main():
                     help="time the fwd+adjoint step replayed from a HIP graph (StepGraph)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="launch one 256 MiB device copy first (reference for HBM counter scaling)")
+    ap.add_argument("--window-only", action="store_true",
+                    help="run only the training-window measurement (16 x 354x322 semi-implicit members) "
+                         "and print its JSON object")
     args = ap.parse_args()
+    if args.window_only:
+        print(json.dumps(truncated_window(args)))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported in this build "
               "(members would shard across GPUs; see DESIGN.md)", file=sys.stderr)
