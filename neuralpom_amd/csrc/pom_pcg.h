@@ -461,4 +461,109 @@ __global__ void __launch_bounds__(TPB) k_spec_gemm(const Member* __restrict__ me
   }
 }
 
+// fp32 products of the DST-I preconditioner on the matrix cores: a 32 x 32
+// tile of C per workgroup of 256 threads.  A and B are staged through LDS 64
+// columns of K at a time; wave w multiplies K sub-steps 16 w .. 16 w + 15 of
+// each stage as 2 x 2 tiles of v_mfma_f32_16x16x4f32 (fp32 products, fp32
+// accumulation), and the four partial tiles are summed through LDS at the end
+// (split K: four times the workgroups of one 64 x 64 tile per workgroup, which
+// left under two waves per SIMD at 352 x 320 and fewer for small members).
+// Operand layout of the instruction (wave64): lane l holds A[l % 16][l / 16]
+// and B[l / 16][l % 16]; result register r of lane l is C[4 (l / 16) + r][l % 16].
+// Same arguments, operands and epilogue as k_spec_gemm; the grid is 32 x 32 tiles.
+typedef float pom_floatx4 __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(TPB) k_spec_gemm_mfma(const Member* __restrict__ mem,
+                                                        const SpecMember* __restrict__ sp,
+                                                        const float* __restrict__ cache, const int* __restrict__ act,
+                                                        GemmArg<float> g, int left) {
+  constexpr int BT = 32, KT = 64;
+  static_assert(TPB == 256 && KT == 16 * (TPB / 64), "four waves, 16 K sub-steps each per stage");
+  __shared__ float As[BT][KT + 1];
+  __shared__ float Bs[KT][BT + 1];
+  __shared__ float Cs[TPB / 64 - 1][BT * BT];
+  const int m = blockIdx.z;
+  if (!act[m]) return;
+  const SpecMember s = sp[m];
+  const Member mm = mem[m];
+  const int M = s.ni, N = s.nj, K = left ? s.ni : s.nj;
+  const int row0 = blockIdx.y * BT, col0 = blockIdx.x * BT;
+  if (row0 >= M || col0 >= N) return;
+  int64_t lda, ldb, ldc;
+  const float* A = gemm_ptr<float>(g.a_kind, g.a_kind < 2 ? cache : g.a, s, mm, lda);
+  const float* Bm = gemm_ptr<float>(g.b_kind, g.b_kind < 2 ? cache : g.b, s, mm, ldb);
+  float* Cm = const_cast<float*>(gemm_ptr<float>(g.c_kind, g.c, s, mm, ldc));
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  pom_floatx4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = pom_floatx4{0.f, 0.f, 0.f, 0.f};
+  // the next stage's operands are loaded into registers while this stage's
+  // products run, and stored to LDS after the stage's closing barrier
+  constexpr int PER = BT * KT / TPB;
+  float ra[PER], rb[PER];
+  auto load_stage = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = threadIdx.x + q * TPB;
+      const int r = e / KT, kk = e % KT;
+      ra[q] = (row0 + r < M && k0 + kk < K) ? A[(int64_t)(row0 + r) * lda + k0 + kk] : 0.f;
+      const int kb = e / BT, cc = e % BT;
+      rb[q] = (k0 + kb < K && col0 + cc < N) ? Bm[(int64_t)(k0 + kb) * ldb + col0 + cc] : 0.f;
+    }
+  };
+  load_stage(0);
+  for (int k0 = 0; k0 < K; k0 += KT) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = threadIdx.x + q * TPB;
+      As[e / KT][e % KT] = ra[q];
+      Bs[e / BT][e % BT] = rb[q];
+    }
+    __syncthreads();
+    if (k0 + KT < K) load_stage(k0 + KT);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int kk = 16 * wave + 4 * t + lk;
+      const float a0 = As[li][kk], a1 = As[16 + li][kk];
+      const float b0 = Bs[kk][li], b1 = Bs[kk][16 + li];
+      acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  if (wave > 0) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Cs[wave - 1][(16 * i + 4 * lk + r) * BT + 16 * j + li] = acc[i][j][r];
+  }
+  __syncthreads();
+  if (wave == 0) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int tr = 16 * i + 4 * lk + r, tc = 16 * j + li;
+          const int row = row0 + tr, col = col0 + tc;
+          if (row < M && col < N) {
+            float v = acc[i][j][r];
+#pragma unroll
+            for (int w = 0; w < TPB / 64 - 1; ++w) v += Cs[w][tr * BT + tc];
+            if (g.mode == 1) v *= cache[s.l + (int64_t)row * N + col];
+            else if (g.mode == 2) v /= float(s.scale);
+            Cm[(int64_t)row * ldc + col] = v;
+          }
+        }
+  }
+}
+
 }  // namespace pom

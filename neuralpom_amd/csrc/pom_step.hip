@@ -11,6 +11,7 @@
 #include <vector>
 #include <cstring>
 #include <algorithm>
+#include <type_traits>
 
 #include "../../include/neuralpom_amd.h"
 #include "pom_kernels.h"
@@ -158,15 +159,23 @@ template <class T> struct Pcg {
   void precond(const Ctx<T>& c, const EllGeom<T>& g, const T* d) {
     const dim3 g2(c.nblk2), tb(TPB);
     if (spectral) {
-      const dim3 gg((maxnj + 15) / 16, (maxni + 15) / 16, c.B.M);
       GemmArg<T> a1{nullptr, r, t1, 0, 2, 3, 0};    // T1 = S_ni R
-      hipLaunchKernelGGL(k_spec_gemm<T>, gg, tb, 0, c.stream, c.B.mem, sp, cache, act, a1, 1);
       GemmArg<T> a2{t1, nullptr, t2, 3, 1, 3, 1};   // T2 = (T1 S_nj) * L
-      hipLaunchKernelGGL(k_spec_gemm<T>, gg, tb, 0, c.stream, c.B.mem, sp, cache, act, a2, 0);
       GemmArg<T> a3{nullptr, t2, t1, 0, 3, 3, 0};   // T1 = S_ni T2
-      hipLaunchKernelGGL(k_spec_gemm<T>, gg, tb, 0, c.stream, c.B.mem, sp, cache, act, a3, 1);
       GemmArg<T> a4{t1, nullptr, z, 3, 1, 2, 2};    // Z = T1 S_nj / scale (interior)
-      hipLaunchKernelGGL(k_spec_gemm<T>, gg, tb, 0, c.stream, c.B.mem, sp, cache, act, a4, 0);
+      if constexpr (std::is_same<T, float>::value) {
+        const dim3 gg((maxnj + 31) / 32, (maxni + 31) / 32, c.B.M);
+        hipLaunchKernelGGL(k_spec_gemm_mfma, gg, tb, 0, c.stream, c.B.mem, sp, cache, act, a1, 1);
+        hipLaunchKernelGGL(k_spec_gemm_mfma, gg, tb, 0, c.stream, c.B.mem, sp, cache, act, a2, 0);
+        hipLaunchKernelGGL(k_spec_gemm_mfma, gg, tb, 0, c.stream, c.B.mem, sp, cache, act, a3, 1);
+        hipLaunchKernelGGL(k_spec_gemm_mfma, gg, tb, 0, c.stream, c.B.mem, sp, cache, act, a4, 0);
+      } else {
+        const dim3 gg((maxnj + 15) / 16, (maxni + 15) / 16, c.B.M);
+        hipLaunchKernelGGL(k_spec_gemm<T>, gg, tb, 0, c.stream, c.B.mem, sp, cache, act, a1, 1);
+        hipLaunchKernelGGL(k_spec_gemm<T>, gg, tb, 0, c.stream, c.B.mem, sp, cache, act, a2, 0);
+        hipLaunchKernelGGL(k_spec_gemm<T>, gg, tb, 0, c.stream, c.B.mem, sp, cache, act, a3, 1);
+        hipLaunchKernelGGL(k_spec_gemm<T>, gg, tb, 0, c.stream, c.B.mem, sp, cache, act, a4, 0);
+      }
       hipLaunchKernelGGL(k_pcg_dot<T>, g2, tb, 0, c.stream, c.B, r, z, act, part, 0);
     } else {
       hipLaunchKernelGGL(k_pcg_jacobi<T>, g2, tb, 0, c.stream, c.B, g, d, r, z, act, part);
