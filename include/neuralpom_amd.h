@@ -107,6 +107,13 @@ typedef struct {
   int32_t core, eta_max_iter, eta_precond, boundary;
   double eta_tol, eta_rel_tol, gamma_baro;
   double tide_wt;   /* tide policy: (speed_deg_per_h * pi / 180) * t_s / 3600 of the step */
+  /* eta_capture = 1: the eta solves launch exactly eta_max_iter iterations and
+   * never synchronise with the host (the reference's capture-safe solver,
+   * implicit_p0_fixed/core.py:504-555, runtime_cuda_graph_fast_path); a member
+   * that has converged keeps its iterate (its kernels are no-ops), so results
+   * equal the default mode's.  0: the host reads the device's count of
+   * unconverged members every few iterations and stops when it is zero. */
+  int32_t eta_capture, reserved0;
 } pom_params_t;
 
 typedef struct pom_batch pom_batch;

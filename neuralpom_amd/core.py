@@ -224,8 +224,10 @@ class POM2K_Core:
 
         Sets ``iint``, ``time`` and ``ramp`` as the reference does.  When the
         depth-mean velocity bound ``vmaxl`` is exceeded it prints
-        "Abnormal end" and returns before ``iend`` (time_loop.py:702-709);
-        ``runtime_vamax_safe_fast_path`` skips that per-step check.
+        "Abnormal end" and returns before ``iend`` (time_loop.py:702-709).
+        As there, the check belongs to the diagnostics branch that
+        ``generation_fast_path`` / ``runtime_no_grad_fast_path`` skip
+        (time_loop.py:30-34, 702): with either flag the loop runs to ``iend``.
         """
         self._before_time_loop()
         ens = self._ensemble()
@@ -267,7 +269,9 @@ class POM2K_Core:
         pass
 
     def _vamax_checked(self) -> bool:
-        return not getattr(self, "runtime_vamax_safe_fast_path", False)
+        """time_loop.py:30-34, 702: only the path with diagnostics checks vamax."""
+        fast = getattr(self, "generation_fast_path", False) or getattr(self, "runtime_no_grad_fast_path", False)
+        return not fast
 
     def _vamax_exceeded(self, vamax: float) -> None:
         """time_loop.py:702-709: print and return before iend."""

@@ -9,6 +9,7 @@
 // gathers of pom_kernels.h, column stages through the hand-written reverse
 // sweeps of pom_column.h / pom_profq.h.
 #include <vector>
+#include <mutex>
 #include <cstring>
 #include <algorithm>
 #include <type_traits>
@@ -129,12 +130,30 @@ template <class T> struct X {   // state of one barotropic substep
       *tps;
 };
 
-// pinned host word for the eta solver's convergence polls (one per process)
-int* pinned_word() {
-  static int* w = nullptr;
-  if (!w && hipHostMalloc(reinterpret_cast<void**>(&w), sizeof(int)) != hipSuccess) w = nullptr;
-  return w;
-}
+// Pinned host words for the eta solver's convergence polls: one per solve in
+// flight, taken from a pool under a lock, so solves running at the same time
+// (other streams, other host threads) never share one.  A word goes back to
+// the pool when its solve returns; by then its last copy has completed (the
+// solve synchronised on it).  Words live until the process ends.
+std::mutex g_pin_mu;
+std::vector<int*> g_pin_free;
+struct PinnedWord {
+  int* w = nullptr;
+  hipError_t acquire() {
+    std::lock_guard<std::mutex> lk(g_pin_mu);
+    if (!g_pin_free.empty()) {
+      w = g_pin_free.back();
+      g_pin_free.pop_back();
+      return hipSuccess;
+    }
+    return hipHostMalloc(reinterpret_cast<void**>(&w), sizeof(int));
+  }
+  ~PinnedWord() {
+    if (!w) return;
+    std::lock_guard<std::mutex> lk(g_pin_mu);
+    g_pin_free.push_back(w);
+  }
+};
 
 // work arrays and driver of the batched eta solve
 template <class T> struct Pcg {
@@ -145,6 +164,8 @@ template <class T> struct Pcg {
   const T* cache = nullptr;
   int maxni = 0, maxnj = 0;
   bool spectral = false;
+  bool capture = false;             // no host polls: a fixed launch count (capture-safe)
+  int status = POM_OK;              // POM_E_HIP after a failed poll
   static constexpr int POLL = 4;    // iterations between convergence polls
 
   void plan(Arena& A, int64_t N2, int nblk2, int M) {
@@ -184,10 +205,20 @@ template <class T> struct Pcg {
 
   // x = A(d)^-1 rhs from x0 (null: zero), per-member absolute tolerance tol[]
   // (device, set by the caller) and relative tolerance rel_tol.  The edges and
-  // the points g.fixed marks keep the lift (null: zero).
+  // the points g.fixed marks keep the lift (null: zero).  Without capture the
+  // host reads the count of unconverged members every POLL iterations and
+  // stops at zero; a failed poll sets status to POM_E_HIP and abandons the
+  // solve.  With capture all max_iter iterations are launched and converged
+  // members' kernels do nothing, so x is the same either way.
   void solve(const Ctx<T>& c, const EllGeom<T>& g, const T* rhs, const T* d, const T* x0, T* x, double rel_tol,
              int max_iter, const T* lift = nullptr) {
     const dim3 g2(c.nblk2), tb(TPB), gm(c.B.M);
+    PinnedWord pin;
+    if (!capture && pin.acquire() != hipSuccess) {
+      status = POM_E_HIP;
+      return;
+    }
+    int* hw = pin.w;
     hipLaunchKernelGGL(k_pcg_x0<T>, g2, tb, 0, c.stream, c.B, x0, g.fixed, lift, x, z);
     hipLaunchKernelGGL(k_pcg_resid<T>, g2, tb, 0, c.stream, c.B, g, d, x, rhs, r, p);
     hipLaunchKernelGGL(k_pcg_act1, dim3(1), tb, 0, c.stream, act, c.B.M);
@@ -195,15 +226,17 @@ template <class T> struct Pcg {
     hipLaunchKernelGGL(k_pcg_dot<T>, g2, tb, 0, c.stream, c.B, r, r, act, part, 1);
     hipLaunchKernelGGL(k_pcg_start, gm, tb, 0, c.stream, c.B, part, sc, act, its, tol, rel_tol);
     hipLaunchKernelGGL(k_pcg_p<T>, g2, tb, 0, c.stream, c.B, p, z, g.fixed, sc, act, 1);
-    int* hw = pinned_word();
     for (int it = 1; it <= max_iter; ++it) {
       hipLaunchKernelGGL(k_pcg_ap<T>, g2, tb, 0, c.stream, c.B, g, d, p, ap, act, part);
       hipLaunchKernelGGL(k_pcg_alpha, gm, tb, 0, c.stream, c.B, part, sc, act, nact);
       hipLaunchKernelGGL(k_pcg_xr<T>, g2, tb, 0, c.stream, c.B, x, r, p, ap, sc, act, part);
       hipLaunchKernelGGL(k_pcg_conv, gm, tb, 0, c.stream, c.B, part, sc, act, its, nact);
-      if (hw && (it % POLL == 0 || it == max_iter)) {
-        if (hipMemcpyAsync(hw, nact, sizeof(int), hipMemcpyDeviceToHost, c.stream) != hipSuccess) return;
-        if (hipStreamSynchronize(c.stream) != hipSuccess) return;
+      if (!capture && (it % POLL == 0 || it == max_iter)) {
+        if (hipMemcpyAsync(hw, nact, sizeof(int), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
+            hipStreamSynchronize(c.stream) != hipSuccess) {
+          status = POM_E_HIP;
+          return;
+        }
         if (*hw == 0) break;
       }
       if (it == max_iter) break;
@@ -869,6 +902,7 @@ template <class T> struct Step {
     launch_map<T, OpExtPrep>(c, t_prep(), OutTab<T>{{out[POM_ADX2D], out[POM_ADY2D], egf0, utf0, vtf0}});
     if (imp) {
       external_implicit();
+      if (pw.status != POM_OK) return;
     } else {
       external_explicit();
     }
@@ -886,6 +920,7 @@ template <class T> struct Step {
                          g[POM_G_TIDE_AMP], g[POM_G_TIDE_PHASE], tide_wt, lift);
     hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(256), 0, c.stream, pw.tol, (int64_t)b->M, hp->eta_tol);
     pw.solve(c, ell(), rhs, in[POM_D], in[POM_ELF], elfraw, hp->eta_rel_tol, hp->eta_max_iter, lift);
+    if (pw.status != POM_OK) return;
     if (iters_out)
       hipLaunchKernelGGL(k_copy<int32_t>, dim3(1), dim3(256), 0, c.stream, iters_out, pw.its, (int64_t)b->M);
     if (open) {
@@ -1286,6 +1321,7 @@ template <class T> struct Adj {
       hipLaunchKernelGGL(k_eta_adj_scale, gm, tb, 0, c.stream, c.B, w.part, w.sc, w.tol, S.hp->eta_tol);
       hipLaunchKernelGGL(k_eta_adj_norm<T>, g2, tb, 0, c.stream, c.B, arhs, w.sc);
       w.solve(c, S.ell(), arhs, S.in[POM_D], nullptr, xadj, S.hp->eta_rel_tol, S.hp->eta_max_iter);
+      if (w.status != POM_OK) return;
       hipLaunchKernelGGL(k_eta_adj_out<T>, g2, tb, 0, c.stream, c.B, xadj, S.g[POM_G_ART], w.sc, b_rhs, nlam);
       COutTab<T> ob{{nlam}};
       T* ib[OpEll::NIN] = {nullptr, bi[POM_D]};
@@ -1579,6 +1615,8 @@ int prepare_cache(const pom_batch* b, const pom_params_t* prm, const void* const
 // preconditioner choice and spectral cache of a PCG driver
 template <class T> void setup_pcg(Pcg<T>& w, const pom_batch* b, const pom_params_t* prm, const void* cache) {
   w.spectral = prm->eta_precond == POM_PRECOND_SPECTRAL;
+  w.capture = prm->eta_capture != 0;
+  w.status = POM_OK;
   w.maxni = w.maxnj = 0;
   for (const Member& m : b->h_mem) {
     w.maxni = std::max(w.maxni, m.im - 2);
@@ -1642,6 +1680,7 @@ int run_step(const pom_batch* b, const pom_params_t* prm, const void* ramp, cons
   S.plan(A, false);
   if (A.off > wsb) return POM_E_ARG;
   S.forward();
+  if (S.pw.status != POM_OK) return S.pw.status;
   return hipGetLastError() == hipSuccess ? POM_OK : POM_E_HIP;
 }
 
@@ -1670,7 +1709,9 @@ int run_adj(const pom_batch* b, const pom_params_t* prm, const void* ramp, const
   J.plan(A, reinterpret_cast<T* const*>(grad));
   if (A.off > wsb) return POM_E_ARG;
   S.forward();
+  if (S.pw.status != POM_OK) return S.pw.status;
   J.run(reinterpret_cast<const T* const*>(cot), reinterpret_cast<T* const*>(grad));
+  if (S.pw.status != POM_OK) return S.pw.status;
   return hipGetLastError() == hipSuccess ? POM_OK : POM_E_HIP;
 }
 
@@ -1892,6 +1933,7 @@ int run_eta_solve(const pom_batch* b, const pom_params_t* prm, const void* cache
   e.w.solve(c, ell_geom<T>(c, geom, prm), static_cast<const T*>(rhs), static_cast<const T*>(d),
             static_cast<const T*>(x0), static_cast<T*>(x), prm->eta_rel_tol, prm->eta_max_iter,
             tide ? e.lift : nullptr);
+  if (e.w.status != POM_OK) return e.w.status;
   if (iters) hipLaunchKernelGGL(k_copy<int32_t>, dim3(1), dim3(256), 0, st, iters, e.w.its, (int64_t)b->M);
   return hipGetLastError() == hipSuccess ? POM_OK : POM_E_HIP;
 }
@@ -1916,6 +1958,7 @@ int run_eta_solve_adj(const pom_batch* b, const pom_params_t* prm, const void* c
   hipLaunchKernelGGL(k_eta_adj_norm<T>, g2, tb, 0, st, c.B, e.a, e.w.sc);
   e.w.solve(c, ell_geom<T>(c, geom, prm), e.a, static_cast<const T*>(d), nullptr, e.x, prm->eta_rel_tol,
             prm->eta_max_iter);
+  if (e.w.status != POM_OK) return e.w.status;
   hipLaunchKernelGGL(k_eta_adj_out<T>, g2, tb, 0, st, c.B, e.x, art, e.w.sc, static_cast<T*>(b_rhs), e.nlam);
   if (iters) hipLaunchKernelGGL(k_copy<int32_t>, dim3(1), dim3(256), 0, st, iters, e.w.its, (int64_t)b->M);
   if (b_d) {

@@ -132,15 +132,18 @@ class Ensemble:
         """Concatenate member tensors of one field into the packed layout."""
         return torch.cat([t.reshape(-1).to(self.device, self.dtype) for t in per_member]).contiguous()
 
-    def _workspace(self, prm: L.Params, adjoint: bool):
+    def _workspace(self, prm: L.Params, adjoint: bool, pool=None):
+        """Scratch of one call, taken from ``pool`` (default: the ensemble's own).
+        A larger request replaces the pool's buffer; a caller that recorded
+        launches on the old one (a HIP graph) passes a pool of its own."""
         n = C.c_size_t()
         L.check(self.lib.pom_explicit_workspace(self._h, C.byref(prm), int(adjoint), C.byref(n)),
                 "pom_explicit_workspace")
-        key = (adjoint, n.value)
-        ws = self._ws.get(adjoint)
+        pool = self._ws if pool is None else pool
+        ws = pool.get(adjoint)
         if ws is None or ws.numel() < n.value:
             ws = torch.empty(n.value, dtype=torch.uint8, device=self.device)
-            self._ws[adjoint] = ws
+            pool[adjoint] = ws
         return ws, n.value
 
     def _check(self, tensors, names, nel):
@@ -150,7 +153,7 @@ class Ensemble:
                                  f"with {nel(n)} elements, got {t.dtype} {t.device} {tuple(t.shape)}")
 
     # ------------------------------------------------------------ step
-    def step(self, state, geom, sc: StepScalars, ramp, vgrid, internal=True):
+    def step(self, state, geom, sc: StepScalars, ramp, vgrid, internal=True, ws_pool=None):
         """One internal step; returns (new state dict, vamax tensor)."""
         prm = sc.struct()
         ins = [state[n] for n in L.FIELDS]
@@ -159,7 +162,7 @@ class Ensemble:
         self._check(gs, L.GEOM, self.numel)
         out = {n: torch.empty_like(state[n]) for n in L.FIELDS}
         vamax = torch.zeros(1, dtype=self.dtype, device=self.device)
-        ws, nb = self._workspace(prm, False)
+        ws, nb = self._workspace(prm, False, ws_pool)
         stream = torch.cuda.current_stream(self.device).cuda_stream
         rc = self.lib.pom_explicit_step(
             self._h, C.byref(prm), ramp.data_ptr(), vgrid.data_ptr(),
@@ -169,7 +172,7 @@ class Ensemble:
         L.check(rc, "pom_explicit_step")
         return out, vamax
 
-    def step_adj(self, state, geom, sc: StepScalars, ramp, vgrid, internal, cot):
+    def step_adj(self, state, geom, sc: StepScalars, ramp, vgrid, internal, cot, ws_pool=None):
         """Gradients of the step inputs for output cotangents ``cot`` (dict, may omit fields)."""
         prm = sc.struct()
         ins = [state[n] for n in L.FIELDS]
@@ -188,7 +191,7 @@ class Ensemble:
             else:
                 cot_ptrs.append(None)
         grad_ptrs = [grads[n].data_ptr() if n in grads else None for n in L.FIELDS]
-        ws, nb = self._workspace(prm, True)
+        ws, nb = self._workspace(prm, True, ws_pool)
         stream = torch.cuda.current_stream(self.device).cuda_stream
         rc = self.lib.pom_explicit_step_adj(
             self._h, C.byref(prm), ramp.data_ptr(), vgrid.data_ptr(),
@@ -235,12 +238,28 @@ class StepGraph:
     cotangents of the step it just took in ``grads``.  Refill ``ramp`` (and
     ``cot``) between replays to change them.  The step launches ~100 kernels
     per barotropic substep cycle and has no host synchronisation, so a replay
-    submits the whole step with one call.  Only the explicit core: the
-    semi-implicit eta solve polls its convergence counter from the host.
+    submits the whole step with one call.  A semi-implicit ensemble
+    (``ImplicitEnsemble``) is recorded the same way with ``eta_capture`` set
+    in its scalars: its eta solves then launch a fixed number of iterations
+    and never synchronise with the host (converged members do nothing, so the
+    replay equals the eager step).
+
+    The graph owns its workspaces (allocated on the current stream before the
+    warm-up and capture, and held for the graph's lifetime), so later eager
+    calls on the same ensemble, whatever scratch they need, never touch the
+    memory the recorded kernels use.
     """
 
     def __init__(self, ens, state, geom, sc, ramp, vgrid, internal=True, cot=None, warmup=1):
+        if hasattr(sc, "eta_capture") and not sc.eta_capture:
+            raise ValueError("a semi-implicit step is recorded with eta_capture=True "
+                             "(its eta solves otherwise poll the host for convergence)")
         self.ens = ens
+        self._ws = {}
+        prm = sc.struct()
+        ens._workspace(prm, False, self._ws)
+        if cot is not None:
+            ens._workspace(prm, True, self._ws)
         self.state = {n: state[n].clone() for n in L.FIELDS}
         self.geom, self.sc, self.vgrid, self.internal = geom, sc, vgrid, internal
         self.ramp = ramp.clone()
@@ -257,10 +276,12 @@ class StepGraph:
             self.grads, self.vamax = self._body(self.state)
 
     def _body(self, st):
-        out, vamax = self.ens.step(st, self.geom, self.sc, self.ramp, self.vgrid, self.internal)
+        out, vamax = self.ens.step(st, self.geom, self.sc, self.ramp, self.vgrid, self.internal,
+                                   ws_pool=self._ws)
         grads = None
         if self.cot is not None:
-            grads = self.ens.step_adj(st, self.geom, self.sc, self.ramp, self.vgrid, self.internal, dict(self.cot))
+            grads = self.ens.step_adj(st, self.geom, self.sc, self.ramp, self.vgrid, self.internal,
+                                      dict(self.cot), ws_pool=self._ws)
         for n in L.FIELDS:
             st[n].copy_(out[n])
         return grads, vamax

@@ -260,7 +260,7 @@ def truncated_windows(ens, state, geom, vgrid, schedule, windows: int, steps: in
     for w in range(int(windows)):
         start = {n: v.detach().requires_grad_(n in diff) for n, v in state.items()}
         end, vamax = integrate(ens, start, geom, vgrid, schedule, it, steps, segment)
-        if vmaxl is not None and float(vamax) > vmaxl:
+        if vmaxl is not None and not float(vamax) <= vmaxl:   # NaN-safe
             raise FloatingPointError(f"Barotropic velocity {float(vamax)} exceeds limit {vmaxl} in window {w}")
         loss = loss_of(end, w)
         loss.backward()

@@ -36,6 +36,9 @@ class ImplicitScalars(StepScalars):
     barotropic_shapiro: float = 0.0
     boundary: int = L.POM_BOUNDARY_CLOSED   # or POM_BOUNDARY_TIDE (open edges, tide nodes)
     tide_wt: float = 0.0                    # (speed pi / 180) t / 3600 of the tide this step
+    # capture-safe eta solves (core.py:504-555): exactly eta_max_iter iterations
+    # are launched, no host synchronisation; converged members do nothing
+    eta_capture: bool = False
 
     def struct(self) -> L.Params:
         p = super().struct()
@@ -50,6 +53,7 @@ class ImplicitScalars(StepScalars):
         p.gamma_baro = float(self.barotropic_shapiro)
         p.boundary = int(self.boundary)
         p.tide_wt = float(self.tide_wt)
+        p.eta_capture = int(bool(self.eta_capture))
         return p
 
 
@@ -72,18 +76,20 @@ class ImplicitEnsemble(Ensemble):
                 "pom_implicit_prepare")
         self._prepared = True
 
-    def _ws_implicit(self, prm, adjoint):
+    def _workspace(self, prm, adjoint, pool=None):
+        """Scratch of one semi-implicit call from ``pool`` (default: the ensemble's own)."""
         n = C.c_size_t()
         L.check(self.lib.pom_implicit_workspace(self._h, C.byref(prm), int(adjoint), C.byref(n)),
                 "pom_implicit_workspace")
+        pool = self._ws if pool is None else pool
         key = ("imp", adjoint)
-        ws = self._ws.get(key)
+        ws = pool.get(key)
         if ws is None or ws.numel() < n.value:
             ws = torch.empty(n.value, dtype=torch.uint8, device=self.device)
-            self._ws[key] = ws
+            pool[key] = ws
         return ws, n.value
 
-    def step(self, state, geom, sc: ImplicitScalars, ramp, vgrid, internal=True, iters=None):
+    def step(self, state, geom, sc: ImplicitScalars, ramp, vgrid, internal=True, iters=None, ws_pool=None):
         """One internal step; returns (new state dict, vamax).  ``iters``: optional
         int32 device tensor [members] receiving the eta-solve iteration counts."""
         if not self._prepared:
@@ -94,7 +100,7 @@ class ImplicitEnsemble(Ensemble):
         gp = self._geom_ptrs(geom)
         out = {n: torch.empty_like(state[n]) for n in L.FIELDS}
         vamax = torch.zeros(1, dtype=self.dtype, device=self.device)
-        ws, nb = self._ws_implicit(prm, False)
+        ws, nb = self._workspace(prm, False, ws_pool)
         stream = torch.cuda.current_stream(self.device).cuda_stream
         rc = self.lib.pom_implicit_step(
             self._h, C.byref(prm), self.cache.data_ptr(), ramp.data_ptr(), vgrid.data_ptr(),
@@ -104,7 +110,7 @@ class ImplicitEnsemble(Ensemble):
         L.check(rc, "pom_implicit_step")
         return out, vamax
 
-    def step_adj(self, state, geom, sc: ImplicitScalars, ramp, vgrid, internal, cot):
+    def step_adj(self, state, geom, sc: ImplicitScalars, ramp, vgrid, internal, cot, ws_pool=None):
         """Gradients of the step inputs for output cotangents ``cot`` (dict, may omit fields)."""
         if not self._prepared:
             raise RuntimeError("ImplicitEnsemble.prepare(geom, scalars) must run before the first step")
@@ -125,7 +131,7 @@ class ImplicitEnsemble(Ensemble):
             else:
                 cot_ptrs.append(None)
         grad_ptrs = [grads[n].data_ptr() if n in grads else None for n in L.FIELDS]
-        ws, nb = self._ws_implicit(prm, True)
+        ws, nb = self._workspace(prm, True, ws_pool)
         stream = torch.cuda.current_stream(self.device).cuda_stream
         rc = self.lib.pom_implicit_step_adj(
             self._h, C.byref(prm), self.cache.data_ptr(), ramp.data_ptr(), vgrid.data_ptr(),

@@ -2,8 +2,10 @@
 reference (every correction zero until trained), the seeded head, and the
 module / parameter names of src/neuralpom/models/corrector_net.py, so that a
 state_dict of the reference's network loads into this one."""
+import pytest
 import torch
 
+from neuralpom_amd import _lib as L
 from neuralpom_amd.hybrid import CorrectorNet
 
 
@@ -71,3 +73,22 @@ def test_integrate_segment_recompute_levels(monkeypatch):
         assert 10 < n <= 10 * (1 + levels), (segment, n)
         for a, b in zip(g, ref):
             assert torch.equal(a, b), segment
+
+
+@pytest.mark.parametrize("bad", [float("nan"), 1.0e3])
+def test_truncated_windows_rejects_nan_or_large_vamax(monkeypatch, bad):
+    """A diverged window (vamax NaN or above vmaxl) raises before its loss is
+    back-propagated, as the core's time loop does."""
+    from neuralpom_amd import hybrid
+
+    def fake_integrate(ens, start, geom, vgrid, schedule, it, steps, segment):
+        return start, torch.tensor(bad)
+
+    monkeypatch.setattr(hybrid, "integrate", fake_integrate)
+    state = {n: torch.zeros(4) for n in L.FIELDS}
+    called = []
+    with pytest.raises(FloatingPointError):
+        hybrid.truncated_windows(None, state, {}, None, None, 2, 3,
+                                 lambda end, w: called.append(w) or torch.zeros((), requires_grad=True),
+                                 vmaxl=100.0)
+    assert not called
