@@ -28,6 +28,7 @@ extern "C" {
 #define POM_E_HIP 2
 #define POM_E_UNSUPPORTED 3
 #define POM_E_VMAX 4
+#define POM_E_SCRATCH 5   /* input larger than an entry point's static scratch */
 
 #define POM_F32 0
 #define POM_F64 1
@@ -221,6 +222,22 @@ int pom_eta_solve(const pom_batch* b, const pom_params_t* prm, const void* cache
 int pom_eta_solve_adj(const pom_batch* b, const pom_params_t* prm, const void* cache, const void* const* geom,
                       const void* x, const void* d, const void* cot_x, void* grad_rhs, void* grad_d,
                       int32_t* iters_out, void* workspace, size_t ws_bytes, void* stream);
+
+/* pom_eta_solve_noalloc: pom_eta_solve without a workspace argument and
+ *   without any allocation (no hipMalloc / hipFree / pinned host memory, no
+ *   host synchronisation), for callers that cannot allocate: code under HIP
+ *   graph capture, tight serving loops.  No counterpart in the reference.
+ *   The work arrays live in a static device array of POM_NOALLOC_SCRATCH_BYTES
+ *   per GPU, held until the process ends; a batch whose eta workspace
+ *   (pom_eta_workspace) exceeds it returns POM_E_SCRATCH.  The solve runs in
+ *   the capture-safe mode (exactly prm->eta_max_iter iterations launched;
+ *   converged members keep their iterate), so x equals pom_eta_solve's.
+ *   Every call on one GPU shares that GPU's array: calls that overlap in time
+ *   on one GPU (other streams, other host threads) race and must be
+ *   serialised by the caller. */
+#define POM_NOALLOC_SCRATCH_BYTES (64u << 20)
+int pom_eta_solve_noalloc(const pom_batch* b, const pom_params_t* prm, const void* cache, const void* const* geom,
+                          const void* rhs, const void* d, const void* x0, void* x, int32_t* iters_out, void* stream);
 
 /* Operator-level entry points: the reference's operator methods that callers
  * invoke outside the time loop (src/neuralpom/hybrid/sparse_coupling.py:10-15

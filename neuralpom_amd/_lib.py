@@ -11,7 +11,8 @@ from pathlib import Path
 
 _LIB_PATH = Path(__file__).resolve().parent / "libneuralpom_amd.so"
 
-POM_OK, POM_E_ARG, POM_E_HIP, POM_E_UNSUPPORTED, POM_E_VMAX = 0, 1, 2, 3, 4
+POM_OK, POM_E_ARG, POM_E_HIP, POM_E_UNSUPPORTED, POM_E_VMAX, POM_E_SCRATCH = 0, 1, 2, 3, 4, 5
+NOALLOC_SCRATCH_BYTES = 64 << 20   # POM_NOALLOC_SCRATCH_BYTES
 POM_F32, POM_F64 = 0, 1
 POM_CORE_EXPLICIT, POM_CORE_IMPLICIT = 0, 1
 POM_PRECOND_JACOBI, POM_PRECOND_SPECTRAL = 0, 1
@@ -96,6 +97,7 @@ def load():
     lib.pom_eta_workspace.argtypes = [vp, C.POINTER(C.c_size_t)]
     lib.pom_eta_solve.argtypes = [vp, C.POINTER(Params), vp, vpp, vp, vp, vp, vp, vp, vp, C.c_size_t, vp]
     lib.pom_eta_solve_adj.argtypes = [vp, C.POINTER(Params), vp, vpp, vp, vp, vp, vp, vp, vp, vp, C.c_size_t, vp]
+    lib.pom_eta_solve_noalloc.argtypes = [vp, C.POINTER(Params), vp, vpp, vp, vp, vp, vp, vp, vp]
     lib.pom_pack_input.argtypes = [vp, vpp, vpp, vpp, vp, vp]
     lib.pom_pack_input_adj.argtypes = [vp, vpp, vp, vpp, vp]
     lib.pom_restrict4.argtypes = [vp, vp, C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp, vp, vp]
@@ -119,7 +121,7 @@ EXPORTED = ("pom_batch_create", "pom_batch_destroy", "pom_explicit_workspace", "
             "pom_explicit_step_adj", "pom_implicit_cache_bytes", "pom_implicit_prepare", "pom_implicit_workspace",
             "pom_implicit_step", "pom_implicit_step_adj", "pom_dens", "pom_dens_adj", "pom_baropg",
             "pom_baropg_adj", "pom_elliptic_apply", "pom_elliptic_apply_adj", "pom_eta_workspace", "pom_eta_solve",
-            "pom_eta_solve_adj", "pom_pack_input", "pom_pack_input_adj", "pom_restrict4", "pom_restrict4_adj", "pom_copy_arrays",
+            "pom_eta_solve_adj", "pom_eta_solve_noalloc", "pom_pack_input", "pom_pack_input_adj", "pom_restrict4", "pom_restrict4_adj", "pom_copy_arrays",
             "pom_bcond", "pom_bcond_adj",
             "pom_correction_workspace", "pom_joint_correction", "pom_joint_correction_adj",
             "pom_profile_stage", "pom_profile_read")
@@ -144,8 +146,10 @@ def check(rc: int, what: str):
     if rc == POM_OK:
         return
     msg = {POM_E_ARG: "invalid argument", POM_E_HIP: "HIP runtime error",
-           POM_E_UNSUPPORTED: "unsupported configuration"}.get(rc, f"error {rc}")
-    if rc == POM_E_ARG:
+           POM_E_UNSUPPORTED: "unsupported configuration",
+           POM_E_SCRATCH: f"the batch needs more than the {NOALLOC_SCRATCH_BYTES >> 20} MiB static scratch "
+                          "of this entry point"}.get(rc, f"error {rc}")
+    if rc in (POM_E_ARG, POM_E_SCRATCH):
         raise ValueError(f"{what}: {msg}")
     raise RuntimeError(f"{what}: {msg}")
 

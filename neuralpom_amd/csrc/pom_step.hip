@@ -1878,6 +1878,34 @@ template <class T> size_t eta_ws_size(const pom_batch* b) {
   return A.off + 256;
 }
 
+// ---- static scratch of pom_eta_solve_noalloc
+// The eta solve's work arrays in a file-scope device array instead of a
+// caller workspace.  Each GPU holds its own copy until the process ends.  In
+// host code `scratch` names a host-side stand-in, never device memory: the
+// device address is looked up once per device (hipGetSymbolAddress, with that
+// device current) and only that address is handed to kernels.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+static_assert(NOALLOC_SCRATCH_BYTES == POM_NOALLOC_SCRATCH_BYTES, "header and library disagree");
+__device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+// device address of the current device's scratch; POM_E_HIP if it cannot be found
+int scratch_address(void** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) return POM_E_HIP;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return POM_E_HIP;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return POM_OK;
+}
+
 template <class T> EllGeom<T> ell_geom(const Ctx<T>& c, const void* const* geom, const pom_params_t* prm) {
   auto G = [&](int f) { return static_cast<const T*>(geom[f]); };
   return EllGeom<T>{G(POM_G_DX), G(POM_G_DY), G(POM_G_ART), G(POM_G_DUM), G(POM_G_DVM),
@@ -2198,6 +2226,24 @@ int pom_eta_solve(const pom_batch* b, const pom_params_t* prm, const void* cache
   if (b->dtype == POM_F32)
     return run_eta_solve<float>(b, prm, cache, geom, rhs, d, x0, x, iters_out, workspace, ws_bytes, st);
   return run_eta_solve<double>(b, prm, cache, geom, rhs, d, x0, x, iters_out, workspace, ws_bytes, st);
+}
+
+int pom_eta_solve_noalloc(const pom_batch* b, const pom_params_t* prm, const void* cache, const void* const* geom,
+                          const void* rhs, const void* d, const void* x0, void* x, int32_t* iters_out, void* stream) {
+  if (!b || !ell_geom_ok(geom) || !rhs || !d || !x) return POM_E_ARG;
+  if (int e = check_params(prm, POM_CORE_IMPLICIT)) return e;
+  if (!tide_geom_ok(prm, geom, true)) return POM_E_ARG;
+  if (prm->eta_precond == POM_PRECOND_SPECTRAL && !cache) return POM_E_ARG;
+  const size_t need = b->dtype == POM_F32 ? eta_ws_size<float>(b) : eta_ws_size<double>(b);
+  if (need > NOALLOC_SCRATCH_BYTES) return POM_E_SCRATCH;
+  void* ws = nullptr;
+  if (int e = scratch_address(&ws)) return e;
+  pom_params_t q = *prm;
+  q.eta_capture = 1;   // no pinned host word, no host synchronisation
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (b->dtype == POM_F32)
+    return run_eta_solve<float>(b, &q, cache, geom, rhs, d, x0, x, iters_out, ws, NOALLOC_SCRATCH_BYTES, st);
+  return run_eta_solve<double>(b, &q, cache, geom, rhs, d, x0, x, iters_out, ws, NOALLOC_SCRATCH_BYTES, st);
 }
 
 int pom_eta_solve_adj(const pom_batch* b, const pom_params_t* prm, const void* cache, const void* const* geom,

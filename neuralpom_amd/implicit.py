@@ -201,6 +201,27 @@ class ImplicitEnsemble(Ensemble):
                                        ws.data_ptr(), nb, stream), "pom_eta_solve")
         return x
 
+    def eta_solve_noalloc(self, geom, sc: ImplicitScalars, rhs, d, x, x0=None, iters=None):
+        """``eta_solve`` into the caller's ``x`` through pom_eta_solve_noalloc:
+        no allocation and no host synchronisation inside the library (its work
+        arrays are the library's static per-GPU scratch; exactly
+        ``sc.eta_max_iter`` iterations are launched).  Calls that overlap on
+        one GPU share that scratch and must not run at the same time."""
+        if not self._prepared:
+            raise RuntimeError("ImplicitEnsemble.prepare(geom, scalars) must run before the eta solve")
+        rhs, d = self._flat2(rhs, "rhs"), self._flat2(d, "d")
+        x0 = self._flat2(x0, "initial_guess") if x0 is not None else None
+        if x.dtype != self.dtype or x.device != self.device or not x.is_contiguous() or x.numel() != rhs.numel():
+            raise ValueError("x: expected a contiguous tensor of the ensemble's dtype, device and 2-D size")
+        prm = sc.struct()
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        L.check(self.lib.pom_eta_solve_noalloc(self._h, C.byref(prm), self.cache.data_ptr(), self._geom_ptrs(geom),
+                                               rhs.data_ptr(), d.data_ptr(),
+                                               x0.data_ptr() if x0 is not None else None, x.data_ptr(),
+                                               iters.data_ptr() if iters is not None else None, stream),
+                "pom_eta_solve_noalloc")
+        return x
+
     def eta_solve_adj(self, geom, sc: ImplicitScalars, x, d, g, iters=None):
         """Backward of the solve (core.py:24-59): (lambda, -(d(A x)/dd)^T lambda)."""
         if not self._prepared:
