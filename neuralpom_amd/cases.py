@@ -8,10 +8,14 @@ src/neuralpom/cores/explicit/initialization.py:10-99 (initial state).
 
 ``seamount`` is the stratified seamount ensemble member of the headline
 benchmark: closed basin, uniform dx = dy, h = H - 0.6 H exp(-r^2/L^2) with
-H = 4500 m and L = 25 km, temperature with constant N^2 = 1e-5 s^-2 (linear
-thermal expansion 2e-4 / K), S = 35, an f-plane at 40 N, and seeded Gaussian
-temperature noise; the mean density rmean is the initial density.  It reuses
-the double-gyre initialisation sequence.
+H = 4500 m and L = 25 km, temperature with constant N^2 = 1e-5 s^-2 through
+the model's equation of state (the potential density of the EOS polynomial,
+momentum.py:507-532 without its pressure term, rises by rhoref N^2 / g per
+metre of depth from T = 25 C at the surface, S = 35), an f-plane at 40 N, and
+seeded Gaussian temperature noise; the mean density rmean is the density of
+the noise-free stratification, so the sigma-coordinate pressure gradient of
+the background is zero and the noise drives the flow.  It reuses the
+double-gyre initialisation sequence.
 
 Set-up is not on the per-step path, so it is written with torch tensor ops;
 the equation of state used here mirrors the kernel's (momentum.py:467-541).
@@ -63,6 +67,39 @@ def eos(t, s, h, zz, fsm, grav=9.806, rhoref=1035.0):
     out = torch.zeros_like(t)
     out[:, :, :kbm1] = (r / rhoref) * fsm.unsqueeze(2)
     return out
+
+
+def eos_potential(t, s):
+    """Density - 1000 (kg/m^3) of the EOS polynomial at zero pressure
+    (momentum.py:507-532 without the pressure term)."""
+    t2 = t * t
+    t3 = t2 * t
+    t4 = t3 * t
+    r = -0.157406 + 6.793952e-2 * t - 9.095290e-3 * t2 + 1.001685e-4 * t3 - 1.120083e-6 * t4 + \
+        6.536332e-9 * t4 * t
+    r = r + (0.824493 - 4.0899e-3 * t + 7.6438e-5 * t2 - 8.2467e-7 * t3 + 5.3875e-9 * t4) * s
+    r = r + (-5.72466e-3 + 1.0227e-4 * t - 1.6546e-6 * t2) * torch.pow(torch.clamp(s.abs(), min=1e-8), 1.5)
+    return r + 4.8314e-4 * s * s
+
+
+def constant_n2_temperature(depth, n2, t0=25.0, s0=35.0, grav=9.806, rhoref=1035.0):
+    """T(depth) (depth >= 0 in metres, any shape) at salinity s0 whose
+    potential density under the model EOS rises linearly, rhoref n2 / grav
+    per metre, from t0 at the surface: the buoyancy frequency is n2 at every
+    depth.  Newton iterations on the EOS polynomial in float64."""
+    d = torch.as_tensor(depth, dtype=torch.float64)
+    s = torch.full_like(d, s0)
+    target = eos_potential(torch.full_like(d, t0), s) + rhoref * n2 / grav * d
+    t = torch.full_like(d, t0) - n2 / (grav * 2.0e-4) * d   # linear-expansion first guess
+    for _ in range(50):
+        tv = t.detach().requires_grad_(True)
+        f = eos_potential(tv, s) - target
+        (g,) = torch.autograd.grad(f.sum(), tv)
+        step = f.detach() / g
+        t = tv.detach() - step
+        if float(step.abs().max()) < 1e-12:
+            break
+    return t
 
 
 def _uniform_sigma(dtype, device):
@@ -243,18 +280,18 @@ def seamount(im, jm, seed=0, dx=8000.0, H=4500.0, L=25.0e3, n2=1.0e-5, noise=0.1
     vg = _uniform_sigma(dtype, device)
     _areas_masks(g)
     st["sb"].fill_(35.0)
-    # constant N^2 with a linear thermal expansion alpha_T = 2e-4 / K: dT/dz = N^2 / (g alpha_T)
-    gradt = n2 / (SCALARS["grav"] * 2.0e-4)
-    zdep = vg[0].view(1, 1, -1) * h.unsqueeze(2)
-    tb = 25.0 + gradt * zdep
+    # constant N^2 through the model EOS, at the depths of the levels (zz h)
+    depth = -(vg[1].view(1, 1, -1).double().cpu() * h.unsqueeze(2).double().cpu())
+    tbg = constant_n2_temperature(depth, n2, t0=25.0, s0=35.0, grav=SCALARS["grav"],
+                                  rhoref=SCALARS["rhoref"]).to(device, dtype)
     gen = torch.Generator(device="cpu").manual_seed(int(seed))
-    tb = tb + noise * torch.randn(tb.shape, generator=gen, dtype=torch.float64).to(device, dtype)
-    st["tb"] = tb
+    st["tb"] = tbg + noise * torch.randn(tbg.shape, generator=gen, dtype=torch.float64).to(device, dtype)
     g["tclim"] = st["tb"].clone()
     g["sclim"] = st["sb"].clone()
-    # mean density = initial density (standard POM practice; keeps the sigma-
-    # coordinate pressure-gradient error over the seamount small)
-    g["rmean"] = eos(st["tb"], st["sb"], h, vg[1], g["fsm"])
+    # mean density = density of the noise-free stratification: the background's
+    # sigma-coordinate pressure gradient over the seamount is then exactly zero,
+    # and the seeded noise (a density anomaly of ~2e-5 relative) drives the flow
+    g["rmean"] = eos(tbg, st["sb"], h, vg[1], g["fsm"])
     g["hmax"] = H
     period = _initialize(st, g, vg, im, jm, float(g["cor"][im // 2, jm // 2]))
     g.pop("hmax")
