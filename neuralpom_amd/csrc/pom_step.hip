@@ -1074,12 +1074,18 @@ template <class T> struct Adj {
     b_egf0 = A.get<T>(N2);
     b_utf0 = A.get<T>(N2);
     b_vtf0 = A.get<T>(N2);
+    // advua / advva are carried unchanged between advave calls, so their
+    // cotangent is one accumulator shared by both substep buffers: the
+    // substeps that read a carried value add into it, and the advave call
+    // that produced the value consumes it and clears it
+    T* b_advua_c = A.get<T>(N2);
+    T* b_advva_c = A.get<T>(N2);
     for (int q = 0; q < 2; ++q) {
       X<T>& x = bx[q];
       x.ua = A.get<T>(N2); x.uab = A.get<T>(N2); x.va = A.get<T>(N2); x.vab = A.get<T>(N2);
       x.el = A.get<T>(N2); x.elb = A.get<T>(N2); x.d = A.get<T>(N2); x.egf = A.get<T>(N2);
-      x.utf = A.get<T>(N2); x.vtf = A.get<T>(N2); x.etf = A.get<T>(N2); x.advua = A.get<T>(N2);
-      x.advva = A.get<T>(N2); x.uaf = A.get<T>(N2); x.vaf = A.get<T>(N2); x.elf = A.get<T>(N2);
+      x.utf = A.get<T>(N2); x.vtf = A.get<T>(N2); x.etf = A.get<T>(N2); x.advua = b_advua_c;
+      x.advva = b_advva_c; x.uaf = A.get<T>(N2); x.vaf = A.get<T>(N2); x.elf = A.get<T>(N2);
       x.fluxua = x.fluxva = x.tps = nullptr;
     }
     b_u1 = A.get<T>(N3); b_v1 = A.get<T>(N3); b_q2f0 = A.get<T>(N3); b_q2lf0 = A.get<T>(N3);
@@ -1178,8 +1184,8 @@ template <class T> struct Adj {
         // No zeroing of bp: the first gather to write each of its buffers in
         // this substep assigns instead of accumulating (ExtFilter: ua uab va
         // vab el elb egf utf vtf etf; ExtUV: d uaf vaf), bp.elf is assigned
-        // by the next substep's ExtFilter (first = n - 1 < isplit), and
-        // advua / advva are copied or zeroed below.
+        // by the next substep's ExtFilter (first = n - 1 < isplit), and the
+        // shared advua / advva accumulator is consumed and cleared below.
         const X<T>& xp = n == 1 ? S.x0_ : S.xs[n - 1];
         const X<T>& xn = S.xs[n];
         Ctx<T> cn = c;
@@ -1206,22 +1212,18 @@ template <class T> struct Adj {
           const uint32_t as = (1u << OpExtUV::D_) | (1u << OpExtUV::UAFIN) | (1u << OpExtUV::VAFIN);
           launch_gather<T, OpExtUV>(cn, S.t_uv(xp, xn), ob, ib, as);
         }
-        if (n % c.prm.ispadv == 0) {
-          zq(bp.advua, N2);
-          zq(bp.advva, N2);
-          zflush();
+        if (n % c.prm.ispadv == 0) {   // advave produced this substep's advua / advva
           COutTab<T> ob{{bn.advua, bn.advva, nullptr, nullptr, nullptr}};
           T* ib[OpAdvave::NIN] = {bp.ua, bp.va, bp.uab, bp.vab, bp.d, bo[POM_AAM2D]};
           launch_gather<T, OpAdvave>(cn, S.t_advave(xp), ob, ib);
-        } else {
-          S.copy(bp.advua, bn.advua, N2);
-          S.copy(bp.advva, bn.advva, N2);
+          zq(bn.advua, N2);   // the earlier value's cotangent starts from zero
+          zq(bn.advva, N2);
+          zflush();
         }
         {
-          zero(s2d, N2);
           COutTab<T> ob{{bn.elf}};
           T* ib[OpBc1::NIN] = {s2d};
-          launch_gather<T, OpBc1>(cn, S.t_bc1(), ob, ib);
+          launch_gather<T, OpBc1>(cn, S.t_bc1(), ob, ib, 1u << OpBc1::RAW);   // assigns every point
           COutTab<T> ob2{{s2d}};
           T* ib2[OpExtElfRaw::NIN] = {bp.ua, bp.va, bp.d, bp.elb};
           launch_gather<T, OpExtElfRaw>(cn, S.t_elf(xp), ob2, ib2);
