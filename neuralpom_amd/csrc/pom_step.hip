@@ -273,6 +273,8 @@ template <class T> struct Step {
   Pcg<T> pw;
   T *u1, *v1, *q2f0, *q2lf0, *q2f1, *q2lf1, *kmraw, *khraw, *tf0, *sf0, *tf1, *sf1, *uf0, *vf0, *uf1, *vf1, *uf2,
       *vf2, *tpsu;
+  bool tape = false;     // the forward keeps profq's tape for the reverse kernels (adjoint recompute)
+  ProfqTape<T> qt{};
 
   int64_t n3() const { return b->n3; }
   int64_t n2() const { return b->n2; }
@@ -354,6 +356,10 @@ template <class T> struct Step {
     uf2 = A.get<T>(N3);
     vf2 = A.get<T>(N3);
     tpsu = A.get<T>(N2);
+    if (tape) {
+      qt.ee1 = A.get<T>(N3); qt.gg1 = A.get<T>(N3); qt.q2n = A.get<T>(N3); qt.ee2 = A.get<T>(N3);
+      qt.gg2 = A.get<T>(N3); qt.q2ln = A.get<T>(N3); qt.ghr = A.get<T>(N3); qt.kn = A.get<T>(N3);
+    }
   }
 
   void copy(T* d, const T* s, int64_t n) {
@@ -809,6 +815,7 @@ template <class T> struct Step {
     s.kqo = out[POM_KQ];
     s.kmo = kmraw;
     s.kho = khraw;
+    s.tape = qt;   // null unless the adjoint's recompute
     return s;
   }
   ProftArgs<T> a_proft(bool sal) const {
@@ -1056,6 +1063,7 @@ template <class T> struct Adj {
   // semi-implicit barotropic step
   T *b_us = nullptr, *b_vs, *b_dus, *b_dvs, *b_raw, *b_rhs, *nlam, *arhs, *xadj;
   T *b_us0 = nullptr, *b_vs0, *b_uc, *b_vc;   // tide policy: closed predictor / corrector outputs
+  T *q_bl, *q_bgh, *q_bq2ln, *q_ba, *q_bc, *q_bkn, *q_bdtef, *q_bboy, *q_bq2bot, *q_bdh;   // between profq's reverse kernels
 
   Adj(Step<T>& s, Ctx<T>& cc) : S(s), c(cc) {}
 
@@ -1094,6 +1102,9 @@ template <class T> struct Adj {
     b_uf0 = A.get<T>(N3); b_vf0 = A.get<T>(N3); b_uf1 = A.get<T>(N3); b_vf1 = A.get<T>(N3);
     b_uf2 = A.get<T>(N3); b_vf2 = A.get<T>(N3);
     s3a = A.get<T>(N3); s3b = A.get<T>(N3); s3c = A.get<T>(N3); s3d = A.get<T>(N3);
+    q_bl = A.get<T>(N3); q_bgh = A.get<T>(N3); q_bq2ln = A.get<T>(N3); q_ba = A.get<T>(N3);
+    q_bc = A.get<T>(N3); q_bkn = A.get<T>(N3); q_bdtef = A.get<T>(N3); q_bboy = A.get<T>(N3);
+    q_bq2bot = A.get<T>(N2); q_bdh = A.get<T>(N2);
     s2a = A.get<T>(N2); s2b = A.get<T>(N2); s2c = A.get<T>(N2); s2d = A.get<T>(N2); s2e = A.get<T>(N2);
     if (S.imp) {
       b_us = A.get<T>(N2); b_vs = A.get<T>(N2); b_dus = A.get<T>(N2); b_dvs = A.get<T>(N2);
@@ -1464,9 +1475,14 @@ template <class T> struct Adj {
       a.b_t = bi[POM_T]; a.b_s = bi[POM_S]; a.b_rho = b_rho1; a.b_wubot = bi[POM_WUBOT];
       a.b_wvbot = bi[POM_WVBOT]; a.b_etf = b_etf30; a.b_dt = bi[POM_DT];
       a.s_unb = s3a; a.s_vnb = s3b; a.s_wubnb = s2a; a.s_wvbnb = s2b;
+      a.r_bl = q_bl; a.r_bgh = q_bgh; a.r_bq2ln = q_bq2ln; a.r_ba = q_ba; a.r_bc = q_bc; a.r_bkn = q_bkn;
+      a.r_bdtef = q_bdtef; a.r_bboy = q_bboy; a.r_bq2bot = q_bq2bot; a.r_bdh = q_bdh;
       {
         ProfScope ps(POM_STAGE_PROFQ_ADJ, c.stream);
-        hipLaunchKernelGGL(k_profq_adj<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a);
+        hipLaunchKernelGGL(k_profq_adj_a<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a);
+        hipLaunchKernelGGL(k_profq_adj_b<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a);
+        hipLaunchKernelGGL(k_profq_adj_c1<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a);
+        hipLaunchKernelGGL(k_profq_adj_c2<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a);
       }
       hipLaunchKernelGGL(k_profq_adj2<T>, dim3(c.nblk3), tb, 0, c.stream, c.B, a);
     }
@@ -1645,6 +1661,7 @@ size_t ws_size(const pom_batch* b, const pom_params_t* p, bool adjoint) {
   S.b = b;
   S.isplit = p->isplit;
   S.c.prm.ispadv = p->ispadv;
+  S.tape = adjoint;
   setup_implicit(S, p, nullptr);
   Arena A{nullptr, 0, 0, true};
   T* dummy[POM_NSTATE] = {};
@@ -1705,6 +1722,7 @@ int run_adj(const pom_batch* b, const pom_params_t* prm, const void* ramp, const
     if (!S.g[f] && (f < POM_NGEOM_BASE || S.open)) return POM_E_ARG;
   }
   Arena A{static_cast<char*>(ws), 0, wsb, false};
+  S.tape = true;
   S.plan(A, true);
   S.vamax = nullptr;   // the recompute needs no depth-mean velocity check
   Adj<T> J(S, S.c);
