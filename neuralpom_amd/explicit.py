@@ -236,7 +236,8 @@ class StepGraph:
     ``state`` in place by one step (the step's outputs are copied back into the
     input buffers inside the graph) and, with ``cot``, leaves the input
     cotangents of the step it just took in ``grads``.  Refill ``ramp`` (and
-    ``cot``) between replays to change them.  The step launches ~100 kernels
+    ``cot``) between replays to change them.  ``on_warm`` is called between
+    the warm-up and the recording.  The step launches ~100 kernels
     per barotropic substep cycle and has no host synchronisation, so a replay
     submits the whole step with one call.  A semi-implicit ensemble
     (``ImplicitEnsemble``) is recorded the same way with ``eta_capture`` set
@@ -250,7 +251,7 @@ class StepGraph:
     memory the recorded kernels use.
     """
 
-    def __init__(self, ens, state, geom, sc, ramp, vgrid, internal=True, cot=None, warmup=1):
+    def __init__(self, ens, state, geom, sc, ramp, vgrid, internal=True, cot=None, warmup=1, on_warm=None):
         if hasattr(sc, "eta_capture") and not sc.eta_capture:
             raise ValueError("a semi-implicit step is recorded with eta_capture=True "
                              "(its eta solves otherwise poll the host for convergence)")
@@ -271,6 +272,9 @@ class StepGraph:
             for _ in range(warmup):
                 self._body({n: t.clone() for n, t in self.state.items()})
         torch.cuda.current_stream(ens.device).wait_stream(s)
+        if on_warm is not None:   # e.g. reset HIP-event tracing before the recorded launches
+            torch.cuda.synchronize(ens.device)
+            on_warm()
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
             self.grads, self.vamax = self._body(self.state)
