@@ -264,6 +264,45 @@ int pom_baropg(const pom_batch* b, const pom_params_t* prm, const void* ramp, co
 int pom_baropg_adj(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vgrid,
                    const void* const* in, const void* const* cot_out, void* const* grad_in, void* stream);
 
+/* pom_profq: the Mellor-Yamada 2.5 closure on its own, the reference's
+ *   operator method profq (src/neuralpom/cores/explicit/tracers.py:158-431,
+ *   positional signature :158-163, returned tuple :431, consumed at
+ *   time_loop.py:509-518), on the step's kernels (k_profq, then the km / kh
+ *   edge copy).
+ *   in  = {uf (q2 forward value), vf (q2l forward value), q2, q2b, q2lb, km, kh,
+ *          kq, u, v, t, s, rho (3-D), wusurf, wvsurf, wubot, wvbot, etf, dt, h,
+ *          fsm (2-D)}                                               (21 entries)
+ *   out = {q2 new (uf), q2l new (vf), l, kq, km, kh (3-D, required),
+ *          sm, sh, cc, a, c (3-D), dh (2-D) (the work arrays the reference
+ *          returns; each may be null)}                              (12 entries)
+ *   Scalars from prm: dti2, umol, grav, kappa, small, tbias, sbias, rhoref,
+ *   core (selects the wall-term guard, 1e-20 explicit / 1e-8 implicit).
+ * pom_profq_adj: grad_in[k] += (d out[0..5] / d in[k])^T cot_out[0..5] for the
+ *   differentiable inputs k = uf .. rho, wubot, wvbot, etf, dt (entries of
+ *   grad_in and cot_out may be null; wusurf, wvsurf, h, fsm are passive).
+ *   Recomputes the forward.
+ * pom_profq_workspace: workspace bytes of both. */
+int pom_profq_workspace(const pom_batch* b, size_t* bytes);
+int pom_profq(const pom_batch* b, const pom_params_t* prm, const void* vgrid, const void* const* in,
+              void* const* out, void* workspace, size_t ws_bytes, void* stream);
+int pom_profq_adj(const pom_batch* b, const pom_params_t* prm, const void* vgrid, const void* const* in,
+                  const void* const* cot_out, void* const* grad_in, void* workspace, size_t ws_bytes, void* stream);
+
+/* pom_proft: vertical diffusion of one tracer, the reference's operator method
+ *   proft (tracers.py:890-1010, signature :890-892, returned tuple :1010,
+ *   consumed at time_loop.py:573-581).
+ *   in  = {f, kh (3-D), wfsurf, fsurf, etf, h (2-D)}, nbc = 1 (flux) or 3
+ *         (prescribed surface value); 2 and 4 (shortwave absorption) return
+ *         POM_E_UNSUPPORTED.
+ *   out = {f new (required), a, c, ee, gg (3-D), dh (2-D) (work arrays; each
+ *         may be null)}.
+ * pom_proft_adj: grad_in = {f, kh, etf} += the transposed sweep of cot_f (each
+ *   may be null; wfsurf and fsurf are passive). */
+int pom_proft(const pom_batch* b, const pom_params_t* prm, const void* vgrid, const void* const* in, int32_t nbc,
+              void* const* out, void* stream);
+int pom_proft_adj(const pom_batch* b, const pom_params_t* prm, const void* vgrid, const void* const* in,
+                  int32_t nbc, const void* cot_f, void* const* grad_in, void* stream);
+
 /* pom_bcond: one lateral boundary condition of the explicit core alone,
  *   POM2K_Core.bcond(idx, elf, uaf, vaf, uf, vf, w) (cores/explicit/
  *   baroclinic.py:496-928), as callers outside the time loop apply it (the
@@ -275,7 +314,13 @@ int pom_baropg_adj(const pom_batch* b, const pom_params_t* prm, const void* ramp
  *     5: {w}                            -> {w}   (levels 0-14 * fsm)
  *     6: {uf, vf, q2, q2l, u, v}        -> {uf, vf} = new q2, q2l   (upstream edge turbulence, + 1e-10)
  *   (dt is 2-D, the others 3-D except elf, uaf, vaf, el); geom is the
- *   pom_geom table.  prm->core must be POM_CORE_EXPLICIT.
+ *   pom_geom table.  With prm->core = POM_CORE_IMPLICIT (closed basin) the
+ *   semi-implicit core's policy applies instead, reading the first entries of
+ *   the same lists (implicit_p0_fixed/baroclinic.py:505-510 ->
+ *   boundary/closed_basin.py:27-87): 1 elf * fsm, 2 uaf * dum, vaf * dvm
+ *   (edge ring zero), 3 uf * dum, vf * dvm (edge ring zero), 4 edge cells of
+ *   uf, vf copied from the nearest interior cell, 5 w * fsm, 6 uf, vf * fsm;
+ *   the tide policy's bcond returns POM_E_UNSUPPORTED.
  * pom_bcond_adj: grad_fields[q] += d(out)/d(fields[q])^T cot_out (entries
  *   may be null). */
 int pom_bcond(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vgrid, int32_t idx,

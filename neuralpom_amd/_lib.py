@@ -104,6 +104,11 @@ def load():
     lib.pom_restrict4_adj.argtypes = [vp, vp, C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     lib.pom_copy_arrays.argtypes = [C.c_int32, vpp, vpp, C.POINTER(C.c_int64), vp]
     lib.pom_bcond.argtypes = [vp, C.POINTER(Params), vp, vp, C.c_int32, vpp, vpp, vpp, vp]
+    lib.pom_profq_workspace.argtypes = [vp, C.POINTER(C.c_size_t)]
+    lib.pom_profq.argtypes = [vp, C.POINTER(Params), vp, vpp, vpp, vp, C.c_size_t, vp]
+    lib.pom_profq_adj.argtypes = [vp, C.POINTER(Params), vp, vpp, vpp, vpp, vp, C.c_size_t, vp]
+    lib.pom_proft.argtypes = [vp, C.POINTER(Params), vp, vpp, C.c_int32, vpp, vp]
+    lib.pom_proft_adj.argtypes = [vp, C.POINTER(Params), vp, vpp, C.c_int32, vp, vpp, vp]
     lib.pom_bcond_adj.argtypes = [vp, C.POINTER(Params), vp, vp, C.c_int32, vpp, vpp, vpp, vpp, vp]
     dp = C.POINTER(C.c_double)
     lib.pom_correction_workspace.argtypes = [vp, C.POINTER(C.c_size_t)]
@@ -122,7 +127,8 @@ EXPORTED = ("pom_batch_create", "pom_batch_destroy", "pom_explicit_workspace", "
             "pom_implicit_step", "pom_implicit_step_adj", "pom_dens", "pom_dens_adj", "pom_baropg",
             "pom_baropg_adj", "pom_elliptic_apply", "pom_elliptic_apply_adj", "pom_eta_workspace", "pom_eta_solve",
             "pom_eta_solve_adj", "pom_eta_solve_noalloc", "pom_pack_input", "pom_pack_input_adj", "pom_restrict4", "pom_restrict4_adj", "pom_copy_arrays",
-            "pom_bcond", "pom_bcond_adj",
+            "pom_bcond", "pom_bcond_adj", "pom_profq_workspace", "pom_profq", "pom_profq_adj", "pom_proft",
+            "pom_proft_adj",
             "pom_correction_workspace", "pom_joint_correction", "pom_joint_correction_adj",
             "pom_profile_stage", "pom_profile_read")
 STAGES = {"profq": 0, "profq_adj": 1, "extuv": 2, "advct": 3}

@@ -21,6 +21,7 @@ library or a GPU every entry point raises.
 """
 from __future__ import annotations
 
+import ctypes as C
 import math
 import os
 from typing import Any
@@ -350,6 +351,67 @@ class POM2K_Core:
             uf, vf = outs[0].view(shape3), outs[1].view(shape3)
         return elf, uaf, vaf, uf, vf, w
 
+    def profq(self, sm, sh, dh, cc, ee, gg, l, kq, km, kh, uf, vf, q2, q2b, q2lb, a, c,
+              h, etf, dti2, umol, dzz, grav, rho, kappa, u, v, dt, small, fsm,
+              im, jm, kb, imm1, jmm1, kbm1, tbias, sbias, dz,
+              wusurf, wubot, wvsurf, wvbot, t, s, rhoref, zz, z):
+        """Mellor-Yamada 2.5 closure, the reference's operator method with its
+        positional signature (tracers.py:158-163) and returned tuple
+        (tracers.py:431): (sm, sh, dh, cc, ee, gg, l, kq, km, kh, uf, vf, q2b,
+        q2lb, a, c), uf / vf being the new q2 / q2l, cc the sound speed on
+        levels 0-14, dh, ee and gg the arguments passed in (the reference
+        computes h + etf and its sweep coefficients in locals) and q2b, q2lb
+        copies of the inputs.  The work arrays passed in are not read.
+        Runs the step's HIP kernels (pom_profq); l, kq, km, kh, uf, vf are
+        differentiable with respect to uf, vf, q2, q2b, q2lb, km, kh, kq, u, v,
+        t, s, rho, wubot, wvbot, etf and dt (pom_profq_adj); the work arrays
+        are returned detached."""
+        if (im, jm, kb) != (self.im, self.jm, self.kb) or (imm1, jmm1, kbm1) != (im - 1, jm - 1, kb - 1):
+            raise ValueError(f"profq: grid {(im, jm, kb)} is not the core's {(self.im, self.jm, self.kb)}")
+        F = self._as_flat
+        prm = self._params()
+        prm.dti2, prm.umol, prm.grav, prm.kappa, prm.small = float(dti2), float(umol), float(grav), float(kappa), \
+            float(small)
+        prm.tbias, prm.sbias, prm.rhoref = float(tbias), float(sbias), float(rhoref)
+        vg = torch.stack([F(z), F(zz), F(dz), F(dzz)]).unsqueeze(0).contiguous()
+        ins = [F(x) for x in (uf, vf, q2, q2b, q2lb, km, kh, kq, u, v, t, s, rho)] + \
+              [F(x).detach() for x in (wusurf, wvsurf)] + [F(x) for x in (wubot, wvbot, etf, dt)] + \
+              [F(x).detach() for x in (h, fsm)]
+        outs = _Profq.apply(self._ensemble(), prm, vg, *ins)
+        s3, s2 = (self.im, self.jm, self.kb), (self.im, self.jm)
+        q2n, q2ln, l_o, kq_o, km_o, kh_o, sm_o, sh_o, cc_o, a_o, c_o, _dh = outs
+        kbm1_ = self.kb - 1
+        return (sm_o.view(s3), sh_o.view(s3), dh, cc_o.view(s3)[:, :, :kbm1_], ee, gg, l_o.view(s3),
+                kq_o.view(s3), km_o.view(s3), kh_o.view(s3), q2n.view(s3), q2ln.view(s3), q2b.clone(), q2lb.clone(),
+                a_o.view(s3), c_o.view(s3))
+
+    def proft(self, f, wfsurf, fsurf, nbc, dh, a, c, ee, gg, h, etf, dti2, dz, dzz, swrad, ntp,
+              im, jm, kb, kbm1, kbm2, kh, umol):
+        """Vertical diffusion of one tracer, the reference's operator method with
+        its positional signature (tracers.py:890-892) and returned tuple
+        (tracers.py:1010): (f, wfsurf, fsurf, nbc, dh, a, c, ee, gg) with f the
+        diffused field, dh = h + etf and a, c, ee, gg the sweep's coefficients.
+        Surface conditions nbc 1 (flux) and 3 (prescribed value); 2 and 4
+        (shortwave absorption, ntp) raise NotImplementedError.  Runs the step's
+        HIP kernel (pom_proft); f is differentiable with respect to f, kh and
+        etf (pom_proft_adj)."""
+        if (im, jm, kb) != (self.im, self.jm, self.kb) or (kbm1, kbm2) != (kb - 1, kb - 2):
+            raise ValueError(f"proft: grid {(im, jm, kb)} is not the core's {(self.im, self.jm, self.kb)}")
+        nbc = int(nbc)
+        if nbc in (2, 4):
+            raise NotImplementedError("proft: surface conditions with shortwave absorption (nbc 2, 4) are not built")
+        if nbc not in (1, 3):
+            raise ValueError(f"proft: nbc must be 1-4, got {nbc}")
+        F = self._as_flat
+        prm = self._params()
+        prm.dti2, prm.umol = float(dti2), float(umol)
+        vg = torch.stack([F(self.z), F(self.zz), F(dz), F(dzz)]).unsqueeze(0).contiguous()
+        outs = _Proft.apply(self._ensemble(), prm, vg, nbc, F(f), F(kh), F(wfsurf).detach(), F(fsurf).detach(), F(etf),
+                            F(h).detach())
+        s3, s2 = (self.im, self.jm, self.kb), (self.im, self.jm)
+        fo, ao, co, eeo, ggo, dho = outs
+        return fo.view(s3), wfsurf, fsurf, nbc, dho.view(s2), ao.view(s3), co.view(s3), eeo.view(s3), ggo.view(s3)
+
     def _as_flat(self, v):
         if v.dtype != self.dtype or v.device != self.device:
             v = v.to(self.device, self.dtype)
@@ -436,3 +498,75 @@ class _Baropg(torch.autograd.Function):
         L.check(ens.lib.pom_baropg_adj(ens._h, ctx.prm, ramp.data_ptr(), vg.data_ptr(), _ptrs(ins), _ptrs(cot),
                                        _ptrs((b_rho, b_dt)), stream), "pom_baropg_adj")
         return None, None, None, None, b_rho, None, b_dt, None, None, None, None
+
+
+class _Profq(torch.autograd.Function):
+    """pom_profq / pom_profq_adj.  Inputs (flat): uf vf q2 q2b q2lb km kh kq u v t s
+    rho wusurf wvsurf wubot wvbot etf dt h fsm; outputs: q2 q2l l kq km kh (3-D,
+    differentiable) then sm sh cc a c dh (work arrays, not differentiable)."""
+
+    N3_OUT = 11
+
+    @staticmethod
+    def forward(ctx, ens, prm, vg, *ins):
+        n3, n2 = ins[0].numel(), ins[-1].numel()
+        outs = [torch.empty(n3, dtype=ins[0].dtype, device=ins[0].device) for _ in range(_Profq.N3_OUT)]
+        outs.append(torch.empty(n2, dtype=ins[0].dtype, device=ins[0].device))
+        nb = C.c_size_t()
+        L.check(ens.lib.pom_profq_workspace(ens._h, C.byref(nb)), "pom_profq_workspace")
+        ws = torch.empty(nb.value, dtype=torch.uint8, device=ens.device)
+        stream = torch.cuda.current_stream(ens.device).cuda_stream
+        L.check(ens.lib.pom_profq(ens._h, C.byref(prm), vg.data_ptr(), _ptrs(ins), _ptrs(outs), ws.data_ptr(),
+                                  nb.value, stream), "pom_profq")
+        ctx.ens, ctx.prm = ens, prm
+        ctx.save_for_backward(vg, *ins)
+        ctx.mark_non_differentiable(*outs[6:])
+        return tuple(outs)
+
+    @staticmethod
+    def backward(ctx, *g):
+        vg, *ins = ctx.saved_tensors
+        ens = ctx.ens
+        cots = [x.contiguous() if x is not None else None for x in g[:6]]
+        diff = set(range(13)) | {15, 16, 17, 18}   # uf .. rho, wubot, wvbot, etf, dt
+        grads = [torch.zeros_like(x) if (q in diff and ctx.needs_input_grad[3 + q]) else None
+                 for q, x in enumerate(ins)]
+        nb = C.c_size_t()
+        L.check(ens.lib.pom_profq_workspace(ens._h, C.byref(nb)), "pom_profq_workspace")
+        ws = torch.empty(nb.value, dtype=torch.uint8, device=ens.device)
+        stream = torch.cuda.current_stream(ens.device).cuda_stream
+        L.check(ens.lib.pom_profq_adj(ens._h, C.byref(ctx.prm), vg.data_ptr(), _ptrs(ins), _ptrs(cots), _ptrs(grads),
+                                      ws.data_ptr(), nb.value, stream), "pom_profq_adj")
+        return (None, None, None) + tuple(grads)
+
+
+class _Proft(torch.autograd.Function):
+    """pom_proft / pom_proft_adj.  Inputs (flat): f kh wfsurf fsurf etf h; outputs
+    f (differentiable) then a c ee gg dh (work arrays)."""
+
+    @staticmethod
+    def forward(ctx, ens, prm, vg, nbc, f, kh, wfsurf, fsurf, etf, h):
+        outs = [torch.empty_like(f) for _ in range(5)] + [torch.empty_like(etf)]
+        ins = (f, kh, wfsurf, fsurf, etf, h)
+        stream = torch.cuda.current_stream(ens.device).cuda_stream
+        L.check(ens.lib.pom_proft(ens._h, C.byref(prm), vg.data_ptr(), _ptrs(ins), int(nbc), _ptrs(outs), stream),
+                "pom_proft")
+        ctx.ens, ctx.prm, ctx.nbc = ens, prm, int(nbc)
+        ctx.save_for_backward(vg, *ins)
+        ctx.mark_non_differentiable(*outs[1:])
+        return tuple(outs)
+
+    @staticmethod
+    def backward(ctx, g, *_):
+        vg, *ins = ctx.saved_tensors
+        ens = ctx.ens
+        if g is None:
+            return (None,) * 10
+        g = g.contiguous()
+        need = ctx.needs_input_grad
+        grads = [torch.zeros_like(ins[0]) if need[4] else None, torch.zeros_like(ins[1]) if need[5] else None,
+                 torch.zeros_like(ins[4]) if need[8] else None]
+        stream = torch.cuda.current_stream(ens.device).cuda_stream
+        L.check(ens.lib.pom_proft_adj(ens._h, C.byref(ctx.prm), vg.data_ptr(), _ptrs(ins), ctx.nbc, g.data_ptr(),
+                                      _ptrs(grads), stream), "pom_proft_adj")
+        return (None, None, None, None, grads[0], grads[1], None, None, grads[2], None)

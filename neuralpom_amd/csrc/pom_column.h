@@ -653,6 +653,9 @@ template <class T> struct ProftArgs {
   const T *f, *kh, *etf, *h, *wfsurf, *fsurf;
   T* out;
   int nbc;
+  // the reference operator's work arrays (tracers.py:1000-1010 returns them):
+  // a, c, ee, gg (3-D) and dh = h + etf (2-D); each may be null
+  T *ao, *co, *eeo, *ggo, *dho;
 };
 template <class T>
 __device__ __forceinline__ void proft_coef(const Pos& p, const Vert<T>& vg, const Params& prm, const T* kh, T dh,
@@ -699,6 +702,15 @@ __global__ void __launch_bounds__(TPB) k_proft(Batch B, Vert<T> vg, Params prm, 
   #pragma unroll
   for (int k = kb - 1; k >= 0; --k) x[k] = ee[k] * x[k + 1] + gg[k];
   st_col(s.out + c, x);
+  if (s.eeo || s.ggo) {   // levels past the sweep hold zeros, as the reference's buffers do
+    #pragma unroll
+    for (int k = KBM2; k < KB; ++k) ee[k] = gg[k] = T(0);
+    if (s.eeo) st_col(s.eeo + c, ee);
+    if (s.ggo) st_col(s.ggo + c, gg);
+  }
+  if (s.ao) st_col(s.ao + c, a);
+  if (s.co) st_col(s.co + c, cc);
+  if (s.dho) s.dho[c2] = dh;
 }
 
 template <class T> struct ProftAdjArgs {

@@ -84,6 +84,36 @@ template <int F, class AC> __device__ __forceinline__ auto edge_copy3(const AC& 
 }
 
 // =====================================================================
+// The closed-basin policy's bcond on its own, idx 1, 2, 4, 5, 6
+// (implicit_p0_fixed/boundary/closed_basin.py:27-87; idx 3 is OpImpBc3):
+//   1: elf * fsm, the edge ring zero          2: uaf * dum, vaf * dvm, edges zero
+//   4: uf, vf (new T, S) edge cells copied from the nearest interior cell
+//   5: w * fsm                                6: uf * fsm, vf * fsm (new q2, q2l)
+// =====================================================================
+template <int IDX> struct OpImpBcond {
+  static constexpr bool OUT3 = IDX >= 3;
+  static constexpr int NOUT = (IDX == 1 || IDX == 5) ? 1 : 2;
+  static constexpr int NIN = NOUT, N3 = OUT3 ? NIN : 0;
+  enum { A0, A1 };
+  enum { GFSM, GDUM, GDVM };
+  static constexpr int RI = IDX == 4 ? 1 : 0, RJ = IDX == 4 ? 1 : 0, RK = 0;
+  static constexpr int EDGE_BAND = 1;
+  template <int O, class AC> __device__ static auto eval(const AC& A, const Params&) {
+    if constexpr (IDX == 1) {
+      return IFZ(interior(A), AT2(A0, 0, 0) * G2(GFSM, 0, 0));
+    } else if constexpr (IDX == 2) {
+      if constexpr (O == 0) return IFZ(interior(A), AT2(A0, 0, 0) * G2(GDUM, 0, 0));
+      else return IFZ(interior(A), AT2(A1, 0, 0) * G2(GDVM, 0, 0));
+    } else if constexpr (IDX == 4) {
+      return edge_copy3<O == 0 ? A0 : A1>(A);
+    } else {   // 5 and 6
+      return AT(O == 0 ? A0 : A1, 0, 0, 0) * G2(GFSM, 0, 0);
+    }
+  }
+};
+
+
+// =====================================================================
 // closed-basin bcond 4 + T/S Asselin filter + tracer mask (time_loop.py:548-575,
 // numerics/double_gyre.py:78-130 with gamma 0).  outputs t, tb, s, sb.
 // =====================================================================

@@ -33,6 +33,11 @@ template <class T> struct ProfqArgs {
   const T *wusurf, *wvsurf, *wubot, *wvbot, *etf, *dt, *h;
   T *q2o, *q2lo, *lo, *kqo, *kmo, *kho;
   ProfqTape<T> tape;
+  // the reference operator's work arrays, for callers of the operator on its
+  // own (tracers.py:423-431 returns them): stability functions sm, sh, sound
+  // speed cc, diffusion coefficients a, c (3-D) and dh = h + etf (2-D); each
+  // may be null
+  T *smo, *sho, *cco, *ao, *co, *dho;
 };
 
 template <class T> __device__ __forceinline__ T sgn(T x) { return x > T(0) ? T(1) : (x < T(0) ? T(-1) : T(0)); }
@@ -203,6 +208,20 @@ __global__ void __launch_bounds__(TPB, 2) k_profq(Batch B, Vert<T> vg, Params pr
   col_st(io, s.q2o, o1);
   col_st(io, s.q2lo, o2);
   col_st(io, s.lo, C.l);
+  if (s.smo || s.sho) {   // uniform over the launch
+    #pragma unroll
+    for (int k = 0; k < KB; ++k) {
+      const T sh = T(COEF1) / (T(1) - T(COEF2) * C.gh[k] + T(1e-20));
+      o1[k] = sh;
+      o2[k] = (T(COEF3) + sh * T(COEF4) * C.gh[k]) / (T(1) - T(COEF5) * C.gh[k] + T(1e-20));
+    }
+    if (s.sho) col_st(io, s.sho, o1);
+    if (s.smo) col_st(io, s.smo, o2);
+  }
+  if (s.cco) col_st(io, s.cco, C.cc);
+  if (s.ao) col_st(io, s.ao, C.a);
+  if (s.co) col_st(io, s.co, C.c);
+  if (s.dho && io.valid) s.dho[c2] = C.dh;
   if (s.tape.ee1) {   // uniform over the launch
     col_st(io, s.tape.ee1, C.ee1);
     col_st(io, s.tape.gg1, C.gg1);

@@ -1785,6 +1785,21 @@ int bcond_nfields(int idx) {
 }
 int bcond_nout(int idx) { return (idx == 1 || idx == 5) ? 1 : 2; }
 
+// the semi-implicit core's closed-basin policy (boundary/closed_basin.py:27-87):
+// reads the first one or two entries of the same field lists
+template <class T, int IDX>
+void imp_bcond(const Ctx<T>& c, const void* const* fields, const void* const* geom, void* const* out,
+               const void* const* cot, void* const* grad, bool adjoint) {
+  using O = OpImpBcond<IDX>;
+  FieldTab<T> t{};
+  for (int q = 0; q < O::NIN; ++q) t.f[q] = static_cast<const T*>(fields[q]);
+  t.g[O::GFSM] = static_cast<const T*>(geom[POM_G_FSM]);
+  t.g[O::GDUM] = static_cast<const T*>(geom[POM_G_DUM]);
+  t.g[O::GDVM] = static_cast<const T*>(geom[POM_G_DVM]);
+  const int fid[2] = {0, 1}, oid[2] = {0, 1};
+  bcond_op<T, O>(c, t, fid, O::NIN, oid, O::NOUT, out, cot, grad, adjoint);
+}
+
 template <class T>
 int run_bcond(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vg, int idx,
               const void* const* fields, const void* const* geom, void* const* out, const void* const* cot,
@@ -1793,6 +1808,25 @@ int run_bcond(const pom_batch* b, const pom_params_t* prm, const void* ramp, con
   auto F = [&](int q) { return static_cast<const T*>(fields[q]); };
   auto G = [&](int q) { return static_cast<const T*>(geom[q]); };
   FieldTab<T> t{};
+  if (prm->core == POM_CORE_IMPLICIT) {
+    switch (idx) {
+      case 1: imp_bcond<T, 1>(c, fields, geom, out, cot, grad, adjoint); break;
+      case 2: imp_bcond<T, 2>(c, fields, geom, out, cot, grad, adjoint); break;
+      case 3: {
+        t.f[OpImpBc3::UF] = F(0);
+        t.f[OpImpBc3::VF] = F(1);
+        t.g[OpImpBc3::GDUM] = G(POM_G_DUM);
+        t.g[OpImpBc3::GDVM] = G(POM_G_DVM);
+        const int fid[2] = {OpImpBc3::UF, OpImpBc3::VF}, oid[2] = {0, 1};
+        bcond_op<T, OpImpBc3>(c, t, fid, 2, oid, 2, out, cot, grad, adjoint);
+        break;
+      }
+      case 4: imp_bcond<T, 4>(c, fields, geom, out, cot, grad, adjoint); break;
+      case 5: imp_bcond<T, 5>(c, fields, geom, out, cot, grad, adjoint); break;
+      default: imp_bcond<T, 6>(c, fields, geom, out, cot, grad, adjoint); break;
+    }
+    return hipGetLastError() == hipSuccess ? POM_OK : POM_E_HIP;
+  }
   switch (idx) {
     case 1: {
       t.f[OpBc1::RAW] = F(0);
@@ -1871,6 +1905,149 @@ int run_baropg(const pom_batch* b, const pom_params_t* prm, const void* ramp, co
                        static_cast<const T*>(cot[0]), static_cast<const T*>(cot[1]), static_cast<const T*>(cot[2]),
                        static_cast<T*>(grad[0]), static_cast<T*>(grad[1])};
     hipLaunchKernelGGL(k_baropg_adj<T>, g2, tb, 0, c.stream, c.B, c.vert, c.ramp, c.prm, a);
+  }
+  return hipGetLastError() == hipSuccess ? POM_OK : POM_E_HIP;
+}
+
+// ---------------------------------------------------------------- profq / proft on their own
+// The Mellor-Yamada closure and the tracer vertical diffusion as the
+// reference's operator methods (tracers.py:158-431, 890-1010), on the step's
+// kernels: profq is k_profq followed by the km / kh edge copy (OpKEdge);
+// its reverse recomputes the forward with the tape, gathers the edge copy
+// and runs the four reverse kernels and the neighbour gather.
+enum { PQ_Q2F, PQ_Q2LF, PQ_Q2, PQ_Q2B, PQ_Q2LB, PQ_KM, PQ_KH, PQ_KQ, PQ_U, PQ_V, PQ_T, PQ_S, PQ_RHO,
+       PQ_WUSURF, PQ_WVSURF, PQ_WUBOT, PQ_WVBOT, PQ_ETF, PQ_DT, PQ_H, PQ_FSM, PQ_NIN };
+enum { PQ_O_Q2, PQ_O_Q2L, PQ_O_L, PQ_O_KQ, PQ_O_KM, PQ_O_KH, PQ_O_SM, PQ_O_SH, PQ_O_CC, PQ_O_A, PQ_O_C, PQ_O_DH,
+       PQ_NOUT };
+constexpr int PQ_NDIFF_OUT = 6;   // cotangents: q2, q2l, l, kq, km, kh
+
+template <class T> ProfqArgs<T> profq_args(const void* const* in) {
+  auto I = [&](int k) { return static_cast<const T*>(in[k]); };
+  ProfqArgs<T> a{};
+  a.q2f = I(PQ_Q2F); a.q2lf = I(PQ_Q2LF); a.q2 = I(PQ_Q2); a.q2b = I(PQ_Q2B); a.q2lb = I(PQ_Q2LB);
+  a.km = I(PQ_KM); a.kh = I(PQ_KH); a.kq = I(PQ_KQ); a.u = I(PQ_U); a.v = I(PQ_V); a.t = I(PQ_T); a.s = I(PQ_S);
+  a.rho = I(PQ_RHO); a.wusurf = I(PQ_WUSURF); a.wvsurf = I(PQ_WVSURF); a.wubot = I(PQ_WUBOT);
+  a.wvbot = I(PQ_WVBOT); a.etf = I(PQ_ETF); a.dt = I(PQ_DT); a.h = I(PQ_H);
+  return a;
+}
+template <class T> FieldTab<T> kedge_tab(const T* km, const T* kh, const void* fsm) {
+  FieldTab<T> t{};
+  t.f[OpKEdge::KM] = km;
+  t.f[OpKEdge::KH] = kh;
+  t.g[OpKEdge::GFSM] = static_cast<const T*>(fsm);
+  return t;
+}
+// workspace of pom_profq_adj: the forward's six outputs, its tape, the
+// reverse kernels' scratch, and the cotangents of km / kh before the edge copy
+template <class T> struct ProfqWs {
+  T *q2o, *q2lo, *lo, *kqo, *kmo, *kho, *b_kmo, *b_kho, *unb, *vnb, *wubnb, *wvbnb;
+  T *r3[8], *r2[2];
+  ProfqTape<T> tape;
+  void plan(Arena& A, int64_t N3, int64_t N2) {
+    q2o = A.get<T>(N3); q2lo = A.get<T>(N3); lo = A.get<T>(N3); kqo = A.get<T>(N3); kmo = A.get<T>(N3);
+    kho = A.get<T>(N3); b_kmo = A.get<T>(N3); b_kho = A.get<T>(N3); unb = A.get<T>(N3); vnb = A.get<T>(N3);
+    wubnb = A.get<T>(N2); wvbnb = A.get<T>(N2);
+    for (T*& r : r3) r = A.get<T>(N3);
+    for (T*& r : r2) r = A.get<T>(N2);
+    tape.ee1 = A.get<T>(N3); tape.gg1 = A.get<T>(N3); tape.q2n = A.get<T>(N3); tape.ee2 = A.get<T>(N3);
+    tape.gg2 = A.get<T>(N3); tape.q2ln = A.get<T>(N3); tape.ghr = A.get<T>(N3); tape.kn = A.get<T>(N3);
+  }
+};
+template <class T> size_t profq_ws_size(const pom_batch* b) {
+  Arena A{nullptr, 0, 0, true};
+  ProfqWs<T> w{};
+  w.plan(A, b->n3, b->n2);
+  return A.off + 256;
+}
+
+template <class T>
+int run_profq(const pom_batch* b, const pom_params_t* prm, const void* vg, const void* const* in, void* const* out,
+              void* ws, size_t wsb, hipStream_t st) {
+  Ctx<T> c = make_ctx<T>(b, prm, nullptr, vg, st);
+  if (c.nblk2 == 0) return POM_OK;
+  Arena A{static_cast<char*>(ws), 0, wsb, false};
+  ProfqWs<T> w{};
+  w.plan(A, b->n3, b->n2);
+  if (A.off > wsb) return POM_E_ARG;
+  ProfqArgs<T> a = profq_args<T>(in);
+  auto O = [&](int k) { return static_cast<T*>(out[k]); };
+  a.q2o = O(PQ_O_Q2); a.q2lo = O(PQ_O_Q2L); a.lo = O(PQ_O_L); a.kqo = O(PQ_O_KQ);
+  a.kmo = w.kmo; a.kho = w.kho;   // before the edge copy
+  a.smo = O(PQ_O_SM); a.sho = O(PQ_O_SH); a.cco = O(PQ_O_CC); a.ao = O(PQ_O_A); a.co = O(PQ_O_C);
+  a.dho = O(PQ_O_DH);
+  hipLaunchKernelGGL(k_profq<T>, dim3(c.nblk2), dim3(TPB), 0, st, c.B, c.vert, c.prm, a);
+  launch_map<T, OpKEdge>(c, kedge_tab<T>(w.kmo, w.kho, in[PQ_FSM]), OutTab<T>{{O(PQ_O_KM), O(PQ_O_KH)}});
+  return hipGetLastError() == hipSuccess ? POM_OK : POM_E_HIP;
+}
+
+// grad[k] += (d outputs / d in[k])^T cot for the 17 differentiable inputs
+// (q2f .. rho, wubot, wvbot, etf, dt; null entries skipped)
+template <class T>
+int run_profq_adj(const pom_batch* b, const pom_params_t* prm, const void* vg, const void* const* in,
+                  const void* const* cot, void* const* grad, void* ws, size_t wsb, hipStream_t st) {
+  Ctx<T> c = make_ctx<T>(b, prm, nullptr, vg, st);
+  if (c.nblk2 == 0) return POM_OK;
+  const dim3 g2(c.nblk2), tb(TPB);
+  Arena A{static_cast<char*>(ws), 0, wsb, false};
+  ProfqWs<T> w{};
+  w.plan(A, b->n3, b->n2);
+  if (A.off > wsb) return POM_E_ARG;
+  ProfqArgs<T> f = profq_args<T>(in);
+  f.q2o = w.q2o; f.q2lo = w.q2lo; f.lo = w.lo; f.kqo = w.kqo; f.kmo = w.kmo; f.kho = w.kho;
+  f.tape = w.tape;
+  hipLaunchKernelGGL(k_profq<T>, g2, tb, 0, st, c.B, c.vert, c.prm, f);
+  auto C = [&](int k) { return static_cast<const T*>(cot[k]); };
+  auto G = [&](int k) { return static_cast<T*>(grad[k]); };
+  {   // the edge copy reversed: cotangents of the raw km, kh (assigned at every point)
+    COutTab<T> ob{{C(PQ_O_KM), C(PQ_O_KH)}};
+    T* ib[OpKEdge::NIN] = {w.b_kmo, w.b_kho};
+    if (C(PQ_O_KM) || C(PQ_O_KH)) {
+      launch_gather<T, OpKEdge>(c, kedge_tab<T>(w.kmo, w.kho, in[PQ_FSM]), ob, ib,
+                                (1u << OpKEdge::KM) | (1u << OpKEdge::KH));
+    }
+  }
+  ProfqAdjArgs<T> a{};
+  a.fw = f;
+  a.b_q2o = C(PQ_O_Q2); a.b_q2lo = C(PQ_O_Q2L); a.b_lo = C(PQ_O_L); a.b_kqo = C(PQ_O_KQ);
+  const bool kcot = C(PQ_O_KM) || C(PQ_O_KH);
+  a.b_kmo = kcot ? w.b_kmo : nullptr;
+  a.b_kho = kcot ? w.b_kho : nullptr;
+  a.b_q2f = G(PQ_Q2F); a.b_q2lf = G(PQ_Q2LF); a.b_q2 = G(PQ_Q2); a.b_q2b = G(PQ_Q2B); a.b_q2lb = G(PQ_Q2LB);
+  a.b_km = G(PQ_KM); a.b_kh = G(PQ_KH); a.b_kq = G(PQ_KQ); a.b_u = G(PQ_U); a.b_v = G(PQ_V); a.b_t = G(PQ_T);
+  a.b_s = G(PQ_S); a.b_rho = G(PQ_RHO); a.b_wubot = G(PQ_WUBOT); a.b_wvbot = G(PQ_WVBOT); a.b_etf = G(PQ_ETF);
+  a.b_dt = G(PQ_DT);
+  a.s_unb = w.unb; a.s_vnb = w.vnb; a.s_wubnb = w.wubnb; a.s_wvbnb = w.wvbnb;
+  a.r_bl = w.r3[0]; a.r_bgh = w.r3[1]; a.r_bq2ln = w.r3[2]; a.r_ba = w.r3[3]; a.r_bc = w.r3[4]; a.r_bkn = w.r3[5];
+  a.r_bdtef = w.r3[6]; a.r_bboy = w.r3[7]; a.r_bq2bot = w.r2[0]; a.r_bdh = w.r2[1];
+  hipLaunchKernelGGL(k_profq_adj_a<T>, g2, tb, 0, st, c.B, c.vert, c.prm, a);
+  hipLaunchKernelGGL(k_profq_adj_b<T>, g2, tb, 0, st, c.B, c.vert, c.prm, a);
+  hipLaunchKernelGGL(k_profq_adj_c1<T>, g2, tb, 0, st, c.B, c.vert, c.prm, a);
+  hipLaunchKernelGGL(k_profq_adj_c2<T>, g2, tb, 0, st, c.B, c.vert, c.prm, a);
+  hipLaunchKernelGGL(k_profq_adj2<T>, dim3(c.nblk3), tb, 0, st, c.B, a);
+  return hipGetLastError() == hipSuccess ? POM_OK : POM_E_HIP;
+}
+
+enum { PT_F, PT_KH, PT_WFSURF, PT_FSURF, PT_ETF, PT_H, PT_NIN };
+enum { PT_O_F, PT_O_A, PT_O_C, PT_O_EE, PT_O_GG, PT_O_DH, PT_NOUT };
+
+template <class T>
+int run_proft(const pom_batch* b, const pom_params_t* prm, const void* vg, const void* const* in, int nbc,
+              void* const* out, const void* cot, void* const* grad, hipStream_t st, bool adjoint) {
+  Ctx<T> c = make_ctx<T>(b, prm, nullptr, vg, st);
+  if (c.nblk2 == 0) return POM_OK;
+  auto I = [&](int k) { return static_cast<const T*>(in[k]); };
+  if (!adjoint) {
+    auto O = [&](int k) { return static_cast<T*>(out[k]); };
+    ProftArgs<T> a{I(PT_F), I(PT_KH), I(PT_ETF), I(PT_H), I(PT_WFSURF), I(PT_FSURF), O(PT_O_F), nbc,
+                   O(PT_O_A), O(PT_O_C), O(PT_O_EE), O(PT_O_GG), O(PT_O_DH)};
+    hipLaunchKernelGGL(k_proft<T>, dim3(c.nblk2), dim3(TPB), 0, st, c.B, c.vert, c.prm, a);
+  } else {
+    ProftAdjArgs<T> a{};
+    a.f = I(PT_F); a.kh = I(PT_KH); a.etf = I(PT_ETF); a.h = I(PT_H); a.wfsurf = I(PT_WFSURF);
+    a.fsurf = I(PT_FSURF); a.b_out = static_cast<const T*>(cot);
+    a.b_f = static_cast<T*>(grad[0]); a.b_kh = static_cast<T*>(grad[1]); a.b_etf = static_cast<T*>(grad[2]);
+    a.nbc = nbc;
+    hipLaunchKernelGGL(k_proft_adj<T>, dim3(c.nblk2), dim3(TPB), 0, st, c.B, c.vert, c.prm, a);
   }
   return hipGetLastError() == hipSuccess ? POM_OK : POM_E_HIP;
 }
@@ -2161,7 +2338,7 @@ int pom_bcond(const pom_batch* b, const pom_params_t* prm, const void* ramp, con
               const void* const* fields, const void* const* geom, void* const* out, void* stream) {
   const int nf = bcond_nfields(idx);
   if (!b || !prm || !ramp || !vgrid || !fields || !geom || !out || nf < 0) return POM_E_ARG;
-  if (prm->core != POM_CORE_EXPLICIT) return POM_E_UNSUPPORTED;
+  if (prm->core == POM_CORE_IMPLICIT && prm->boundary != POM_BOUNDARY_CLOSED) return POM_E_UNSUPPORTED;
   for (int q = 0; q < nf; ++q)
     if (!fields[q]) return POM_E_ARG;
   for (int q = 0; q < bcond_nout(idx); ++q)
@@ -2178,7 +2355,7 @@ int pom_bcond_adj(const pom_batch* b, const pom_params_t* prm, const void* ramp,
                   void* const* grad_fields, void* stream) {
   const int nf = bcond_nfields(idx);
   if (!b || !prm || !ramp || !vgrid || !fields || !geom || !cot_out || !grad_fields || nf < 0) return POM_E_ARG;
-  if (prm->core != POM_CORE_EXPLICIT) return POM_E_UNSUPPORTED;
+  if (prm->core == POM_CORE_IMPLICIT && prm->boundary != POM_BOUNDARY_CLOSED) return POM_E_UNSUPPORTED;
   for (int q = 0; q < nf; ++q)
     if (!fields[q]) return POM_E_ARG;
   for (int q = 0; q < POM_NGEOM_BASE; ++q)
@@ -2211,6 +2388,58 @@ int pom_baropg_adj(const pom_batch* b, const pom_params_t* prm, const void* ramp
   return run_baropg<double>(b, prm, ramp, vgrid, in, nullptr, cot_out, grad_in, st, true);
 }
 
+
+int pom_profq_workspace(const pom_batch* b, size_t* bytes) {
+  if (!b || !bytes) return POM_E_ARG;
+  *bytes = b->dtype == POM_F32 ? profq_ws_size<float>(b) : profq_ws_size<double>(b);
+  return POM_OK;
+}
+
+int pom_profq(const pom_batch* b, const pom_params_t* prm, const void* vgrid, const void* const* in,
+              void* const* out, void* workspace, size_t ws_bytes, void* stream) {
+  if (!b || !prm || !vgrid || !in || !out || !workspace) return POM_E_ARG;
+  for (int k = 0; k < PQ_NIN; ++k)
+    if (!in[k]) return POM_E_ARG;
+  for (int k = 0; k < PQ_NDIFF_OUT; ++k)
+    if (!out[k]) return POM_E_ARG;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (b->dtype == POM_F32) return run_profq<float>(b, prm, vgrid, in, out, workspace, ws_bytes, st);
+  return run_profq<double>(b, prm, vgrid, in, out, workspace, ws_bytes, st);
+}
+
+int pom_profq_adj(const pom_batch* b, const pom_params_t* prm, const void* vgrid, const void* const* in,
+                  const void* const* cot_out, void* const* grad_in, void* workspace, size_t ws_bytes, void* stream) {
+  if (!b || !prm || !vgrid || !in || !cot_out || !grad_in || !workspace) return POM_E_ARG;
+  for (int k = 0; k < PQ_NIN; ++k)
+    if (!in[k]) return POM_E_ARG;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (b->dtype == POM_F32) return run_profq_adj<float>(b, prm, vgrid, in, cot_out, grad_in, workspace, ws_bytes, st);
+  return run_profq_adj<double>(b, prm, vgrid, in, cot_out, grad_in, workspace, ws_bytes, st);
+}
+
+int pom_proft(const pom_batch* b, const pom_params_t* prm, const void* vgrid, const void* const* in, int32_t nbc,
+              void* const* out, void* stream) {
+  if (!b || !prm || !vgrid || !in || !out || !out[0]) return POM_E_ARG;
+  if (nbc == 2 || nbc == 4) return POM_E_UNSUPPORTED;   // shortwave absorption: not built
+  if (nbc != 1 && nbc != 3) return POM_E_ARG;
+  for (int k = 0; k < PT_NIN; ++k)
+    if (!in[k]) return POM_E_ARG;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (b->dtype == POM_F32) return run_proft<float>(b, prm, vgrid, in, nbc, out, nullptr, nullptr, st, false);
+  return run_proft<double>(b, prm, vgrid, in, nbc, out, nullptr, nullptr, st, false);
+}
+
+int pom_proft_adj(const pom_batch* b, const pom_params_t* prm, const void* vgrid, const void* const* in,
+                  int32_t nbc, const void* cot_f, void* const* grad_in, void* stream) {
+  if (!b || !prm || !vgrid || !in || !cot_f || !grad_in) return POM_E_ARG;
+  if (nbc == 2 || nbc == 4) return POM_E_UNSUPPORTED;
+  if (nbc != 1 && nbc != 3) return POM_E_ARG;
+  for (int k = 0; k < PT_NIN; ++k)
+    if (!in[k]) return POM_E_ARG;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (b->dtype == POM_F32) return run_proft<float>(b, prm, vgrid, in, nbc, nullptr, cot_f, grad_in, st, true);
+  return run_proft<double>(b, prm, vgrid, in, nbc, nullptr, cot_f, grad_in, st, true);
+}
 
 int pom_elliptic_apply(const pom_batch* b, const pom_params_t* prm, const void* const* geom, const void* eta,
                        const void* d, void* out, void* stream) {
