@@ -115,6 +115,12 @@ typedef struct {
    * equal the default mode's.  0: the host reads the device's count of
    * unconverged members every few iterations and stops when it is zero. */
   int32_t eta_capture, reserved0;
+  /* spatial Shapiro filters of the semi-implicit core's closed-basin numerics
+   * (implicit_p0_fixed/numerics/double_gyre.py:78-184): weights of the new
+   * T, S and of the filtered internal velocity (0 in production,
+   * v1/core_spec.py:24-29); read only with core = POM_CORE_IMPLICIT and the
+   * closed basin. */
+  double tracer_shapiro, velocity_shapiro;
 } pom_params_t;
 
 typedef struct pom_batch pom_batch;

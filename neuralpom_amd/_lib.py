@@ -54,7 +54,8 @@ class Params(C.Structure):
         "rfn", "rfs")] + [(n, C.c_int32) for n in ("isplit", "ispadv", "ntp", "nbct", "nbcs", "iext")] + [
         ("vmaxl", C.c_double)] + [(n, C.c_int32) for n in ("core", "eta_max_iter", "eta_precond", "boundary")] + [
         (n, C.c_double) for n in ("eta_tol", "eta_rel_tol", "gamma_baro", "tide_wt")] + [
-        (n, C.c_int32) for n in ("eta_capture", "reserved0")]
+        (n, C.c_int32) for n in ("eta_capture", "reserved0")] + [
+        (n, C.c_double) for n in ("tracer_shapiro", "velocity_shapiro")]
 
 
 _lib = None

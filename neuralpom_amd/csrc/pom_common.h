@@ -465,6 +465,7 @@ struct Params {
   double alpha, smoth, horcon, grav, rhoref, kappa, umol, tprni, small;
   double tbias, sbias, hmax, rfe, rfw, rfn, rfs;
   double gamma_baro;                               // post-solve Shapiro weight (semi-implicit core)
+  double gamma_ts, gamma_uv;                       // tracer / internal-velocity Shapiro weights (semi-implicit core)
   int32_t isplit, ispadv, ntp, nbct, nbcs, iext;   // iext: current substep
   int32_t core;                                    // 0 explicit, 1 semi-implicit
 };

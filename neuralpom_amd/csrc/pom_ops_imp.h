@@ -117,7 +117,9 @@ template <int IDX> struct OpImpBcond {
 // closed-basin bcond 4 + T/S Asselin filter + tracer mask (time_loop.py:548-575,
 // numerics/double_gyre.py:78-130 with gamma 0).  outputs t, tb, s, sb.
 // =====================================================================
-struct OpImpBc4Filter {
+// MASKED = false: the new T, S are left unmasked, for the spatial Shapiro
+// filter that follows them (OpShapiro multiplies by fsm after filtering)
+template <bool MASKED> struct OpImpBc4FilterG {
   enum { TF, SF, TT, SS, TB, SB, N3, NIN = N3 };
   enum { GFSM };
   static constexpr int NOUT = 4;
@@ -129,7 +131,8 @@ struct OpImpBc4Filter {
     constexpr int FC = (O < 2) ? TT : SS;
     constexpr int FBK = (O < 2) ? TB : SB;
     if constexpr (O == 0 || O == 2) {
-      return edge_copy3<FNEW>(A) * G2(GFSM, 0, 0);
+      if constexpr (MASKED) return edge_copy3<FNEW>(A) * G2(GFSM, 0, 0);
+      else return edge_copy3<FNEW>(A);
     } else {
       // t, tb carry advt1's bottom copy f(kb) = f(kbm1)
       const bool bot = A.p.k == KB - 1;
@@ -137,6 +140,28 @@ struct OpImpBc4Filter {
       auto tbk = BR(bot, AT(FBK, 0, 0, -1), AT(FBK, 0, 0, 0));
       return tc + (0.5 * prm.smoth) * (edge_copy3<FNEW>(A) + tbk - 2.0 * tc);
     }
+  }
+};
+using OpImpBc4Filter = OpImpBc4FilterG<true>;
+
+// =====================================================================
+// spatial Shapiro filter of a 3-D field on the interior columns, levels
+// 0-14, then the mask (numerics/double_gyre.py:78-130 for T, S with fsm,
+// :132-184 for the filtered u, v with dum / dvm).  VEL selects the weight.
+// =====================================================================
+template <bool VEL> struct OpShapiro {
+  enum { X, N3, NIN = N3 };
+  enum { GMASK };
+  static constexpr int NOUT = 1;
+  static constexpr bool OUT3 = true;
+  static constexpr int RI = 1, RJ = 1, RK = 0;
+  template <int O, class AC> __device__ static auto eval(const AC& A, const Params& prm) {
+    const double gamma = VEL ? prm.gamma_uv : prm.gamma_ts;
+    const bool in = interior(A) && A.p.k < KBM1;
+    auto f = BR(in, AT(X, 0, 0, 0) + (gamma / 4.0) * (AT(X, 1, 0, 0) + AT(X, -1, 0, 0) + AT(X, 0, 1, 0) +
+                                                      AT(X, 0, -1, 0) - 4.0 * AT(X, 0, 0, 0)),
+                AT(X, 0, 0, 0));
+    return f * G2(GMASK, 0, 0);
   }
 };
 

@@ -275,6 +275,10 @@ template <class T> struct Step {
       *vf2, *tpsu;
   bool tape = false;     // the forward keeps profq's tape for the reverse kernels (adjoint recompute)
   ProfqTape<T> qt{};
+  // spatial Shapiro filters of the closed-basin numerics (weights > 0): the
+  // unfiltered new T, S and filtered u, v go through these before the filter
+  bool shap_ts = false, shap_uv = false;
+  T *tsh = nullptr, *ssh = nullptr, *ush = nullptr, *vsh = nullptr;
 
   int64_t n3() const { return b->n3; }
   int64_t n2() const { return b->n2; }
@@ -356,6 +360,8 @@ template <class T> struct Step {
     uf2 = A.get<T>(N3);
     vf2 = A.get<T>(N3);
     tpsu = A.get<T>(N2);
+    if (shap_ts) { tsh = A.get<T>(N3); ssh = A.get<T>(N3); }
+    if (shap_uv) { ush = A.get<T>(N3); vsh = A.get<T>(N3); }
     if (tape) {
       qt.ee1 = A.get<T>(N3); qt.gg1 = A.get<T>(N3); qt.q2n = A.get<T>(N3); qt.ee2 = A.get<T>(N3);
       qt.gg2 = A.get<T>(N3); qt.q2ln = A.get<T>(N3); qt.ghr = A.get<T>(N3); qt.kn = A.get<T>(N3);
@@ -622,6 +628,12 @@ template <class T> struct Step {
     t.f[OpImpBc4Filter::TF] = tf1; t.f[OpImpBc4Filter::SF] = sf1; t.f[OpImpBc4Filter::TT] = in[POM_T];
     t.f[OpImpBc4Filter::SS] = in[POM_S]; t.f[OpImpBc4Filter::TB] = in[POM_TB]; t.f[OpImpBc4Filter::SB] = in[POM_SB];
     t.g[OpImpBc4Filter::GFSM] = g[POM_G_FSM];
+    return t;
+  }
+  FieldTab<T> t_shap(const T* x, const T* mask) const {
+    FieldTab<T> t{};
+    t.f[OpShapiro<false>::X] = x;
+    t.g[OpShapiro<false>::GMASK] = mask;
     return t;
   }
   FieldTab<T> t_ibc3() const {
@@ -1008,7 +1020,11 @@ template <class T> struct Step {
       hipLaunchKernelGGL(k_proft<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a_proft(true));
       if (open)
         launch_map<T, OpBc4FilterExt>(c, t_bc4ext(), OutTab<T>{{out[POM_T], out[POM_TB], out[POM_S], out[POM_SB]}});
-      else if (imp)
+      else if (imp && shap_ts) {   // numerics/double_gyre.py:78-130 with gamma > 0
+        launch_map<T, OpImpBc4FilterG<false>>(c, t_ibc4(), OutTab<T>{{tsh, out[POM_TB], ssh, out[POM_SB]}});
+        launch_map<T, OpShapiro<false>>(c, t_shap(tsh, g[POM_G_FSM]), OutTab<T>{{out[POM_T]}});
+        launch_map<T, OpShapiro<false>>(c, t_shap(ssh, g[POM_G_FSM]), OutTab<T>{{out[POM_S]}});
+      } else if (imp)
         launch_map<T, OpImpBc4Filter>(c, t_ibc4(), OutTab<T>{{out[POM_T], out[POM_TB], out[POM_S], out[POM_SB]}});
       else
         launch_map<T, OpBc4Filter>(c, t_bc4(), OutTab<T>{{out[POM_T], out[POM_TB], out[POM_S], out[POM_SB]}});
@@ -1020,7 +1036,14 @@ template <class T> struct Step {
       if (open) launch_map<T, OpBc3D>(c, t_bc3d(), OutTab<T>{{uf2, vf2}});
       else if (imp) launch_map<T, OpImpBc3>(c, t_ibc3(), OutTab<T>{{uf2, vf2}});
       else launch_map<T, OpBc3>(c, t_bc3(), OutTab<T>{{uf2, vf2}});
-      {
+      if (shap_uv) {   // numerics/double_gyre.py:132-184 with gamma > 0: filter, then mask
+        UVFiltArgs<T> a{uf2, in[POM_UB], u1, ush, out[POM_U], nullptr};
+        hipLaunchKernelGGL(k_uvfilt<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a);
+        UVFiltArgs<T> bv{vf2, in[POM_VB], v1, vsh, out[POM_V], nullptr};
+        hipLaunchKernelGGL(k_uvfilt<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, bv);
+        launch_map<T, OpShapiro<true>>(c, t_shap(ush, g[POM_G_DUM]), OutTab<T>{{out[POM_UB]}});
+        launch_map<T, OpShapiro<true>>(c, t_shap(vsh, g[POM_G_DVM]), OutTab<T>{{out[POM_VB]}});
+      } else {
         UVFiltArgs<T> a{uf2, in[POM_UB], u1, out[POM_UB], out[POM_U], imp ? g[POM_G_DUM] : nullptr};
         hipLaunchKernelGGL(k_uvfilt<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a);
         UVFiltArgs<T> bv{vf2, in[POM_VB], v1, out[POM_VB], out[POM_V], imp ? g[POM_G_DVM] : nullptr};
@@ -1366,8 +1389,19 @@ template <class T> struct Adj {
   void internal_rev(T* b_etf30, T* b_egf30, T* b_utf30, T* b_vtf30, T* b_d30) {
     const dim3 g2(c.nblk2), tb(TPB);
     Step<T>& s = S;
-    // u/v Asselin filter and shift
-    {
+    // u/v Asselin filter and shift (with the Shapiro filter reversed first when on)
+    if (s.shap_uv) {
+      COutTab<T> ou{{bo[POM_UB]}};
+      T* iu[OpShapiro<true>::NIN] = {s3c};
+      launch_gather<T, OpShapiro<true>>(c, s.t_shap(s.ush, s.g[POM_G_DUM]), ou, iu, 1u);
+      COutTab<T> ov{{bo[POM_VB]}};
+      T* iv[OpShapiro<true>::NIN] = {s3d};
+      launch_gather<T, OpShapiro<true>>(c, s.t_shap(s.vsh, s.g[POM_G_DVM]), ov, iv, 1u);
+      UVFiltAdjArgs<T> a{s3c, bo[POM_U], b_uf2, bi[POM_UB], b_u1, nullptr};
+      hipLaunchKernelGGL(k_uvfilt_adj<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a);
+      UVFiltAdjArgs<T> bv{s3d, bo[POM_V], b_vf2, bi[POM_VB], b_v1, nullptr};
+      hipLaunchKernelGGL(k_uvfilt_adj<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, bv);
+    } else {
       UVFiltAdjArgs<T> a{bo[POM_UB], bo[POM_U], b_uf2, bi[POM_UB], b_u1, s.imp ? s.g[POM_G_DUM] : nullptr};
       hipLaunchKernelGGL(k_uvfilt_adj<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a);
       UVFiltAdjArgs<T> bv{bo[POM_VB], bo[POM_V], b_vf2, bi[POM_VB], b_v1, s.imp ? s.g[POM_G_DVM] : nullptr};
@@ -1424,6 +1458,16 @@ template <class T> struct Adj {
       T* ib[OpBc4FilterExt::NIN] = {b_tf1, b_sf1, bi[POM_T], bi[POM_S], bi[POM_TB], bi[POM_SB], b_u1, b_v1,
                                     bo[POM_W], nullptr, nullptr, bi[POM_DT]};
       launch_gather<T, OpBc4FilterExt>(c, s.t_bc4ext(), ob, ib);
+    } else if (s.imp && s.shap_ts) {   // Shapiro filter of the new T, S, then bcond 4 + T/S filter
+      COutTab<T> ot{{bo[POM_T]}};
+      T* it[OpShapiro<false>::NIN] = {s3c};
+      launch_gather<T, OpShapiro<false>>(c, s.t_shap(s.tsh, s.g[POM_G_FSM]), ot, it, 1u);
+      COutTab<T> os{{bo[POM_S]}};
+      T* is[OpShapiro<false>::NIN] = {s3d};
+      launch_gather<T, OpShapiro<false>>(c, s.t_shap(s.ssh, s.g[POM_G_FSM]), os, is, 1u);
+      COutTab<T> ob{{s3c, bo[POM_TB], s3d, bo[POM_SB]}};
+      T* ib[OpImpBc4FilterG<false>::NIN] = {b_tf1, b_sf1, bi[POM_T], bi[POM_S], bi[POM_TB], bi[POM_SB]};
+      launch_gather<T, OpImpBc4FilterG<false>>(c, s.t_ibc4(), ob, ib);
     } else if (s.imp) {   // closed-basin bcond 4 + T/S filter + tracer mask
       COutTab<T> ob{{bo[POM_T], bo[POM_TB], bo[POM_S], bo[POM_SB]}};
       T* ib[OpImpBc4Filter::NIN] = {b_tf1, b_sf1, bi[POM_T], bi[POM_S], bi[POM_TB], bi[POM_SB]};
@@ -1521,6 +1565,9 @@ template <class T> Ctx<T> make_ctx(const pom_batch* b, const pom_params_t* p, co
   q.isplit = p->isplit; q.ispadv = p->ispadv; q.ntp = p->ntp; q.nbct = p->nbct; q.nbcs = p->nbcs; q.iext = 0;
   q.core = p->core;
   q.gamma_baro = p->core == POM_CORE_IMPLICIT ? p->gamma_baro : 0.0;
+  const bool closed_imp = p->core == POM_CORE_IMPLICIT && p->boundary == POM_BOUNDARY_CLOSED;
+  q.gamma_ts = closed_imp ? p->tracer_shapiro : 0.0;
+  q.gamma_uv = closed_imp ? p->velocity_shapiro : 0.0;
   c.nblk3 = b->nblk3;
   c.nblk2 = b->nblk2;
   c.stream = st;
@@ -1650,6 +1697,8 @@ template <class T> void setup_pcg(Pcg<T>& w, const pom_batch* b, const pom_param
 template <class T> void setup_implicit(Step<T>& S, const pom_params_t* prm, const void* cache) {
   S.imp = prm->core == POM_CORE_IMPLICIT;
   S.open = S.imp && prm->boundary == POM_BOUNDARY_TIDE;
+  S.shap_ts = S.imp && !S.open && prm->tracer_shapiro != 0.0;
+  S.shap_uv = S.imp && !S.open && prm->velocity_shapiro != 0.0;
   S.tide_wt = prm->tide_wt;
   S.hp = prm;
   if (S.imp) setup_pcg(S.pw, S.b, prm, cache);

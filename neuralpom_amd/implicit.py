@@ -34,6 +34,8 @@ class ImplicitScalars(StepScalars):
     eta_max_iter: int = 200
     eta_preconditioner: str = "spectral"
     barotropic_shapiro: float = 0.0
+    tracer_shapiro: float = 0.0     # numerics/double_gyre.py:78-130 (closed basin)
+    velocity_shapiro: float = 0.0   # numerics/double_gyre.py:132-184 (closed basin)
     boundary: int = L.POM_BOUNDARY_CLOSED   # or POM_BOUNDARY_TIDE (open edges, tide nodes)
     tide_wt: float = 0.0                    # (speed pi / 180) t / 3600 of the tide this step
     # capture-safe eta solves (core.py:504-555): exactly eta_max_iter iterations
@@ -51,6 +53,8 @@ class ImplicitScalars(StepScalars):
             raise ValueError(f"eta preconditioner must be 'spectral' or 'jacobi', got {self.eta_preconditioner!r}")
         p.eta_precond = L.POM_PRECOND_SPECTRAL if pre == "spectral" else L.POM_PRECOND_JACOBI
         p.gamma_baro = float(self.barotropic_shapiro)
+        p.tracer_shapiro = float(self.tracer_shapiro)
+        p.velocity_shapiro = float(self.velocity_shapiro)
         p.boundary = int(self.boundary)
         p.tide_wt = float(self.tide_wt)
         p.eta_capture = int(bool(self.eta_capture))

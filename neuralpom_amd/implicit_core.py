@@ -88,9 +88,6 @@ class POM2K_Core(_ExplicitCore):
     def step_scalars(self, max_iter: int | None = None, tol: float | None = None) -> ImplicitScalars:
         if self.nadv != 1:
             raise NotImplementedError("only the centred advection scheme (nadv = 1) is implemented")
-        if float(self.tracer_shapiro) != 0.0 or float(self.velocity_shapiro) != 0.0:
-            raise NotImplementedError("the spatial Shapiro filters of tracers and internal velocity are not "
-                                      "implemented (production runs at gamma 0, v1/core_spec.py:24-29)")
         if float(getattr(self, "aam_min", 0.0) or 0.0) > 0.0:
             raise NotImplementedError("aam_min > 0 is not implemented (production: 0)")
         kw = {n: getattr(self, n) for n in _STEP_SCALARS}
@@ -98,9 +95,13 @@ class POM2K_Core(_ExplicitCore):
                   eta_rel_tol=float(self.eta_solver_rel_tol),
                   eta_max_iter=int(getattr(self, "eta_solver_max_iter", 200) if max_iter is None else max_iter),
                   eta_preconditioner=self.eta_solver_preconditioner,
-                  barotropic_shapiro=float(self.barotropic_shapiro))
+                  barotropic_shapiro=float(self.barotropic_shapiro),
+                  tracer_shapiro=float(self.tracer_shapiro), velocity_shapiro=float(self.velocity_shapiro))
         tide = self._tide_policy()
         if tide is not None:
+            if float(self.tracer_shapiro) != 0.0 or float(self.velocity_shapiro) != 0.0:
+                raise NotImplementedError("the tide policy's numerics have no spatial Shapiro filters "
+                                          "(tracer_shapiro / velocity_shapiro must be 0)")
             kw.update(boundary=L.POM_BOUNDARY_TIDE, tide_wt=tide.wt(tide.seconds(self)))
         sc = ImplicitScalars(**kw)
         if abs(sc.dti - self.dti) > 1e-9 * abs(self.dti):

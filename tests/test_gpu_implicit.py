@@ -156,3 +156,61 @@ def test_adjoint_matches_reference_vjp(dtype, tol):
         if not e <= tol:
             bad.append(f"{n}: {e:.2e}")
     assert not bad, "gradients off: " + ", ".join(bad)
+
+
+def _shapiro_state():
+    g = golden("implicit_shapiro_12x10.npz")
+    S = load_state(g, "in")
+    IM.configure(S)
+    S.barotropic_shapiro = S.tracer_shapiro = S.velocity_shapiro = float(g["scalar/shapiro_gamma"])
+    S.time0 = S.time = 0.1
+    _wind(S)
+    return S, g
+
+
+def _shapiro_scalars(S):
+    import dataclasses
+    return dataclasses.replace(_scalars(S), tracer_shapiro=S.tracer_shapiro, velocity_shapiro=S.velocity_shapiro)
+
+
+@pytest.mark.parametrize("label", ["a", "b"])
+def test_shapiro_step_matches_reference(label):
+    """Spatial Shapiro filters on (weights 0.05 on eta, T, S and the filtered
+    u, v): the fp64 step against the reference (1 and 3 steps)."""
+    S, g = _shapiro_state()
+    nstep = 1 if label == "a" else 3
+    print(f"[gpu] implicit step with Shapiro filters, {nstep} step(s)", flush=True)
+    ens, state, geom, vg, _ = _inputs([S], torch.float64)
+    sc = _shapiro_scalars(S)
+    iters = torch.zeros(1, dtype=torch.int32, device="cuda")
+    its = []
+    for it in range(1, nstep + 1):
+        ramp = torch.tensor([ramp_of(S, it)], dtype=torch.float64, device="cuda")
+        state, vamax = ens.step(state, geom, sc, ramp, vg, True, iters)
+        its.append(int(iters[0]))
+    assert its == list(g[f"{label}/iters"])
+    bad = []
+    for n in L.FIELDS:
+        e = rel_err(unpack(ens, state[n], n)[0].cpu(), torch.from_numpy(g[f"{label}/{n}"]))
+        if not e <= (1e-9 if label == "b" else 1e-10):
+            bad.append(f"{n}: {e:.2e}")
+    assert not bad, bad
+
+
+def test_shapiro_adjoint_matches_reference_vjp():
+    S, g = _shapiro_state()
+    S.eta_solver_rel_tol, S.eta_solver_tol = 1e-12, 0.0
+    print("[gpu] implicit adjoint with Shapiro filters", flush=True)
+    ens, state, geom, vg, _ = _inputs([S], torch.float64)
+    sc = _shapiro_scalars(S)
+    ramp = torch.tensor([ramp_of(S, 1)], dtype=torch.float64, device="cuda")
+    cot = {k[4:]: torch.from_numpy(g[k]).reshape(-1).to("cuda", torch.float64) for k in g.files
+           if k.startswith("cot/")}
+    grads = ens.step_adj(state, geom, sc, ramp, vg, True, cot)
+    torch.cuda.synchronize()
+    bad = []
+    for n in VJP_IN:
+        e = rel_err(grads[n].cpu().reshape(g[f"grad/{n}"].shape), torch.from_numpy(g[f"grad/{n}"]))
+        if not e <= 1e-8:
+            bad.append(f"{n}: {e:.2e}")
+    assert not bad, "gradients off: " + ", ".join(bad)

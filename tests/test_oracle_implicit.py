@@ -202,3 +202,25 @@ def test_spectral_preconditioner_inverts_constant_operator(steps):
     x[1:-1, 1:-1] = torch.randn((S.im - 2, S.jm - 2), generator=g, dtype=torch.float64)
     y = IM.spectral_apply(S, IM.elliptic_apply(S, x, S.h), P)
     assert_rel(y, x, 1e-12, "M^-1 A x")
+
+
+@pytest.mark.parametrize("label", ["a", "b"])
+def test_shapiro_steps_match_reference(label):
+    """The spatial Shapiro filters (barotropic, tracer and velocity weights
+    0.05, numerics/double_gyre.py:78-184): 1 and 3 steps against the
+    reference (tests/golden/make_golden_shapiro.py)."""
+    g = golden(f"implicit_shapiro_{TAG}.npz")
+    S = _state(g)
+    gamma = float(g["scalar/shapiro_gamma"])
+    S.barotropic_shapiro = S.tracer_shapiro = S.velocity_shapiro = gamma
+    S.time0 = S.time = 0.1
+    S.iend = 1 if label == "a" else 3
+    S.eta_iters = []
+    IM.run_time_loop(S)
+    assert S.eta_iters == list(g[f"{label}/iters"])
+    bad = []
+    for n in DYNAMIC:
+        e = rel_err(getattr(S, n), T(g, f"{label}/{n}"))
+        if not e <= (1e-10 if label == "b" else 1e-11):
+            bad.append(f"{n}: {e:.2e}")
+    assert not bad, bad
