@@ -121,6 +121,9 @@ typedef struct {
    * v1/core_spec.py:24-29); read only with core = POM_CORE_IMPLICIT and the
    * closed basin. */
   double tracer_shapiro, velocity_shapiro;
+  /* lower bound of the semi-implicit core's Smagorinsky viscosity
+   * (implicit_p0_fixed/time_loop.py:186-188; 0 = none, the production value) */
+  double aam_min;
 } pom_params_t;
 
 typedef struct pom_batch pom_batch;

@@ -466,6 +466,7 @@ struct Params {
   double tbias, sbias, hmax, rfe, rfw, rfn, rfs;
   double gamma_baro;                               // post-solve Shapiro weight (semi-implicit core)
   double gamma_ts, gamma_uv;                       // tracer / internal-velocity Shapiro weights (semi-implicit core)
+  double aam_min;                                  // Smagorinsky viscosity floor (semi-implicit core; 0: none)
   int32_t isplit, ispadv, ntp, nbct, nbcs, iext;   // iext: current substep
   int32_t core;                                    // 0 explicit, 1 semi-implicit
 };

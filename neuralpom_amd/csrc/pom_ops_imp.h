@@ -45,7 +45,8 @@ struct OpSmagImp {
       auto ut = 0.25 * (AT(U, 0, 1, 0) + AT(U, 1, 1, 0) - AT(U, 0, -1, 0) - AT(U, 1, -1, 0)) / dy;
       auto vt = 0.25 * (AT(V, 1, 0, 0) + AT(V, 1, 1, 0) - AT(V, -1, 0, 0) - AT(V, -1, 1, 0)) / dx;
       auto t3 = 0.5 * sq(ut + vt);
-      return prm.horcon * dx * dy * strain(t1 + t2 + t3);
+      auto visc = prm.horcon * dx * dy * strain(t1 + t2 + t3);
+      return BR(prm.aam_min > 0.0, clamp_min(visc, prm.aam_min), visc);   // time_loop.py:186-188
     };
     return BR(in, f(), AT(AAM, 0, 0, 0));
   }

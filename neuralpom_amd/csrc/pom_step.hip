@@ -1568,6 +1568,7 @@ template <class T> Ctx<T> make_ctx(const pom_batch* b, const pom_params_t* p, co
   const bool closed_imp = p->core == POM_CORE_IMPLICIT && p->boundary == POM_BOUNDARY_CLOSED;
   q.gamma_ts = closed_imp ? p->tracer_shapiro : 0.0;
   q.gamma_uv = closed_imp ? p->velocity_shapiro : 0.0;
+  q.aam_min = p->core == POM_CORE_IMPLICIT ? p->aam_min : 0.0;
   c.nblk3 = b->nblk3;
   c.nblk2 = b->nblk2;
   c.stream = st;

@@ -88,15 +88,14 @@ class POM2K_Core(_ExplicitCore):
     def step_scalars(self, max_iter: int | None = None, tol: float | None = None) -> ImplicitScalars:
         if self.nadv != 1:
             raise NotImplementedError("only the centred advection scheme (nadv = 1) is implemented")
-        if float(getattr(self, "aam_min", 0.0) or 0.0) > 0.0:
-            raise NotImplementedError("aam_min > 0 is not implemented (production: 0)")
         kw = {n: getattr(self, n) for n in _STEP_SCALARS}
         kw.update(eta_tol=float(getattr(self, "eta_solver_tol", 1.0e-12) if tol is None else tol),
                   eta_rel_tol=float(self.eta_solver_rel_tol),
                   eta_max_iter=int(getattr(self, "eta_solver_max_iter", 200) if max_iter is None else max_iter),
                   eta_preconditioner=self.eta_solver_preconditioner,
                   barotropic_shapiro=float(self.barotropic_shapiro),
-                  tracer_shapiro=float(self.tracer_shapiro), velocity_shapiro=float(self.velocity_shapiro))
+                  tracer_shapiro=float(self.tracer_shapiro), velocity_shapiro=float(self.velocity_shapiro),
+                  aam_min=float(getattr(self, "aam_min", 0.0) or 0.0))
         tide = self._tide_policy()
         if tide is not None:
             if float(self.tracer_shapiro) != 0.0 or float(self.velocity_shapiro) != 0.0:

@@ -10,7 +10,9 @@ of make_golden_implicit.py.  Output ``tests/golden/implicit_shapiro_12x10.npz``
 (float64): the perturbed input state ("in/"), the state after 1 step (a) and
 3 steps (b) from time0 = 0.1 d with their eta-solve iteration counts, and a
 seeded vector-Jacobian product of one step at a tight eta tolerance
-("cot/", "grad/").
+("cot/", "grad/").  The same file holds, under "m/", one step with the
+Shapiro filters off and the Smagorinsky floor aam_min = AAM_MIN
+(implicit_p0_fixed/time_loop.py:186-188), and its VJP ("mcot/", "mgrad/").
 """
 from __future__ import annotations
 
@@ -26,6 +28,7 @@ import make_golden_implicit as GI   # noqa: E402
 
 OUT = Path(__file__).resolve().parent
 GAMMA = 0.05
+AAM_MIN = 800.0
 
 
 def shapiro_core(im, jm):
@@ -83,6 +86,40 @@ def main(im=12, jm=10, seed=1234):
         grads = torch.autograd.grad(loss, [leaves[n] for n in GI.VJP_INPUTS], allow_unused=True)
         for name, gr in zip(GI.VJP_INPUTS, grads):
             out[f"grad/{name}"] = G._np(gr) if gr is not None else np.zeros(leaves[name].shape)
+        # ---- Smagorinsky floor, Shapiro filters off
+        for grad_pass in (False, True):
+            core = GI.build_core(im, jm)
+            core.aam_min = AAM_MIN
+            G.perturb(core, seed)
+            core.time0 = core.time = 0.1
+            core.eta_solver_rel_tol = 1e-12
+            core.eta_solver_tol = 0.0
+            core.iint = 0
+            core.iend = 1
+            if not grad_pass:
+                with torch.no_grad():
+                    core.run_time_loop()
+                for k, v in G._tensors(core).items():
+                    out[f"m/{k}"] = G._np(v)
+                out["m/iters"] = np.array(core._eta_solver_iteration_history, dtype=np.int64)
+                out["scalar/aam_min"] = np.array(AAM_MIN)
+                continue
+            leaves = {}
+            for name in GI.VJP_INPUTS:
+                leaf = getattr(core, name).detach().clone().requires_grad_(True)
+                leaves[name] = leaf
+                setattr(core, name, leaf.clone())
+            core.run_time_loop()
+            g = torch.Generator().manual_seed(seed + 2)
+            loss = 0.0
+            for name in GI.VJP_OUTPUTS:
+                o = getattr(core, name)
+                cot = torch.randn(o.shape, generator=g, dtype=torch.float64)
+                out[f"mcot/{name}"] = G._np(cot)
+                loss = loss + (o * cot).sum()
+            grads = torch.autograd.grad(loss, [leaves[n] for n in GI.VJP_INPUTS], allow_unused=True)
+            for name, gr in zip(GI.VJP_INPUTS, grads):
+                out[f"mgrad/{name}"] = G._np(gr) if gr is not None else np.zeros(leaves[name].shape)
         np.savez_compressed(OUT / f"implicit_shapiro_{im}x{jm}.npz", **out)
     finally:
         for name, fn in originals.items():

@@ -214,3 +214,42 @@ def test_shapiro_adjoint_matches_reference_vjp():
         if not e <= 1e-8:
             bad.append(f"{n}: {e:.2e}")
     assert not bad, "gradients off: " + ", ".join(bad)
+
+
+def _floor_state():
+    g = golden("implicit_shapiro_12x10.npz")
+    S = load_state(g, "in")
+    IM.configure(S)
+    S.aam_min = float(g["scalar/aam_min"])
+    S.eta_solver_rel_tol, S.eta_solver_tol = 1e-12, 0.0
+    S.time0 = S.time = 0.1
+    _wind(S)
+    return S, g
+
+
+def test_smagorinsky_floor_step_and_vjp_match_reference():
+    """aam_min = 800 m^2/s (implicit_p0_fixed/time_loop.py:186-188): one fp64
+    step and its VJP against the reference, Shapiro filters off."""
+    import dataclasses
+    S, g = _floor_state()
+    print("[gpu] implicit step + adjoint with the Smagorinsky floor", flush=True)
+    ens, state, geom, vg, sc = _inputs([S], torch.float64)
+    sc = dataclasses.replace(sc, aam_min=S.aam_min)
+    iters = torch.zeros(1, dtype=torch.int32, device="cuda")
+    ramp = torch.tensor([ramp_of(S, 1)], dtype=torch.float64, device="cuda")
+    out, _ = ens.step(state, geom, sc, ramp, vg, True, iters)
+    assert [int(iters[0])] == list(g["m/iters"])
+    bad = []
+    for n in L.FIELDS:
+        e = rel_err(unpack(ens, out[n], n)[0].cpu(), torch.from_numpy(g[f"m/{n}"]))
+        if not e <= 1e-10:
+            bad.append(f"{n}: {e:.2e}")
+    cot = {k[5:]: torch.from_numpy(g[k]).reshape(-1).to("cuda", torch.float64) for k in g.files
+           if k.startswith("mcot/")}
+    grads = ens.step_adj(state, geom, sc, ramp, vg, True, cot)
+    torch.cuda.synchronize()
+    for n in VJP_IN:
+        e = rel_err(grads[n].cpu().reshape(g[f"mgrad/{n}"].shape), torch.from_numpy(g[f"mgrad/{n}"]))
+        if not e <= 1e-8:
+            bad.append(f"grad {n}: {e:.2e}")
+    assert not bad, bad

@@ -224,3 +224,24 @@ def test_shapiro_steps_match_reference(label):
         if not e <= (1e-10 if label == "b" else 1e-11):
             bad.append(f"{n}: {e:.2e}")
     assert not bad, bad
+
+
+def test_smagorinsky_floor_step_matches_reference():
+    """aam_min = 800 m^2/s floors the Smagorinsky viscosity
+    (implicit_p0_fixed/time_loop.py:186-188; the perturbed state has cells
+    below it): one step against the reference with the Shapiro filters off."""
+    g = golden(f"implicit_shapiro_{TAG}.npz")
+    S = _state(g)
+    S.aam_min = float(g["scalar/aam_min"])
+    S.time0 = S.time = 0.1
+    S.eta_solver_rel_tol, S.eta_solver_tol = 1e-12, 0.0
+    S.iend = 1
+    S.eta_iters = []
+    IM.run_time_loop(S)
+    assert S.eta_iters == list(g["m/iters"])
+    bad = []
+    for n in DYNAMIC:
+        e = rel_err(getattr(S, n), T(g, f"m/{n}"))
+        if not e <= 1e-11:
+            bad.append(f"{n}: {e:.2e}")
+    assert not bad, bad
