@@ -406,6 +406,32 @@ int pom_joint_correction_adj(const pom_batch* b, const void* raw, const void* co
                              const double* scales, double cap, void* work, const void* const* cot_out,
                              const void* cot_diag, void* grad_raw, void* grad_et, void* stream);
 
+/* ---- The density-75 projection of the DND policy, fused
+ * (src/neuralpom/evaluation/density.py:14-64: eos_gradient,
+ * constrained_amplitude, apply_density75), run after pom_joint_correction.
+ *
+ * pom_density75_workspace: device scratch bytes of pom_density75.
+ * pom_density75: for every member, x = (after - before) / scale for T and S
+ *   on levels 0-14 (scales = {t, s}, nonzero); d = g / max(|g|, 1e-30) with
+ *   g = (drho/dT scale_t, drho/dS scale_s), the partials of the model EOS
+ *   (prm: tbias, sbias, grav, rhoref) at `before` in float64; w = art fsm
+ *   (h + et) dz; a = x . d.  Per level the Gram matrix and moments of d
+ *   (weights normalised by the level sum) give c through the 2 x 2
+ *   pseudo-inverse that drops eigenvalues at or below 1e-10 of the largest;
+ *   adjusted = a - d . c; proposed = (x - 0.25 adjusted d) scale; one factor
+ *   min(cap / max(peak, cap), 1), peak = max |proposed / scale| over T and S;
+ *   out = before + factor proposed on levels 0-14 (level 15 from before) for
+ *   T, its history level, S and its history level.  A member whose x is all
+ *   zero gets out = after (the reference returns early).  geom = {fsm, art,
+ *   h, et} (2-D); before / after / state_out = {t, tb, s, sb}; vgrid as for
+ *   the step (zz row 1, dz row 2).  diag (device double [M][3], may be null)
+ *   = (factor, max |constraint residual|, 1 if x was nonzero else 0; factor 1
+ *   and residual 0 then).  The density refresh that follows is pom_dens. */
+int pom_density75_workspace(const pom_batch* b, size_t* bytes);
+int pom_density75(const pom_batch* b, const pom_params_t* prm, const void* vgrid, const void* const* geom,
+                  const double* scales, double cap, void* work, const void* const* before,
+                  const void* const* after, void* const* state_out, void* diag, void* stream);
+
 /* Tracing: record HIP events around every launch of one stage on its stream
  * (stage = -1 disables).  pom_profile_read waits for the recorded events and
  * returns the summed kernel time and the number of launches since the last

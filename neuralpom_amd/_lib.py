@@ -115,6 +115,8 @@ def load():
     lib.pom_correction_workspace.argtypes = [vp, C.POINTER(C.c_size_t)]
     lib.pom_joint_correction.argtypes = [vp, vp, vpp, vp, dp, C.c_double, vp, vpp, vpp, vp, vp]
     lib.pom_joint_correction_adj.argtypes = [vp, vp, vpp, vp, dp, C.c_double, vp, vpp, vp, vp, vp, vp]
+    lib.pom_density75_workspace.argtypes = [vp, C.POINTER(C.c_size_t)]
+    lib.pom_density75.argtypes = [vp, C.POINTER(Params), vp, vpp, dp, C.c_double, vp, vpp, vpp, vpp, vp, vp]
     lib.pom_profile_stage.argtypes = [C.c_int32]
     lib.pom_profile_read.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_int32)]
     for fn in EXPORTED:
@@ -131,6 +133,7 @@ EXPORTED = ("pom_batch_create", "pom_batch_destroy", "pom_explicit_workspace", "
             "pom_bcond", "pom_bcond_adj", "pom_profq_workspace", "pom_profq", "pom_profq_adj", "pom_proft",
             "pom_proft_adj",
             "pom_correction_workspace", "pom_joint_correction", "pom_joint_correction_adj",
+            "pom_density75_workspace", "pom_density75",
             "pom_profile_stage", "pom_profile_read")
 STAGES = {"profq": 0, "profq_adj": 1, "extuv": 2, "advct": 3}
 

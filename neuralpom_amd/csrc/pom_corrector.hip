@@ -7,6 +7,8 @@
 // reference: src/neuralpom/hybrid/sparse_coupling.py:18-67 (volume_weights,
 // budget_project, paired_limit, apply_joint_correction).  The density refresh
 // that follows (:61) is the EOS entry point, called by the host after this.
+// The same file holds the density-75 projection that follows the correction
+// in the DND policy (src/neuralpom/evaluation/density.py:14-64).
 //
 // Per member, five launches: column pass (velocity column means, layer sums
 // of T w, S w, w, velocity peak) -> member reduction -> column pass (tracer
@@ -17,6 +19,7 @@
 // are read coalesced (consecutive threads, consecutive columns).
 #include "../../include/neuralpom_amd.h"
 #include "pom_batch.h"
+#include "pom_ops.h"
 
 using namespace pom;
 
@@ -77,7 +80,7 @@ template <class T> __device__ __forceinline__ void vel_inc(const CorrArgs<T>& a,
 
 // block reduction of NV per-thread doubles (sums, or maxima when MAX) into
 // part[lb * NP + off ...]; fixed order, so deterministic
-template <int NV, bool MAX>
+template <int NV, bool MAX, int STRIDE = NP>
 __device__ __forceinline__ void block_reduce(double* v, double* part, int off) {
   __shared__ double red[TPB / 64][NV];
 #pragma unroll
@@ -94,20 +97,20 @@ __device__ __forceinline__ void block_reduce(double* v, double* part, int off) {
     double x = red[0][threadIdx.x];
 #pragma unroll
     for (int w = 1; w < TPB / 64; ++w) x = MAX ? fmax(x, red[w][threadIdx.x]) : x + red[w][threadIdx.x];
-    part[(int64_t)logical_block() * NP + off + threadIdx.x] = x;
+    part[(int64_t)logical_block() * STRIDE + off + threadIdx.x] = x;
   }
   __syncthreads();
 }
 
 // per-member reduction of block partials [off, off + nv) (one block per member)
 __device__ __forceinline__ void member_reduce(const Batch& B, const double* part, int off, int nv, bool mx,
-                                              double* out) {
+                                              double* out, int stride = NP) {
   const int m = blockIdx.x;
   const int b0 = B.blk2[m], b1 = B.blk2[m + 1];
   for (int q = threadIdx.x; q < nv; q += TPB) {
     double x = 0.0;   // sums from zero; maxima of absolute values from zero
     for (int b = b0; b < b1; ++b) {
-      const double y = part[(int64_t)b * NP + off + q];
+      const double y = part[(int64_t)b * stride + off + q];
       x = mx ? fmax(x, y) : x + y;
     }
     out[q] = x;
@@ -517,6 +520,293 @@ int run_corr_adj(const pom_batch* b, const void* raw, const void* const* geom, c
   return hipGetLastError() == hipSuccess ? POM_OK : POM_E_HIP;
 }
 
+// ---------------------------------------------------------------- density-75
+// The DND projection that follows the joint correction
+// (src/neuralpom/evaluation/density.py:14-64).  Per point of the 15 active
+// levels: x = (after - before) / scale for T and S; the direction d = g / |g|
+// of g = (drho/dT scale_T, drho/dS scale_S), the EOS partials at the state
+// before the correction (the model EOS in float64, T and S seeded at the
+// point); the volume weight w; the amplitude a = x . d.  Per member and
+// level: the weighted 2 x 2 Gram matrix of d and the moments of d a (weights
+// normalised by the level's sum), whose pseudo-inverse (eigenvalues at or
+// below 1e-10 of the largest dropped) gives coefficients c.  Then
+// adjusted = a - d . c, proposed = (x - 0.25 adjusted d) scale, one cap
+// factor for the pair, and T, S and their history levels become before +
+// factor proposed (ghost level from before).  A member whose T/S did not
+// change keeps the corrected state (the reference's early return).
+// Launches: column pass -> member solve -> column pass -> member factor ->
+// apply.  Reductions in double, fixed order.
+constexpr int NPD = 96;   // block partials per logical block (90 sums + 1 max)
+enum {
+  D_C0 = 0,             // coefficient c_0 per level (15)
+  D_C1 = LEV,           // coefficient c_1 per level (15)
+  D_WC = 2 * LEV,       // max(level weight sum, 1e-30) (15)
+  D_ANY = 3 * LEV,      // max |x| over the member
+  D_FAC,                // cap factor of the projected increments
+  D_N
+};
+
+template <class T> struct D75Args {
+  const T *bt, *bs;      // T and S before the joint correction
+  const T *at, *as;      // after it
+  const T *fsm, *art, *h, *et;
+  const T* vg;           // [M][4][16]: zz row 1, dz row 2
+  double st, ss, cap;
+  Params prm;            // tbias, sbias, grav, rhoref of the EOS
+  double* part;
+  double* ms;            // D_N per member
+};
+
+// OpDens at one point with T and S both seeded, in double
+struct DensAcc {
+  Pos p;
+  double t, s, h, fsm, zzk;   // the density input is read only below the active levels
+  template <int F, int DI, int DJ, int DK> __device__ __forceinline__ auto at() const {
+    static_assert(DI == 0 && DJ == 0 && DK == 0, "the EOS is pointwise");
+    if constexpr (F == OpDens::TT) return DS<double, (1u << OpDens::TT)>{t, {1.0}};
+    else if constexpr (F == OpDens::SS) return DS<double, (1u << OpDens::SS)>{s, {1.0}};
+    else return P<double>{0.0};
+  }
+  template <int G, int DI, int DJ> __device__ __forceinline__ P<double> g() const {
+    return P<double>{G == OpDens::GH ? h : fsm};
+  }
+  __device__ __forceinline__ double zz(int) const { return zzk; }
+};
+
+struct D75Pt {
+  double x0, x1, d0, d1, w;
+};
+
+template <class T>
+__device__ __forceinline__ D75Pt d75_point(const D75Args<T>& a, Pos p, int64_t c2, int64_t c, int k) {
+  D75Pt r;
+  r.x0 = double(a.at[c] - a.bt[c]) / a.st;
+  r.x1 = double(a.as[c] - a.bs[c]) / a.ss;
+  p.k = k;
+  const DensAcc A{p, double(a.bt[c]), double(a.bs[c]), double(a.h[c2]), double(a.fsm[c2]),
+                  double(a.vg[(p.m * 4 + 1) * KB + k])};
+  const auto e = OpDens::template eval<0>(A, a.prm);
+  constexpr uint32_t M = mask_of<std::decay_t<decltype(e)>>::value;
+  static_assert(((M >> OpDens::TT) & 1u) && ((M >> OpDens::SS) & 1u), "both partials");
+  const double g0 = e.d[slot(M, OpDens::TT)] * a.st, g1 = e.d[slot(M, OpDens::SS)] * a.ss;
+  const double n = fmax(sqrt(g0 * g0 + g1 * g1), 1e-30);
+  r.d0 = g0 / n;
+  r.d1 = g1 / n;
+  const T col = a.art[c2] * a.fsm[c2] * (a.h[c2] + a.et[c2]);
+  r.w = double(col * a.vg[(p.m * 4 + 2) * KB + k]);
+  return r;
+}
+
+template <class T> __global__ void __launch_bounds__(TPB) k_d75_pass1(Batch B, D75Args<T> a) {
+  Pos p;
+  const bool live = pos2(B, p);
+  const int64_t c2 = live ? ix2(p, p.i, p.j) : 0, c = live ? ix3(p, p.i, p.j, 0) : 0;
+  double any = 0.0;
+  for (int k = 0; k < LEV; ++k) {
+    double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (live) {
+      const D75Pt q = d75_point(a, p, c2, c + k, k);
+      const double amp = q.x0 * q.d0 + q.x1 * q.d1;
+      v[0] = q.w;
+      v[1] = q.w * q.d0 * q.d0;
+      v[2] = q.w * q.d0 * q.d1;
+      v[3] = q.w * q.d1 * q.d1;
+      v[4] = q.w * q.d0 * amp;
+      v[5] = q.w * q.d1 * amp;
+      any = fmax(any, fmax(fabs(q.x0), fabs(q.x1)));
+    }
+    block_reduce<6, false, NPD>(v, a.part, 6 * k);
+  }
+  block_reduce<1, true, NPD>(&any, a.part, 6 * LEV);
+}
+
+// coefficients V diag(1/lambda, lambda > 1e-10 lambda_max) V^T r of the
+// symmetric [[g00, g01], [g01, g11]]: closed-form eigenpairs
+__device__ __forceinline__ void pinv2(double g00, double g01, double g11, double r0, double r1, double& c0,
+                                      double& c1) {
+  const double mean = 0.5 * (g00 + g11), diff = 0.5 * (g00 - g11);
+  const double rad = hypot(diff, g01);
+  const double l1 = mean + rad;                                   // largest
+  const double det = g00 * g11 - g01 * g01;
+  const double l0 = l1 > 0.0 ? det / l1 : mean - rad;             // smallest, without cancellation
+  double v0, v1;                                                  // unit eigenvector of l1
+  if (rad == 0.0) {
+    v0 = 1.0;
+    v1 = 0.0;
+  } else if (diff >= 0.0) {
+    v0 = diff + rad;
+    v1 = g01;
+  } else {
+    v0 = g01;
+    v1 = rad - diff;
+  }
+  const double nv = hypot(v0, v1);
+  v0 /= nv;
+  v1 /= nv;
+  const double cut = l1 * 1e-10;
+  const double i1 = l1 > cut ? 1.0 / fmax(l1, 1e-30) : 0.0;
+  const double i0 = l0 > cut ? 1.0 / fmax(l0, 1e-30) : 0.0;
+  const double p1 = (v0 * r0 + v1 * r1) * i1;                      // along (v0, v1)
+  const double p0 = (-v1 * r0 + v0 * r1) * i0;                     // along (-v1, v0)
+  c0 = v0 * p1 - v1 * p0;
+  c1 = v1 * p1 + v0 * p0;
+}
+
+template <class T> __global__ void __launch_bounds__(TPB) k_d75_member1(Batch B, D75Args<T> a) {
+  double* ms = a.ms + (int64_t)blockIdx.x * D_N;
+  __shared__ double s[6 * LEV + 1];
+  member_reduce(B, a.part, 0, 6 * LEV, false, s, NPD);
+  member_reduce(B, a.part, 6 * LEV, 1, true, s + 6 * LEV, NPD);
+  __syncthreads();
+  const int k = threadIdx.x;
+  if (k < LEV) {
+    const double* q = s + 6 * k;
+    const double wc = fmax(q[0], 1e-30);
+    double c0, c1;
+    pinv2(q[1] / wc, q[2] / wc, q[3] / wc, q[4] / wc, q[5] / wc, c0, c1);
+    ms[D_C0 + k] = c0;
+    ms[D_C1 + k] = c1;
+    ms[D_WC + k] = wc;
+  }
+  if (threadIdx.x == 0) ms[D_ANY] = s[6 * LEV];
+}
+
+// proposed increments (already times the scales) of one point
+template <class T>
+__device__ __forceinline__ void d75_proposed(const D75Args<T>& a, const double* ms, const D75Pt& q, int k,
+                                             double& adj, double& p0, double& p1) {
+  adj = (q.x0 * q.d0 + q.x1 * q.d1) - (q.d0 * ms[D_C0 + k] + q.d1 * ms[D_C1 + k]);
+  const double ch = (0.75 - 1.0) * adj;
+  p0 = (q.x0 + ch * q.d0) * a.st;
+  p1 = (q.x1 + ch * q.d1) * a.ss;
+}
+
+template <class T> __global__ void __launch_bounds__(TPB) k_d75_pass2(Batch B, D75Args<T> a) {
+  Pos p;
+  const bool live = pos2(B, p);
+  const int64_t c2 = live ? ix2(p, p.i, p.j) : 0, c = live ? ix3(p, p.i, p.j, 0) : 0;
+  const double* ms = a.ms + (int64_t)(live ? p.m : 0) * D_N;
+  double peak = 0.0;
+  for (int k = 0; k < LEV; ++k) {
+    double v[2] = {0.0, 0.0};
+    if (live) {
+      const D75Pt q = d75_point(a, p, c2, c + k, k);
+      double adj, p0, p1;
+      d75_proposed(a, ms, q, k, adj, p0, p1);
+      v[0] = q.w * q.d0 * adj;
+      v[1] = q.w * q.d1 * adj;
+      peak = fmax(peak, fmax(fabs(p0 / a.st), fabs(p1 / a.ss)));
+    }
+    block_reduce<2, false, NPD>(v, a.part, 2 * k);
+  }
+  block_reduce<1, true, NPD>(&peak, a.part, 2 * LEV);
+}
+
+__global__ void __launch_bounds__(TPB) k_d75_member2(Batch B, double* ms_all, const double* part, double cap,
+                                                      double* diag) {
+  double* ms = ms_all + (int64_t)blockIdx.x * D_N;
+  __shared__ double s[2 * LEV + 1];
+  member_reduce(B, part, 0, 2 * LEV, false, s, NPD);
+  member_reduce(B, part, 2 * LEV, 1, true, s + 2 * LEV, NPD);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double f = pair_factor(s[2 * LEV], cap);
+    double res = 0.0;
+    for (int k = 0; k < LEV; ++k)
+      res = fmax(res, fmax(fabs(s[2 * k] / ms[D_WC + k]), fabs(s[2 * k + 1] / ms[D_WC + k])));
+    const bool changed = ms[D_ANY] != 0.0;
+    ms[D_FAC] = f;
+    if (diag) {
+      diag[3 * blockIdx.x + 0] = changed ? f : 1.0;
+      diag[3 * blockIdx.x + 1] = changed ? res : 0.0;
+      diag[3 * blockIdx.x + 2] = changed ? 1.0 : 0.0;
+    }
+  }
+}
+
+template <class T> struct D75Out {
+  const T* before[4];   // t tb s sb
+  const T* after[4];
+  T* out[4];
+};
+
+template <class T> __global__ void __launch_bounds__(TPB) k_d75_apply(Batch B, D75Args<T> a, D75Out<T> f) {
+  Pos p;
+  if (!pos2(B, p)) return;
+  const int64_t c2 = ix2(p, p.i, p.j), c = ix3(p, p.i, p.j, 0);
+  const double* ms = a.ms + (int64_t)p.m * D_N;
+  if (ms[D_ANY] == 0.0) {   // no T/S change: the corrected state stands
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      T y[KB];
+#pragma unroll
+      for (int k = 0; k < KB; ++k) y[k] = f.after[q][c + k];
+#pragma unroll
+      for (int k = 0; k < KB; ++k) f.out[q][c + k] = y[k];
+    }
+    return;
+  }
+  const double fac = ms[D_FAC];
+  T dt[LEV], ds[LEV];
+  for (int k = 0; k < LEV; ++k) {
+    const D75Pt q = d75_point(a, p, c2, c + k, k);
+    double adj, p0, p1;
+    d75_proposed(a, ms, q, k, adj, p0, p1);
+    dt[k] = T(fac * p0);
+    ds[k] = T(fac * p1);
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const T* src = f.before[q] + c;
+    T* dst = f.out[q] + c;
+    const T* inc = q < 2 ? dt : ds;
+    T y[KB];
+#pragma unroll
+    for (int k = 0; k < KB; ++k) y[k] = src[k];
+#pragma unroll
+    for (int k = 0; k < KB; ++k) dst[k] = k < LEV ? y[k] + inc[k] : y[k];
+  }
+}
+
+template <class T>
+int run_d75(const pom_batch* b, const pom_params_t* prm, const void* vgrid, const void* const* geom,
+            const double* scales, double cap, void* work, const void* const* before, const void* const* after,
+            void* const* out, double* diag, hipStream_t st) {
+  D75Args<T> a{};
+  a.bt = static_cast<const T*>(before[0]);
+  a.bs = static_cast<const T*>(before[2]);
+  a.at = static_cast<const T*>(after[0]);
+  a.as = static_cast<const T*>(after[2]);
+  a.fsm = static_cast<const T*>(geom[0]);
+  a.art = static_cast<const T*>(geom[1]);
+  a.h = static_cast<const T*>(geom[2]);
+  a.et = static_cast<const T*>(geom[3]);
+  a.vg = static_cast<const T*>(vgrid);
+  a.st = scales[0];
+  a.ss = scales[1];
+  a.cap = cap;
+  a.prm.tbias = prm->tbias;
+  a.prm.sbias = prm->sbias;
+  a.prm.grav = prm->grav;
+  a.prm.rhoref = prm->rhoref;
+  a.part = static_cast<double*>(work);
+  a.ms = a.part + (int64_t)NPD * b->nblk2;
+  D75Out<T> f{};
+  for (int q = 0; q < 4; ++q) {
+    f.before[q] = static_cast<const T*>(before[q]);
+    f.after[q] = static_cast<const T*>(after[q]);
+    f.out[q] = static_cast<T*>(out[q]);
+  }
+  const Batch B{b->d_mem, b->d_blk3, b->d_blk2, b->M};
+  const dim3 g2(b->nblk2), gm(b->M), tb(TPB);
+  hipLaunchKernelGGL(k_d75_pass1<T>, g2, tb, 0, st, B, a);
+  hipLaunchKernelGGL(k_d75_member1<T>, gm, tb, 0, st, B, a);
+  hipLaunchKernelGGL(k_d75_pass2<T>, g2, tb, 0, st, B, a);
+  hipLaunchKernelGGL(k_d75_member2, gm, tb, 0, st, B, a.ms, a.part, cap, diag);
+  hipLaunchKernelGGL(k_d75_apply<T>, g2, tb, 0, st, B, a, f);
+  return hipGetLastError() == hipSuccess ? POM_OK : POM_E_HIP;
+}
+
 bool corr_args_ok(const pom_batch* b, const void* raw, const void* const* geom, const void* vgrid,
                   const double* scales, double cap, void* work) {
   if (!b || !raw || !geom || !vgrid || !scales || !work || !(cap > 0.0)) return false;
@@ -556,6 +846,27 @@ int pom_joint_correction_adj(const pom_batch* b, const void* raw, const void* co
   if (b->dtype == POM_F32)
     return run_corr_adj<float>(b, raw, geom, vgrid, scales, cap, work, cot_out, cot_diag, grad_raw, grad_et, st);
   return run_corr_adj<double>(b, raw, geom, vgrid, scales, cap, work, cot_out, cot_diag, grad_raw, grad_et, st);
+}
+
+int pom_density75_workspace(const pom_batch* b, size_t* bytes) {
+  if (!b || !bytes) return POM_E_ARG;
+  *bytes = sizeof(double) * ((size_t)NPD * b->nblk2 + (size_t)D_N * b->M);
+  return POM_OK;
+}
+
+int pom_density75(const pom_batch* b, const pom_params_t* prm, const void* vgrid, const void* const* geom,
+                  const double* scales, double cap, void* work, const void* const* before,
+                  const void* const* after, void* const* state_out, void* diag, void* stream) {
+  if (!b || !prm || !vgrid || !geom || !scales || !work || !before || !after || !state_out || !(cap > 0.0))
+    return POM_E_ARG;
+  if (!(scales[0] != 0.0) || !(scales[1] != 0.0)) return POM_E_ARG;
+  for (int q = 0; q < 4; ++q)
+    if (!geom[q] || !before[q] || !after[q] || !state_out[q]) return POM_E_ARG;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  double* dg = static_cast<double*>(diag);
+  if (b->dtype == POM_F32)
+    return run_d75<float>(b, prm, vgrid, geom, scales, cap, work, before, after, state_out, dg, st);
+  return run_d75<double>(b, prm, vgrid, geom, scales, cap, work, before, after, state_out, dg, st);
 }
 
 }  // extern "C"
