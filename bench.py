@@ -82,8 +82,8 @@ def run_gpu(args, dtype=torch.float32):
         it[0] += 1
         t = time0 + sc.dti * it[0] / 86400.0
         r = ramps(periods, t, dtype, dev)
-        out, vamax = ens.step(st, geom, sc, r, vg, True)
-        grads = ens.step_adj(st, geom, sc, r, vg, True, dict(cot))
+        out, vamax = ens.step_taped(st, geom, sc, r, vg, True)
+        grads = ens.step_reverse(st, geom, sc, r, vg, True, dict(cot))
         return out, grads, vamax
 
     from neuralpom_amd import _lib as L
@@ -159,14 +159,14 @@ def graph_vs_eager(args, shapes=((90, 82),), nstep=20):
     r = ramps(periods, 1.0, dtype, dev)
     st = state
     for _ in range(3):
-        out, _ = ens.step(st, geom, sc, r, vg, True)
-        ens.step_adj(st, geom, sc, r, vg, True, dict(cot))
+        out, _ = ens.step_taped(st, geom, sc, r, vg, True)
+        ens.step_reverse(st, geom, sc, r, vg, True, dict(cot))
         st = out
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(nstep):
-        out, _ = ens.step(st, geom, sc, r, vg, True)
-        ens.step_adj(st, geom, sc, r, vg, True, dict(cot))
+        out, _ = ens.step_taped(st, geom, sc, r, vg, True)
+        ens.step_reverse(st, geom, sc, r, vg, True, dict(cot))
         st = out
     torch.cuda.synchronize()
     eager = (time.perf_counter() - t0) / nstep
@@ -403,8 +403,8 @@ def run_gpu_implicit(args):
     def one(st):
         it[0] += 1
         r = ramps(periods, 1.0 + sc.dti * it[0] / 86400.0, dtype, dev)
-        out, vamax = ens.step(st, geom, sc, r, vg, True, iters)
-        ens.step_adj(st, geom, sc, r, vg, True, dict(cot))
+        out, vamax = ens.step_taped(st, geom, sc, r, vg, True, iters)
+        ens.step_reverse(st, geom, sc, r, vg, True, dict(cot))
         return out, vamax
 
     for _ in range(args.warmup):

@@ -163,6 +163,27 @@ int pom_explicit_step_adj(const pom_batch* b, const pom_params_t* prm, const voi
                           const void* const* cot_out, void* const* grad_in, void* workspace, size_t ws_bytes,
                           void* stream);
 
+/* The same vector-Jacobian product in two calls, as autograd's forward and
+ * backward split it (the reference differentiates its step with torch
+ * autograd, which keeps the forward's intermediates for the backward,
+ * src/neuralpom/cores/explicit/time_loop.py:20-709 under torch.enable_grad):
+ *   pom_explicit_step_taped: the step of pom_explicit_step (same outputs and
+ *     vamax_out) that also leaves its intermediates in the workspace
+ *     (pom_explicit_workspace with adjoint = 1);
+ *   pom_explicit_step_reverse: the products of pom_explicit_step_adj for the
+ *     last taped step in that workspace, without recomputing the forward.
+ * Between the two calls the workspace, state_in and geom must be left as
+ * they were and prm, ramp, vgrid and internal must be the same; state_out is
+ * the caller's and may be changed.  Results equal pom_explicit_step and
+ * pom_explicit_step_adj bitwise. */
+int pom_explicit_step_taped(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vgrid,
+                            const void* const* state_in, void* const* state_out, const void* const* geom,
+                            int32_t internal, void* vamax_out, void* workspace, size_t ws_bytes, void* stream);
+int pom_explicit_step_reverse(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vgrid,
+                              const void* const* state_in, const void* const* geom, int32_t internal,
+                              const void* const* cot_out, void* const* grad_in, void* workspace, size_t ws_bytes,
+                              void* stream);
+
 /* ---- Semi-implicit (P0-fixed) core: src/neuralpom/cores/implicit_p0_fixed/
  * time_loop.py:38-688 with the closed-basin double-gyre policy
  * (boundary/closed_basin.py, forcing/surface.py, numerics/double_gyre.py).
@@ -195,6 +216,17 @@ int pom_implicit_step_adj(const pom_batch* b, const pom_params_t* prm, const voi
                           const void* vgrid, const void* const* state_in, const void* const* geom, int32_t internal,
                           const void* const* cot_out, void* const* grad_in, void* workspace, size_t ws_bytes,
                           void* stream);
+/* pom_implicit_step_taped / pom_implicit_step_reverse: the split of
+ * pom_implicit_step_adj, with the conventions of pom_explicit_step_taped /
+ * pom_explicit_step_reverse (cache unchanged between the calls as well). */
+int pom_implicit_step_taped(const pom_batch* b, const pom_params_t* prm, const void* cache, const void* ramp,
+                            const void* vgrid, const void* const* state_in, void* const* state_out,
+                            const void* const* geom, int32_t internal, void* vamax_out, int32_t* eta_iters_out,
+                            void* workspace, size_t ws_bytes, void* stream);
+int pom_implicit_step_reverse(const pom_batch* b, const pom_params_t* prm, const void* cache, const void* ramp,
+                              const void* vgrid, const void* const* state_in, const void* const* geom,
+                              int32_t internal, const void* const* cot_out, void* const* grad_in, void* workspace,
+                              size_t ws_bytes, void* stream);
 
 /* The semi-implicit core's elliptic operator and eta solve on their own, for
  * callers that invoke or override them outside the step (case adapters,

@@ -82,6 +82,8 @@ def load():
                                       C.c_size_t, vp]
     lib.pom_explicit_step_adj.argtypes = [vp, C.POINTER(Params), vp, vp, vpp, vpp, C.c_int32, vpp, vpp,
                                           vp, C.c_size_t, vp]
+    lib.pom_explicit_step_taped.argtypes = lib.pom_explicit_step.argtypes
+    lib.pom_explicit_step_reverse.argtypes = lib.pom_explicit_step_adj.argtypes
     lib.pom_dens.argtypes = [vp, C.POINTER(Params), vp, vp, vp, vp, vp, vp, vp, vp]
     lib.pom_dens_adj.argtypes = [vp, C.POINTER(Params), vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     lib.pom_baropg.argtypes = [vp, C.POINTER(Params), vp, vp, vpp, vpp, vp]
@@ -93,6 +95,8 @@ def load():
                                       C.c_size_t, vp]
     lib.pom_implicit_step_adj.argtypes = [vp, C.POINTER(Params), vp, vp, vp, vpp, vpp, C.c_int32, vpp, vpp, vp,
                                           C.c_size_t, vp]
+    lib.pom_implicit_step_taped.argtypes = lib.pom_implicit_step.argtypes
+    lib.pom_implicit_step_reverse.argtypes = lib.pom_implicit_step_adj.argtypes
     lib.pom_elliptic_apply.argtypes = [vp, C.POINTER(Params), vpp, vp, vp, vp, vp]
     lib.pom_elliptic_apply_adj.argtypes = [vp, C.POINTER(Params), vpp, vp, vp, vp, vp, vp, vp]
     lib.pom_eta_workspace.argtypes = [vp, C.POINTER(C.c_size_t)]
@@ -126,8 +130,9 @@ def load():
 
 
 EXPORTED = ("pom_batch_create", "pom_batch_destroy", "pom_explicit_workspace", "pom_explicit_step",
-            "pom_explicit_step_adj", "pom_implicit_cache_bytes", "pom_implicit_prepare", "pom_implicit_workspace",
-            "pom_implicit_step", "pom_implicit_step_adj", "pom_dens", "pom_dens_adj", "pom_baropg",
+            "pom_explicit_step_adj", "pom_explicit_step_taped", "pom_explicit_step_reverse",
+            "pom_implicit_cache_bytes", "pom_implicit_prepare", "pom_implicit_workspace",
+            "pom_implicit_step", "pom_implicit_step_adj", "pom_implicit_step_taped", "pom_implicit_step_reverse", "pom_dens", "pom_dens_adj", "pom_baropg",
             "pom_baropg_adj", "pom_elliptic_apply", "pom_elliptic_apply_adj", "pom_eta_workspace", "pom_eta_solve",
             "pom_eta_solve_adj", "pom_eta_solve_noalloc", "pom_pack_input", "pom_pack_input_adj", "pom_restrict4", "pom_restrict4_adj", "pom_copy_arrays",
             "pom_bcond", "pom_bcond_adj", "pom_profq_workspace", "pom_profq", "pom_profq_adj", "pom_proft",

@@ -409,6 +409,14 @@ template <class T> struct Step {
     }
   }
   X<T> x0_;
+  // The host-side pointers the forward settles on its way (x0_ and the
+  // carried advave fields of every substep), for a reverse that runs without
+  // its forward in the same call; they depend on the plan only.
+  void replay_host() {
+    x0_ = x0();
+    if (!imp)
+      for (int n = 1; n <= isplit; ++n) resolve(n, xs[n]);
+  }
 
   // ------------------------------------------------------------ field tables
   FieldTab<T> t_advct() const {
@@ -1785,6 +1793,82 @@ int run_adj(const pom_batch* b, const pom_params_t* prm, const void* ramp, const
   return hipGetLastError() == hipSuccess ? POM_OK : POM_E_HIP;
 }
 
+// The forward of run_adj with its tape kept in the workspace and the step's
+// outputs copied to the caller's buffers; run_reverse then runs the reverse
+// of that forward from the same workspace without recomputing it.  The two
+// plan the workspace identically (the plan depends on the batch and the
+// parameters only).
+template <class T>
+int run_taped(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vg, const void* const* in,
+              void* const* out, const void* const* geom, int internal, void* vamax, void* ws, size_t wsb,
+              hipStream_t st, const void* cache = nullptr, int32_t* iters = nullptr) {
+  Step<T> S{};
+  S.b = b;
+  S.c = make_ctx<T>(b, prm, ramp, vg, st);
+  setup_implicit(S, prm, cache);
+  S.iters_out = iters;
+  S.isplit = prm->isplit;
+  S.internal = internal != 0;
+  for (int f = 0; f < POM_NSTATE; ++f) {
+    S.in[f] = static_cast<const T*>(in[f]);
+    if (!S.in[f] || !out[f]) return POM_E_ARG;
+  }
+  for (int f = 0; f < POM_NGEOM; ++f) {
+    S.g[f] = static_cast<const T*>(geom[f]);
+    if (!S.g[f] && (f < POM_NGEOM_BASE || S.open)) return POM_E_ARG;
+  }
+  Arena A{static_cast<char*>(ws), 0, wsb, false};
+  S.tape = true;
+  S.plan(A, true);
+  S.vamax = static_cast<T*>(vamax);
+  Adj<T> J(S, S.c);
+  J.plan(A, nullptr);   // the reverse's share of the workspace must fit as well
+  if (A.off > wsb) return POM_E_ARG;
+  S.forward();
+  if (S.pw.status != POM_OK) return S.pw.status;
+  void* dst[POM_NSTATE];
+  const void* src[POM_NSTATE];
+  int64_t bytes[POM_NSTATE];
+  for (int f = 0; f < POM_NSTATE; ++f) {
+    dst[f] = out[f];
+    src[f] = S.out[f];
+    bytes[f] = (int64_t)sizeof(T) * (f < POM_N3D ? S.n3() : S.n2());
+  }
+  if (int e = pom_copy_arrays(POM_NSTATE, dst, src, bytes, st)) return e;
+  return hipGetLastError() == hipSuccess ? POM_OK : POM_E_HIP;
+}
+
+template <class T>
+int run_reverse(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vg, const void* const* in,
+                const void* const* geom, int internal, const void* const* cot, void* const* grad, void* ws,
+                size_t wsb, hipStream_t st, const void* cache = nullptr) {
+  Step<T> S{};
+  S.b = b;
+  S.c = make_ctx<T>(b, prm, ramp, vg, st);
+  setup_implicit(S, prm, cache);
+  S.isplit = prm->isplit;
+  S.internal = internal != 0;
+  for (int f = 0; f < POM_NSTATE; ++f) {
+    S.in[f] = static_cast<const T*>(in[f]);
+    if (!S.in[f]) return POM_E_ARG;
+  }
+  for (int f = 0; f < POM_NGEOM; ++f) {
+    S.g[f] = static_cast<const T*>(geom[f]);
+    if (!S.g[f] && (f < POM_NGEOM_BASE || S.open)) return POM_E_ARG;
+  }
+  Arena A{static_cast<char*>(ws), 0, wsb, false};
+  S.tape = true;
+  S.plan(A, true);
+  S.vamax = nullptr;
+  Adj<T> J(S, S.c);
+  J.plan(A, reinterpret_cast<T* const*>(grad));
+  if (A.off > wsb) return POM_E_ARG;
+  S.replay_host();
+  J.run(reinterpret_cast<const T* const*>(cot), reinterpret_cast<T* const*>(grad));
+  if (S.pw.status != POM_OK) return S.pw.status;
+  return hipGetLastError() == hipSuccess ? POM_OK : POM_E_HIP;
+}
+
 // ---------------------------------------------------------------- operator-level entry points
 // The reference's operator methods that callers invoke on their own (dens from
 // the corrector hook, baropg for the pressure-velocity error); each runs the
@@ -2365,6 +2449,31 @@ int pom_explicit_step_adj(const pom_batch* b, const pom_params_t* prm, const voi
   return run_adj<double>(b, prm, ramp, vgrid, state_in, geom, internal, cot_out, grad_in, workspace, ws_bytes, st);
 }
 
+int pom_explicit_step_taped(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vgrid,
+                            const void* const* state_in, void* const* state_out, const void* const* geom,
+                            int32_t internal, void* vamax_out, void* workspace, size_t ws_bytes, void* stream) {
+  if (!b || !ramp || !vgrid || !state_in || !state_out || !geom || !vamax_out || !workspace) return POM_E_ARG;
+  if (int e = check_params(prm, POM_CORE_EXPLICIT)) return e;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (b->dtype == POM_F32)
+    return run_taped<float>(b, prm, ramp, vgrid, state_in, state_out, geom, internal, vamax_out, workspace, ws_bytes,
+                            st);
+  return run_taped<double>(b, prm, ramp, vgrid, state_in, state_out, geom, internal, vamax_out, workspace, ws_bytes,
+                           st);
+}
+
+int pom_explicit_step_reverse(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vgrid,
+                              const void* const* state_in, const void* const* geom, int32_t internal,
+                              const void* const* cot_out, void* const* grad_in, void* workspace, size_t ws_bytes,
+                              void* stream) {
+  if (!b || !ramp || !vgrid || !state_in || !geom || !workspace) return POM_E_ARG;
+  if (int e = check_params(prm, POM_CORE_EXPLICIT)) return e;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (b->dtype == POM_F32)
+    return run_reverse<float>(b, prm, ramp, vgrid, state_in, geom, internal, cot_out, grad_in, workspace, ws_bytes, st);
+  return run_reverse<double>(b, prm, ramp, vgrid, state_in, geom, internal, cot_out, grad_in, workspace, ws_bytes, st);
+}
+
 int pom_dens(const pom_batch* b, const pom_params_t* prm, const void* vgrid, const void* t, const void* s,
              const void* rho_in, const void* h, const void* fsm, void* rho_out, void* stream) {
   if (!b || !prm || !vgrid || !t || !s || !rho_in || !h || !fsm || !rho_out) return POM_E_ARG;
@@ -2612,6 +2721,35 @@ int pom_implicit_step_adj(const pom_batch* b, const pom_params_t* prm, const voi
                           cache);
   return run_adj<double>(b, prm, ramp, vgrid, state_in, geom, internal, cot_out, grad_in, workspace, ws_bytes, st,
                          cache);
+}
+
+int pom_implicit_step_taped(const pom_batch* b, const pom_params_t* prm, const void* cache, const void* ramp,
+                            const void* vgrid, const void* const* state_in, void* const* state_out,
+                            const void* const* geom, int32_t internal, void* vamax_out, int32_t* eta_iters_out,
+                            void* workspace, size_t ws_bytes, void* stream) {
+  if (!b || !cache || !ramp || !vgrid || !state_in || !state_out || !geom || !vamax_out || !workspace)
+    return POM_E_ARG;
+  if (int e = check_params(prm, POM_CORE_IMPLICIT)) return e;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (b->dtype == POM_F32)
+    return run_taped<float>(b, prm, ramp, vgrid, state_in, state_out, geom, internal, vamax_out, workspace, ws_bytes,
+                            st, cache, eta_iters_out);
+  return run_taped<double>(b, prm, ramp, vgrid, state_in, state_out, geom, internal, vamax_out, workspace, ws_bytes,
+                           st, cache, eta_iters_out);
+}
+
+int pom_implicit_step_reverse(const pom_batch* b, const pom_params_t* prm, const void* cache, const void* ramp,
+                              const void* vgrid, const void* const* state_in, const void* const* geom,
+                              int32_t internal, const void* const* cot_out, void* const* grad_in, void* workspace,
+                              size_t ws_bytes, void* stream) {
+  if (!b || !cache || !ramp || !vgrid || !state_in || !geom || !workspace) return POM_E_ARG;
+  if (int e = check_params(prm, POM_CORE_IMPLICIT)) return e;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (b->dtype == POM_F32)
+    return run_reverse<float>(b, prm, ramp, vgrid, state_in, geom, internal, cot_out, grad_in, workspace, ws_bytes,
+                              st, cache);
+  return run_reverse<double>(b, prm, ramp, vgrid, state_in, geom, internal, cot_out, grad_in, workspace, ws_bytes,
+                             st, cache);
 }
 
 }  // extern "C"

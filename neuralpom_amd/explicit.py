@@ -172,12 +172,10 @@ class Ensemble:
         L.check(rc, "pom_explicit_step")
         return out, vamax
 
-    def step_adj(self, state, geom, sc: StepScalars, ramp, vgrid, internal, cot, ws_pool=None):
-        """Gradients of the step inputs for output cotangents ``cot`` (dict, may omit fields)."""
-        prm = sc.struct()
-        ins = [state[n] for n in L.FIELDS]
-        self._check(ins, L.FIELDS, self.numel)
-        # one zero-filled allocation, split into per-field views
+    def _adj_buffers(self, cot):
+        """Zero-filled gradient views (one allocation) and the cotangent / gradient
+        pointer tables of an adjoint call; ``cot`` entries are made contiguous
+        in the ensemble's dtype in place."""
         sizes = [self.numel(n) for n in L.DIFF_INPUTS]
         flat = torch.zeros(sum(sizes), dtype=self.dtype, device=self.device)
         grads = dict(zip(L.DIFF_INPUTS, torch.split(flat, sizes)))
@@ -191,13 +189,49 @@ class Ensemble:
             else:
                 cot_ptrs.append(None)
         grad_ptrs = [grads[n].data_ptr() if n in grads else None for n in L.FIELDS]
+        return grads, L.ptr_array(cot_ptrs), L.ptr_array(grad_ptrs)
+
+    def step_adj(self, state, geom, sc: StepScalars, ramp, vgrid, internal, cot, ws_pool=None):
+        """Gradients of the step inputs for output cotangents ``cot`` (dict, may omit fields)."""
+        return self._reverse_call("pom_explicit_step_adj", state, geom, sc, ramp, vgrid, internal, cot, ws_pool)
+
+    def step_taped(self, state, geom, sc: StepScalars, ramp, vgrid, internal=True, ws_pool=None):
+        """``step`` that keeps its intermediates in the adjoint workspace of
+        ``ws_pool`` (default: the ensemble's) for ``step_reverse``."""
+        prm = sc.struct()
+        ins = [state[n] for n in L.FIELDS]
+        gs = [geom[n] for n in L.GEOM]
+        self._check(ins, L.FIELDS, self.numel)
+        self._check(gs, L.GEOM, self.numel)
+        out = {n: torch.empty_like(state[n]) for n in L.FIELDS}
+        vamax = torch.zeros(1, dtype=self.dtype, device=self.device)
         ws, nb = self._workspace(prm, True, ws_pool)
         stream = torch.cuda.current_stream(self.device).cuda_stream
-        rc = self.lib.pom_explicit_step_adj(
+        rc = self.lib.pom_explicit_step_taped(
+            self._h, C.byref(prm), ramp.data_ptr(), vgrid.data_ptr(),
+            L.ptr_array([t.data_ptr() for t in ins]), L.ptr_array([out[n].data_ptr() for n in L.FIELDS]),
+            L.geom_ptrs(geom), int(bool(internal)), vamax.data_ptr(), ws.data_ptr(), nb, stream)
+        L.check(rc, "pom_explicit_step_taped")
+        return out, vamax
+
+    def step_reverse(self, state, geom, sc: StepScalars, ramp, vgrid, internal, cot, ws_pool=None):
+        """``step_adj`` of the last ``step_taped`` of the same arguments, from its
+        kept intermediates (no forward recompute).  ``state`` and ``geom`` must
+        be the taped call's, unchanged, and the workspace untouched since."""
+        return self._reverse_call("pom_explicit_step_reverse", state, geom, sc, ramp, vgrid, internal, cot, ws_pool)
+
+    def _reverse_call(self, fn, state, geom, sc, ramp, vgrid, internal, cot, ws_pool):
+        prm = sc.struct()
+        ins = [state[n] for n in L.FIELDS]
+        self._check(ins, L.FIELDS, self.numel)
+        grads, cot_ptrs, grad_ptrs = self._adj_buffers(cot)
+        ws, nb = self._workspace(prm, True, ws_pool)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        rc = getattr(self.lib, fn)(
             self._h, C.byref(prm), ramp.data_ptr(), vgrid.data_ptr(),
             L.ptr_array([t.data_ptr() for t in ins]), L.geom_ptrs(geom),
-            int(bool(internal)), L.ptr_array(cot_ptrs), L.ptr_array(grad_ptrs), ws.data_ptr(), nb, stream)
-        L.check(rc, "pom_explicit_step_adj")
+            int(bool(internal)), cot_ptrs, grad_ptrs, ws.data_ptr(), nb, stream)
+        L.check(rc, fn)
         return grads
 
 
@@ -235,7 +269,9 @@ class StepGraph:
     ``ramp`` and, with ``cot``, the output cotangents; a replay advances
     ``state`` in place by one step (the step's outputs are copied back into the
     input buffers inside the graph) and, with ``cot``, leaves the input
-    cotangents of the step it just took in ``grads``.  Refill ``ramp`` (and
+    cotangents of the step it just took in ``grads``: the forward keeps its
+    intermediates (``step_taped``) and the reverse runs from them
+    (``step_reverse``), so the step's forward runs once.  Refill ``ramp`` (and
     ``cot``) between replays to change them.  ``on_warm`` is called between
     the warm-up and the recording.  The step launches ~100 kernels
     per barotropic substep cycle and has no host synchronisation, so a replay
@@ -258,9 +294,7 @@ class StepGraph:
         self.ens = ens
         self._ws = {}
         prm = sc.struct()
-        ens._workspace(prm, False, self._ws)
-        if cot is not None:
-            ens._workspace(prm, True, self._ws)
+        ens._workspace(prm, cot is not None, self._ws)
         self.state = {n: state[n].clone() for n in L.FIELDS}
         self.geom, self.sc, self.vgrid, self.internal = geom, sc, vgrid, internal
         self.ramp = ramp.clone()
@@ -280,12 +314,15 @@ class StepGraph:
             self.grads, self.vamax = self._body(self.state)
 
     def _body(self, st):
-        out, vamax = self.ens.step(st, self.geom, self.sc, self.ramp, self.vgrid, self.internal,
-                                   ws_pool=self._ws)
         grads = None
-        if self.cot is not None:
-            grads = self.ens.step_adj(st, self.geom, self.sc, self.ramp, self.vgrid, self.internal,
-                                      dict(self.cot), ws_pool=self._ws)
+        if self.cot is None:
+            out, vamax = self.ens.step(st, self.geom, self.sc, self.ramp, self.vgrid, self.internal,
+                                       ws_pool=self._ws)
+        else:   # the forward keeps its intermediates and the reverse runs from them
+            out, vamax = self.ens.step_taped(st, self.geom, self.sc, self.ramp, self.vgrid, self.internal,
+                                             ws_pool=self._ws)
+            grads = self.ens.step_reverse(st, self.geom, self.sc, self.ramp, self.vgrid, self.internal,
+                                          dict(self.cot), ws_pool=self._ws)
         for n in L.FIELDS:
             st[n].copy_(out[n])
         return grads, vamax

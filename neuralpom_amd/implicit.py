@@ -118,34 +118,48 @@ class ImplicitEnsemble(Ensemble):
 
     def step_adj(self, state, geom, sc: ImplicitScalars, ramp, vgrid, internal, cot, ws_pool=None):
         """Gradients of the step inputs for output cotangents ``cot`` (dict, may omit fields)."""
+        return self._reverse_call("pom_implicit_step_adj", state, geom, sc, ramp, vgrid, internal, cot, ws_pool)
+
+    def step_taped(self, state, geom, sc: ImplicitScalars, ramp, vgrid, internal=True, iters=None, ws_pool=None):
+        """``step`` that keeps its intermediates in the adjoint workspace for ``step_reverse``."""
         if not self._prepared:
             raise RuntimeError("ImplicitEnsemble.prepare(geom, scalars) must run before the first step")
         prm = sc.struct()
         ins = [state[n] for n in L.FIELDS]
         self._check(ins, L.FIELDS, self.numel)
         gp = self._geom_ptrs(geom)
-        sizes = [self.numel(n) for n in L.DIFF_INPUTS]
-        flat = torch.zeros(sum(sizes), dtype=self.dtype, device=self.device)
-        grads = dict(zip(L.DIFF_INPUTS, torch.split(flat, sizes)))
-        cot_ptrs = []
-        for n in L.FIELDS:
-            c = cot.get(n)
-            if c is not None:
-                c = c.contiguous().to(self.dtype)
-                cot[n] = c
-                cot_ptrs.append(c.data_ptr())
-            else:
-                cot_ptrs.append(None)
-        grad_ptrs = [grads[n].data_ptr() if n in grads else None for n in L.FIELDS]
+        out = {n: torch.empty_like(state[n]) for n in L.FIELDS}
+        vamax = torch.zeros(1, dtype=self.dtype, device=self.device)
         ws, nb = self._workspace(prm, True, ws_pool)
         stream = torch.cuda.current_stream(self.device).cuda_stream
-        rc = self.lib.pom_implicit_step_adj(
+        rc = self.lib.pom_implicit_step_taped(
+            self._h, C.byref(prm), self.cache.data_ptr(), ramp.data_ptr(), vgrid.data_ptr(),
+            L.ptr_array([t.data_ptr() for t in ins]), L.ptr_array([out[n].data_ptr() for n in L.FIELDS]),
+            gp, int(bool(internal)), vamax.data_ptr(),
+            iters.data_ptr() if iters is not None else None, ws.data_ptr(), nb, stream)
+        L.check(rc, "pom_implicit_step_taped")
+        return out, vamax
+
+    def step_reverse(self, state, geom, sc: ImplicitScalars, ramp, vgrid, internal, cot, ws_pool=None):
+        """``step_adj`` of the last ``step_taped`` of the same arguments, from its kept intermediates."""
+        return self._reverse_call("pom_implicit_step_reverse", state, geom, sc, ramp, vgrid, internal, cot, ws_pool)
+
+    def _reverse_call(self, fn, state, geom, sc, ramp, vgrid, internal, cot, ws_pool):
+        if not self._prepared:
+            raise RuntimeError("ImplicitEnsemble.prepare(geom, scalars) must run before the first step")
+        prm = sc.struct()
+        ins = [state[n] for n in L.FIELDS]
+        self._check(ins, L.FIELDS, self.numel)
+        gp = self._geom_ptrs(geom)
+        grads, cot_ptrs, grad_ptrs = self._adj_buffers(cot)
+        ws, nb = self._workspace(prm, True, ws_pool)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        rc = getattr(self.lib, fn)(
             self._h, C.byref(prm), self.cache.data_ptr(), ramp.data_ptr(), vgrid.data_ptr(),
             L.ptr_array([t.data_ptr() for t in ins]), gp,
-            int(bool(internal)), L.ptr_array(cot_ptrs), L.ptr_array(grad_ptrs), ws.data_ptr(), nb, stream)
-        L.check(rc, "pom_implicit_step_adj")
+            int(bool(internal)), cot_ptrs, grad_ptrs, ws.data_ptr(), nb, stream)
+        L.check(rc, fn)
         return grads
-
 
     # ---------------------------------------------------------------- eta solve on its own
     def _ws_eta(self):
