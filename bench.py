@@ -83,7 +83,7 @@ def run_gpu(args, dtype=torch.float32):
         t = time0 + sc.dti * it[0] / 86400.0
         r = ramps(periods, t, dtype, dev)
         out, vamax = ens.step_taped(st, geom, sc, r, vg, True)
-        grads = ens.step_reverse(st, geom, sc, r, vg, True, dict(cot))
+        grads = ens.step_reverse(st, out, geom, sc, r, vg, True, dict(cot))
         return out, grads, vamax
 
     from neuralpom_amd import _lib as L
@@ -97,15 +97,16 @@ def run_gpu(args, dtype=torch.float32):
         torch.cuda.synchronize()
         return dict(cells=cells, seconds=time.perf_counter() - t0)
     if not args.eager:
-        # the fwd+adjoint step recorded once into a HIP graph, replayed per step.  The
-        # dominant kernel's HIP events are recorded into the graph too (tracing is reset
-        # between the graph's warm-up and its recording), so the read after the timed
-        # replays returns that kernel's launches of the last replay.
+        # the fwd+adjoint step recorded into two HIP graphs (state buffers A -> B and
+        # B -> A), replayed alternately.  The dominant kernel's HIP events are recorded
+        # into the first graph (tracing is reset between the warm-up and the recording
+        # and stopped after the first graph), so the read after the timed replays
+        # returns that kernel's launches of the first graph's last replay.
         from neuralpom_amd.explicit import StepGraph
         it[0] += 1
         L.profile_stage(DOMINANT)
         G = StepGraph(ens, state, geom, sc, ramps(periods, time0 + sc.dti * it[0] / 86400.0, dtype, dev), vg,
-                      True, cot, on_warm=L.profile_read)
+                      True, cot, on_warm=L.profile_read, on_first=lambda: L.profile_stage(None))
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
@@ -160,13 +161,13 @@ def graph_vs_eager(args, shapes=((90, 82),), nstep=20):
     st = state
     for _ in range(3):
         out, _ = ens.step_taped(st, geom, sc, r, vg, True)
-        ens.step_reverse(st, geom, sc, r, vg, True, dict(cot))
+        ens.step_reverse(st, out, geom, sc, r, vg, True, dict(cot))
         st = out
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(nstep):
         out, _ = ens.step_taped(st, geom, sc, r, vg, True)
-        ens.step_reverse(st, geom, sc, r, vg, True, dict(cot))
+        ens.step_reverse(st, out, geom, sc, r, vg, True, dict(cot))
         st = out
     torch.cuda.synchronize()
     eager = (time.perf_counter() - t0) / nstep
@@ -404,7 +405,7 @@ def run_gpu_implicit(args):
         it[0] += 1
         r = ramps(periods, 1.0 + sc.dti * it[0] / 86400.0, dtype, dev)
         out, vamax = ens.step_taped(st, geom, sc, r, vg, True, iters)
-        ens.step_reverse(st, geom, sc, r, vg, True, dict(cot))
+        ens.step_reverse(st, out, geom, sc, r, vg, True, dict(cot))
         return out, vamax
 
     for _ in range(args.warmup):

@@ -171,18 +171,18 @@ int pom_explicit_step_adj(const pom_batch* b, const pom_params_t* prm, const voi
  *     vamax_out) that also leaves its intermediates in the workspace
  *     (pom_explicit_workspace with adjoint = 1);
  *   pom_explicit_step_reverse: the products of pom_explicit_step_adj for the
- *     last taped step in that workspace, without recomputing the forward.
- * Between the two calls the workspace, state_in and geom must be left as
- * they were and prm, ramp, vgrid and internal must be the same; state_out is
- * the caller's and may be changed.  Results equal pom_explicit_step and
- * pom_explicit_step_adj bitwise. */
+ *     last taped step in that workspace, without recomputing the forward;
+ *     state_out is the taped call's (the reverse reads, never writes, it).
+ * Between the two calls the workspace, state_in, state_out and geom must be
+ * left as they were and prm, ramp, vgrid and internal must be the same.
+ * Results equal pom_explicit_step and pom_explicit_step_adj bitwise. */
 int pom_explicit_step_taped(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vgrid,
                             const void* const* state_in, void* const* state_out, const void* const* geom,
                             int32_t internal, void* vamax_out, void* workspace, size_t ws_bytes, void* stream);
 int pom_explicit_step_reverse(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vgrid,
-                              const void* const* state_in, const void* const* geom, int32_t internal,
-                              const void* const* cot_out, void* const* grad_in, void* workspace, size_t ws_bytes,
-                              void* stream);
+                              const void* const* state_in, const void* const* state_out, const void* const* geom,
+                              int32_t internal, const void* const* cot_out, void* const* grad_in, void* workspace,
+                              size_t ws_bytes, void* stream);
 
 /* ---- Semi-implicit (P0-fixed) core: src/neuralpom/cores/implicit_p0_fixed/
  * time_loop.py:38-688 with the closed-basin double-gyre policy
@@ -224,9 +224,9 @@ int pom_implicit_step_taped(const pom_batch* b, const pom_params_t* prm, const v
                             const void* const* geom, int32_t internal, void* vamax_out, int32_t* eta_iters_out,
                             void* workspace, size_t ws_bytes, void* stream);
 int pom_implicit_step_reverse(const pom_batch* b, const pom_params_t* prm, const void* cache, const void* ramp,
-                              const void* vgrid, const void* const* state_in, const void* const* geom,
-                              int32_t internal, const void* const* cot_out, void* const* grad_in, void* workspace,
-                              size_t ws_bytes, void* stream);
+                              const void* vgrid, const void* const* state_in, const void* const* state_out,
+                              const void* const* geom, int32_t internal, const void* const* cot_out,
+                              void* const* grad_in, void* workspace, size_t ws_bytes, void* stream);
 
 /* The semi-implicit core's elliptic operator and eta solve on their own, for
  * callers that invoke or override them outside the step (case adapters,
@@ -465,9 +465,9 @@ int pom_density75(const pom_batch* b, const pom_params_t* prm, const void* vgrid
                   const void* const* after, void* const* state_out, void* diag, void* stream);
 
 /* Tracing: record HIP events around every launch of one stage on its stream
- * (stage = -1 disables).  pom_profile_read waits for the recorded events and
- * returns the summed kernel time and the number of launches since the last
- * read.  No counterpart in the reference (it times host wall clock only,
+ * (a stage >= 0 starts a new record; -1 stops recording and keeps what was
+ * recorded).  pom_profile_read waits for the recorded events and returns the
+ * summed kernel time and the number of launches since the last read.  No counterpart in the reference (it times host wall clock only,
  * src/neuralpom/v1/engine.py:341-362). */
 typedef enum {
   POM_STAGE_PROFQ = 0,      /* k_profq: Mellor-Yamada closure, forward */

@@ -83,7 +83,8 @@ def load():
     lib.pom_explicit_step_adj.argtypes = [vp, C.POINTER(Params), vp, vp, vpp, vpp, C.c_int32, vpp, vpp,
                                           vp, C.c_size_t, vp]
     lib.pom_explicit_step_taped.argtypes = lib.pom_explicit_step.argtypes
-    lib.pom_explicit_step_reverse.argtypes = lib.pom_explicit_step_adj.argtypes
+    lib.pom_explicit_step_reverse.argtypes = [vp, C.POINTER(Params), vp, vp, vpp, vpp, vpp, C.c_int32, vpp, vpp,
+                                              vp, C.c_size_t, vp]
     lib.pom_dens.argtypes = [vp, C.POINTER(Params), vp, vp, vp, vp, vp, vp, vp, vp]
     lib.pom_dens_adj.argtypes = [vp, C.POINTER(Params), vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     lib.pom_baropg.argtypes = [vp, C.POINTER(Params), vp, vp, vpp, vpp, vp]
@@ -96,7 +97,8 @@ def load():
     lib.pom_implicit_step_adj.argtypes = [vp, C.POINTER(Params), vp, vp, vp, vpp, vpp, C.c_int32, vpp, vpp, vp,
                                           C.c_size_t, vp]
     lib.pom_implicit_step_taped.argtypes = lib.pom_implicit_step.argtypes
-    lib.pom_implicit_step_reverse.argtypes = lib.pom_implicit_step_adj.argtypes
+    lib.pom_implicit_step_reverse.argtypes = [vp, C.POINTER(Params), vp, vp, vp, vpp, vpp, vpp, C.c_int32, vpp,
+                                              vpp, vp, C.c_size_t, vp]
     lib.pom_elliptic_apply.argtypes = [vp, C.POINTER(Params), vpp, vp, vp, vp, vp]
     lib.pom_elliptic_apply_adj.argtypes = [vp, C.POINTER(Params), vpp, vp, vp, vp, vp, vp, vp]
     lib.pom_eta_workspace.argtypes = [vp, C.POINTER(C.c_size_t)]

@@ -37,6 +37,7 @@ struct StageProf {
   size_t used = 0;
 };
 StageProf g_prof;
+constexpr size_t PROF_POOL = 512;   // events created up front (256 launches)
 
 struct ProfScope {
   hipEvent_t b = nullptr, e = nullptr;
@@ -1793,11 +1794,12 @@ int run_adj(const pom_batch* b, const pom_params_t* prm, const void* ramp, const
   return hipGetLastError() == hipSuccess ? POM_OK : POM_E_HIP;
 }
 
-// The forward of run_adj with its tape kept in the workspace and the step's
-// outputs copied to the caller's buffers; run_reverse then runs the reverse
-// of that forward from the same workspace without recomputing it.  The two
-// plan the workspace identically (the plan depends on the batch and the
-// parameters only).
+// The forward of run_adj with its tape kept in the workspace and in the
+// caller's output buffers (the last barotropic substep and the step's new
+// fields live there, as in the plain step); run_reverse then runs the reverse
+// of that forward from the same workspace and outputs without recomputing it.
+// The two plan the workspace identically (the plan depends on the batch and
+// the parameters only), and the reverse only reads the outputs.
 template <class T>
 int run_taped(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vg, const void* const* in,
               void* const* out, const void* const* geom, int internal, void* vamax, void* ws, size_t wsb,
@@ -1811,7 +1813,8 @@ int run_taped(const pom_batch* b, const pom_params_t* prm, const void* ramp, con
   S.internal = internal != 0;
   for (int f = 0; f < POM_NSTATE; ++f) {
     S.in[f] = static_cast<const T*>(in[f]);
-    if (!S.in[f] || !out[f]) return POM_E_ARG;
+    S.out[f] = static_cast<T*>(out[f]);
+    if (!S.in[f] || !S.out[f]) return POM_E_ARG;
   }
   for (int f = 0; f < POM_NGEOM; ++f) {
     S.g[f] = static_cast<const T*>(geom[f]);
@@ -1819,29 +1822,20 @@ int run_taped(const pom_batch* b, const pom_params_t* prm, const void* ramp, con
   }
   Arena A{static_cast<char*>(ws), 0, wsb, false};
   S.tape = true;
-  S.plan(A, true);
+  S.plan(A, false);
   S.vamax = static_cast<T*>(vamax);
   Adj<T> J(S, S.c);
   J.plan(A, nullptr);   // the reverse's share of the workspace must fit as well
   if (A.off > wsb) return POM_E_ARG;
   S.forward();
   if (S.pw.status != POM_OK) return S.pw.status;
-  void* dst[POM_NSTATE];
-  const void* src[POM_NSTATE];
-  int64_t bytes[POM_NSTATE];
-  for (int f = 0; f < POM_NSTATE; ++f) {
-    dst[f] = out[f];
-    src[f] = S.out[f];
-    bytes[f] = (int64_t)sizeof(T) * (f < POM_N3D ? S.n3() : S.n2());
-  }
-  if (int e = pom_copy_arrays(POM_NSTATE, dst, src, bytes, st)) return e;
   return hipGetLastError() == hipSuccess ? POM_OK : POM_E_HIP;
 }
 
 template <class T>
 int run_reverse(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vg, const void* const* in,
-                const void* const* geom, int internal, const void* const* cot, void* const* grad, void* ws,
-                size_t wsb, hipStream_t st, const void* cache = nullptr) {
+                const void* const* out, const void* const* geom, int internal, const void* const* cot,
+                void* const* grad, void* ws, size_t wsb, hipStream_t st, const void* cache = nullptr) {
   Step<T> S{};
   S.b = b;
   S.c = make_ctx<T>(b, prm, ramp, vg, st);
@@ -1850,7 +1844,8 @@ int run_reverse(const pom_batch* b, const pom_params_t* prm, const void* ramp, c
   S.internal = internal != 0;
   for (int f = 0; f < POM_NSTATE; ++f) {
     S.in[f] = static_cast<const T*>(in[f]);
-    if (!S.in[f]) return POM_E_ARG;
+    S.out[f] = const_cast<T*>(static_cast<const T*>(out[f]));   // read only in the reverse
+    if (!S.in[f] || !S.out[f]) return POM_E_ARG;
   }
   for (int f = 0; f < POM_NGEOM; ++f) {
     S.g[f] = static_cast<const T*>(geom[f]);
@@ -1858,7 +1853,7 @@ int run_reverse(const pom_batch* b, const pom_params_t* prm, const void* ramp, c
   }
   Arena A{static_cast<char*>(ws), 0, wsb, false};
   S.tape = true;
-  S.plan(A, true);
+  S.plan(A, false);
   S.vamax = nullptr;
   Adj<T> J(S, S.c);
   J.plan(A, reinterpret_cast<T* const*>(grad));
@@ -2390,8 +2385,17 @@ int pom_batch_create(const pom_member_t* members, int32_t n, int32_t dtype, pom_
 
 int pom_profile_stage(int32_t stage) {
   if (stage < -1 || stage >= POM_NSTAGE) return POM_E_ARG;
+  // The events are created and recorded once here, outside any stream
+  // capture: an event whose only records were captured into a HIP graph is
+  // not a valid handle for hipEventElapsedTime, even after the graph ran.
+  while (stage >= 0 && g_prof.pool.size() < PROF_POOL) {
+    hipEvent_t x;
+    if (hipEventCreate(&x) != hipSuccess) return POM_E_HIP;
+    g_prof.pool.push_back(x);
+    if (hipEventRecord(x, nullptr) != hipSuccess || hipEventSynchronize(x) != hipSuccess) return POM_E_HIP;
+  }
   g_prof.stage = stage;
-  g_prof.used = 0;
+  if (stage >= 0) g_prof.used = 0;   // stopping (-1) keeps the recorded events readable
   return POM_OK;
 }
 
@@ -2463,15 +2467,17 @@ int pom_explicit_step_taped(const pom_batch* b, const pom_params_t* prm, const v
 }
 
 int pom_explicit_step_reverse(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vgrid,
-                              const void* const* state_in, const void* const* geom, int32_t internal,
-                              const void* const* cot_out, void* const* grad_in, void* workspace, size_t ws_bytes,
-                              void* stream) {
-  if (!b || !ramp || !vgrid || !state_in || !geom || !workspace) return POM_E_ARG;
+                              const void* const* state_in, const void* const* state_out, const void* const* geom,
+                              int32_t internal, const void* const* cot_out, void* const* grad_in, void* workspace,
+                              size_t ws_bytes, void* stream) {
+  if (!b || !ramp || !vgrid || !state_in || !state_out || !geom || !workspace) return POM_E_ARG;
   if (int e = check_params(prm, POM_CORE_EXPLICIT)) return e;
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (b->dtype == POM_F32)
-    return run_reverse<float>(b, prm, ramp, vgrid, state_in, geom, internal, cot_out, grad_in, workspace, ws_bytes, st);
-  return run_reverse<double>(b, prm, ramp, vgrid, state_in, geom, internal, cot_out, grad_in, workspace, ws_bytes, st);
+    return run_reverse<float>(b, prm, ramp, vgrid, state_in, state_out, geom, internal, cot_out, grad_in, workspace,
+                              ws_bytes, st);
+  return run_reverse<double>(b, prm, ramp, vgrid, state_in, state_out, geom, internal, cot_out, grad_in, workspace,
+                             ws_bytes, st);
 }
 
 int pom_dens(const pom_batch* b, const pom_params_t* prm, const void* vgrid, const void* t, const void* s,
@@ -2739,17 +2745,17 @@ int pom_implicit_step_taped(const pom_batch* b, const pom_params_t* prm, const v
 }
 
 int pom_implicit_step_reverse(const pom_batch* b, const pom_params_t* prm, const void* cache, const void* ramp,
-                              const void* vgrid, const void* const* state_in, const void* const* geom,
-                              int32_t internal, const void* const* cot_out, void* const* grad_in, void* workspace,
-                              size_t ws_bytes, void* stream) {
-  if (!b || !cache || !ramp || !vgrid || !state_in || !geom || !workspace) return POM_E_ARG;
+                              const void* vgrid, const void* const* state_in, const void* const* state_out,
+                              const void* const* geom, int32_t internal, const void* const* cot_out,
+                              void* const* grad_in, void* workspace, size_t ws_bytes, void* stream) {
+  if (!b || !cache || !ramp || !vgrid || !state_in || !state_out || !geom || !workspace) return POM_E_ARG;
   if (int e = check_params(prm, POM_CORE_IMPLICIT)) return e;
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (b->dtype == POM_F32)
-    return run_reverse<float>(b, prm, ramp, vgrid, state_in, geom, internal, cot_out, grad_in, workspace, ws_bytes,
-                              st, cache);
-  return run_reverse<double>(b, prm, ramp, vgrid, state_in, geom, internal, cot_out, grad_in, workspace, ws_bytes,
-                             st, cache);
+    return run_reverse<float>(b, prm, ramp, vgrid, state_in, state_out, geom, internal, cot_out, grad_in, workspace,
+                              ws_bytes, st, cache);
+  return run_reverse<double>(b, prm, ramp, vgrid, state_in, state_out, geom, internal, cot_out, grad_in, workspace,
+                             ws_bytes, st, cache);
 }
 
 }  // extern "C"

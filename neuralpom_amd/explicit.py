@@ -153,14 +153,21 @@ class Ensemble:
                                  f"with {nel(n)} elements, got {t.dtype} {t.device} {tuple(t.shape)}")
 
     # ------------------------------------------------------------ step
-    def step(self, state, geom, sc: StepScalars, ramp, vgrid, internal=True, ws_pool=None):
-        """One internal step; returns (new state dict, vamax tensor)."""
+    def _outputs(self, state, out):
+        if out is None:
+            return {n: torch.empty_like(state[n]) for n in L.FIELDS}
+        self._check([out[n] for n in L.FIELDS], L.FIELDS, self.numel)
+        return out
+
+    def step(self, state, geom, sc: StepScalars, ramp, vgrid, internal=True, ws_pool=None, out=None):
+        """One internal step; returns (new state dict, vamax tensor).  ``out``:
+        optional buffers (every field, not aliasing ``state``) for the new state."""
         prm = sc.struct()
         ins = [state[n] for n in L.FIELDS]
         gs = [geom[n] for n in L.GEOM]
         self._check(ins, L.FIELDS, self.numel)
         self._check(gs, L.GEOM, self.numel)
-        out = {n: torch.empty_like(state[n]) for n in L.FIELDS}
+        out = self._outputs(state, out)
         vamax = torch.zeros(1, dtype=self.dtype, device=self.device)
         ws, nb = self._workspace(prm, False, ws_pool)
         stream = torch.cuda.current_stream(self.device).cuda_stream
@@ -193,17 +200,18 @@ class Ensemble:
 
     def step_adj(self, state, geom, sc: StepScalars, ramp, vgrid, internal, cot, ws_pool=None):
         """Gradients of the step inputs for output cotangents ``cot`` (dict, may omit fields)."""
-        return self._reverse_call("pom_explicit_step_adj", state, geom, sc, ramp, vgrid, internal, cot, ws_pool)
+        return self._reverse_call("pom_explicit_step_adj", state, None, geom, sc, ramp, vgrid, internal, cot,
+                                  ws_pool)
 
-    def step_taped(self, state, geom, sc: StepScalars, ramp, vgrid, internal=True, ws_pool=None):
+    def step_taped(self, state, geom, sc: StepScalars, ramp, vgrid, internal=True, ws_pool=None, out=None):
         """``step`` that keeps its intermediates in the adjoint workspace of
-        ``ws_pool`` (default: the ensemble's) for ``step_reverse``."""
+        ``ws_pool`` (default: the ensemble's) and in its outputs, for ``step_reverse``."""
         prm = sc.struct()
         ins = [state[n] for n in L.FIELDS]
         gs = [geom[n] for n in L.GEOM]
         self._check(ins, L.FIELDS, self.numel)
         self._check(gs, L.GEOM, self.numel)
-        out = {n: torch.empty_like(state[n]) for n in L.FIELDS}
+        out = self._outputs(state, out)
         vamax = torch.zeros(1, dtype=self.dtype, device=self.device)
         ws, nb = self._workspace(prm, True, ws_pool)
         stream = torch.cuda.current_stream(self.device).cuda_stream
@@ -214,24 +222,27 @@ class Ensemble:
         L.check(rc, "pom_explicit_step_taped")
         return out, vamax
 
-    def step_reverse(self, state, geom, sc: StepScalars, ramp, vgrid, internal, cot, ws_pool=None):
+    def step_reverse(self, state, out, geom, sc: StepScalars, ramp, vgrid, internal, cot, ws_pool=None):
         """``step_adj`` of the last ``step_taped`` of the same arguments, from its
-        kept intermediates (no forward recompute).  ``state`` and ``geom`` must
-        be the taped call's, unchanged, and the workspace untouched since."""
-        return self._reverse_call("pom_explicit_step_reverse", state, geom, sc, ramp, vgrid, internal, cot, ws_pool)
+        kept intermediates (no forward recompute).  ``state``, ``out`` (the
+        taped call's outputs) and ``geom`` must be unchanged since that call, and
+        its workspace untouched."""
+        return self._reverse_call("pom_explicit_step_reverse", state, out, geom, sc, ramp, vgrid, internal, cot,
+                                  ws_pool)
 
-    def _reverse_call(self, fn, state, geom, sc, ramp, vgrid, internal, cot, ws_pool):
+    def _reverse_call(self, fn, state, out, geom, sc, ramp, vgrid, internal, cot, ws_pool):
         prm = sc.struct()
         ins = [state[n] for n in L.FIELDS]
         self._check(ins, L.FIELDS, self.numel)
         grads, cot_ptrs, grad_ptrs = self._adj_buffers(cot)
         ws, nb = self._workspace(prm, True, ws_pool)
         stream = torch.cuda.current_stream(self.device).cuda_stream
-        rc = getattr(self.lib, fn)(
-            self._h, C.byref(prm), ramp.data_ptr(), vgrid.data_ptr(),
-            L.ptr_array([t.data_ptr() for t in ins]), L.geom_ptrs(geom),
-            int(bool(internal)), cot_ptrs, grad_ptrs, ws.data_ptr(), nb, stream)
-        L.check(rc, fn)
+        args = [self._h, C.byref(prm), ramp.data_ptr(), vgrid.data_ptr(), L.ptr_array([t.data_ptr() for t in ins])]
+        if out is not None:
+            self._check([out[n] for n in L.FIELDS], L.FIELDS, self.numel)
+            args.append(L.ptr_array([out[n].data_ptr() for n in L.FIELDS]))
+        args += [L.geom_ptrs(geom), int(bool(internal)), cot_ptrs, grad_ptrs, ws.data_ptr(), nb, stream]
+        L.check(getattr(self.lib, fn)(*args), fn)
         return grads
 
 
@@ -262,32 +273,35 @@ class ExplicitStep(torch.autograd.Function):
 
 
 class StepGraph:
-    """One explicit internal step (and optionally its adjoint) recorded once
-    into a HIP graph and replayed.
+    """One explicit internal step (and optionally its adjoint) recorded into
+    HIP graphs and replayed.
 
-    The graph reads the static buffers ``state`` (every field of FIELDS),
-    ``ramp`` and, with ``cot``, the output cotangents; a replay advances
-    ``state`` in place by one step (the step's outputs are copied back into the
-    input buffers inside the graph) and, with ``cot``, leaves the input
-    cotangents of the step it just took in ``grads``: the forward keeps its
-    intermediates (``step_taped``) and the reverse runs from them
-    (``step_reverse``), so the step's forward runs once.  Refill ``ramp`` (and
-    ``cot``) between replays to change them.  ``on_warm`` is called between
-    the warm-up and the recording.  The step launches ~100 kernels
-    per barotropic substep cycle and has no host synchronisation, so a replay
-    submits the whole step with one call.  A semi-implicit ensemble
+    Two graphs are recorded over two state buffers: one steps buffer A into
+    buffer B, the other B into A, and replays alternate between them, so the
+    new state is never copied back.  ``state`` is the buffer holding the
+    current state (read it after a replay; it changes with every replay), and
+    with ``cot``, ``grads`` holds the input cotangents of the step just taken:
+    the forward keeps its intermediates (``step_taped``) and the reverse runs
+    from them (``step_reverse``), so the step's forward runs once.  The graphs
+    read the static buffers ``ramp`` and, with ``cot``, the output
+    cotangents; refill them between replays to change them.  ``on_warm`` is
+    called between the warm-up and the recording, ``on_first`` after the first
+    of the two graphs is recorded.  The step launches ~100
+    kernels per barotropic substep cycle and has no host synchronisation, so
+    a replay submits the whole step with one call.  A semi-implicit ensemble
     (``ImplicitEnsemble``) is recorded the same way with ``eta_capture`` set
     in its scalars: its eta solves then launch a fixed number of iterations
     and never synchronise with the host (converged members do nothing, so the
     replay equals the eager step).
 
-    The graph owns its workspaces (allocated on the current stream before the
-    warm-up and capture, and held for the graph's lifetime), so later eager
-    calls on the same ensemble, whatever scratch they need, never touch the
-    memory the recorded kernels use.
+    The graphs own their workspace (allocated on the current stream before
+    the warm-up and capture, and held for the graphs' lifetime), so later
+    eager calls on the same ensemble, whatever scratch they need, never touch
+    the memory the recorded kernels use.
     """
 
-    def __init__(self, ens, state, geom, sc, ramp, vgrid, internal=True, cot=None, warmup=1, on_warm=None):
+    def __init__(self, ens, state, geom, sc, ramp, vgrid, internal=True, cot=None, warmup=1, on_warm=None,
+                 on_first=None):
         if hasattr(sc, "eta_capture") and not sc.eta_capture:
             raise ValueError("a semi-implicit step is recorded with eta_capture=True "
                              "(its eta solves otherwise poll the host for convergence)")
@@ -295,41 +309,54 @@ class StepGraph:
         self._ws = {}
         prm = sc.struct()
         ens._workspace(prm, cot is not None, self._ws)
-        self.state = {n: state[n].clone() for n in L.FIELDS}
+        self._bufs = [{n: state[n].clone() for n in L.FIELDS}, {n: torch.empty_like(state[n]) for n in L.FIELDS}]
+        self._cur = 0
         self.geom, self.sc, self.vgrid, self.internal = geom, sc, vgrid, internal
         self.ramp = ramp.clone()
         self.cot = None if cot is None else {n: c.contiguous().to(ens.dtype).clone() for n, c in cot.items()}
-        self.grads = None
         s = torch.cuda.Stream(device=ens.device)
         s.wait_stream(torch.cuda.current_stream(ens.device))
-        with torch.cuda.stream(s):   # warm-up outside the graph (on copies): workspaces are allocated here
+        with torch.cuda.stream(s):   # warm-up outside the graphs (on copies): workspaces are allocated here
             for _ in range(warmup):
-                self._body({n: t.clone() for n, t in self.state.items()})
+                self._body({n: t.clone() for n, t in self._bufs[0].items()},
+                           {n: torch.empty_like(t) for n, t in self._bufs[0].items()})
         torch.cuda.current_stream(ens.device).wait_stream(s)
         if on_warm is not None:   # e.g. reset HIP-event tracing before the recorded launches
             torch.cuda.synchronize(ens.device)
             on_warm()
-        self.graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(self.graph):
-            self.grads, self.vamax = self._body(self.state)
+        self._graphs, self._res = [], []
+        for k in (0, 1):
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self._res.append(self._body(self._bufs[k], self._bufs[1 - k]))
+            self._graphs.append(g)
+            if k == 0 and on_first is not None:   # e.g. stop HIP-event tracing after the first graph
+                on_first()
+        self.grads, self.vamax = None, None
 
-    def _body(self, st):
+    def _body(self, st, out):
         grads = None
         if self.cot is None:
-            out, vamax = self.ens.step(st, self.geom, self.sc, self.ramp, self.vgrid, self.internal,
-                                       ws_pool=self._ws)
+            _, vamax = self.ens.step(st, self.geom, self.sc, self.ramp, self.vgrid, self.internal,
+                                     ws_pool=self._ws, out=out)
         else:   # the forward keeps its intermediates and the reverse runs from them
-            out, vamax = self.ens.step_taped(st, self.geom, self.sc, self.ramp, self.vgrid, self.internal,
-                                             ws_pool=self._ws)
-            grads = self.ens.step_reverse(st, self.geom, self.sc, self.ramp, self.vgrid, self.internal,
+            _, vamax = self.ens.step_taped(st, self.geom, self.sc, self.ramp, self.vgrid, self.internal,
+                                           ws_pool=self._ws, out=out)
+            grads = self.ens.step_reverse(st, out, self.geom, self.sc, self.ramp, self.vgrid, self.internal,
                                           dict(self.cot), ws_pool=self._ws)
-        for n in L.FIELDS:
-            st[n].copy_(out[n])
         return grads, vamax
+
+    @property
+    def state(self):
+        """The buffers holding the current state."""
+        return self._bufs[self._cur]
 
     def replay(self):
         """Advance ``state`` by one step (and fill ``grads``); returns the step's max |vaf| (device)."""
-        self.graph.replay()
+        k = self._cur
+        self._graphs[k].replay()
+        self.grads, self.vamax = self._res[k]
+        self._cur = 1 - k
         return self.vamax
 
 

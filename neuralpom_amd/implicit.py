@@ -95,16 +95,18 @@ class ImplicitEnsemble(Ensemble):
             pool[key] = ws
         return ws, n.value
 
-    def step(self, state, geom, sc: ImplicitScalars, ramp, vgrid, internal=True, iters=None, ws_pool=None):
+    def step(self, state, geom, sc: ImplicitScalars, ramp, vgrid, internal=True, iters=None, ws_pool=None,
+             out=None):
         """One internal step; returns (new state dict, vamax).  ``iters``: optional
-        int32 device tensor [members] receiving the eta-solve iteration counts."""
+        int32 device tensor [members] receiving the eta-solve iteration counts;
+        ``out``: optional buffers for the new state."""
         if not self._prepared:
             raise RuntimeError("ImplicitEnsemble.prepare(geom, scalars) must run before the first step")
         prm = sc.struct()
         ins = [state[n] for n in L.FIELDS]
         self._check(ins, L.FIELDS, self.numel)
         gp = self._geom_ptrs(geom)
-        out = {n: torch.empty_like(state[n]) for n in L.FIELDS}
+        out = self._outputs(state, out)
         vamax = torch.zeros(1, dtype=self.dtype, device=self.device)
         ws, nb = self._workspace(prm, False, ws_pool)
         stream = torch.cuda.current_stream(self.device).cuda_stream
@@ -118,17 +120,20 @@ class ImplicitEnsemble(Ensemble):
 
     def step_adj(self, state, geom, sc: ImplicitScalars, ramp, vgrid, internal, cot, ws_pool=None):
         """Gradients of the step inputs for output cotangents ``cot`` (dict, may omit fields)."""
-        return self._reverse_call("pom_implicit_step_adj", state, geom, sc, ramp, vgrid, internal, cot, ws_pool)
+        return self._reverse_call("pom_implicit_step_adj", state, None, geom, sc, ramp, vgrid, internal, cot,
+                                  ws_pool)
 
-    def step_taped(self, state, geom, sc: ImplicitScalars, ramp, vgrid, internal=True, iters=None, ws_pool=None):
-        """``step`` that keeps its intermediates in the adjoint workspace for ``step_reverse``."""
+    def step_taped(self, state, geom, sc: ImplicitScalars, ramp, vgrid, internal=True, iters=None, ws_pool=None,
+                   out=None):
+        """``step`` that keeps its intermediates in the adjoint workspace and its
+        outputs for ``step_reverse``."""
         if not self._prepared:
             raise RuntimeError("ImplicitEnsemble.prepare(geom, scalars) must run before the first step")
         prm = sc.struct()
         ins = [state[n] for n in L.FIELDS]
         self._check(ins, L.FIELDS, self.numel)
         gp = self._geom_ptrs(geom)
-        out = {n: torch.empty_like(state[n]) for n in L.FIELDS}
+        out = self._outputs(state, out)
         vamax = torch.zeros(1, dtype=self.dtype, device=self.device)
         ws, nb = self._workspace(prm, True, ws_pool)
         stream = torch.cuda.current_stream(self.device).cuda_stream
@@ -140,11 +145,13 @@ class ImplicitEnsemble(Ensemble):
         L.check(rc, "pom_implicit_step_taped")
         return out, vamax
 
-    def step_reverse(self, state, geom, sc: ImplicitScalars, ramp, vgrid, internal, cot, ws_pool=None):
-        """``step_adj`` of the last ``step_taped`` of the same arguments, from its kept intermediates."""
-        return self._reverse_call("pom_implicit_step_reverse", state, geom, sc, ramp, vgrid, internal, cot, ws_pool)
+    def step_reverse(self, state, out, geom, sc: ImplicitScalars, ramp, vgrid, internal, cot, ws_pool=None):
+        """``step_adj`` of the last ``step_taped`` of the same arguments, from its
+        kept intermediates; ``state``, ``out`` and ``geom`` unchanged since."""
+        return self._reverse_call("pom_implicit_step_reverse", state, out, geom, sc, ramp, vgrid, internal, cot,
+                                  ws_pool)
 
-    def _reverse_call(self, fn, state, geom, sc, ramp, vgrid, internal, cot, ws_pool):
+    def _reverse_call(self, fn, state, out, geom, sc, ramp, vgrid, internal, cot, ws_pool):
         if not self._prepared:
             raise RuntimeError("ImplicitEnsemble.prepare(geom, scalars) must run before the first step")
         prm = sc.struct()
@@ -154,11 +161,13 @@ class ImplicitEnsemble(Ensemble):
         grads, cot_ptrs, grad_ptrs = self._adj_buffers(cot)
         ws, nb = self._workspace(prm, True, ws_pool)
         stream = torch.cuda.current_stream(self.device).cuda_stream
-        rc = getattr(self.lib, fn)(
-            self._h, C.byref(prm), self.cache.data_ptr(), ramp.data_ptr(), vgrid.data_ptr(),
-            L.ptr_array([t.data_ptr() for t in ins]), gp,
-            int(bool(internal)), cot_ptrs, grad_ptrs, ws.data_ptr(), nb, stream)
-        L.check(rc, fn)
+        args = [self._h, C.byref(prm), self.cache.data_ptr(), ramp.data_ptr(), vgrid.data_ptr(),
+                L.ptr_array([t.data_ptr() for t in ins])]
+        if out is not None:
+            self._check([out[n] for n in L.FIELDS], L.FIELDS, self.numel)
+            args.append(L.ptr_array([out[n].data_ptr() for n in L.FIELDS]))
+        args += [gp, int(bool(internal)), cot_ptrs, grad_ptrs, ws.data_ptr(), nb, stream]
+        L.check(getattr(self.lib, fn)(*args), fn)
         return grads
 
     # ---------------------------------------------------------------- eta solve on its own
