@@ -1502,7 +1502,7 @@ template <class T> struct Adj {
       COutTab<T> ob{{sal ? b_sf0 : b_tf0}};
       T* ib[OpAdvt1::NIN] = {bi[sal ? POM_SB : POM_TB], bi[sal ? POM_S : POM_T], nullptr, b_u1, b_v1, bo[POM_W],
                              bo[POM_AAM], bi[POM_DT], bi[POM_ETB], b_etf30};
-      launch_gather<T, OpAdvt1>(c, s.t_advt1(sal == 1), ob, ib);
+      launch_flux_advt1<T>(c, s.t_advt1(sal == 1), ob, ib);
     }
     if (s.imp && !s.open) {   // closed-basin bcond 6 + q2 filter
       COutTab<T> ob{{bo[POM_Q2], bo[POM_Q2B], bo[POM_Q2L], bo[POM_Q2LB]}};
