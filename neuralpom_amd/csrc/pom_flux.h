@@ -1,4 +1,4 @@
-// Flux-form adjoints of the tracer-advection operators: transposed stencils
+// Flux-form adjoints of the tracer and turbulence advection operators: transposed stencils
 // that take each face flux's partials once, in place of the generic gather
 // (pom_kernels.h), which re-evaluates the whole point function once per
 // stencil offset in dual numbers.
@@ -168,6 +168,156 @@ __global__ void __launch_bounds__(TPB) k_flux_advt1(Batch B, Vert<T> vert, Param
   if (ib.o[O::W]) put_in(ib, O::W, c3, b_w);
   if (ib.o[O::AAM]) put_in(ib, O::AAM, c3, b_aam);
   // 2-D inputs: sums over the column group
+#pragma unroll
+  for (int o = 8; o > 0; o >>= 1) {
+    b_dt += __shfl_xor(b_dt, o, 16);
+    b_etb += __shfl_xor(b_etb, o, 16);
+    b_etf += __shfl_xor(b_etf, o, 16);
+  }
+  if (k == 0) {
+    const int64_t c2 = ix2(q, i, j);
+    if (ib.o[O::DT]) put_in(ib, O::DT, c2, b_dt);
+    if (ib.o[O::ETB]) put_in(ib, O::ETB, c2, b_etb);
+    if (ib.o[O::ETF]) put_in(ib, O::ETF, c2, b_etf);
+  }
+}
+
+// partials of one horizontal face flux of the turbulence advection at a
+// w level k (tracers.py:41-99): flux = (0.125 (q0 + q1) (dt0 + dt1) (vel0(k) + vel0(k-1))
+//   - 0.25 (aam0(k) + aam1(k) + aam0(k-1) + aam1(k-1)) (h0 + h1) (qb0 - qb1) mask0 / (ds0 + ds1))
+//   * 0.5 (dn0 + dn1); sides and spacings as for face_t1.  Scaled by the face cotangent c.
+template <class T> struct FaceQ {
+  T flux;
+  T d_q, d_dt, d_vel, d_qb, d_aam;   // (q0 = q1, dt0 = dt1, both vel levels, qb0 = -qb1, each of the four aam)
+};
+
+template <class T>
+__device__ __forceinline__ FaceQ<T> face_q(T q0, T q1, T dt0, T dt1, T vk, T vkm, T a0k, T a1k, T a0m, T a1m, T qb0,
+                                           T qb1, T h0, T h1, T mask0, T ds0, T ds1, T dn0, T dn1, T c) {
+  FaceQ<T> r;
+  const T sq = q0 + q1, sdt = dt0 + dt1, sv = vk + vkm;
+  const T dnh = T(0.5) * (dn0 + dn1);
+  const T g = T(0.25) * (h0 + h1) * mask0 * rcp_lane(ds0 + ds1);
+  const T asum = a0k + a1k + a0m + a1m;
+  r.flux = (T(0.125) * sq * sdt * sv - asum * g * (qb0 - qb1)) * dnh;
+  const T cd = c * dnh;
+  r.d_q = T(0.125) * sdt * sv * cd;
+  r.d_dt = T(0.125) * sq * sv * cd;
+  r.d_vel = T(0.125) * sq * sdt * cd;
+  r.d_qb = -asum * g * cd;
+  r.d_aam = -g * (qb0 - qb1) * cd;
+  return r;
+}
+
+// the transposed advq (OpAdvq in pom_ops.h; tracers.py:14-156), branch-free as
+// k_flux_advt1.  A face at level k also reads vel and aam at level k - 1:
+// those partials go to the lane of level k - 1 by a shuffle.
+template <class T>
+__global__ void __launch_bounds__(TPB) k_flux_advq(Batch B, Vert<T> vert, Params prm, FieldTab<T> tab,
+                                                    COutTab<T> obar, InTab<T> ib) {
+  using O = OpAdvq;
+  Pos q;
+  if (!pos3(B, q)) return;   // whole 16-lane column groups are live or not
+  const FluxLoad<T> L{&tab, q};
+  const int i = q.i, j = q.j, k = q.k, im = q.im, jm = q.jm;
+  const int iw = i > 0 ? i - 1 : 0, ie = i < im - 1 ? i + 1 : im - 1;
+  const int js = j > 0 ? j - 1 : 0, jn = j < jm - 1 ? j + 1 : jm - 1;
+  const T dti2t = T(prm.dti2);
+  const T* ob = obar.o[0] + q.o3;
+
+  // ---- loads
+  const T qq = L.f3(O::Q, i, j, k), qbq = L.f3(O::QB, i, j, k), uq = L.f3(O::U, i, j, k), vq = L.f3(O::V, i, j, k);
+  const T wq = L.f3(O::W, i, j, k), amq = L.f3(O::AAM, i, j, k), obq = ld_local(ob, lix3(q, i, j, k));
+  const T dtq = L.f2(O::DT, i, j), etfq = L.f2(O::ETF, i, j), etbq = L.f2(O::ETB, i, j);
+  const T hq = L.g(O::GH, i, j), dxq = L.g(O::GDX, i, j), dyq = L.g(O::GDY, i, j), artq = L.g(O::GART, i, j);
+  const T dumq = L.g(O::GDUM, i, j), dvmq = L.g(O::GDVM, i, j);
+  const T q_w = L.f3(O::Q, iw, j, k), qb_w = L.f3(O::QB, iw, j, k), am_w = L.f3(O::AAM, iw, j, k);
+  const T ob_w = ld_local(ob, lix3(q, iw, j, k));
+  const T dt_w = L.f2(O::DT, iw, j), etf_w = L.f2(O::ETF, iw, j);
+  const T h_w = L.g(O::GH, iw, j), dx_w = L.g(O::GDX, iw, j), dy_w = L.g(O::GDY, iw, j), art_w = L.g(O::GART, iw, j);
+  const T q_e = L.f3(O::Q, ie, j, k), qb_e = L.f3(O::QB, ie, j, k), am_e = L.f3(O::AAM, ie, j, k);
+  const T u_e = L.f3(O::U, ie, j, k), ob_e = ld_local(ob, lix3(q, ie, j, k));
+  const T dt_e = L.f2(O::DT, ie, j), etf_e = L.f2(O::ETF, ie, j);
+  const T h_e = L.g(O::GH, ie, j), dx_e = L.g(O::GDX, ie, j), dy_e = L.g(O::GDY, ie, j), art_e = L.g(O::GART, ie, j);
+  const T dum_e = L.g(O::GDUM, ie, j);
+  const T q_s = L.f3(O::Q, i, js, k), qb_s = L.f3(O::QB, i, js, k), am_s = L.f3(O::AAM, i, js, k);
+  const T ob_s = ld_local(ob, lix3(q, i, js, k));
+  const T dt_s = L.f2(O::DT, i, js), etf_s = L.f2(O::ETF, i, js);
+  const T h_s = L.g(O::GH, i, js), dx_s = L.g(O::GDX, i, js), dy_s = L.g(O::GDY, i, js), art_s = L.g(O::GART, i, js);
+  const T q_n = L.f3(O::Q, i, jn, k), qb_n = L.f3(O::QB, i, jn, k), am_n = L.f3(O::AAM, i, jn, k);
+  const T v_n = L.f3(O::V, i, jn, k), ob_n = ld_local(ob, lix3(q, i, jn, k));
+  const T dt_n = L.f2(O::DT, i, jn), etf_n = L.f2(O::ETF, i, jn);
+  const T h_n = L.g(O::GH, i, jn), dx_n = L.g(O::GDX, i, jn), dy_n = L.g(O::GDY, i, jn), art_n = L.g(O::GART, i, jn);
+  const T dvm_n = L.g(O::GDVM, i, jn);
+  const T dzk = vert.dz(q.m, k), dzu = vert.dz(q.m, k - 1);
+  // values at level k - 1 of this lane's columns (level 0 takes its own: its
+  // faces carry a zero cotangent, since no output lives on level 0)
+  const T u_km = __shfl_up(uq, 1, 16), v_km = __shfl_up(vq, 1, 16), am_km = __shfl_up(amq, 1, 16);
+  const T u_e_km = __shfl_up(u_e, 1, 16), v_n_km = __shfl_up(v_n, 1, 16);
+  const T am_w_km = __shfl_up(am_w, 1, 16), am_e_km = __shfl_up(am_e, 1, 16);
+  const T am_s_km = __shfl_up(am_s, 1, 16), am_n_km = __shfl_up(am_n, 1, 16);
+
+  // ---- r = obar / (tf + 1e-20) on the interior w levels 1 .. KBM1 - 1, zero elsewhere
+  const bool kin = k >= 1 && k < KBM1;
+  auto inter = [&](int ii, int jj) { return kin && ii >= 1 && ii <= im - 2 && jj >= 1 && jj <= jm - 2; };
+  auto rsel = [&](bool in, T o, T h, T etf, T art) { return in ? o * rcp_lane((h + etf) * art + T(1e-20)) : T(0); };
+  const bool in = inter(i, j);
+  const T rc = rsel(in, obq, hq, etfq, artq);
+  const T rw = rsel(i >= 1 && inter(i - 1, j), ob_w, h_w, etf_w, art_w);
+  const T re = rsel(i + 1 <= im - 1 && inter(i + 1, j), ob_e, h_e, etf_e, art_e);
+  const T rs = rsel(j >= 1 && inter(i, j - 1), ob_s, h_s, etf_s, art_s);
+  const T rn = rsel(j + 1 <= jm - 1 && inter(i, j + 1), ob_n, h_n, etf_n, art_n);
+
+  // ---- horizontal faces at this level
+  const FaceQ<T> Xw = face_q(qq, q_w, dtq, dt_w, uq, u_km, amq, am_w, am_km, am_w_km, qbq, qb_w, hq, h_w, dumq, dxq,
+                             dx_w, dyq, dy_w, i >= 1 ? dti2t * (rc - rw) : T(0));
+  const FaceQ<T> Xe = face_q(q_e, qq, dt_e, dtq, u_e, u_e_km, am_e, amq, am_e_km, am_km, qb_e, qbq, h_e, hq, dum_e,
+                             dx_e, dxq, dy_e, dyq, i + 1 <= im - 1 ? dti2t * (re - rc) : T(0));
+  const FaceQ<T> Ys = face_q(qq, q_s, dtq, dt_s, vq, v_km, amq, am_s, am_km, am_s_km, qbq, qb_s, hq, h_s, dvmq, dyq,
+                             dy_s, dxq, dx_s, j >= 1 ? dti2t * (rc - rs) : T(0));
+  const FaceQ<T> Yn = face_q(q_n, qq, dt_n, dtq, v_n, v_n_km, am_n, amq, am_n_km, am_km, qb_n, qbq, h_n, hq, dvm_n,
+                             dy_n, dyq, dx_n, dxq, j + 1 <= jm - 1 ? dti2t * (rn - rc) : T(0));
+  // this level's inputs
+  T b_q = Xw.d_q + Xe.d_q + Ys.d_q + Yn.d_q;
+  T b_qb = Xw.d_qb - Xe.d_qb + Ys.d_qb - Yn.d_qb;
+  T b_dt = Xw.d_dt + Xe.d_dt + Ys.d_dt + Yn.d_dt;
+  const T am_here = Xw.d_aam + Xe.d_aam + Ys.d_aam + Yn.d_aam;
+  // the partials the faces at level k + 1 take of this level (vel, aam at their k - 1)
+  const T up_u = __shfl_down(Xw.d_vel, 1, 16), up_v = __shfl_down(Ys.d_vel, 1, 16);
+  const T up_am = __shfl_down(am_here, 1, 16);
+  const bool below = k + 1 < KB;
+  T b_u = Xw.d_vel + (below ? up_u : T(0));
+  T b_v = Ys.d_vel + (below ? up_v : T(0));
+  T b_aam = am_here + (below ? up_am : T(0));
+
+  // ---- vertical term vert(k) = (w(k-1) q(k-1) - w(k+1) q(k+1)) art / (dz(k) + dz(k-1))
+  const T w_up = __shfl_up(wq, 1, 16), q_up = __shfl_up(qq, 1, 16);
+  const T w_dn = __shfl_down(wq, 1, 16), q_dn = __shfl_down(qq, 1, 16);
+  const T idzs = rcp_lane(dzk + dzu);
+  const T vb = -dti2t * rc * artq * idzs;   // cotangent of w(k-1) q(k-1) (and minus that of w(k+1) q(k+1))
+  const T vb_dn = __shfl_down(vb, 1, 16), vb_up = __shfl_up(vb, 1, 16);
+  const T from_dn = below ? vb_dn : T(0);          // vert(k + 1) reads this level as its k - 1
+  const T from_up = k >= 1 ? vb_up : T(0);         // vert(k - 1) reads this level as its k + 1
+  b_q += (from_dn - from_up) * wq;
+  const T b_w = (from_dn - from_up) * qq;
+
+  // ---- the point itself: tb qb / (tf + 1e-20) and the depth terms
+  const T tb = (hq + etbq) * artq;
+  const T tf = (hq + etfq) * artq;
+  const T vert_k = (w_up * q_up - w_dn * q_dn) * artq * idzs;
+  const T hdiv = Xe.flux - Xw.flux + Yn.flux - Ys.flux;
+  const T num = tb * qbq - dti2t * (vert_k + hdiv);
+  b_qb += in ? tb * rc : T(0);
+  T b_etb = in ? artq * qbq * rc : T(0);
+  T b_etf = in ? -artq * rc * num * rcp_lane(tf + T(1e-20)) : T(0);
+
+  const int64_t c3 = ix3(q, i, j, k);
+  if (ib.o[O::QB]) put_in(ib, O::QB, c3, b_qb);
+  if (ib.o[O::Q]) put_in(ib, O::Q, c3, b_q);
+  if (ib.o[O::U]) put_in(ib, O::U, c3, b_u);
+  if (ib.o[O::V]) put_in(ib, O::V, c3, b_v);
+  if (ib.o[O::W]) put_in(ib, O::W, c3, b_w);
+  if (ib.o[O::AAM]) put_in(ib, O::AAM, c3, b_aam);
 #pragma unroll
   for (int o = 8; o > 0; o >>= 1) {
     b_dt += __shfl_xor(b_dt, o, 16);

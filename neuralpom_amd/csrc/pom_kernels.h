@@ -291,10 +291,13 @@ template <class T, class Op>
 void launch_gather(const Ctx<T>& c, const FieldTab<T>& tab, const COutTab<T>& obar, T* const* inbar,
                    uint32_t assign = 0u);
 
-// the same products for OpAdvt1 by its flux-form transposed stencil
-// (pom_flux.h; instantiated in flux_advt1.hip)
+// the same products for OpAdvt1 and OpAdvq by their flux-form transposed
+// stencils (pom_flux.h; instantiated in flux_adv.hip)
 template <class T>
 void launch_flux_advt1(const Ctx<T>& c, const FieldTab<T>& tab, const COutTab<T>& obar, T* const* inbar,
                        uint32_t assign = 0u);
+template <class T>
+void launch_flux_advq(const Ctx<T>& c, const FieldTab<T>& tab, const COutTab<T>& obar, T* const* inbar,
+                      uint32_t assign = 0u);
 
 }  // namespace pom

@@ -1543,7 +1543,7 @@ template <class T> struct Adj {
       COutTab<T> ob{{l2 ? b_q2lf0 : b_q2f0}};
       T* ib[OpAdvq::NIN] = {bi[l2 ? POM_Q2LB : POM_Q2B], bi[l2 ? POM_Q2L : POM_Q2], b_u1, b_v1, bo[POM_W],
                             bo[POM_AAM], bi[POM_DT], bi[POM_ETB], b_etf30};
-      launch_gather<T, OpAdvq>(c, s.t_advq(l2 == 1), ob, ib);
+      launch_flux_advq<T>(c, s.t_advq(l2 == 1), ob, ib);
     }
     {   // vertvl (w input is not differentiated)
       VertvlAdjArgs<T> a{s.u1, s.v1, s.in[POM_DT], s.g[POM_G_DX], s.g[POM_G_DY], s.g[POM_G_FSM], bo[POM_W], s3a,
