@@ -616,6 +616,18 @@ struct AccT {
   __device__ __forceinline__ T rampv() const { return ramp[p.m]; }
 };
 
+// The gather accessor of an evaluation point known, at compile time, to lie
+// away from every edge (farther than the operator's EDGE_BAND): operators
+// with an edge band then take their interior formula alone (deep_v), so the
+// dual-number types carry only the lanes that formula has.
+template <class T, uint32_t MASK, int OI, int OJ, int OK>
+struct AccTDeep : AccT<T, MASK, OI, OJ, OK> {
+  static constexpr bool deep = true;
+};
+template <class AC, class = void> struct deep_acc : std::false_type {};
+template <class AC> struct deep_acc<AC, std::void_t<decltype(AC::deep)>> : std::bool_constant<AC::deep> {};
+template <class AC> inline constexpr bool deep_v = deep_acc<AC>::value;
+
 // Accessor seeding every field of MASK at the offset (SI, SJ, SK): 3-D fields
 // at (SI, SJ, SK), 2-D fields at (SI, SJ) (the caller leaves 2-D bits out of
 // MASK when SK != 0).

@@ -145,7 +145,7 @@ template <class T> __device__ __forceinline__ void put_in(const InTab<T>& t, int
   else t.o[f][i] += v;
 }
 
-template <class T, class Op, int DI, int DJ, int DK, uint32_t MASK>
+template <class T, class Op, int DI, int DJ, int DK, uint32_t MASK, bool DEEP = false>
 __device__ __forceinline__ void gather_all_term(const FieldTab<T>& tab, const COutTab<T>& obar, const Pos& q, int pk,
                                                 const Vert<T>& vert, const T* ramp, const Params& prm,
                                                 const NbrIdx& nb, T* acc) {
@@ -164,7 +164,9 @@ __device__ __forceinline__ void gather_all_term(const FieldTab<T>& tab, const CO
     if constexpr (Op::RK > 0) {
       if (!valid) return;
     }
-    AccT<T, MASK, DI, DJ, DK> A{&tab, p, vert, ramp, &nb};
+    using ACC = std::conditional_t<DEEP, AccTDeep<T, MASK, DI, DJ, DK>, AccT<T, MASK, DI, DJ, DK>>;
+    const AccT<T, MASK, DI, DJ, DK> base{&tab, p, vert, ramp, &nb};
+    const ACC A{base};
     const int ci = clampi(p.i, p.im), cj = clampi(p.j, p.jm);
     const uint32_t idx = Op::OUT3 ? lix3(p, ci, cj, clampi(p.k, KB)) : lix2(p, ci, cj);
     static_for<0, Op::NOUT - 1>([&](auto O) {
@@ -203,7 +205,6 @@ template <class Op> __device__ __forceinline__ bool deep_point(const Pos& q) {
     return q.i >= EB + Op::RI && q.i <= q.im - 1 - EB - Op::RI && q.j >= EB + Op::RJ && q.j <= q.jm - 1 - EB - Op::RJ;
   }
 }
-
 // 3-D operators: one thread per 3-D point q; 3-D fields accumulate at q, 2-D
 // fields per level (offsets with dk = 0 only) and are reduced over the 16
 // lanes of the water column
@@ -218,17 +219,24 @@ __global__ void __launch_bounds__(TPB) k_gather_all3(Batch B, Vert<T> vert, cons
   if (live) {
     NbrIdx nb;
     make_nbr(q, nb);
-    const bool deep = deep_point<Op>(q);
-    static_for<-Op::RI, Op::RI>([&](auto I) {
-      static_for<-Op::RJ, Op::RJ>([&](auto J) {
-        static_for<-Op::RK, Op::RK>([&](auto K) {
-          constexpr int di = decltype(I)::value, dj = decltype(J)::value, dk = decltype(K)::value;
-          if (!(edge_band<Op>::value >= 0 && (di != 0 || dj != 0) && deep))
+    if (edge_band<Op>::value >= 0 && deep_point<Op>(q)) {
+      // own column only: a compact path of the vertical offsets
+      static_for<-Op::RK, Op::RK>([&](auto K) {
+        constexpr int dk = decltype(K)::value;
+        gather_all_term<T, Op, 0, 0, dk, (M3 | (dk == 0 ? M2 : 0u)), true>(tab, obar, q, q.k - dk, vert, ramp, prm,
+                                                                          nb, acc);
+      });
+    } else {
+      static_for<-Op::RI, Op::RI>([&](auto I) {
+        static_for<-Op::RJ, Op::RJ>([&](auto J) {
+          static_for<-Op::RK, Op::RK>([&](auto K) {
+            constexpr int di = decltype(I)::value, dj = decltype(J)::value, dk = decltype(K)::value;
             gather_all_term<T, Op, di, dj, dk, (M3 | (dk == 0 ? M2 : 0u))>(tab, obar, q, q.k - dk, vert, ramp,
                                                                            prm, nb, acc);
+          });
         });
       });
-    });
+    }
   }
   static_for<0, Op::NIN - 1>([&](auto Fc) {
     constexpr int f = decltype(Fc)::value;
@@ -254,14 +262,16 @@ __global__ void __launch_bounds__(TPB) k_gather_all2(Batch B, Vert<T> vert, cons
   for (int f = 0; f < Op::NIN; ++f) acc[f] = T(0);
   NbrIdx nb;
   make_nbr(q, nb);
-  const bool deep = deep_point<Op>(q);
-  static_for<-Op::RI, Op::RI>([&](auto I) {
-    static_for<-Op::RJ, Op::RJ>([&](auto J) {
-      constexpr int di = decltype(I)::value, dj = decltype(J)::value;
-      if (!(edge_band<Op>::value >= 0 && (di != 0 || dj != 0) && deep))
+  if (edge_band<Op>::value >= 0 && deep_point<Op>(q)) {   // own column only
+    gather_all_term<T, Op, 0, 0, 0, M2, true>(tab, obar, q, 0, vert, ramp, prm, nb, acc);
+  } else {
+    static_for<-Op::RI, Op::RI>([&](auto I) {
+      static_for<-Op::RJ, Op::RJ>([&](auto J) {
+        constexpr int di = decltype(I)::value, dj = decltype(J)::value;
         gather_all_term<T, Op, di, dj, 0, M2>(tab, obar, q, 0, vert, ramp, prm, nb, acc);
+      });
     });
-  });
+  }
   static_for<0, Op::NIN - 1>([&](auto Fc) {
     constexpr int f = decltype(Fc)::value;
     if (inbar.o[f]) put_in(inbar, f, ix2(q, q.i, q.j), acc[f]);

@@ -393,6 +393,14 @@ template <bool DEPTH> struct OpBc3G {
     }
   }
   template <int O, class AC> __device__ static auto eval(const AC& A, const Params& prm) {
+    if constexpr (deep_v<AC>) {   // away from the edges: the masked interior values
+      if constexpr (O == 0) return AT(UF, 0, 0, 0) * G2(GDUM, 0, 0);
+      else return AT(VF, 0, 0, 0) * G2(GDVM, 0, 0);
+    } else {
+      return eval_edge<O>(A, prm);
+    }
+  }
+  template <int O, class AC> __device__ static auto eval_edge(const AC& A, const Params& prm) {
     const int i = A.p.i, j = A.p.j, k = A.p.k, im = A.p.im, jm = A.p.jm;
     const bool kin = k < KBM1;
     const bool jin = j >= 1 && j <= jm - 2, iin = i >= 1 && i <= im - 2;
@@ -503,18 +511,22 @@ template <bool EXT> struct OpBc4FilterG {
   }
 
   template <int FNEW, int FC, class AC> __device__ static auto tnew(const AC& A, const Params& prm) {
-    const int i = A.p.i, j = A.p.j, k = A.p.k, im = A.p.im, jm = A.p.jm;
-    using R = common_t<common_t<common_t<decltype(edge<FC, 0>(A, prm)), decltype(edge<FC, 1>(A, prm))>,
-                                common_t<decltype(edge<FC, 2>(A, prm)), decltype(edge<FC, 3>(A, prm))>>,
-                       decltype(AT(FNEW, 0, 0, 0))>;
-    R r = cvt<R>(AT(FNEW, 0, 0, 0));
-    if (k < KBM1) {  // later writes win: S > N > W > E
-      if (j == 0) r = cvt<R>(edge<FC, 3>(A, prm));
-      else if (j == jm - 1) r = cvt<R>(edge<FC, 2>(A, prm));
-      else if (i == 0) r = cvt<R>(edge<FC, 1>(A, prm));
-      else if (i == im - 1) r = cvt<R>(edge<FC, 0>(A, prm));
+    if constexpr (deep_v<AC>) {   // away from the edges: the masked new value
+      return AT(FNEW, 0, 0, 0) * G2(GFSM, 0, 0);
+    } else {
+      const int i = A.p.i, j = A.p.j, k = A.p.k, im = A.p.im, jm = A.p.jm;
+      using R = common_t<common_t<common_t<decltype(edge<FC, 0>(A, prm)), decltype(edge<FC, 1>(A, prm))>,
+                                  common_t<decltype(edge<FC, 2>(A, prm)), decltype(edge<FC, 3>(A, prm))>>,
+                         decltype(AT(FNEW, 0, 0, 0))>;
+      R r = cvt<R>(AT(FNEW, 0, 0, 0));
+      if (k < KBM1) {  // later writes win: S > N > W > E
+        if (j == 0) r = cvt<R>(edge<FC, 3>(A, prm));
+        else if (j == jm - 1) r = cvt<R>(edge<FC, 2>(A, prm));
+        else if (i == 0) r = cvt<R>(edge<FC, 1>(A, prm));
+        else if (i == im - 1) r = cvt<R>(edge<FC, 0>(A, prm));
+      }
+      return r * G2(GFSM, 0, 0);
     }
-    return r * G2(GFSM, 0, 0);
   }
 
   template <int O, class AC> __device__ static auto eval(const AC& A, const Params& prm) {
@@ -570,16 +582,20 @@ struct OpBc6Filter {
     return BR(inflow, q_in, q_out);
   }
   template <int FNEW, int FC, class AC> __device__ static auto qnew(const AC& A, const Params& prm) {
-    const int i = A.p.i, j = A.p.j, im = A.p.im, jm = A.p.jm;
-    using R = common_t<common_t<common_t<decltype(edge<FC, 0>(A, prm)), decltype(edge<FC, 1>(A, prm))>,
-                                common_t<decltype(edge<FC, 2>(A, prm)), decltype(edge<FC, 3>(A, prm))>>,
-                       decltype(AT(FNEW, 0, 0, 0))>;
-    R r = cvt<R>(AT(FNEW, 0, 0, 0));
-    if (j == 0) r = cvt<R>(edge<FC, 3>(A, prm));
-    else if (j == jm - 1) r = cvt<R>(edge<FC, 2>(A, prm));
-    else if (i == 0) r = cvt<R>(edge<FC, 1>(A, prm));
-    else if (i == im - 1) r = cvt<R>(edge<FC, 0>(A, prm));
-    return r * G2(GFSM, 0, 0) + 1.e-10;
+    if constexpr (deep_v<AC>) {   // away from the edges: the masked new value
+      return AT(FNEW, 0, 0, 0) * G2(GFSM, 0, 0) + 1.e-10;
+    } else {
+      const int i = A.p.i, j = A.p.j, im = A.p.im, jm = A.p.jm;
+      using R = common_t<common_t<common_t<decltype(edge<FC, 0>(A, prm)), decltype(edge<FC, 1>(A, prm))>,
+                                  common_t<decltype(edge<FC, 2>(A, prm)), decltype(edge<FC, 3>(A, prm))>>,
+                         decltype(AT(FNEW, 0, 0, 0))>;
+      R r = cvt<R>(AT(FNEW, 0, 0, 0));
+      if (j == 0) r = cvt<R>(edge<FC, 3>(A, prm));
+      else if (j == jm - 1) r = cvt<R>(edge<FC, 2>(A, prm));
+      else if (i == 0) r = cvt<R>(edge<FC, 1>(A, prm));
+      else if (i == im - 1) r = cvt<R>(edge<FC, 0>(A, prm));
+      return r * G2(GFSM, 0, 0) + 1.e-10;
+    }
   }
   template <int O, class AC> __device__ static auto eval(const AC& A, const Params& prm) {
     constexpr int FNEW = (O < 2) ? Q2F : Q2LF;
@@ -642,12 +658,16 @@ struct OpKEdge {
   }
   template <int O, class AC> __device__ static auto eval(const AC& A, const Params&) {
     constexpr int F = (O == 0) ? KM : KH;
-    const int j = A.p.j, jm = A.p.jm;
-    using R = common_t<common_t<decltype(ew<F, 0>(A)), decltype(ew<F, -1>(A))>, decltype(ew<F, 1>(A))>;
-    R r = cvt<R>(ew<F, 0>(A));
-    if (j == jm - 1) r = cvt<R>(ew<F, -1>(A) * G2(GFSM, 0, 0));
-    else if (j == 0) r = cvt<R>(ew<F, 1>(A) * G2(GFSM, 0, 0));
-    return r;
+    if constexpr (deep_v<AC>) {   // away from the edges: the point's own value
+      return AT(F, 0, 0, 0);
+    } else {
+      const int j = A.p.j, jm = A.p.jm;
+      using R = common_t<common_t<decltype(ew<F, 0>(A)), decltype(ew<F, -1>(A))>, decltype(ew<F, 1>(A))>;
+      R r = cvt<R>(ew<F, 0>(A));
+      if (j == jm - 1) r = cvt<R>(ew<F, -1>(A) * G2(GFSM, 0, 0));
+      else if (j == 0) r = cvt<R>(ew<F, 1>(A) * G2(GFSM, 0, 0));
+      return r;
+    }
   }
 };
 
@@ -689,6 +709,13 @@ struct OpBc1 {
   static constexpr int RI = 1, RJ = 1, RK = 0;
   static constexpr int EDGE_BAND = 1;   // only points within 1 of the edge read neighbours
   template <int O, class AC> __device__ static auto eval(const AC& A, const Params&) {
+    if constexpr (deep_v<AC>) {   // away from the edges: the point's own value
+      return AT2(RAW, 0, 0) * G2(GFSM, 0, 0);
+    } else {
+      return eval_edge<O>(A);
+    }
+  }
+  template <int O, class AC> __device__ static auto eval_edge(const AC& A) {
     const int i = A.p.i, j = A.p.j, im = A.p.im, jm = A.p.jm;
     const int di = (i == 0) ? 1 : (i == im - 1 ? -1 : 0);
     const int dj = (j == 0) ? 1 : (j == jm - 1 ? -1 : 0);

@@ -70,6 +70,8 @@ struct OpImpBc3 {
 // value of a field after the closed-basin bcond-4 edge copies: edge cells take
 // their nearest interior cell (i edges first, then j edges)
 template <int F, class AC> __device__ __forceinline__ auto edge_copy3(const AC& A) {
+  if constexpr (deep_v<AC>) return AT(F, 0, 0, 0);   // away from the edges: the point's own value
+  else {
   const int i = A.p.i, j = A.p.j, im = A.p.im, jm = A.p.jm;
   const int di = (i == 0) ? 1 : (i == im - 1 ? -1 : 0);
   const int dj = (j == 0) ? 1 : (j == jm - 1 ? -1 : 0);
@@ -82,6 +84,7 @@ template <int F, class AC> __device__ __forceinline__ auto edge_copy3(const AC& 
   else POM_EC_CASE(-1, -1)
 #undef POM_EC_CASE
   return r;
+  }
 }
 
 // =====================================================================
@@ -360,6 +363,14 @@ template <bool NEWD> struct OpImpRadG {
     return rf * psqrt(grav / depth<AI, AJ>(A)) * (AT2(EL, AI, AJ) - AT2(EL, BI, BJ));
   }
   template <int O, class AC> __device__ static auto eval(const AC& A, const Params& prm) {
+    if constexpr (deep_v<AC>) {   // away from the edges: the interior values
+      if constexpr (O == 0) return AT2(UI, 0, 0) * G2(GDUM, 0, 0);
+      else return AT2(VI, 0, 0) * G2(GDVM, 0, 0);
+    } else {
+      return eval_edge<O>(A, prm);
+    }
+  }
+  template <int O, class AC> __device__ static auto eval_edge(const AC& A, const Params& prm) {
     const int i = A.p.i, j = A.p.j, E = A.p.im - 1, N = A.p.jm - 1;
     const bool jin = j >= 1 && j <= N - 1, iin = i >= 1 && i <= E - 1;
     if constexpr (O == 0) {
