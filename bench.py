@@ -144,7 +144,8 @@ def run_gpu(args, dtype=torch.float32):
         st, vamax = ens.step(st, geom, sc, r, vg, True)
     torch.cuda.synchronize()
     dtf = time.perf_counter() - t1
-    return dict(cells=cells, seconds=dt, seconds_fwd=dtf, ens=ens, kern_ms=kern_ms, kern_n=kern_n)
+    return dict(cells=cells, seconds=dt, seconds_fwd=dtf, ens=ens, kern_ms=kern_ms, kern_n=kern_n,
+                tuning=ens.tuning_entry)
 
 
 def graph_vs_eager(args, shapes=((90, 82),), nstep=20):
@@ -618,7 +619,7 @@ def main():
         "config": {"workload": "stratified seamount ensemble, 64 members (32x90x82, 24x120x110, "
                                "8x177x161) x 16 levels, explicit core, fwd+adjoint per step",
                    "cells_per_step": res["cells"], "dti_s": DTE * 30, "isplit": 30,
-                   "launch": "eager" if args.eager else "hip graph"},
+                   "launch": "eager" if args.eager else "hip graph", "tuning": res["tuning"]},
         "forward_only": {"value": rate_fwd, "unit": "grid-cell-steps/s"},
         "roofline": {"bound": "hbm", "achieved": kern_bw / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": kern_bw / HBM_PEAK, "traffic": DOMINANT_TRAFFIC, "kernel": DOMINANT_KERNEL,

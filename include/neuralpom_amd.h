@@ -135,6 +135,24 @@ typedef struct pom_batch pom_batch;
 int pom_batch_create(const pom_member_t* members, int32_t n_members, int32_t dtype, pom_batch** out);
 int pom_batch_destroy(pom_batch* b);
 
+/* Launch shapes of a batch's point kernels.  A launch over n blocks of 256
+ * points takes n workgroups, or, once n exceeds *_min_groups, ceil(n / bpg)
+ * of them (at least *_min_groups), each striding over the blocks:
+ *   map2_*:  the fused barotropic substep (default 4 blocks, 768 minimum);
+ *   flat2_*: 2-D maps and adjoint gathers (default 1, 768);
+ *   flat3_*: 3-D maps and adjoint gathers (default 1, 768).
+ * Results do not depend on them.  Every value is in [1, 2^20].  The tuner
+ * (neuralpom_amd/autotune.py) stores the best shapes per workload in
+ * neuralpom_amd/tuning.json, which the Python ensembles apply.  No
+ * counterpart in the reference (its kernels are torch's). */
+typedef struct {
+  int32_t map2_blocks_per_group, map2_min_groups;
+  int32_t flat2_blocks_per_group, flat2_min_groups;
+  int32_t flat3_blocks_per_group, flat3_min_groups;
+} pom_tuning_t;
+int pom_batch_tune(pom_batch* b, const pom_tuning_t* t);
+int pom_batch_get_tuning(const pom_batch* b, pom_tuning_t* t);
+
 /* Workspace bytes needed by pom_explicit_step (keep=0) and by
  * pom_explicit_step_adj. */
 int pom_explicit_workspace(const pom_batch* b, const pom_params_t* prm, int32_t adjoint, size_t* bytes);

@@ -47,6 +47,11 @@ class Member(C.Structure):
     _fields_ = [("im", C.c_int32), ("jm", C.c_int32), ("off2", C.c_int64), ("off3", C.c_int64)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("map2_blocks_per_group", "map2_min_groups", "flat2_blocks_per_group",
+                                         "flat2_min_groups", "flat3_blocks_per_group", "flat3_min_groups")]
+
+
 class Params(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "dti", "dte", "dti2", "dte2", "ispi", "isp2i", "alpha", "smoth", "horcon", "grav",
@@ -77,6 +82,8 @@ def load():
     vp, vpp = C.c_void_p, C.POINTER(C.c_void_p)
     lib.pom_batch_create.argtypes = [C.POINTER(Member), C.c_int32, C.c_int32, C.POINTER(vp)]
     lib.pom_batch_destroy.argtypes = [vp]
+    lib.pom_batch_tune.argtypes = [vp, C.POINTER(Tuning)]
+    lib.pom_batch_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
     lib.pom_explicit_workspace.argtypes = [vp, C.POINTER(Params), C.c_int32, C.POINTER(C.c_size_t)]
     lib.pom_explicit_step.argtypes = [vp, C.POINTER(Params), vp, vp, vpp, vpp, vpp, C.c_int32, vp, vp,
                                       C.c_size_t, vp]
@@ -131,7 +138,8 @@ def load():
     return lib
 
 
-EXPORTED = ("pom_batch_create", "pom_batch_destroy", "pom_explicit_workspace", "pom_explicit_step",
+EXPORTED = ("pom_batch_create", "pom_batch_destroy", "pom_batch_tune", "pom_batch_get_tuning",
+            "pom_explicit_workspace", "pom_explicit_step",
             "pom_explicit_step_adj", "pom_explicit_step_taped", "pom_explicit_step_reverse",
             "pom_implicit_cache_bytes", "pom_implicit_prepare", "pom_implicit_workspace",
             "pom_implicit_step", "pom_implicit_step_adj", "pom_implicit_step_taped", "pom_implicit_step_reverse", "pom_dens", "pom_dens_adj", "pom_baropg",

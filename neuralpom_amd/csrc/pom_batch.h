@@ -3,7 +3,7 @@
 #pragma once
 #include <vector>
 
-#include "pom_common.h"
+#include "pom_kernels.h"
 
 struct pom_batch {
   int32_t dtype;
@@ -14,4 +14,5 @@ struct pom_batch {
   int32_t nblk3, nblk2;
   int64_t n2, n3;
   std::vector<pom::Member> h_mem;   // host copy of the member table
+  pom::Tuning tune;                 // launch shapes (pom_batch_tune)
 };

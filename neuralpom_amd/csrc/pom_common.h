@@ -404,8 +404,8 @@ struct Pos {
   int64_t o2, o3;   // member base offsets
 };
 
-__device__ __forceinline__ bool pos3(const Batch& B, Pos& p) {
-  int b = logical_block();
+__device__ __forceinline__ bool pos3_at(const Batch& B, int b, Pos& p) {
+  if (b >= B.blk3[B.M]) return false;
   int m = member_of(B.blk3, B.M, b);
   Member mm = B.mem[m];
   int64_t local = (int64_t)(b - B.blk3[m]) * TPB + threadIdx.x;
@@ -418,6 +418,8 @@ __device__ __forceinline__ bool pos3(const Batch& B, Pos& p) {
   p.j = (int)(col - (int64_t)p.i * mm.jm);
   return true;
 }
+
+__device__ __forceinline__ bool pos3(const Batch& B, Pos& p) { return pos3_at(B, logical_block(), p); }
 
 __device__ __forceinline__ bool pos2_at(const Batch& B, int b, Pos& p) {
   if (b >= B.blk2[B.M]) return false;

@@ -23,12 +23,12 @@ void launch_gather(const Ctx<T>& c, const FieldTab<T>& tab, const COutTab<T>& ob
   constexpr uint32_t M2 = ALL & ~M3;
   if constexpr (Op::OUT3) {
     if (c.nblk3 == 0) return;
-    hipLaunchKernelGGL((k_gather_all3<T, Op, M3, M2>), dim3(c.nblk3), dim3(TPB), 0, c.stream, c.B, c.vert, c.ramp,
-                       c.prm, tab, obar, it);
+    hipLaunchKernelGGL((k_gather_all3<T, Op, M3, M2>), dim3(groups(c.nblk3, c.tune.flat3_bpg, c.tune.flat3_min)),
+                       dim3(TPB), 0, c.stream, c.B, c.vert, c.ramp, c.prm, tab, obar, it);
   } else {
     if (c.nblk2 == 0) return;
-    hipLaunchKernelGGL((k_gather_all2<T, Op, M2>), dim3(c.nblk2), dim3(TPB), 0, c.stream, c.B, c.vert, c.ramp,
-                       c.prm, tab, obar, it);
+    hipLaunchKernelGGL((k_gather_all2<T, Op, M2>), dim3(groups(c.nblk2, c.tune.flat2_bpg, c.tune.flat2_min)),
+                       dim3(TPB), 0, c.stream, c.B, c.vert, c.ramp, c.prm, tab, obar, it);
   }
 }
 

@@ -19,6 +19,23 @@ constexpr int MAXO = 12;
 template <class T> struct OutTab { T* o[MAXO]; };
 template <class T> struct COutTab { const T* o[MAXO]; };
 
+// Launch shapes of one batch (pom_batch_tune; the defaults are the values
+// measured by hand on the 64-member seamount batch).  A launch of nblk
+// 256-point blocks takes groups(nblk, bpg, min) workgroups, each striding
+// over the blocks by the grid size (XCD-aware order kept): bpg blocks per
+// workgroup once the launch has more than min blocks.  The arithmetic per
+// point does not depend on them.
+struct Tuning {
+  int32_t map2_bpg = 4, map2_min = 768;     // k_map2 (the barotropic substep)
+  int32_t flat2_bpg = 1, flat2_min = 768;   // 2-D k_map and k_gather_all2
+  int32_t flat3_bpg = 1, flat3_min = 768;   // 3-D k_map and k_gather_all3
+};
+inline int groups(int nblk, int bpg, int mn) {
+  if (bpg <= 1 || nblk <= mn) return nblk;
+  const int g = (nblk + bpg - 1) / bpg;
+  return g > mn ? g : mn;
+}
+
 template <class T> struct Ctx {
   Batch B;
   Vert<T> vert;
@@ -26,16 +43,19 @@ template <class T> struct Ctx {
   Params prm;
   int32_t nblk3, nblk2;
   hipStream_t stream;
+  Tuning tune;
 };
 
 template <class T, class Op>
 __global__ void __launch_bounds__(TPB) k_map(Batch B, Vert<T> vert, const T* ramp, Params prm, FieldTab<T> tab,
                                               OutTab<T> out) {
+  const int nblk = Op::OUT3 ? B.blk3[B.M] : B.blk2[B.M];
+  for (int blk = logical_block(); blk < nblk; blk += (int)gridDim.x) {
   Pos p;
   if constexpr (Op::OUT3) {
-    if (!pos3(B, p)) return;
+    if (!pos3_at(B, blk, p)) continue;
   } else {
-    if (!pos2(B, p)) return;
+    if (!pos2_at(B, blk, p)) continue;
   }
   Acc<T> A{&tab, p, vert, ramp};
   const int64_t idx = Op::OUT3 ? ix3(p, p.i, p.j, p.k) : ix2(p, p.i, p.j);
@@ -51,6 +71,7 @@ __global__ void __launch_bounds__(TPB) k_map(Batch B, Vert<T> vert, const T* ram
     constexpr int o = decltype(O)::value;
     if (out.o[o]) out.o[o][idx] = res[o];
   });
+  }
 }
 
 // ---- running max of |x| over a launch (the depth-mean velocity check).  NaN
@@ -75,20 +96,14 @@ template <class T> __device__ __forceinline__ void block_amax(T v, T* out) {
   }
 }
 
-// Workgroups of k_map2.  One workgroup per 256-point block would give the
-// seamount batch 3 051 workgroups for 1 792 resident slots (7 waves per SIMD),
-// one block-max atomic each and a last, partly filled wave of workgroups.  A
-// large launch instead takes a quarter as many workgroups, each striding over
-// the blocks by the grid size, so that every XCD still sweeps one contiguous
-// range of blocks at a time (measured per launch: 36.2 us with one block per
-// workgroup, 30.1 us with four, 32.0 with six; the HBM fetch fell slightly).
-constexpr int MAP2_BLOCKS_PER_GROUP = 4;
-constexpr int MAP2_MIN_GROUPS = 768;
-inline int map2_groups(int nblk2) {
-  if (nblk2 <= MAP2_MIN_GROUPS) return nblk2;
-  const int g = (nblk2 + MAP2_BLOCKS_PER_GROUP - 1) / MAP2_BLOCKS_PER_GROUP;
-  return g > MAP2_MIN_GROUPS ? g : MAP2_MIN_GROUPS;
-}
+// Workgroups of k_map2 (Tuning::map2_*).  One workgroup per 256-point block
+// would give the seamount batch 3 051 workgroups for 1 792 resident slots
+// (7 waves per SIMD), one block-max atomic each and a last, partly filled wave
+// of workgroups.  A large launch instead takes a quarter as many workgroups,
+// each striding over the blocks by the grid size, so that every XCD still
+// sweeps one contiguous range of blocks at a time (measured per launch:
+// 36.2 us with one block per workgroup, 30.1 us with four, 32.0 with six; the
+// HBM fetch fell slightly).
 
 // Two 2-D operators in one pass over the points: B reads A's outputs 0 and 1
 // at the point itself as its fields FB0 and FB1 (from registers, through
@@ -211,8 +226,10 @@ template <class Op> __device__ __forceinline__ bool deep_point(const Pos& q) {
 template <class T, class Op, uint32_t M3, uint32_t M2>
 __global__ void __launch_bounds__(TPB) k_gather_all3(Batch B, Vert<T> vert, const T* ramp, Params prm,
                                                       FieldTab<T> tab, COutTab<T> obar, InTab<T> inbar) {
+  const int nblk = B.blk3[B.M];
+  for (int blk = logical_block(); blk < nblk; blk += (int)gridDim.x) {   // uniform over the workgroup
   Pos q;
-  const bool live = pos3(B, q);
+  const bool live = pos3_at(B, blk, q);
   T acc[Op::NIN];
 #pragma unroll
   for (int f = 0; f < Op::NIN; ++f) acc[f] = T(0);
@@ -249,14 +266,17 @@ __global__ void __launch_bounds__(TPB) k_gather_all3(Batch B, Vert<T> vert, cons
       if (live && q.k == 0 && inbar.o[f]) put_in(inbar, f, ix2(q, q.i, q.j), v);
     }
   });
+  }
 }
 
 // 2-D operators: one thread per column
 template <class T, class Op, uint32_t M2>
 __global__ void __launch_bounds__(TPB) k_gather_all2(Batch B, Vert<T> vert, const T* ramp, Params prm,
                                                       FieldTab<T> tab, COutTab<T> obar, InTab<T> inbar) {
+  const int nblk = B.blk2[B.M];
+  for (int blk = logical_block(); blk < nblk; blk += (int)gridDim.x) {
   Pos q;
-  if (!pos2(B, q)) return;
+  if (!pos2_at(B, blk, q)) continue;
   T acc[Op::NIN];
 #pragma unroll
   for (int f = 0; f < Op::NIN; ++f) acc[f] = T(0);
@@ -276,6 +296,7 @@ __global__ void __launch_bounds__(TPB) k_gather_all2(Batch B, Vert<T> vert, cons
     constexpr int f = decltype(Fc)::value;
     if (inbar.o[f]) put_in(inbar, f, ix2(q, q.i, q.j), acc[f]);
   });
+  }
 }
 
 // ------------------------------------------------------------ host launchers
@@ -283,14 +304,15 @@ template <class T, class Op>
 inline void launch_map(const Ctx<T>& c, const FieldTab<T>& tab, const OutTab<T>& out) {
   const int nb = Op::OUT3 ? c.nblk3 : c.nblk2;
   if (nb == 0) return;
-  hipLaunchKernelGGL((k_map<T, Op>), dim3(nb), dim3(TPB), 0, c.stream, c.B, c.vert, c.ramp, c.prm, tab, out);
+  const int g = Op::OUT3 ? groups(nb, c.tune.flat3_bpg, c.tune.flat3_min) : groups(nb, c.tune.flat2_bpg, c.tune.flat2_min);
+  hipLaunchKernelGGL((k_map<T, Op>), dim3(g), dim3(TPB), 0, c.stream, c.B, c.vert, c.ramp, c.prm, tab, out);
 }
 
 template <class T, class OpA, class OpB, int FB0, int FB1>
 inline void launch_map2(const Ctx<T>& c, const FieldTab<T>& ta, const OutTab<T>& oa, const FieldTab<T>& tb,
                         const OutTab<T>& ob, T* amax) {
   if (c.nblk2 == 0) return;
-  hipLaunchKernelGGL((k_map2<T, OpA, OpB, FB0, FB1>), dim3(map2_groups(c.nblk2)), dim3(TPB), 0, c.stream, c.B, c.vert, c.ramp,
+  hipLaunchKernelGGL((k_map2<T, OpA, OpB, FB0, FB1>), dim3(groups(c.nblk2, c.tune.map2_bpg, c.tune.map2_min)), dim3(TPB), 0, c.stream, c.B, c.vert, c.ramp,
                      c.prm, ta, oa, tb, ob, amax);
 }
 

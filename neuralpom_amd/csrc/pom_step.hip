@@ -1581,6 +1581,7 @@ template <class T> Ctx<T> make_ctx(const pom_batch* b, const pom_params_t* p, co
   c.nblk3 = b->nblk3;
   c.nblk2 = b->nblk2;
   c.stream = st;
+  c.tune = b->tune;
   return c;
 }
 
@@ -2380,6 +2381,26 @@ int pom_batch_create(const pom_member_t* members, int32_t n, int32_t dtype, pom_
     return POM_E_HIP;
   }
   *out = b;
+  return POM_OK;
+}
+
+int pom_batch_tune(pom_batch* b, const pom_tuning_t* t) {
+  if (!b || !t) return POM_E_ARG;
+  const int32_t v[6] = {t->map2_blocks_per_group, t->map2_min_groups, t->flat2_blocks_per_group,
+                        t->flat2_min_groups, t->flat3_blocks_per_group, t->flat3_min_groups};
+  for (int q = 0; q < 6; ++q)
+    if (v[q] < 1 || v[q] > (1 << 20)) return POM_E_ARG;
+  b->tune.map2_bpg = v[0]; b->tune.map2_min = v[1];
+  b->tune.flat2_bpg = v[2]; b->tune.flat2_min = v[3];
+  b->tune.flat3_bpg = v[4]; b->tune.flat3_min = v[5];
+  return POM_OK;
+}
+
+int pom_batch_get_tuning(const pom_batch* b, pom_tuning_t* t) {
+  if (!b || !t) return POM_E_ARG;
+  t->map2_blocks_per_group = b->tune.map2_bpg; t->map2_min_groups = b->tune.map2_min;
+  t->flat2_blocks_per_group = b->tune.flat2_bpg; t->flat2_min_groups = b->tune.flat2_min;
+  t->flat3_blocks_per_group = b->tune.flat3_bpg; t->flat3_min_groups = b->tune.flat3_min;
   return POM_OK;
 }
 

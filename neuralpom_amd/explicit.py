@@ -102,6 +102,16 @@ class Ensemble:
             L.check(self.lib.pom_batch_create(mem, len(self.shapes), code, C.byref(h)), "pom_batch_create")
         self._h = h
         self._ws = {}
+        # launch shapes of the nearest tuned workload (tuning.json)
+        from . import tuning as TU
+        self.tuning_entry, shapes_ = TU.select(self.shapes, "f64" if dtype == torch.float64 else "f32")
+        TU.apply(self.lib, self._h, shapes_)
+
+    def set_tuning(self, shapes: dict, entry: str = "manual"):
+        """Override the launch shapes (keys of tuning.KEYS)."""
+        from . import tuning as TU
+        TU.apply(self.lib, self._h, shapes)
+        self.tuning_entry = entry
 
     def __del__(self):
         h = getattr(self, "_h", None)
