@@ -174,6 +174,8 @@ int pom_explicit_step(const pom_batch* b, const pom_params_t* prm, const void* r
  *   grad_in[f] += (d state_out / d state_in[f])^T cot_out
  * for every field f with a non-null grad_in[f] (zero-initialised by the caller
  * or holding a running sum).  Null cot_out[f] entries are zero cotangents.
+ * The cot_out arrays are read only and are read until the call's last
+ * kernel: they must not overlap any grad_in array.
  * Differentiable inputs: u ub v vb t tb s sb q2 q2b q2l q2lb km kh kq aam rho
  * el elb ua uab va vab d dt et etb egb utf utb vtf vtb wubot wvbot. */
 int pom_explicit_step_adj(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vgrid,

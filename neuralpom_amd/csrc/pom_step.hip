@@ -1083,7 +1083,15 @@ template <class T> struct Step {
 template <class T> struct Adj {
   Step<T>& S;
   Ctx<T>& c;
-  T* bo[POM_NSTATE];    // cotangents of step outputs (accumulated)
+  // Cotangents of the step outputs, read through bo.  An output that the step
+  // itself reads later (T, S, km, kh, aam, w, the advection and density
+  // gradient terms) collects those uses' cotangents as well: it has an
+  // accumulator in bw, filled with the caller's cotangent first.  Every other
+  // output's cotangent is read from the caller's array as it is, or from a
+  // shared zero array when the caller passes none.
+  const T* bo[POM_NSTATE];
+  T* bw[POM_NSTATE];
+  T *zero3, *zero2;
   T* bi[POM_NSTATE];    // cotangents of step inputs (accumulated)
   // intermediates
   T *b_rho1, *b_adx2d_raw, *b_ady2d_raw, *b_advua0, *b_advva0, *b_egf0, *b_utf0, *b_vtf0;
@@ -1099,13 +1107,32 @@ template <class T> struct Adj {
 
   Adj(Step<T>& s, Ctx<T>& cc) : S(s), c(cc) {}
 
+  static constexpr bool accumulated(int f) {
+    switch (f) {
+      case POM_T: case POM_S: case POM_KM: case POM_KH: case POM_AAM: case POM_W: case POM_ADVX: case POM_ADVY:
+      case POM_DRHOX: case POM_DRHOY: case POM_ADX2D: case POM_ADY2D: case POM_DRX2D: case POM_DRY2D:
+      case POM_AAM2D:
+        return true;
+      default:
+        return false;
+    }
+  }
+  // the accumulator of output F: the only writable view of an output cotangent
+  template <int F> T* w() const {
+    static_assert(accumulated(F), "this output's cotangent has no accumulator");
+    return bw[F];
+  }
+
   void plan(Arena& A, T* const* grad_in) {
     const int64_t N3 = S.n3(), N2 = S.n2();
     for (int f = 0; f < POM_NSTATE; ++f) {
       const int64_t n = f < POM_N3D ? N3 : N2;
-      bo[f] = A.get<T>(n);
+      bw[f] = accumulated(f) ? A.get<T>(n) : nullptr;
+      bo[f] = bw[f];
       bi[f] = grad_in && grad_in[f] ? grad_in[f] : A.get<T>(n);
     }
+    zero3 = A.get<T>(N3);
+    zero2 = A.get<T>(N2);
     b_rho1 = A.get<T>(N3);
     b_adx2d_raw = A.get<T>(N2);
     b_ady2d_raw = A.get<T>(N2);
@@ -1178,11 +1205,18 @@ template <class T> struct Adj {
   void run(const T* const* cot_out, T* const* grad_in) {
     const int64_t N3 = S.n3(), N2 = S.n2();
     const dim3 g2(c.nblk2), tb(TPB);
-    // output cotangents (copied: they are accumulated into below)
+    // output cotangents: copied where they are accumulated into below, read in place elsewhere
+    zq(zero3, N3);
+    zq(zero2, N2);
     for (int f = 0; f < POM_NSTATE; ++f) {
       const int64_t n = f < POM_N3D ? N3 : N2;
-      if (cot_out && cot_out[f]) S.copy(bo[f], cot_out[f], n);
-      else zq(bo[f], n);
+      const T* given = cot_out ? cot_out[f] : nullptr;
+      if (bw[f]) {
+        if (given) S.copy(bw[f], given, n);
+        else zq(bw[f], n);
+      } else {
+        bo[f] = given ? given : (f < POM_N3D ? zero3 : zero2);
+      }
       if (!(grad_in && grad_in[f])) zq(bi[f], n);
     }
     for (T* p : {b_rho1, b_u1, b_v1, b_q2f0, b_q2lf0, b_q2f1, b_q2lf1, b_kmraw, b_khraw, b_tf0, b_sf0, b_tf1, b_sf1,
@@ -1250,14 +1284,14 @@ template <class T> struct Adj {
         {
           COutTab<T> ob{{bn.uaf, bn.vaf}};
           T* ib[OpExtUV::NIN] = {bp.el, bp.elb, bn.elf, bp.d, bp.ua, bp.va, bp.uab, bp.vab, bn.advua, bn.advva,
-                                 bo[POM_ADX2D], bo[POM_ADY2D], bo[POM_DRX2D], bo[POM_DRY2D], bi[POM_WUBOT],
+                                 w<POM_ADX2D>(), w<POM_ADY2D>(), w<POM_DRX2D>(), w<POM_DRY2D>(), bi[POM_WUBOT],
                                  bi[POM_WVBOT], bp.uaf, bp.vaf};
           const uint32_t as = (1u << OpExtUV::D_) | (1u << OpExtUV::UAFIN) | (1u << OpExtUV::VAFIN);
           launch_gather<T, OpExtUV>(cn, S.t_uv(xp, xn), ob, ib, as);
         }
         if (n % c.prm.ispadv == 0) {   // advave produced this substep's advua / advva
           COutTab<T> ob{{bn.advua, bn.advva, nullptr, nullptr, nullptr}};
-          T* ib[OpAdvave::NIN] = {bp.ua, bp.va, bp.uab, bp.vab, bp.d, bo[POM_AAM2D]};
+          T* ib[OpAdvave::NIN] = {bp.ua, bp.va, bp.uab, bp.vab, bp.d, w<POM_AAM2D>()};
           launch_gather<T, OpAdvave>(cn, S.t_advave(xp), ob, ib);
           zq(bn.advua, N2);   // the earlier value's cotangent starts from zero
           zq(bn.advva, N2);
@@ -1297,16 +1331,16 @@ template <class T> struct Adj {
       xin.fluxva = (T*)S.in[POM_FLUXVA];
       xin.tps = (T*)S.in[POM_TPS];
       COutTab<T> ob{{b_advua0, b_advva0, nullptr, nullptr, nullptr}};
-      T* ib[OpAdvave::NIN] = {bi[POM_UA], bi[POM_VA], bi[POM_UAB], bi[POM_VAB], bi[POM_D], bo[POM_AAM2D]};
+      T* ib[OpAdvave::NIN] = {bi[POM_UA], bi[POM_VA], bi[POM_UAB], bi[POM_VAB], bi[POM_D], w<POM_AAM2D>()};
       launch_gather<T, OpAdvave>(c, S.t_advave(xin), ob, ib);
     }
     {
       ColsumAdjArgs<T> a{};
-      a.bout[0] = b_adx2d_raw; a.bf[0] = bo[POM_ADVX];
-      a.bout[1] = b_ady2d_raw; a.bf[1] = bo[POM_ADVY];
-      a.bout[2] = bo[POM_DRX2D]; a.bf[2] = bo[POM_DRHOX];
-      a.bout[3] = bo[POM_DRY2D]; a.bf[3] = bo[POM_DRHOY];
-      a.bout[4] = bo[POM_AAM2D]; a.bf[4] = bo[POM_AAM];
+      a.bout[0] = b_adx2d_raw; a.bf[0] = w<POM_ADVX>();
+      a.bout[1] = b_ady2d_raw; a.bf[1] = w<POM_ADVY>();
+      a.bout[2] = bo[POM_DRX2D]; a.bf[2] = w<POM_DRHOX>();
+      a.bout[3] = bo[POM_DRY2D]; a.bf[3] = w<POM_DRHOY>();
+      a.bout[4] = bo[POM_AAM2D]; a.bf[4] = w<POM_AAM>();
       a.n = 5;
       hipLaunchKernelGGL(k_colsum_adj<T>, g2, tb, 0, c.stream, c.B, c.vert, a);
     }
@@ -1386,7 +1420,7 @@ template <class T> struct Adj {
     {   // predictor
       COutTab<T> ob{{S.open ? b_us0 : b_us, S.open ? b_vs0 : b_vs, b_dus, b_dvs}};
       T* ib[OpImpPred::NIN] = {bi[POM_UA], bi[POM_VA], bi[POM_EL], bi[POM_ELB], bi[POM_D], bi[POM_UAB], bi[POM_VAB],
-                               bo[POM_ADX2D], bo[POM_ADY2D], b_advua0, b_advva0, bo[POM_DRX2D], bo[POM_DRY2D],
+                               w<POM_ADX2D>(), w<POM_ADY2D>(), b_advua0, b_advva0, w<POM_DRX2D>(), w<POM_DRY2D>(),
                                bi[POM_WUBOT], bi[POM_WVBOT]};
       launch_gather<T, OpImpPred>(c, S.t_pred(), ob, ib);
     }
@@ -1438,7 +1472,7 @@ template <class T> struct Adj {
       a.s_km0 = s3a; a.s_km1 = s3b; a.s_etf0 = s2a; a.s_etf1 = s2b; a.s_mag = s2c;
       a.b_wbot_in = bi[dir ? POM_WVBOT : POM_WUBOT];
       ProfuvAdj2Args<T> b2{s.in[POM_UB], s.in[POM_VB], s.g[POM_G_CBC], s3a, s3b, s2a, s2b, s2c,
-                           bo[POM_KM], b_etf30, bi[POM_UB], bi[POM_VB]};
+                           w<POM_KM>(), b_etf30, bi[POM_UB], bi[POM_VB]};
       if (dir == 0) {
         hipLaunchKernelGGL((k_profuv_adj1<T, 0>), g2, tb, 0, c.stream, c.B, c.vert, c.prm, a);
         hipLaunchKernelGGL((k_profuv_adj2<T, 0>), g2, tb, 0, c.stream, c.B, c.vert, b2);
@@ -1449,23 +1483,23 @@ template <class T> struct Adj {
     }
     {   // advu / advv
       COutTab<T> ob{{b_uf0}};
-      T* ib[OpAdvu::NIN] = {bo[POM_ADVX], bo[POM_DRHOX], b_u1, b_v1, bo[POM_W], bi[POM_UB], bi[POM_DT],
+      T* ib[OpAdvu::NIN] = {w<POM_ADVX>(), w<POM_DRHOX>(), b_u1, b_v1, w<POM_W>(), bi[POM_UB], bi[POM_DT],
                             b_egf30, bi[POM_EGB], b_etf30, bi[POM_ETB]};
       launch_gather<T, OpAdvu>(c, s.t_advu(), ob, ib);
       COutTab<T> ov{{b_vf0}};
-      T* iv[OpAdvv::NIN] = {bo[POM_ADVY], bo[POM_DRHOY], b_u1, b_v1, bo[POM_W], bi[POM_VB], bi[POM_DT],
+      T* iv[OpAdvv::NIN] = {w<POM_ADVY>(), w<POM_DRHOY>(), b_u1, b_v1, w<POM_W>(), bi[POM_VB], bi[POM_DT],
                             b_egf30, bi[POM_EGB], b_etf30, bi[POM_ETB]};
       launch_gather<T, OpAdvv>(c, s.t_advv(), ov, iv);
     }
     {   // dens
       COutTab<T> ob{{bo[POM_RHO]}};
-      T* ib[OpDens::NIN] = {bo[POM_T], bo[POM_S], b_rho1};
+      T* ib[OpDens::NIN] = {w<POM_T>(), w<POM_S>(), b_rho1};
       launch_gather<T, OpDens>(c, s.t_dens(), ob, ib);
     }
     if (s.open) {   // open bcond 4 with exterior values + T/S filter
       COutTab<T> ob{{bo[POM_T], bo[POM_TB], bo[POM_S], bo[POM_SB]}};
       T* ib[OpBc4FilterExt::NIN] = {b_tf1, b_sf1, bi[POM_T], bi[POM_S], bi[POM_TB], bi[POM_SB], b_u1, b_v1,
-                                    bo[POM_W], nullptr, nullptr, bi[POM_DT]};
+                                    w<POM_W>(), nullptr, nullptr, bi[POM_DT]};
       launch_gather<T, OpBc4FilterExt>(c, s.t_bc4ext(), ob, ib);
     } else if (s.imp && s.shap_ts) {   // Shapiro filter of the new T, S, then bcond 4 + T/S filter
       COutTab<T> ot{{bo[POM_T]}};
@@ -1484,7 +1518,7 @@ template <class T> struct Adj {
     } else {   // bcond 4 + T/S filter
       COutTab<T> ob{{bo[POM_T], bo[POM_TB], bo[POM_S], bo[POM_SB]}};
       T* ib[OpBc4Filter::NIN] = {b_tf1, b_sf1, bi[POM_T], bi[POM_S], bi[POM_TB], bi[POM_SB], b_u1, b_v1,
-                                 bo[POM_W], bi[POM_DT]};
+                                 w<POM_W>(), bi[POM_DT]};
       launch_gather<T, OpBc4Filter>(c, s.t_bc4(), ob, ib);
     }
     for (int sal = 0; sal < 2; ++sal) {   // proft
@@ -1493,15 +1527,15 @@ template <class T> struct Adj {
       a.f = f.f; a.kh = f.kh; a.etf = f.etf; a.h = f.h; a.wfsurf = f.wfsurf; a.fsurf = f.fsurf;
       a.b_out = sal ? b_sf1 : b_tf1;
       a.b_f = sal ? b_sf0 : b_tf0;
-      a.b_kh = bo[POM_KH];
+      a.b_kh = w<POM_KH>();
       a.b_etf = b_etf30;
       a.nbc = f.nbc;
       hipLaunchKernelGGL(k_proft_adj<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a);
     }
     for (int sal = 0; sal < 2; ++sal) {   // advt1
       COutTab<T> ob{{sal ? b_sf0 : b_tf0}};
-      T* ib[OpAdvt1::NIN] = {bi[sal ? POM_SB : POM_TB], bi[sal ? POM_S : POM_T], nullptr, b_u1, b_v1, bo[POM_W],
-                             bo[POM_AAM], bi[POM_DT], bi[POM_ETB], b_etf30};
+      T* ib[OpAdvt1::NIN] = {bi[sal ? POM_SB : POM_TB], bi[sal ? POM_S : POM_T], nullptr, b_u1, b_v1, w<POM_W>(),
+                             w<POM_AAM>(), bi[POM_DT], bi[POM_ETB], b_etf30};
       launch_flux_advt1<T>(c, s.t_advt1(sal == 1), ob, ib);
     }
     if (s.imp && !s.open) {   // closed-basin bcond 6 + q2 filter
@@ -1541,8 +1575,8 @@ template <class T> struct Adj {
     }
     for (int l2 = 0; l2 < 2; ++l2) {   // advq
       COutTab<T> ob{{l2 ? b_q2lf0 : b_q2f0}};
-      T* ib[OpAdvq::NIN] = {bi[l2 ? POM_Q2LB : POM_Q2B], bi[l2 ? POM_Q2L : POM_Q2], b_u1, b_v1, bo[POM_W],
-                            bo[POM_AAM], bi[POM_DT], bi[POM_ETB], b_etf30};
+      T* ib[OpAdvq::NIN] = {bi[l2 ? POM_Q2LB : POM_Q2B], bi[l2 ? POM_Q2L : POM_Q2], b_u1, b_v1, w<POM_W>(),
+                            w<POM_AAM>(), bi[POM_DT], bi[POM_ETB], b_etf30};
       launch_flux_advq<T>(c, s.t_advq(l2 == 1), ob, ib);
     }
     {   // vertvl (w input is not differentiated)

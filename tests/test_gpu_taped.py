@@ -2,7 +2,8 @@
 the taped forward equals the plain step and the reverse from its kept
 intermediates equals the recomputing adjoint, bitwise, on a ragged batch.
 The taped outputs are part of the tape: the reverse reads them and leaves
-them as they were."""
+them as they were, and so it does with the output cotangents (read in place
+where the step does not add into them)."""
 import dataclasses
 
 import pytest
@@ -27,7 +28,8 @@ def test_explicit_taped_and_reverse_equal_step_and_adjoint(dtype):
     ens, state, geom, vg = ensemble_inputs(states, dtype)
     sc = scalars_of(states[0])
     ramp = torch.tensor([ramp_of(S, 1) for S in states], dtype=dtype, device="cuda")
-    cot = _cot(ens, ("u", "v", "t", "s", "el", "ua", "va", "q2", "km"), 3, dtype)
+    cot = _cot(ens, L.FIELDS, 3, dtype)
+    cot0 = {n: c.clone() for n, c in cot.items()}
     out_s, va_s = ens.step(state, geom, sc, ramp, vg, True)
     grads_a = ens.step_adj(state, geom, sc, ramp, vg, True, dict(cot))
     out_t, va_t = ens.step_taped(state, geom, sc, ramp, vg, True)
@@ -39,8 +41,9 @@ def test_explicit_taped_and_reverse_equal_step_and_adjoint(dtype):
     bad = [n for n in L.DIFF_INPUTS if not torch.equal(grads_r[n], grads_a[n])]
     assert not bad, bad
     assert float(grads_r["t"].abs().max()) > 0.0
-    # the reverse reads the taped outputs and leaves them as they were
+    # the reverse reads the taped outputs and the output cotangents and leaves them as they were
     assert not [n for n in L.FIELDS if not torch.equal(out_t[n], out_s[n])]
+    assert not [n for n in L.FIELDS if not torch.equal(cot[n], cot0[n])]
 
 
 def test_implicit_taped_and_reverse_equal_step_and_adjoint():
