@@ -324,12 +324,14 @@ void launch_gather(const Ctx<T>& c, const FieldTab<T>& tab, const COutTab<T>& ob
                    uint32_t assign = 0u);
 
 // the same products for OpAdvt1 and OpAdvq by their flux-form transposed
-// stencils (pom_flux.h; instantiated in flux_adv.hip)
+// stencils (pom_flux.h; instantiated in flux_adv.hip), for a pair of fields
+// (T and S; q2 and q2l): tab, obar and inbar hold two entries each, and one
+// launch serves both when they share the velocity, aam, w and depth cotangents
 template <class T>
-void launch_flux_advt1(const Ctx<T>& c, const FieldTab<T>& tab, const COutTab<T>& obar, T* const* inbar,
+void launch_flux_advt1(const Ctx<T>& c, const FieldTab<T>* tab, const COutTab<T>* obar, T* const* const* inbar,
                        uint32_t assign = 0u);
 template <class T>
-void launch_flux_advq(const Ctx<T>& c, const FieldTab<T>& tab, const COutTab<T>& obar, T* const* inbar,
+void launch_flux_advq(const Ctx<T>& c, const FieldTab<T>* tab, const COutTab<T>* obar, T* const* const* inbar,
                       uint32_t assign = 0u);
 
 }  // namespace pom

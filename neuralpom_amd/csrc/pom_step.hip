@@ -1532,11 +1532,15 @@ template <class T> struct Adj {
       a.nbc = f.nbc;
       hipLaunchKernelGGL(k_proft_adj<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a);
     }
-    for (int sal = 0; sal < 2; ++sal) {   // advt1
-      COutTab<T> ob{{sal ? b_sf0 : b_tf0}};
-      T* ib[OpAdvt1::NIN] = {bi[sal ? POM_SB : POM_TB], bi[sal ? POM_S : POM_T], nullptr, b_u1, b_v1, w<POM_W>(),
-                             w<POM_AAM>(), bi[POM_DT], bi[POM_ETB], b_etf30};
-      launch_flux_advt1<T>(c, s.t_advt1(sal == 1), ob, ib);
+    {   // advt1 of T and S
+      const FieldTab<T> tab[2] = {s.t_advt1(false), s.t_advt1(true)};
+      const COutTab<T> ob[2] = {{{b_tf0}}, {{b_sf0}}};
+      T* ibt[OpAdvt1::NIN] = {bi[POM_TB], bi[POM_T], nullptr, b_u1, b_v1, w<POM_W>(), w<POM_AAM>(), bi[POM_DT],
+                              bi[POM_ETB], b_etf30};
+      T* ibs[OpAdvt1::NIN] = {bi[POM_SB], bi[POM_S], nullptr, b_u1, b_v1, w<POM_W>(), w<POM_AAM>(), bi[POM_DT],
+                              bi[POM_ETB], b_etf30};
+      T* const* ib[2] = {ibt, ibs};
+      launch_flux_advt1<T>(c, tab, ob, ib);
     }
     if (s.imp && !s.open) {   // closed-basin bcond 6 + q2 filter
       COutTab<T> ob{{bo[POM_Q2], bo[POM_Q2B], bo[POM_Q2L], bo[POM_Q2LB]}};
@@ -1573,11 +1577,15 @@ template <class T> struct Adj {
       }
       hipLaunchKernelGGL(k_profq_adj2<T>, dim3(c.nblk3), tb, 0, c.stream, c.B, a);
     }
-    for (int l2 = 0; l2 < 2; ++l2) {   // advq
-      COutTab<T> ob{{l2 ? b_q2lf0 : b_q2f0}};
-      T* ib[OpAdvq::NIN] = {bi[l2 ? POM_Q2LB : POM_Q2B], bi[l2 ? POM_Q2L : POM_Q2], b_u1, b_v1, w<POM_W>(),
-                            w<POM_AAM>(), bi[POM_DT], bi[POM_ETB], b_etf30};
-      launch_flux_advq<T>(c, s.t_advq(l2 == 1), ob, ib);
+    {   // advq of q2 and q2l
+      const FieldTab<T> tab[2] = {s.t_advq(false), s.t_advq(true)};
+      const COutTab<T> ob[2] = {{{b_q2f0}}, {{b_q2lf0}}};
+      T* ibq[OpAdvq::NIN] = {bi[POM_Q2B], bi[POM_Q2], b_u1, b_v1, w<POM_W>(), w<POM_AAM>(), bi[POM_DT], bi[POM_ETB],
+                             b_etf30};
+      T* ibl[OpAdvq::NIN] = {bi[POM_Q2LB], bi[POM_Q2L], b_u1, b_v1, w<POM_W>(), w<POM_AAM>(), bi[POM_DT],
+                             bi[POM_ETB], b_etf30};
+      T* const* ib[2] = {ibq, ibl};
+      launch_flux_advq<T>(c, tab, ob, ib);
     }
     {   // vertvl (w input is not differentiated)
       VertvlAdjArgs<T> a{s.u1, s.v1, s.in[POM_DT], s.g[POM_G_DX], s.g[POM_G_DY], s.g[POM_G_FSM], bo[POM_W], s3a,
