@@ -19,6 +19,7 @@
 #include "pom_ops_imp.h"
 #include "pom_pcg.h"
 #include "pom_profq.h"
+#include "pom_extrev.h"
 
 #include "pom_batch.h"
 
@@ -1147,11 +1148,18 @@ template <class T> struct Adj {
     // that produced the value consumes it and clears it
     T* b_advua_c = A.get<T>(N2);
     T* b_advva_c = A.get<T>(N2);
+    // egf, utf, vtf and etf: a substep passes their cotangents on unchanged
+    // (etf's is scaled in place on the last three), so the two substep
+    // buffers share one array each
+    T* b_egf_c = A.get<T>(N2);
+    T* b_utf_c = A.get<T>(N2);
+    T* b_vtf_c = A.get<T>(N2);
+    T* b_etf_c = A.get<T>(N2);
     for (int q = 0; q < 2; ++q) {
       X<T>& x = bx[q];
       x.ua = A.get<T>(N2); x.uab = A.get<T>(N2); x.va = A.get<T>(N2); x.vab = A.get<T>(N2);
-      x.el = A.get<T>(N2); x.elb = A.get<T>(N2); x.d = A.get<T>(N2); x.egf = A.get<T>(N2);
-      x.utf = A.get<T>(N2); x.vtf = A.get<T>(N2); x.etf = A.get<T>(N2); x.advua = b_advua_c;
+      x.el = A.get<T>(N2); x.elb = A.get<T>(N2); x.d = A.get<T>(N2); x.egf = b_egf_c;
+      x.utf = b_utf_c; x.vtf = b_vtf_c; x.etf = b_etf_c; x.advua = b_advua_c;
       x.advva = b_advva_c; x.uaf = A.get<T>(N2); x.vaf = A.get<T>(N2); x.elf = A.get<T>(N2);
       x.fluxua = x.fluxva = x.tps = nullptr;
     }
@@ -1258,36 +1266,32 @@ template <class T> struct Adj {
       for (int n = S.isplit; n >= 1; --n) {
         X<T>& bn = bx[n & 1];
         X<T>& bp = bx[(n - 1) & 1];
-        // No zeroing of bp: the first gather to write each of its buffers in
-        // this substep assigns instead of accumulating (ExtFilter: ua uab va
-        // vab el elb egf utf vtf etf; ExtUV: d uaf vaf), bp.elf is assigned
-        // by the next substep's ExtFilter (first = n - 1 < isplit), and the
-        // shared advua / advva accumulator is consumed and cleared below.
+        // No zeroing of bp: k_ext_rev_uv assigns every buffer of bp it
+        // writes (ua uab va vab el elb d uaf vaf) and completes bn.elf
+        // (assigned for n < isplit, where nothing else wrote it in this pass);
+        // egf utf vtf etf are shared by both buffers, and the shared advua /
+        // advva accumulator is consumed and cleared below.
         const X<T>& xp = n == 1 ? S.x0_ : S.xs[n - 1];
         const X<T>& xn = S.xs[n];
         Ctx<T> cn = c;
         cn.prm.iext = n;
-        const bool first_elf = n < S.isplit;   // bn.elf not written yet in this pass
-        // filter: outputs (ua uab va vab el elb d egf utf vtf etf)
-        {
-          COutTab<T> ob{{bn.ua, bn.uab, bn.va, bn.vab, bn.el, bn.elb, bn.d, bn.egf, bn.utf, bn.vtf, bn.etf}};
-          T* ib[OpExtFilter::NIN] = {bp.ua, bp.uab, bp.va, bp.vab, bp.el, bp.elb, bn.uaf, bn.vaf, bn.elf,
-                                     bp.egf, bp.utf, bp.vtf, bp.etf};
-          uint32_t as = 0u;
-          for (int f : {OpExtFilter::UA, OpExtFilter::UAB, OpExtFilter::VA, OpExtFilter::VAB, OpExtFilter::EL,
-                        OpExtFilter::ELB, OpExtFilter::EGF, OpExtFilter::UTF, OpExtFilter::VTF, OpExtFilter::ETF})
-            as |= 1u << f;
-          if (first_elf) as |= 1u << OpExtFilter::ELF;
-          launch_gather<T, OpExtFilter>(cn, S.t_filter(xp, xn), ob, ib, as);
-        }
-        // depth-mean momentum
-        {
-          COutTab<T> ob{{bn.uaf, bn.vaf}};
-          T* ib[OpExtUV::NIN] = {bp.el, bp.elb, bn.elf, bp.d, bp.ua, bp.va, bp.uab, bp.vab, bn.advua, bn.advva,
-                                 w<POM_ADX2D>(), w<POM_ADY2D>(), w<POM_DRX2D>(), w<POM_DRY2D>(), bi[POM_WUBOT],
-                                 bi[POM_WVBOT], bp.uaf, bp.vaf};
-          const uint32_t as = (1u << OpExtUV::D_) | (1u << OpExtUV::UAFIN) | (1u << OpExtUV::VAFIN);
-          launch_gather<T, OpExtUV>(cn, S.t_uv(xp, xn), ob, ib, as);
+        {   // filter and depth-mean momentum (with bcond 2)
+          ExtRevArgs<T> a{};
+          a.el = xp.el; a.elb = xp.elb; a.d = xp.d; a.ua = xp.ua; a.va = xp.va; a.uab = xp.uab; a.vab = xp.vab;
+          a.elf = xn.elf; a.uaf = xn.uaf; a.vaf = xn.vaf;
+          a.cor = S.g[POM_G_COR]; a.dx = S.g[POM_G_DX]; a.dy = S.g[POM_G_DY]; a.h = S.g[POM_G_H];
+          a.aru = S.g[POM_G_ARU]; a.arv = S.g[POM_G_ARV]; a.dum = S.g[POM_G_DUM]; a.dvm = S.g[POM_G_DVM];
+          a.eatm = S.out[POM_E_ATMOS]; a.fsm = S.g[POM_G_FSM];
+          a.c_ua = bn.ua; a.c_uab = bn.uab; a.c_va = bn.va; a.c_vab = bn.vab; a.c_el = bn.el; a.c_elb = bn.elb;
+          a.c_d = bn.d; a.c_egf = bn.egf; a.c_utf = bn.utf; a.c_vtf = bn.vtf; a.c_uaf = bn.uaf; a.c_vaf = bn.vaf;
+          a.c_etf = bn.etf; a.c_elf = bn.elf;
+          a.p_ua = bp.ua; a.p_uab = bp.uab; a.p_va = bp.va; a.p_vab = bp.vab; a.p_el = bp.el; a.p_elb = bp.elb;
+          a.p_d = bp.d; a.p_uaf = bp.uaf; a.p_vaf = bp.vaf;
+          a.a_advua = bn.advua; a.a_advva = bn.advva; a.a_adx2d = w<POM_ADX2D>(); a.a_ady2d = w<POM_ADY2D>();
+          a.a_drx2d = w<POM_DRX2D>(); a.a_dry2d = w<POM_DRY2D>(); a.a_wubot = bi[POM_WUBOT];
+          a.a_wvbot = bi[POM_WVBOT];
+          a.elf_first = n < S.isplit;
+          launch_ext_rev_uv(cn, a);
         }
         if (n % c.prm.ispadv == 0) {   // advave produced this substep's advua / advva
           COutTab<T> ob{{bn.advua, bn.advva, nullptr, nullptr, nullptr}};
@@ -1297,13 +1301,10 @@ template <class T> struct Adj {
           zq(bn.advva, N2);
           zflush();
         }
-        {
-          COutTab<T> ob{{bn.elf}};
-          T* ib[OpBc1::NIN] = {s2d};
-          launch_gather<T, OpBc1>(cn, S.t_bc1(), ob, ib, 1u << OpBc1::RAW);   // assigns every point
-          COutTab<T> ob2{{s2d}};
-          T* ib2[OpExtElfRaw::NIN] = {bp.ua, bp.va, bp.d, bp.elb};
-          launch_gather<T, OpExtElfRaw>(cn, S.t_elf(xp), ob2, ib2);
+        {   // bcond 1 and continuity
+          ExtRevElfArgs<T> a{bn.elf, xp.d, xp.ua, xp.va, S.g[POM_G_DX], S.g[POM_G_DY], S.g[POM_G_ART],
+                             S.g[POM_G_FSM], bp.ua, bp.va, bp.d, bp.elb};
+          launch_ext_rev_elf(cn, a);
         }
       }
       X<T>& b0 = bx[0];
