@@ -13,13 +13,11 @@ namespace {
 template <class T> struct L2 {
   const Pos& q;
   __device__ __forceinline__ T operator()(const T* f, int i, int j) const {
-    return ld_local(f + q.o2, lix2(q, clampi(i, q.im), clampi(j, q.jm)));
+    return ld_at(f, q.o2, lix2(q, clampi(i, q.im), clampi(j, q.jm)));
   }
 };
 
-template <class T> __device__ __forceinline__ void st2(const Pos& q, T* f, uint32_t ix, T v) {
-  *reinterpret_cast<T*>(reinterpret_cast<char*>(f + q.o2) + ix * (uint32_t)sizeof(T)) = v;
-}
+template <class T> __device__ __forceinline__ void st2(const Pos& q, T* f, uint32_t ix, T v) { ref_at(f, q.o2, ix) = v; }
 
 // wave speed of the radiation forms, safe_sqrt(g / clamp(h)) as OpExtUVG::cwave
 template <class T> __device__ __forceinline__ T cwave(T h, double grav) {
@@ -190,7 +188,7 @@ __global__ void __launch_bounds__(TPB) k_ext_rev_uv(Batch B, const T* ramp, Para
 
     // ---- filter, level shift and time means at q (OpExtFilter)
     const uint32_t ix = lix2(q, qi, qj);
-    auto ld = [&](const T* p) { return ld_local(p + q.o2, ix); };
+    auto ld = [&](const T* p) { return ld_at(p, q.o2, ix); };
     const T c_uab = ld(a.c_uab), c_vab = ld(a.c_vab), c_elb = ld(a.c_elb);
     T celf = (a.elf_first ? T(0) : ld(a.c_elf)) + ld(a.c_el) + f * c_elb + ld(a.c_d) + r_elf;
     {
@@ -270,7 +268,7 @@ __global__ void __launch_bounds__(TPB) k_ext_rev_elf(Batch B, Params prm, ExtRev
                  sel(qj >= 1, cfv0 * T(0.25) * sdx * L(a.va, qi, qj)) +
                  sel(qj + 1 <= jm - 1, cfv1 * T(0.25) * (L(a.dx, qi, qj + 1) + dx_q) * L(a.va, qi, qj + 1));
     const uint32_t ix = lix2(q, qi, qj);
-    auto ld = [&](const T* p) { return ld_local(p + q.o2, ix); };
+    auto ld = [&](const T* p) { return ld_at(p, q.o2, ix); };
     st2(q, a.p_ua, ix, ld(a.p_ua) + cua);
     st2(q, a.p_va, ix, ld(a.p_va) + cva);
     st2(q, a.p_d, ix, ld(a.p_d) + cd);

@@ -404,18 +404,25 @@ struct Pos {
   int64_t o2, o3;   // member base offsets
 };
 
+// The member's fields are set before the per-lane range check, so that they
+// hold the block's one value on every path: the compiler then keeps them (and
+// every field base formed from them) in scalar registers, and a load is that
+// base plus a 32-bit lane offset.  Set after the check, they would be merged
+// with the undefined values of the lanes past the member's end and counted as
+// per-lane, and every load would form a 64-bit address per lane.  Local point
+// indices fit in 32 bits (a member's 3-D field is below 2 GiB).
 __device__ __forceinline__ bool pos3_at(const Batch& B, int b, Pos& p) {
   if (b >= B.blk3[B.M]) return false;
-  int m = member_of(B.blk3, B.M, b);
-  Member mm = B.mem[m];
-  int64_t local = (int64_t)(b - B.blk3[m]) * TPB + threadIdx.x;
-  int64_t n = (int64_t)mm.im * mm.jm * KB;
-  if (local >= n) return false;
+  const int m = member_of(B.blk3, B.M, b);
+  const Member mm = B.mem[m];
   p.m = m; p.im = mm.im; p.jm = mm.jm; p.o2 = mm.off2; p.o3 = mm.off3;
+  const uint32_t local = (uint32_t)(b - B.blk3[m]) * TPB + threadIdx.x;
+  const uint32_t n = (uint32_t)mm.im * (uint32_t)mm.jm * KB;
+  if (local >= n) return false;
   p.k = (int)(local & (KB - 1));
-  int64_t col = local >> 4;
-  p.i = (int)(col / mm.jm);
-  p.j = (int)(col - (int64_t)p.i * mm.jm);
+  const uint32_t col = local >> 4;
+  p.i = (int)(col / (uint32_t)mm.jm);
+  p.j = (int)(col - (uint32_t)p.i * (uint32_t)mm.jm);
   return true;
 }
 
@@ -423,15 +430,15 @@ __device__ __forceinline__ bool pos3(const Batch& B, Pos& p) { return pos3_at(B,
 
 __device__ __forceinline__ bool pos2_at(const Batch& B, int b, Pos& p) {
   if (b >= B.blk2[B.M]) return false;
-  int m = member_of(B.blk2, B.M, b);
-  Member mm = B.mem[m];
-  int64_t local = (int64_t)(b - B.blk2[m]) * TPB + threadIdx.x;
-  int64_t n = (int64_t)mm.im * mm.jm;
+  const int m = member_of(B.blk2, B.M, b);
+  const Member mm = B.mem[m];
+  p.m = m; p.im = mm.im; p.jm = mm.jm; p.o2 = mm.off2; p.o3 = mm.off3;   // before the per-lane check (pos3_at)
+  const uint32_t local = (uint32_t)(b - B.blk2[m]) * TPB + threadIdx.x;
+  const uint32_t n = (uint32_t)mm.im * (uint32_t)mm.jm;
   if (local >= n) return false;
-  p.m = m; p.im = mm.im; p.jm = mm.jm; p.o2 = mm.off2; p.o3 = mm.off3;
   p.k = 0;
-  p.i = (int)(local / mm.jm);
-  p.j = (int)(local - (int64_t)p.i * mm.jm);
+  p.i = (int)(local / (uint32_t)mm.jm);
+  p.j = (int)(local - (uint32_t)p.i * (uint32_t)mm.jm);
   return true;
 }
 
@@ -451,12 +458,27 @@ __device__ __forceinline__ int64_t ix2(const Pos& p, int i, int j) {
 }
 
 // Element idx of a member's field, given the member's base pointer.  The
-// base is uniform over a block (a block never straddles members) and the
 // byte offset is formed in 32 bits (a member's field is below 2 GiB, checked
-// when the batch is created), so the load takes a scalar 64-bit base and a
-// 32-bit lane offset instead of per-lane 64-bit address arithmetic.
+// when the batch is created).
 template <class T> __device__ __forceinline__ T ld_local(const T* base, uint32_t idx) {
   return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (idx * (uint32_t)sizeof(T)));
+}
+// Element idx of a member's field f, o being the member's first element (o2
+// or o3).  The whole byte offset (o + idx) * sizeof(T) is formed in 32 bits
+// (a field of the whole batch is below 4 GiB, checked when the batch is
+// created), so a load takes the field pointer itself as its scalar base and
+// one 32-bit lane offset per point: no 64-bit base is formed per field, and a
+// kernel that reads many fields keeps only their pointers in scalar registers
+// (with a base per field, forty fields exceed the scalar register file and
+// the bases spill into vector lanes).
+template <class T> __device__ __forceinline__ uint32_t boff(int64_t o, uint32_t idx) {
+  return ((uint32_t)o + idx) * (uint32_t)sizeof(T);
+}
+template <class T> __device__ __forceinline__ T ld_at(const T* f, int64_t o, uint32_t idx) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(f) + boff<T>(o, idx));
+}
+template <class T> __device__ __forceinline__ T& ref_at(T* f, int64_t o, uint32_t idx) {
+  return *reinterpret_cast<T*>(reinterpret_cast<char*>(f) + boff<T>(o, idx));
 }
 __device__ __forceinline__ uint32_t lix3(const Pos& p, int i, int j, int k) {
   return ((uint32_t)i * (uint32_t)p.jm + (uint32_t)j) * (uint32_t)KB + (uint32_t)k;
@@ -519,19 +541,19 @@ struct Acc {
 
   template <int F, int DI, int DJ, int DK>
   __device__ __forceinline__ auto at() const {
-    T x = ld_local(tab->f[F] + p.o3, lix3(p, ci(p.i + DI), cj(p.j + DJ), ck(p.k + DK)));
+    T x = ld_at(tab->f[F], p.o3, lix3(p, ci(p.i + DI), cj(p.j + DJ), ck(p.k + DK)));
     if constexpr (F == SF && DI == SI && DJ == SJ && DK == SK) return D<T>{x, T(1)};
     else return P<T>{x};
   }
   template <int F, int DI, int DJ>
   __device__ __forceinline__ auto at2() const {
-    T x = ld_local(tab->f[F] + p.o2, lix2(p, ci(p.i + DI), cj(p.j + DJ)));
+    T x = ld_at(tab->f[F], p.o2, lix2(p, ci(p.i + DI), cj(p.j + DJ)));
     if constexpr (F == SF && DI == SI && DJ == SJ) return D<T>{x, T(1)};
     else return P<T>{x};
   }
   template <int G, int DI, int DJ>
   __device__ __forceinline__ P<T> g() const {
-    return P<T>{ld_local(tab->g[G] + p.o2, lix2(p, ci(p.i + DI), cj(p.j + DJ)))};
+    return P<T>{ld_at(tab->g[G], p.o2, lix2(p, ci(p.i + DI), cj(p.j + DJ)))};
   }
   __device__ __forceinline__ T z(int k) const { return vert.z(p.m, k); }
   __device__ __forceinline__ T zz(int k) const { return vert.zz(p.m, k); }
@@ -597,19 +619,19 @@ struct AccT {
   template <int F, int DI, int DJ, int DK>
   __device__ __forceinline__ auto at() const {
     static_assert(DK - OK >= -2 && DK - OK <= 2, "stencil beyond table");
-    T x = ld_local(tab->f[F] + p.o3, (uint32_t)(loc<DI, DJ>() * KB + nb->ck[DK - OK + 2]));
+    T x = ld_at(tab->f[F], p.o3, (uint32_t)(loc<DI, DJ>() * KB + nb->ck[DK - OK + 2]));
     if constexpr (((MASK >> F) & 1u) && DI == OI && DJ == OJ && DK == OK) return DS<T, (1u << F)>{x, {T(1)}};
     else return P<T>{x};
   }
   template <int F, int DI, int DJ>
   __device__ __forceinline__ auto at2() const {
-    T x = ld_local(tab->f[F] + p.o2, (uint32_t)loc<DI, DJ>());
+    T x = ld_at(tab->f[F], p.o2, (uint32_t)loc<DI, DJ>());
     if constexpr (((MASK >> F) & 1u) && DI == OI && DJ == OJ) return DS<T, (1u << F)>{x, {T(1)}};
     else return P<T>{x};
   }
   template <int G, int DI, int DJ>
   __device__ __forceinline__ P<T> g() const {
-    return P<T>{ld_local(tab->g[G] + p.o2, (uint32_t)loc<DI, DJ>())};
+    return P<T>{ld_at(tab->g[G], p.o2, (uint32_t)loc<DI, DJ>())};
   }
   __device__ __forceinline__ T z(int k) const { return vert.z(p.m, k); }
   __device__ __forceinline__ T zz(int k) const { return vert.zz(p.m, k); }
@@ -644,19 +666,19 @@ struct AccM {
   __device__ __forceinline__ int ck(int k) const { return k < 0 ? 0 : (k >= KB ? KB - 1 : k); }
   template <int F, int DI, int DJ, int DK>
   __device__ __forceinline__ auto at() const {
-    T x = ld_local(tab->f[F] + p.o3, lix3(p, ci(p.i + DI), cj(p.j + DJ), ck(p.k + DK)));
+    T x = ld_at(tab->f[F], p.o3, lix3(p, ci(p.i + DI), cj(p.j + DJ), ck(p.k + DK)));
     if constexpr (((MASK >> F) & 1u) && DI == SI && DJ == SJ && DK == SK) return DS<T, (1u << F)>{x, {T(1)}};
     else return P<T>{x};
   }
   template <int F, int DI, int DJ>
   __device__ __forceinline__ auto at2() const {
-    T x = ld_local(tab->f[F] + p.o2, lix2(p, ci(p.i + DI), cj(p.j + DJ)));
+    T x = ld_at(tab->f[F], p.o2, lix2(p, ci(p.i + DI), cj(p.j + DJ)));
     if constexpr (((MASK >> F) & 1u) && DI == SI && DJ == SJ) return DS<T, (1u << F)>{x, {T(1)}};
     else return P<T>{x};
   }
   template <int G, int DI, int DJ>
   __device__ __forceinline__ P<T> g() const {
-    return P<T>{ld_local(tab->g[G] + p.o2, lix2(p, ci(p.i + DI), cj(p.j + DJ)))};
+    return P<T>{ld_at(tab->g[G], p.o2, lix2(p, ci(p.i + DI), cj(p.j + DJ)))};
   }
   __device__ __forceinline__ T z(int k) const { return vert.z(p.m, k); }
   __device__ __forceinline__ T zz(int k) const { return vert.zz(p.m, k); }

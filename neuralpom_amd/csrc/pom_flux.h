@@ -23,10 +23,10 @@ template <class T> struct FluxLoad {
   const FieldTab<T>* tab;
   Pos q;
   __device__ __forceinline__ T f3(int fid, int i, int j, int k) const {
-    return ld_local(tab->f[fid] + q.o3, lix3(q, i, j, k));
+    return ld_at(tab->f[fid], q.o3, lix3(q, i, j, k));
   }
-  __device__ __forceinline__ T f2(int fid, int i, int j) const { return ld_local(tab->f[fid] + q.o2, lix2(q, i, j)); }
-  __device__ __forceinline__ T g(int gid, int i, int j) const { return ld_local(tab->g[gid] + q.o2, lix2(q, i, j)); }
+  __device__ __forceinline__ T f2(int fid, int i, int j) const { return ld_at(tab->f[fid], q.o2, lix2(q, i, j)); }
+  __device__ __forceinline__ T g(int gid, int i, int j) const { return ld_at(tab->g[gid], q.o2, lix2(q, i, j)); }
 };
 
 // partials of one horizontal face flux of the centred tracer scheme
@@ -73,11 +73,12 @@ template <class T, int NT> struct FluxSet {
 
 // o[f][i] (+)= a[0], then += a[1], ... (assign: the first partial replaces the old value)
 template <class T, int NT>
-__device__ __forceinline__ void put_in_seq(const InTab<T>& t, int f, int64_t i, const T (&a)[NT]) {
-  T v = ((t.assign >> f) & 1u) ? a[0] : t.o[f][i] + a[0];
+__device__ __forceinline__ void put_in_seq(const InTab<T>& t, int f, int64_t o, uint32_t li, const T (&a)[NT]) {
+  T& r = ref_at(t.o[f], o, li);
+  T v = ((t.assign >> f) & 1u) ? a[0] : r + a[0];
 #pragma unroll
   for (int n = 1; n < NT; ++n) v += a[n];
-  t.o[f][i] = v;
+  r = v;
 }
 
 // 2-D partials summed over the 16-lane column group
@@ -128,17 +129,17 @@ __global__ void __launch_bounds__(TPB) k_flux_advt1(Batch B, Vert<T> vert, Param
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     const FluxLoad<T> Lt{&fs.tab[t], q};
-    const T* ob = fs.ob[t] + q.o3;
+    const T* ob = fs.ob[t];
     fq[t] = Lt.f3(O::F, i, j, k); fbq[t] = Lt.f3(O::FB, i, j, k); fcq[t] = Lt.f3(O::FCLIM, i, j, k);
-    obq[t] = ld_local(ob, lix3(q, i, j, k));
+    obq[t] = ld_at(ob, q.o3, lix3(q, i, j, k));
     f_w[t] = Lt.f3(O::F, iw, j, k); fa_w[t] = Lt.f3(O::FB, iw, j, k) - Lt.f3(O::FCLIM, iw, j, k);
-    ob_w[t] = ld_local(ob, lix3(q, iw, j, k));
+    ob_w[t] = ld_at(ob, q.o3, lix3(q, iw, j, k));
     f_e[t] = Lt.f3(O::F, ie, j, k); fa_e[t] = Lt.f3(O::FB, ie, j, k) - Lt.f3(O::FCLIM, ie, j, k);
-    ob_e[t] = ld_local(ob, lix3(q, ie, j, k));
+    ob_e[t] = ld_at(ob, q.o3, lix3(q, ie, j, k));
     f_s[t] = Lt.f3(O::F, i, js, k); fa_s[t] = Lt.f3(O::FB, i, js, k) - Lt.f3(O::FCLIM, i, js, k);
-    ob_s[t] = ld_local(ob, lix3(q, i, js, k));
+    ob_s[t] = ld_at(ob, q.o3, lix3(q, i, js, k));
     f_n[t] = Lt.f3(O::F, i, jn, k); fa_n[t] = Lt.f3(O::FB, i, jn, k) - Lt.f3(O::FCLIM, i, jn, k);
-    ob_n[t] = ld_local(ob, lix3(q, i, jn, k));
+    ob_n[t] = ld_at(ob, q.o3, lix3(q, i, jn, k));
   }
 
   // ---- r = obar / (tf + 1e-20) on the interior, zero elsewhere
@@ -156,7 +157,7 @@ __global__ void __launch_bounds__(TPB) k_flux_advt1(Batch B, Vert<T> vert, Param
   const T fz_w = k == 0 ? T(1) : T(0.5);   // d (face value at level k) / d f(k)
 
   T p_u[NT], p_v[NT], p_w[NT], p_aam[NT], p_dt[NT], p_etb[NT], p_etf[NT];
-  const int64_t c3 = ix3(q, i, j, k);
+  const uint32_t c3 = lix3(q, i, j, k);
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     const T rc = in ? obq[t] * den : T(0), rw = in_w ? ob_w[t] * den_w : T(0), re = in_e ? ob_e[t] * den_e : T(0);
@@ -201,15 +202,15 @@ __global__ void __launch_bounds__(TPB) k_flux_advt1(Batch B, Vert<T> vert, Param
     p_etf[t] = in ? -artq * rc * num * rcp_lane(tf + T(1e-20)) : T(0);
 
     const InTab<T>& ib = fs.ib[t];
-    if (ib.o[O::FB]) put_in(ib, O::FB, c3, b_fb);
-    if (ib.o[O::F]) put_in(ib, O::F, c3, b_f);
-    if (ib.o[O::FCLIM]) put_in(ib, O::FCLIM, c3, -b_fd);
+    if (ib.o[O::FB]) put_in(ib, O::FB, q.o3, c3, b_fb);
+    if (ib.o[O::F]) put_in(ib, O::F, q.o3, c3, b_f);
+    if (ib.o[O::FCLIM]) put_in(ib, O::FCLIM, q.o3, c3, -b_fd);
   }
   const InTab<T>& ib = fs.ib[0];
-  if (ib.o[O::U]) put_in_seq(ib, O::U, c3, p_u);
-  if (ib.o[O::V]) put_in_seq(ib, O::V, c3, p_v);
-  if (ib.o[O::W]) put_in_seq(ib, O::W, c3, p_w);
-  if (ib.o[O::AAM]) put_in_seq(ib, O::AAM, c3, p_aam);
+  if (ib.o[O::U]) put_in_seq(ib, O::U, q.o3, c3, p_u);
+  if (ib.o[O::V]) put_in_seq(ib, O::V, q.o3, c3, p_v);
+  if (ib.o[O::W]) put_in_seq(ib, O::W, q.o3, c3, p_w);
+  if (ib.o[O::AAM]) put_in_seq(ib, O::AAM, q.o3, c3, p_aam);
   // 2-D inputs: sums over the column group
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
@@ -218,10 +219,10 @@ __global__ void __launch_bounds__(TPB) k_flux_advt1(Batch B, Vert<T> vert, Param
     p_etf[t] = column_sum(p_etf[t]);
   }
   if (k == 0) {
-    const int64_t c2 = ix2(q, i, j);
-    if (ib.o[O::DT]) put_in_seq(ib, O::DT, c2, p_dt);
-    if (ib.o[O::ETB]) put_in_seq(ib, O::ETB, c2, p_etb);
-    if (ib.o[O::ETF]) put_in_seq(ib, O::ETF, c2, p_etf);
+    const uint32_t c2 = lix2(q, i, j);
+    if (ib.o[O::DT]) put_in_seq(ib, O::DT, q.o2, c2, p_dt);
+    if (ib.o[O::ETB]) put_in_seq(ib, O::ETB, q.o2, c2, p_etb);
+    if (ib.o[O::ETF]) put_in_seq(ib, O::ETF, q.o2, c2, p_etf);
   }
 }
 
@@ -291,12 +292,12 @@ __global__ void __launch_bounds__(TPB) k_flux_advq(Batch B, Vert<T> vert, Params
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     const FluxLoad<T> Lt{&fs.tab[t], q};
-    const T* ob = fs.ob[t] + q.o3;
-    qq[t] = Lt.f3(O::Q, i, j, k); qbq[t] = Lt.f3(O::QB, i, j, k); obq[t] = ld_local(ob, lix3(q, i, j, k));
-    q_w[t] = Lt.f3(O::Q, iw, j, k); qb_w[t] = Lt.f3(O::QB, iw, j, k); ob_w[t] = ld_local(ob, lix3(q, iw, j, k));
-    q_e[t] = Lt.f3(O::Q, ie, j, k); qb_e[t] = Lt.f3(O::QB, ie, j, k); ob_e[t] = ld_local(ob, lix3(q, ie, j, k));
-    q_s[t] = Lt.f3(O::Q, i, js, k); qb_s[t] = Lt.f3(O::QB, i, js, k); ob_s[t] = ld_local(ob, lix3(q, i, js, k));
-    q_n[t] = Lt.f3(O::Q, i, jn, k); qb_n[t] = Lt.f3(O::QB, i, jn, k); ob_n[t] = ld_local(ob, lix3(q, i, jn, k));
+    const T* ob = fs.ob[t];
+    qq[t] = Lt.f3(O::Q, i, j, k); qbq[t] = Lt.f3(O::QB, i, j, k); obq[t] = ld_at(ob, q.o3, lix3(q, i, j, k));
+    q_w[t] = Lt.f3(O::Q, iw, j, k); qb_w[t] = Lt.f3(O::QB, iw, j, k); ob_w[t] = ld_at(ob, q.o3, lix3(q, iw, j, k));
+    q_e[t] = Lt.f3(O::Q, ie, j, k); qb_e[t] = Lt.f3(O::QB, ie, j, k); ob_e[t] = ld_at(ob, q.o3, lix3(q, ie, j, k));
+    q_s[t] = Lt.f3(O::Q, i, js, k); qb_s[t] = Lt.f3(O::QB, i, js, k); ob_s[t] = ld_at(ob, q.o3, lix3(q, i, js, k));
+    q_n[t] = Lt.f3(O::Q, i, jn, k); qb_n[t] = Lt.f3(O::QB, i, jn, k); ob_n[t] = ld_at(ob, q.o3, lix3(q, i, jn, k));
   }
   // values at level k - 1 of this lane's columns (level 0 takes its own: its
   // faces carry a zero cotangent, since no output lives on level 0)
@@ -320,7 +321,7 @@ __global__ void __launch_bounds__(TPB) k_flux_advq(Batch B, Vert<T> vert, Params
   const T tf = (hq + etfq) * artq;
 
   T p_u[NT], p_v[NT], p_w[NT], p_aam[NT], p_dt[NT], p_etb[NT], p_etf[NT];
-  const int64_t c3 = ix3(q, i, j, k);
+  const uint32_t c3 = lix3(q, i, j, k);
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     const T rc = in ? obq[t] * den : T(0), rw = in_w ? ob_w[t] * den_w : T(0), re = in_e ? ob_e[t] * den_e : T(0);
@@ -364,14 +365,14 @@ __global__ void __launch_bounds__(TPB) k_flux_advq(Batch B, Vert<T> vert, Params
     p_etf[t] = in ? -artq * rc * num * rcp_lane(tf + T(1e-20)) : T(0);
 
     const InTab<T>& ib = fs.ib[t];
-    if (ib.o[O::QB]) put_in(ib, O::QB, c3, b_qb);
-    if (ib.o[O::Q]) put_in(ib, O::Q, c3, b_q);
+    if (ib.o[O::QB]) put_in(ib, O::QB, q.o3, c3, b_qb);
+    if (ib.o[O::Q]) put_in(ib, O::Q, q.o3, c3, b_q);
   }
   const InTab<T>& ib = fs.ib[0];
-  if (ib.o[O::U]) put_in_seq(ib, O::U, c3, p_u);
-  if (ib.o[O::V]) put_in_seq(ib, O::V, c3, p_v);
-  if (ib.o[O::W]) put_in_seq(ib, O::W, c3, p_w);
-  if (ib.o[O::AAM]) put_in_seq(ib, O::AAM, c3, p_aam);
+  if (ib.o[O::U]) put_in_seq(ib, O::U, q.o3, c3, p_u);
+  if (ib.o[O::V]) put_in_seq(ib, O::V, q.o3, c3, p_v);
+  if (ib.o[O::W]) put_in_seq(ib, O::W, q.o3, c3, p_w);
+  if (ib.o[O::AAM]) put_in_seq(ib, O::AAM, q.o3, c3, p_aam);
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     p_dt[t] = column_sum(p_dt[t]);
@@ -379,10 +380,10 @@ __global__ void __launch_bounds__(TPB) k_flux_advq(Batch B, Vert<T> vert, Params
     p_etf[t] = column_sum(p_etf[t]);
   }
   if (k == 0) {
-    const int64_t c2 = ix2(q, i, j);
-    if (ib.o[O::DT]) put_in_seq(ib, O::DT, c2, p_dt);
-    if (ib.o[O::ETB]) put_in_seq(ib, O::ETB, c2, p_etb);
-    if (ib.o[O::ETF]) put_in_seq(ib, O::ETF, c2, p_etf);
+    const uint32_t c2 = lix2(q, i, j);
+    if (ib.o[O::DT]) put_in_seq(ib, O::DT, q.o2, c2, p_dt);
+    if (ib.o[O::ETB]) put_in_seq(ib, O::ETB, q.o2, c2, p_etb);
+    if (ib.o[O::ETF]) put_in_seq(ib, O::ETF, q.o2, c2, p_etf);
   }
 }
 

@@ -58,7 +58,8 @@ __global__ void __launch_bounds__(TPB) k_map(Batch B, Vert<T> vert, const T* ram
     if (!pos2_at(B, blk, p)) continue;
   }
   Acc<T> A{&tab, p, vert, ramp};
-  const int64_t idx = Op::OUT3 ? ix3(p, p.i, p.j, p.k) : ix2(p, p.i, p.j);
+  const int64_t o_ = Op::OUT3 ? p.o3 : p.o2;   // the member's first element
+  const uint32_t li = Op::OUT3 ? lix3(p, p.i, p.j, p.k) : lix2(p, p.i, p.j);
   // every output evaluated before the first store: a store may alias the
   // inputs, so loads after it could neither be shared between outputs nor
   // issued early
@@ -69,7 +70,7 @@ __global__ void __launch_bounds__(TPB) k_map(Batch B, Vert<T> vert, const T* ram
   });
   static_for<0, Op::NOUT - 1>([&](auto O) {
     constexpr int o = decltype(O)::value;
-    if (out.o[o]) out.o[o][idx] = res[o];
+    if (out.o[o]) ref_at(out.o[o], o_, li) = res[o];
   });
   }
 }
@@ -118,7 +119,7 @@ __global__ void __launch_bounds__(TPB) k_map2(Batch B, Vert<T> vert, const T* ra
   for (int blk = logical_block(); blk < B.blk2[B.M]; blk += (int)gridDim.x) {
   Pos p;
   if (pos2_at(B, blk, p)) {
-    const int64_t idx = ix2(p, p.i, p.j);
+    const uint32_t li = lix2(p, p.i, p.j);
     T ra[2], rb[OpB::NOUT];
     {
       Acc<T> A{&ta, p, vert, ramp};
@@ -133,11 +134,11 @@ __global__ void __launch_bounds__(TPB) k_map2(Batch B, Vert<T> vert, const T* ra
       });
     }
     vmax = nanmax(vmax, fabs(ra[1]));
-    if (oa.o[0]) oa.o[0][idx] = ra[0];
-    if (oa.o[1]) oa.o[1][idx] = ra[1];
+    if (oa.o[0]) ref_at(oa.o[0], p.o2, li) = ra[0];
+    if (oa.o[1]) ref_at(oa.o[1], p.o2, li) = ra[1];
     static_for<0, OpB::NOUT - 1>([&](auto O) {
       constexpr int o = decltype(O)::value;
-      if (ob.o[o]) ob.o[o][idx] = rb[o];
+      if (ob.o[o]) ref_at(ob.o[o], p.o2, li) = rb[o];
     });
   }
   }
@@ -155,9 +156,10 @@ template <class T> struct InTab {
   T* o[MAXF];
   uint32_t assign;
 };
-template <class T> __device__ __forceinline__ void put_in(const InTab<T>& t, int f, int64_t i, T v) {
-  if ((t.assign >> f) & 1u) t.o[f][i] = v;
-  else t.o[f][i] += v;
+template <class T> __device__ __forceinline__ void put_in(const InTab<T>& t, int f, int64_t o, uint32_t li, T v) {
+  T& r = ref_at(t.o[f], o, li);
+  if ((t.assign >> f) & 1u) r = v;
+  else r += v;
 }
 
 template <class T, class Op, int DI, int DJ, int DK, uint32_t MASK, bool DEEP = false>
@@ -190,7 +192,7 @@ __device__ __forceinline__ void gather_all_term(const FieldTab<T>& tab, const CO
       constexpr uint32_t MR = mask_of<R>::value;
       if constexpr (MR != 0u) {
         if (obar.o[o]) {
-          const T ob = ld_local(obar.o[o] + (Op::OUT3 ? p.o3 : p.o2), idx);
+          const T ob = ld_at(obar.o[o], Op::OUT3 ? p.o3 : p.o2, idx);
           const R r = Op::template eval<o>(A, prm);
           for_bits<MR>([&](auto I) {
             constexpr int f = decltype(I)::value;
@@ -258,12 +260,12 @@ __global__ void __launch_bounds__(TPB) k_gather_all3(Batch B, Vert<T> vert, cons
   static_for<0, Op::NIN - 1>([&](auto Fc) {
     constexpr int f = decltype(Fc)::value;
     if constexpr ((M3 >> f) & 1u) {
-      if (live && inbar.o[f]) put_in(inbar, f, ix3(q, q.i, q.j, q.k), acc[f]);
+      if (live && inbar.o[f]) put_in(inbar, f, q.o3, lix3(q, q.i, q.j, q.k), acc[f]);
     } else if constexpr ((M2 >> f) & 1u) {
       T v = acc[f];
 #pragma unroll
       for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o, 16);
-      if (live && q.k == 0 && inbar.o[f]) put_in(inbar, f, ix2(q, q.i, q.j), v);
+      if (live && q.k == 0 && inbar.o[f]) put_in(inbar, f, q.o2, lix2(q, q.i, q.j), v);
     }
   });
   }
@@ -294,7 +296,7 @@ __global__ void __launch_bounds__(TPB) k_gather_all2(Batch B, Vert<T> vert, cons
   }
   static_for<0, Op::NIN - 1>([&](auto Fc) {
     constexpr int f = decltype(Fc)::value;
-    if (inbar.o[f]) put_in(inbar, f, ix2(q, q.i, q.j), acc[f]);
+    if (inbar.o[f]) put_in(inbar, f, q.o2, lix2(q, q.i, q.j), acc[f]);
   });
   }
 }

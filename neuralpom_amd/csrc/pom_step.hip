@@ -20,6 +20,7 @@
 #include "pom_pcg.h"
 #include "pom_profq.h"
 #include "pom_extrev.h"
+#include "pom_momadj.h"
 
 #include "pom_batch.h"
 
@@ -1482,15 +1483,17 @@ template <class T> struct Adj {
         hipLaunchKernelGGL((k_profuv_adj2<T, 1>), g2, tb, 0, c.stream, c.B, c.vert, b2);
       }
     }
-    {   // advu / advv
-      COutTab<T> ob{{b_uf0}};
-      T* ib[OpAdvu::NIN] = {w<POM_ADVX>(), w<POM_DRHOX>(), b_u1, b_v1, w<POM_W>(), bi[POM_UB], bi[POM_DT],
-                            b_egf30, bi[POM_EGB], b_etf30, bi[POM_ETB]};
-      launch_gather<T, OpAdvu>(c, s.t_advu(), ob, ib);
-      COutTab<T> ov{{b_vf0}};
-      T* iv[OpAdvv::NIN] = {w<POM_ADVY>(), w<POM_DRHOY>(), b_u1, b_v1, w<POM_W>(), bi[POM_VB], bi[POM_DT],
-                            b_egf30, bi[POM_EGB], b_etf30, bi[POM_ETB]};
-      launch_gather<T, OpAdvv>(c, s.t_advv(), ov, iv);
+    {   // advu / advv: one transposed stencil for both equations
+      MomAdjArgs<T> a{};
+      a.u = s.u1; a.v = s.v1; a.w = s.out[POM_W]; a.ub = s.in[POM_UB]; a.vb = s.in[POM_VB]; a.uf = s.uf0;
+      a.vf = s.vf0; a.dt = s.in[POM_DT]; a.egf = s.xf().egf; a.egb = s.in[POM_EGB]; a.etf = s.xf().etf;
+      a.etb = s.in[POM_ETB]; a.aru = s.g[POM_G_ARU]; a.arv = s.g[POM_G_ARV]; a.cor = s.g[POM_G_COR];
+      a.dx = s.g[POM_G_DX]; a.dy = s.g[POM_G_DY]; a.h = s.g[POM_G_H]; a.eatm = s.out[POM_E_ATMOS];
+      a.ob_u = b_uf0; a.ob_v = b_vf0;
+      a.c_advx = w<POM_ADVX>(); a.c_drhox = w<POM_DRHOX>(); a.c_advy = w<POM_ADVY>(); a.c_drhoy = w<POM_DRHOY>();
+      a.c_u = b_u1; a.c_v = b_v1; a.c_w = w<POM_W>(); a.c_ub = bi[POM_UB]; a.c_vb = bi[POM_VB];
+      a.c_dt = bi[POM_DT]; a.c_egf = b_egf30; a.c_egb = bi[POM_EGB]; a.c_etf = b_etf30; a.c_etb = bi[POM_ETB];
+      launch_adj_advuv(c, a);
     }
     {   // dens
       COutTab<T> ob{{bo[POM_RHO]}};
@@ -2403,6 +2406,8 @@ int pom_batch_create(const pom_member_t* members, int32_t n, int32_t dtype, pom_
     n2 = std::max(n2, s.off2 + c2);
     n3 = std::max(n3, s.off3 + c3);
   }
+  // and a field of the whole batch with 32-bit byte offsets from its pointer
+  if (n3 * (dtype == POM_F64 ? 8 : 4) >= (int64_t(1) << 32)) return POM_E_ARG;
   pom_batch* b = new pom_batch{};
   b->h_mem = hm;
   b->dtype = dtype;
