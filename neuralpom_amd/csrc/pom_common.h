@@ -362,11 +362,20 @@ struct Member {
   int64_t off2, off3;
 };
 
+// Water columns near a member's edge, for the operators that read
+// neighbours only there (EDGE_BAND): entry (m, i * jm + j) for every column
+// of member m whose distance d = min(i, im-1-i, j, jm-1-j) to the nearest
+// edge is below BAND_W, ordered by d, so that the first nband[w] entries are
+// the columns with d < w.
+constexpr int BAND_W = 4;
+
 struct Batch {
   const Member* mem;   // [M]
   const int32_t* blk3; // [M+1] prefix of 3-D point blocks
   const int32_t* blk2; // [M+1] prefix of 2-D point blocks
   int32_t M;
+  const int2* band = nullptr;          // the edge-band columns (null: none built)
+  int32_t nband[BAND_W + 1] = {};
 };
 
 // Workgroups are dispatched to the 8 XCDs round-robin (workgroup x runs on

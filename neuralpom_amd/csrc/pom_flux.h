@@ -81,6 +81,22 @@ __device__ __forceinline__ void put_in_seq(const InTab<T>& t, int f, int64_t o, 
   r = v;
 }
 
+// Accumulation in two phases: old values of every field first (each under
+// its own uniform null check, with no use in between, so that the reads are
+// in flight together), then the stores.  The arrays of one launch are distinct.
+template <class T>
+__device__ __forceinline__ T old_in(const InTab<T>& t, int f, int64_t o, uint32_t li) {
+  return (t.o[f] && !((t.assign >> f) & 1u)) ? ld_at(t.o[f], o, li) : T(0);
+}
+template <class T, int NT>
+__device__ __forceinline__ void put_seq(const InTab<T>& t, int f, int64_t o, uint32_t li, T old, const T (&a)[NT]) {
+  if (!t.o[f]) return;
+  T v = old + a[0];
+#pragma unroll
+  for (int n = 1; n < NT; ++n) v += a[n];
+  ref_at(t.o[f], o, li) = v;
+}
+
 // 2-D partials summed over the 16-lane column group
 template <class T> __device__ __forceinline__ T column_sum(T v) {
 #pragma unroll
@@ -157,6 +173,7 @@ __global__ void __launch_bounds__(TPB) k_flux_advt1(Batch B, Vert<T> vert, Param
   const T fz_w = k == 0 ? T(1) : T(0.5);   // d (face value at level k) / d f(k)
 
   T p_u[NT], p_v[NT], p_w[NT], p_aam[NT], p_dt[NT], p_etb[NT], p_etf[NT];
+  T s_fb[NT], s_f[NT], s_fc[NT];   // the per-field partials, stored after every old value is read
   const uint32_t c3 = lix3(q, i, j, k);
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
@@ -201,16 +218,11 @@ __global__ void __launch_bounds__(TPB) k_flux_advt1(Batch B, Vert<T> vert, Param
     p_etb[t] = in ? artq * fbq[t] * rc : T(0);
     p_etf[t] = in ? -artq * rc * num * rcp_lane(tf + T(1e-20)) : T(0);
 
-    const InTab<T>& ib = fs.ib[t];
-    if (ib.o[O::FB]) put_in(ib, O::FB, q.o3, c3, b_fb);
-    if (ib.o[O::F]) put_in(ib, O::F, q.o3, c3, b_f);
-    if (ib.o[O::FCLIM]) put_in(ib, O::FCLIM, q.o3, c3, -b_fd);
+    s_fb[t] = b_fb;
+    s_f[t] = b_f;
+    s_fc[t] = -b_fd;
   }
   const InTab<T>& ib = fs.ib[0];
-  if (ib.o[O::U]) put_in_seq(ib, O::U, q.o3, c3, p_u);
-  if (ib.o[O::V]) put_in_seq(ib, O::V, q.o3, c3, p_v);
-  if (ib.o[O::W]) put_in_seq(ib, O::W, q.o3, c3, p_w);
-  if (ib.o[O::AAM]) put_in_seq(ib, O::AAM, q.o3, c3, p_aam);
   // 2-D inputs: sums over the column group
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
@@ -218,11 +230,37 @@ __global__ void __launch_bounds__(TPB) k_flux_advt1(Batch B, Vert<T> vert, Param
     p_etb[t] = column_sum(p_etb[t]);
     p_etf[t] = column_sum(p_etf[t]);
   }
+  const uint32_t c2 = lix2(q, i, j);
+  T o_fb[NT], o_f[NT], o_fc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    o_fb[t] = old_in(fs.ib[t], O::FB, q.o3, c3);
+    o_f[t] = old_in(fs.ib[t], O::F, q.o3, c3);
+    o_fc[t] = old_in(fs.ib[t], O::FCLIM, q.o3, c3);
+  }
+  const T o_u = old_in(ib, O::U, q.o3, c3), o_v = old_in(ib, O::V, q.o3, c3), o_w = old_in(ib, O::W, q.o3, c3);
+  const T o_aam = old_in(ib, O::AAM, q.o3, c3);
+  T o_dt = T(0), o_etb = T(0), o_etf = T(0);
   if (k == 0) {
-    const uint32_t c2 = lix2(q, i, j);
-    if (ib.o[O::DT]) put_in_seq(ib, O::DT, q.o2, c2, p_dt);
-    if (ib.o[O::ETB]) put_in_seq(ib, O::ETB, q.o2, c2, p_etb);
-    if (ib.o[O::ETF]) put_in_seq(ib, O::ETF, q.o2, c2, p_etf);
+    o_dt = old_in(ib, O::DT, q.o2, c2);
+    o_etb = old_in(ib, O::ETB, q.o2, c2);
+    o_etf = old_in(ib, O::ETF, q.o2, c2);
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const InTab<T>& it = fs.ib[t];
+    if (it.o[O::FB]) ref_at(it.o[O::FB], q.o3, c3) = o_fb[t] + s_fb[t];
+    if (it.o[O::F]) ref_at(it.o[O::F], q.o3, c3) = o_f[t] + s_f[t];
+    if (it.o[O::FCLIM]) ref_at(it.o[O::FCLIM], q.o3, c3) = o_fc[t] + s_fc[t];
+  }
+  put_seq(ib, O::U, q.o3, c3, o_u, p_u);
+  put_seq(ib, O::V, q.o3, c3, o_v, p_v);
+  put_seq(ib, O::W, q.o3, c3, o_w, p_w);
+  put_seq(ib, O::AAM, q.o3, c3, o_aam, p_aam);
+  if (k == 0) {
+    put_seq(ib, O::DT, q.o2, c2, o_dt, p_dt);
+    put_seq(ib, O::ETB, q.o2, c2, o_etb, p_etb);
+    put_seq(ib, O::ETF, q.o2, c2, o_etf, p_etf);
   }
 }
 
@@ -321,6 +359,7 @@ __global__ void __launch_bounds__(TPB) k_flux_advq(Batch B, Vert<T> vert, Params
   const T tf = (hq + etfq) * artq;
 
   T p_u[NT], p_v[NT], p_w[NT], p_aam[NT], p_dt[NT], p_etb[NT], p_etf[NT];
+  T s_qb[NT], s_q[NT];   // the per-field partials, stored after every old value is read
   const uint32_t c3 = lix3(q, i, j, k);
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
@@ -364,26 +403,45 @@ __global__ void __launch_bounds__(TPB) k_flux_advq(Batch B, Vert<T> vert, Params
     p_etb[t] = in ? artq * qbq[t] * rc : T(0);
     p_etf[t] = in ? -artq * rc * num * rcp_lane(tf + T(1e-20)) : T(0);
 
-    const InTab<T>& ib = fs.ib[t];
-    if (ib.o[O::QB]) put_in(ib, O::QB, q.o3, c3, b_qb);
-    if (ib.o[O::Q]) put_in(ib, O::Q, q.o3, c3, b_q);
+    s_qb[t] = b_qb;
+    s_q[t] = b_q;
   }
   const InTab<T>& ib = fs.ib[0];
-  if (ib.o[O::U]) put_in_seq(ib, O::U, q.o3, c3, p_u);
-  if (ib.o[O::V]) put_in_seq(ib, O::V, q.o3, c3, p_v);
-  if (ib.o[O::W]) put_in_seq(ib, O::W, q.o3, c3, p_w);
-  if (ib.o[O::AAM]) put_in_seq(ib, O::AAM, q.o3, c3, p_aam);
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     p_dt[t] = column_sum(p_dt[t]);
     p_etb[t] = column_sum(p_etb[t]);
     p_etf[t] = column_sum(p_etf[t]);
   }
+  const uint32_t c2 = lix2(q, i, j);
+  T o_qb[NT], o_q[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    o_qb[t] = old_in(fs.ib[t], O::QB, q.o3, c3);
+    o_q[t] = old_in(fs.ib[t], O::Q, q.o3, c3);
+  }
+  const T o_u = old_in(ib, O::U, q.o3, c3), o_v = old_in(ib, O::V, q.o3, c3), o_w = old_in(ib, O::W, q.o3, c3);
+  const T o_aam = old_in(ib, O::AAM, q.o3, c3);
+  T o_dt = T(0), o_etb = T(0), o_etf = T(0);
   if (k == 0) {
-    const uint32_t c2 = lix2(q, i, j);
-    if (ib.o[O::DT]) put_in_seq(ib, O::DT, q.o2, c2, p_dt);
-    if (ib.o[O::ETB]) put_in_seq(ib, O::ETB, q.o2, c2, p_etb);
-    if (ib.o[O::ETF]) put_in_seq(ib, O::ETF, q.o2, c2, p_etf);
+    o_dt = old_in(ib, O::DT, q.o2, c2);
+    o_etb = old_in(ib, O::ETB, q.o2, c2);
+    o_etf = old_in(ib, O::ETF, q.o2, c2);
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const InTab<T>& it = fs.ib[t];
+    if (it.o[O::QB]) ref_at(it.o[O::QB], q.o3, c3) = o_qb[t] + s_qb[t];
+    if (it.o[O::Q]) ref_at(it.o[O::Q], q.o3, c3) = o_q[t] + s_q[t];
+  }
+  put_seq(ib, O::U, q.o3, c3, o_u, p_u);
+  put_seq(ib, O::V, q.o3, c3, o_v, p_v);
+  put_seq(ib, O::W, q.o3, c3, o_w, p_w);
+  put_seq(ib, O::AAM, q.o3, c3, o_aam, p_aam);
+  if (k == 0) {
+    put_seq(ib, O::DT, q.o2, c2, o_dt, p_dt);
+    put_seq(ib, O::ETB, q.o2, c2, o_etb, p_etb);
+    put_seq(ib, O::ETF, q.o2, c2, o_etf, p_etf);
   }
 }
 

@@ -186,17 +186,26 @@ __device__ __forceinline__ void gather_all_term(const FieldTab<T>& tab, const CO
     const ACC A{base};
     const int ci = clampi(p.i, p.im), cj = clampi(p.j, p.jm);
     const uint32_t idx = Op::OUT3 ? lix3(p, ci, cj, clampi(p.k, KB)) : lix2(p, ci, cj);
+    // the output cotangents first, each under its own (uniform) null check and
+    // with no use in between, so that their loads are in flight together
+    T ob[Op::NOUT];
+    static_for<0, Op::NOUT - 1>([&](auto O) {
+      constexpr int o = decltype(O)::value;
+      if constexpr (mask_of<decltype(Op::template eval<o>(A, prm))>::value != 0u) {
+        ob[o] = T(0);
+        if (obar.o[o]) ob[o] = ld_at(obar.o[o], Op::OUT3 ? p.o3 : p.o2, idx);
+      }
+    });
     static_for<0, Op::NOUT - 1>([&](auto O) {
       constexpr int o = decltype(O)::value;
       using R = decltype(Op::template eval<o>(A, prm));
       constexpr uint32_t MR = mask_of<R>::value;
       if constexpr (MR != 0u) {
         if (obar.o[o]) {
-          const T ob = ld_at(obar.o[o], Op::OUT3 ? p.o3 : p.o2, idx);
           const R r = Op::template eval<o>(A, prm);
           for_bits<MR>([&](auto I) {
             constexpr int f = decltype(I)::value;
-            const T t = ob * r.d[slot(MR, f)];
+            const T t = ob[o] * r.d[slot(MR, f)];
             acc[f] += valid ? t : T(0);
           });
         }
@@ -222,30 +231,35 @@ template <class Op> __device__ __forceinline__ bool deep_point(const Pos& q) {
     return q.i >= EB + Op::RI && q.i <= q.im - 1 - EB - Op::RI && q.j >= EB + Op::RJ && q.j <= q.jm - 1 - EB - Op::RJ;
   }
 }
-// 3-D operators: one thread per 3-D point q; 3-D fields accumulate at q, 2-D
-// fields per level (offsets with dk = 0 only) and are reduced over the 16
-// lanes of the water column
-template <class T, class Op, uint32_t M3, uint32_t M2>
-__global__ void __launch_bounds__(TPB) k_gather_all3(Batch B, Vert<T> vert, const T* ramp, Params prm,
-                                                      FieldTab<T> tab, COutTab<T> obar, InTab<T> inbar) {
-  const int nblk = B.blk3[B.M];
-  for (int blk = logical_block(); blk < nblk; blk += (int)gridDim.x) {   // uniform over the workgroup
-  Pos q;
-  const bool live = pos3_at(B, blk, q);
+// distance of the water column of q to its member's nearest edge is below w
+__device__ __forceinline__ bool in_band(const Pos& q, int w) {
+  return q.i < w || q.i > q.im - 1 - w || q.j < w || q.j > q.jm - 1 - w;
+}
+
+// The gather at 3-D point q: 3-D fields accumulate at q, 2-D fields per
+// level (offsets with dk = 0 only) and are reduced over the 16 lanes of the
+// water column (every lane of the column calls this, live or not).  PATH 0
+// picks the path per point; 1 takes the own-column path of points away from
+// the edges (deep_point), 2 the full stencil.
+template <class T, class Op, uint32_t M3, uint32_t M2, int PATH>
+__device__ __forceinline__ void gather3_at(const Pos& q, bool live, const Vert<T>& vert, const T* ramp,
+                                           const Params& prm, const FieldTab<T>& tab, const COutTab<T>& obar,
+                                           const InTab<T>& inbar) {
   T acc[Op::NIN];
 #pragma unroll
   for (int f = 0; f < Op::NIN; ++f) acc[f] = T(0);
   if (live) {
     NbrIdx nb;
     make_nbr(q, nb);
-    if (edge_band<Op>::value >= 0 && deep_point<Op>(q)) {
+    const bool deep = PATH == 1 || (PATH == 0 && edge_band<Op>::value >= 0 && deep_point<Op>(q));
+    if (deep) {
       // own column only: a compact path of the vertical offsets
       static_for<-Op::RK, Op::RK>([&](auto K) {
         constexpr int dk = decltype(K)::value;
         gather_all_term<T, Op, 0, 0, dk, (M3 | (dk == 0 ? M2 : 0u)), true>(tab, obar, q, q.k - dk, vert, ramp, prm,
                                                                           nb, acc);
       });
-    } else {
+    } else if constexpr (PATH != 1) {
       static_for<-Op::RI, Op::RI>([&](auto I) {
         static_for<-Op::RJ, Op::RJ>([&](auto J) {
           static_for<-Op::RK, Op::RK>([&](auto K) {
@@ -257,17 +271,82 @@ __global__ void __launch_bounds__(TPB) k_gather_all3(Batch B, Vert<T> vert, cons
       });
     }
   }
+  // column sums of the 2-D fields' partials
+  static_for<0, Op::NIN - 1>([&](auto Fc) {
+    constexpr int f = decltype(Fc)::value;
+    if constexpr (!((M3 >> f) & 1u) && ((M2 >> f) & 1u)) {
+#pragma unroll
+      for (int o = 8; o > 0; o >>= 1) acc[f] += __shfl_xor(acc[f], o, 16);
+    }
+  });
+  // accumulate: every old value is read before the first store (the input
+  // cotangent arrays of one launch are distinct), so the reads are in flight together
+  T old[Op::NIN];
+  static_for<0, Op::NIN - 1>([&](auto Fc) {
+    constexpr int f = decltype(Fc)::value;
+    old[f] = T(0);
+    if constexpr ((M3 >> f) & 1u) {
+      if (live && inbar.o[f] && !((inbar.assign >> f) & 1u)) old[f] = ld_at(inbar.o[f], q.o3, lix3(q, q.i, q.j, q.k));
+    } else if constexpr ((M2 >> f) & 1u) {
+      if (live && q.k == 0 && inbar.o[f] && !((inbar.assign >> f) & 1u))
+        old[f] = ld_at(inbar.o[f], q.o2, lix2(q, q.i, q.j));
+    }
+  });
   static_for<0, Op::NIN - 1>([&](auto Fc) {
     constexpr int f = decltype(Fc)::value;
     if constexpr ((M3 >> f) & 1u) {
-      if (live && inbar.o[f]) put_in(inbar, f, q.o3, lix3(q, q.i, q.j, q.k), acc[f]);
+      if (live && inbar.o[f]) ref_at(inbar.o[f], q.o3, lix3(q, q.i, q.j, q.k)) = old[f] + acc[f];
     } else if constexpr ((M2 >> f) & 1u) {
-      T v = acc[f];
-#pragma unroll
-      for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o, 16);
-      if (live && q.k == 0 && inbar.o[f]) put_in(inbar, f, q.o2, lix2(q, q.i, q.j), v);
+      if (live && q.k == 0 && inbar.o[f]) ref_at(inbar.o[f], q.o2, lix2(q, q.i, q.j)) = old[f] + acc[f];
     }
   });
+}
+
+// 3-D operators: one thread per 3-D point
+template <class T, class Op, uint32_t M3, uint32_t M2>
+__global__ void __launch_bounds__(TPB) k_gather_all3(Batch B, Vert<T> vert, const T* ramp, Params prm,
+                                                      FieldTab<T> tab, COutTab<T> obar, InTab<T> inbar) {
+  const int nblk = B.blk3[B.M];
+  for (int blk = logical_block(); blk < nblk; blk += (int)gridDim.x) {   // uniform over the workgroup
+    Pos q;
+    const bool live = pos3_at(B, blk, q);
+    gather3_at<T, Op, M3, M2, 0>(q, live, vert, ramp, prm, tab, obar, inbar);
+  }
+}
+
+// Operators with an edge band take two launches: the points off the band of
+// width W = EDGE_BAND + R (own-column path only), then the band's columns
+// from the batch's band list (full stencil only), so that no wave carries
+// both paths: with one launch, every wave holding one band column (a wave
+// covers four) ran the full stencil as well.
+template <class T, class Op, uint32_t M3, uint32_t M2, int W>
+__global__ void __launch_bounds__(TPB) k_gather_all3_off(Batch B, Vert<T> vert, const T* ramp, Params prm,
+                                                          FieldTab<T> tab, COutTab<T> obar, InTab<T> inbar) {
+  const int nblk = B.blk3[B.M];
+  for (int blk = logical_block(); blk < nblk; blk += (int)gridDim.x) {
+    Pos q;
+    bool live = pos3_at(B, blk, q);
+    live = live && !in_band(q, W);
+    gather3_at<T, Op, M3, M2, 1>(q, live, vert, ramp, prm, tab, obar, inbar);
+  }
+}
+template <class T, class Op, uint32_t M3, uint32_t M2, int W>
+__global__ void __launch_bounds__(TPB) k_gather_all3_band(Batch B, Vert<T> vert, const T* ramp, Params prm,
+                                                           FieldTab<T> tab, COutTab<T> obar, InTab<T> inbar) {
+  const int n = B.nband[W] * KB;   // 16 lanes per listed column
+  for (int base = blockIdx.x * TPB; base < n; base += (int)gridDim.x * TPB) {   // uniform over the workgroup
+    const int t = base + (int)threadIdx.x;
+    const bool live = t < n;
+    Pos q{};
+    if (live) {
+      const int2 e = B.band[t >> 4];
+      const Member mm = B.mem[e.x];
+      q.m = e.x; q.im = mm.im; q.jm = mm.jm; q.o2 = mm.off2; q.o3 = mm.off3;
+      q.i = e.y / mm.jm;
+      q.j = e.y - q.i * mm.jm;
+      q.k = t & (KB - 1);
+    }
+    gather3_at<T, Op, M3, M2, 2>(q, live, vert, ramp, prm, tab, obar, inbar);
   }
 }
 
@@ -294,9 +373,16 @@ __global__ void __launch_bounds__(TPB) k_gather_all2(Batch B, Vert<T> vert, cons
       });
     });
   }
+  const uint32_t li = lix2(q, q.i, q.j);
+  T old[Op::NIN];   // every old value read before the first store (as gather3_at)
   static_for<0, Op::NIN - 1>([&](auto Fc) {
     constexpr int f = decltype(Fc)::value;
-    if (inbar.o[f]) put_in(inbar, f, q.o2, lix2(q, q.i, q.j), acc[f]);
+    old[f] = T(0);
+    if (inbar.o[f] && !((inbar.assign >> f) & 1u)) old[f] = ld_at(inbar.o[f], q.o2, li);
+  });
+  static_for<0, Op::NIN - 1>([&](auto Fc) {
+    constexpr int f = decltype(Fc)::value;
+    if (inbar.o[f]) ref_at(inbar.o[f], q.o2, li) = old[f] + acc[f];
   });
   }
 }

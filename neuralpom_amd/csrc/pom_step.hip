@@ -1609,7 +1609,7 @@ template <class T> struct Adj {
 template <class T> Ctx<T> make_ctx(const pom_batch* b, const pom_params_t* p, const void* ramp, const void* vg,
                                     hipStream_t st) {
   Ctx<T> c{};
-  c.B = Batch{b->d_mem, b->d_blk3, b->d_blk2, b->M};
+  c.B = batch_view(b);
   c.vert = Vert<T>{static_cast<const T*>(vg)};
   c.ramp = static_cast<const T*>(ramp);
   Params& q = c.prm;
@@ -2428,6 +2428,27 @@ int pom_batch_create(const pom_member_t* members, int32_t n, int32_t dtype, pom_
     pom_batch_destroy(b);
     return POM_E_HIP;
   }
+  // edge-band columns, by distance to the nearest edge (Batch::band)
+  std::vector<int2> bucket[BAND_W];
+  for (int m = 0; m < n; ++m) {
+    const int im = hm[m].im, jm = hm[m].jm;
+    for (int i = 0; i < im; ++i)
+      for (int j = 0; j < jm; ++j) {
+        const int d = std::min(std::min(i, im - 1 - i), std::min(j, jm - 1 - j));
+        if (d < BAND_W) bucket[d].push_back(make_int2(m, i * jm + j));
+      }
+  }
+  std::vector<int2> band;
+  b->nband[0] = 0;
+  for (int d = 0; d < BAND_W; ++d) {
+    band.insert(band.end(), bucket[d].begin(), bucket[d].end());
+    b->nband[d + 1] = (int32_t)band.size();
+  }
+  if (hipMalloc(&b->d_band, sizeof(int2) * band.size()) != hipSuccess ||
+      hipMemcpy(b->d_band, band.data(), sizeof(int2) * band.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    pom_batch_destroy(b);
+    return POM_E_HIP;
+  }
   *out = b;
   return POM_OK;
 }
@@ -2488,6 +2509,7 @@ int pom_batch_destroy(pom_batch* b) {
   (void)hipFree(b->d_mem);
   (void)hipFree(b->d_blk3);
   (void)hipFree(b->d_blk2);
+  (void)hipFree(b->d_band);
   delete b;
   return POM_OK;
 }
