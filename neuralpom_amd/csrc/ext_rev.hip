@@ -66,168 +66,165 @@ template <class T> struct Mom {
 
 }  // namespace
 
+// factor planes of the outputs uaf(p), vaf(p) (pom_extrev.h)
 template <class T>
-__global__ void __launch_bounds__(TPB) k_ext_rev_uv(Batch B, const T* ramp, Params prm, ExtRevArgs<T> a) {
+__global__ void __launch_bounds__(TPB) k_ext_rev_fac(Batch B, const T* ramp, Params prm, ExtRevFacArgs<T> a) {
   const int nblk = B.blk2[B.M];
   const T f = T(0.5 * prm.smoth);
-  const T f12 = T(1) + f * T(-2);   // d uab_new / d ua
   const bool acc = prm.iext != prm.isplit;
-  const T isp2i = T(prm.isp2i), ispi = T(prm.ispi);
+  const T isp2i = T(prm.isp2i);
   const T al = T(prm.alpha), al12 = T(1.0 - 2.0 * prm.alpha);
   const T dte4 = T(4.0 * prm.dte), g4 = T(0.25 * prm.grav);
+  const T rfe = T(prm.rfe), rfw = T(prm.rfw), rfn = T(prm.rfn), rfs = T(prm.rfs);
   for (int blk = logical_block(); blk < nblk; blk += (int)gridDim.x) {
     Pos q;
     if (!pos2_at(B, blk, q)) continue;
     const L2<T> L{q};
-    const int im = q.im, jm = q.jm, qi = q.i, qj = q.j;
+    const int im = q.im, jm = q.jm, i = q.i, j = q.j;
     const T rmp = ramp[q.m];
-    // total cotangent of uaf (vaf) at (i, j): the next substep's, which read
-    // it as its given uaf, plus the filter's (ua = uaf, uab, utf)
-    auto ctu = [&](int i, int j) {
-      const T c = L(a.c_uaf, i, j) + L(a.c_ua, i, j) + f * L(a.c_uab, i, j);
-      const T t = L(a.c_utf, i, j) * ((L(a.h, i, j) + L(a.elf, i, j)) + (L(a.h, i - 1, j) + L(a.elf, i - 1, j))) * isp2i;
-      return (acc && i >= 1) ? c + t : c;
-    };
-    auto ctv = [&](int i, int j) {
-      const T c = L(a.c_vaf, i, j) + L(a.c_va, i, j) + f * L(a.c_vab, i, j);
-      const T t = L(a.c_vtf, i, j) * ((L(a.h, i, j) + L(a.elf, i, j)) + (L(a.h, i, j - 1) + L(a.elf, i, j - 1))) * isp2i;
-      return (acc && j >= 1) ? c + t : c;
-    };
-    auto mom = [&](int i, int j, int di, int dj, T ct, const T* ar_f, const T* dm_f, const T* ds_f, const T* out) {
+    auto sel = [](bool c, T x) { return c ? x : T(0); };
+    const uint32_t ix = lix2(q, i, j);
+    auto ld = [&](const T* p) { return ld_at(p, q.o2, ix); };
+    // interior partials of the output at (i, j), neighbour (i - di, j - dj)
+    auto mom = [&](int di, int dj, T ct, T ar, T dm, const T* ds_f, T out) {
       Mom<T> r;
-      const T ar = L(ar_f, i, j);
-      const T s = ct * L(dm_f, i, j);
-      const T den = (L(a.h, i, j) + L(a.elf, i, j) + L(a.h, i - di, j - dj) + L(a.elf, i - di, j - dj)) * ar;
+      const T den = (ld(a.h) + ld(a.elf) + L(a.h, i - di, j - dj) + L(a.elf, i - di, j - dj)) * ar;
       const T rden = rcp_lane(den + T(1e-20));
-      r.g = s * rden;
+      r.g = ct * dm * rden;
       r.cgt = -dte4 * r.g;
-      r.G = g4 * (L(ds_f, i, j) + L(ds_f, i - di, j - dj));
-      r.D = L(a.d, i, j) + L(a.d, i - di, j - dj);
-      r.E = al12 * (L(a.el, i, j) - L(a.el, i - di, j - dj)) +
-            al * (L(a.elb, i, j) - L(a.elb, i - di, j - dj) + L(a.elf, i, j) - L(a.elf, i - di, j - dj)) +
-            (L(a.eatm, i, j) - L(a.eatm, i - di, j - dj));
+      r.G = g4 * (ld(ds_f) + L(ds_f, i - di, j - dj));
+      r.D = ld(a.d) + L(a.d, i - di, j - dj);
+      r.E = al12 * (ld(a.el) - L(a.el, i - di, j - dj)) +
+            al * (ld(a.elb) - L(a.elb, i - di, j - dj) + ld(a.elf) - L(a.elf, i - di, j - dj)) +
+            (ld(a.eatm) - L(a.eatm, i - di, j - dj));
       // d out / d den = -dm num / den^2 = -(dm num / den) / den, the stored output over den
-      r.cden = -(ct * rden) * L(out, i, j) * ar;
+      r.cden = -(ct * rden) * out * ar;
       return r;
     };
+    const T h_q = ld(a.h), elf_q = ld(a.elf), elb_q = ld(a.elb);
+    T fv[XF_N];
+    // ---- uaf(p): total cotangent = the next substep's (its given uaf) + the filter's (ua, uab, utf)
+    {
+      const int k = ukind(i, j, im, jm);
+      const T ar = ld(a.aru), dm = ld(a.dum), uaf = ld(a.uaf);
+      const T tu = ld(a.c_utf) * uaf * isp2i;   // d utf / d elf(p) = d utf / d elf(p-i), times c_utf
+      const T ct = ld(a.c_uaf) + ld(a.c_ua) + f * ld(a.c_uab) +
+                   sel(acc && i >= 1, ld(a.c_utf) * ((h_q + elf_q) + (L(a.h, i - 1, j) + L(a.elf, i - 1, j))) * isp2i);
+      const Mom<T> P = mom(1, 0, ct, ar, dm, a.dy, uaf);
+      const bool in = k == K_INT;
+      const T s = ct * dm;
+      fv[XF_XU] = sel(in, P.g * ar * ld(a.uab));
+      fv[XF_YU] = sel(in, P.cgt * P.G * P.D);
+      fv[XF_ZU] = sel(in, P.cden) + sel(acc && i >= 1, tu);
+      fv[XF_WU] = sel(in, P.cgt * P.G * P.E);
+      fv[XF_CU] = sel(in, -ar * P.cgt);
+      // radiation: east form reads el(i-1, j), west forms el(1, j)
+      fv[XF_RU] = sel(k == K_EDGE_HI && prm.rfe != 0.0, rfe * cwave(L(a.h, i - 1, j), prm.grav) * rmp * s) +
+                  sel(k == K_EDGE_1 && prm.rfw != 0.0, -(rfw * cwave(h_q, prm.grav)) * rmp * s) +
+                  sel(k == K_EDGE_0 && prm.rfw != 0.0, -(rfw * cwave(L(a.h, i + 1, j), prm.grav)) * rmp * s);
+      st2(q, a.p_uab, ix, f * ld(a.c_uab) + sel(in, P.g * (h_q + elb_q + L(a.h, i - 1, j) + L(a.elb, i - 1, j)) * ar));
+      st2(q, a.p_uaf, ix, sel(k == K_PASS, s));
+      const T o = ld(a.a_advua);
+      st2(q, a.a_advua, ix, o + sel(in, P.cgt));
+    }
+    // ---- vaf(p)
+    {
+      const int k = vkind(i, j, im, jm);
+      const T ar = ld(a.arv), dm = ld(a.dvm), vaf = ld(a.vaf);
+      const T tv = ld(a.c_vtf) * vaf * isp2i;
+      const T ct = ld(a.c_vaf) + ld(a.c_va) + f * ld(a.c_vab) +
+                   sel(acc && j >= 1, ld(a.c_vtf) * ((h_q + elf_q) + (L(a.h, i, j - 1) + L(a.elf, i, j - 1))) * isp2i);
+      const Mom<T> P = mom(0, 1, ct, ar, dm, a.dx, vaf);
+      const bool in = k == K_INT;
+      const T s = ct * dm;
+      fv[XF_XV] = sel(in, P.g * ar * ld(a.vab));
+      fv[XF_YV] = sel(in, P.cgt * P.G * P.D);
+      fv[XF_ZV] = sel(in, P.cden) + sel(acc && j >= 1, tv);
+      fv[XF_WV] = sel(in, P.cgt * P.G * P.E);
+      fv[XF_CV] = sel(in, ar * P.cgt);
+      // radiation: north form reads el(i, j-1), south forms el(i, 1)
+      fv[XF_RV] = sel(k == K_EDGE_HI && prm.rfn != 0.0, rfn * cwave(L(a.h, i, j - 1), prm.grav) * rmp * s) +
+                  sel(k == K_EDGE_1 && prm.rfs != 0.0, -(rfs * cwave(h_q, prm.grav)) * rmp * s) +
+                  sel(k == K_EDGE_0 && prm.rfs != 0.0, -(rfs * cwave(L(a.h, i, j + 1), prm.grav)) * rmp * s);
+      st2(q, a.p_vab, ix, f * ld(a.c_vab) + sel(in, P.g * (h_q + elb_q + L(a.h, i, j - 1) + L(a.elb, i, j - 1)) * ar));
+      st2(q, a.p_vaf, ix, sel(k == K_PASS, s));
+      const T o = ld(a.a_advva);
+      st2(q, a.a_advva, ix, o + sel(in, P.cgt));
+    }
+#pragma unroll
+    for (int n = 0; n < XF_N; ++n) ref_at(a.fac, q.o2 + n * a.n2, ix) = fv[n];
+  }
+}
 
-    // Every candidate output is evaluated (clamped reads) and its terms are
-    // selected by the form the output takes: with no branch around them, the
-    // loads of all candidates are in flight together.
+// sums of the factor planes over the outputs that read q, and the filter's own terms
+template <class T>
+__global__ void __launch_bounds__(TPB) k_ext_rev_gat(Batch B, Params prm, ExtRevGatArgs<T> a) {
+  const int nblk = B.blk2[B.M];
+  const T f = T(0.5 * prm.smoth);
+  const T f12 = T(1) + f * T(-2);   // d uab_new / d ua
+  const bool acc = prm.iext != prm.isplit;
+  const T ispi = T(prm.ispi);
+  const T al = T(prm.alpha), al12 = T(1.0 - 2.0 * prm.alpha);
+  for (int blk = logical_block(); blk < nblk; blk += (int)gridDim.x) {
+    Pos q;
+    if (!pos2_at(B, blk, q)) continue;
+    const L2<T> L{q};
+    const int im = q.im, jm = q.jm, i = q.i, j = q.j;
     auto sel = [](bool c, T x) { return c ? x : T(0); };
-    const T aru_q = L(a.aru, qi, qj), arv_q = L(a.arv, qi, qj);
-    const T cor_q = L(a.cor, qi, qj), d_q = L(a.d, qi, qj), h_q = L(a.h, qi, qj);
-    const T cw_q = cwave(h_q, prm.grav);
-    const T rfe = T(prm.rfe), rfw = T(prm.rfw), rfn = T(prm.rfn), rfs = T(prm.rfs);
-    T r_ua, r_uab, r_va, r_vab, r_el, r_elb, r_elf, r_d, r_uafin, r_vafin, cg_u, cg_v;
-
-    // ---- uaf outputs that read q: at q, q+i (q is its west neighbour), q-j
-    // and q+i-j (va(q) in their Coriolis terms), q-i (west form at i = 0)
-    {
-      const int k0 = ukind(qi, qj, im, jm), k1 = ukind(qi + 1, qj, im, jm);
-      const int k2 = ukind(qi, qj - 1, im, jm), k3 = ukind(qi + 1, qj - 1, im, jm);
-      const int k4 = ukind(qi - 1, qj, im, jm);
-      const T ct0 = ctu(qi, qj), ct1 = ctu(qi + 1, qj);
-      const Mom<T> P0 = mom(qi, qj, 1, 0, ct0, a.aru, a.dum, a.dy, a.uaf);
-      const Mom<T> P1 = mom(qi + 1, qj, 1, 0, ct1, a.aru, a.dum, a.dy, a.uaf);
-      const Mom<T> P2 = mom(qi, qj - 1, 1, 0, ctu(qi, qj - 1), a.aru, a.dum, a.dy, a.uaf);
-      const Mom<T> P3 = mom(qi + 1, qj - 1, 1, 0, ctu(qi + 1, qj - 1), a.aru, a.dum, a.dy, a.uaf);
-      const T s0 = ct0 * L(a.dum, qi, qj), s1 = ct1 * L(a.dum, qi + 1, qj);
-      const T s4 = ctu(qi - 1, qj) * L(a.dum, qi - 1, qj);
-      const T ar1 = L(a.aru, qi + 1, qj);
-      const T cv0 = T(0.25) * cor_q * (L(a.va, qi, qj + 1) + L(a.va, qi, qj));
-      const T cd_s = T(0.25) * L(a.cor, qi, qj - 1) * L(a.d, qi, qj - 1);
-      const bool i0 = k0 == K_INT, i1 = k1 == K_INT;
-      const T gdd0 = P0.cgt * P0.G * P0.D, gdd1 = P1.cgt * P1.G * P1.D;
-      const T cct0 = -aru_q * P0.cgt, cct1 = -ar1 * P1.cgt;
-      r_uab = sel(i0, P0.g * (h_q + L(a.elb, qi, qj) + L(a.h, qi - 1, qj) + L(a.elb, qi - 1, qj)) * aru_q);
-      r_elb = sel(i0, P0.g * aru_q * L(a.uab, qi, qj) + gdd0 * al) + sel(i1, P1.g * ar1 * L(a.uab, qi + 1, qj) - gdd1 * al);
-      r_el = sel(i0, gdd0 * al12) - sel(i1, gdd1 * al12) +
-             sel(k0 == K_EDGE_1 && prm.rfw != 0.0, -(rfw * cw_q) * rmp * s0) + sel(k1 == K_EDGE_HI && prm.rfe != 0.0, rfe * cw_q * rmp * s1) +
-             sel(k4 == K_EDGE_0 && prm.rfw != 0.0, -(rfw * cw_q) * rmp * s4);
-      r_elf = sel(i0, gdd0 * al + P0.cden) + sel(i1, P1.cden - gdd1 * al);
-      r_d = sel(i0, P0.cgt * P0.G * P0.E + cct0 * cv0) + sel(i1, P1.cgt * P1.G * P1.E + cct1 * cv0);
-      r_va = sel(i0, cct0 * T(0.25) * cor_q * d_q) + sel(i1, cct1 * T(0.25) * cor_q * d_q) +
-             sel(k2 == K_INT, -L(a.aru, qi, qj - 1) * P2.cgt * cd_s) +
-             sel(k3 == K_INT, -L(a.aru, qi + 1, qj - 1) * P3.cgt * cd_s);
-      r_uafin = sel(k0 == K_PASS, s0);
-      cg_u = sel(i0, P0.cgt);
-    }
-    // ---- vaf outputs that read q: at q, q+j (q is its south neighbour), q-i
-    // and q-i+j (ua(q) in their Coriolis terms), q-j (south form at j = 0)
-    {
-      const int k0 = vkind(qi, qj, im, jm), k1 = vkind(qi, qj + 1, im, jm);
-      const int k2 = vkind(qi - 1, qj, im, jm), k3 = vkind(qi - 1, qj + 1, im, jm);
-      const int k4 = vkind(qi, qj - 1, im, jm);
-      const T ct0 = ctv(qi, qj), ct1 = ctv(qi, qj + 1);
-      const Mom<T> P0 = mom(qi, qj, 0, 1, ct0, a.arv, a.dvm, a.dx, a.vaf);
-      const Mom<T> P1 = mom(qi, qj + 1, 0, 1, ct1, a.arv, a.dvm, a.dx, a.vaf);
-      const Mom<T> P2 = mom(qi - 1, qj, 0, 1, ctv(qi - 1, qj), a.arv, a.dvm, a.dx, a.vaf);
-      const Mom<T> P3 = mom(qi - 1, qj + 1, 0, 1, ctv(qi - 1, qj + 1), a.arv, a.dvm, a.dx, a.vaf);
-      const T s0 = ct0 * L(a.dvm, qi, qj), s1 = ct1 * L(a.dvm, qi, qj + 1);
-      const T s4 = ctv(qi, qj - 1) * L(a.dvm, qi, qj - 1);
-      const T ar1 = L(a.arv, qi, qj + 1);
-      const T cu0 = T(0.25) * cor_q * (L(a.ua, qi + 1, qj) + L(a.ua, qi, qj));
-      const T cd_w = T(0.25) * L(a.cor, qi - 1, qj) * L(a.d, qi - 1, qj);
-      const bool i0 = k0 == K_INT, i1 = k1 == K_INT;
-      const T gdd0 = P0.cgt * P0.G * P0.D, gdd1 = P1.cgt * P1.G * P1.D;
-      const T cct0 = arv_q * P0.cgt, cct1 = ar1 * P1.cgt;
-      r_vab = sel(i0, P0.g * (h_q + L(a.elb, qi, qj) + L(a.h, qi, qj - 1) + L(a.elb, qi, qj - 1)) * arv_q);
-      r_elb += sel(i0, P0.g * arv_q * L(a.vab, qi, qj) + gdd0 * al) + sel(i1, P1.g * ar1 * L(a.vab, qi, qj + 1) - gdd1 * al);
-      r_el += sel(i0, gdd0 * al12) - sel(i1, gdd1 * al12) +
-              sel(k0 == K_EDGE_1 && prm.rfs != 0.0, -(rfs * cw_q) * rmp * s0) + sel(k1 == K_EDGE_HI && prm.rfn != 0.0, rfn * cw_q * rmp * s1) +
-              sel(k4 == K_EDGE_0 && prm.rfs != 0.0, -(rfs * cw_q) * rmp * s4);
-      r_elf += sel(i0, gdd0 * al + P0.cden) + sel(i1, P1.cden - gdd1 * al);
-      r_d += sel(i0, P0.cgt * P0.G * P0.E + cct0 * cu0) + sel(i1, P1.cgt * P1.G * P1.E + cct1 * cu0);
-      r_ua = sel(i0, cct0 * T(0.25) * cor_q * d_q) + sel(i1, cct1 * T(0.25) * cor_q * d_q) +
-             sel(k2 == K_INT, L(a.arv, qi - 1, qj) * P2.cgt * cd_w) +
-             sel(k3 == K_INT, L(a.arv, qi - 1, qj + 1) * P3.cgt * cd_w);
-      r_vafin = sel(k0 == K_PASS, s0);
-      cg_v = sel(i0, P0.cgt);
-    }
-
-    // ---- filter, level shift and time means at q (OpExtFilter)
-    const uint32_t ix = lix2(q, qi, qj);
+    auto F = [&](int n, int ii, int jj) {
+      return ld_at(a.fac, q.o2 + n * a.n2, lix2(q, clampi(ii, im), clampi(jj, jm)));
+    };
+    const uint32_t ix = lix2(q, i, j);
     auto ld = [&](const T* p) { return ld_at(p, q.o2, ix); };
+    // factors of the outputs that read q; a neighbour outside the member has
+    // no output, and its clamped read is discarded by the existence test
+    const bool ie = i + 1 <= im - 1, iw = i >= 1, jn = j + 1 <= jm - 1, js = j >= 1;
+    const T YU0 = F(XF_YU, i, j), YU1 = sel(ie, F(XF_YU, i + 1, j));
+    const T YV0 = F(XF_YV, i, j), YV1 = sel(jn, F(XF_YV, i, j + 1));
+    const T XU = F(XF_XU, i, j) + sel(ie, F(XF_XU, i + 1, j));
+    const T XV = F(XF_XV, i, j) + sel(jn, F(XF_XV, i, j + 1));
+    const T ZU = F(XF_ZU, i, j) + sel(ie, F(XF_ZU, i + 1, j));
+    const T ZV = F(XF_ZV, i, j) + sel(jn, F(XF_ZV, i, j + 1));
+    const T WU = F(XF_WU, i, j) + sel(ie, F(XF_WU, i + 1, j));
+    const T WV = F(XF_WV, i, j) + sel(jn, F(XF_WV, i, j + 1));
+    const T CU0 = F(XF_CU, i, j) + sel(ie, F(XF_CU, i + 1, j));                       // va(q), d(q)
+    const T CUs = sel(js, F(XF_CU, i, j - 1)) + sel(js && ie, F(XF_CU, i + 1, j - 1));   // va(q) via cor d at q-j
+    const T CV0 = F(XF_CV, i, j) + sel(jn, F(XF_CV, i, j + 1));                       // ua(q), d(q)
+    const T CVw = sel(iw, F(XF_CV, i - 1, j)) + sel(iw && jn, F(XF_CV, i - 1, j + 1));   // ua(q) via cor d at q-i
+    const T rad = sel(ukind(i, j, im, jm) == K_EDGE_1, F(XF_RU, i, j)) +
+                  sel(ukind(i + 1, j, im, jm) == K_EDGE_HI, F(XF_RU, i + 1, j)) +
+                  sel(ukind(i - 1, j, im, jm) == K_EDGE_0, F(XF_RU, i - 1, j)) +
+                  sel(vkind(i, j, im, jm) == K_EDGE_1, F(XF_RV, i, j)) +
+                  sel(vkind(i, j + 1, im, jm) == K_EDGE_HI, F(XF_RV, i, j + 1)) +
+                  sel(vkind(i, j - 1, im, jm) == K_EDGE_0, F(XF_RV, i, j - 1));
+    const T qa = T(0.25);
+    const T cor_q = ld(a.cor), d_q = ld(a.d);
+    const T cd_q = qa * cor_q * d_q, cd_s = qa * L(a.cor, i, j - 1) * L(a.d, i, j - 1),
+            cd_w = qa * L(a.cor, i - 1, j) * L(a.d, i - 1, j);
+    const T vsum = L(a.va, i, j + 1) + ld(a.va), usum = L(a.ua, i + 1, j) + ld(a.ua);
     const T c_uab = ld(a.c_uab), c_vab = ld(a.c_vab), c_elb = ld(a.c_elb);
-    T celf = (a.elf_first ? T(0) : ld(a.c_elf)) + ld(a.c_el) + f * c_elb + ld(a.c_d) + r_elf;
-    {
-      const T t0 = ld(a.c_egf) * ispi, t1 = ld(a.c_utf) * ld(a.uaf) * isp2i;
-      const T t2 = L(a.c_utf, qi + 1, qj) * L(a.uaf, qi + 1, qj) * isp2i, t3 = ld(a.c_vtf) * ld(a.vaf) * isp2i;
-      const T t4 = L(a.c_vtf, qi, qj + 1) * L(a.vaf, qi, qj + 1) * isp2i;
-      celf += sel(acc, t0) + sel(acc && qi >= 1, t1) + sel(acc && qi + 1 <= im - 1, t2) + sel(acc && qj >= 1, t3) +
-              sel(acc && qj + 1 <= jm - 1, t4);
-    }
+    const T dY = (YU0 - YU1) + (YV0 - YV1);
+    // filter's own terms of elf: el' = elf, elb' (f), d' = h + elf, egf' (ispi), etf'
+    T celf = (a.elf_first ? T(0) : ld(a.c_elf)) + ld(a.c_el) + f * c_elb + ld(a.c_d) + sel(acc, ld(a.c_egf) * ispi) +
+             ZU + ZV + al * dY;
     const int n = prm.iext, ns = prm.isplit;
+    const T ce = ld(a.c_etf);
     if (n == ns - 2) {
-      celf += T(0.25 * prm.smoth) * ld(a.c_etf);
+      celf += T(0.25 * prm.smoth) * ce;
       st2(q, a.c_etf, ix, T(0));
     } else if (n == ns - 1) {
-      celf += T(0.5 * (1.0 - 0.5 * prm.smoth)) * ld(a.c_etf);
+      celf += T(0.5 * (1.0 - 0.5 * prm.smoth)) * ce;
     } else if (n == ns) {
-      const T ce = ld(a.c_etf), fs = ld(a.fsm);
+      const T fs = ld(a.fsm);
       celf += T(0.5) * ce * fs;
       st2(q, a.c_etf, ix, ce * fs);
     }
     st2(q, a.c_elf, ix, celf);
-    st2(q, a.p_ua, ix, f12 * c_uab + r_ua);
-    st2(q, a.p_uab, ix, f * c_uab + r_uab);
-    st2(q, a.p_va, ix, f12 * c_vab + r_va);
-    st2(q, a.p_vab, ix, f * c_vab + r_vab);
-    st2(q, a.p_el, ix, f12 * c_elb + r_el);
-    st2(q, a.p_elb, ix, f * c_elb + r_elb);
-    st2(q, a.p_d, ix, r_d);
-    st2(q, a.p_uaf, ix, r_uafin);
-    st2(q, a.p_vaf, ix, r_vafin);
-    // forcing cotangents: prov's partials 1 (adx, adv, drx) and -ar (wbot)
-    st2(q, a.a_advua, ix, ld(a.a_advua) + cg_u);
-    st2(q, a.a_adx2d, ix, ld(a.a_adx2d) + cg_u);
-    st2(q, a.a_drx2d, ix, ld(a.a_drx2d) + cg_u);
-    st2(q, a.a_wubot, ix, ld(a.a_wubot) + (-aru_q) * cg_u);
-    st2(q, a.a_advva, ix, ld(a.a_advva) + cg_v);
-    st2(q, a.a_ady2d, ix, ld(a.a_ady2d) + cg_v);
-    st2(q, a.a_dry2d, ix, ld(a.a_dry2d) + cg_v);
-    st2(q, a.a_wvbot, ix, ld(a.a_wvbot) + (-arv_q) * cg_v);
+    st2(q, a.p_ua, ix, f12 * c_uab + CV0 * cd_q + CVw * cd_w);
+    st2(q, a.p_va, ix, f12 * c_vab + CU0 * cd_q + CUs * cd_s);
+    st2(q, a.p_el, ix, f12 * c_elb + al12 * dY + rad);
+    st2(q, a.p_elb, ix, f * c_elb + XU + XV + al * dY);
+    st2(q, a.p_d, ix, WU + WV + CU0 * (qa * cor_q * vsum) + CV0 * (qa * cor_q * usum));
   }
 }
 
@@ -269,26 +266,63 @@ __global__ void __launch_bounds__(TPB) k_ext_rev_elf(Batch B, Params prm, ExtRev
                  sel(qj + 1 <= jm - 1, cfv1 * T(0.25) * (L(a.dx, qi, qj + 1) + dx_q) * L(a.va, qi, qj + 1));
     const uint32_t ix = lix2(q, qi, qj);
     auto ld = [&](const T* p) { return ld_at(p, q.o2, ix); };
-    st2(q, a.p_ua, ix, ld(a.p_ua) + cua);
-    st2(q, a.p_va, ix, ld(a.p_va) + cva);
-    st2(q, a.p_d, ix, ld(a.p_d) + cd);
-    if (inner(qi, qj)) st2(q, a.p_elb, ix, ld(a.p_elb) + craw(qi, qj));   // the store alone is conditional
+    const T o_ua = ld(a.p_ua), o_va = ld(a.p_va), o_d = ld(a.p_d), o_elb = ld(a.p_elb);   // reads before stores
+    st2(q, a.p_ua, ix, o_ua + cua);
+    st2(q, a.p_va, ix, o_va + cva);
+    st2(q, a.p_d, ix, o_d + cd);
+    if (inner(qi, qj)) st2(q, a.p_elb, ix, o_elb + craw(qi, qj));   // the store alone is conditional
   }
 }
 
-template <class T> void launch_ext_rev_uv(const Ctx<T>& c, const ExtRevArgs<T>& a) {
+template <class T>
+__global__ void __launch_bounds__(TPB) k_ext_fold(Batch B, ExtFoldArgs<T> a) {
+  const int nblk = B.blk2[B.M];
+  for (int blk = logical_block(); blk < nblk; blk += (int)gridDim.x) {
+    Pos q;
+    if (!pos2_at(B, blk, q)) continue;
+    const uint32_t ix = lix2(q, q.i, q.j);
+    T* const dst[6] = {a.c_adx, a.c_drx, a.c_wubot, a.c_ady, a.c_dry, a.c_wvbot};
+    const T su = ld_at(a.acc_u, q.o2, ix), sv = ld_at(a.acc_v, q.o2, ix);
+    const T add[6] = {su, su, -ld_at(a.aru, q.o2, ix) * su, sv, sv, -ld_at(a.arv, q.o2, ix) * sv};
+    T old[6];
+#pragma unroll
+    for (int n = 0; n < 6; ++n) old[n] = ld_at(dst[n], q.o2, ix);
+#pragma unroll
+    for (int n = 0; n < 6; ++n) ref_at(dst[n], q.o2, ix) = old[n] + add[n];
+    if (a.clear) {
+      ref_at(a.acc_u, q.o2, ix) = T(0);
+      ref_at(a.acc_v, q.o2, ix) = T(0);
+    }
+  }
+}
+
+template <class T> void launch_ext_rev_fac(const Ctx<T>& c, const ExtRevFacArgs<T>& a) {
   if (c.nblk2 == 0) return;
-  hipLaunchKernelGGL((k_ext_rev_uv<T>), dim3(groups(c.nblk2, c.tune.flat2_bpg, c.tune.flat2_min)), dim3(TPB), 0,
+  hipLaunchKernelGGL((k_ext_rev_fac<T>), dim3(groups(c.nblk2, c.tune.flat2_bpg, c.tune.flat2_min)), dim3(TPB), 0,
                      c.stream, c.B, c.ramp, c.prm, a);
+}
+template <class T> void launch_ext_rev_gat(const Ctx<T>& c, const ExtRevGatArgs<T>& a) {
+  if (c.nblk2 == 0) return;
+  hipLaunchKernelGGL((k_ext_rev_gat<T>), dim3(groups(c.nblk2, c.tune.flat2_bpg, c.tune.flat2_min)), dim3(TPB), 0,
+                     c.stream, c.B, c.prm, a);
 }
 template <class T> void launch_ext_rev_elf(const Ctx<T>& c, const ExtRevElfArgs<T>& a) {
   if (c.nblk2 == 0) return;
   hipLaunchKernelGGL((k_ext_rev_elf<T>), dim3(groups(c.nblk2, c.tune.flat2_bpg, c.tune.flat2_min)), dim3(TPB), 0,
                      c.stream, c.B, c.prm, a);
 }
+template <class T> void launch_ext_fold(const Ctx<T>& c, const ExtFoldArgs<T>& a) {
+  if (c.nblk2 == 0) return;
+  hipLaunchKernelGGL((k_ext_fold<T>), dim3(groups(c.nblk2, c.tune.flat2_bpg, c.tune.flat2_min)), dim3(TPB), 0,
+                     c.stream, c.B, a);
+}
 
-template void launch_ext_rev_uv<float>(const Ctx<float>&, const ExtRevArgs<float>&);
-template void launch_ext_rev_uv<double>(const Ctx<double>&, const ExtRevArgs<double>&);
+template void launch_ext_rev_fac<float>(const Ctx<float>&, const ExtRevFacArgs<float>&);
+template void launch_ext_rev_fac<double>(const Ctx<double>&, const ExtRevFacArgs<double>&);
+template void launch_ext_rev_gat<float>(const Ctx<float>&, const ExtRevGatArgs<float>&);
+template void launch_ext_rev_gat<double>(const Ctx<double>&, const ExtRevGatArgs<double>&);
+template void launch_ext_fold<float>(const Ctx<float>&, const ExtFoldArgs<float>&);
+template void launch_ext_fold<double>(const Ctx<double>&, const ExtFoldArgs<double>&);
 template void launch_ext_rev_elf<float>(const Ctx<float>&, const ExtRevElfArgs<float>&);
 template void launch_ext_rev_elf<double>(const Ctx<double>&, const ExtRevElfArgs<double>&);
 

@@ -99,19 +99,16 @@ __global__ void __launch_bounds__(TPB) k_adj_advuv(Batch B, Vert<T> vert, Params
       const T dold_v = ((h_q + L2(a.h, qi, qj - 1)) + etb_q + L2(a.etb, qi, qj - 1)) * arv_q;
       const T ub_q = L3(a.ub, qi, qj, k), vb_q = L3(a.vb, qi, qj, k);
       const uint32_t ix = lix3(q, qi, qj, k);
-      auto acc3 = [&](T* f, T v) {
-        T& r = ref_at(f, q.o3, ix);
-        r = r + v;
-      };
-      acc3(a.c_advx, U0.crhs);
-      acc3(a.c_drhox, U0.crhs);
-      acc3(a.c_advy, V0.crhs);
-      acc3(a.c_drhoy, V0.crhs);
-      acc3(a.c_ub, U0.g * dold_u);
-      acc3(a.c_vb, V0.g * dold_v);
-      acc3(a.c_u, cu_q);
-      acc3(a.c_v, cv_q);
-      acc3(a.c_w, cw_q);
+      // accumulate: every old value read before the first store (the arrays may
+      // not be assumed distinct by the compiler, so it would not move a read
+      // above an earlier store)
+      T* const dst[9] = {a.c_advx, a.c_drhox, a.c_advy, a.c_drhoy, a.c_ub, a.c_vb, a.c_u, a.c_v, a.c_w};
+      const T add[9] = {U0.crhs, U0.crhs, V0.crhs, V0.crhs, U0.g * dold_u, V0.g * dold_v, cu_q, cv_q, cw_q};
+      T old[9];
+#pragma unroll
+      for (int n = 0; n < 9; ++n) old[n] = ld_at(dst[n], q.o3, ix);
+#pragma unroll
+      for (int n = 0; n < 9; ++n) ref_at(dst[n], q.o3, ix) = old[n] + add[n];
       // 2-D inputs: dt (Coriolis and pressure gradient), egf = egb, etf (dnew), etb (dold)
       const T egf_q = L2(a.egf, qi, qj), egb_q = L2(a.egb, qi, qj), ea_q = L2(a.eatm, qi, qj);
       const T dy_q = L2(a.dy, qi, qj), dx_q = L2(a.dx, qi, qj);
@@ -147,15 +144,13 @@ __global__ void __launch_bounds__(TPB) k_adj_advuv(Batch B, Vert<T> vert, Params
     }
     if (live && k == 0) {
       const uint32_t ix = lix2(q, q.i, q.j);
-      auto acc2 = [&](T* f, T v) {
-        T& r = ref_at(f, q.o2, ix);
-        r = r + v;
-      };
-      acc2(a.c_dt, p_dt);
-      acc2(a.c_egf, p_eg);
-      acc2(a.c_egb, p_eg);
-      acc2(a.c_etf, p_etf);
-      acc2(a.c_etb, p_etb);
+      T* const dst[5] = {a.c_dt, a.c_egf, a.c_egb, a.c_etf, a.c_etb};
+      const T add[5] = {p_dt, p_eg, p_eg, p_etf, p_etb};
+      T old[5];
+#pragma unroll
+      for (int n = 0; n < 5; ++n) old[n] = ld_at(dst[n], q.o2, ix);
+#pragma unroll
+      for (int n = 0; n < 5; ++n) ref_at(dst[n], q.o2, ix) = old[n] + add[n];
     }
   }
 }

@@ -3,19 +3,20 @@
 // Forward substep n (explicit/time_loop.py:292-431): el_raw = continuity
 // (OpExtElfRaw), elf = bcond 1 of el_raw (OpBc1), uaf / vaf = depth-mean
 // momentum with bcond 2 (OpExtUV), then the filter, level shift and time
-// means (OpExtFilter).  Its reverse runs in two launches instead of one
+// means (OpExtFilter).  Its reverse runs in three launches instead of one
 // generic gather per operator (each of which re-evaluates the operator's
 // point function in dual numbers once per stencil offset):
 //
-//   k_ext_rev_uv   filter and depth-mean momentum.  The thread of point q
-//                  takes the partials of the outputs that read q: uaf at q,
-//                  q+i, q-j, q+i-j (and the west edge form at q-i), vaf at q,
-//                  q+j, q-i, q-i+j (and the south edge form at q-j), the
-//                  filter's outputs at q and the time means at q+i, q+j.  The
-//                  filter's cotangents of uaf / vaf are recomputed at each of
-//                  those points from the output cotangents.  Writes every
-//                  input cotangent of the substep once (assign), completes
-//                  the cotangent of elf, and accumulates the forcing terms.
+//   k_ext_rev_fac  per point p, the partials of the outputs uaf(p) and
+//                  vaf(p), each taken once, as 12 factor planes (below); the
+//                  cotangents that only p's own outputs reach (uab, vab, the
+//                  given uaf / vaf) are written directly, and the forcing
+//                  partial into the advave-period accumulator.
+//   k_ext_rev_gat  per point q, the sums of the factors of the outputs that
+//                  read q (uaf at q, q+i, q-j, q+i-j, q-i; vaf at q, q+j, q-i,
+//                  q-i+j, q-j) and the filter's own terms: the input
+//                  cotangents ua va el elb d (assigned) and the complete
+//                  cotangent of elf.
 //   k_ext_rev_elf  bcond 1 and continuity: el_raw's cotangent at q and its
 //                  four neighbours (the edge copies folded onto the interior
 //                  point they copy), then the face-flux partials.
@@ -23,38 +24,70 @@
 // The cotangents of egf, utf and vtf pass through a substep unchanged (each is
 // its own previous value plus a term), so both substep buffers name one array
 // and nothing is copied; etf's is scaled in place on the three substeps that
-// change it.  Every element is written by one thread: deterministic.
+// change it.  The forcing terms adx2d, drx2d, wubot (ady2d, dry2d, wvbot)
+// enter uaf (vaf) as advua (advva) does, so their cotangents are the sums of
+// the advave-period accumulators, folded in (k_ext_fold) when a period's
+// advave call consumes its accumulator.  Every element is written by one
+// thread: deterministic.
 #pragma once
 #include "pom_kernels.h"
 
 namespace pom {
 
-template <class T> struct ExtRevArgs {
-  // primal: the substep's inputs (state n-1), its elf (after bcond 1), uaf, vaf
-  const T *el, *elb, *d, *ua, *va, *uab, *vab;
-  const T *elf, *uaf, *vaf;
-  // geometry and the substep forcing that enters uaf / vaf as given fields
-  const T *cor, *dx, *dy, *h, *aru, *arv, *dum, *dvm, *eatm, *fsm;
-  // cotangents of the substep's outputs
-  const T *c_ua, *c_uab, *c_va, *c_vab, *c_el, *c_elb, *c_d, *c_egf, *c_utf, *c_vtf, *c_uaf, *c_vaf;
+// factor planes of the momentum outputs at a point p (zero where the form
+// does not apply), as offsets into one array of 12 planes of n2 elements:
+// per direction (u: e = i, v: e = j; ar = aru / arv, mask dum / dvm)
+//   X = g ar uab(p)             elb(p) and elb(p-e)
+//   Y = cgt G D                 el (+-(1-2 alpha)), elb and elf (+-alpha) at p, p-e
+//   Z = cden + time-mean term   elf(p) and elf(p-e)
+//   W = cgt G E                 d(p) and d(p-e)
+//   C = -+ ar cgt               the Coriolis partials, scaled by cor d / 4 or cor (vel sum) / 4
+//   R = radiation partial       el at p (forms at index 1), p - e (the far edge) or p + e (index 0)
+// with g = ct dm / (den + 1e-20), cgt = -4 dte g, den, G, D, E, cden as in OpExtUVG.
+enum { XF_XU, XF_YU, XF_ZU, XF_WU, XF_CU, XF_RU, XF_XV, XF_YV, XF_ZV, XF_WV, XF_CV, XF_RV, XF_N };
+
+template <class T> struct ExtRevFacArgs {
+  // cotangents of the substep's outputs read at the point
+  const T *c_uaf, *c_ua, *c_uab, *c_utf, *c_vaf, *c_va, *c_vab, *c_vtf;
+  // primal: the substep's inputs (state n-1), its elf (after bcond 1), uaf, vaf; geometry
+  const T *h, *elf, *aru, *arv, *dum, *dvm, *dx, *dy, *d, *el, *elb, *eatm, *uab, *vab, *uaf, *vaf;
+  T* fac;                   // XF_N planes of n2
+  int64_t n2;
+  T *p_uab, *p_vab, *p_uaf, *p_vaf;   // input cotangents reached from the point's own outputs only (assigned)
+  T *a_advua, *a_advva;     // advave-period accumulators
+};
+
+template <class T> struct ExtRevGatArgs {
+  const T* fac;
+  int64_t n2;
+  const T *cor, *d, *ua, *va, *fsm;   // primal (state n-1) and geometry
+  const T *c_uab, *c_vab, *c_elb, *c_el, *c_d, *c_egf;
   T* c_etf;   // in: of the output etf; out: of the input etf (in place)
   T* c_elf;   // of this substep's elf: completed here (assigned when elf_first)
-  // cotangents of the substep's inputs (assigned)
-  T *p_ua, *p_uab, *p_va, *p_vab, *p_el, *p_elb, *p_d, *p_uaf, *p_vaf;
-  // accumulated over the substeps: advua / advva (the carried advave fields)
-  // and the step's depth-mean forcing adx2d, ady2d, drx2d, dry2d, wubot, wvbot
-  T *a_advua, *a_advva, *a_adx2d, *a_ady2d, *a_drx2d, *a_dry2d, *a_wubot, *a_wvbot;
+  T *p_ua, *p_va, *p_el, *p_elb, *p_d;   // input cotangents (assigned)
   int32_t elf_first;
 };
 
 template <class T> struct ExtRevElfArgs {
   const T* c_elf;                                // complete cotangent of the substep's elf
   const T *d, *ua, *va, *dx, *dy, *art, *fsm;    // primal (state n-1) and geometry
-  T *p_ua, *p_va, *p_d, *p_elb;                  // accumulated (k_ext_rev_uv assigned them)
+  T *p_ua, *p_va, *p_d, *p_elb;                  // accumulated (k_ext_rev_gat assigned them)
+};
+
+// the forcing cotangents from an advave-period accumulator: c_adx += acc_u,
+// c_drx += acc_u, c_wubot -= aru acc_u (and the v terms); the accumulators
+// are cleared when clear is set
+template <class T> struct ExtFoldArgs {
+  T *acc_u, *acc_v;
+  const T *aru, *arv;
+  T *c_adx, *c_drx, *c_wubot, *c_ady, *c_dry, *c_wvbot;
+  int32_t clear;
 };
 
 // c.prm.iext is the substep
-template <class T> void launch_ext_rev_uv(const Ctx<T>& c, const ExtRevArgs<T>& a);
+template <class T> void launch_ext_rev_fac(const Ctx<T>& c, const ExtRevFacArgs<T>& a);
+template <class T> void launch_ext_rev_gat(const Ctx<T>& c, const ExtRevGatArgs<T>& a);
 template <class T> void launch_ext_rev_elf(const Ctx<T>& c, const ExtRevElfArgs<T>& a);
+template <class T> void launch_ext_fold(const Ctx<T>& c, const ExtFoldArgs<T>& a);
 
 }  // namespace pom

@@ -1102,6 +1102,7 @@ template <class T> struct Adj {
   T *b_uf0, *b_vf0, *b_uf1, *b_vf1, *b_uf2, *b_vf2;
   // scratch
   T *s3a, *s3b, *s3c, *s3d, *s2a, *s2b, *s2c, *s2d, *s2e;
+  T* xfac = nullptr;    // factor planes of the substep reverse (XF_N x n2)
   // semi-implicit barotropic step
   T *b_us = nullptr, *b_vs, *b_dus, *b_dvs, *b_raw, *b_rhs, *nlam, *arhs, *xadj;
   T *b_us0 = nullptr, *b_vs0, *b_uc, *b_vc;   // tide policy: closed predictor / corrector outputs
@@ -1174,6 +1175,7 @@ template <class T> struct Adj {
     q_bc = A.get<T>(N3); q_bkn = A.get<T>(N3); q_bdtef = A.get<T>(N3); q_bboy = A.get<T>(N3);
     q_bq2bot = A.get<T>(N2); q_bdh = A.get<T>(N2);
     s2a = A.get<T>(N2); s2b = A.get<T>(N2); s2c = A.get<T>(N2); s2d = A.get<T>(N2); s2e = A.get<T>(N2);
+    xfac = A.get<T>((int64_t)XF_N * N2);
     if (S.imp) {
       b_us = A.get<T>(N2); b_vs = A.get<T>(N2); b_dus = A.get<T>(N2); b_dvs = A.get<T>(N2);
       b_raw = A.get<T>(N2); b_rhs = A.get<T>(N2); nlam = A.get<T>(N2); arhs = A.get<T>(N2); xadj = A.get<T>(N2);
@@ -1200,6 +1202,13 @@ template <class T> struct Adj {
     hipLaunchKernelGGL(k_fill_multi<T>, dim3(flat_blocks(per > 0 ? per : 1), nft), dim3(256), 0, c.stream, ft);
     nft = 0;
     ftmax = 0;
+  }
+  // the forcing cotangents adx2d, drx2d, wubot, ady2d, dry2d, wvbot from the
+  // advave-period accumulators of x (cleared when clear is set)
+  void fold_forcing(X<T>& x, bool clear) {
+    ExtFoldArgs<T> a{x.advua, x.advva, S.g[POM_G_ARU], S.g[POM_G_ARV], w<POM_ADX2D>(), w<POM_DRX2D>(),
+                     bi[POM_WUBOT], w<POM_ADY2D>(), w<POM_DRY2D>(), bi[POM_WVBOT], clear ? 1 : 0};
+    launch_ext_fold(c, a);
   }
   void add(T* d, const T* s, int64_t n) {
     hipLaunchKernelGGL(k_add<T>, dim3(flat_blocks(n)), dim3(256), 0, c.stream, d, s, n);
@@ -1276,31 +1285,34 @@ template <class T> struct Adj {
         const X<T>& xn = S.xs[n];
         Ctx<T> cn = c;
         cn.prm.iext = n;
-        {   // filter and depth-mean momentum (with bcond 2)
-          ExtRevArgs<T> a{};
-          a.el = xp.el; a.elb = xp.elb; a.d = xp.d; a.ua = xp.ua; a.va = xp.va; a.uab = xp.uab; a.vab = xp.vab;
-          a.elf = xn.elf; a.uaf = xn.uaf; a.vaf = xn.vaf;
-          a.cor = S.g[POM_G_COR]; a.dx = S.g[POM_G_DX]; a.dy = S.g[POM_G_DY]; a.h = S.g[POM_G_H];
-          a.aru = S.g[POM_G_ARU]; a.arv = S.g[POM_G_ARV]; a.dum = S.g[POM_G_DUM]; a.dvm = S.g[POM_G_DVM];
-          a.eatm = S.out[POM_E_ATMOS]; a.fsm = S.g[POM_G_FSM];
-          a.c_ua = bn.ua; a.c_uab = bn.uab; a.c_va = bn.va; a.c_vab = bn.vab; a.c_el = bn.el; a.c_elb = bn.elb;
-          a.c_d = bn.d; a.c_egf = bn.egf; a.c_utf = bn.utf; a.c_vtf = bn.vtf; a.c_uaf = bn.uaf; a.c_vaf = bn.vaf;
+        {   // partials of the depth-mean momentum outputs (with bcond 2), once per point
+          ExtRevFacArgs<T> a{};
+          a.c_uaf = bn.uaf; a.c_ua = bn.ua; a.c_uab = bn.uab; a.c_utf = bn.utf;
+          a.c_vaf = bn.vaf; a.c_va = bn.va; a.c_vab = bn.vab; a.c_vtf = bn.vtf;
+          a.h = S.g[POM_G_H]; a.elf = xn.elf; a.aru = S.g[POM_G_ARU]; a.arv = S.g[POM_G_ARV];
+          a.dum = S.g[POM_G_DUM]; a.dvm = S.g[POM_G_DVM]; a.dx = S.g[POM_G_DX]; a.dy = S.g[POM_G_DY];
+          a.d = xp.d; a.el = xp.el; a.elb = xp.elb; a.eatm = S.out[POM_E_ATMOS]; a.uab = xp.uab; a.vab = xp.vab;
+          a.uaf = xn.uaf; a.vaf = xn.vaf;
+          a.fac = xfac; a.n2 = N2;
+          a.p_uab = bp.uab; a.p_vab = bp.vab; a.p_uaf = bp.uaf; a.p_vaf = bp.vaf;
+          a.a_advua = bn.advua; a.a_advva = bn.advva;
+          launch_ext_rev_fac(cn, a);
+        }
+        {   // their sums at each input point, and the filter
+          ExtRevGatArgs<T> a{};
+          a.fac = xfac; a.n2 = N2;
+          a.cor = S.g[POM_G_COR]; a.d = xp.d; a.ua = xp.ua; a.va = xp.va; a.fsm = S.g[POM_G_FSM];
+          a.c_uab = bn.uab; a.c_vab = bn.vab; a.c_elb = bn.elb; a.c_el = bn.el; a.c_d = bn.d; a.c_egf = bn.egf;
           a.c_etf = bn.etf; a.c_elf = bn.elf;
-          a.p_ua = bp.ua; a.p_uab = bp.uab; a.p_va = bp.va; a.p_vab = bp.vab; a.p_el = bp.el; a.p_elb = bp.elb;
-          a.p_d = bp.d; a.p_uaf = bp.uaf; a.p_vaf = bp.vaf;
-          a.a_advua = bn.advua; a.a_advva = bn.advva; a.a_adx2d = w<POM_ADX2D>(); a.a_ady2d = w<POM_ADY2D>();
-          a.a_drx2d = w<POM_DRX2D>(); a.a_dry2d = w<POM_DRY2D>(); a.a_wubot = bi[POM_WUBOT];
-          a.a_wvbot = bi[POM_WVBOT];
+          a.p_ua = bp.ua; a.p_va = bp.va; a.p_el = bp.el; a.p_elb = bp.elb; a.p_d = bp.d;
           a.elf_first = n < S.isplit;
-          launch_ext_rev_uv(cn, a);
+          launch_ext_rev_gat(cn, a);
         }
         if (n % c.prm.ispadv == 0) {   // advave produced this substep's advua / advva
           COutTab<T> ob{{bn.advua, bn.advva, nullptr, nullptr, nullptr}};
           T* ib[OpAdvave::NIN] = {bp.ua, bp.va, bp.uab, bp.vab, bp.d, w<POM_AAM2D>()};
           launch_gather<T, OpAdvave>(cn, S.t_advave(xp), ob, ib);
-          zq(bn.advua, N2);   // the earlier value's cotangent starts from zero
-          zq(bn.advva, N2);
-          zflush();
+          fold_forcing(bn, true);   // the period's forcing cotangents; the earlier value's starts from zero
         }
         {   // bcond 1 and continuity
           ExtRevElfArgs<T> a{bn.elf, xp.d, xp.ua, xp.va, S.g[POM_G_DX], S.g[POM_G_DY], S.g[POM_G_ART],
@@ -1309,6 +1321,7 @@ template <class T> struct Adj {
         }
       }
       X<T>& b0 = bx[0];
+      fold_forcing(b0, false);   // the first period's forcing cotangents (its accumulator is read below)
       // no substep reads the entering elf, and nothing assigned b0.elf
       zq(b0.elf, N2);
       zflush();
