@@ -296,6 +296,42 @@ __global__ void __launch_bounds__(TPB) k_ext_fold(Batch B, ExtFoldArgs<T> a) {
   }
 }
 
+template <class T>
+__global__ void __launch_bounds__(TPB) k_ext_elf(Batch B, Params prm, ExtElfArgs<T> a) {
+  const int nblk = B.blk2[B.M];
+  const T dte2 = T(prm.dte2), qa = T(0.25);
+  for (int blk = logical_block(); blk < nblk; blk += (int)gridDim.x) {
+    Pos q;
+    if (!pos2_at(B, blk, q)) continue;
+    const L2<T> L{q};
+    const int im = q.im, jm = q.jm;
+    // the interior point p copies (OpBc1: edges take their nearest interior
+    // point, corners the diagonal one)
+    const int mi = q.i == 0 ? 1 : (q.i == im - 1 ? im - 2 : q.i);
+    const int mj = q.j == 0 ? 1 : (q.j == jm - 1 ? jm - 2 : q.j);
+    auto fu = [&](int i, int j) {
+      return qa * (L(a.d, i, j) + L(a.d, i - 1, j)) * (L(a.dy, i, j) + L(a.dy, i - 1, j)) * L(a.ua, i, j);
+    };
+    auto fv = [&](int i, int j) {
+      return qa * (L(a.d, i, j) + L(a.d, i, j - 1)) * (L(a.dx, i, j) + L(a.dx, i, j - 1)) * L(a.va, i, j);
+    };
+    const T div = fu(mi + 1, mj) - fu(mi, mj) + fv(mi, mj + 1) - fv(mi, mj);
+    const T term = -(div / L(a.art, mi, mj)) - L(a.vfluxf, mi, mj);
+    const T raw = L(a.elb, mi, mj) + dte2 * term;
+    const bool in = mi >= 1 && mi <= im - 2 && mj >= 1 && mj <= jm - 2;
+    const uint32_t ix = lix2(q, q.i, q.j);
+    st2(q, a.elf, ix, (in ? raw : T(0)) * ld_at(a.fsm, q.o2, ix));
+  }
+}
+
+template <class T> void launch_ext_elf(const Ctx<T>& c, const ExtElfArgs<T>& a) {
+  if (c.nblk2 == 0) return;
+  hipLaunchKernelGGL((k_ext_elf<T>), dim3(groups(c.nblk2, c.tune.flat2_bpg, c.tune.flat2_min)), dim3(TPB), 0,
+                     c.stream, c.B, c.prm, a);
+}
+template void launch_ext_elf<float>(const Ctx<float>&, const ExtElfArgs<float>&);
+template void launch_ext_elf<double>(const Ctx<double>&, const ExtElfArgs<double>&);
+
 template <class T> void launch_ext_rev_fac(const Ctx<T>& c, const ExtRevFacArgs<T>& a) {
   if (c.nblk2 == 0) return;
   hipLaunchKernelGGL((k_ext_rev_fac<T>), dim3(groups(c.nblk2, c.tune.flat2_bpg, c.tune.flat2_min)), dim3(TPB), 0,

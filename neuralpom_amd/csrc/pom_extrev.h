@@ -84,10 +84,20 @@ template <class T> struct ExtFoldArgs {
   int32_t clear;
 };
 
+// Forward: the substep's elf in one pass, bcond 1 of the continuity update:
+// elf(p) = el_raw(m) fsm(p), m the interior point that p copies (p itself
+// inside), el_raw(m) = elb(m) + dte2 (-(div(m) / art(m)) - vfluxf(m)) as
+// OpExtElfRaw, and zero where m is not interior
+template <class T> struct ExtElfArgs {
+  const T *ua, *va, *d, *elb, *dx, *dy, *art, *vfluxf, *fsm;
+  T* elf;
+};
+
 // c.prm.iext is the substep
 template <class T> void launch_ext_rev_fac(const Ctx<T>& c, const ExtRevFacArgs<T>& a);
 template <class T> void launch_ext_rev_gat(const Ctx<T>& c, const ExtRevGatArgs<T>& a);
 template <class T> void launch_ext_rev_elf(const Ctx<T>& c, const ExtRevElfArgs<T>& a);
 template <class T> void launch_ext_fold(const Ctx<T>& c, const ExtFoldArgs<T>& a);
+template <class T> void launch_ext_elf(const Ctx<T>& c, const ExtElfArgs<T>& a);
 
 }  // namespace pom

@@ -260,7 +260,7 @@ template <class T> struct Step {
   bool internal;
   T* vamax;
   // intermediates
-  T *rho1, *adx2d_raw, *ady2d_raw, *advua0, *advva0, *fluxua0, *fluxva0, *tps0, *egf0, *utf0, *vtf0, *elfs;
+  T *rho1, *adx2d_raw, *ady2d_raw, *advua0, *advva0, *fluxua0, *fluxva0, *tps0, *egf0, *utf0, *vtf0;
   X<T> xs[MAXSPLIT + 1];
   // semi-implicit core
   bool imp = false;
@@ -301,7 +301,6 @@ template <class T> struct Step {
     egf0 = A.get<T>(N2);
     utf0 = A.get<T>(N2);
     vtf0 = A.get<T>(N2);
-    elfs = A.get<T>(N2);
     if (imp) {
       us = A.get<T>(N2); vs = A.get<T>(N2); dus = A.get<T>(N2); dvs = A.get<T>(N2);
       rhs = A.get<T>(N2); elfraw = A.get<T>(N2);
@@ -472,24 +471,6 @@ template <class T> struct Step {
     t.f[OpExtPrep::D_] = in[POM_D];
     t.f[OpExtPrep::UTF] = in[POM_UTF];
     t.f[OpExtPrep::VTF] = in[POM_VTF];
-    return t;
-  }
-  FieldTab<T> t_elf(const X<T>& x) const {
-    FieldTab<T> t{};
-    t.f[OpExtElfRaw::UA] = x.ua;
-    t.f[OpExtElfRaw::VA] = x.va;
-    t.f[OpExtElfRaw::D_] = x.d;
-    t.f[OpExtElfRaw::ELB] = x.elb;
-    t.g[OpExtElfRaw::GDX] = g[POM_G_DX];
-    t.g[OpExtElfRaw::GDY] = g[POM_G_DY];
-    t.g[OpExtElfRaw::GART] = g[POM_G_ART];
-    t.g[OpExtElfRaw::GVFLUXF] = out[POM_VFLUXF];
-    return t;
-  }
-  FieldTab<T> t_bc1() const {
-    FieldTab<T> t{};
-    t.f[OpBc1::RAW] = elfs;
-    t.g[OpBc1::GFSM] = g[POM_G_FSM];
     return t;
   }
   FieldTab<T> t_uv(const X<T>& xp, const X<T>& xn) const {
@@ -983,8 +964,11 @@ template <class T> struct Step {
       prm.iext = n;
       Ctx<T> cn = c;
       cn.prm = prm;
-      launch_map<T, OpExtElfRaw>(cn, t_elf(xp), OutTab<T>{{elfs}});
-      launch_map<T, OpBc1>(cn, t_bc1(), OutTab<T>{{xn.elf}});
+      {   // continuity with bcond 1 in one pass (OpExtElfRaw, then OpBc1)
+        ExtElfArgs<T> a{xp.ua, xp.va, xp.d, xp.elb, g[POM_G_DX], g[POM_G_DY], g[POM_G_ART], out[POM_VFLUXF],
+                        g[POM_G_FSM], xn.elf};
+        launch_ext_elf(cn, a);
+      }
       if (n % c.prm.ispadv == 0) {
         launch_map<T, OpAdvave>(cn, t_advave(xp), OutTab<T>{{xn.advua, xn.advva, xn.fluxua, xn.fluxva, xn.tps}});
       }
