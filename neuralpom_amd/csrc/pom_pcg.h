@@ -21,7 +21,9 @@
 namespace pom {
 
 // per-member scalar slots
-enum { PC_RHO = 0, PC_THR, PC_TOL2, PC_ALPHA, PC_BETA, PC_RS, PC_RS0, PC_AUX, PC_N };
+// PC_RHO + 0 / 1: r.z, by iteration parity (an iteration's p update writes
+// the next rho while other blocks of the same launch still read this one)
+enum { PC_RHO = 0, PC_RHO1, PC_THR, PC_TOL2, PC_RS, PC_RS0, PC_AUX, PC_N };
 
 template <class T> struct EllGeom {
   const T *dx, *dy, *art, *dum, *dvm;
@@ -210,8 +212,9 @@ __global__ void __launch_bounds__(TPB) k_pcg_start(Batch B, const double* __rest
 template <class T>
 __global__ void __launch_bounds__(TPB) k_pcg_ap(Batch B, EllGeom<T> g, const T* __restrict__ d,
                                                 const T* __restrict__ p_, T* __restrict__ ap,
-                                                const int* __restrict__ act, double* __restrict__ part) {
+                                                const int* __restrict__ act, double* __restrict__ part, int* nact) {
   __shared__ double sh[TPB];
+  if (logical_block() == 0 && threadIdx.x == 0) *nact = 0;   // counted again by this iteration's test
   const int m = block_member(B);
   if (!act[m]) {
     if (threadIdx.x == 0) part[2 * logical_block()] = 0.0;
@@ -233,33 +236,28 @@ __global__ void __launch_bounds__(TPB) k_pcg_ap(Batch B, EllGeom<T> g, const T* 
   if (threadIdx.x == 0) part[2 * logical_block()] = s;
 }
 
-// alpha = rho / (p.ap + 1e-30)
-__global__ void __launch_bounds__(TPB) k_pcg_alpha(Batch B, const double* __restrict__ part, double* sc,
-                                                   const int* __restrict__ act, int* nact) {
-  __shared__ double sh[TPB];
-  const int m = blockIdx.x;
-  if (m == 0 && threadIdx.x == 0) *nact = 0;
-  if (!act[m]) return;
-  const double den = member_sum(B, part, m, 0, sh);
-  if (threadIdx.x == 0) {
-    double* s = sc + PC_N * m;
-    s[PC_ALPHA] = s[PC_RHO] / (den + 1e-30);
-  }
-}
+// Each block of a member forms the member's scalars (alpha here, the
+// convergence test and beta below) itself, from the member's block partial
+// sums in one fixed order: every block gets the same value, and the
+// iteration needs no launch per reduction.  The member's first block keeps
+// the values the next launches read (its[], act[], rho).
+__device__ __forceinline__ bool member_first(const Batch& B, int m) { return logical_block() == B.blk2[m]; }
 
-// x += alpha p, r -= alpha ap; partial sums of r.r
+// alpha = rho / (p.ap + 1e-30); x += alpha p, r -= alpha ap; partial sums of r.r
 template <class T>
 __global__ void __launch_bounds__(TPB) k_pcg_xr(Batch B, T* __restrict__ x, T* __restrict__ r,
                                                 const T* __restrict__ p_, const T* __restrict__ ap,
                                                 const double* __restrict__ sc, const int* __restrict__ act,
-                                                double* __restrict__ part) {
+                                                double* __restrict__ part, int par) {
   __shared__ double sh[TPB];
   const int m = block_member(B);
   if (!act[m]) {
     if (threadIdx.x == 0) part[2 * logical_block() + 1] = 0.0;
     return;
   }
-  const T alpha = T(sc[PC_N * m + PC_ALPHA]);
+  const double den = member_sum(B, part, m, 0, sh);
+  __syncthreads();   // sh is reused below
+  const T alpha = T(sc[PC_N * m + PC_RHO + par] / (den + 1e-30));
   Pos p;
   double v = 0.0;
   if (pos2(B, p)) {
@@ -276,50 +274,94 @@ __global__ void __launch_bounds__(TPB) k_pcg_xr(Batch B, T* __restrict__ x, T* _
 }
 
 // convergence: rs = r.r; a member stops when rs <= tol^2, rs <= rel_tol^2 rs0
-// or rs <= 1e-30 (its x and r keep this iteration's update)
-__global__ void __launch_bounds__(TPB) k_pcg_conv(Batch B, const double* __restrict__ part, double* sc, int* act,
-                                                  int* its, int* nact) {
-  __shared__ double sh[TPB];
-  const int m = blockIdx.x;
-  if (!act[m]) return;
+// or rs <= 1e-30 (its x and r keep this iteration's update).  Called by every
+// block of an active member (act[m] read as set when the launch began, or
+// cleared by the member's first block in this launch, which happens only
+// when every block finds the member converged); true if it goes on.
+__device__ __forceinline__ bool pcg_test(const Batch& B, const double* part, double* sc, int* act, int* its,
+                                         int* nact, int m, double* sh) {
   const double rs = member_sum(B, part, m, 1, sh);
-  if (threadIdx.x == 0) {
-    double* s = sc + PC_N * m;
-    s[PC_RS] = rs;
+  __syncthreads();   // sh is reused by the caller
+  const double* s = sc + PC_N * m;
+  const bool go = !(rs <= s[PC_TOL2] || rs <= s[PC_THR] || rs <= 1e-30);
+  if (member_first(B, m) && threadIdx.x == 0) {
+    sc[PC_N * m + PC_RS] = rs;
     its[m] += 1;
-    if (rs <= s[PC_TOL2] || rs <= s[PC_THR] || rs <= 1e-30) act[m] = 0;
-    else atomicAdd(nact, 1);
+    if (go) atomicAdd(nact, 1);
+    else act[m] = 0;
   }
+  return go;
 }
 
-// beta = (r.z) / (rho + 1e-30); rho = r.z
-__global__ void __launch_bounds__(TPB) k_pcg_beta(Batch B, const double* __restrict__ part, double* sc,
-                                                  const int* __restrict__ act) {
+// the convergence test, then partial sums of r.z of the (spectral) preconditioned z
+template <class T>
+__global__ void __launch_bounds__(TPB) k_pcg_dot_test(Batch B, const T* __restrict__ r, const T* __restrict__ z,
+                                                      int* act, double* __restrict__ part, double* sc, int* its,
+                                                      int* nact) {
   __shared__ double sh[TPB];
-  const int m = blockIdx.x;
-  if (!act[m]) return;
-  const double rz = member_sum(B, part, m, 0, sh);
-  if (threadIdx.x == 0) {
-    double* s = sc + PC_N * m;
-    s[PC_BETA] = rz / (s[PC_RHO] + 1e-30);
-    s[PC_RHO] = rz;
+  const int m = block_member(B);
+  if (!act[m] || !pcg_test(B, part, sc, act, its, nact, m, sh)) {
+    if (threadIdx.x == 0) part[2 * logical_block()] = 0.0;
+    return;
   }
+  Pos p;
+  double v = 0.0;
+  if (pos2(B, p) && interior_p(p)) {
+    const int64_t c = ix2(p, p.i, p.j);
+    v = double(r[c]) * double(z[c]);
+  }
+  const double s = block_sum(v, sh);
+  if (threadIdx.x == 0) part[2 * logical_block()] = s;
 }
 
-// p = p beta + z at the unknowns, 0 at the edges and fixed points
+// the convergence test, then the Jacobi preconditioner z = r / diag and partial sums of r.z
+template <class T>
+__global__ void __launch_bounds__(TPB) k_pcg_jacobi_test(Batch B, EllGeom<T> g, const T* __restrict__ d,
+                                                         const T* __restrict__ r, T* __restrict__ z, int* act,
+                                                         double* __restrict__ part, double* sc, int* its,
+                                                         int* nact) {
+  __shared__ double sh[TPB];
+  const int m = block_member(B);
+  if (!act[m] || !pcg_test(B, part, sc, act, its, nact, m, sh)) {
+    if (threadIdx.x == 0) part[2 * logical_block()] = 0.0;
+    return;
+  }
+  Pos p;
+  double rz = 0.0;
+  if (pos2(B, p) && free_p(p, g.fixed, ix2(p, p.i, p.j))) {
+    const int64_t c = ix2(p, p.i, p.j);
+    const T zv = r[c] / (ell_diag(p, g, d) + T(1e-30));
+    z[c] = zv;
+    rz = double(r[c]) * double(zv);
+  }
+  const double s = block_sum(rz, sh);
+  if (threadIdx.x == 0) part[2 * logical_block()] = s;
+}
+
+// p = z at the start (first), else beta = (r.z) / (rho + 1e-30), p = p beta + z
+// and rho = r.z for the next iteration (slot par ^ 1); 0 at the edges and fixed points
 template <class T>
 __global__ void __launch_bounds__(TPB) k_pcg_p(Batch B, T* __restrict__ p_, const T* __restrict__ z,
-                                               const T* __restrict__ fixed, const double* __restrict__ sc,
-                                               const int* __restrict__ act, int first) {
+                                               const T* __restrict__ fixed, double* __restrict__ sc,
+                                               const int* __restrict__ act, const double* __restrict__ part,
+                                               int first, int par) {
+  __shared__ double sh[TPB];
+  const int m = block_member(B);
+  if (!act[m]) return;
+  T beta = T(0);
+  if (!first) {
+    const double rz = member_sum(B, part, m, 0, sh);
+    beta = T(rz / (sc[PC_N * m + PC_RHO + par] + 1e-30));
+    if (member_first(B, m) && threadIdx.x == 0) sc[PC_N * m + PC_RHO + (par ^ 1)] = rz;
+  }
   Pos p;
   if (!pos2(B, p)) return;
-  if (!act[p.m]) return;
   const int64_t c = ix2(p, p.i, p.j);
   if (!free_p(p, fixed, c)) {
     p_[c] = T(0);
     return;
   }
-  p_[c] = first ? z[c] : p_[c] * T(sc[PC_N * p.m + PC_BETA]) + z[c];
+  p_[c] = first ? z[c] : p_[c] * beta + z[c];
 }
 
 // ---------------------------------------------------------------- adjoint right-hand side

@@ -180,7 +180,10 @@ template <class T> struct Pcg {
     act = A.get<int>(M); its = A.get<int>(M); nact = A.get<int>(1);
   }
 
-  void precond(const Ctx<T>& c, const EllGeom<T>& g, const T* d) {
+  // z = M^-1 r and partial sums of r.z (slot 0).  With test set (an
+  // iteration's preconditioner), the kernel that forms the partial sums first
+  // applies the convergence test (pcg_test): members that stop write zeros.
+  void precond(const Ctx<T>& c, const EllGeom<T>& g, const T* d, bool test) {
     const dim3 g2(c.nblk2), tb(TPB);
     if (spectral) {
       GemmArg<T> a1{nullptr, r, t1, 0, 2, 3, 0};    // T1 = S_ni R
@@ -200,7 +203,12 @@ template <class T> struct Pcg {
         hipLaunchKernelGGL(k_spec_gemm<T>, gg, tb, 0, c.stream, c.B.mem, sp, cache, act, a3, 1);
         hipLaunchKernelGGL(k_spec_gemm<T>, gg, tb, 0, c.stream, c.B.mem, sp, cache, act, a4, 0);
       }
-      hipLaunchKernelGGL(k_pcg_dot<T>, g2, tb, 0, c.stream, c.B, r, z, act, part, 0);
+      if (test)
+        hipLaunchKernelGGL(k_pcg_dot_test<T>, g2, tb, 0, c.stream, c.B, r, z, act, part, sc, its, nact);
+      else
+        hipLaunchKernelGGL(k_pcg_dot<T>, g2, tb, 0, c.stream, c.B, r, z, act, part, 0);
+    } else if (test) {
+      hipLaunchKernelGGL(k_pcg_jacobi_test<T>, g2, tb, 0, c.stream, c.B, g, d, r, z, act, part, sc, its, nact);
     } else {
       hipLaunchKernelGGL(k_pcg_jacobi<T>, g2, tb, 0, c.stream, c.B, g, d, r, z, act, part);
     }
@@ -212,7 +220,11 @@ template <class T> struct Pcg {
   // host reads the count of unconverged members every POLL iterations and
   // stops at zero; a failed poll sets status to POM_E_HIP and abandons the
   // solve.  With capture all max_iter iterations are launched and converged
-  // members' kernels do nothing, so x is the same either way.
+  // members' kernels do nothing, so x is the same either way.  An iteration
+  // is four launches (Jacobi) or eight (spectral: four of them the sine
+  // products): A p with the p.ap sums; alpha and the x, r update with the
+  // r.r sums; the convergence test with the preconditioner's r.z sums; beta
+  // and the p update (pom_pcg.h).
   void solve(const Ctx<T>& c, const EllGeom<T>& g, const T* rhs, const T* d, const T* x0, T* x, double rel_tol,
              int max_iter, const T* lift = nullptr) {
     const dim3 g2(c.nblk2), tb(TPB), gm(c.B.M);
@@ -225,15 +237,15 @@ template <class T> struct Pcg {
     hipLaunchKernelGGL(k_pcg_x0<T>, g2, tb, 0, c.stream, c.B, x0, g.fixed, lift, x, z);
     hipLaunchKernelGGL(k_pcg_resid<T>, g2, tb, 0, c.stream, c.B, g, d, x, rhs, r, p);
     hipLaunchKernelGGL(k_pcg_act1, dim3(1), tb, 0, c.stream, act, c.B.M);
-    precond(c, g, d);
+    precond(c, g, d, false);
     hipLaunchKernelGGL(k_pcg_dot<T>, g2, tb, 0, c.stream, c.B, r, r, act, part, 1);
     hipLaunchKernelGGL(k_pcg_start, gm, tb, 0, c.stream, c.B, part, sc, act, its, tol, rel_tol);
-    hipLaunchKernelGGL(k_pcg_p<T>, g2, tb, 0, c.stream, c.B, p, z, g.fixed, sc, act, 1);
+    hipLaunchKernelGGL(k_pcg_p<T>, g2, tb, 0, c.stream, c.B, p, z, g.fixed, sc, act, part, 1, 0);
     for (int it = 1; it <= max_iter; ++it) {
-      hipLaunchKernelGGL(k_pcg_ap<T>, g2, tb, 0, c.stream, c.B, g, d, p, ap, act, part);
-      hipLaunchKernelGGL(k_pcg_alpha, gm, tb, 0, c.stream, c.B, part, sc, act, nact);
-      hipLaunchKernelGGL(k_pcg_xr<T>, g2, tb, 0, c.stream, c.B, x, r, p, ap, sc, act, part);
-      hipLaunchKernelGGL(k_pcg_conv, gm, tb, 0, c.stream, c.B, part, sc, act, its, nact);
+      const int par = (it - 1) & 1;   // rho's slot in this iteration
+      hipLaunchKernelGGL(k_pcg_ap<T>, g2, tb, 0, c.stream, c.B, g, d, p, ap, act, part, nact);
+      hipLaunchKernelGGL(k_pcg_xr<T>, g2, tb, 0, c.stream, c.B, x, r, p, ap, sc, act, part, par);
+      precond(c, g, d, true);
       if (!capture && (it % POLL == 0 || it == max_iter)) {
         if (hipMemcpyAsync(hw, nact, sizeof(int), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
             hipStreamSynchronize(c.stream) != hipSuccess) {
@@ -243,9 +255,7 @@ template <class T> struct Pcg {
         if (*hw == 0) break;
       }
       if (it == max_iter) break;
-      precond(c, g, d);
-      hipLaunchKernelGGL(k_pcg_beta, gm, tb, 0, c.stream, c.B, part, sc, act);
-      hipLaunchKernelGGL(k_pcg_p<T>, g2, tb, 0, c.stream, c.B, p, z, g.fixed, sc, act, 0);
+      hipLaunchKernelGGL(k_pcg_p<T>, g2, tb, 0, c.stream, c.B, p, z, g.fixed, sc, act, part, 0, par);
     }
   }
 };
