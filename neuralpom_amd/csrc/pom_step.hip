@@ -21,6 +21,7 @@
 #include "pom_profq.h"
 #include "pom_extrev.h"
 #include "pom_momadj.h"
+#include "pom_advctadj.h"
 
 #include "pom_batch.h"
 
@@ -1365,10 +1366,11 @@ template <class T> struct Adj {
                          bi[POM_DT]};
       hipLaunchKernelGGL(k_baropg_adj<T>, g2, tb, 0, c.stream, c.B, c.vert, c.ramp, c.prm, a);
     }
-    {
-      COutTab<T> ob{{bo[POM_ADVX], bo[POM_ADVY]}};
-      T* ib[OpAdvct::NIN] = {bi[POM_U], bi[POM_V], bi[POM_UB], bi[POM_VB], bi[POM_AAM], bi[POM_DT]};
-      launch_gather<T, OpAdvct>(c, S.t_advct(), ob, ib);
+    {   // advct: transposed stencil over its face fluxes
+      AdvctAdjArgs<T> a{S.in[POM_U], S.in[POM_V], S.in[POM_UB], S.in[POM_VB], S.in[POM_AAM], S.in[POM_DT],
+                        S.g[POM_G_DX], S.g[POM_G_DY], S.g[POM_G_ARU], S.g[POM_G_ARV], bo[POM_ADVX], bo[POM_ADVY],
+                        bi[POM_U], bi[POM_V], bi[POM_UB], bi[POM_VB], bi[POM_AAM], bi[POM_DT]};
+      launch_adj_advct(c, a);
     }
   }
 
