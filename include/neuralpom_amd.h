@@ -98,7 +98,14 @@ typedef struct {
  * conditions on the depth-mean transports with rfe rfw rfn rfs, the eta solve
  * holding the edges, dry cells and nodes fixed, and the open forms of bcond
  * 3-6 (kernels/coastal_explicit_baroclinic.py:496-928, exterior T / S from
- * POM_G_TBND / POM_G_SBND).  The explicit core takes POM_BOUNDARY_CLOSED only. */
+ * POM_G_TBND / POM_G_SBND).  The explicit core takes POM_BOUNDARY_TIDE as the
+ * adapter's explicit family (coastal_boundary.py:11-30): on every substep the
+ * nodes take the tide at the substep's time (tide_coef, tide_sec,
+ * tide_substep0 below) and the other edge points keep their previous
+ * elevation, the radiation conditions act on uaf / vaf, and bcond 3-6 take the
+ * open forms; that family is forward-only (the adjoint entry points return
+ * POM_E_UNSUPPORTED, as the reference validates no adjoint for it,
+ * case_adapters/registry.py:39-40). */
 typedef struct {
   double dti, dte, dti2, dte2, ispi, isp2i;
   double alpha, smoth, horcon, grav, rhoref, kappa, umol, tprni, small;
@@ -124,14 +131,25 @@ typedef struct {
   /* lower bound of the semi-implicit core's Smagorinsky viscosity
    * (implicit_p0_fixed/time_loop.py:186-188; 0 = none, the production value) */
   double aam_min;
+  /* tide policy, explicit core: substep n of the step (1..isplit) sets the
+   * nodes to amp cos(tide_coef * sec_n / 3600 - phase) with
+   *   sec_n = tide_sec + dte * (tide_substep0 + n - 1),
+   * tide_coef = speed_deg_per_h * pi / 180, tide_sec = time * 86400 - dti of
+   * the step and tide_substep0 the adapter's substep counter before the step
+   * (coastal_boundary.py:9-14; the counter is 0 after install and after
+   * every reset of the case registry, registry.py:57-58) */
+  double tide_coef, tide_sec;
+  int32_t tide_substep0, reserved1;
 } pom_params_t;
 
 typedef struct pom_batch pom_batch;
 
 /* Member table of an ensemble.  Copies the table to the device (one small
  * allocation owned by the handle).  dtype: POM_F32 or POM_F64.  Each member
- * is at least 4 x 4 columns and one 3-D field of a member is below 2 GiB
- * (im * jm * 16 * sizeof(element) < 2^31); otherwise POM_E_ARG. */
+ * is at least 4 x 4 columns, one 3-D field of a member is below 2 GiB
+ * (im * jm * 16 * sizeof(element) < 2^31) and one 3-D field of the whole
+ * batch below 4 GiB (the kernels address a batch field with 32-bit byte
+ * offsets from its base pointer); otherwise POM_E_ARG. */
 int pom_batch_create(const pom_member_t* members, int32_t n_members, int32_t dtype, pom_batch** out);
 int pom_batch_destroy(pom_batch* b);
 
@@ -154,7 +172,10 @@ int pom_batch_tune(pom_batch* b, const pom_tuning_t* t);
 int pom_batch_get_tuning(const pom_batch* b, pom_tuning_t* t);
 
 /* Workspace bytes needed by pom_explicit_step (keep=0) and by
- * pom_explicit_step_adj. */
+ * pom_explicit_step_adj.  With the tide policy (prm->boundary =
+ * POM_BOUNDARY_TIDE) the adjoint workspace, pom_explicit_step_adj,
+ * pom_explicit_step_taped and pom_explicit_step_reverse return
+ * POM_E_UNSUPPORTED (a forward-only family). */
 int pom_explicit_workspace(const pom_batch* b, const pom_params_t* prm, int32_t adjoint, size_t* bytes);
 
 /* One internal step (time_loop.py:75-696) for every member.
@@ -380,8 +401,10 @@ int pom_proft_adj(const pom_batch* b, const pom_params_t* prm, const void* vgrid
  *   the same lists (implicit_p0_fixed/baroclinic.py:505-510 ->
  *   boundary/closed_basin.py:27-87): 1 elf * fsm, 2 uaf * dum, vaf * dvm
  *   (edge ring zero), 3 uf * dum, vf * dvm (edge ring zero), 4 edge cells of
- *   uf, vf copied from the nearest interior cell, 5 w * fsm, 6 uf, vf * fsm;
- *   the tide policy's bcond returns POM_E_UNSUPPORTED.
+ *   uf, vf copied from the nearest interior cell, 5 w * fsm, 6 uf, vf * fsm.
+ *   The tide policy's bcond (either core) returns POM_E_UNSUPPORTED: its
+ *   bcond 1 advances the adapter's substep counter, so it runs inside the
+ *   step only.
  * pom_bcond_adj: grad_fields[q] += d(out)/d(fields[q])^T cot_out (entries
  *   may be null). */
 int pom_bcond(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vgrid, int32_t idx,

@@ -60,7 +60,8 @@ class Params(C.Structure):
         ("vmaxl", C.c_double)] + [(n, C.c_int32) for n in ("core", "eta_max_iter", "eta_precond", "boundary")] + [
         (n, C.c_double) for n in ("eta_tol", "eta_rel_tol", "gamma_baro", "tide_wt")] + [
         (n, C.c_int32) for n in ("eta_capture", "reserved0")] + [
-        (n, C.c_double) for n in ("tracer_shapiro", "velocity_shapiro", "aam_min")]
+        (n, C.c_double) for n in ("tracer_shapiro", "velocity_shapiro", "aam_min", "tide_coef", "tide_sec")] + [
+        (n, C.c_int32) for n in ("tide_substep0", "reserved1")]
 
 
 _lib = None

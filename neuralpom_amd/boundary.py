@@ -1,26 +1,33 @@
-"""Lateral boundary policies of the semi-implicit drop-in core.
+"""Lateral boundary policies of the drop-in cores.
 
-The semi-implicit core runs one of two policies inside its HIP step:
+Each core runs one of two policies inside its HIP step:
 
-* the closed basin (src/neuralpom/cores/implicit_p0_fixed/boundary/
-  closed_basin.py:27-101), the default;
+* its closed basin (explicit: cores/explicit/baroclinic.py:496-928 with zero
+  exterior values; semi-implicit: src/neuralpom/cores/implicit_p0_fixed/
+  boundary/closed_basin.py:27-101), the default;
 * the open-boundary tide policy of the reference's case adapter
-  (src/neuralpom/cores/case_adapters/coastal_boundary.py:9-67, implicit
-  family), installed with :func:`install` exactly as the adapter's ``install``
-  is: tide nodes read from an ``i j amplitude phase`` table (1-based indices,
-  phase in degrees, at most 99 rows), all four radiation coefficients set to
-  1 (or to ``radiation``), and the eta solve holding the edges, dry cells and
-  nodes fixed.
+  (src/neuralpom/cores/case_adapters/coastal_boundary.py:9-67), installed with
+  :func:`install` exactly as the adapter's ``install`` is: tide nodes read
+  from an ``i j amplitude phase`` table (1-based indices, phase in degrees, at
+  most 99 rows), all four radiation coefficients set to 1 (or to
+  ``radiation``), and the fixed set of the eta solve (edges, dry cells,
+  nodes).  The implicit family runs on the semi-implicit core, the explicit
+  family (forward-only, as the reference's case registry runs it) on the
+  explicit core.
 
 Where the adapter replaces ``bcond`` and ``solve_implicit_eta_cg`` with
 Python closures, here the policy object is a description the core hands to
-the step: elevation at the nodes (bcond 1), radiation conditions on the
-depth-mean transports (bcond 2), the lifted eta solve, and the open forms of
+the step: elevation at the nodes (bcond 1; the explicit family on every
+substep at the time of its substep counter, the other edge points keeping
+their elevation), radiation conditions on the depth-mean transports
+(bcond 2), the lifted eta solve (implicit family), and the open forms of
 bcond 3-6 with the exterior tracer values tbe tbw tbn tbs / sbe sbw sbn sbs.
 The attributes ``install`` sets on the core carry the adapter's names
 (``_tide_i``, ``_tide_j``, ``_tide_amp``, ``_tide_phase``, ``_fixed``,
 ``_validation_family``, ``_tide_substep``), so state capture over
-``vars(core)`` sees what it sees with the reference.
+``vars(core)`` sees what it sees with the reference; the explicit core also
+advances ``_tide_substep`` by ``isplit`` per step, as the adapter's bcond 1
+does once per substep.
 """
 from __future__ import annotations
 
@@ -43,9 +50,20 @@ class TideBoundary:
         self.speed = float(speed)
         self.pi = float(pi)
 
+    @property
+    def coef(self) -> float:
+        """speed pi / 180, the factor of t / 3600 in the argument (coastal_boundary.py:10)."""
+        return self.speed * self.pi / 180
+
     def wt(self, seconds: float) -> float:
         """(speed pi / 180) t / 3600, the argument's time part (coastal_boundary.py:9-10)."""
-        return (self.speed * self.pi / 180) * seconds / 3600
+        return self.coef * seconds / 3600
+
+    @staticmethod
+    def substep_seconds(core, counter: int) -> float:
+        """Evaluation time of the explicit family at substep counter value
+        ``counter`` (1-based, after the increment; coastal_boundary.py:14)."""
+        return float(core.time) * 86400 - float(core.dti) + float(core.dte) * (counter - 1)
 
     @staticmethod
     def seconds(core) -> float:
@@ -94,15 +112,20 @@ def fixed_set(fsm, i, j):
 
 def install(core, family: str, path=None, nodes=None, speed: float = TIDE_SPEED, pi: float = TIDE_PI,
             radiation: float = 1.0):
-    """Install the tide policy on a semi-implicit core (coastal_boundary.py:60-67).
+    """Install the tide policy on a core (coastal_boundary.py:55-64).
 
-    ``path``: text table ``i j amplitude phase`` (1-based, at most 99 rows are
-    read, as the adapter's ``np.loadtxt(path, max_rows=99)``); or ``nodes``, an
-    (n, 4) array of the same columns.  ``radiation`` is the value given to
-    rfe rfw rfn rfs (the adapter's: 1).  Only the implicit family runs on the
-    HIP step."""
-    if family != "implicit":
-        raise NotImplementedError("the tide policy runs on the semi-implicit core only (family 'implicit')")
+    ``family``: ``"implicit"`` on the semi-implicit core, ``"explicit"`` on
+    the explicit core (forward-only).  ``path``: text table ``i j amplitude
+    phase`` (1-based, at most 99 rows are read, as the adapter's
+    ``np.loadtxt(path, max_rows=99)``); or ``nodes``, an (n, 4) array of the
+    same columns.  ``radiation`` is the value given to rfe rfw rfn rfs (the
+    adapter's: 1)."""
+    from .implicit_core import POM2K_Core as ImplicitCore
+    if family not in ("implicit", "explicit"):
+        raise ValueError(f"tide family must be 'implicit' or 'explicit', got {family!r}")
+    if (family == "implicit") != isinstance(core, ImplicitCore):
+        raise NotImplementedError(f"the tide policy's {family} family runs on the "
+                                  f"{'semi-implicit' if family == 'implicit' else 'explicit'} core")
     if (path is None) == (nodes is None):
         raise ValueError("give exactly one of path and nodes")
     data = np.loadtxt(path, max_rows=99) if path is not None else np.asarray(nodes, dtype=np.float64)
@@ -120,5 +143,8 @@ def install(core, family: str, path=None, nodes=None, speed: float = TIDE_SPEED,
     core._tide_phase = torch.as_tensor(data[:, 3], dtype=core.el.dtype, device=core.device)
     core.rfe = core.rfw = core.rfn = core.rfs = float(radiation)
     core._fixed = fixed_set(core.fsm, i, j)
-    core._implicit_boundary_policy = TideBoundary(speed, pi)
+    if family == "implicit":
+        core._implicit_boundary_policy = TideBoundary(speed, pi)
+    else:
+        core._explicit_boundary_policy = TideBoundary(speed, pi)
     return core

@@ -24,12 +24,14 @@ from __future__ import annotations
 import ctypes as C
 import math
 import os
+from dataclasses import replace
 from typing import Any
 
 import torch
 
 from . import _lib as L
 from . import cases
+from .boundary import TideBoundary
 from .explicit import Ensemble, StepScalars, step_autograd
 
 KB = 16
@@ -211,7 +213,17 @@ class POM2K_Core:
         sc = StepScalars(**{n: getattr(self, n) for n in _STEP_SCALARS})
         if abs(sc.dti - self.dti) > 1e-9 * abs(self.dti):
             raise ValueError(f"dti {self.dti} != dte * isplit {sc.dti}")
+        tide = self._tide_policy()
+        if tide is not None:
+            sc = replace(sc, boundary=L.POM_BOUNDARY_TIDE, tide_coef=tide.coef)
         return sc
+
+    def _tide_policy(self):
+        """The tide policy's explicit family installed by
+        ``neuralpom_amd.boundary.install`` (the reference replaces ``bcond``
+        instead, case_adapters/coastal_boundary.py:62), or None."""
+        pol = getattr(self, "_explicit_boundary_policy", None)
+        return pol if isinstance(pol, TideBoundary) else None
 
     def _vgrid(self):
         return torch.stack([self.z, self.zz, self.dz, self.dzz]).to(self.device, self.dtype).reshape(1, 4, KB) \
@@ -257,11 +269,33 @@ class POM2K_Core:
         self._after_time_loop()
 
     def _geom(self):
-        return {n: self._flat(n).detach() for n in L.GEOM}
+        geom = {n: self._flat(n).detach() for n in L.GEOM}
+        tide = self._tide_policy()
+        if tide is not None:   # node, amplitude, phase, fixed set and exterior T / S, rebuilt when they change
+            acc = self._accel()
+            src = tuple((t.data_ptr(), t._version) for t in
+                        (self._tide_i, self._tide_j, self._tide_amp, self._tide_phase, self._fixed, self.tbe,
+                         self.tbw, self.tbn, self.tbs, self.sbe, self.sbw, self.sbn, self.sbs))
+            key = (src, tide.speed, tide.pi, self.dtype)
+            if acc.get("tide_key") != key:
+                acc["tide_geom"] = {n: self._as_flat(v).detach() for n, v in tide.geometry(self).items()}
+                acc["tide_key"] = key
+            geom.update(acc["tide_geom"])
+        return geom
 
     # hooks of the time loop (the semi-implicit core overrides them)
     def _scalars_at_step(self, sc):
-        return sc
+        """The explicit tide family's clock: the adapter advances its substep
+        counter once per substep and never resets it itself
+        (coastal_boundary.py:13-14), so a step starts from the counter's
+        current value; callers that follow the case registry's one-step
+        protocol set ``_tide_substep = 0`` before each step
+        (case_adapters/registry.py:57-58)."""
+        if self._tide_policy() is None:
+            return sc
+        n0 = int(self._tide_substep)
+        self._tide_substep = n0 + int(sc.isplit)
+        return replace(sc, tide_sec=float(self.time) * 86400 - float(self.dti), tide_substep0=n0)
 
     def _before_time_loop(self) -> None:
         pass
@@ -280,6 +314,9 @@ class POM2K_Core:
 
     def _advance(self, ens, state, geom, sc, ramp, vg, internal):
         if torch.is_grad_enabled() and any(v.requires_grad for v in state.values()):
+            if sc.boundary == L.POM_BOUNDARY_TIDE:
+                raise NotImplementedError("the tide policy's explicit family is forward-only (the reference's case "
+                                          "registry advances it under torch.no_grad() only, registry.py:39-40)")
             return step_autograd(ens, state, geom, sc, ramp, vg, internal)
         with torch.no_grad():
             return ens.step({n: v.detach() for n, v in state.items()}, geom, sc, ramp, vg, internal)
@@ -320,7 +357,11 @@ class POM2K_Core:
         of ``idx`` replaced, reading the core's el and ramp (idx 2), u and v
         (3, 4, 6), t, s, w and dt (4), q2 and q2l (6) as the reference does.  The
         exterior values of open edges (ele, uabe, tbe, ...) are zero: the
-        double gyre's closed basin.  Differentiable."""
+        double gyre's closed basin.  Differentiable.  With the tide policy
+        installed it raises: the adapter's bcond 1 advances its substep
+        counter and runs inside ``run_time_loop`` only."""
+        if self._tide_policy() is not None:
+            raise NotImplementedError("the tide policy's bcond runs inside run_time_loop only")
         F = self._as_flat
         if idx == 1:
             fields = [F(elf)]

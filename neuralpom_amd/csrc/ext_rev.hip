@@ -296,7 +296,7 @@ __global__ void __launch_bounds__(TPB) k_ext_fold(Batch B, ExtFoldArgs<T> a) {
   }
 }
 
-template <class T>
+template <class T, bool TIDE>
 __global__ void __launch_bounds__(TPB) k_ext_elf(Batch B, Params prm, ExtElfArgs<T> a) {
   const int nblk = B.blk2[B.M];
   const T dte2 = T(prm.dte2), qa = T(0.25);
@@ -306,7 +306,9 @@ __global__ void __launch_bounds__(TPB) k_ext_elf(Batch B, Params prm, ExtElfArgs
     const L2<T> L{q};
     const int im = q.im, jm = q.jm;
     // the interior point p copies (OpBc1: edges take their nearest interior
-    // point, corners the diagonal one)
+    // point, corners the diagonal one); with the tide policy the flux terms
+    // below are used at interior points only and this keeps their loads in
+    // the member
     const int mi = q.i == 0 ? 1 : (q.i == im - 1 ? im - 2 : q.i);
     const int mj = q.j == 0 ? 1 : (q.j == jm - 1 ? jm - 2 : q.j);
     auto fu = [&](int i, int j) {
@@ -320,14 +322,25 @@ __global__ void __launch_bounds__(TPB) k_ext_elf(Batch B, Params prm, ExtElfArgs
     const T raw = L(a.elb, mi, mj) + dte2 * term;
     const bool in = mi >= 1 && mi <= im - 2 && mj >= 1 && mj <= jm - 2;
     const uint32_t ix = lix2(q, q.i, q.j);
-    st2(q, a.elf, ix, (in ? raw : T(0)) * ld_at(a.fsm, q.o2, ix));
+    if constexpr (TIDE) {
+      const bool inside = q.i >= 1 && q.i <= im - 2 && q.j >= 1 && q.j <= jm - 2;
+      T v = inside ? raw : ld_at(a.elf_prev, q.o2, ix);
+      if (ld_at(a.node, q.o2, ix) != T(0))
+        v = T(double(ld_at(a.amp, q.o2, ix)) * cos(a.wt - double(ld_at(a.phase, q.o2, ix))));
+      st2(q, a.elf, ix, v * ld_at(a.fsm, q.o2, ix));
+    } else {
+      st2(q, a.elf, ix, (in ? raw : T(0)) * ld_at(a.fsm, q.o2, ix));
+    }
   }
 }
 
 template <class T> void launch_ext_elf(const Ctx<T>& c, const ExtElfArgs<T>& a) {
   if (c.nblk2 == 0) return;
-  hipLaunchKernelGGL((k_ext_elf<T>), dim3(groups(c.nblk2, c.tune.flat2_bpg, c.tune.flat2_min)), dim3(TPB), 0,
-                     c.stream, c.B, c.prm, a);
+  const dim3 grid(groups(c.nblk2, c.tune.flat2_bpg, c.tune.flat2_min));
+  if (a.node)
+    hipLaunchKernelGGL((k_ext_elf<T, true>), grid, dim3(TPB), 0, c.stream, c.B, c.prm, a);
+  else
+    hipLaunchKernelGGL((k_ext_elf<T, false>), grid, dim3(TPB), 0, c.stream, c.B, c.prm, a);
 }
 template void launch_ext_elf<float>(const Ctx<float>&, const ExtElfArgs<float>&);
 template void launch_ext_elf<double>(const Ctx<double>&, const ExtElfArgs<double>&);

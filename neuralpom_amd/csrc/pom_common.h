@@ -588,6 +588,18 @@ struct AccSub : Acc<T> {
   }
 };
 
+// Forward accessor that reads every 2-D field and geometry array at a fixed
+// offset (SI, SJ) from the wrapped accessor's point: an operator's formula
+// evaluated at a neighbour of the evaluation point.
+template <class AC, int SI, int SJ>
+struct AccShift {
+  const AC& a;
+  template <int F, int DI, int DJ>
+  __device__ __forceinline__ auto at2() const { return a.template at2<F, DI + SI, DJ + SJ>(); }
+  template <int G, int DI, int DJ>
+  __device__ __forceinline__ auto g() const { return a.template g<G, DI + SI, DJ + SJ>(); }
+};
+
 // Clamped neighbour indices around a gather thread's point q: every access of
 // a gather (evaluation point q - o, access offset a) lands on q + (a - o),
 // |a - o| <= 4 horizontally and <= 2 vertically, so one table per thread

@@ -87,10 +87,15 @@ template <class T> struct ExtFoldArgs {
 // Forward: the substep's elf in one pass, bcond 1 of the continuity update:
 // elf(p) = el_raw(m) fsm(p), m the interior point that p copies (p itself
 // inside), el_raw(m) = elb(m) + dte2 (-(div(m) / art(m)) - vfluxf(m)) as
-// OpExtElfRaw, and zero where m is not interior
+// OpExtElfRaw, and zero where m is not interior.  With node set (the tide
+// policy's bcond 1, case_adapters/coastal_boundary.py:12-22) there is no
+// copy: elf(p) = fsm(p) (amp(p) cos(wt - phase(p)) at a node, el_raw(p)
+// inside, elf_prev(p), the previous substep's elf, on the other edge points).
 template <class T> struct ExtElfArgs {
   const T *ua, *va, *d, *elb, *dx, *dy, *art, *vfluxf, *fsm;
   T* elf;
+  const T *node = nullptr, *amp = nullptr, *phase = nullptr, *elf_prev = nullptr;
+  double wt = 0.0;   // the substep's tide argument (speed pi / 180) t / 3600
 };
 
 // c.prm.iext is the substep

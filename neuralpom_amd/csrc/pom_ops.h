@@ -933,6 +933,63 @@ template <bool BC_ONLY> struct OpExtUVG {
 using OpExtUV = OpExtUVG<false>;
 using OpBcond2 = OpExtUVG<true>;
 
+// The depth-mean momentum update with the tide policy's bcond 2
+// (case_adapters/coastal_boundary.py:23-29) in place of the closed one:
+// radiation of the elevation difference across each open edge, added to the
+// transport just inside it, with the wave speed sqrt(g / d) of the substep's
+// d and no ramp.  Statement by statement, at interior j (i for v):
+//   uaf(im-1) = uaf(im-2) + rfe c(im-2) (el(im-2) - el(im-1))
+//   uaf(1)   -= rfw c(1) (el(1) - el(0)),   uaf(0) = uaf(1)
+//   vaf(jm-1) = vaf(jm-2) + rfn c(jm-2) (el(jm-2) - el(jm-1))
+//   vaf(1)   -= rfs c(1) (el(1) - el(0)),   vaf(0) = vaf(1)
+// the other velocity zero on each edge, corners keep the given uaf / vaf;
+// then * dum / dvm.  The transports inside are the interior formula at the
+// neighbour (AccShift).  Forward only: its adjoint is not built.
+struct OpExtUVTide : OpExtUVG<false> {
+  using Base = OpExtUVG<false>;
+  template <class X> __device__ static auto wave(X d, double grav) {   // sqrt(g / d)
+    using T = typename scalar_of<X>::type;
+    return P<T>{sqrt(T(grav) / val(d))};
+  }
+  template <int O, class AC> __device__ static auto eval(const AC& A, const Params& prm) {
+    const int i = A.p.i, j = A.p.j, im = A.p.im, jm = A.p.jm;
+    const bool jin = j >= 1 && j <= jm - 2, iin = i >= 1 && i <= im - 2;
+    if constexpr (O == 0) {
+      using R = decltype(Base::uaf_in(A, prm));
+      R r = cvt<R>(AT2(UAFIN, 0, 0));
+      if (jin && i == im - 1) {
+        const AccShift<AC, -1, 0> S{A};
+        r = Base::uaf_in(S, prm) + prm.rfe * wave(AT2(D_, -1, 0), prm.grav) * (AT2(EL, -1, 0) - AT2(EL, 0, 0));
+      } else if (jin && i == 0) {
+        const AccShift<AC, 1, 0> S{A};
+        r = Base::uaf_in(S, prm) - prm.rfw * wave(AT2(D_, 1, 0), prm.grav) * (AT2(EL, 1, 0) - AT2(EL, 0, 0));
+      } else if (iin && (j == jm - 1 || j == 0)) {
+        r = R{};
+      } else if (iin && jin) {
+        r = Base::uaf_in(A, prm);
+        if (i == 1) r = r - prm.rfw * wave(AT2(D_, 0, 0), prm.grav) * (AT2(EL, 0, 0) - AT2(EL, -1, 0));
+      }
+      return r * G2(GDUM, 0, 0);
+    } else {
+      using R = decltype(Base::vaf_in(A, prm));
+      R r = cvt<R>(AT2(VAFIN, 0, 0));
+      if (jin && (i == im - 1 || i == 0)) {
+        r = R{};
+      } else if (iin && j == jm - 1) {
+        const AccShift<AC, 0, -1> S{A};
+        r = Base::vaf_in(S, prm) + prm.rfn * wave(AT2(D_, 0, -1), prm.grav) * (AT2(EL, 0, -1) - AT2(EL, 0, 0));
+      } else if (iin && j == 0) {
+        const AccShift<AC, 0, 1> S{A};
+        r = Base::vaf_in(S, prm) - prm.rfs * wave(AT2(D_, 0, 1), prm.grav) * (AT2(EL, 0, 1) - AT2(EL, 0, 0));
+      } else if (iin && jin) {
+        r = Base::vaf_in(A, prm);
+        if (j == 1) r = r - prm.rfs * wave(AT2(D_, 0, 0), prm.grav) * (AT2(EL, 0, 0) - AT2(EL, 0, -1));
+      }
+      return r * G2(GDVM, 0, 0);
+    }
+  }
+};
+
 // Robert-Asselin filter, level shift and time-mean accumulation of one
 // external substep (time_loop.py:386-431).  The substep index selects the
 // etf filter phase and whether egf/utf/vtf accumulate.

@@ -11,6 +11,7 @@
 #include <vector>
 #include <mutex>
 #include <cstring>
+#include <cmath>
 #include <algorithm>
 #include <type_traits>
 
@@ -975,9 +976,17 @@ template <class T> struct Step {
       prm.iext = n;
       Ctx<T> cn = c;
       cn.prm = prm;
-      {   // continuity with bcond 1 in one pass (OpExtElfRaw, then OpBc1)
+      {   // continuity with bcond 1 in one pass (OpExtElfRaw, then OpBc1 or the tide's bcond 1)
         ExtElfArgs<T> a{xp.ua, xp.va, xp.d, xp.elb, g[POM_G_DX], g[POM_G_DY], g[POM_G_ART], out[POM_VFLUXF],
                         g[POM_G_FSM], xn.elf};
+        if (open) {   // the adapter's time of substep n (coastal_boundary.py:14)
+          a.node = g[POM_G_TIDE_NODE];
+          a.amp = g[POM_G_TIDE_AMP];
+          a.phase = g[POM_G_TIDE_PHASE];
+          a.elf_prev = xp.elf;
+          const double sec = hp->tide_sec + hp->dte * double(hp->tide_substep0 + n - 1);
+          a.wt = hp->tide_coef * sec / 3600.0;
+        }
         launch_ext_elf(cn, a);
       }
       if (n % c.prm.ispadv == 0) {
@@ -986,9 +995,13 @@ template <class T> struct Step {
       resolve(n, xn);
       {   // depth-mean momentum, then its filter / shift / time means at the same point, and max |vaf|
         ProfScope ps(POM_STAGE_EXTUV, c.stream);
-        launch_map2<T, OpExtUV, OpExtFilter, OpExtFilter::UAF, OpExtFilter::VAF>(
-            cn, t_uv(xp, xn), OutTab<T>{{xn.uaf, xn.vaf}}, t_filter(xp, xn),
-            OutTab<T>{{xn.ua, xn.uab, xn.va, xn.vab, xn.el, xn.elb, xn.d, xn.egf, xn.utf, xn.vtf, xn.etf}}, vamax);
+        const OutTab<T> of{{xn.ua, xn.uab, xn.va, xn.vab, xn.el, xn.elb, xn.d, xn.egf, xn.utf, xn.vtf, xn.etf}};
+        if (open)
+          launch_map2<T, OpExtUVTide, OpExtFilter, OpExtFilter::UAF, OpExtFilter::VAF>(
+              cn, t_uv(xp, xn), OutTab<T>{{xn.uaf, xn.vaf}}, t_filter(xp, xn), of, vamax);
+        else
+          launch_map2<T, OpExtUV, OpExtFilter, OpExtFilter::UAF, OpExtFilter::VAF>(
+              cn, t_uv(xp, xn), OutTab<T>{{xn.uaf, xn.vaf}}, t_filter(xp, xn), of, vamax);
       }
     }
     const X<T>& xl = xs[isplit];
@@ -1643,7 +1656,9 @@ template <class T> Ctx<T> make_ctx(const pom_batch* b, const pom_params_t* p, co
 int check_params(const pom_params_t* p, int core) {
   if (!p || p->core != core) return POM_E_ARG;
   if (p->boundary != POM_BOUNDARY_CLOSED && p->boundary != POM_BOUNDARY_TIDE) return POM_E_ARG;
-  if (p->boundary == POM_BOUNDARY_TIDE && core != POM_CORE_IMPLICIT) return POM_E_UNSUPPORTED;
+  if (p->boundary == POM_BOUNDARY_TIDE && core == POM_CORE_EXPLICIT &&
+      (!std::isfinite(p->tide_coef) || !std::isfinite(p->tide_sec) || p->tide_substep0 < 0))
+    return POM_E_ARG;
   if (p->isplit < 3 || p->isplit > MAXSPLIT || p->ispadv < 1) return POM_E_ARG;
   if (!((p->nbct == 1 || p->nbct == 3) && (p->nbcs == 1 || p->nbcs == 3))) return POM_E_UNSUPPORTED;
   if (core == POM_CORE_IMPLICIT) {
@@ -1759,10 +1774,10 @@ template <class T> void setup_pcg(Pcg<T>& w, const pom_batch* b, const pom_param
   }
 }
 
-// attach the semi-implicit settings and cache to a step
+// attach the core, the boundary policy and the semi-implicit settings and cache to a step
 template <class T> void setup_implicit(Step<T>& S, const pom_params_t* prm, const void* cache) {
   S.imp = prm->core == POM_CORE_IMPLICIT;
-  S.open = S.imp && prm->boundary == POM_BOUNDARY_TIDE;
+  S.open = prm->boundary == POM_BOUNDARY_TIDE;
   S.shap_ts = S.imp && !S.open && prm->tracer_shapiro != 0.0;
   S.shap_uv = S.imp && !S.open && prm->velocity_shapiro != 0.0;
   S.tide_wt = prm->tide_wt;
@@ -2526,6 +2541,7 @@ int pom_batch_destroy(pom_batch* b) {
 int pom_explicit_workspace(const pom_batch* b, const pom_params_t* prm, int32_t adjoint, size_t* bytes) {
   if (!b || !bytes) return POM_E_ARG;
   if (int e = check_params(prm, POM_CORE_EXPLICIT)) return e;
+  if (adjoint && prm->boundary == POM_BOUNDARY_TIDE) return POM_E_UNSUPPORTED;   // forward-only family
   *bytes = b->dtype == POM_F32 ? ws_size<float>(b, prm, adjoint != 0) : ws_size<double>(b, prm, adjoint != 0);
   return POM_OK;
 }
@@ -2547,6 +2563,7 @@ int pom_explicit_step_adj(const pom_batch* b, const pom_params_t* prm, const voi
                           void* stream) {
   if (!b || !ramp || !vgrid || !state_in || !geom || !workspace) return POM_E_ARG;
   if (int e = check_params(prm, POM_CORE_EXPLICIT)) return e;
+  if (prm->boundary == POM_BOUNDARY_TIDE) return POM_E_UNSUPPORTED;   // forward-only family
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (b->dtype == POM_F32)
     return run_adj<float>(b, prm, ramp, vgrid, state_in, geom, internal, cot_out, grad_in, workspace, ws_bytes, st);
@@ -2558,6 +2575,7 @@ int pom_explicit_step_taped(const pom_batch* b, const pom_params_t* prm, const v
                             int32_t internal, void* vamax_out, void* workspace, size_t ws_bytes, void* stream) {
   if (!b || !ramp || !vgrid || !state_in || !state_out || !geom || !vamax_out || !workspace) return POM_E_ARG;
   if (int e = check_params(prm, POM_CORE_EXPLICIT)) return e;
+  if (prm->boundary == POM_BOUNDARY_TIDE) return POM_E_UNSUPPORTED;   // forward-only family
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (b->dtype == POM_F32)
     return run_taped<float>(b, prm, ramp, vgrid, state_in, state_out, geom, internal, vamax_out, workspace, ws_bytes,
@@ -2572,6 +2590,7 @@ int pom_explicit_step_reverse(const pom_batch* b, const pom_params_t* prm, const
                               size_t ws_bytes, void* stream) {
   if (!b || !ramp || !vgrid || !state_in || !state_out || !geom || !workspace) return POM_E_ARG;
   if (int e = check_params(prm, POM_CORE_EXPLICIT)) return e;
+  if (prm->boundary == POM_BOUNDARY_TIDE) return POM_E_UNSUPPORTED;   // forward-only family
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (b->dtype == POM_F32)
     return run_reverse<float>(b, prm, ramp, vgrid, state_in, state_out, geom, internal, cot_out, grad_in, workspace,
@@ -2603,7 +2622,7 @@ int pom_bcond(const pom_batch* b, const pom_params_t* prm, const void* ramp, con
               const void* const* fields, const void* const* geom, void* const* out, void* stream) {
   const int nf = bcond_nfields(idx);
   if (!b || !prm || !ramp || !vgrid || !fields || !geom || !out || nf < 0) return POM_E_ARG;
-  if (prm->core == POM_CORE_IMPLICIT && prm->boundary != POM_BOUNDARY_CLOSED) return POM_E_UNSUPPORTED;
+  if (prm->boundary != POM_BOUNDARY_CLOSED) return POM_E_UNSUPPORTED;
   for (int q = 0; q < nf; ++q)
     if (!fields[q]) return POM_E_ARG;
   for (int q = 0; q < bcond_nout(idx); ++q)
@@ -2620,7 +2639,7 @@ int pom_bcond_adj(const pom_batch* b, const pom_params_t* prm, const void* ramp,
                   void* const* grad_fields, void* stream) {
   const int nf = bcond_nfields(idx);
   if (!b || !prm || !ramp || !vgrid || !fields || !geom || !cot_out || !grad_fields || nf < 0) return POM_E_ARG;
-  if (prm->core == POM_CORE_IMPLICIT && prm->boundary != POM_BOUNDARY_CLOSED) return POM_E_UNSUPPORTED;
+  if (prm->boundary != POM_BOUNDARY_CLOSED) return POM_E_UNSUPPORTED;
   for (int q = 0; q < nf; ++q)
     if (!fields[q]) return POM_E_ARG;
   for (int q = 0; q < POM_NGEOM_BASE; ++q)

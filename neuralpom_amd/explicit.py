@@ -49,6 +49,13 @@ class StepScalars:
     nbct: int = 1
     nbcs: int = 1
     vmaxl: float = 100.0
+    boundary: int = L.POM_BOUNDARY_CLOSED   # or POM_BOUNDARY_TIDE (open edges, tide nodes)
+    # tide policy, explicit family (coastal_boundary.py:9-14): the nodes of
+    # substep n take the tide at tide_sec + dte (tide_substep0 + n - 1)
+    # seconds, with the argument tide_coef t / 3600
+    tide_coef: float = 0.0                  # speed_deg_per_h * pi / 180
+    tide_sec: float = 0.0                   # time * 86400 - dti of the step
+    tide_substep0: int = 0                  # the adapter's substep counter before the step
 
     @property
     def dti(self):
@@ -69,6 +76,10 @@ class StepScalars:
             setattr(p, n, int(getattr(self, n)))
         p.iext = 0
         p.core = L.POM_CORE_EXPLICIT
+        p.boundary = int(self.boundary)
+        p.tide_coef = float(self.tide_coef)
+        p.tide_sec = float(self.tide_sec)
+        p.tide_substep0 = int(self.tide_substep0)
         return p
 
 

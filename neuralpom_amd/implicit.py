@@ -37,7 +37,6 @@ class ImplicitScalars(StepScalars):
     tracer_shapiro: float = 0.0     # numerics/double_gyre.py:78-130 (closed basin)
     velocity_shapiro: float = 0.0   # numerics/double_gyre.py:132-184 (closed basin)
     aam_min: float = 0.0            # Smagorinsky viscosity floor (time_loop.py:186-188)
-    boundary: int = L.POM_BOUNDARY_CLOSED   # or POM_BOUNDARY_TIDE (open edges, tide nodes)
     tide_wt: float = 0.0                    # (speed pi / 180) t / 3600 of the tide this step
     # capture-safe eta solves (core.py:504-555): exactly eta_max_iter iterations
     # are launched, no host synchronisation; converged members do nothing
@@ -57,7 +56,6 @@ class ImplicitScalars(StepScalars):
         p.tracer_shapiro = float(self.tracer_shapiro)
         p.velocity_shapiro = float(self.velocity_shapiro)
         p.aam_min = float(self.aam_min)
-        p.boundary = int(self.boundary)
         p.tide_wt = float(self.tide_wt)
         p.eta_capture = int(bool(self.eta_capture))
         return p

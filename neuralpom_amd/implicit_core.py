@@ -111,21 +111,6 @@ class POM2K_Core(_ExplicitCore):
         pol = getattr(self, "_implicit_boundary_policy", None)
         return pol if isinstance(pol, TideBoundary) else None
 
-    def _geom(self):
-        geom = {n: self._flat(n).detach() for n in L.GEOM}
-        tide = self._tide_policy()
-        if tide is not None:
-            acc = self._accel()
-            src = tuple((t.data_ptr(), t._version) for t in
-                        (self._tide_i, self._tide_j, self._tide_amp, self._tide_phase, self._fixed, self.tbe,
-                         self.tbw, self.tbn, self.tbs, self.sbe, self.sbw, self.sbn, self.sbs))
-            key = (src, tide.speed, tide.pi, self.dtype)
-            if acc.get("tide_key") != key:
-                acc["tide_geom"] = {n: self._as_flat(v).detach() for n, v in tide.geometry(self).items()}
-                acc["tide_key"] = key
-            geom.update(acc["tide_geom"])
-        return geom
-
     def _scalars_at_step(self, sc):
         """The tide's time argument at this step's self.time (coastal_boundary.py:36, 45)."""
         tide = self._tide_policy()

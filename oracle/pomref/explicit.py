@@ -25,6 +25,13 @@ parity), the explicit core of the reference:
 * double-gyre set-up       -- cores/explicit/cases.py:30-164,
                               cores/explicit/initialization.py:10-99,
                               cores/explicit/grid.py:113-166
+* the tide adapter's explicit family -- case_adapters/coastal_boundary.py:11-30,
+                              55-62 (installed with ``implicit.install_tide(...,
+                              family="explicit")``): ``tide_bcond1`` at the
+                              substep counter's time, the radiation bcond 2 of
+                              ``implicit.tide_bc2`` on the substep's el and d,
+                              and bcond 3 weighted by d
+                              (kernels/coastal_explicit_baroclinic.py:586-641)
 
 Every operator keeps the reference's association order of floating-point
 operations so that the float64 results agree with the reference's own float64
@@ -35,7 +42,9 @@ gradient oracle for the hand-written adjoint kernels.
 Parity pins (``tests/test_oracle_golden.py``): every operator against the
 golden vectors written from the reference by ``tests/golden/make_golden.py``,
 the 1- and 3-step trajectories, the time0 = 0 internal-mode skip, and the
-vector-Jacobian product of a step.
+vector-Jacobian product of a step; the tide family by
+``tests/test_oracle_explicit_tide.py`` against
+``tests/golden/make_golden_explicit_tide.py``.
 """
 from __future__ import annotations
 
@@ -290,6 +299,18 @@ def bcond1(S, elf):
     elf[0, :] = elf[1, :]
     elf[:, -1] = elf[:, -2]
     elf[:, 0] = elf[:, 1]
+    return elf * S.fsm
+
+
+def tide_bcond1(S, elf):
+    """bcond 1 of the tide adapter's explicit family (coastal_boundary.py:12-22):
+    each call advances the substep counter and sets the nodes to the tide at
+    time * 86400 - dti + dte (counter - 1); no edge copy."""
+    from . import implicit as IM
+    S.tide_substep += 1
+    sec = S.time * 86400 - S.dti + S.dte * (S.tide_substep - 1)
+    elf = elf.clone()
+    elf[S.tide.i, S.tide.j] = IM.tide_value(S, sec)
     return elf * S.fsm
 
 
@@ -899,6 +920,7 @@ def external_mode(S, X):
     B, Bm = slice(1, jm), slice(0, jm - 1)
     dx, dy, art, aru, arv, cor, h, grav = S.dx, S.dy, S.art, S.aru, S.arv, S.cor, S.h, S.grav
     alpha, dte, smoth = S.alpha, S.dte, S.smoth
+    tide = getattr(S, "tide", None) is not None
     vamax = None
     for iext in range(1, int(S.isplit) + 1):
         d = X.d
@@ -909,7 +931,7 @@ def external_mode(S, X):
         div = fluxua[Ip, J] - fluxua[I, J] + fluxva[I, Jp] - fluxva[I, J]
         elf = X.elf.clone()
         elf[I, J] = X.elb[I, J] + S.dte2 * (-(div / art[I, J]) - X.vfluxf[I, J])
-        elf = bcond1(S, elf)
+        elf = tide_bcond1(S, elf) if tide else bcond1(S, elf)
         X.fluxua, X.fluxva = fluxua, fluxva
         if iext % int(S.ispadv) == 0:
             X.advua, X.advva, X.fluxua, X.fluxva, X.tps = advave(
@@ -939,7 +961,11 @@ def external_mode(S, X):
         num = (h[I, J] + elb[I, J] + h[I, Jm] + elb[I, Jm]) * arv[I, J] * vab[I, J] - 4.0 * dte * prov
         vaf = X.vaf.clone()
         vaf[I, J] = num / (den + EPS_DIV)
-        uaf, vaf = bcond2(S, uaf, vaf, el, X.ramp)
+        if tide:
+            from . import implicit as IM
+            uaf, vaf = IM.tide_bc2(S, uaf, vaf, el=el, d=d)
+        else:
+            uaf, vaf = bcond2(S, uaf, vaf, el, X.ramp)
         if iext == int(S.isplit) - 2:
             X.etf = 0.25 * smoth * elf
         elif iext == int(S.isplit) - 1:
@@ -1076,7 +1102,7 @@ def internal_mode(S):
     vf = advv(S, S.advy, S.drhoy, S.u, S.v, S.w, S.vb, S.dt, S.egf, S.egb, S.etf, S.etb, S.e_atmos)
     uf, S.wubot, _ = profu(S, uf, S.wusurf, S.wubot, S.km, S.etf, S.ub, S.vb)
     vf, S.wvbot, S.tps = profv(S, vf, S.wvsurf, S.wvbot, S.km, S.etf, S.ub, S.vb)
-    uf, vf = bcond3(S, uf, vf, S.u, S.v)
+    uf, vf = bcond3(S, uf, vf, S.u, S.v, depth=S.d if getattr(S, "tide", None) is not None else None)
 
     dzw = S.dz[:kbm1].view(1, 1, -1)
     term = uf[:, :, K] + S.ub[:, :, K] - 2.0 * S.u[:, :, K]

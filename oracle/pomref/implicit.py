@@ -95,8 +95,14 @@ class Tide:
         self.fixed = fixed
 
 
-def install_tide(S, i, j, amp, phase, **kw):
+def install_tide(S, i, j, amp, phase, family="implicit", **kw):
+    """coastal_boundary.py:55-62: nodes, rfe = rfw = rfn = rfs = 1, and the
+    explicit family's substep counter, set to 0."""
+    if family not in ("implicit", "explicit"):
+        raise ValueError(family)
     S.tide = Tide(S, i, j, amp, phase, **kw)
+    S.tide_family = family
+    S.tide_substep = 0
     S.rfe = S.rfw = S.rfn = S.rfs = 1.0
     return S
 
@@ -119,12 +125,15 @@ def tide_bc1(S, elf):
     return elf * S.fsm
 
 
-def tide_bc2(S, uaf, vaf):
-    """radiation conditions on c.el and c.d (coastal_boundary.py:24-30)"""
+def tide_bc2(S, uaf, vaf, el=None, d=None):
+    """radiation conditions on c.el and c.d (coastal_boundary.py:24-30); the
+    explicit family's substeps pass their current el and d"""
     uaf = uaf.clone()
     vaf = vaf.clone()
     J = I = slice(1, -1)
-    g, d, el = S.grav, S.d, S.el
+    g = S.grav
+    d = S.d if d is None else d
+    el = S.el if el is None else el
     uaf[-1, J] = uaf[-2, J] + S.rfe * torch.sqrt(g / d[-2, J]) * (el[-2, J] - el[-1, J])
     vaf[-1, J] = 0
     uaf[1, J] = uaf[1, J] - S.rfw * torch.sqrt(g / d[1, J]) * (el[1, J] - el[0, J])
