@@ -23,6 +23,7 @@
 #include "pom_extrev.h"
 #include "pom_momadj.h"
 #include "pom_advctadj.h"
+#include "pom_advaveadj.h"
 
 #include "pom_batch.h"
 
@@ -1221,6 +1222,14 @@ template <class T> struct Adj {
   void add(T* d, const T* s, int64_t n) {
     hipLaunchKernelGGL(k_add<T>, dim3(flat_blocks(n)), dim3(256), 0, c.stream, d, s, n);
   }
+  // advave's input cotangents from those of its advua / advva (its flux and
+  // tps outputs are work arrays without one); aam2d's accumulates across calls
+  void advave_rev(const X<T>& x, const T* c_advua, const T* c_advva, T* p_ua, T* p_va, T* p_uab, T* p_vab,
+                  T* p_d) {
+    AdvaveAdjArgs<T> a{x.ua, x.va, x.uab, x.vab, x.d, S.out[POM_AAM2D], S.g[POM_G_DX], S.g[POM_G_DY],
+                       c_advua, c_advva, p_ua, p_va, p_uab, p_vab, p_d, w<POM_AAM2D>()};
+    launch_adj_advave(c, a);
+  }
   void zero_x(X<T>& x, int64_t n) {
     for (T* p : {x.ua, x.uab, x.va, x.vab, x.el, x.elb, x.d, x.egf, x.utf, x.vtf, x.etf, x.advua, x.advva, x.uaf,
                  x.vaf, x.elf})
@@ -1317,9 +1326,7 @@ template <class T> struct Adj {
           launch_ext_rev_gat(cn, a);
         }
         if (n % c.prm.ispadv == 0) {   // advave produced this substep's advua / advva
-          COutTab<T> ob{{bn.advua, bn.advva, nullptr, nullptr, nullptr}};
-          T* ib[OpAdvave::NIN] = {bp.ua, bp.va, bp.uab, bp.vab, bp.d, w<POM_AAM2D>()};
-          launch_gather<T, OpAdvave>(cn, S.t_advave(xp), ob, ib);
+          advave_rev(xp, bn.advua, bn.advva, bp.ua, bp.va, bp.uab, bp.vab, bp.d);
           fold_forcing(bn, true);   // the period's forcing cotangents; the earlier value's starts from zero
         }
         {   // bcond 1 and continuity
@@ -1353,9 +1360,7 @@ template <class T> struct Adj {
       xin.fluxua = (T*)S.in[POM_FLUXUA];
       xin.fluxva = (T*)S.in[POM_FLUXVA];
       xin.tps = (T*)S.in[POM_TPS];
-      COutTab<T> ob{{b_advua0, b_advva0, nullptr, nullptr, nullptr}};
-      T* ib[OpAdvave::NIN] = {bi[POM_UA], bi[POM_VA], bi[POM_UAB], bi[POM_VAB], bi[POM_D], w<POM_AAM2D>()};
-      launch_gather<T, OpAdvave>(c, S.t_advave(xin), ob, ib);
+      advave_rev(xin, b_advua0, b_advva0, bi[POM_UA], bi[POM_VA], bi[POM_UAB], bi[POM_VAB], bi[POM_D]);
     }
     {
       ColsumAdjArgs<T> a{};
