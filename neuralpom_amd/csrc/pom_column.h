@@ -593,57 +593,59 @@ __global__ void __launch_bounds__(TPB) k_vertvl_adj1(Batch B, Vert<T> vg, Params
   }
   st_col(a.bdiv + c, bd);
 }
-// reverse, pass 2 (per column): gather u, v, dt cotangents from bdiv
+// reverse, pass 2 (one thread per cell, a column's 16 levels in 16 adjacent
+// lanes): the u, v and dt cotangents from bdiv.  Face i of cell (i, j, k)
+// (i, j >= 1) carries xflux = dy4 (dt(i) + dt(i-1)) u(i) with
+// dy4 = (dy(i) + dy(i-1)) / 4, entering div(i-1) with + and div(i) with -;
+// face j likewise with v.  dt's partial is summed over the column's levels
+// k < kbm1 by lane shuffles.
 template <class T>
-__global__ void __launch_bounds__(TPB) k_vertvl_adj2(Batch B, Vert<T> vg, Params prm, VertvlAdjArgs<T> a) {
-  Pos p;
-  if (!pos2(B, p)) return;
-  const int i = p.i, j = p.j;
-  const int64_t c = ix3(p, i, j, 0), c2 = ix2(p, i, j);
-  auto bdv = [&](int ii, int jj, int k) -> T {   // bdiv at column (ii, jj); 0 outside
-    if (ii < 0 || ii >= p.im || jj < 0 || jj >= p.jm) return T(0);
-    return a.bdiv[ix3(p, ii, jj, k)];
-  };
-  T bdt = T(0);
-  // xflux at face i (i, j >= 1): div(i-1) gets +, div(i) gets -
-  if (i >= 1 && j >= 1) {
-    const T cx = T(0.25) * (a.dy[c2] + a.dy[ix2(p, i - 1, j)]);
-    const T dts = a.dt[c2] + a.dt[ix2(p, i - 1, j)];
-    const T cy = T(0.25) * (a.dx[c2] + a.dx[ix2(p, i, j - 1)]);
-    const T dty = a.dt[c2] + a.dt[ix2(p, i, j - 1)];
-    T gu[KB], gv[KB];
-    T bx = T(0), by = T(0);
-    #pragma unroll
-    for (int k = 0; k < KB; ++k) gu[k] = gv[k] = T(0);
-    #pragma unroll
-    for (int k = 0; k < KBM1; ++k) {
-      const T fx = bdv(i - 1, j, k) - bdv(i, j, k);
-      gu[k] = cx * dts * fx;
-      bx += cx * a.u[c + k] * fx;
-      const T fy = bdv(i, j - 1, k) - bdv(i, j, k);
-      gv[k] = cy * dty * fy;
-      by += cy * a.v[c + k] * fy;
+__global__ void __launch_bounds__(TPB) k_vertvl_adj2(Batch B, VertvlAdjArgs<T> a) {
+  const int nblk = B.blk3[B.M];
+  for (int blk = logical_block(); blk < nblk; blk += (int)gridDim.x) {   // uniform over the workgroup
+    Pos q;
+    const bool live = pos3_at(B, blk, q);   // a water column's 16 lanes are all live or all not
+    const int k = threadIdx.x & (KB - 1);
+    T pdt = T(0);
+    if (live && k < KBM1) {
+      const int i = q.i, j = q.j, im = q.im, jm = q.jm;
+      auto bdv = [&](int ii, int jj) -> T {   // bdiv at column (ii, jj), level k; 0 outside the member
+        const bool in = ii >= 0 && ii < im && jj >= 0 && jj < jm;
+        const T x = ld_at(a.bdiv, q.o3, lix3(q, clampi(ii, im), clampi(jj, jm), k));
+        return in ? x : T(0);
+      };
+      auto L2 = [&](const T* f, int ii, int jj) { return ld_at(f, q.o2, lix2(q, clampi(ii, im), clampi(jj, jm))); };
+      const uint32_t ix = lix3(q, i, j, k);
+      const T b0 = bdv(i, j), bw = bdv(i - 1, j), bs = bdv(i, j - 1), be = bdv(i + 1, j), bn = bdv(i, j + 1);
+      const T uq = ld_at(a.u, q.o3, ix), vq = ld_at(a.v, q.o3, ix);
+      const T ue = ld_at(a.u, q.o3, lix3(q, clampi(i + 1, im), j, k));
+      const T vn = ld_at(a.v, q.o3, lix3(q, i, clampi(j + 1, jm), k));
+      const T dyq = L2(a.dy, i, j), dxq = L2(a.dx, i, j), dtq = L2(a.dt, i, j);
+      if (i >= 1 && j >= 1) {   // faces i and j of this cell
+        const T cx = T(0.25) * (dyq + L2(a.dy, i - 1, j)), dts = dtq + L2(a.dt, i - 1, j);
+        const T cy = T(0.25) * (dxq + L2(a.dx, i, j - 1)), dty = dtq + L2(a.dt, i, j - 1);
+        const T fx = bw - b0, fy = bs - b0;
+        pdt += cx * uq * fx + cy * vq * fy;
+        if (a.b_u) {
+          T& r = ref_at(a.b_u, q.o3, ix);
+          r = r + cx * dts * fx;
+        }
+        if (a.b_v) {
+          T& r = ref_at(a.b_v, q.o3, ix);
+          r = r + cy * dty * fy;
+        }
+      }
+      // dt(i, j) as dt(i'-1) of face i' = i + 1 and dt(j'-1) of face j' = j + 1
+      if (i + 1 < im && j >= 1) pdt += T(0.25) * (L2(a.dy, i + 1, j) + dyq) * ue * (b0 - be);
+      if (j + 1 < jm && i >= 1) pdt += T(0.25) * (L2(a.dx, i, j + 1) + dxq) * vn * (b0 - bn);
     }
-    if (a.b_u) add_col(a.b_u + c, gu);
-    if (a.b_v) add_col(a.b_v + c, gv);
-    bdt += bx + by;
+#pragma unroll
+    for (int o = KB / 2; o > 0; o >>= 1) pdt += __shfl_xor(pdt, o, KB);
+    if (live && k == 0 && a.b_dt) {
+      T& r = ref_at(a.b_dt, q.o2, lix2(q, q.i, q.j));
+      r = r + pdt;
+    }
   }
-  // dt(i,j) as dt(i'-1) of face i' = i+1, and dt(j'-1) of face j' = j+1
-  if (i + 1 < p.im && j >= 1) {
-    const int64_t n2 = ix2(p, i + 1, j);
-    const T cx = T(0.25) * (a.dy[n2] + a.dy[c2]);
-    const int64_t nc = ix3(p, i + 1, j, 0);
-    #pragma unroll
-    for (int k = 0; k < KBM1; ++k) bdt += cx * a.u[nc + k] * (bdv(i, j, k) - bdv(i + 1, j, k));
-  }
-  if (j + 1 < p.jm && i >= 1) {
-    const int64_t n2 = ix2(p, i, j + 1);
-    const T cy = T(0.25) * (a.dx[n2] + a.dx[c2]);
-    const int64_t nc = ix3(p, i, j + 1, 0);
-    #pragma unroll
-    for (int k = 0; k < KBM1; ++k) bdt += cy * a.v[nc + k] * (bdv(i, j, k) - bdv(i, j + 1, k));
-  }
-  if (a.b_dt) a.b_dt[c2] += bdt;
 }
 
 // =====================================================================

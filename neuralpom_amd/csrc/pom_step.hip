@@ -331,20 +331,22 @@ template <class T> struct Step {
     for (int n = 1; n <= (imp ? 0 : isplit); ++n) {
       X<T>& x = xs[n];
       const bool last = n == isplit;
-      x.ua = last ? out[POM_UA] : A.get<T>(N2);
+      // the level shift makes ua, va, el of substep n its uaf, vaf, elf: one
+      // array each, except for the step's outputs
+      x.uaf = last ? out[POM_UAF] : A.get<T>(N2);
+      x.vaf = last ? out[POM_VAF] : A.get<T>(N2);
+      x.elf = last ? out[POM_ELF] : A.get<T>(N2);
+      x.ua = last ? out[POM_UA] : x.uaf;
       x.uab = last ? out[POM_UAB] : A.get<T>(N2);
-      x.va = last ? out[POM_VA] : A.get<T>(N2);
+      x.va = last ? out[POM_VA] : x.vaf;
       x.vab = last ? out[POM_VAB] : A.get<T>(N2);
-      x.el = last ? out[POM_EL] : A.get<T>(N2);
+      x.el = last ? out[POM_EL] : x.elf;
       x.elb = last ? out[POM_ELB] : A.get<T>(N2);
       x.d = last ? out[POM_D] : A.get<T>(N2);
       x.egf = last ? out[POM_EGF] : A.get<T>(N2);
       x.utf = last ? out[POM_UTF] : A.get<T>(N2);
       x.vtf = last ? out[POM_VTF] : A.get<T>(N2);
       x.etf = last ? out[POM_ETF] : A.get<T>(N2);
-      x.uaf = last ? out[POM_UAF] : A.get<T>(N2);
-      x.vaf = last ? out[POM_VAF] : A.get<T>(N2);
-      x.elf = last ? out[POM_ELF] : A.get<T>(N2);
       if (n % c.prm.ispadv == 0) {
         const bool lastadv = n + c.prm.ispadv > isplit;
         x.advua = lastadv ? out[POM_ADVUA] : A.get<T>(N2);
@@ -996,7 +998,9 @@ template <class T> struct Step {
       resolve(n, xn);
       {   // depth-mean momentum, then its filter / shift / time means at the same point, and max |vaf|
         ProfScope ps(POM_STAGE_EXTUV, c.stream);
-        const OutTab<T> of{{xn.ua, xn.uab, xn.va, xn.vab, xn.el, xn.elb, xn.d, xn.egf, xn.utf, xn.vtf, xn.etf}};
+        // ua, va, el are not stored where they share uaf's, vaf's, elf's array
+        const OutTab<T> of{{xn.ua == xn.uaf ? nullptr : xn.ua, xn.uab, xn.va == xn.vaf ? nullptr : xn.va, xn.vab,
+                            xn.el == xn.elf ? nullptr : xn.el, xn.elb, xn.d, xn.egf, xn.utf, xn.vtf, xn.etf}};
         if (open)
           launch_map2<T, OpExtUVTide, OpExtFilter, OpExtFilter::UAF, OpExtFilter::VAF>(
               cn, t_uv(xp, xn), OutTab<T>{{xn.uaf, xn.vaf}}, t_filter(xp, xn), of, vamax);
@@ -1622,7 +1626,8 @@ template <class T> struct Adj {
       VertvlAdjArgs<T> a{s.u1, s.v1, s.in[POM_DT], s.g[POM_G_DX], s.g[POM_G_DY], s.g[POM_G_FSM], bo[POM_W], s3a,
                          b_u1, b_v1, bi[POM_DT], b_etf30, bi[POM_ETB], s.imp && !s.open};
       hipLaunchKernelGGL(k_vertvl_adj1<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a);
-      hipLaunchKernelGGL(k_vertvl_adj2<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a);
+      hipLaunchKernelGGL(k_vertvl_adj2<T>, dim3(groups(c.nblk3, c.tune.flat3_bpg, c.tune.flat3_min)), tb, 0,
+                         c.stream, c.B, a);
     }
     {   // transport adjustment
       AdjustAdjArgs<T> a{s.in[POM_UTB], s.xf().utf, s.in[POM_VTB], s.xf().vtf, s.in[POM_DT],
