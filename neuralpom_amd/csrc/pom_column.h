@@ -32,6 +32,15 @@ template <class T> __device__ __forceinline__ void add_col(T* base, const T* x) 
 #pragma unroll
   for (int k = 0; k < KB; ++k) base[k] = y[k] + x[k];
 }
+// add_col, or a plain store when the kernel is the array's first writer
+template <class T> __device__ __forceinline__ void put_col(T* base, const T* x, bool assign) {
+  if (assign) {
+#pragma unroll
+    for (int k = 0; k < KB; ++k) base[k] = x[k];
+  } else {
+    add_col(base, x);
+  }
+}
 
 // ---------------------------------------------------------------------
 // Coalesced column I/O.  In a 2-D launch the TPB threads of a block own TPB
@@ -720,6 +729,7 @@ template <class T> struct ProftAdjArgs {
   const T* b_out;
   T *b_f, *b_kh, *b_etf;
   int nbc;
+  int assign_f;   // 1: b_f is assigned (this kernel is its first writer), 0: accumulated
 };
 template <class T>
 __global__ void __launch_bounds__(TPB) k_proft_adj(Batch B, Vert<T> vg, Params prm, ProftAdjArgs<T> s) {
@@ -810,7 +820,7 @@ __global__ void __launch_bounds__(TPB) k_proft_adj(Batch B, Vert<T> vg, Params p
     bkh[k] += bnum * (-dti2);
   }
   bdh += T(2) * dh * bdh2;
-  if (s.b_f) add_col(s.b_f + c, bf);
+  if (s.b_f) put_col(s.b_f + c, bf, s.assign_f != 0);
   if (s.b_kh) add_col(s.b_kh + c, bkh);
   if (s.b_etf) s.b_etf[c2] += bdh;
 }
@@ -920,6 +930,7 @@ template <class T> struct ProfuvAdjArgs {
   T *s_etf0, *s_etf1;     // scratch [2-D]
   T* s_mag;               // scratch [2-D]: cotangent of mag
   T* b_wbot_in;           // accumulated [2-D] (pass-through at non-interior columns)
+  int assign_f;           // 1: b_f is assigned (this kernel is its first writer), 0: accumulated
 };
 template <class T, int DIR>
 __global__ void __launch_bounds__(TPB) k_profuv_adj1(Batch B, Vert<T> vg, Params prm, ProfuvAdjArgs<T> s) {
@@ -932,7 +943,7 @@ __global__ void __launch_bounds__(TPB) k_profuv_adj1(Batch B, Vert<T> vg, Params
   if (!interior_pos(p)) {
     T g[KB];
     ld_col(s.b_out + c, g);
-    if (s.b_f) add_col(s.b_f + c, g);
+    if (s.b_f) put_col(s.b_f + c, g, s.assign_f != 0);
     if (s.b_wbot_in && s.b_wbot) s.b_wbot_in[c2] += s.b_wbot[c2];
     st_col(s.s_km0 + c, zero);
     st_col(s.s_km1 + c, zero);
@@ -1016,7 +1027,7 @@ __global__ void __launch_bounds__(TPB) k_profuv_adj1(Batch B, Vert<T> vg, Params
   bdhf += T(2) * C.dhf * bdh2;
   #pragma unroll
   for (int k = 0; k < KB; ++k) bkmf[k] *= T(0.5);
-  if (s.b_f) add_col(s.b_f + c, bf);
+  if (s.b_f) put_col(s.b_f + c, bf, s.assign_f != 0);
   st_col(s.s_km0 + c, bkmf);
   st_col(s.s_km1 + c, bkmf);
   s.s_etf0[c2] = T(0.5) * bdhf;
@@ -1124,35 +1135,42 @@ template <class T> struct UVFiltAdjArgs {
   const T *b_ub_new, *b_u_new;
   T *b_uf, *b_ub, *b_u1;
   const T* mask;
+  uint32_t assign;   // bit 0: b_uf, bit 2: b_u1 are assigned (their first writer), not accumulated
 };
+// one thread per cell, a column's 16 levels in 16 adjacent lanes; the column
+// sum of the masked ub cotangent by lane shuffles
 template <class T>
 __global__ void __launch_bounds__(TPB) k_uvfilt_adj(Batch B, Vert<T> vg, Params prm, UVFiltAdjArgs<T> a) {
-  Pos p;
-  if (!pos2(B, p)) return;
-  const int64_t c = ix3(p, p.i, p.j, 0);
+  const int nblk = B.blk3[B.M];
   const T f = T(0.5 * prm.smoth);
-  T g[KB], gt[KB], gu[KB], gf[KB];
-  #pragma unroll
-  for (int k = 0; k < KB; ++k) g[k] = a.b_ub_new ? a.b_ub_new[c + k] : T(0);
-  if (a.mask) {
-    const T mk = a.mask[ix2(p, p.i, p.j)];
-    #pragma unroll
-    for (int k = 0; k < KB; ++k) g[k] = g[k] * mk;
+  for (int blk = logical_block(); blk < nblk; blk += (int)gridDim.x) {   // uniform over the workgroup
+    Pos q;
+    const bool live = pos3_at(B, blk, q);   // a water column's 16 lanes are all live or all not
+    const int k = threadIdx.x & (KB - 1);
+    const uint32_t ix = live ? lix3(q, q.i, q.j, k) : 0u;
+    // every load issued before the column sum waits on the first
+    T g = T(0), un = T(0), mk = T(1), ouf = T(0), oub = T(0), ou1 = T(0), dz = T(0);
+    if (live) {
+      if (a.b_ub_new) g = ld_at(a.b_ub_new, q.o3, ix);
+      if (a.b_u_new) un = ld_at(a.b_u_new, q.o3, ix);
+      if (a.mask) mk = ld_at(a.mask, q.o2, lix2(q, q.i, q.j));
+      if (a.b_uf && !(a.assign & 1u)) ouf = ld_at(a.b_uf, q.o3, ix);
+      if (a.b_ub) oub = ld_at(a.b_ub, q.o3, ix);
+      if (a.b_u1 && !(a.assign & 4u)) ou1 = ld_at(a.b_u1, q.o3, ix);
+      dz = vg.dz(q.m, k);
+    }
+    if (a.mask) g = g * mk;
+    T S = k < KBM1 ? g : T(0);
+#pragma unroll
+    for (int o = KB / 2; o > 0; o >>= 1) S += __shfl_xor(S, o, KB);
+    if (!live) continue;
+    const T gt = k < KBM1 ? f * g - f * S * dz : T(0);
+    const T gu = k < KBM1 ? g - T(2) * gt : g;
+    const T gf = gt + un;
+    if (a.b_uf) ref_at(a.b_uf, q.o3, ix) = ouf + gf;
+    if (a.b_ub) ref_at(a.b_ub, q.o3, ix) = oub + gt;
+    if (a.b_u1) ref_at(a.b_u1, q.o3, ix) = ou1 + gu;
   }
-  T S = T(0);
-  #pragma unroll
-  for (int k = 0; k < KBM1; ++k) S += g[k];
-  #pragma unroll
-  for (int k = 0; k < KBM1; ++k) gt[k] = f * g[k] - f * S * vg.dz(p.m, k);
-  gt[KB - 1] = T(0);
-  #pragma unroll
-  for (int k = 0; k < KBM1; ++k) gu[k] = g[k] - T(2) * gt[k];
-  gu[KB - 1] = g[KB - 1];
-  #pragma unroll
-  for (int k = 0; k < KB; ++k) gf[k] = gt[k] + (a.b_u_new ? a.b_u_new[c + k] : T(0));
-  if (a.b_uf) add_col(a.b_uf + c, gf);
-  if (a.b_ub) add_col(a.b_ub + c, gt);
-  if (a.b_u1) add_col(a.b_u1 + c, gu);
 }
 
 // =====================================================================

@@ -272,6 +272,7 @@ template <class T> struct ProfqAdjArgs {
   // scratch between the reverse kernels: cotangents of l, gh, q2l, a, c, kn,
   // dtef (3-D) and of the bottom value q2bot and of dh (2-D)
   T *r_bl, *r_bgh, *r_bq2ln, *r_ba, *r_bc, *r_bkn, *r_bdtef, *r_bboy, *r_bq2bot, *r_bdh;
+  int assign_q2f;   // 1: b_q2f and b_q2lf are assigned (their first writer), 0: accumulated
 };
 
 
@@ -424,7 +425,7 @@ __global__ void __launch_bounds__(TPB) k_profq_adj_a(Batch B, Vert<T> vg, Params
     bdtef[k] += -bde * T(2) * dti2;
   }
   // gg1(0) = const * utau2: utau2 depends only on the passive surface stress
-  if (S.b_q2f) add_col(S.b_q2f + c3, bq2f);
+  if (S.b_q2f) put_col(S.b_q2f + c3, bq2f, S.assign_q2f != 0);
   st_col(S.r_ba + c3, ba);
   st_col(S.r_bc + c3, bc);
   st_col(S.r_bkn + c3, bkn);
@@ -503,7 +504,7 @@ __global__ void __launch_bounds__(TPB) k_profq_adj_b(Batch B, Vert<T> vg, Params
     bl[k] += bwall * W / (dh * kappa);
     bdh += -bwall * wall / dh;
   }
-  if (S.b_q2lf) add_col(S.b_q2lf + c3, bq2lf);
+  if (S.b_q2lf) put_col(S.b_q2lf + c3, bq2lf, S.assign_q2f != 0);
   add_col(S.r_ba + c3, ba);
   add_col(S.r_bc + c3, bc);
   add_col(S.r_bkn + c3, bkn);

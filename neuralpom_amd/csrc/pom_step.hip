@@ -98,6 +98,25 @@ __global__ void k_fill_multi(FillTab<T> t) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     p[i] = T(0);
 }
+// up to 24 sums d = s0 + s1 + s2 of equal-length arrays in one launch
+// (grid.y selects the entry; null sources are skipped, d may be one of them)
+template <class T> struct SumTab {
+  T* d[24];
+  const T* s[24][3];
+  int64_t n;
+};
+template <class T>
+__global__ void k_sum_multi(SumTab<T> t) {
+  const int e = blockIdx.y;
+  T* d = t.d[e];
+  const T *s0 = t.s[e][0], *s1 = t.s[e][1], *s2 = t.s[e][2];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < t.n; i += (int64_t)gridDim.x * blockDim.x) {
+    T v = s0[i];
+    if (s1) v += s1[i];
+    if (s2) v += s2[i];
+    d[i] = v;
+  }
+}
 template <class T>
 __global__ void k_copy(T* __restrict__ d, const T* __restrict__ s, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -1234,12 +1253,6 @@ template <class T> struct Adj {
                        c_advua, c_advva, p_ua, p_va, p_uab, p_vab, p_d, w<POM_AAM2D>()};
     launch_adj_advave(c, a);
   }
-  void zero_x(X<T>& x, int64_t n) {
-    for (T* p : {x.ua, x.uab, x.va, x.vab, x.el, x.elb, x.d, x.egf, x.utf, x.vtf, x.etf, x.advua, x.advva, x.uaf,
-                 x.vaf, x.elf})
-      zq(p, n);
-    zflush();
-  }
 
   void run(const T* const* cot_out, T* const* grad_in) {
     const int64_t N3 = S.n3(), N2 = S.n2();
@@ -1258,24 +1271,38 @@ template <class T> struct Adj {
       }
       if (!(grad_in && grad_in[f])) zq(bi[f], n);
     }
-    for (T* p : {b_rho1, b_u1, b_v1, b_q2f0, b_q2lf0, b_q2f1, b_q2lf1, b_kmraw, b_khraw, b_tf0, b_sf0, b_tf1, b_sf1,
-                 b_uf0, b_vf0, b_uf1, b_vf1, b_uf2, b_vf2})
-      zq(p, N3);
+    // with the internal mode the first writer of each of these assigns it (internal_rev)
+    if (!S.internal)
+      for (T* p : {b_rho1, b_u1, b_v1, b_q2f0, b_q2lf0, b_q2f1, b_q2lf1, b_kmraw, b_khraw, b_tf0, b_sf0, b_tf1,
+                   b_sf1, b_uf0, b_vf0, b_uf1, b_vf1, b_uf2, b_vf2})
+        zq(p, N3);
     for (T* p : {b_adx2d_raw, b_ady2d_raw, b_advua0, b_advva0, b_egf0, b_utf0, b_vtf0}) zq(p, N2);
     if (S.imp)
       for (T* p : {b_us, b_vs, b_dus, b_dvs, b_raw}) zq(p, N2);
     zflush();
     // ---- shifts reversed: egb<-egf, etb<-et, et<-etf, dt<-h+et, utb<-utf, vtb<-vtf
+    // (the final substep's cotangents assigned, et's accumulated, in one launch)
     X<T>& cur = bx[S.isplit & 1];
-    zero_x(cur, N2);
-    add(cur.ua, bo[POM_UA], N2); add(cur.uab, bo[POM_UAB], N2); add(cur.va, bo[POM_VA], N2);
-    add(cur.vab, bo[POM_VAB], N2); add(cur.el, bo[POM_EL], N2); add(cur.elb, bo[POM_ELB], N2);
-    add(cur.d, bo[POM_D], N2); add(cur.egf, bo[POM_EGF], N2); add(cur.egf, bo[POM_EGB], N2);
-    add(cur.utf, bo[POM_UTF], N2); add(cur.utf, bo[POM_UTB], N2); add(cur.vtf, bo[POM_VTF], N2);
-    add(cur.vtf, bo[POM_VTB], N2); add(cur.etf, bo[POM_ETF], N2); add(cur.etf, bo[POM_ET], N2);
-    add(cur.etf, bo[POM_DT], N2); add(cur.uaf, bo[POM_UAF], N2); add(cur.vaf, bo[POM_VAF], N2);
-    add(cur.elf, bo[POM_ELF], N2); add(cur.advua, bo[POM_ADVUA], N2); add(cur.advva, bo[POM_ADVVA], N2);
-    add(bi[POM_ET], bo[POM_ETB], N2);
+    {
+      SumTab<T> st{};
+      int ne = 0;
+      auto put = [&](T* d, const T* a, const T* b = nullptr, const T* e = nullptr) {
+        st.d[ne] = d;
+        st.s[ne][0] = a;
+        st.s[ne][1] = b;
+        st.s[ne][2] = e;
+        ++ne;
+      };
+      put(cur.ua, bo[POM_UA]); put(cur.uab, bo[POM_UAB]); put(cur.va, bo[POM_VA]); put(cur.vab, bo[POM_VAB]);
+      put(cur.el, bo[POM_EL]); put(cur.elb, bo[POM_ELB]); put(cur.d, bo[POM_D]);
+      put(cur.egf, bo[POM_EGF], bo[POM_EGB]); put(cur.utf, bo[POM_UTF], bo[POM_UTB]);
+      put(cur.vtf, bo[POM_VTF], bo[POM_VTB]); put(cur.etf, bo[POM_ETF], bo[POM_ET], bo[POM_DT]);
+      put(cur.uaf, bo[POM_UAF]); put(cur.vaf, bo[POM_VAF]); put(cur.elf, bo[POM_ELF]);
+      put(cur.advua, bo[POM_ADVUA]); put(cur.advva, bo[POM_ADVVA]);
+      put(bi[POM_ET], bi[POM_ET], bo[POM_ETB]);
+      st.n = N2;
+      hipLaunchKernelGGL(k_sum_multi<T>, dim3(flat_blocks(N2 / 8 > 0 ? N2 / 8 : 1), ne), dim3(256), 0, c.stream, st);
+    }
     // cotangents of the final substep fields used by the internal mode
     T* b_etf30 = cur.etf;
     T* b_egf30 = cur.egf;
@@ -1463,7 +1490,7 @@ template <class T> struct Adj {
   }
 
   void internal_rev(T* b_etf30, T* b_egf30, T* b_utf30, T* b_vtf30, T* b_d30) {
-    const dim3 g2(c.nblk2), tb(TPB);
+    const dim3 g2(c.nblk2), g3(groups(c.nblk3, c.tune.flat3_bpg, c.tune.flat3_min)), tb(TPB);
     Step<T>& s = S;
     // u/v Asselin filter and shift (with the Shapiro filter reversed first when on)
     if (s.shap_uv) {
@@ -1473,28 +1500,28 @@ template <class T> struct Adj {
       COutTab<T> ov{{bo[POM_VB]}};
       T* iv[OpShapiro<true>::NIN] = {s3d};
       launch_gather<T, OpShapiro<true>>(c, s.t_shap(s.vsh, s.g[POM_G_DVM]), ov, iv, 1u);
-      UVFiltAdjArgs<T> a{s3c, bo[POM_U], b_uf2, bi[POM_UB], b_u1, nullptr};
-      hipLaunchKernelGGL(k_uvfilt_adj<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a);
-      UVFiltAdjArgs<T> bv{s3d, bo[POM_V], b_vf2, bi[POM_VB], b_v1, nullptr};
-      hipLaunchKernelGGL(k_uvfilt_adj<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, bv);
+      UVFiltAdjArgs<T> a{s3c, bo[POM_U], b_uf2, bi[POM_UB], b_u1, nullptr, 5u};
+      hipLaunchKernelGGL(k_uvfilt_adj<T>, g3, tb, 0, c.stream, c.B, c.vert, c.prm, a);
+      UVFiltAdjArgs<T> bv{s3d, bo[POM_V], b_vf2, bi[POM_VB], b_v1, nullptr, 5u};
+      hipLaunchKernelGGL(k_uvfilt_adj<T>, g3, tb, 0, c.stream, c.B, c.vert, c.prm, bv);
     } else {
-      UVFiltAdjArgs<T> a{bo[POM_UB], bo[POM_U], b_uf2, bi[POM_UB], b_u1, s.imp ? s.g[POM_G_DUM] : nullptr};
-      hipLaunchKernelGGL(k_uvfilt_adj<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a);
-      UVFiltAdjArgs<T> bv{bo[POM_VB], bo[POM_V], b_vf2, bi[POM_VB], b_v1, s.imp ? s.g[POM_G_DVM] : nullptr};
-      hipLaunchKernelGGL(k_uvfilt_adj<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, bv);
+      UVFiltAdjArgs<T> a{bo[POM_UB], bo[POM_U], b_uf2, bi[POM_UB], b_u1, s.imp ? s.g[POM_G_DUM] : nullptr, 5u};
+      hipLaunchKernelGGL(k_uvfilt_adj<T>, g3, tb, 0, c.stream, c.B, c.vert, c.prm, a);
+      UVFiltAdjArgs<T> bv{bo[POM_VB], bo[POM_V], b_vf2, bi[POM_VB], b_v1, s.imp ? s.g[POM_G_DVM] : nullptr, 5u};
+      hipLaunchKernelGGL(k_uvfilt_adj<T>, g3, tb, 0, c.stream, c.B, c.vert, c.prm, bv);
     }
     if (s.open) {   // open bcond 3, edge weight from d
       COutTab<T> ob{{b_uf2, b_vf2}};
       T* ib[OpBc3D::NIN] = {b_uf1, b_vf1, b_u1, b_v1, b_d30};
-      launch_gather<T, OpBc3D>(c, s.t_bc3d(), ob, ib);
+      launch_gather<T, OpBc3D>(c, s.t_bc3d(), ob, ib, 3u);
     } else if (s.imp) {   // closed-basin bcond 3
       COutTab<T> ob{{b_uf2, b_vf2}};
       T* ib[OpImpBc3::NIN] = {b_uf1, b_vf1};
-      launch_gather<T, OpImpBc3>(c, s.t_ibc3(), ob, ib);
+      launch_gather<T, OpImpBc3>(c, s.t_ibc3(), ob, ib, 3u);
     } else {   // bcond 3
       COutTab<T> ob{{b_uf2, b_vf2}};
       T* ib[OpBc3::NIN] = {b_uf1, b_vf1, b_u1, b_v1};
-      launch_gather<T, OpBc3>(c, s.t_bc3(), ob, ib);
+      launch_gather<T, OpBc3>(c, s.t_bc3(), ob, ib, 3u);
     }
     for (int dir = 0; dir < 2; ++dir) {   // profu / profv
       ProfuvAdjArgs<T> a{};
@@ -1504,6 +1531,7 @@ template <class T> struct Adj {
       a.b_f = dir ? b_vf0 : b_uf0;
       a.s_km0 = s3a; a.s_km1 = s3b; a.s_etf0 = s2a; a.s_etf1 = s2b; a.s_mag = s2c;
       a.b_wbot_in = bi[dir ? POM_WVBOT : POM_WUBOT];
+      a.assign_f = 1;   // first writer of b_uf0 / b_vf0
       ProfuvAdj2Args<T> b2{s.in[POM_UB], s.in[POM_VB], s.g[POM_G_CBC], s3a, s3b, s2a, s2b, s2c,
                            w<POM_KM>(), b_etf30, bi[POM_UB], bi[POM_VB]};
       if (dir == 0) {
@@ -1529,13 +1557,13 @@ template <class T> struct Adj {
     {   // dens
       COutTab<T> ob{{bo[POM_RHO]}};
       T* ib[OpDens::NIN] = {w<POM_T>(), w<POM_S>(), b_rho1};
-      launch_gather<T, OpDens>(c, s.t_dens(), ob, ib);
+      launch_gather<T, OpDens>(c, s.t_dens(), ob, ib, 4u);   // first writer of b_rho1
     }
     if (s.open) {   // open bcond 4 with exterior values + T/S filter
       COutTab<T> ob{{bo[POM_T], bo[POM_TB], bo[POM_S], bo[POM_SB]}};
       T* ib[OpBc4FilterExt::NIN] = {b_tf1, b_sf1, bi[POM_T], bi[POM_S], bi[POM_TB], bi[POM_SB], b_u1, b_v1,
                                     w<POM_W>(), nullptr, nullptr, bi[POM_DT]};
-      launch_gather<T, OpBc4FilterExt>(c, s.t_bc4ext(), ob, ib);
+      launch_gather<T, OpBc4FilterExt>(c, s.t_bc4ext(), ob, ib, 3u);
     } else if (s.imp && s.shap_ts) {   // Shapiro filter of the new T, S, then bcond 4 + T/S filter
       COutTab<T> ot{{bo[POM_T]}};
       T* it[OpShapiro<false>::NIN] = {s3c};
@@ -1545,16 +1573,16 @@ template <class T> struct Adj {
       launch_gather<T, OpShapiro<false>>(c, s.t_shap(s.ssh, s.g[POM_G_FSM]), os, is, 1u);
       COutTab<T> ob{{s3c, bo[POM_TB], s3d, bo[POM_SB]}};
       T* ib[OpImpBc4FilterG<false>::NIN] = {b_tf1, b_sf1, bi[POM_T], bi[POM_S], bi[POM_TB], bi[POM_SB]};
-      launch_gather<T, OpImpBc4FilterG<false>>(c, s.t_ibc4(), ob, ib);
+      launch_gather<T, OpImpBc4FilterG<false>>(c, s.t_ibc4(), ob, ib, 3u);
     } else if (s.imp) {   // closed-basin bcond 4 + T/S filter + tracer mask
       COutTab<T> ob{{bo[POM_T], bo[POM_TB], bo[POM_S], bo[POM_SB]}};
       T* ib[OpImpBc4Filter::NIN] = {b_tf1, b_sf1, bi[POM_T], bi[POM_S], bi[POM_TB], bi[POM_SB]};
-      launch_gather<T, OpImpBc4Filter>(c, s.t_ibc4(), ob, ib);
+      launch_gather<T, OpImpBc4Filter>(c, s.t_ibc4(), ob, ib, 3u);
     } else {   // bcond 4 + T/S filter
       COutTab<T> ob{{bo[POM_T], bo[POM_TB], bo[POM_S], bo[POM_SB]}};
       T* ib[OpBc4Filter::NIN] = {b_tf1, b_sf1, bi[POM_T], bi[POM_S], bi[POM_TB], bi[POM_SB], b_u1, b_v1,
                                  w<POM_W>(), bi[POM_DT]};
-      launch_gather<T, OpBc4Filter>(c, s.t_bc4(), ob, ib);
+      launch_gather<T, OpBc4Filter>(c, s.t_bc4(), ob, ib, 3u);
     }
     for (int sal = 0; sal < 2; ++sal) {   // proft
       ProftAdjArgs<T> a{};
@@ -1565,6 +1593,7 @@ template <class T> struct Adj {
       a.b_kh = w<POM_KH>();
       a.b_etf = b_etf30;
       a.nbc = f.nbc;
+      a.assign_f = 1;   // first writer of b_tf0 / b_sf0
       hipLaunchKernelGGL(k_proft_adj<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a);
     }
     {   // advt1 of T and S
@@ -1580,16 +1609,16 @@ template <class T> struct Adj {
     if (s.imp && !s.open) {   // closed-basin bcond 6 + q2 filter
       COutTab<T> ob{{bo[POM_Q2], bo[POM_Q2B], bo[POM_Q2L], bo[POM_Q2LB]}};
       T* ib[OpImpBc6Filter::NIN] = {b_q2f1, b_q2lf1, bi[POM_Q2], bi[POM_Q2L], bi[POM_Q2B], bi[POM_Q2LB]};
-      launch_gather<T, OpImpBc6Filter>(c, s.t_ibc6(), ob, ib);
+      launch_gather<T, OpImpBc6Filter>(c, s.t_ibc6(), ob, ib, 3u);
     } else {   // bcond 6 + q2 filter
       COutTab<T> ob{{bo[POM_Q2], bo[POM_Q2B], bo[POM_Q2L], bo[POM_Q2LB]}};
       T* ib[OpBc6Filter::NIN] = {b_q2f1, b_q2lf1, bi[POM_Q2], bi[POM_Q2L], bi[POM_Q2B], bi[POM_Q2LB], b_u1, b_v1};
-      launch_gather<T, OpBc6Filter>(c, s.t_bc6(), ob, ib);
+      launch_gather<T, OpBc6Filter>(c, s.t_bc6(), ob, ib, 3u);
     }
     {   // km/kh edge copy
       COutTab<T> ob{{bo[POM_KM], bo[POM_KH]}};
       T* ib[OpKEdge::NIN] = {b_kmraw, b_khraw};
-      launch_gather<T, OpKEdge>(c, s.t_kedge(), ob, ib);
+      launch_gather<T, OpKEdge>(c, s.t_kedge(), ob, ib, 3u);
     }
     {   // profq
       ProfqAdjArgs<T> a{};
@@ -1603,6 +1632,7 @@ template <class T> struct Adj {
       a.s_unb = s3a; a.s_vnb = s3b; a.s_wubnb = s2a; a.s_wvbnb = s2b;
       a.r_bl = q_bl; a.r_bgh = q_bgh; a.r_bq2ln = q_bq2ln; a.r_ba = q_ba; a.r_bc = q_bc; a.r_bkn = q_bkn;
       a.r_bdtef = q_bdtef; a.r_bboy = q_bboy; a.r_bq2bot = q_bq2bot; a.r_bdh = q_bdh;
+      a.assign_q2f = 1;   // first writer of b_q2f0 / b_q2lf0
       {
         ProfScope ps(POM_STAGE_PROFQ_ADJ, c.stream);
         hipLaunchKernelGGL(k_profq_adj_a<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a);
