@@ -185,7 +185,9 @@ int pom_explicit_workspace(const pom_batch* b, const pom_params_t* prm, int32_t 
  *            time0 = 0 (time_loop.py:438)
  *   vamax_out: device scalar receiving max |vaf| over all substeps
  * Replaces POM2K_Core.run_time_loop's loop body; entries of state_out that the
- * step leaves unchanged are copied from state_in. */
+ * step leaves unchanged are copied from state_in.  The state_out arrays are
+ * distinct from each other and from every state_in and geom array (the step
+ * reads its inputs after writing outputs). */
 int pom_explicit_step(const pom_batch* b, const pom_params_t* prm, const void* ramp, const void* vgrid,
                       const void* const* state_in, void* const* state_out, const void* const* geom,
                       int32_t internal, void* vamax_out, void* workspace, size_t ws_bytes, void* stream);

@@ -1103,13 +1103,21 @@ template <class T> struct Step {
       copy(out[POM_TPS], xl.tps, N2);
     }
     // shifts (time_loop.py:680-696)
-    copy(out[POM_EGB], xl.egf, N2);
-    copy(out[POM_ETB], in[POM_ET], N2);
-    copy(out[POM_ET], xl.etf, N2);
-    hipLaunchKernelGGL(k_hplus<T>, dim3(blocks(N2)), dim3(256), 0, c.stream, out[POM_DT], g[POM_G_H], xl.etf, N2);
-    copy(out[POM_UTB], xl.utf, N2);
-    copy(out[POM_VTB], xl.vtf, N2);
-    copy(out[POM_VFLUXB], out[POM_VFLUXF], N2);
+    // (one launch: egb, etb, et, dt = h + et, utb, vtb, vfluxb)
+    {
+      SumTab<T> st{};
+      const T* src[7][2] = {{xl.egf, nullptr}, {in[POM_ET], nullptr}, {xl.etf, nullptr}, {g[POM_G_H], xl.etf},
+                            {xl.utf, nullptr}, {xl.vtf, nullptr}, {out[POM_VFLUXF], nullptr}};
+      T* dst[7] = {out[POM_EGB], out[POM_ETB], out[POM_ET], out[POM_DT], out[POM_UTB], out[POM_VTB], out[POM_VFLUXB]};
+      for (int e = 0; e < 7; ++e) {
+        st.d[e] = dst[e];
+        st.s[e][0] = src[e][0];
+        st.s[e][1] = src[e][1];
+        st.s[e][2] = nullptr;
+      }
+      st.n = N2;
+      hipLaunchKernelGGL(k_sum_multi<T>, dim3(flat_blocks(N2 / 8 > 0 ? N2 / 8 : 1), 7), dim3(256), 0, c.stream, st);
+    }
   }
 };
 
@@ -1368,16 +1376,24 @@ template <class T> struct Adj {
       }
       X<T>& b0 = bx[0];
       fold_forcing(b0, false);   // the first period's forcing cotangents (its accumulator is read below)
-      // no substep reads the entering elf, and nothing assigned b0.elf
-      zq(b0.elf, N2);
-      zflush();
-      // X_0 -> inputs and pre-loop intermediates
-      add(bi[POM_UA], b0.ua, N2); add(bi[POM_UAB], b0.uab, N2); add(bi[POM_VA], b0.va, N2);
-      add(bi[POM_VAB], b0.vab, N2); add(bi[POM_EL], b0.el, N2); add(bi[POM_ELB], b0.elb, N2);
-      add(bi[POM_D], b0.d, N2); add(bi[POM_ETF], b0.etf, N2); add(bi[POM_UAF], b0.uaf, N2);
-      add(bi[POM_VAF], b0.vaf, N2); add(bi[POM_ELF], b0.elf, N2);
-      add(b_egf0, b0.egf, N2); add(b_utf0, b0.utf, N2); add(b_vtf0, b0.vtf, N2);
-      add(b_advua0, b0.advua, N2); add(b_advva0, b0.advva, N2);
+      // X_0 -> inputs and pre-loop intermediates, in one launch (no substep
+      // reads the entering elf: its cotangent is zero and is not added)
+      SumTab<T> st{};
+      int ne = 0;
+      auto acc = [&](T* d, const T* a) {
+        st.d[ne] = d;
+        st.s[ne][0] = d;
+        st.s[ne][1] = a;
+        st.s[ne][2] = nullptr;
+        ++ne;
+      };
+      acc(bi[POM_UA], b0.ua); acc(bi[POM_UAB], b0.uab); acc(bi[POM_VA], b0.va); acc(bi[POM_VAB], b0.vab);
+      acc(bi[POM_EL], b0.el); acc(bi[POM_ELB], b0.elb); acc(bi[POM_D], b0.d); acc(bi[POM_ETF], b0.etf);
+      acc(bi[POM_UAF], b0.uaf); acc(bi[POM_VAF], b0.vaf);
+      acc(b_egf0, b0.egf); acc(b_utf0, b0.utf); acc(b_vtf0, b0.vtf); acc(b_advua0, b0.advua);
+      acc(b_advva0, b0.advva);
+      st.n = N2;
+      hipLaunchKernelGGL(k_sum_multi<T>, dim3(flat_blocks(N2 / 8 > 0 ? N2 / 8 : 1), ne), dim3(256), 0, c.stream, st);
     }
     // pre-loop: adx2d = raw - advua0 etc.
     {
