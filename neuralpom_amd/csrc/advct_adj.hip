@@ -3,7 +3,11 @@
 
 namespace pom {
 
-template <class T>
+// PART 0: the cotangents of u and v and the advective and curvature share of
+// dt's; PART 1: those of ub, vb, aam and the viscous share of dt's.  The two
+// parts read the same output cotangents and hold half the live values each
+// (one kernel with all five outputs held 175 VGPRs, two waves per SIMD).
+template <class T, int PART>
 __global__ void __launch_bounds__(TPB) k_adj_advct(Batch B, AdvctAdjArgs<T> a) {
   const int nblk = B.blk3[B.M];
   const T c8 = T(0.125), c4 = T(0.25), two = T(2), eps = T(1e-20);
@@ -87,66 +91,76 @@ __global__ void __launch_bounds__(TPB) k_adj_advct(Batch B, AdvctAdjArgs<T> a) {
         return (DT(di, dj) + DT(di - 1, dj)) * U(di, dj) + (DT(di, dj - 1) + DT(di - 1, dj - 1)) * U(di, dj - 1);
       };
       const T dtq = DT(0, 0), dtw = DT(-1, 0), dts = DT(0, -1);
-      const T gu0 = (L3(a.ub, 1, 0) - L3(a.ub, 0, 0)) * rcp_lane(DX(0, 0) + eps);
-      const T gv0 = (L3(a.vb, 0, 1) - L3(a.vb, 0, 0)) * rcp_lane(DY(0, 0) + eps);
-      const T cXu0 = cXu(0), cXuw = cXu(-1), cYv0 = cYv(0), cYvs = cYv(-1);
-      const T cCV00 = cCV(0, 0), cCV10 = cCV(1, 0), cCV01 = cCV(0, 1), cCV11 = cCV(1, 1);
-      const T curv0 = curv(0, 0);
 
-      // ---- u(q): Xu(q-i) (as u(a+i)), Xu(q) (as u(a)), Yu(q), Yu(q+j), Xv(q), Xv(q+j), curv and P2 at q-i, q
-      T cu = (cXuw * DY(-1, 0) * c8 * ((dtq + dtw) * xuS(-1) + xuT(-1)) +
-              cXu0 * DY(0, 0) * c8 * ((dtq + dtw) * xuS(0) + xuT(0))) +
-             c4 * c8 * (cYu(0, 0) * s4x(0, 0) * yuV(0, 0) + cYu(0, 1) * s4x(0, 1) * yuV(0, 1)) +
-             c4 * c8 * (dtq + dtw) *
-                 (cXv(0, 0) * s4y(0, 0) * (V(0, 0) + V(-1, 0)) + cXv(0, 1) * s4y(0, 1) * (V(0, 1) + V(-1, 1))) +
-             ccurv(-1, 0) * dcu(-1, 0) + ccurv(0, 0) * dcu(0, 0) + Kv(-1, 0) * curv(-1, 0) * dtw + Kv(0, 0) * curv0 * dtq;
-      // ---- v(q): Yv(q-j) (as v(b+j)), Yv(q) (as v(b)), Xv(q), Xv(q+i), Yu(q), Yu(q+i), curv and P1 at q-j, q
-      T cv = (cYvs * DX(0, -1) * c8 * ((dtq + dts) * yvS(-1) + yvW(-1)) +
-              cYv0 * DX(0, 0) * c8 * ((dtq + dts) * yvS(0) + yvW(0))) +
-             c4 * c8 * (cXv(0, 0) * s4y(0, 0) * xvU(0, 0) + cXv(1, 0) * s4y(1, 0) * xvU(1, 0)) +
-             c4 * c8 * (dtq + dts) *
-                 (cYu(0, 0) * s4x(0, 0) * (U(0, 0) + U(0, -1)) + cYu(1, 0) * s4x(1, 0) * (U(1, 0) + U(1, -1))) +
-             ccurv(0, -1) * dcv(0, -1) + ccurv(0, 0) * dcv(0, 0) + Ku(0, -1) * curv(0, -1) * dts + Ku(0, 0) * curv0 * dtq;
-      // ---- ub(q): the viscous part of Xu(q-i), Xu(q); CV(q), CV(q+j)
-      const T cub = cXuw * (-DY(-1, 0) * dtw * AM(-1, 0) * two * rcp_lane(DX(-1, 0) + eps)) +
-                    cXu0 * (DY(0, 0) * dtq * AM(0, 0) * two * rcp_lane(DX(0, 0) + eps)) +
-                    cCV00 * dt4(0, 0) * am4(0, 0) * rdy4(0, 0) - cCV01 * dt4(0, 1) * am4(0, 1) * rdy4(0, 1);
-      // ---- vb(q): the viscous part of Yv(q-j), Yv(q); CV(q), CV(q+i)
-      const T cvb = cYvs * (-DX(0, -1) * dts * AM(0, -1) * two * rcp_lane(DY(0, -1) + eps)) +
-                    cYv0 * (DX(0, 0) * dtq * AM(0, 0) * two * rcp_lane(DY(0, 0) + eps)) +
-                    cCV00 * dt4(0, 0) * am4(0, 0) * rdx4(0, 0) - cCV10 * dt4(1, 0) * am4(1, 0) * rdx4(1, 0);
-      // ---- aam(q): Xu(q), Yv(q), the four corners CV(q), CV(q+i), CV(q+j), CV(q+i+j)
-      const T e00 = duy(0, 0) + dvx(0, 0), e10 = duy(1, 0) + dvx(1, 0), e01 = duy(0, 1) + dvx(0, 1),
-              e11 = duy(1, 1) + dvx(1, 1);
-      const T caam = cXu0 * (-DY(0, 0) * dtq * two * gu0) + cYv0 * (-DX(0, 0) * dtq * two * gv0) +
-                     cCV00 * dt4(0, 0) * e00 + cCV10 * dt4(1, 0) * e10 + cCV01 * dt4(0, 1) * e01 +
-                     cCV11 * dt4(1, 1) * e11;
-      // ---- dt(q), this level's partials
-      pdt = cXu0 * DY(0, 0) * (c8 * xuS(0) * xuS(0) - two * AM(0, 0) * gu0) +
-            cXuw * DY(-1, 0) * c8 * xuS(-1) * U(0, 0) + cXu(1) * DY(1, 0) * c8 * xuS(1) * U(1, 0) +
-            c4 * c8 *
-                (cYu(0, 0) * s4x(0, 0) * (U(0, 0) + U(0, -1)) * V(0, 0) +
-                 cYu(0, 1) * s4x(0, 1) * (U(0, 1) + U(0, 0)) * V(0, 1) +
-                 cYu(1, 0) * s4x(1, 0) * (U(1, 0) + U(1, -1)) * V(0, 0) +
-                 cYu(1, 1) * s4x(1, 1) * (U(1, 1) + U(1, 0)) * V(0, 1)) +
-            c4 * c8 *
-                (cXv(0, 0) * s4y(0, 0) * (V(0, 0) + V(-1, 0)) * U(0, 0) +
-                 cXv(1, 0) * s4y(1, 0) * (V(1, 0) + V(0, 0)) * U(1, 0) +
-                 cXv(0, 1) * s4y(0, 1) * (V(0, 1) + V(-1, 1)) * U(0, 0) +
-                 cXv(1, 1) * s4y(1, 1) * (V(1, 1) + V(0, 1)) * U(1, 0)) +
-            cYv0 * DX(0, 0) * (c8 * yvS(0) * yvS(0) - two * AM(0, 0) * gv0) +
-            cYvs * DX(0, -1) * c8 * yvS(-1) * V(0, 0) + cYv(1) * DX(0, 1) * c8 * yvS(1) * V(0, 1) +
-            c4 * (cCV00 * am4(0, 0) * e00 + cCV10 * am4(1, 0) * e10 + cCV01 * am4(0, 1) * e01 + cCV11 * am4(1, 1) * e11) +
-            Ku(0, 0) * curv0 * (V(0, 1) + V(0, 0)) + Kv(0, 0) * curv0 * (U(1, 0) + U(0, 0));
-      // accumulate: every old value read before the first store
       const uint32_t ix = lix3(q, qi, qj, k);
-      T* const dst[5] = {a.c_u, a.c_v, a.c_ub, a.c_vb, a.c_aam};
-      const T add[5] = {cu, cv, cub, cvb, caam};
-      T old[5];
-#pragma unroll
-      for (int n = 0; n < 5; ++n) old[n] = ld_at(dst[n], q.o3, ix);
-#pragma unroll
-      for (int n = 0; n < 5; ++n) ref_at(dst[n], q.o3, ix) = old[n] + add[n];
+      if constexpr (PART == 0) {
+        const T cXu0 = cXu(0), cXuw = cXu(-1), cYv0 = cYv(0), cYvs = cYv(-1);
+        const T curv0 = curv(0, 0);
+        // ---- u(q): Xu(q-i) (as u(a+i)), Xu(q) (as u(a)), Yu(q), Yu(q+j), Xv(q), Xv(q+j), curv and P2 at q-i, q
+        const T cu = (cXuw * DY(-1, 0) * c8 * ((dtq + dtw) * xuS(-1) + xuT(-1)) +
+                      cXu0 * DY(0, 0) * c8 * ((dtq + dtw) * xuS(0) + xuT(0))) +
+                     c4 * c8 * (cYu(0, 0) * s4x(0, 0) * yuV(0, 0) + cYu(0, 1) * s4x(0, 1) * yuV(0, 1)) +
+                     c4 * c8 * (dtq + dtw) *
+                         (cXv(0, 0) * s4y(0, 0) * (V(0, 0) + V(-1, 0)) + cXv(0, 1) * s4y(0, 1) * (V(0, 1) + V(-1, 1))) +
+                     ccurv(-1, 0) * dcu(-1, 0) + ccurv(0, 0) * dcu(0, 0) + Kv(-1, 0) * curv(-1, 0) * dtw +
+                     Kv(0, 0) * curv0 * dtq;
+        // ---- v(q): Yv(q-j) (as v(b+j)), Yv(q) (as v(b)), Xv(q), Xv(q+i), Yu(q), Yu(q+i), curv and P1 at q-j, q
+        const T cv = (cYvs * DX(0, -1) * c8 * ((dtq + dts) * yvS(-1) + yvW(-1)) +
+                      cYv0 * DX(0, 0) * c8 * ((dtq + dts) * yvS(0) + yvW(0))) +
+                     c4 * c8 * (cXv(0, 0) * s4y(0, 0) * xvU(0, 0) + cXv(1, 0) * s4y(1, 0) * xvU(1, 0)) +
+                     c4 * c8 * (dtq + dts) *
+                         (cYu(0, 0) * s4x(0, 0) * (U(0, 0) + U(0, -1)) + cYu(1, 0) * s4x(1, 0) * (U(1, 0) + U(1, -1))) +
+                     ccurv(0, -1) * dcv(0, -1) + ccurv(0, 0) * dcv(0, 0) + Ku(0, -1) * curv(0, -1) * dts +
+                     Ku(0, 0) * curv0 * dtq;
+        // ---- dt(q), this level's advective and curvature partials
+        pdt = cXu0 * DY(0, 0) * c8 * xuS(0) * xuS(0) +
+              cXuw * DY(-1, 0) * c8 * xuS(-1) * U(0, 0) + cXu(1) * DY(1, 0) * c8 * xuS(1) * U(1, 0) +
+              c4 * c8 *
+                  (cYu(0, 0) * s4x(0, 0) * (U(0, 0) + U(0, -1)) * V(0, 0) +
+                   cYu(0, 1) * s4x(0, 1) * (U(0, 1) + U(0, 0)) * V(0, 1) +
+                   cYu(1, 0) * s4x(1, 0) * (U(1, 0) + U(1, -1)) * V(0, 0) +
+                   cYu(1, 1) * s4x(1, 1) * (U(1, 1) + U(1, 0)) * V(0, 1)) +
+              c4 * c8 *
+                  (cXv(0, 0) * s4y(0, 0) * (V(0, 0) + V(-1, 0)) * U(0, 0) +
+                   cXv(1, 0) * s4y(1, 0) * (V(1, 0) + V(0, 0)) * U(1, 0) +
+                   cXv(0, 1) * s4y(0, 1) * (V(0, 1) + V(-1, 1)) * U(0, 0) +
+                   cXv(1, 1) * s4y(1, 1) * (V(1, 1) + V(0, 1)) * U(1, 0)) +
+              cYv0 * DX(0, 0) * c8 * yvS(0) * yvS(0) +
+              cYvs * DX(0, -1) * c8 * yvS(-1) * V(0, 0) + cYv(1) * DX(0, 1) * c8 * yvS(1) * V(0, 1) +
+              Ku(0, 0) * curv0 * (V(0, 1) + V(0, 0)) + Kv(0, 0) * curv0 * (U(1, 0) + U(0, 0));
+        // accumulate: every old value read before the first store
+        const T ou = ld_at(a.c_u, q.o3, ix), ov = ld_at(a.c_v, q.o3, ix);
+        ref_at(a.c_u, q.o3, ix) = ou + cu;
+        ref_at(a.c_v, q.o3, ix) = ov + cv;
+      } else {
+        const T gu0 = (L3(a.ub, 1, 0) - L3(a.ub, 0, 0)) * rcp_lane(DX(0, 0) + eps);
+        const T gv0 = (L3(a.vb, 0, 1) - L3(a.vb, 0, 0)) * rcp_lane(DY(0, 0) + eps);
+        const T cXu0 = cXu(0), cXuw = cXu(-1), cYv0 = cYv(0), cYvs = cYv(-1);
+        const T cCV00 = cCV(0, 0), cCV10 = cCV(1, 0), cCV01 = cCV(0, 1), cCV11 = cCV(1, 1);
+        // ---- ub(q): the viscous part of Xu(q-i), Xu(q); CV(q), CV(q+j)
+        const T cub = cXuw * (-DY(-1, 0) * dtw * AM(-1, 0) * two * rcp_lane(DX(-1, 0) + eps)) +
+                      cXu0 * (DY(0, 0) * dtq * AM(0, 0) * two * rcp_lane(DX(0, 0) + eps)) +
+                      cCV00 * dt4(0, 0) * am4(0, 0) * rdy4(0, 0) - cCV01 * dt4(0, 1) * am4(0, 1) * rdy4(0, 1);
+        // ---- vb(q): the viscous part of Yv(q-j), Yv(q); CV(q), CV(q+i)
+        const T cvb = cYvs * (-DX(0, -1) * dts * AM(0, -1) * two * rcp_lane(DY(0, -1) + eps)) +
+                      cYv0 * (DX(0, 0) * dtq * AM(0, 0) * two * rcp_lane(DY(0, 0) + eps)) +
+                      cCV00 * dt4(0, 0) * am4(0, 0) * rdx4(0, 0) - cCV10 * dt4(1, 0) * am4(1, 0) * rdx4(1, 0);
+        // ---- aam(q): Xu(q), Yv(q), the four corners CV(q), CV(q+i), CV(q+j), CV(q+i+j)
+        const T e00 = duy(0, 0) + dvx(0, 0), e10 = duy(1, 0) + dvx(1, 0), e01 = duy(0, 1) + dvx(0, 1),
+                e11 = duy(1, 1) + dvx(1, 1);
+        const T caam = cXu0 * (-DY(0, 0) * dtq * two * gu0) + cYv0 * (-DX(0, 0) * dtq * two * gv0) +
+                       cCV00 * dt4(0, 0) * e00 + cCV10 * dt4(1, 0) * e10 + cCV01 * dt4(0, 1) * e01 +
+                       cCV11 * dt4(1, 1) * e11;
+        // ---- dt(q), this level's viscous partials
+        pdt = -cXu0 * DY(0, 0) * two * AM(0, 0) * gu0 - cYv0 * DX(0, 0) * two * AM(0, 0) * gv0 +
+              c4 * (cCV00 * am4(0, 0) * e00 + cCV10 * am4(1, 0) * e10 + cCV01 * am4(0, 1) * e01 +
+                    cCV11 * am4(1, 1) * e11);
+        // accumulate: every old value read before the first store
+        const T oub = ld_at(a.c_ub, q.o3, ix), ovb = ld_at(a.c_vb, q.o3, ix), oam = ld_at(a.c_aam, q.o3, ix);
+        ref_at(a.c_ub, q.o3, ix) = oub + cub;
+        ref_at(a.c_vb, q.o3, ix) = ovb + cvb;
+        ref_at(a.c_aam, q.o3, ix) = oam + caam;
+      }
     }
 #pragma unroll
     for (int o = KB / 2; o > 0; o >>= 1) pdt += __shfl_xor(pdt, o, KB);
@@ -159,8 +173,9 @@ __global__ void __launch_bounds__(TPB) k_adj_advct(Batch B, AdvctAdjArgs<T> a) {
 
 template <class T> void launch_adj_advct(const Ctx<T>& c, const AdvctAdjArgs<T>& a) {
   if (c.nblk3 == 0) return;
-  hipLaunchKernelGGL((k_adj_advct<T>), dim3(groups(c.nblk3, c.tune.flat3_bpg, c.tune.flat3_min)), dim3(TPB), 0,
-                     c.stream, c.B, a);
+  const dim3 grid(groups(c.nblk3, c.tune.flat3_bpg, c.tune.flat3_min));
+  hipLaunchKernelGGL((k_adj_advct<T, 0>), grid, dim3(TPB), 0, c.stream, c.B, a);
+  hipLaunchKernelGGL((k_adj_advct<T, 1>), grid, dim3(TPB), 0, c.stream, c.B, a);
 }
 
 template void launch_adj_advct<float>(const Ctx<float>&, const AdvctAdjArgs<float>&);

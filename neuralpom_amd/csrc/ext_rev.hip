@@ -299,40 +299,67 @@ __global__ void __launch_bounds__(TPB) k_ext_fold(Batch B, ExtFoldArgs<T> a) {
 template <class T, bool TIDE>
 __global__ void __launch_bounds__(TPB) k_ext_elf(Batch B, Params prm, ExtElfArgs<T> a) {
   const int nblk = B.blk2[B.M];
-  const T dte2 = T(prm.dte2), qa = T(0.25);
   for (int blk = logical_block(); blk < nblk; blk += (int)gridDim.x) {
     Pos q;
     if (!pos2_at(B, blk, q)) continue;
-    const L2<T> L{q};
-    const int im = q.im, jm = q.jm;
-    // the interior point p copies (OpBc1: edges take their nearest interior
-    // point, corners the diagonal one); with the tide policy the flux terms
-    // below are used at interior points only and this keeps their loads in
-    // the member
-    const int mi = q.i == 0 ? 1 : (q.i == im - 1 ? im - 2 : q.i);
-    const int mj = q.j == 0 ? 1 : (q.j == jm - 1 ? jm - 2 : q.j);
-    auto fu = [&](int i, int j) {
-      return qa * (L(a.d, i, j) + L(a.d, i - 1, j)) * (L(a.dy, i, j) + L(a.dy, i - 1, j)) * L(a.ua, i, j);
-    };
-    auto fv = [&](int i, int j) {
-      return qa * (L(a.d, i, j) + L(a.d, i, j - 1)) * (L(a.dx, i, j) + L(a.dx, i, j - 1)) * L(a.va, i, j);
-    };
-    const T div = fu(mi + 1, mj) - fu(mi, mj) + fv(mi, mj + 1) - fv(mi, mj);
-    const T term = -(div / L(a.art, mi, mj)) - L(a.vfluxf, mi, mj);
-    const T raw = L(a.elb, mi, mj) + dte2 * term;
-    const bool in = mi >= 1 && mi <= im - 2 && mj >= 1 && mj <= jm - 2;
-    const uint32_t ix = lix2(q, q.i, q.j);
-    if constexpr (TIDE) {
-      const bool inside = q.i >= 1 && q.i <= im - 2 && q.j >= 1 && q.j <= jm - 2;
-      T v = inside ? raw : ld_at(a.elf_prev, q.o2, ix);
-      if (ld_at(a.node, q.o2, ix) != T(0))
-        v = T(double(ld_at(a.amp, q.o2, ix)) * cos(a.wt - double(ld_at(a.phase, q.o2, ix))));
-      st2(q, a.elf, ix, v * ld_at(a.fsm, q.o2, ix));
-    } else {
-      st2(q, a.elf, ix, (in ? raw : T(0)) * ld_at(a.fsm, q.o2, ix));
-    }
+    st2(q, a.elf, lix2(q, q.i, q.j), ext_elf_at<T, TIDE>(q, q.i, q.j, a, prm.dte2));
   }
 }
+
+// the fused forward substep (pom_extrev.h launch_ext_substep)
+template <class T>
+__global__ void __launch_bounds__(TPB) k_ext_substep(Batch B, Vert<T> vert, const T* ramp, Params prm,
+                                                     ExtElfArgs<T> e, FieldTab<T> ta, OutTab<T> oa, FieldTab<T> tb,
+                                                     OutTab<T> ob, T* amax) {
+  T vmax = T(0);
+  for (int blk = logical_block(); blk < B.blk2[B.M]; blk += (int)gridDim.x) {
+    Pos p;
+    if (pos2_at(B, blk, p)) {
+      const uint32_t li = lix2(p, p.i, p.j);
+      // elf where OpExtUV and OpExtFilter read it; a neighbour off the member
+      // is clamped onto it, as the memory reads of the two-launch form are
+      const T e0 = ext_elf_at<T, false>(p, p.i, p.j, e, prm.dte2);
+      const T ew = ext_elf_at<T, false>(p, clampi(p.i - 1, p.im), p.j, e, prm.dte2);
+      const T es = ext_elf_at<T, false>(p, p.i, clampi(p.j - 1, p.jm), e, prm.dte2);
+      T ra[2], rb[OpExtFilter::NOUT];
+      {
+        const AccElf<T, OpExtUV::ELF> A{{&ta, p, vert, ramp}, e0, ew, es, T(0), T(0)};
+        ra[0] = val(OpExtUV::template eval<0>(A, prm));
+        ra[1] = val(OpExtUV::template eval<1>(A, prm));
+      }
+      {
+        const AccElf<T, OpExtFilter::ELF, OpExtFilter::UAF, OpExtFilter::VAF> A{{&tb, p, vert, ramp}, e0, ew, es,
+                                                                               ra[0], ra[1]};
+        static_for<0, OpExtFilter::NOUT - 1>([&](auto O) {
+          constexpr int o = decltype(O)::value;
+          rb[o] = val(OpExtFilter::template eval<o>(A, prm));
+        });
+      }
+      vmax = nanmax(vmax, fabs(ra[1]));
+      st2(p, e.elf, li, e0);
+      if (oa.o[0]) st2(p, oa.o[0], li, ra[0]);
+      if (oa.o[1]) st2(p, oa.o[1], li, ra[1]);
+      static_for<0, OpExtFilter::NOUT - 1>([&](auto O) {
+        constexpr int o = decltype(O)::value;
+        if (ob.o[o]) st2(p, ob.o[o], li, rb[o]);
+      });
+    }
+  }
+  if (amax) block_amax(vmax, amax);   // uniform over the launch
+}
+
+template <class T>
+void launch_ext_substep(const Ctx<T>& c, const ExtElfArgs<T>& e, const FieldTab<T>& tuv, const OutTab<T>& ouv,
+                        const FieldTab<T>& tf, const OutTab<T>& of, T* amax) {
+  if (c.nblk2 == 0) return;
+  hipLaunchKernelGGL((k_ext_substep<T>), dim3(groups(c.nblk2, c.tune.map2_bpg, c.tune.map2_min)), dim3(TPB), 0,
+                     c.stream, c.B, c.vert, c.ramp, c.prm, e, tuv, ouv, tf, of, amax);
+}
+template void launch_ext_substep<float>(const Ctx<float>&, const ExtElfArgs<float>&, const FieldTab<float>&,
+                                        const OutTab<float>&, const FieldTab<float>&, const OutTab<float>&, float*);
+template void launch_ext_substep<double>(const Ctx<double>&, const ExtElfArgs<double>&, const FieldTab<double>&,
+                                         const OutTab<double>&, const FieldTab<double>&, const OutTab<double>&,
+                                         double*);
 
 template <class T> void launch_ext_elf(const Ctx<T>& c, const ExtElfArgs<T>& a) {
   if (c.nblk2 == 0) return;

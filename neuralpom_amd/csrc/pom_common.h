@@ -588,6 +588,28 @@ struct AccSub : Acc<T> {
   }
 };
 
+// Forward accessor whose field FE at the offsets (0, 0), (-1, 0) and (0, -1)
+// and fields F0, F1 (when not -1) at the point itself are given values (computed
+// just before at the same point); every other access reads memory as Acc does.
+template <class T, int FE, int F0 = -1, int F1 = -1>
+struct AccElf : Acc<T> {
+  T e0, ew, es;   // FE at (0, 0), (-1, 0), (0, -1)
+  T v0, v1;       // F0, F1 at (0, 0)
+  template <int F, int DI, int DJ>
+  __device__ __forceinline__ P<T> at2() const {
+    if constexpr (F == FE) {
+      static_assert((DI == 0 && DJ == 0) || (DI == -1 && DJ == 0) || (DI == 0 && DJ == -1),
+                    "the given field is read at (0, 0), (-1, 0) and (0, -1) only");
+      return P<T>{DI == -1 ? ew : (DJ == -1 ? es : e0)};
+    } else if constexpr (F == F0 || F == F1) {
+      static_assert(DI == 0 && DJ == 0, "a substituted field is read at the evaluation point only");
+      return P<T>{F == F0 ? v0 : v1};
+    } else {
+      return Acc<T>::template at2<F, DI, DJ>();
+    }
+  }
+};
+
 // Forward accessor that reads every 2-D field and geometry array at a fixed
 // offset (SI, SJ) from the wrapped accessor's point: an operator's formula
 // evaluated at a neighbour of the evaluation point.

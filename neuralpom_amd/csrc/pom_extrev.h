@@ -98,6 +98,50 @@ template <class T> struct ExtElfArgs {
   double wt = 0.0;   // the substep's tide argument (speed pi / 180) t / 3600
 };
 
+// elf at the point (i, j) of q's member (in the member; ExtElfArgs): the
+// continuity update at the interior point (i, j) copies, then bcond 1, or
+// with the tide policy (TIDE) its node / interior / previous-value form
+template <class T, bool TIDE>
+__device__ __forceinline__ T ext_elf_at(const Pos& q, int i, int j, const ExtElfArgs<T>& a, double dte2) {
+  const int im = q.im, jm = q.jm;
+  auto L = [&](const T* f, int ii, int jj) { return ld_at(f, q.o2, lix2(q, clampi(ii, im), clampi(jj, jm))); };
+  const int mi = i == 0 ? 1 : (i == im - 1 ? im - 2 : i);
+  const int mj = j == 0 ? 1 : (j == jm - 1 ? jm - 2 : j);
+  const T qa = T(0.25);
+  auto fu = [&](int ii, int jj) {
+    return qa * (L(a.d, ii, jj) + L(a.d, ii - 1, jj)) * (L(a.dy, ii, jj) + L(a.dy, ii - 1, jj)) * L(a.ua, ii, jj);
+  };
+  auto fv = [&](int ii, int jj) {
+    return qa * (L(a.d, ii, jj) + L(a.d, ii, jj - 1)) * (L(a.dx, ii, jj) + L(a.dx, ii, jj - 1)) * L(a.va, ii, jj);
+  };
+  const T div = fu(mi + 1, mj) - fu(mi, mj) + fv(mi, mj + 1) - fv(mi, mj);
+  const T term = -(div / L(a.art, mi, mj)) - L(a.vfluxf, mi, mj);
+  const T raw = L(a.elb, mi, mj) + T(dte2) * term;
+  const uint32_t ix = lix2(q, i, j);
+  const T fsm = ld_at(a.fsm, q.o2, ix);
+  if constexpr (TIDE) {
+    const bool inside = i >= 1 && i <= im - 2 && j >= 1 && j <= jm - 2;
+    T v = inside ? raw : ld_at(a.elf_prev, q.o2, ix);
+    if (ld_at(a.node, q.o2, ix) != T(0))
+      v = T(double(ld_at(a.amp, q.o2, ix)) * cos(a.wt - double(ld_at(a.phase, q.o2, ix))));
+    return v * fsm;
+  } else {
+    const bool in = mi >= 1 && mi <= im - 2 && mj >= 1 && mj <= jm - 2;
+    return (in ? raw : T(0)) * fsm;
+  }
+}
+
+// Forward: one substep of the closed-basin explicit core in one pass: elf at
+// the point and at its (i-1) and (j-1) neighbours (ext_elf_at, recomputed from
+// the substep's inputs instead of read back), the depth-mean momentum update
+// with bcond 2 (OpExtUV, table tuv, outputs ouv = uaf, vaf) and the filter,
+// level shift and time means (OpExtFilter, table tf, outputs of), elf stored
+// to e.elf; with amax non-null the running max of |vaf| is accumulated there.
+// Same values as k_ext_elf followed by k_map2 over the same tables.
+template <class T>
+void launch_ext_substep(const Ctx<T>& c, const ExtElfArgs<T>& e, const FieldTab<T>& tuv, const OutTab<T>& ouv,
+                        const FieldTab<T>& tf, const OutTab<T>& of, T* amax);
+
 // c.prm.iext is the substep
 template <class T> void launch_ext_rev_fac(const Ctx<T>& c, const ExtRevFacArgs<T>& a);
 template <class T> void launch_ext_rev_gat(const Ctx<T>& c, const ExtRevGatArgs<T>& a);

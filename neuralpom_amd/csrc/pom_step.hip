@@ -998,20 +998,18 @@ template <class T> struct Step {
       prm.iext = n;
       Ctx<T> cn = c;
       cn.prm = prm;
-      {   // continuity with bcond 1 in one pass (OpExtElfRaw, then OpBc1 or the tide's bcond 1)
-        ExtElfArgs<T> a{xp.ua, xp.va, xp.d, xp.elb, g[POM_G_DX], g[POM_G_DY], g[POM_G_ART], out[POM_VFLUXF],
-                        g[POM_G_FSM], xn.elf};
-        if (open) {   // the adapter's time of substep n (coastal_boundary.py:14)
-          a.node = g[POM_G_TIDE_NODE];
-          a.amp = g[POM_G_TIDE_AMP];
-          a.phase = g[POM_G_TIDE_PHASE];
-          a.elf_prev = xp.elf;
-          const double sec = hp->tide_sec + hp->dte * double(hp->tide_substep0 + n - 1);
-          a.wt = hp->tide_coef * sec / 3600.0;
-        }
-        launch_ext_elf(cn, a);
+      ExtElfArgs<T> ea{xp.ua, xp.va, xp.d, xp.elb, g[POM_G_DX], g[POM_G_DY], g[POM_G_ART], out[POM_VFLUXF],
+                       g[POM_G_FSM], xn.elf};
+      if (open) {   // the adapter's time of substep n (coastal_boundary.py:14)
+        ea.node = g[POM_G_TIDE_NODE];
+        ea.amp = g[POM_G_TIDE_AMP];
+        ea.phase = g[POM_G_TIDE_PHASE];
+        ea.elf_prev = xp.elf;
+        const double sec = hp->tide_sec + hp->dte * double(hp->tide_substep0 + n - 1);
+        ea.wt = hp->tide_coef * sec / 3600.0;
+        launch_ext_elf(cn, ea);   // continuity with the tide's bcond 1
       }
-      if (n % c.prm.ispadv == 0) {
+      if (n % c.prm.ispadv == 0) {   // reads the substep's inputs only
         launch_map<T, OpAdvave>(cn, t_advave(xp), OutTab<T>{{xn.advua, xn.advva, xn.fluxua, xn.fluxva, xn.tps}});
       }
       resolve(n, xn);
@@ -1020,12 +1018,11 @@ template <class T> struct Step {
         // ua, va, el are not stored where they share uaf's, vaf's, elf's array
         const OutTab<T> of{{xn.ua == xn.uaf ? nullptr : xn.ua, xn.uab, xn.va == xn.vaf ? nullptr : xn.va, xn.vab,
                             xn.el == xn.elf ? nullptr : xn.el, xn.elb, xn.d, xn.egf, xn.utf, xn.vtf, xn.etf}};
-        if (open)
+        if (open)   // the tide's bcond 2 reads elf beyond the three points the fused pass forms
           launch_map2<T, OpExtUVTide, OpExtFilter, OpExtFilter::UAF, OpExtFilter::VAF>(
               cn, t_uv(xp, xn), OutTab<T>{{xn.uaf, xn.vaf}}, t_filter(xp, xn), of, vamax);
-        else
-          launch_map2<T, OpExtUV, OpExtFilter, OpExtFilter::UAF, OpExtFilter::VAF>(
-              cn, t_uv(xp, xn), OutTab<T>{{xn.uaf, xn.vaf}}, t_filter(xp, xn), of, vamax);
+        else   // continuity and bcond 1 fused in: elf recomputed where it is read
+          launch_ext_substep(cn, ea, t_uv(xp, xn), OutTab<T>{{xn.uaf, xn.vaf}}, t_filter(xp, xn), of, vamax);
       }
     }
     const X<T>& xl = xs[isplit];
