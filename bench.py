@@ -29,21 +29,23 @@ B_FA = 468.0     # algorithmic bytes per cell-step, forward + adjoint with recom
 HBM_PEAK = 8.0e12
 
 # Dominant kernel of the fwd+adjoint step by total time (rocprof kernel-trace,
-# profiles/): the fused external substep k_map2<OpExtUV, OpExtFilter>, 30
-# launches per forward pass and two forward passes per fwd+adjoint step.  Its
-# compulsory bytes per 2-D point: reads 22 state fields (el elb elf d ua va uab
-# vab advua advva adx2d ady2d drx2d dry2d wubot wvbot uaf vaf of the previous
-# substep, egf utf vtf etf) and 12 geometry fields (cor dx dy h aru arv dum dvm
-# e_atmos wusurf wvsurf fsm), writes 13 (uaf vaf ua uab va vab el elb d egf utf
-# vtf etf), 4 B each.
+# profiles/): the fused external substep k_ext_substep (continuity, bcond 1,
+# depth-mean momentum with bcond 2, filter, shift and time means), 30 launches
+# per forward pass and two forward passes per fwd+adjoint step.  Its
+# compulsory bytes per 2-D point: reads 19 state fields of the previous
+# substep (el elb d ua va uab vab advua advva adx2d ady2d drx2d dry2d wubot
+# wvbot egf utf vtf etf; its uaf vaf elf are the arrays of ua va el) and 14
+# step-constant fields (cor dx dy h art aru arv dum dvm fsm e_atmos wusurf
+# wvsurf vfluxf), writes 11 (elf uaf vaf uab vab elb d egf utf vtf etf), 4 B
+# each.
 DOMINANT = "extuv"
-DOMINANT_KERNEL = "pom::k_map2<float, pom::OpExtUV, pom::OpExtFilter, 6, 7>"
-DOMINANT_B_PER_POINT = (22 + 12 + 13) * 4.0
+DOMINANT_KERNEL = "pom::k_ext_substep<float>"
+DOMINANT_B_PER_POINT = (19 + 14 + 11) * 4.0
 # HBM bytes per launch on this workload from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (KiB per dispatch), each scaled by what the counter reads for a
 # 256 MiB read + write elementwise kernel in the same run
 # (profiles/external_substep_seamount64.md)
-DOMINANT_TRAFFIC = (50682.1 * 2.000 + 39903.3 * 1.000) * 1024.0
+DOMINANT_TRAFFIC = (50805.8 * 2.000 + 34048.3 * 1.000) * 1024.0
 DTE = 6.0        # external step of the 8 km seamount grid (barotropic CFL); dti = 180 s
 SHAPES = [(90, 82)] * 32 + [(120, 110)] * 24 + [(177, 161)] * 8
 DIFF_OUT = ("u", "ub", "v", "vb", "t", "tb", "s", "sb", "el", "elb", "ua", "uab", "va", "vab",

@@ -24,6 +24,7 @@
 #include "pom_momadj.h"
 #include "pom_advctadj.h"
 #include "pom_advaveadj.h"
+#include "pom_smagadj.h"
 
 #include "pom_batch.h"
 
@@ -1419,8 +1420,13 @@ template <class T> struct Adj {
     {
       COutTab<T> ob{{bo[POM_AAM]}};
       T* ib[OpSmag::NIN] = {bi[POM_U], bi[POM_V], bi[POM_AAM]};
-      if (S.imp) launch_gather<T, OpSmagImp>(c, S.t_smag_imp(), ob, ib);
-      else launch_gather<T, OpSmag>(c, S.t_smag(), ob, ib);
+      if (S.imp) {
+        launch_gather<T, OpSmagImp>(c, S.t_smag_imp(), ob, ib);
+      } else {   // transposed stencil through four factor planes
+        SmagAdjArgs<T> a{S.in[POM_U], S.in[POM_V], S.g[POM_G_DX], S.g[POM_G_DY], bo[POM_AAM], s3a, s3b, s3c, s3d,
+                         bi[POM_U], bi[POM_V], bi[POM_AAM]};
+        launch_adj_smag(c, a);
+      }
     }
     {
       BaropgAdjArgs<T> a{S.in[POM_RHO], S.g[POM_G_RMEAN], S.in[POM_DT], S.g[POM_G_DX], S.g[POM_G_DY],
