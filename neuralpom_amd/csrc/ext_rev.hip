@@ -66,165 +66,181 @@ template <class T> struct Mom {
 
 }  // namespace
 
-// factor planes of the outputs uaf(p), vaf(p) (pom_extrev.h)
+// the factor planes fv[XF_N] of the outputs uaf(q), vaf(q) (pom_extrev.h);
+// the input cotangents reached from q's own outputs only and the advave-period
+// accumulators are written when own is set
 template <class T>
-__global__ void __launch_bounds__(TPB) k_ext_rev_fac(Batch B, const T* ramp, Params prm, ExtRevFacArgs<T> a) {
-  const int nblk = B.blk2[B.M];
+__device__ __forceinline__ void ext_fac_point(const Pos& q, T rmp, const Params& prm, const ExtRevFacArgs<T>& a,
+                                              bool own, T* fv) {
   const T f = T(0.5 * prm.smoth);
   const bool acc = prm.iext != prm.isplit;
   const T isp2i = T(prm.isp2i);
   const T al = T(prm.alpha), al12 = T(1.0 - 2.0 * prm.alpha);
   const T dte4 = T(4.0 * prm.dte), g4 = T(0.25 * prm.grav);
   const T rfe = T(prm.rfe), rfw = T(prm.rfw), rfn = T(prm.rfn), rfs = T(prm.rfs);
+  const L2<T> L{q};
+  const int im = q.im, jm = q.jm, i = q.i, j = q.j;
+  auto sel = [](bool c, T x) { return c ? x : T(0); };
+  const uint32_t ix = lix2(q, i, j);
+  auto ld = [&](const T* p) { return ld_at(p, q.o2, ix); };
+  // interior partials of the output at (i, j), neighbour (i - di, j - dj)
+  auto mom = [&](int di, int dj, T ct, T ar, T dm, const T* ds_f, T out) {
+    Mom<T> r;
+    const T den = (ld(a.h) + ld(a.elf) + L(a.h, i - di, j - dj) + L(a.elf, i - di, j - dj)) * ar;
+    const T rden = rcp_lane(den + T(1e-20));
+    r.g = ct * dm * rden;
+    r.cgt = -dte4 * r.g;
+    r.G = g4 * (ld(ds_f) + L(ds_f, i - di, j - dj));
+    r.D = ld(a.d) + L(a.d, i - di, j - dj);
+    r.E = al12 * (ld(a.el) - L(a.el, i - di, j - dj)) +
+          al * (ld(a.elb) - L(a.elb, i - di, j - dj) + ld(a.elf) - L(a.elf, i - di, j - dj)) +
+          (ld(a.eatm) - L(a.eatm, i - di, j - dj));
+    // d out / d den = -dm num / den^2 = -(dm num / den) / den, the stored output over den
+    r.cden = -(ct * rden) * out * ar;
+    return r;
+  };
+  const T h_q = ld(a.h), elf_q = ld(a.elf), elb_q = ld(a.elb);
+  // ---- uaf(p): total cotangent = the next substep's (its given uaf) + the filter's (ua, uab, utf)
+  {
+    const int k = ukind(i, j, im, jm);
+    const T ar = ld(a.aru), dm = ld(a.dum), uaf = ld(a.uaf);
+    const T tu = ld(a.c_utf) * uaf * isp2i;   // d utf / d elf(p) = d utf / d elf(p-i), times c_utf
+    const T ct = ld(a.c_uaf) + ld(a.c_ua) + f * ld(a.c_uab) +
+                 sel(acc && i >= 1, ld(a.c_utf) * ((h_q + elf_q) + (L(a.h, i - 1, j) + L(a.elf, i - 1, j))) * isp2i);
+    const Mom<T> P = mom(1, 0, ct, ar, dm, a.dy, uaf);
+    const bool in = k == K_INT;
+    const T s = ct * dm;
+    fv[XF_XU] = sel(in, P.g * ar * ld(a.uab));
+    fv[XF_YU] = sel(in, P.cgt * P.G * P.D);
+    fv[XF_ZU] = sel(in, P.cden) + sel(acc && i >= 1, tu);
+    fv[XF_WU] = sel(in, P.cgt * P.G * P.E);
+    fv[XF_CU] = sel(in, -ar * P.cgt);
+    // radiation: east form reads el(i-1, j), west forms el(1, j)
+    fv[XF_RU] = sel(k == K_EDGE_HI && prm.rfe != 0.0, rfe * cwave(L(a.h, i - 1, j), prm.grav) * rmp * s) +
+                sel(k == K_EDGE_1 && prm.rfw != 0.0, -(rfw * cwave(h_q, prm.grav)) * rmp * s) +
+                sel(k == K_EDGE_0 && prm.rfw != 0.0, -(rfw * cwave(L(a.h, i + 1, j), prm.grav)) * rmp * s);
+    if (own) st2(q, a.p_uab, ix, f * ld(a.c_uab) + sel(in, P.g * (h_q + elb_q + L(a.h, i - 1, j) + L(a.elb, i - 1, j)) * ar));
+    if (own) st2(q, a.p_uaf, ix, sel(k == K_PASS, s));
+    if (own) st2(q, a.a_advua, ix, ld(a.a_advua) + sel(in, P.cgt));
+  }
+  // ---- vaf(p)
+  {
+    const int k = vkind(i, j, im, jm);
+    const T ar = ld(a.arv), dm = ld(a.dvm), vaf = ld(a.vaf);
+    const T tv = ld(a.c_vtf) * vaf * isp2i;
+    const T ct = ld(a.c_vaf) + ld(a.c_va) + f * ld(a.c_vab) +
+                 sel(acc && j >= 1, ld(a.c_vtf) * ((h_q + elf_q) + (L(a.h, i, j - 1) + L(a.elf, i, j - 1))) * isp2i);
+    const Mom<T> P = mom(0, 1, ct, ar, dm, a.dx, vaf);
+    const bool in = k == K_INT;
+    const T s = ct * dm;
+    fv[XF_XV] = sel(in, P.g * ar * ld(a.vab));
+    fv[XF_YV] = sel(in, P.cgt * P.G * P.D);
+    fv[XF_ZV] = sel(in, P.cden) + sel(acc && j >= 1, tv);
+    fv[XF_WV] = sel(in, P.cgt * P.G * P.E);
+    fv[XF_CV] = sel(in, ar * P.cgt);
+    // radiation: north form reads el(i, j-1), south forms el(i, 1)
+    fv[XF_RV] = sel(k == K_EDGE_HI && prm.rfn != 0.0, rfn * cwave(L(a.h, i, j - 1), prm.grav) * rmp * s) +
+                sel(k == K_EDGE_1 && prm.rfs != 0.0, -(rfs * cwave(h_q, prm.grav)) * rmp * s) +
+                sel(k == K_EDGE_0 && prm.rfs != 0.0, -(rfs * cwave(L(a.h, i, j + 1), prm.grav)) * rmp * s);
+    if (own) st2(q, a.p_vab, ix, f * ld(a.c_vab) + sel(in, P.g * (h_q + elb_q + L(a.h, i, j - 1) + L(a.elb, i, j - 1)) * ar));
+    if (own) st2(q, a.p_vaf, ix, sel(k == K_PASS, s));
+    if (own) st2(q, a.a_advva, ix, ld(a.a_advva) + sel(in, P.cgt));
+  }
+}
+
+// factor planes of the outputs uaf(p), vaf(p) (pom_extrev.h)
+template <class T>
+__global__ void __launch_bounds__(TPB) k_ext_rev_fac(Batch B, const T* ramp, Params prm, ExtRevFacArgs<T> a) {
+  const int nblk = B.blk2[B.M];
   for (int blk = logical_block(); blk < nblk; blk += (int)gridDim.x) {
     Pos q;
     if (!pos2_at(B, blk, q)) continue;
-    const L2<T> L{q};
-    const int im = q.im, jm = q.jm, i = q.i, j = q.j;
-    const T rmp = ramp[q.m];
-    auto sel = [](bool c, T x) { return c ? x : T(0); };
-    const uint32_t ix = lix2(q, i, j);
-    auto ld = [&](const T* p) { return ld_at(p, q.o2, ix); };
-    // interior partials of the output at (i, j), neighbour (i - di, j - dj)
-    auto mom = [&](int di, int dj, T ct, T ar, T dm, const T* ds_f, T out) {
-      Mom<T> r;
-      const T den = (ld(a.h) + ld(a.elf) + L(a.h, i - di, j - dj) + L(a.elf, i - di, j - dj)) * ar;
-      const T rden = rcp_lane(den + T(1e-20));
-      r.g = ct * dm * rden;
-      r.cgt = -dte4 * r.g;
-      r.G = g4 * (ld(ds_f) + L(ds_f, i - di, j - dj));
-      r.D = ld(a.d) + L(a.d, i - di, j - dj);
-      r.E = al12 * (ld(a.el) - L(a.el, i - di, j - dj)) +
-            al * (ld(a.elb) - L(a.elb, i - di, j - dj) + ld(a.elf) - L(a.elf, i - di, j - dj)) +
-            (ld(a.eatm) - L(a.eatm, i - di, j - dj));
-      // d out / d den = -dm num / den^2 = -(dm num / den) / den, the stored output over den
-      r.cden = -(ct * rden) * out * ar;
-      return r;
-    };
-    const T h_q = ld(a.h), elf_q = ld(a.elf), elb_q = ld(a.elb);
     T fv[XF_N];
-    // ---- uaf(p): total cotangent = the next substep's (its given uaf) + the filter's (ua, uab, utf)
-    {
-      const int k = ukind(i, j, im, jm);
-      const T ar = ld(a.aru), dm = ld(a.dum), uaf = ld(a.uaf);
-      const T tu = ld(a.c_utf) * uaf * isp2i;   // d utf / d elf(p) = d utf / d elf(p-i), times c_utf
-      const T ct = ld(a.c_uaf) + ld(a.c_ua) + f * ld(a.c_uab) +
-                   sel(acc && i >= 1, ld(a.c_utf) * ((h_q + elf_q) + (L(a.h, i - 1, j) + L(a.elf, i - 1, j))) * isp2i);
-      const Mom<T> P = mom(1, 0, ct, ar, dm, a.dy, uaf);
-      const bool in = k == K_INT;
-      const T s = ct * dm;
-      fv[XF_XU] = sel(in, P.g * ar * ld(a.uab));
-      fv[XF_YU] = sel(in, P.cgt * P.G * P.D);
-      fv[XF_ZU] = sel(in, P.cden) + sel(acc && i >= 1, tu);
-      fv[XF_WU] = sel(in, P.cgt * P.G * P.E);
-      fv[XF_CU] = sel(in, -ar * P.cgt);
-      // radiation: east form reads el(i-1, j), west forms el(1, j)
-      fv[XF_RU] = sel(k == K_EDGE_HI && prm.rfe != 0.0, rfe * cwave(L(a.h, i - 1, j), prm.grav) * rmp * s) +
-                  sel(k == K_EDGE_1 && prm.rfw != 0.0, -(rfw * cwave(h_q, prm.grav)) * rmp * s) +
-                  sel(k == K_EDGE_0 && prm.rfw != 0.0, -(rfw * cwave(L(a.h, i + 1, j), prm.grav)) * rmp * s);
-      st2(q, a.p_uab, ix, f * ld(a.c_uab) + sel(in, P.g * (h_q + elb_q + L(a.h, i - 1, j) + L(a.elb, i - 1, j)) * ar));
-      st2(q, a.p_uaf, ix, sel(k == K_PASS, s));
-      const T o = ld(a.a_advua);
-      st2(q, a.a_advua, ix, o + sel(in, P.cgt));
-    }
-    // ---- vaf(p)
-    {
-      const int k = vkind(i, j, im, jm);
-      const T ar = ld(a.arv), dm = ld(a.dvm), vaf = ld(a.vaf);
-      const T tv = ld(a.c_vtf) * vaf * isp2i;
-      const T ct = ld(a.c_vaf) + ld(a.c_va) + f * ld(a.c_vab) +
-                   sel(acc && j >= 1, ld(a.c_vtf) * ((h_q + elf_q) + (L(a.h, i, j - 1) + L(a.elf, i, j - 1))) * isp2i);
-      const Mom<T> P = mom(0, 1, ct, ar, dm, a.dx, vaf);
-      const bool in = k == K_INT;
-      const T s = ct * dm;
-      fv[XF_XV] = sel(in, P.g * ar * ld(a.vab));
-      fv[XF_YV] = sel(in, P.cgt * P.G * P.D);
-      fv[XF_ZV] = sel(in, P.cden) + sel(acc && j >= 1, tv);
-      fv[XF_WV] = sel(in, P.cgt * P.G * P.E);
-      fv[XF_CV] = sel(in, ar * P.cgt);
-      // radiation: north form reads el(i, j-1), south forms el(i, 1)
-      fv[XF_RV] = sel(k == K_EDGE_HI && prm.rfn != 0.0, rfn * cwave(L(a.h, i, j - 1), prm.grav) * rmp * s) +
-                  sel(k == K_EDGE_1 && prm.rfs != 0.0, -(rfs * cwave(h_q, prm.grav)) * rmp * s) +
-                  sel(k == K_EDGE_0 && prm.rfs != 0.0, -(rfs * cwave(L(a.h, i, j + 1), prm.grav)) * rmp * s);
-      st2(q, a.p_vab, ix, f * ld(a.c_vab) + sel(in, P.g * (h_q + elb_q + L(a.h, i, j - 1) + L(a.elb, i, j - 1)) * ar));
-      st2(q, a.p_vaf, ix, sel(k == K_PASS, s));
-      const T o = ld(a.a_advva);
-      st2(q, a.a_advva, ix, o + sel(in, P.cgt));
-    }
+    ext_fac_point(q, ramp[q.m], prm, a, true, fv);
+    const uint32_t ix = lix2(q, q.i, q.j);
 #pragma unroll
     for (int n = 0; n < XF_N; ++n) ref_at(a.fac, q.o2 + n * a.n2, ix) = fv[n];
   }
+}
+
+// the sums of the factors of the outputs that read q, and the filter's own
+// terms; F(n, i, j) is plane n at (i, j) (any value where (i, j) is off the
+// member: the existence tests below discard it)
+template <class T, class FA>
+__device__ __forceinline__ void ext_gat_point(const Pos& q, const Params& prm, const ExtRevGatArgs<T>& a, const FA& F) {
+  const T f = T(0.5 * prm.smoth);
+  const T f12 = T(1) + f * T(-2);   // d uab_new / d ua
+  const bool acc = prm.iext != prm.isplit;
+  const T ispi = T(prm.ispi);
+  const T al = T(prm.alpha), al12 = T(1.0 - 2.0 * prm.alpha);
+  const L2<T> L{q};
+  const int im = q.im, jm = q.jm, i = q.i, j = q.j;
+  auto sel = [](bool c, T x) { return c ? x : T(0); };
+  const uint32_t ix = lix2(q, i, j);
+  auto ld = [&](const T* p) { return ld_at(p, q.o2, ix); };
+  // factors of the outputs that read q; a neighbour outside the member has
+  // no output, and its clamped read is discarded by the existence test
+  const bool ie = i + 1 <= im - 1, iw = i >= 1, jn = j + 1 <= jm - 1, js = j >= 1;
+  const T YU0 = F(XF_YU, i, j), YU1 = sel(ie, F(XF_YU, i + 1, j));
+  const T YV0 = F(XF_YV, i, j), YV1 = sel(jn, F(XF_YV, i, j + 1));
+  const T XU = F(XF_XU, i, j) + sel(ie, F(XF_XU, i + 1, j));
+  const T XV = F(XF_XV, i, j) + sel(jn, F(XF_XV, i, j + 1));
+  const T ZU = F(XF_ZU, i, j) + sel(ie, F(XF_ZU, i + 1, j));
+  const T ZV = F(XF_ZV, i, j) + sel(jn, F(XF_ZV, i, j + 1));
+  const T WU = F(XF_WU, i, j) + sel(ie, F(XF_WU, i + 1, j));
+  const T WV = F(XF_WV, i, j) + sel(jn, F(XF_WV, i, j + 1));
+  const T CU0 = F(XF_CU, i, j) + sel(ie, F(XF_CU, i + 1, j));                       // va(q), d(q)
+  const T CUs = sel(js, F(XF_CU, i, j - 1)) + sel(js && ie, F(XF_CU, i + 1, j - 1));   // va(q) via cor d at q-j
+  const T CV0 = F(XF_CV, i, j) + sel(jn, F(XF_CV, i, j + 1));                       // ua(q), d(q)
+  const T CVw = sel(iw, F(XF_CV, i - 1, j)) + sel(iw && jn, F(XF_CV, i - 1, j + 1));   // ua(q) via cor d at q-i
+  const T rad = sel(ukind(i, j, im, jm) == K_EDGE_1, F(XF_RU, i, j)) +
+                sel(ukind(i + 1, j, im, jm) == K_EDGE_HI, F(XF_RU, i + 1, j)) +
+                sel(ukind(i - 1, j, im, jm) == K_EDGE_0, F(XF_RU, i - 1, j)) +
+                sel(vkind(i, j, im, jm) == K_EDGE_1, F(XF_RV, i, j)) +
+                sel(vkind(i, j + 1, im, jm) == K_EDGE_HI, F(XF_RV, i, j + 1)) +
+                sel(vkind(i, j - 1, im, jm) == K_EDGE_0, F(XF_RV, i, j - 1));
+  const T qa = T(0.25);
+  const T cor_q = ld(a.cor), d_q = ld(a.d);
+  const T cd_q = qa * cor_q * d_q, cd_s = qa * L(a.cor, i, j - 1) * L(a.d, i, j - 1),
+          cd_w = qa * L(a.cor, i - 1, j) * L(a.d, i - 1, j);
+  const T vsum = L(a.va, i, j + 1) + ld(a.va), usum = L(a.ua, i + 1, j) + ld(a.ua);
+  const T c_uab = ld(a.c_uab), c_vab = ld(a.c_vab), c_elb = ld(a.c_elb);
+  const T dY = (YU0 - YU1) + (YV0 - YV1);
+  // filter's own terms of elf: el' = elf, elb' (f), d' = h + elf, egf' (ispi), etf'
+  T celf = (a.elf_first ? T(0) : ld(a.c_elf)) + ld(a.c_el) + f * c_elb + ld(a.c_d) + sel(acc, ld(a.c_egf) * ispi) +
+           ZU + ZV + al * dY;
+  const int n = prm.iext, ns = prm.isplit;
+  const T ce = ld(a.c_etf);
+  if (n == ns - 2) {
+    celf += T(0.25 * prm.smoth) * ce;
+    st2(q, a.c_etf, ix, T(0));
+  } else if (n == ns - 1) {
+    celf += T(0.5 * (1.0 - 0.5 * prm.smoth)) * ce;
+  } else if (n == ns) {
+    const T fs = ld(a.fsm);
+    celf += T(0.5) * ce * fs;
+    st2(q, a.c_etf, ix, ce * fs);
+  }
+  st2(q, a.c_elf, ix, celf);
+  st2(q, a.p_ua, ix, f12 * c_uab + CV0 * cd_q + CVw * cd_w);
+  st2(q, a.p_va, ix, f12 * c_vab + CU0 * cd_q + CUs * cd_s);
+  st2(q, a.p_el, ix, f12 * c_elb + al12 * dY + rad);
+  st2(q, a.p_elb, ix, f * c_elb + XU + XV + al * dY);
+  st2(q, a.p_d, ix, WU + WV + CU0 * (qa * cor_q * vsum) + CV0 * (qa * cor_q * usum));
 }
 
 // sums of the factor planes over the outputs that read q, and the filter's own terms
 template <class T>
 __global__ void __launch_bounds__(TPB) k_ext_rev_gat(Batch B, Params prm, ExtRevGatArgs<T> a) {
   const int nblk = B.blk2[B.M];
-  const T f = T(0.5 * prm.smoth);
-  const T f12 = T(1) + f * T(-2);   // d uab_new / d ua
-  const bool acc = prm.iext != prm.isplit;
-  const T ispi = T(prm.ispi);
-  const T al = T(prm.alpha), al12 = T(1.0 - 2.0 * prm.alpha);
   for (int blk = logical_block(); blk < nblk; blk += (int)gridDim.x) {
     Pos q;
     if (!pos2_at(B, blk, q)) continue;
-    const L2<T> L{q};
-    const int im = q.im, jm = q.jm, i = q.i, j = q.j;
-    auto sel = [](bool c, T x) { return c ? x : T(0); };
+    const int im = q.im, jm = q.jm;
     auto F = [&](int n, int ii, int jj) {
       return ld_at(a.fac, q.o2 + n * a.n2, lix2(q, clampi(ii, im), clampi(jj, jm)));
     };
-    const uint32_t ix = lix2(q, i, j);
-    auto ld = [&](const T* p) { return ld_at(p, q.o2, ix); };
-    // factors of the outputs that read q; a neighbour outside the member has
-    // no output, and its clamped read is discarded by the existence test
-    const bool ie = i + 1 <= im - 1, iw = i >= 1, jn = j + 1 <= jm - 1, js = j >= 1;
-    const T YU0 = F(XF_YU, i, j), YU1 = sel(ie, F(XF_YU, i + 1, j));
-    const T YV0 = F(XF_YV, i, j), YV1 = sel(jn, F(XF_YV, i, j + 1));
-    const T XU = F(XF_XU, i, j) + sel(ie, F(XF_XU, i + 1, j));
-    const T XV = F(XF_XV, i, j) + sel(jn, F(XF_XV, i, j + 1));
-    const T ZU = F(XF_ZU, i, j) + sel(ie, F(XF_ZU, i + 1, j));
-    const T ZV = F(XF_ZV, i, j) + sel(jn, F(XF_ZV, i, j + 1));
-    const T WU = F(XF_WU, i, j) + sel(ie, F(XF_WU, i + 1, j));
-    const T WV = F(XF_WV, i, j) + sel(jn, F(XF_WV, i, j + 1));
-    const T CU0 = F(XF_CU, i, j) + sel(ie, F(XF_CU, i + 1, j));                       // va(q), d(q)
-    const T CUs = sel(js, F(XF_CU, i, j - 1)) + sel(js && ie, F(XF_CU, i + 1, j - 1));   // va(q) via cor d at q-j
-    const T CV0 = F(XF_CV, i, j) + sel(jn, F(XF_CV, i, j + 1));                       // ua(q), d(q)
-    const T CVw = sel(iw, F(XF_CV, i - 1, j)) + sel(iw && jn, F(XF_CV, i - 1, j + 1));   // ua(q) via cor d at q-i
-    const T rad = sel(ukind(i, j, im, jm) == K_EDGE_1, F(XF_RU, i, j)) +
-                  sel(ukind(i + 1, j, im, jm) == K_EDGE_HI, F(XF_RU, i + 1, j)) +
-                  sel(ukind(i - 1, j, im, jm) == K_EDGE_0, F(XF_RU, i - 1, j)) +
-                  sel(vkind(i, j, im, jm) == K_EDGE_1, F(XF_RV, i, j)) +
-                  sel(vkind(i, j + 1, im, jm) == K_EDGE_HI, F(XF_RV, i, j + 1)) +
-                  sel(vkind(i, j - 1, im, jm) == K_EDGE_0, F(XF_RV, i, j - 1));
-    const T qa = T(0.25);
-    const T cor_q = ld(a.cor), d_q = ld(a.d);
-    const T cd_q = qa * cor_q * d_q, cd_s = qa * L(a.cor, i, j - 1) * L(a.d, i, j - 1),
-            cd_w = qa * L(a.cor, i - 1, j) * L(a.d, i - 1, j);
-    const T vsum = L(a.va, i, j + 1) + ld(a.va), usum = L(a.ua, i + 1, j) + ld(a.ua);
-    const T c_uab = ld(a.c_uab), c_vab = ld(a.c_vab), c_elb = ld(a.c_elb);
-    const T dY = (YU0 - YU1) + (YV0 - YV1);
-    // filter's own terms of elf: el' = elf, elb' (f), d' = h + elf, egf' (ispi), etf'
-    T celf = (a.elf_first ? T(0) : ld(a.c_elf)) + ld(a.c_el) + f * c_elb + ld(a.c_d) + sel(acc, ld(a.c_egf) * ispi) +
-             ZU + ZV + al * dY;
-    const int n = prm.iext, ns = prm.isplit;
-    const T ce = ld(a.c_etf);
-    if (n == ns - 2) {
-      celf += T(0.25 * prm.smoth) * ce;
-      st2(q, a.c_etf, ix, T(0));
-    } else if (n == ns - 1) {
-      celf += T(0.5 * (1.0 - 0.5 * prm.smoth)) * ce;
-    } else if (n == ns) {
-      const T fs = ld(a.fsm);
-      celf += T(0.5) * ce * fs;
-      st2(q, a.c_etf, ix, ce * fs);
-    }
-    st2(q, a.c_elf, ix, celf);
-    st2(q, a.p_ua, ix, f12 * c_uab + CV0 * cd_q + CVw * cd_w);
-    st2(q, a.p_va, ix, f12 * c_vab + CU0 * cd_q + CUs * cd_s);
-    st2(q, a.p_el, ix, f12 * c_elb + al12 * dY + rad);
-    st2(q, a.p_elb, ix, f * c_elb + XU + XV + al * dY);
-    st2(q, a.p_d, ix, WU + WV + CU0 * (qa * cor_q * vsum) + CV0 * (qa * cor_q * usum));
+    ext_gat_point(q, prm, a, F);
   }
 }
 

@@ -1339,7 +1339,8 @@ template <class T> struct Adj {
         const X<T>& xn = S.xs[n];
         Ctx<T> cn = c;
         cn.prm.iext = n;
-        {   // partials of the depth-mean momentum outputs (with bcond 2), once per point
+        {   // partials of the depth-mean momentum outputs (with bcond 2), once per point, then their sums
+            // at each input point with the filter's own terms
           ExtRevFacArgs<T> a{};
           a.c_uaf = bn.uaf; a.c_ua = bn.ua; a.c_uab = bn.uab; a.c_utf = bn.utf;
           a.c_vaf = bn.vaf; a.c_va = bn.va; a.c_vab = bn.vab; a.c_vtf = bn.vtf;
@@ -1350,17 +1351,15 @@ template <class T> struct Adj {
           a.fac = xfac; a.n2 = N2;
           a.p_uab = bp.uab; a.p_vab = bp.vab; a.p_uaf = bp.uaf; a.p_vaf = bp.vaf;
           a.a_advua = bn.advua; a.a_advva = bn.advva;
+          ExtRevGatArgs<T> g{};
+          g.fac = xfac; g.n2 = N2;
+          g.cor = S.g[POM_G_COR]; g.d = xp.d; g.ua = xp.ua; g.va = xp.va; g.fsm = S.g[POM_G_FSM];
+          g.c_uab = bn.uab; g.c_vab = bn.vab; g.c_elb = bn.elb; g.c_el = bn.el; g.c_d = bn.d; g.c_egf = bn.egf;
+          g.c_etf = bn.etf; g.c_elf = bn.elf;
+          g.p_ua = bp.ua; g.p_va = bp.va; g.p_el = bp.el; g.p_elb = bp.elb; g.p_d = bp.d;
+          g.elf_first = n < S.isplit;
           launch_ext_rev_fac(cn, a);
-        }
-        {   // their sums at each input point, and the filter
-          ExtRevGatArgs<T> a{};
-          a.fac = xfac; a.n2 = N2;
-          a.cor = S.g[POM_G_COR]; a.d = xp.d; a.ua = xp.ua; a.va = xp.va; a.fsm = S.g[POM_G_FSM];
-          a.c_uab = bn.uab; a.c_vab = bn.vab; a.c_elb = bn.elb; a.c_el = bn.el; a.c_d = bn.d; a.c_egf = bn.egf;
-          a.c_etf = bn.etf; a.c_elf = bn.elf;
-          a.p_ua = bp.ua; a.p_va = bp.va; a.p_el = bp.el; a.p_elb = bp.elb; a.p_d = bp.d;
-          a.elf_first = n < S.isplit;
-          launch_ext_rev_gat(cn, a);
+          launch_ext_rev_gat(cn, g);
         }
         if (n % c.prm.ispadv == 0) {   // advave produced this substep's advua / advva
           advave_rev(xp, bn.advua, bn.advva, bp.ua, bp.va, bp.uab, bp.vab, bp.d);
