@@ -1043,8 +1043,7 @@ template <class T> struct Step {
         hipLaunchKernelGGL(k_adjust<T>, g2, tb, 0, c.stream, c.B, c.vert, a);
       }
       hipLaunchKernelGGL(k_vertvl<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a_vertvl());
-      launch_map<T, OpAdvq>(c, t_advq(false), OutTab<T>{{q2f0}});
-      launch_map<T, OpAdvq>(c, t_advq(true), OutTab<T>{{q2lf0}});
+      launch_map_pair<T, OpAdvq>(c, t_advq(false), OutTab<T>{{q2f0}}, t_advq(true), OutTab<T>{{q2lf0}});
       {
         ProfScope ps(POM_STAGE_PROFQ, c.stream);
         hipLaunchKernelGGL(k_profq<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a_profq());
@@ -1055,8 +1054,7 @@ template <class T> struct Step {
                                       OutTab<T>{{out[POM_Q2], out[POM_Q2B], out[POM_Q2L], out[POM_Q2LB]}});
       else
         launch_map<T, OpBc6Filter>(c, t_bc6(), OutTab<T>{{out[POM_Q2], out[POM_Q2B], out[POM_Q2L], out[POM_Q2LB]}});
-      launch_map<T, OpAdvt1>(c, t_advt1(false), OutTab<T>{{tf0}});
-      launch_map<T, OpAdvt1>(c, t_advt1(true), OutTab<T>{{sf0}});
+      launch_map_pair<T, OpAdvt1>(c, t_advt1(false), OutTab<T>{{tf0}}, t_advt1(true), OutTab<T>{{sf0}});
       hipLaunchKernelGGL(k_proft<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a_proft(false));
       hipLaunchKernelGGL(k_proft<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a_proft(true));
       if (open)
