@@ -45,7 +45,7 @@ DOMINANT_B_PER_POINT = (19 + 14 + 11) * 4.0
 # WRITE_SIZE (KiB per dispatch), each scaled by what the counter reads for a
 # 256 MiB read + write elementwise kernel in the same run
 # (profiles/external_substep_seamount64.md)
-DOMINANT_TRAFFIC = (50805.8 * 2.000 + 34048.3 * 1.000) * 1024.0
+DOMINANT_TRAFFIC = (50734.8 * 2.000 + 34058.5 * 1.000) * 1024.0
 DTE = 6.0        # external step of the 8 km seamount grid (barotropic CFL); dti = 180 s
 SHAPES = [(90, 82)] * 32 + [(120, 110)] * 24 + [(177, 161)] * 8
 DIFF_OUT = ("u", "ub", "v", "vb", "t", "tb", "s", "sb", "el", "elb", "ua", "uab", "va", "vab",
