@@ -11,6 +11,7 @@
 // cotangent element is written by exactly one thread, so the result is
 // deterministic.
 #pragma once
+#include <utility>
 #include "pom_ops.h"
 
 namespace pom {
@@ -263,6 +264,25 @@ __device__ __forceinline__ bool in_band(const Pos& q, int w) {
   return q.i < w || q.i > q.im - 1 - w || q.j < w || q.j > q.jm - 1 - w;
 }
 
+// The inputs the own-column path (PATH 1) can reach: the union of the tangent
+// lanes of every output at each vertical offset it evaluates.  An input
+// outside it receives no contribution from that path, so its cotangent is
+// neither read nor written there (an assigned one is written zero).
+template <class T, class Op, uint32_t M, int DK, int... O>
+constexpr uint32_t deep_reach_o(std::integer_sequence<int, O...>) {
+  using A = AccTDeep<T, M, 0, 0, DK>;
+  return (0u | ... | mask_of<decltype(Op::template eval<O>(std::declval<const A&>(), std::declval<const Params&>()))>::value);
+}
+template <class T, class Op, uint32_t M3, uint32_t M2, int... K>
+constexpr uint32_t deep_reach_k(std::integer_sequence<int, K...>) {
+  return (0u | ... | deep_reach_o<T, Op, (M3 | ((K - Op::RK) == 0 ? M2 : 0u)), K - Op::RK>(
+                         std::make_integer_sequence<int, Op::NOUT>{}));
+}
+template <class T, class Op, uint32_t M3, uint32_t M2>
+constexpr uint32_t deep_reach() {
+  return deep_reach_k<T, Op, M3, M2>(std::make_integer_sequence<int, 2 * Op::RK + 1>{});
+}
+
 // The gather at 3-D point q: 3-D fields accumulate at q, 2-D fields per
 // level (offsets with dk = 0 only) and are reduced over the 16 lanes of the
 // water column (every lane of the column calls this, live or not).  PATH 0
@@ -307,24 +327,29 @@ __device__ __forceinline__ void gather3_at(const Pos& q, bool live, const Vert<T
     }
   });
   // accumulate: every old value is read before the first store (the input
-  // cotangent arrays of one launch are distinct), so the reads are in flight together
+  // cotangent arrays of one launch are distinct), so the reads are in flight
+  // together; on the own-column path only the inputs it reaches (RM)
+  constexpr uint32_t RM = PATH == 1 ? deep_reach<T, Op, M3, M2>() : ~0u;
   T old[Op::NIN];
   static_for<0, Op::NIN - 1>([&](auto Fc) {
     constexpr int f = decltype(Fc)::value;
     old[f] = T(0);
-    if constexpr ((M3 >> f) & 1u) {
+    if constexpr (((M3 >> f) & 1u) && ((RM >> f) & 1u)) {
       if (live && inbar.o[f] && !((inbar.assign >> f) & 1u)) old[f] = ld_at(inbar.o[f], q.o3, lix3(q, q.i, q.j, q.k));
-    } else if constexpr ((M2 >> f) & 1u) {
+    } else if constexpr (!((M3 >> f) & 1u) && ((M2 >> f) & 1u) && ((RM >> f) & 1u)) {
       if (live && q.k == 0 && inbar.o[f] && !((inbar.assign >> f) & 1u))
         old[f] = ld_at(inbar.o[f], q.o2, lix2(q, q.i, q.j));
     }
   });
   static_for<0, Op::NIN - 1>([&](auto Fc) {
     constexpr int f = decltype(Fc)::value;
+    constexpr bool reach = (RM >> f) & 1u;
     if constexpr ((M3 >> f) & 1u) {
-      if (live && inbar.o[f]) ref_at(inbar.o[f], q.o3, lix3(q, q.i, q.j, q.k)) = old[f] + acc[f];
+      if (live && inbar.o[f] && (reach || ((inbar.assign >> f) & 1u)))
+        ref_at(inbar.o[f], q.o3, lix3(q, q.i, q.j, q.k)) = old[f] + acc[f];
     } else if constexpr ((M2 >> f) & 1u) {
-      if (live && q.k == 0 && inbar.o[f]) ref_at(inbar.o[f], q.o2, lix2(q, q.i, q.j)) = old[f] + acc[f];
+      if (live && q.k == 0 && inbar.o[f] && (reach || ((inbar.assign >> f) & 1u)))
+        ref_at(inbar.o[f], q.o2, lix2(q, q.i, q.j)) = old[f] + acc[f];
     }
   });
 }
