@@ -76,33 +76,6 @@ __global__ void __launch_bounds__(TPB) k_map(Batch B, Vert<T> vert, const T* ram
   }
 }
 
-// One operator over two field tables in one pass (T and S, or q2 and q2l,
-// which share their velocity, viscosity and depth fields): each point
-// evaluates both, so the shared fields are read from HBM once and their
-// second reads hit the cache.  Same values as two k_map launches.
-template <class T, class Op>
-__global__ void __launch_bounds__(TPB) k_map_pair(Batch B, Vert<T> vert, const T* ramp, Params prm, FieldTab<T> ta,
-                                                   OutTab<T> oa, FieldTab<T> tb, OutTab<T> ob) {
-  static_assert(Op::NOUT == 1, "one output per table");
-  const int nblk = Op::OUT3 ? B.blk3[B.M] : B.blk2[B.M];
-  for (int blk = logical_block(); blk < nblk; blk += (int)gridDim.x) {
-    Pos p;
-    if constexpr (Op::OUT3) {
-      if (!pos3_at(B, blk, p)) continue;
-    } else {
-      if (!pos2_at(B, blk, p)) continue;
-    }
-    const int64_t o_ = Op::OUT3 ? p.o3 : p.o2;
-    const uint32_t li = Op::OUT3 ? lix3(p, p.i, p.j, p.k) : lix2(p, p.i, p.j);
-    const Acc<T> A{&ta, p, vert, ramp};
-    const Acc<T> Bq{&tb, p, vert, ramp};
-    const T ra = val(Op::template eval<0>(A, prm));
-    const T rb = val(Op::template eval<0>(Bq, prm));
-    if (oa.o[0]) ref_at(oa.o[0], o_, li) = ra;
-    if (ob.o[0]) ref_at(ob.o[0], o_, li) = rb;
-  }
-}
-
 // ---- running max of |x| over a launch (the depth-mean velocity check).  NaN
 // propagates, as torch.max does: fabs(NaN) is a positive NaN, whose bits
 // exceed every finite value's, so the unsigned atomic max keeps it.
@@ -446,15 +419,6 @@ inline void launch_map(const Ctx<T>& c, const FieldTab<T>& tab, const OutTab<T>&
   if (nb == 0) return;
   const int g = Op::OUT3 ? groups(nb, c.tune.flat3_bpg, c.tune.flat3_min) : groups(nb, c.tune.flat2_bpg, c.tune.flat2_min);
   hipLaunchKernelGGL((k_map<T, Op>), dim3(g), dim3(TPB), 0, c.stream, c.B, c.vert, c.ramp, c.prm, tab, out);
-}
-
-template <class T, class Op>
-inline void launch_map_pair(const Ctx<T>& c, const FieldTab<T>& ta, const OutTab<T>& oa, const FieldTab<T>& tb,
-                            const OutTab<T>& ob) {
-  const int nb = Op::OUT3 ? c.nblk3 : c.nblk2;
-  if (nb == 0) return;
-  const int g = Op::OUT3 ? groups(nb, c.tune.flat3_bpg, c.tune.flat3_min) : groups(nb, c.tune.flat2_bpg, c.tune.flat2_min);
-  hipLaunchKernelGGL((k_map_pair<T, Op>), dim3(g), dim3(TPB), 0, c.stream, c.B, c.vert, c.ramp, c.prm, ta, oa, tb, ob);
 }
 
 template <class T, class OpA, class OpB, int FB0, int FB1>

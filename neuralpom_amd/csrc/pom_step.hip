@@ -25,6 +25,7 @@
 #include "pom_advctadj.h"
 #include "pom_advaveadj.h"
 #include "pom_smagadj.h"
+#include "pom_advfwd.h"
 
 #include "pom_batch.h"
 
@@ -675,6 +676,13 @@ template <class T> struct Step {
     t.g[OpSmagImp::GDX] = g[POM_G_DX]; t.g[OpSmagImp::GDY] = g[POM_G_DY];
     return t;
   }
+  // the fields advt1 and advq share between their two tracers (t_advt1, t_advq)
+  void adv_shared(AdvPairArgs<T>& a) const {
+    a.u = u1; a.v = v1; a.w = out[POM_W]; a.aam = out[POM_AAM];
+    a.dt = in[POM_DT]; a.etb = in[POM_ETB]; a.etf = xf().etf;
+    a.dx = g[POM_G_DX]; a.dy = g[POM_G_DY]; a.h = g[POM_G_H]; a.art = g[POM_G_ART];
+    a.dum = g[POM_G_DUM]; a.dvm = g[POM_G_DVM];
+  }
   FieldTab<T> t_advq(bool l2) const {
     FieldTab<T> t{};
     t.f[OpAdvq::QB] = in[l2 ? POM_Q2LB : POM_Q2B];
@@ -1043,7 +1051,13 @@ template <class T> struct Step {
         hipLaunchKernelGGL(k_adjust<T>, g2, tb, 0, c.stream, c.B, c.vert, a);
       }
       hipLaunchKernelGGL(k_vertvl<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a_vertvl());
-      launch_map_pair<T, OpAdvq>(c, t_advq(false), OutTab<T>{{q2f0}}, t_advq(true), OutTab<T>{{q2lf0}});
+      {   // advq of q2 and q2l in one pass
+        AdvPairArgs<T> a{};
+        a.fb[0] = in[POM_Q2B]; a.f[0] = in[POM_Q2]; a.fb[1] = in[POM_Q2LB]; a.f[1] = in[POM_Q2L];
+        adv_shared(a);
+        a.out[0] = q2f0; a.out[1] = q2lf0;
+        launch_advq_pair(c, a);
+      }
       {
         ProfScope ps(POM_STAGE_PROFQ, c.stream);
         hipLaunchKernelGGL(k_profq<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a_profq());
@@ -1054,7 +1068,14 @@ template <class T> struct Step {
                                       OutTab<T>{{out[POM_Q2], out[POM_Q2B], out[POM_Q2L], out[POM_Q2LB]}});
       else
         launch_map<T, OpBc6Filter>(c, t_bc6(), OutTab<T>{{out[POM_Q2], out[POM_Q2B], out[POM_Q2L], out[POM_Q2LB]}});
-      launch_map_pair<T, OpAdvt1>(c, t_advt1(false), OutTab<T>{{tf0}}, t_advt1(true), OutTab<T>{{sf0}});
+      {   // advt1 of T and S in one pass
+        AdvPairArgs<T> a{};
+        a.fb[0] = in[POM_TB]; a.f[0] = in[POM_T]; a.fclim[0] = g[POM_G_TCLIM];
+        a.fb[1] = in[POM_SB]; a.f[1] = in[POM_S]; a.fclim[1] = g[POM_G_SCLIM];
+        adv_shared(a);
+        a.out[0] = tf0; a.out[1] = sf0;
+        launch_advt1_pair(c, a);
+      }
       hipLaunchKernelGGL(k_proft<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a_proft(false));
       hipLaunchKernelGGL(k_proft<T>, g2, tb, 0, c.stream, c.B, c.vert, c.prm, a_proft(true));
       if (open)
