@@ -135,6 +135,84 @@ __global__ void __launch_bounds__(TPB) k_advq_pair(Batch B, Vert<T> vg, Params p
   }
 }
 
+template <class T>
+__global__ void __launch_bounds__(TPB) k_advuv(Batch B, Vert<T> vg, Params prm, AdvUVArgs<T> a) {
+  const int nblk = B.blk3[B.M];
+  const T q4 = T(0.25), eps = T(1e-20), g8 = T(prm.grav * 0.125), dti4 = T(2.0 * prm.dti2), two = T(2);
+  for (int blk = logical_block(); blk < nblk; blk += (int)gridDim.x) {   // uniform over the workgroup
+    Pos q{};
+    const bool live = pos3_at(B, blk, q);   // a water column's 16 lanes are all live or all not
+    const int k = threadIdx.x & (KB - 1);
+    const int im = q.im, jm = q.jm, i = q.i, j = q.j;
+    const uint32_t ix = live ? lix3(q, i, j, k) : 0u;
+    auto L3 = [&](const T* f, int di, int dj) {
+      return live ? ld_at(f, q.o3, lix3(q, clampi(i + di, im), clampi(j + dj, jm), k)) : T(0);
+    };
+    auto L2 = [&](const T* f, int di, int dj) { return ld_at(f, q.o2, lix2(q, clampi(i + di, im), clampi(j + dj, jm))); };
+    const T uc = L3(a.u, 0, 0), vc = L3(a.v, 0, 0), wc = L3(a.w, 0, 0), ww = L3(a.w, -1, 0), ws = L3(a.w, 0, -1);
+    const T uup = __shfl_up(uc, 1, KB), udn = __shfl_down(uc, 1, KB);   // u at k - 1, k + 1
+    const T vup = __shfl_up(vc, 1, KB), vdn = __shfl_down(vc, 1, KB);
+    const T wcd = __shfl_down(wc, 1, KB), wwd = __shfl_down(ww, 1, KB), wsd = __shfl_down(ws, 1, KB);   // w at k + 1
+    if (!live) continue;
+    const bool in = i >= 1 && i <= im - 2 && j >= 1 && j <= jm - 2 && k < KBM1;
+    const T dz = vg.dz(q.m, k);
+    const T dtc = L2(a.dt, 0, 0), dtw = L2(a.dt, -1, 0), dts = L2(a.dt, 0, -1);
+    const T hc = L2(a.h, 0, 0), hw = L2(a.h, -1, 0), hs_ = L2(a.h, 0, -1);
+    const T ec = L2(a.eatm, 0, 0), ew = L2(a.eatm, -1, 0), es = L2(a.eatm, 0, -1);
+    const T efc = L2(a.egf, 0, 0), efw = L2(a.egf, -1, 0), efs = L2(a.egf, 0, -1);
+    const T ebc = L2(a.egb, 0, 0), ebw = L2(a.egb, -1, 0), ebs = L2(a.egb, 0, -1);
+    const T tfc = L2(a.etf, 0, 0), tfw = L2(a.etf, -1, 0), tfs = L2(a.etf, 0, -1);
+    const T tbc = L2(a.etb, 0, 0), tbw = L2(a.etb, -1, 0), tbs = L2(a.etb, 0, -1);
+    const T corc = L2(a.cor, 0, 0), corw = L2(a.cor, -1, 0), cors = L2(a.cor, 0, -1);
+    // ---- u: OpAdvu
+    {
+      const T aru = L2(a.aru, 0, 0);
+      const T f0 = (k >= 1 && k < KBM1) ? q4 * (wc + ww) * (uc + uup) : T(0);       // vfl<0>
+      const T f1 = (k + 1 < KBM1) ? q4 * (wcd + wwd) * (udn + uc) : T(0);          // vfl<1> (k + 1 >= 1)
+      const T vdiv = (f0 - f1) * aru / dz;
+      const T c1 = (corc * dtc) * (L3(a.v, 0, 1) + vc);
+      const T c2 = (corw * dtw) * (L3(a.v, -1, 1) + L3(a.v, -1, 0));
+      const T corio = -aru * q4 * (c1 + c2);
+      const T geg = (efc - efw) + (ebc - ebw);
+      const T gpa = two * (ec - ew);
+      const T pg = g8 * ((dtc + dtw) * (geg + gpa) * (L2(a.dy, 0, 0) + L2(a.dy, -1, 0)));
+      const T rhs = L3(a.advx, 0, 0) + vdiv + corio + pg + L3(a.drhox, 0, 0);
+      const T hsum = hc + hw;
+      const T dnew = (hsum + tfc + tfw) * aru;
+      const T dold = (hsum + tbc + tbw) * aru;
+      const T r = (dold * L3(a.ub, 0, 0) - dti4 * rhs) / (dnew + eps);
+      ref_at(a.uf, q.o3, ix) = in ? r : T(0);
+    }
+    // ---- v: OpAdvv
+    {
+      const T arv = L2(a.arv, 0, 0);
+      const T f0 = (k >= 1 && k < KBM1) ? q4 * (wc + ws) * (vc + vup) : T(0);
+      const T f1 = (k + 1 < KBM1) ? q4 * (wcd + wsd) * (vdn + vc) : T(0);
+      const T vdiv = (f0 - f1) * arv / dz;
+      const T c1 = (corc * dtc) * (L3(a.u, 1, 0) + uc);
+      const T c2 = (cors * dts) * (L3(a.u, 1, -1) + L3(a.u, 0, -1));
+      const T corio = arv * q4 * (c1 + c2);
+      const T geg = (efc - efs) + (ebc - ebs);
+      const T gpa = two * (ec - es);
+      const T pg = g8 * ((dtc + dts) * (geg + gpa) * (L2(a.dx, 0, 0) + L2(a.dx, 0, -1)));
+      const T rhs = L3(a.advy, 0, 0) + vdiv + corio + pg + L3(a.drhoy, 0, 0);
+      const T hsum = hc + hs_;
+      const T dnew = (hsum + tfc + tfs) * arv;
+      const T dold = (hsum + tbc + tbs) * arv;
+      const T r = (dold * L3(a.vb, 0, 0) - dti4 * rhs) / (dnew + eps);
+      ref_at(a.vf, q.o3, ix) = in ? r : T(0);
+    }
+  }
+}
+
+template <class T> void launch_advuv(const Ctx<T>& c, const AdvUVArgs<T>& a) {
+  if (c.nblk3 == 0) return;
+  hipLaunchKernelGGL((k_advuv<T>), dim3(groups(c.nblk3, c.tune.flat3_bpg, c.tune.flat3_min)), dim3(TPB), 0,
+                     c.stream, c.B, c.vert, c.prm, a);
+}
+template void launch_advuv<float>(const Ctx<float>&, const AdvUVArgs<float>&);
+template void launch_advuv<double>(const Ctx<double>&, const AdvUVArgs<double>&);
+
 template <class T> void launch_advt1_pair(const Ctx<T>& c, const AdvPairArgs<T>& a) {
   if (c.nblk3 == 0) return;
   hipLaunchKernelGGL((k_advt1_pair<T>), dim3(groups(c.nblk3, c.tune.flat3_bpg, c.tune.flat3_min)), dim3(TPB), 0,

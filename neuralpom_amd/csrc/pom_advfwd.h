@@ -21,7 +21,18 @@ template <class T> struct AdvPairArgs {
   T* out[2];
 };
 
+// the momentum tendencies and leapfrog of u and v together (OpAdvu, OpAdvv;
+// reference src/neuralpom/cores/explicit/momentum.py:14-248)
+template <class T> struct AdvUVArgs {
+  const T *advx, *drhox, *ub, *advy, *drhoy, *vb;   // per equation (3-D)
+  const T *u, *v, *w;                               // shared 3-D
+  const T *dt, *egf, *egb, *etf, *etb;              // shared 2-D
+  const T *aru, *arv, *cor, *dx, *dy, *h, *eatm;    // geometry (e_atmos)
+  T *uf, *vf;
+};
+
 template <class T> void launch_advt1_pair(const Ctx<T>& c, const AdvPairArgs<T>& a);
+template <class T> void launch_advuv(const Ctx<T>& c, const AdvUVArgs<T>& a);
 template <class T> void launch_advq_pair(const Ctx<T>& c, const AdvPairArgs<T>& a);
 
 }  // namespace pom

@@ -1089,8 +1089,17 @@ template <class T> struct Step {
       else
         launch_map<T, OpBc4Filter>(c, t_bc4(), OutTab<T>{{out[POM_T], out[POM_TB], out[POM_S], out[POM_SB]}});
       launch_map<T, OpDens>(c, t_dens(), OutTab<T>{{out[POM_RHO]}});
-      launch_map<T, OpAdvu>(c, t_advu(), OutTab<T>{{uf0}});
-      launch_map<T, OpAdvv>(c, t_advv(), OutTab<T>{{vf0}});
+      {   // advu and advv in one pass
+        AdvUVArgs<T> a{};
+        a.advx = out[POM_ADVX]; a.drhox = out[POM_DRHOX]; a.ub = in[POM_UB];
+        a.advy = out[POM_ADVY]; a.drhoy = out[POM_DRHOY]; a.vb = in[POM_VB];
+        a.u = u1; a.v = v1; a.w = out[POM_W];
+        a.dt = in[POM_DT]; a.egf = xf().egf; a.egb = in[POM_EGB]; a.etf = xf().etf; a.etb = in[POM_ETB];
+        a.aru = g[POM_G_ARU]; a.arv = g[POM_G_ARV]; a.cor = g[POM_G_COR]; a.dx = g[POM_G_DX]; a.dy = g[POM_G_DY];
+        a.h = g[POM_G_H]; a.eatm = out[POM_E_ATMOS];
+        a.uf = uf0; a.vf = vf0;
+        launch_advuv(c, a);
+      }
       hipLaunchKernelGGL((k_profuv<T, 0>), g2, tb, 0, c.stream, c.B, c.vert, c.prm, a_profuv(false));
       hipLaunchKernelGGL((k_profuv<T, 1>), g2, tb, 0, c.stream, c.B, c.vert, c.prm, a_profuv(true));
       if (open) launch_map<T, OpBc3D>(c, t_bc3d(), OutTab<T>{{uf2, vf2}});
