@@ -47,6 +47,29 @@ template <class T> struct Ctx {
   Tuning tune;
 };
 
+// Boundary operators read horizontal neighbours only at evaluation points
+// within EDGE_BAND of the member's edge; elsewhere they are pointwise in
+// (i, j).  An input point q farther than EDGE_BAND + R from every edge then
+// receives contributions from its own column only, and the gather skips the
+// other offsets there.
+template <class Op, class = void> struct edge_band { static constexpr int value = -1; };
+template <class Op> struct edge_band<Op, std::void_t<decltype(Op::EDGE_BAND)>> {
+  static constexpr int value = Op::EDGE_BAND;
+};
+template <class Op> __device__ __forceinline__ bool deep_point(const Pos& q) {
+  constexpr int EB = edge_band<Op>::value;
+  if constexpr (EB < 0) {
+    return false;
+  } else {
+    return q.i >= EB + Op::RI && q.i <= q.im - 1 - EB - Op::RI && q.j >= EB + Op::RJ && q.j <= q.jm - 1 - EB - Op::RJ;
+  }
+}
+// The forward accessor of an evaluation point away from every edge (deep_point):
+// operators with an edge band then take their interior formula alone (deep_v).
+template <class T> struct AccFwdDeep : Acc<T> {
+  static constexpr bool deep = true;
+};
+
 template <class T, class Op>
 __global__ void __launch_bounds__(TPB) k_map(Batch B, Vert<T> vert, const T* ramp, Params prm, FieldTab<T> tab,
                                               OutTab<T> out) {
@@ -63,12 +86,26 @@ __global__ void __launch_bounds__(TPB) k_map(Batch B, Vert<T> vert, const T* ram
   const uint32_t li = Op::OUT3 ? lix3(p, p.i, p.j, p.k) : lix2(p, p.i, p.j);
   // every output evaluated before the first store: a store may alias the
   // inputs, so loads after it could neither be shared between outputs nor
-  // issued early
+  // issued early; points away from the edges of an edge-band operator take
+  // its interior formula alone
   T res[Op::NOUT];
-  static_for<0, Op::NOUT - 1>([&](auto O) {
-    constexpr int o = decltype(O)::value;
-    res[o] = val(Op::template eval<o>(A, prm));
-  });
+  bool done = false;
+  if constexpr (edge_band<Op>::value >= 0) {
+    if (deep_point<Op>(p)) {
+      const AccFwdDeep<T> AD{A};
+      static_for<0, Op::NOUT - 1>([&](auto O) {
+        constexpr int o = decltype(O)::value;
+        res[o] = val(Op::template eval<o>(AD, prm));
+      });
+      done = true;
+    }
+  }
+  if (!done) {
+    static_for<0, Op::NOUT - 1>([&](auto O) {
+      constexpr int o = decltype(O)::value;
+      res[o] = val(Op::template eval<o>(A, prm));
+    });
+  }
   static_for<0, Op::NOUT - 1>([&](auto O) {
     constexpr int o = decltype(O)::value;
     if (out.o[o]) ref_at(out.o[o], o_, li) = res[o];
@@ -215,23 +252,6 @@ __device__ __forceinline__ void gather_all_term(const FieldTab<T>& tab, const CO
   }
 }
 
-// Boundary operators read horizontal neighbours only at evaluation points
-// within EDGE_BAND of the member's edge; elsewhere they are pointwise in
-// (i, j).  An input point q farther than EDGE_BAND + R from every edge then
-// receives contributions from its own column only, and the gather skips the
-// other offsets there.
-template <class Op, class = void> struct edge_band { static constexpr int value = -1; };
-template <class Op> struct edge_band<Op, std::void_t<decltype(Op::EDGE_BAND)>> {
-  static constexpr int value = Op::EDGE_BAND;
-};
-template <class Op> __device__ __forceinline__ bool deep_point(const Pos& q) {
-  constexpr int EB = edge_band<Op>::value;
-  if constexpr (EB < 0) {
-    return false;
-  } else {
-    return q.i >= EB + Op::RI && q.i <= q.im - 1 - EB - Op::RI && q.j >= EB + Op::RJ && q.j <= q.jm - 1 - EB - Op::RJ;
-  }
-}
 // distance of the water column of q to its member's nearest edge is below w
 __device__ __forceinline__ bool in_band(const Pos& q, int w) {
   return q.i < w || q.i > q.im - 1 - w || q.j < w || q.j > q.jm - 1 - w;
