@@ -1,6 +1,8 @@
 // Kernel and launcher of the advct adjoint (pom_advctadj.h).
 #include "pom_advctadj.h"
 
+#include <type_traits>
+
 namespace pom {
 
 // PART 0: the cotangents of u and v and the advective and curvature share of
@@ -17,14 +19,31 @@ __global__ void __launch_bounds__(TPB) k_adj_advct(Batch B, AdvctAdjArgs<T> a) {
     const int k = threadIdx.x & (KB - 1);
     auto sel = [](bool c, T x) { return c ? x : T(0); };
     T pdt = T(0);   // this level's share of dt's cotangent
-    if (live) {
+    // the body, once for columns at least 3 from every edge (FAST: every
+    // neighbour read is in the member and every edge condition holds, so the
+    // reads are a byte offset from the column's own element) and once for the
+    // columns nearer the edge (clamped indices, edge conditions evaluated)
+    auto body = [&](auto fast) {
+      constexpr bool FAST = decltype(fast)::value;
       const int im = q.im, jm = q.jm, qi = q.i, qj = q.j;
+      const uint32_t b3 = boff<T>(q.o3, lix3(q, qi, qj, k)), b2 = boff<T>(q.o2, lix2(q, qi, qj));
+      const int rs3 = jm * KB * (int)sizeof(T), rs2 = jm * (int)sizeof(T);
       // values at q + (di, dj), level k; indices clamped into the member
       auto L3 = [&](const T* f, int di, int dj) {
-        return ld_at(f, q.o3, lix3(q, clampi(qi + di, im), clampi(qj + dj, jm), k));
+        if constexpr (FAST) {
+          return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(f) +
+                                             (b3 + (uint32_t)(di * rs3 + dj * KB * (int)sizeof(T))));
+        } else {
+          return ld_at(f, q.o3, lix3(q, clampi(qi + di, im), clampi(qj + dj, jm), k));
+        }
       };
       auto L2 = [&](const T* f, int di, int dj) {
-        return ld_at(f, q.o2, lix2(q, clampi(qi + di, im), clampi(qj + dj, jm)));
+        if constexpr (FAST) {
+          return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(f) +
+                                             (b2 + (uint32_t)(di * rs2 + dj * (int)sizeof(T))));
+        } else {
+          return ld_at(f, q.o2, lix2(q, clampi(qi + di, im), clampi(qj + dj, jm)));
+        }
       };
       auto U = [&](int di, int dj) { return L3(a.u, di, dj); };
       auto V = [&](int di, int dj) { return L3(a.v, di, dj); };
@@ -34,16 +53,16 @@ __global__ void __launch_bounds__(TPB) k_adj_advct(Batch B, AdvctAdjArgs<T> a) {
       auto DY = [&](int di, int dj) { return L2(a.dy, di, dj); };
       auto inner = [&](int di, int dj) {
         const int i = qi + di, j = qj + dj;
-        return i >= 1 && i <= im - 2 && j >= 1 && j <= jm - 2 && k < KBM1;
+        return (FAST || (i >= 1 && i <= im - 2 && j >= 1 && j <= jm - 2)) && k < KBM1;
       };
       // output cotangents where the output is an interior update, else zero
       auto sx = [&](int di, int dj) { return sel(inner(di, dj), L3(a.ob_x, di, dj)); };
       auto sy = [&](int di, int dj) { return sel(inner(di, dj), L3(a.ob_y, di, dj)); };
       // face cotangents at a (or b) = q + (di, dj)
-      auto cXu = [&](int di) { return sel(qi + di >= 1, sx(di, 0) - sx(di + 1, 0)); };
+      auto cXu = [&](int di) { return sel(FAST || qi + di >= 1, sx(di, 0) - sx(di + 1, 0)); };
       auto cYu = [&](int di, int dj) { return sx(di, dj - 1) - sx(di, dj); };
       auto cXv = [&](int di, int dj) { return sy(di - 1, dj) - sy(di, dj); };
-      auto cYv = [&](int dj) { return sel(qj + dj >= 1, sy(0, dj) - sy(0, dj + 1)); };
+      auto cYv = [&](int dj) { return sel(FAST || qj + dj >= 1, sy(0, dj) - sy(0, dj + 1)); };
       // corner sums over b, b-i, b-j, b-i-j
       auto s4x = [&](int di, int dj) { return DX(di, dj) + DX(di - 1, dj) + DX(di, dj - 1) + DX(di - 1, dj - 1); };
       auto s4y = [&](int di, int dj) { return DY(di, dj) + DY(di - 1, dj) + DY(di, dj - 1) + DY(di - 1, dj - 1); };
@@ -62,12 +81,12 @@ __global__ void __launch_bounds__(TPB) k_adj_advct(Batch B, AdvctAdjArgs<T> a) {
                      (U(di + 1, dj) + U(di, dj)) * (DX(di, dj + 1) - DX(di, dj - 1))) * rdxy(di, dj);
       };
       auto Ku = [&](int di, int dj) {   // cotangent of P1(a): advx(a) (i >= 2) and advx(a+i) read it
-        return -c4 * (sel(qi + di >= 2, sx(di, dj) * L2(a.aru, di, dj)) +
-                      sel(qi + di >= 1, sx(di + 1, dj) * L2(a.aru, di + 1, dj)));
+        return -c4 * (sel(FAST || qi + di >= 2, sx(di, dj) * L2(a.aru, di, dj)) +
+                      sel(FAST || qi + di >= 1, sx(di + 1, dj) * L2(a.aru, di + 1, dj)));
       };
       auto Kv = [&](int di, int dj) {   // cotangent of P2(a): advy(a) (j >= 2) and advy(a+j) read it
-        return c4 * (sel(qj + dj >= 2, sy(di, dj) * L2(a.arv, di, dj)) +
-                     sel(qj + dj >= 1, sy(di, dj + 1) * L2(a.arv, di, dj + 1)));
+        return c4 * (sel(FAST || qj + dj >= 2, sy(di, dj) * L2(a.arv, di, dj)) +
+                     sel(FAST || qj + dj >= 1, sy(di, dj + 1) * L2(a.arv, di, dj + 1)));
       };
       auto ccurv = [&](int di, int dj) {   // cotangent of curv(a)
         const T d = DT(di, dj);
@@ -161,6 +180,10 @@ __global__ void __launch_bounds__(TPB) k_adj_advct(Batch B, AdvctAdjArgs<T> a) {
         ref_at(a.c_vb, q.o3, ix) = ovb + cvb;
         ref_at(a.c_aam, q.o3, ix) = oam + caam;
       }
+    };
+    if (live) {
+      if (q.i >= 3 && q.i <= q.im - 4 && q.j >= 3 && q.j <= q.jm - 4) body(std::true_type{});
+      else body(std::false_type{});
     }
 #pragma unroll
     for (int o = KB / 2; o > 0; o >>= 1) pdt += __shfl_xor(pdt, o, KB);
