@@ -124,6 +124,11 @@ __global__ void k_copy(T* __restrict__ d, const T* __restrict__ s, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     d[i] = s[i];
 }
+// the copy in 16-byte words (both arrays 16-byte aligned, n16 words)
+__global__ void k_copy16(uint4* __restrict__ d, const uint4* __restrict__ s, int64_t n16) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
+    d[i] = s[i];
+}
 template <class T>
 __global__ void k_add(T* __restrict__ d, const T* __restrict__ s, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -408,6 +413,14 @@ template <class T> struct Step {
 
   void copy(T* d, const T* s, int64_t n) {
     if (d == s) return;
+    const int64_t bytes = n * (int64_t)sizeof(T);
+    if ((((uintptr_t)d | (uintptr_t)s) & 15u) == 0 && bytes % 16 == 0) {
+      // one 16-byte word per thread (a grid-stride loop beyond 65 536 workgroups)
+      const int64_t n16 = bytes / 16, blocks = (n16 + 255) / 256;
+      hipLaunchKernelGGL(k_copy16, dim3((unsigned)(blocks < 1 ? 1 : (blocks > 65536 ? 65536 : blocks))), dim3(256), 0,
+                         c.stream, reinterpret_cast<uint4*>(d), reinterpret_cast<const uint4*>(s), n16);
+      return;
+    }
     hipLaunchKernelGGL(k_copy<T>, dim3(flat_blocks(n)), dim3(256), 0, c.stream, d, s, n);
   }
 
@@ -1142,7 +1155,7 @@ template <class T> struct Step {
         st.s[e][2] = nullptr;
       }
       st.n = N2;
-      hipLaunchKernelGGL(k_sum_multi<T>, dim3(flat_blocks(N2 / 8 > 0 ? N2 / 8 : 1), 7), dim3(256), 0, c.stream, st);
+      hipLaunchKernelGGL(k_sum_multi<T>, dim3(flat_blocks(N2), 7), dim3(256), 0, c.stream, st);
     }
   }
 };
@@ -1335,7 +1348,7 @@ template <class T> struct Adj {
       put(cur.advua, bo[POM_ADVUA]); put(cur.advva, bo[POM_ADVVA]);
       put(bi[POM_ET], bi[POM_ET], bo[POM_ETB]);
       st.n = N2;
-      hipLaunchKernelGGL(k_sum_multi<T>, dim3(flat_blocks(N2 / 8 > 0 ? N2 / 8 : 1), ne), dim3(256), 0, c.stream, st);
+      hipLaunchKernelGGL(k_sum_multi<T>, dim3(flat_blocks(N2), ne), dim3(256), 0, c.stream, st);
     }
     // cotangents of the final substep fields used by the internal mode
     T* b_etf30 = cur.etf;
@@ -1418,7 +1431,7 @@ template <class T> struct Adj {
       acc(b_egf0, b0.egf); acc(b_utf0, b0.utf); acc(b_vtf0, b0.vtf); acc(b_advua0, b0.advua);
       acc(b_advva0, b0.advva);
       st.n = N2;
-      hipLaunchKernelGGL(k_sum_multi<T>, dim3(flat_blocks(N2 / 8 > 0 ? N2 / 8 : 1), ne), dim3(256), 0, c.stream, st);
+      hipLaunchKernelGGL(k_sum_multi<T>, dim3(flat_blocks(N2), ne), dim3(256), 0, c.stream, st);
     }
     // pre-loop: adx2d = raw - advua0 etc.
     {
