@@ -13,7 +13,8 @@ __global__ void __launch_bounds__(TPB) k_adj_advave(Batch B, AdvaveAdjArgs<T> a)
     const int im = q.im, jm = q.jm, qi = q.i, qj = q.j;
     auto sel = [](bool c, T x) { return c ? x : T(0); };
     // values at q + (di, dj); indices clamped into the member (a clamped value
-    // only feeds a term whose weight is zero)
+    // only feeds a term whose weight is zero).  Quotients by the grid spacings
+    // are products with their lane reciprocals (rcp_lane).
     auto L = [&](const T* f, int di, int dj) {
       return ld_at(f, q.o2, lix2(q, clampi(qi + di, im), clampi(qj + dj, jm)));
     };
@@ -52,13 +53,13 @@ __global__ void __launch_bounds__(TPB) k_adj_advave(Batch B, AdvaveAdjArgs<T> a)
       const T uu = u1 + u0, s = (dp + d0) * u1 + (d0 + dm) * u0, wy = w * DY(-1, 0);
       gu += wy * c8 * ((dp + d0) * uu + s);
       gd += wy * c8 * u1 * uu;
-      gub -= wy * d0 * two * AM(-1, 0) / DX(-1, 0);
+      gub -= wy * d0 * two * AM(-1, 0) * rcp_lane(DX(-1, 0));
     }
     {
       const T w = wU1(0);    // a = q
       const T u1 = U(1, 0), u0 = U(0, 0), dp = D(1, 0), d0 = D(0, 0), dm = D(-1, 0);
       const T uu = u1 + u0, s = (dp + d0) * u1 + (d0 + dm) * u0, wy = w * DY(0, 0);
-      const T rdx = T(1) / DX(0, 0), am = AM(0, 0), diff = (UB(1, 0) - UB(0, 0)) * rdx;
+      const T rdx = rcp_lane(DX(0, 0)), am = AM(0, 0), diff = (UB(1, 0) - UB(0, 0)) * rdx;
       gu += wy * c8 * ((d0 + dm) * uu + s);
       gd += wy * (c8 * uu * uu - two * am * diff);
       gam -= wy * d0 * two * diff;
@@ -76,13 +77,13 @@ __global__ void __launch_bounds__(TPB) k_adj_advave(Batch B, AdvaveAdjArgs<T> a)
       const T vv = v1 + v0, s = (dp + d0) * v1 + (d0 + dm) * v0, wx = w * DX(0, -1);
       gv += wx * c8 * ((dp + d0) * vv + s);
       gd += wx * c8 * v1 * vv;
-      gvb -= wx * d0 * two * AM(0, -1) / DY(0, -1);
+      gvb -= wx * d0 * two * AM(0, -1) * rcp_lane(DY(0, -1));
     }
     {
       const T w = wV2(0);
       const T v1 = V(0, 1), v0 = V(0, 0), dp = D(0, 1), d0 = D(0, 0), dm = D(0, -1);
       const T vv = v1 + v0, s = (dp + d0) * v1 + (d0 + dm) * v0, wx = w * DX(0, 0);
-      const T rdy = T(1) / DY(0, 0), am = AM(0, 0), diff = (VB(0, 1) - VB(0, 0)) * rdy;
+      const T rdy = rcp_lane(DY(0, 0)), am = AM(0, 0), diff = (VB(0, 1) - VB(0, 0)) * rdy;
       gv += wx * c8 * ((d0 + dm) * vv + s);
       gd += wx * (c8 * vv * vv - two * am * diff);
       gam -= wx * d0 * two * diff;
@@ -120,7 +121,7 @@ __global__ void __launch_bounds__(TPB) k_adj_advave(Batch B, AdvaveAdjArgs<T> a)
         // TPS(b) enters FV1(b) and FU2(b) with the factor -1
         {
           const T wt = -(w1 + w2);
-          const T ds = d4(di, dj), as = am4(di, dj), rsy = T(1) / sy, rsx = T(1) / sx;
+          const T ds = d4(di, dj), as = am4(di, dj), rsy = rcp_lane(sy), rsx = rcp_lane(sx);
           const T qq = (UB(di, dj) - UB(di, dj - 1)) * rsy + (VB(di, dj) - VB(di - 1, dj)) * rsx;
           gd += wt * c4 * as * qq;
           gam += wt * c4 * ds * qq;
